@@ -1,0 +1,170 @@
+/* ccgpu.h -- C ABI of the MI355X collective-coordinate (CC) stealthy-potential engine.
+ *
+ * Plain C: pointers and sizes only, no C++ or torch types.  Every function returns 0 (CCGPU_OK) or a
+ * negative status; ccgpu_last_error() gives the message of the last failure on the calling thread.
+ * Host pointers are read or written before the call returns (copies in and out of HBM).
+ *
+ * Citations are to the reference (jaeukk/modules) interface each entry point replaces.
+ */
+#ifndef CCGPU_H
+#define CCGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    CCGPU_OK = 0,
+    CCGPU_ERR_ARG = -1,       /* invalid argument (std::invalid_argument in the reference)          */
+    CCGPU_ERR_STATE = -2,     /* call out of order, e.g. energy before a configuration is bound     */
+    CCGPU_ERR_RANGE = -3,     /* index out of range (std::out_of_range)                             */
+    CCGPU_ERR_DEVICE = -4,    /* HIP runtime failure or no device                                    */
+    CCGPU_ERR_MEMORY = -5,    /* device allocation failed / over the HBM budget                      */
+    CCGPU_ERR_NAN = -6        /* non-finite coordinates (NotANumberFound, StructureOptimization.h:47-61) */
+};
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *ccgpu_last_error(void);
+
+/* ---------------------------------------------------------------------------------------------
+ * Lattice helpers (host, bit-exact restatements)
+ * ------------------------------------------------------------------------------------------- */
+
+/* getK(chi, rho, d) -- apps/CCOptimization/main.cpp:2018-2021 */
+double ccgpu_get_k(double chi, double rho, int d);
+
+/* GetKs(config, K, K, 1) for a cell with basis rows `basis` (d x d) -- core/PeriodicCellList.cpp:509-735.
+ * Writes min(cap, M) wavevectors (k: M x d Cartesian, n: M x d integer indices; either may be NULL) and
+ * stores the full count in *M.  Ties of (|k|^2, k_x), whose order the reference leaves to std::sort, are
+ * ordered by the remaining components. */
+int ccgpu_get_ks(int d, const double *basis, double K, int64_t cap, double *k, int32_t *n, int64_t *M);
+
+/* Reciprocal basis (core/PeriodicCellList.h:999-1031) and |det| volume (core/GeometryVector.cpp:63-77). */
+int ccgpu_reciprocal_basis(int d, const double *basis, double *b_out);
+double ccgpu_cell_volume(int d, const double *basis);
+
+/* Seeded uniform configuration: RandomGenerator(seed) + Insert x N (apps/CCOptimization/main.cpp:207-215,
+ * core/RandomGenerator.cpp:24-27, core/PeriodicCellList.h:517-549).  s: N x d relative coordinates. */
+int ccgpu_random_configuration(uint32_t seed, int64_t N, int d, double *s);
+
+/* ---------------------------------------------------------------------------------------------
+ * ccgpu_potential: one ShiftedCCPotential (potential/CollectiveCoordinatePotential.h:53-1497),
+ * optionally wrapped with the soft-core term of RepulsiveCCPotential
+ * (apps/CCOptimization/RepulsiveCCPotential.h:21-157).  Evaluated on the GPU.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct ccgpu_potential ccgpu_potential;
+
+/* ShiftedCCPotential(DimensionType) -- CollectiveCoordinatePotential.h:1318-1334.  1 <= dim <= 4. */
+int ccgpu_potential_create(int dim, ccgpu_potential **out);
+int ccgpu_potential_destroy(ccgpu_potential *p);
+
+/* constraints.emplace_back / AddConstraint(k, {V}) -- CollectiveCoordinatePotential.h:96, 105-109.
+ * k: dim doubles (absolute wavevector).  Invalidates rho. */
+int ccgpu_potential_add_constraint(ccgpu_potential *p, const double *k, double V);
+/* Replace the whole constraint vector (constraints = ...): k M x dim, V M. */
+int ccgpu_potential_set_constraints(ccgpu_potential *p, int64_t M, const double *k, const double *V);
+int ccgpu_potential_num_constraints(const ccgpu_potential *p, int64_t *M);
+
+/* public member Shift -- CollectiveCoordinatePotential.h:99 */
+int ccgpu_potential_set_shift(ccgpu_potential *p, double shift);
+
+/* RepulsiveCCPotential(dim, epsilon, sigma) soft core: E += eps sum_{i<j, r<sigma} (1 - r/sigma)^2.
+ * The term is on when |epsilon| > 1e-10 (RepulsiveCCPotential.h:49). */
+int ccgpu_potential_set_softcore(ccgpu_potential *p, double epsilon, double sigma);
+
+/* SetConfiguration(const Configuration &) -- CollectiveCoordinatePotential.h:1457-1463.
+ * basis: dim x dim rows (the cell's basis vectors); relative: N x dim coordinates (wrapped into [0,1)
+ * as Insert does).  Copies the particles to HBM and invalidates rho. */
+int ccgpu_potential_set_configuration(ccgpu_potential *p, const double *basis, int64_t N, const double *relative);
+
+/* GetRho() + RhoReal / RhoImag -- CollectiveCoordinatePotential.h:1290-1314.  re, im: M doubles each. */
+int ccgpu_potential_get_rho(ccgpu_potential *p, double *re, double *im);
+
+/* Energy() -- CollectiveCoordinatePotential.h:1351-1372 (plus the soft core, RepulsiveCCPotential.h:61-91). */
+int ccgpu_potential_energy(ccgpu_potential *p, double *E);
+
+/* Force(result, i) -- CollectiveCoordinatePotential.h:1377-1394.  F: dim doubles. */
+int ccgpu_potential_force(ccgpu_potential *p, int64_t i, double *F);
+
+/* AllForce(results) -- CollectiveCoordinatePotential.h:1434-1448 (plus RepulsiveCCPotential.h:132-145).
+ * F: N x dim doubles. */
+int ccgpu_potential_all_force(ccgpu_potential *p, double *F);
+
+/* CombinedEnergyForce::EnergyAndAllForce -- potential/CombinedEnergyForce.h:17-28: both in one pass,
+ * equal to Energy() followed by AllForce(). */
+int ccgpu_potential_energy_and_all_force(ccgpu_potential *p, double *E, double *F);
+
+/* Tile shape of the kernels (segment length 4/8/16/24/32, particles per rho chunk <= 512, constraint
+ * chunk of the force pass; 0 keeps the automatic choice).  No counterpart in the reference: the hook of
+ * the autotuner. */
+int ccgpu_potential_set_tiles(ccgpu_potential *p, int segment_len, int rho_chunk, int force_chunk);
+
+/* Which kernel family the bound constraint set uses: 1 lattice (segment recurrence), 0 general. */
+int ccgpu_potential_uses_lattice_path(const ccgpu_potential *p, int *lattice);
+
+/* ---------------------------------------------------------------------------------------------
+ * L-BFGS (Luksan's PLIS, the algorithm of NLopt's NLOPT_LD_LBFGS) on the host
+ * ------------------------------------------------------------------------------------------- */
+
+/* Objective with the signature of NLopt's nlopt_func / LuksanObjective (potential/LuksanPLIS.h:13-19). */
+typedef double (*ccgpu_objective)(unsigned n, const double *x, double *gradient, void *data);
+
+/* LuksanPLIS(n, f, f_data, x, minf, stopval, maxeval, mf, evaluations, fast=0) -- potential/LuksanPLIS.h:35-36.
+ * NLopt's arithmetic order (iterates bitwise those of NLOPT_LD_LBFGS for unbounded variables), history
+ * 1 <= mf <= 64.  Returns NLopt's result code: 1 gradient tolerance, 2 stop value, 3 f tolerance,
+ * 4 x tolerance, 5 evaluation limit, -1 failure, -2 invalid arguments. */
+int ccgpu_lbfgs_minimize(int n, ccgpu_objective f, void *f_data, double *x, double *minf, double stopval,
+                         int maxeval, int mf, int *evaluations);
+
+/* ---------------------------------------------------------------------------------------------
+ * Stealthy ground-state solves
+ * ------------------------------------------------------------------------------------------- */
+
+/* RelaxStructure_NLOPT_Emin(config, pot, 0, 0, 0, maxstep, Emin) -- potential/StructureOptimization.cpp:1073-1084
+ * (or RelaxStructure_NLOPT, :1046-1072, when Emin <= -1e10) for the configuration bound to p, single solve:
+ * the L-BFGS runs on the host in NLopt's arithmetic, every objective evaluation (Objective, :447-578) is one
+ * E+F pass on the GPU.  relative: N x dim, in: start, out: relaxed configuration (wrapped).  Outputs the
+ * final energy, the objective evaluations of all passes and the NLopt code of the last pass. */
+int ccgpu_potential_relax(ccgpu_potential *p, double *relative, int maxstep, double Emin, int history, double *energy,
+                          int *evaluations, int *status);
+
+/* Device-resident batched solver with continuous batching.  Jobs wait in a queue and are admitted into
+ * one of max_slots slots while their HBM footprint fits hbm_budget bytes; every resident job advances
+ * in the same batched E+F launches and one L-BFGS step launch (one workgroup per job) per evaluation.
+ * A job's results do not depend on which other jobs share the launches (fixed summation orders).
+ * No counterpart in the reference (which relaxes one configuration at a time, main.cpp:1811-1872). */
+typedef struct ccgpu_solver ccgpu_solver;
+int ccgpu_solver_create(int max_slots, int64_t hbm_budget, ccgpu_solver **out);
+int ccgpu_solver_destroy(ccgpu_solver *s);
+/* One RelaxStructure_NLOPT[_Emin] request: cell basis (dim x dim), N particles (relative N x dim), M
+ * constraints (k M x dim, V M), Shift, optional soft core (epsilon, sigma), per-pass evaluation limit,
+ * Emin (<= -1e10: plain RelaxStructure_NLOPT) and L-BFGS history (1..64). */
+int ccgpu_solver_submit(ccgpu_solver *s, int dim, const double *basis, int64_t N, const double *relative, int64_t M,
+                        const double *k, const double *V, double shift, double epsilon, double sigma, int maxstep,
+                        double Emin, int history, int64_t *job);
+/* Admit queued jobs and advance resident ones by up to `evaluations` batched evaluations.
+ * *pending: jobs still queued or resident afterwards. */
+int ccgpu_solver_step(ccgpu_solver *s, int evaluations, int *pending);
+/* Evaluations between two host checks for finished jobs (default 8). */
+int ccgpu_solver_set_check_interval(ccgpu_solver *s, int evaluations);
+/* State of a job: 0 queued, 1 resident, 2 done, 3 failed.  When done: final energy, objective evaluations,
+ * E+F evaluations on the device, NLopt code of the last pass, and times (submit, admit, done; seconds on the
+ * solver clock) in t[3]; relative (N x dim, may be NULL) receives the relaxed configuration. */
+int ccgpu_solver_result(ccgpu_solver *s, int64_t job, int *state, double *energy, int *evaluations,
+                        int *device_evaluations, int *status, double *t, double *relative);
+int ccgpu_solver_release(ccgpu_solver *s, int64_t job);
+int ccgpu_solver_stats(ccgpu_solver *s, int *resident, int *queued, int64_t *bytes_in_use);
+/* Kernel timing (no counterpart in the reference): when on, hipEvents on the solver's stream bracket the
+ * rho and force contraction launches of every evaluation; read the accumulated milliseconds and launch
+ * counts (reset = 1 clears them after reading). */
+int ccgpu_solver_set_timing(ccgpu_solver *s, int on);
+int ccgpu_solver_timing(ccgpu_solver *s, double *rho_ms, double *force_ms, int64_t *rho_launches,
+                        int64_t *force_launches, int reset);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CCGPU_H */

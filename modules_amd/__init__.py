@@ -1,0 +1,8 @@
+"""modules_amd -- MI355X-native collective-coordinate stealthy-potential engine.
+
+The hot path (rho(k), energy, forces, soft core, L-BFGS) runs in hand-written gfx950 kernels behind the
+C ABI of include/ccgpu.h (modules_amd/libccgpu.so).  This package is the host-side mirror of the
+reference's Potential / StructureOptimization interface over that ABI.
+"""
+from .potential import (Configuration, GetKs, GetUnitCubicBox, KPointConstraint, RandomConfiguration,  # noqa: F401
+                        RepulsiveCCPotential, ShiftedCCPotential, get_chi, getK, pi)

@@ -1,0 +1,95 @@
+"""Loader of the native engine (modules_amd/libccgpu.so) through its C ABI (include/ccgpu.h).
+
+The library is the product: there is no Python or CPU fallback.  Loading fails loudly when the shared
+object is missing (run ``python -c "import __graft_entry__ as g; g.build()"`` or ``make -C modules_amd/csrc``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libccgpu.so")
+
+dp = C.POINTER(C.c_double)
+ip32 = C.POINTER(C.c_int32)
+i64p = C.POINTER(C.c_int64)
+vp = C.c_void_p
+
+_lib = None
+
+
+class CCGPUError(RuntimeError):
+    """Raised for a nonzero status of the C ABI; .code holds the CCGPU_ERR_* value."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"ccgpu error {code}: {msg}")
+        self.code = code
+
+
+# status codes (include/ccgpu.h)
+OK, ERR_ARG, ERR_STATE, ERR_RANGE, ERR_DEVICE, ERR_MEMORY, ERR_NAN = 0, -1, -2, -3, -4, -5, -6
+
+# function name -> (restype, argtypes); every function of include/ccgpu.h is declared here
+SIGNATURES = {
+    "ccgpu_last_error": (C.c_char_p, []),
+    "ccgpu_get_k": (C.c_double, [C.c_double, C.c_double, C.c_int]),
+    "ccgpu_get_ks": (C.c_int, [C.c_int, dp, C.c_double, C.c_int64, dp, ip32, i64p]),
+    "ccgpu_reciprocal_basis": (C.c_int, [C.c_int, dp, dp]),
+    "ccgpu_cell_volume": (C.c_double, [C.c_int, dp]),
+    "ccgpu_random_configuration": (C.c_int, [C.c_uint32, C.c_int64, C.c_int, dp]),
+    "ccgpu_potential_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+    "ccgpu_potential_destroy": (C.c_int, [vp]),
+    "ccgpu_potential_add_constraint": (C.c_int, [vp, dp, C.c_double]),
+    "ccgpu_potential_set_constraints": (C.c_int, [vp, C.c_int64, dp, dp]),
+    "ccgpu_potential_num_constraints": (C.c_int, [vp, i64p]),
+    "ccgpu_potential_set_shift": (C.c_int, [vp, C.c_double]),
+    "ccgpu_potential_set_softcore": (C.c_int, [vp, C.c_double, C.c_double]),
+    "ccgpu_potential_set_configuration": (C.c_int, [vp, dp, C.c_int64, dp]),
+    "ccgpu_potential_get_rho": (C.c_int, [vp, dp, dp]),
+    "ccgpu_potential_energy": (C.c_int, [vp, dp]),
+    "ccgpu_potential_force": (C.c_int, [vp, C.c_int64, dp]),
+    "ccgpu_potential_all_force": (C.c_int, [vp, dp]),
+    "ccgpu_potential_energy_and_all_force": (C.c_int, [vp, dp, dp]),
+    "ccgpu_potential_set_tiles": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+    "ccgpu_potential_uses_lattice_path": (C.c_int, [vp, C.POINTER(C.c_int)]),
+    "ccgpu_lbfgs_minimize": (C.c_int, [C.c_int, vp, vp, dp, dp, C.c_double, C.c_int, C.c_int,
+                                       C.POINTER(C.c_int)]),
+    "ccgpu_potential_relax": (C.c_int, [vp, dp, C.c_int, C.c_double, C.c_int, dp, C.POINTER(C.c_int),
+                                        C.POINTER(C.c_int)]),
+    "ccgpu_solver_create": (C.c_int, [C.c_int, C.c_int64, C.POINTER(vp)]),
+    "ccgpu_solver_destroy": (C.c_int, [vp]),
+    "ccgpu_solver_submit": (C.c_int, [vp, C.c_int, dp, C.c_int64, dp, C.c_int64, dp, dp, C.c_double, C.c_double,
+                                      C.c_double, C.c_int, C.c_double, C.c_int, i64p]),
+    "ccgpu_solver_step": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int)]),
+    "ccgpu_solver_set_check_interval": (C.c_int, [vp, C.c_int]),
+    "ccgpu_solver_result": (C.c_int, [vp, C.c_int64, C.POINTER(C.c_int), dp, C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int), C.POINTER(C.c_int), dp, dp]),
+    "ccgpu_solver_release": (C.c_int, [vp, C.c_int64]),
+    "ccgpu_solver_stats": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), i64p]),
+    "ccgpu_solver_set_timing": (C.c_int, [vp, C.c_int]),
+    "ccgpu_solver_timing": (C.c_int, [vp, dp, dp, i64p, i64p, C.c_int]),
+}
+
+OBJECTIVE = C.CFUNCTYPE(C.c_double, C.c_uint, dp, dp, C.c_void_p)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"native engine not built: {LIB_PATH} is missing")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status: int):
+    if status != OK:
+        msg = lib().ccgpu_last_error()
+        raise CCGPUError(status, msg.decode() if msg else "")
+    return status

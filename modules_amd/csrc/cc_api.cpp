@@ -1,0 +1,531 @@
+// cc_api.cpp -- the C ABI (include/ccgpu.h) over the CC runtime.
+#include "../../include/ccgpu.h"
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "cc_lattice.h"
+#include "cc_relax.h"
+#include "cc_runtime.h"
+#include "plis_host_ops.h"
+
+using namespace ccgpu;
+
+namespace {
+thread_local std::string g_last_error;
+
+template <class F>
+int Guard(F &&f)
+{
+    try {
+        g_last_error.clear();
+        f();
+        return CCGPU_OK;
+    } catch (const Error &e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_last_error = "host allocation failed";
+        return CCGPU_ERR_MEMORY;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return CCGPU_ERR_DEVICE;
+    }
+}
+
+void Require(bool ok, int code, const char *msg)
+{
+    if (!ok) throw Error(code, msg);
+}
+
+void RequireDevice()
+{
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n < 1) {
+        (void)hipGetLastError();
+        throw Error(CCGPU_ERR_DEVICE, "no HIP device available");
+    }
+}
+}  // namespace
+
+// one ShiftedCCPotential (optionally RepulsiveCCPotential) bound to one configuration on the GPU
+struct ccgpu_potential {
+    explicit ccgpu_potential(int d) : sys(d) { batch.systems.push_back(&sys); }
+    System sys;
+    Batch batch;
+    bool bound = false;
+    bool rho_valid = false;     // RhoValid (CollectiveCoordinatePotential.h:1290-1291)
+    bool forces_valid = false;
+    std::vector<double> forces;  // host copy of the last force pass
+
+    void Invalidate()
+    {
+        rho_valid = false;
+        forces_valid = false;
+    }
+    void EnsureBuilt()
+    {
+        Require(bound, CCGPU_ERR_STATE, "no configuration bound: call SetConfiguration first");
+        if (batch.dirty || sys.layout_dirty) batch.Build();
+    }
+    void EnsureRho()
+    {
+        EnsureBuilt();
+        if (!rho_valid) {
+            batch.Enqueue(true, false);
+            rho_valid = true;
+            forces_valid = false;
+        }
+    }
+    void EnsureForces()
+    {
+        EnsureBuilt();
+        if (!forces_valid) {
+            batch.Enqueue(!rho_valid, true);
+            rho_valid = true;
+            forces.resize((size_t)sys.N * sys.d);
+            if (!forces.empty())
+                CC_HIP(hipMemcpyAsync(forces.data(), sys.F.get(), forces.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                      batch.stream));
+            batch.Sync();
+            forces_valid = true;
+        }
+    }
+    double ReadEnergy()
+    {
+        double e[3];
+        CC_HIP(hipMemcpyAsync(e, sys.energy.get(), sizeof(e), hipMemcpyDeviceToHost, batch.stream));
+        batch.Sync();
+        return e[0];
+    }
+};
+
+extern "C" {
+
+const char *ccgpu_last_error(void) { return g_last_error.c_str(); }
+
+double ccgpu_get_k(double chi, double rho, int d) { return GetK(chi, rho, d); }
+
+int ccgpu_get_ks(int d, const double *basis, double K, int64_t cap, double *k, int32_t *n, int64_t *M)
+{
+    return Guard([&] {
+        Require(d >= 1 && d <= kMaxDim, CCGPU_ERR_ARG, "dimension must be 1..4");
+        Require(basis != nullptr && M != nullptr, CCGPU_ERR_ARG, "null argument");
+        std::vector<double> kk;
+        std::vector<int32_t> nn;
+        GetKs(d, basis, K, kk, nn);
+        const int64_t m = (int64_t)(kk.size() / d);
+        *M = m;
+        const int64_t w = std::min<int64_t>(m, std::max<int64_t>(cap, 0));
+        if (k && w > 0) std::memcpy(k, kk.data(), (size_t)(w * d) * sizeof(double));
+        if (n && w > 0) std::memcpy(n, nn.data(), (size_t)(w * d) * sizeof(int32_t));
+    });
+}
+
+int ccgpu_reciprocal_basis(int d, const double *basis, double *b_out)
+{
+    return Guard([&] {
+        Require(d >= 1 && d <= kMaxDim && basis && b_out, CCGPU_ERR_ARG, "bad argument");
+        ReciprocalBasis(d, basis, b_out);
+    });
+}
+
+double ccgpu_cell_volume(int d, const double *basis)
+{
+    if (d < 1 || d > kMaxDim || !basis) return 0.0;
+    return CellVolume(d, basis);
+}
+
+int ccgpu_random_configuration(uint32_t seed, int64_t N, int d, double *s)
+{
+    return Guard([&] {
+        Require(N >= 0 && d >= 1 && d <= kMaxDim && (s || N == 0), CCGPU_ERR_ARG, "bad argument");
+        RandomRelativeConfiguration(seed, N, d, s);
+    });
+}
+
+int ccgpu_potential_create(int dim, ccgpu_potential **out)
+{
+    return Guard([&] {
+        Require(out != nullptr, CCGPU_ERR_ARG, "null output handle");
+        *out = nullptr;
+        Require(dim >= 1 && dim <= kMaxDim, CCGPU_ERR_ARG, "dimension must be 1..4");
+        RequireDevice();
+        *out = new ccgpu_potential(dim);
+    });
+}
+
+int ccgpu_potential_destroy(ccgpu_potential *p)
+{
+    return Guard([&] { delete p; });
+}
+
+int ccgpu_potential_add_constraint(ccgpu_potential *p, const double *k, double V)
+{
+    return Guard([&] {
+        Require(p && k, CCGPU_ERR_ARG, "null argument");
+        p->sys.AddConstraint(k, V);
+        p->batch.dirty = true;
+        p->Invalidate();
+    });
+}
+
+int ccgpu_potential_set_constraints(ccgpu_potential *p, int64_t M, const double *k, const double *V)
+{
+    return Guard([&] {
+        Require(p && M >= 0 && (M == 0 || (k && V)), CCGPU_ERR_ARG, "bad argument");
+        p->sys.SetConstraints(M, k, V);
+        p->batch.dirty = true;
+        p->Invalidate();
+    });
+}
+
+int ccgpu_potential_num_constraints(const ccgpu_potential *p, int64_t *M)
+{
+    return Guard([&] {
+        Require(p && M, CCGPU_ERR_ARG, "null argument");
+        *M = p->sys.M;
+    });
+}
+
+int ccgpu_potential_set_shift(ccgpu_potential *p, double shift)
+{
+    return Guard([&] {
+        Require(p != nullptr, CCGPU_ERR_ARG, "null handle");
+        p->sys.shift = shift;
+        p->batch.dirty = true;
+        p->Invalidate();
+    });
+}
+
+int ccgpu_potential_set_softcore(ccgpu_potential *p, double epsilon, double sigma)
+{
+    return Guard([&] {
+        Require(p != nullptr, CCGPU_ERR_ARG, "null handle");
+        Require(std::isfinite(epsilon) && std::isfinite(sigma) && sigma >= 0, CCGPU_ERR_ARG, "bad soft-core parameters");
+        Require(std::abs(epsilon) <= 1e-10 || sigma > 0, CCGPU_ERR_ARG, "soft core needs sigma > 0");
+        p->sys.e0 = epsilon;
+        p->sys.sigma = sigma;
+        p->batch.dirty = true;
+        p->Invalidate();
+    });
+}
+
+int ccgpu_potential_set_configuration(ccgpu_potential *p, const double *basis, int64_t N, const double *relative)
+{
+    return Guard([&] {
+        Require(p && basis && N >= 0 && (N == 0 || relative), CCGPU_ERR_ARG, "bad argument");
+        Require(N < (int64_t)1 << 30, CCGPU_ERR_ARG, "too many particles");
+        const int d = p->sys.d;
+        for (int i = 0; i < d * d; i++) Require(std::isfinite(basis[i]), CCGPU_ERR_ARG, "non-finite basis");
+        for (int64_t i = 0; i < N * d; i++)
+            if (!std::isfinite(relative[i])) throw Error(CCGPU_ERR_NAN, "non-finite particle coordinate");
+        p->sys.SetCell(basis);
+        if (p->sys.N != N) {
+            p->sys.N = N;
+            p->batch.dirty = true;
+        }
+        if (p->sys.layout_dirty) p->batch.dirty = true;
+        if (p->sys.rescale != 1.0) {
+            p->sys.rescale = 1.0;
+            p->batch.dirty = true;
+        }
+        p->bound = true;
+        p->EnsureBuilt();
+        if (N > 0)
+            CC_HIP(hipMemcpyAsync(p->sys.x.get(), relative, (size_t)(N * d) * sizeof(double), hipMemcpyHostToDevice,
+                                  p->batch.stream));
+        p->Invalidate();
+    });
+}
+
+int ccgpu_potential_get_rho(ccgpu_potential *p, double *re, double *im)
+{
+    return Guard([&] {
+        Require(p != nullptr, CCGPU_ERR_ARG, "null handle");
+        p->EnsureRho();
+        const int64_t M = p->sys.M;
+        if (M > 0) {
+            if (re) CC_HIP(hipMemcpyAsync(re, p->sys.rho.get(), M * sizeof(double), hipMemcpyDeviceToHost, p->batch.stream));
+            if (im)
+                CC_HIP(hipMemcpyAsync(im, p->sys.rho.get() + M, M * sizeof(double), hipMemcpyDeviceToHost, p->batch.stream));
+        }
+        p->batch.Sync();
+    });
+}
+
+int ccgpu_potential_energy(ccgpu_potential *p, double *E)
+{
+    return Guard([&] {
+        Require(p && E, CCGPU_ERR_ARG, "null argument");
+        p->EnsureRho();
+        *E = p->ReadEnergy();
+    });
+}
+
+int ccgpu_potential_force(ccgpu_potential *p, int64_t i, double *F)
+{
+    return Guard([&] {
+        Require(p && F, CCGPU_ERR_ARG, "null argument");
+        Require(p->bound, CCGPU_ERR_STATE, "no configuration bound: call SetConfiguration first");
+        if (i < 0 || i >= p->sys.N) throw Error(CCGPU_ERR_RANGE, "particle index out of range");
+        p->EnsureForces();
+        std::memcpy(F, p->forces.data() + i * p->sys.d, p->sys.d * sizeof(double));
+    });
+}
+
+int ccgpu_potential_all_force(ccgpu_potential *p, double *F)
+{
+    return Guard([&] {
+        Require(p != nullptr, CCGPU_ERR_ARG, "null handle");
+        p->EnsureForces();
+        Require(F != nullptr || p->forces.empty(), CCGPU_ERR_ARG, "null force array");
+        if (!p->forces.empty()) std::memcpy(F, p->forces.data(), p->forces.size() * sizeof(double));
+    });
+}
+
+int ccgpu_potential_energy_and_all_force(ccgpu_potential *p, double *E, double *F)
+{
+    return Guard([&] {
+        Require(p && E, CCGPU_ERR_ARG, "null argument");
+        p->EnsureForces();
+        *E = p->ReadEnergy();
+        Require(F != nullptr || p->forces.empty(), CCGPU_ERR_ARG, "null force array");
+        if (!p->forces.empty()) std::memcpy(F, p->forces.data(), p->forces.size() * sizeof(double));
+    });
+}
+
+int ccgpu_potential_set_tiles(ccgpu_potential *p, int segment_len, int rho_chunk, int force_chunk)
+{
+    return Guard([&] {
+        Require(p != nullptr, CCGPU_ERR_ARG, "null handle");
+        Require(segment_len == 0 || segment_len == 4 || segment_len == 8 || segment_len == 16 || segment_len == 24 ||
+                    segment_len == 32,
+                CCGPU_ERR_ARG, "segment length must be 0, 4, 8, 16, 24 or 32");
+        Require(rho_chunk >= 0 && rho_chunk <= 512 && force_chunk >= 0, CCGPU_ERR_ARG, "bad chunk size");
+        p->sys.tiles.seg = segment_len;
+        p->sys.tiles.rho_chunk = rho_chunk;
+        p->sys.tiles.force_chunk = force_chunk;
+        p->sys.layout_dirty = true;
+        p->batch.dirty = true;
+        p->Invalidate();
+    });
+}
+
+int ccgpu_potential_relax(ccgpu_potential *p, double *relative, int maxstep, double Emin, int history, double *energy,
+                          int *evaluations, int *status)
+{
+    return Guard([&] {
+        Require(p && relative && energy, CCGPU_ERR_ARG, "null argument");
+        Require(p->bound, CCGPU_ERR_STATE, "no configuration bound: call SetConfiguration first");
+        Require(history >= 1 && history <= kPlisMaxHistory, CCGPU_ERR_ARG, "history must be 1..64");
+        System &S = p->sys;
+        const int d = S.d;
+        const int64_t N = S.N, n = N * d;
+        const double fmin = Emin;
+        // Objective (StructureOptimization.cpp:447-578, Switch 0): x -> s = wrap(x / rescale) in the prepare
+        // kernel, E + F on the GPU, gradient -F.a_j / rescale in the force epilogue
+        auto evaluate = [&](const double *x, double rescale, double *g) -> double {
+            if (n > 0 && !(x[0] == x[0])) throw Error(CCGPU_ERR_NAN, "NotANumberFound");
+            if (S.rescale != rescale) {
+                S.rescale = rescale;
+                p->batch.dirty = true;
+            }
+            p->EnsureBuilt();
+            if (n > 0) CC_HIP(hipMemcpyAsync(S.x.get(), x, n * sizeof(double), hipMemcpyHostToDevice, p->batch.stream));
+            p->batch.Enqueue(true, g != nullptr);
+            double e[3];
+            CC_HIP(hipMemcpyAsync(e, S.energy.get(), sizeof(e), hipMemcpyDeviceToHost, p->batch.stream));
+            if (g && n > 0) CC_HIP(hipMemcpyAsync(g, S.grad.get(), n * sizeof(double), hipMemcpyDeviceToHost, p->batch.stream));
+            p->batch.Sync();
+            return e[0];
+        };
+        std::vector<double> s(relative, relative + n), x(n), g(n), dir(n), xo(n * history), go(n * history),
+            uo(history), vo(history);
+        int total = 0, last = 0;
+        double E = 0;
+        for (int pass = 0; pass < 3; pass++) {
+            const double rescale = kRelaxRescale[pass];
+            // RelaxStructure_NLOPT_inner (:881-979): skip if the energy is already below fmin
+            E = evaluate(s.data(), 1.0, nullptr);
+            if (E < fmin) break;
+            for (int64_t i = 0; i < n; i++) x[i] = s[i] * rescale;
+            std::fill(xo.begin(), xo.end(), 0.0);
+            std::fill(go.begin(), go.end(), 0.0);
+            HostPlisOps ops{(int)n, history, x.data(), g.data(), dir.data(), xo.data(), go.data(), uo.data(), vo.data()};
+            PlisState st;
+            PlisInit(st, (int)n, history, maxstep, fmin);
+            int act = PlisAdvance(st, ops);
+            while (act == kPlisNeedEval) {
+                st.f = evaluate(x.data(), rescale, g.data());
+                act = PlisAdvance(st, ops);
+            }
+            total += st.nevals;
+            last = PlisResult(st);
+            for (int64_t i = 0; i < n; i++) s[i] = WrapUnit(x[i] / rescale);
+            E = evaluate(s.data(), 1.0, nullptr);
+            if (Emin > -kMaxEnergy && E < Emin) break;
+        }
+        S.rescale = 1.0;
+        p->batch.dirty = true;
+        std::copy(s.begin(), s.end(), relative);
+        // leave the relaxed configuration bound, as RelaxStructure leaves List relaxed
+        p->EnsureBuilt();
+        if (n > 0) CC_HIP(hipMemcpyAsync(S.x.get(), s.data(), n * sizeof(double), hipMemcpyHostToDevice, p->batch.stream));
+        p->batch.Sync();
+        p->Invalidate();
+        *energy = E;
+        if (evaluations) *evaluations = total;
+        if (status) *status = last;
+    });
+}
+
+int ccgpu_potential_uses_lattice_path(const ccgpu_potential *p, int *lattice)
+{
+    return Guard([&] {
+        Require(p && lattice, CCGPU_ERR_ARG, "null argument");
+        *lattice = p->sys.lattice ? 1 : 0;
+    });
+}
+
+struct ccgpu_solver {
+    ccgpu_solver(int slots, int64_t budget) : solver(slots, budget) {}
+    Solver solver;
+};
+
+int ccgpu_solver_create(int max_slots, int64_t hbm_budget, ccgpu_solver **out)
+{
+    return Guard([&] {
+        Require(out != nullptr, CCGPU_ERR_ARG, "null output handle");
+        *out = nullptr;
+        Require(max_slots >= 1 && hbm_budget > 0, CCGPU_ERR_ARG, "bad solver limits");
+        RequireDevice();
+        *out = new ccgpu_solver(max_slots, hbm_budget);
+    });
+}
+
+int ccgpu_solver_destroy(ccgpu_solver *s)
+{
+    return Guard([&] { delete s; });
+}
+
+int ccgpu_solver_submit(ccgpu_solver *s, int dim, const double *basis, int64_t N, const double *relative, int64_t M,
+                        const double *k, const double *V, double shift, double epsilon, double sigma, int maxstep,
+                        double Emin, int history, int64_t *job)
+{
+    return Guard([&] {
+        Require(s && basis && job && N >= 0 && M >= 0 && (N == 0 || relative) && (M == 0 || (k && V)), CCGPU_ERR_ARG,
+                "bad argument");
+        Require(dim >= 1 && dim <= kMaxDim, CCGPU_ERR_ARG, "dimension must be 1..4");
+        Require(history >= 1 && history <= kPlisMaxHistory, CCGPU_ERR_ARG, "history must be 1..64");
+        Require(N * dim < (int64_t)1 << 30, CCGPU_ERR_ARG, "too many particles");
+        for (int64_t i = 0; i < N * dim; i++)
+            if (!std::isfinite(relative[i])) throw Error(CCGPU_ERR_NAN, "non-finite particle coordinate");
+        std::unique_ptr<Job> j(new Job);
+        j->d = dim;
+        j->N = N;
+        j->basis.assign(basis, basis + dim * dim);
+        j->s0.resize(N * dim);
+        for (int64_t i = 0; i < N * dim; i++) j->s0[i] = WrapUnit(relative[i]);
+        j->k.assign(k, k + M * dim);
+        j->v.assign(V, V + M);
+        j->shift = shift;
+        j->e0 = std::abs(epsilon) > 1e-10 ? epsilon : 0.0;
+        j->sigma = sigma;
+        j->maxstep = maxstep;
+        j->Emin = Emin;
+        j->mf = history;
+        *job = s->solver.Submit(std::move(j));
+    });
+}
+
+int ccgpu_solver_step(ccgpu_solver *s, int evaluations, int *pending)
+{
+    return Guard([&] {
+        Require(s != nullptr && evaluations >= 0, CCGPU_ERR_ARG, "bad argument");
+        const int left = s->solver.Step(evaluations);
+        if (pending) *pending = left;
+    });
+}
+
+int ccgpu_solver_set_check_interval(ccgpu_solver *s, int evaluations)
+{
+    return Guard([&] {
+        Require(s != nullptr && evaluations >= 1, CCGPU_ERR_ARG, "bad argument");
+        s->solver.SetCheckInterval(evaluations);
+    });
+}
+
+int ccgpu_solver_result(ccgpu_solver *s, int64_t job, int *state, double *energy, int *evaluations,
+                        int *device_evaluations, int *status, double *t, double *relative)
+{
+    return Guard([&] {
+        Require(s != nullptr, CCGPU_ERR_ARG, "null handle");
+        Job *j = s->solver.Find(job);
+        if (!j) throw Error(CCGPU_ERR_RANGE, "unknown job");
+        if (state) *state = (int)j->state;
+        if (j->state == Job::kFailed) throw Error(CCGPU_ERR_DEVICE, "job failed: " + j->error);
+        if (energy) *energy = j->E;
+        if (evaluations) *evaluations = j->evals;
+        if (device_evaluations) *device_evaluations = j->device_evals;
+        if (status) *status = j->status;
+        if (t) {
+            t[0] = j->t_submit;
+            t[1] = j->t_admit;
+            t[2] = j->t_done;
+        }
+        if (relative && j->state == Job::kDone && !j->s_final.empty())
+            std::memcpy(relative, j->s_final.data(), j->s_final.size() * sizeof(double));
+    });
+}
+
+int ccgpu_solver_release(ccgpu_solver *s, int64_t job)
+{
+    return Guard([&] {
+        Require(s != nullptr, CCGPU_ERR_ARG, "null handle");
+        s->solver.Release(job);
+    });
+}
+
+int ccgpu_solver_stats(ccgpu_solver *s, int *resident, int *queued, int64_t *bytes_in_use)
+{
+    return Guard([&] {
+        Require(s != nullptr, CCGPU_ERR_ARG, "null handle");
+        if (resident) *resident = s->solver.Resident();
+        if (queued) *queued = s->solver.Queued();
+        if (bytes_in_use) *bytes_in_use = s->solver.BytesInUse();
+    });
+}
+
+int ccgpu_solver_set_timing(ccgpu_solver *s, int on)
+{
+    return Guard([&] {
+        Require(s != nullptr, CCGPU_ERR_ARG, "null handle");
+        s->solver.batch.timing = on != 0;
+    });
+}
+
+int ccgpu_solver_timing(ccgpu_solver *s, double *rho_ms, double *force_ms, int64_t *rho_launches,
+                        int64_t *force_launches, int reset)
+{
+    return Guard([&] {
+        Require(s != nullptr, CCGPU_ERR_ARG, "null handle");
+        Batch &b = s->solver.batch;
+        b.Sync();
+        b.CollectTiming();
+        if (rho_ms) *rho_ms = b.t_rho_ms;
+        if (force_ms) *force_ms = b.t_force_ms;
+        if (rho_launches) *rho_launches = b.n_rho;
+        if (force_launches) *force_launches = b.n_force;
+        if (reset) {
+            b.t_rho_ms = b.t_force_ms = 0;
+            b.n_rho = b.n_force = 0;
+        }
+    });
+}
+
+}  // extern "C"

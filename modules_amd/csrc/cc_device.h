@@ -1,0 +1,73 @@
+// cc_device.h -- data layout shared by the CC kernels and their host runtime.
+//
+// A "system" is one periodic configuration with its constraint set (one ShiftedCCPotential binding).
+// A batch is an array of SysDesc in HBM plus work-item lists; every kernel reads the descriptor of the
+// system its work item names, so one launch serves one system or hundreds of mixed (N, d, M) systems.
+#pragma once
+#include <cstdint>
+#include <hip/hip_runtime.h>
+
+namespace ccgpu {
+
+constexpr int kBlock = 256;  // threads per work item in every CC kernel (4 wavefronts of 64)
+
+enum SysFlags : int32_t {
+    kSysActive = 1,    // evaluate this system
+    kSysLattice = 2,   // constraints are reciprocal lattice vectors: segment kernels (phase recurrence)
+    kSysSoftCore = 4,  // add the soft-core pair term e0 (1 - r/sigma)^2
+};
+
+struct SysDesc {
+    int32_t d, N, M, flags;
+    int32_t nseg;     // lattice segments (lattice path)
+    int32_t nslots;   // lattice: nseg*S ; general: M
+    int32_t pchunks;  // particle chunks of the rho pass (rho partials per chunk)
+    int32_t kchunks;  // constraint/segment chunks of the force pass (force partials per chunk)
+    int32_t pchunk_len;  // particles per rho chunk
+    int32_t kchunk_len;  // segments (lattice) or constraints (general) per force chunk
+    int32_t ncell_side[4];  // soft-core cell grid (cells per axis)
+    int32_t ncells;
+
+    // positions: the optimizer variable x (N*d) holds relative coordinates times `rescale`
+    const double *x;
+    double4 *pos;       // wrapped relative coordinates s (N), zero padded to 4
+    double4 *cart;      // Cartesian r = sum_i a_i s_i (N)
+    double2 *step;      // exp(2 pi i s_{d-1}) (N): one lattice step along the last reciprocal axis
+
+    // constraints
+    const int4 *seg_head;   // lattice: start index of each segment (nseg)
+    const int32_t *slot_k;  // lattice: constraint index of each slot (nslots), -1 for padding
+    const double *slot_v;   // weight V of each slot (nslots); padding slots hold 0
+    const double4 *kvec;    // general: Cartesian k (M)
+
+    // rho and energy
+    double *rpart;   // pchunks x nslots x 2 partial sums
+    double *rho;     // 2 x M: RhoReal then RhoImag in constraint order
+    double *coef;    // nslots x 2: (2 V/vol) * (Im rho, Re rho)
+    double *epart;   // per reduce block: sum V |rho|^2
+    double *energy;  // [0] total, [1] CC part, [2] soft-core part
+
+    // forces
+    double *fpart;   // kchunks x N x 4 partials
+    double *F;       // N x d forces (Cartesian)
+    double *grad;    // N x d objective gradient -F.a_j / rescale
+
+    // soft core cell list (counting sort by cell)
+    int32_t *cell_of;     // N
+    int32_t *cell_start;  // ncells + 1
+    int32_t *cell_items;  // N (particle indices sorted by cell)
+    double *sc_epart;     // per particle soft-core energy (i<j pairs)
+
+    double a[16];   // basis rows, d x d
+    double b[16];   // reciprocal basis rows, d x d
+    double vol, inv_vol, shift, rescale;
+    double sigma, e0;
+    int32_t pad[2];
+};
+
+// one work item of a launch: the system index and up to three tile coordinates
+struct WorkItem {
+    int32_t sys, t0, t1, t2;
+};
+
+}  // namespace ccgpu

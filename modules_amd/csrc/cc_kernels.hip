@@ -1,0 +1,422 @@
+// cc_kernels.hip -- gfx950 kernels of the collective-coordinate stealthy potential (FP64 path).
+//
+// Reference semantics (potential/CollectiveCoordinatePotential.h):
+//   rho_k  = sum_j exp(i k.r_j)                                   GetRho_Direct :1255-1286
+//   Phi    = (1/V) sum_k V_k |rho_k|^2 + Shift                    Energy        :1351-1372
+//   F_j    = -(2/V) sum_k V_k (Im rho_k cos k.r_j - Re rho_k sin k.r_j) k      Force :1377-1394
+// and the objective gradient g[i*d+j] = -F_i . a_j / rescale      potential/StructureOptimization.cpp:527-535.
+//
+// Two kernel families:
+//   lattice  -- constraints are reciprocal lattice vectors k = sum_j n_j b_j, so k.r = 2 pi n.s with s the
+//               relative coordinates.  Constraints are grouped into segments of consecutive last-axis
+//               index; one polynomial sincos per (segment, particle) starts the segment and a complex
+//               multiply by exp(2 pi i s_{d-1}) steps along it.  VALU-bound (FP64 FMA), no MFMA: the
+//               contraction depth is d <= 4.
+//   general  -- arbitrary Cartesian k: one sincos of k.r per pair.
+// Every sum has a fixed order (chunk partials reduced in chunk order, block trees), so results are
+// bitwise repeatable and do not depend on which other systems share the launch.
+#include "cc_device.h"
+#include "cc_math.h"
+
+namespace ccgpu {
+
+__device__ __forceinline__ double wrap_unit(double x)
+{
+    double y = x - floor(x);
+    y -= floor(y);
+    return y;
+}
+
+// block-wide sum of one double per thread, fixed tree order; result valid in thread 0
+__device__ __forceinline__ double block_sum(double v, double *lds)
+{
+    const int tid = threadIdx.x;
+    lds[tid] = v;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (tid < w) lds[tid] = lds[tid] + lds[tid + w];
+        __syncthreads();
+    }
+    double r = lds[0];
+    __syncthreads();
+    return r;
+}
+
+// ---------------------------------------------------------------------------------------------
+// prepare: s = wrap(x / rescale) (FillParticles + Insert, StructureOptimization.cpp:339-350),
+// r = sum_i a_i s_i (UpdateCartesianCoordinates), and the last-axis lattice step exp(2 pi i s_{d-1}).
+// items: (sys, particle block)
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_prepare(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
+{
+    const WorkItem it = items[blockIdx.x];
+    const SysDesc &S = sys[it.sys];
+    if (!(S.flags & kSysActive)) return;
+    const int i = it.t0 * kBlock + threadIdx.x;
+    if (i >= S.N) return;
+    const int d = S.d;
+    double s[4] = {0, 0, 0, 0};
+    for (int c = 0; c < d; c++) s[c] = wrap_unit(S.x[(int64_t)i * d + c] / S.rescale);
+    double r[4] = {0, 0, 0, 0};
+    for (int q = 0; q < d; q++)
+        for (int c = 0; c < d; c++) r[c] = r[c] + S.a[q * d + c] * s[q];
+    S.pos[i] = make_double4(s[0], s[1], s[2], s[3]);
+    S.cart[i] = make_double4(r[0], r[1], r[2], r[3]);
+    double sn, cs;
+    sincos_rev(s[d - 1], sn, cs);
+    S.step[i] = make_double2(cs, sn);
+}
+
+// ---------------------------------------------------------------------------------------------
+// rho, lattice path.  items: (sys, segment block of kBlock segments, particle chunk)
+// One lane = one segment of up to SEG consecutive constraints; the particles of the chunk are staged in
+// LDS and read by all lanes at the same address (broadcast).
+// ---------------------------------------------------------------------------------------------
+template <int SEG>
+__global__ void __launch_bounds__(kBlock) k_rho_lattice(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
+{
+    constexpr int kMaxChunk = 512;
+    __shared__ double4 lpos[kMaxChunk];
+    __shared__ double2 lstep[kMaxChunk];
+    const WorkItem it = items[blockIdx.x];
+    const SysDesc &S = sys[it.sys];
+    if (!(S.flags & kSysActive)) return;
+    const int j0 = it.t1 * S.pchunk_len;
+    const int j1 = min(S.N, j0 + S.pchunk_len);
+    for (int j = j0 + (int)threadIdx.x; j < j1; j += kBlock) {
+        lpos[j - j0] = S.pos[j];
+        lstep[j - j0] = S.step[j];
+    }
+    __syncthreads();
+    const int seg = it.t0 * kBlock + threadIdx.x;
+    if (seg >= S.nseg) return;
+    const int4 h = S.seg_head[seg];
+    const double h0 = (double)h.x, h1 = (double)h.y, h2 = (double)h.z, h3 = (double)h.w;
+    double re[SEG], im[SEG];
+#pragma unroll
+    for (int t = 0; t < SEG; t++) { re[t] = 0.0; im[t] = 0.0; }
+    for (int j = 0; j < j1 - j0; j++) {
+        const double4 p = lpos[j];
+        const double u = __builtin_fma(h0, p.x, __builtin_fma(h1, p.y, __builtin_fma(h2, p.z, h3 * p.w)));
+        double sn, cs;
+        sincos_rev(u, sn, cs);
+        const double2 st = lstep[j];
+#pragma unroll
+        for (int t = 0; t < SEG; t++) {
+            re[t] += cs;
+            im[t] += sn;
+            const double nc = __builtin_fma(cs, st.x, -sn * st.y);
+            const double ns = __builtin_fma(cs, st.y, sn * st.x);
+            cs = nc;
+            sn = ns;
+        }
+    }
+    double *out = S.rpart + ((int64_t)it.t1 * S.nslots + (int64_t)seg * SEG) * 2;
+#pragma unroll
+    for (int t = 0; t < SEG; t++) {
+        out[2 * t] = re[t];
+        out[2 * t + 1] = im[t];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// rho, general path.  items: (sys, constraint block, particle chunk).  One lane = one constraint.
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_rho_general(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
+{
+    constexpr int kMaxChunk = 512;
+    __shared__ double4 lr[kMaxChunk];
+    const WorkItem it = items[blockIdx.x];
+    const SysDesc &S = sys[it.sys];
+    if (!(S.flags & kSysActive)) return;
+    const int j0 = it.t1 * S.pchunk_len;
+    const int j1 = min(S.N, j0 + S.pchunk_len);
+    for (int j = j0 + (int)threadIdx.x; j < j1; j += kBlock) lr[j - j0] = S.cart[j];
+    __syncthreads();
+    const int q = it.t0 * kBlock + threadIdx.x;
+    if (q >= S.M) return;
+    const double4 k = S.kvec[q];
+    double tr = 0.0, ti = 0.0;
+    for (int j = 0; j < j1 - j0; j++) {
+        const double4 r = lr[j];
+        const double th = k.x * r.x + k.y * r.y + k.z * r.z + k.w * r.w;
+        double sn, cs;
+        sincos_rad(th, sn, cs);
+        tr += cs;
+        ti += sn;
+    }
+    double *out = S.rpart + ((int64_t)it.t1 * S.nslots + q) * 2;
+    out[0] = tr;
+    out[1] = ti;
+}
+
+// ---------------------------------------------------------------------------------------------
+// rho reduce: rho_q = sum over chunks (chunk order), scatter to constraint order, energy partial
+// sum V |rho|^2 per block, force coefficients (2V/vol) (Im rho, Re rho).  items: (sys, slot block)
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_rho_reduce(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
+{
+    __shared__ double lds[kBlock];
+    const WorkItem it = items[blockIdx.x];
+    const SysDesc &S = sys[it.sys];
+    if (!(S.flags & kSysActive)) return;
+    const int q = it.t0 * kBlock + threadIdx.x;
+    double e = 0.0;
+    if (q < S.nslots) {
+        double re = 0.0, im = 0.0;
+        for (int c = 0; c < S.pchunks; c++) {
+            const double *p = S.rpart + ((int64_t)c * S.nslots + q) * 2;
+            re += p[0];
+            im += p[1];
+        }
+        const double v = S.slot_v[q];
+        const int kidx = (S.flags & kSysLattice) ? S.slot_k[q] : q;
+        if (kidx >= 0) {
+            S.rho[kidx] = re;
+            S.rho[S.M + kidx] = im;
+            e = v * (re * re + im * im);
+        }
+        const double w = (2.0 * v) * S.inv_vol;
+        S.coef[2 * q] = kidx >= 0 ? w * im : 0.0;
+        S.coef[2 * q + 1] = kidx >= 0 ? w * re : 0.0;
+    }
+    const double tot = block_sum(e, lds);
+    if (threadIdx.x == 0) S.epart[it.t0] = tot;
+}
+
+// ---------------------------------------------------------------------------------------------
+// energy: one block per system; Phi = sum(epart)/vol + Shift, plus the soft-core sum.  items: (sys)
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_energy_final(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
+{
+    __shared__ double lds[kBlock];
+    const WorkItem it = items[blockIdx.x];
+    const SysDesc &S = sys[it.sys];
+    if (!(S.flags & kSysActive)) return;
+    const int nb = (S.nslots + kBlock - 1) / kBlock;
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kBlock) acc += S.epart[b];
+    const double cc_sum = block_sum(acc, lds);
+    double sc = 0.0;
+    if (S.flags & kSysSoftCore) {
+        double a2 = 0.0;
+        for (int i = threadIdx.x; i < S.N; i += kBlock) a2 += S.sc_epart[i];
+        sc = block_sum(a2, lds);
+    }
+    if (threadIdx.x == 0) {
+        const double cc = cc_sum / S.vol + S.shift;
+        S.energy[0] = sc + cc;
+        S.energy[1] = cc;
+        S.energy[2] = sc;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// forces, lattice path.  items: (sys, particle block, segment chunk).  One lane = one particle;
+// segment data is uniform across the wavefront (scalar loads).  Accumulates G_j = sum_k c_k n_kj with
+// c_k = (2V/vol)(Im rho cos - Re rho sin); then F = -sum_j G_j b_j in k_force_final.
+// ---------------------------------------------------------------------------------------------
+template <int SEG>
+__global__ void __launch_bounds__(kBlock) k_force_lattice(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
+{
+    const WorkItem it = items[blockIdx.x];
+    const SysDesc &S = sys[it.sys];
+    if (!(S.flags & kSysActive)) return;
+    const int i = it.t0 * kBlock + threadIdx.x;
+    const bool valid = i < S.N;
+    const double4 p = valid ? S.pos[i] : make_double4(0, 0, 0, 0);
+    const double2 st = valid ? S.step[i] : make_double2(1, 0);
+    const int s0 = it.t1 * S.kchunk_len;
+    const int s1 = min(S.nseg, s0 + S.kchunk_len);
+    const int d = S.d;
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0;
+    const double2 *coef = reinterpret_cast<const double2 *>(S.coef);
+    for (int sg = s0; sg < s1; sg++) {
+        const int4 h = S.seg_head[sg];
+        const double h0 = (double)h.x, h1 = (double)h.y, h2 = (double)h.z, h3 = (double)h.w;
+        const double u = __builtin_fma(h0, p.x, __builtin_fma(h1, p.y, __builtin_fma(h2, p.z, h3 * p.w)));
+        double sn, cs;
+        sincos_rev(u, sn, cs);
+        double acc = 0.0, accl = 0.0;
+        const double hl = (d == 1) ? h0 : (d == 2) ? h1 : (d == 3) ? h2 : h3;
+        const double2 *cf = coef + (int64_t)sg * SEG;
+#pragma unroll
+        for (int t = 0; t < SEG; t++) {
+            const double2 ab = cf[t];
+            const double c = __builtin_fma(ab.x, cs, -ab.y * sn);
+            acc += c;
+            accl = __builtin_fma(c, hl + (double)t, accl);
+            const double nc = __builtin_fma(cs, st.x, -sn * st.y);
+            const double ns = __builtin_fma(cs, st.y, sn * st.x);
+            cs = nc;
+            sn = ns;
+        }
+        // components before the last axis are constant along the segment
+        if (d == 1) {
+            g0 += accl;
+        } else if (d == 2) {
+            g0 = __builtin_fma(h0, acc, g0);
+            g1 += accl;
+        } else if (d == 3) {
+            g0 = __builtin_fma(h0, acc, g0);
+            g1 = __builtin_fma(h1, acc, g1);
+            g2 += accl;
+        } else {
+            g0 = __builtin_fma(h0, acc, g0);
+            g1 = __builtin_fma(h1, acc, g1);
+            g2 = __builtin_fma(h2, acc, g2);
+            g3 += accl;
+        }
+    }
+    if (valid) reinterpret_cast<double4 *>(S.fpart)[(int64_t)it.t1 * S.N + i] = make_double4(g0, g1, g2, g3);
+}
+
+// ---------------------------------------------------------------------------------------------
+// forces, general path.  items: (sys, particle block, constraint chunk).  Accumulates sum_k c_k k (Cartesian).
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_force_general(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
+{
+    const WorkItem it = items[blockIdx.x];
+    const SysDesc &S = sys[it.sys];
+    if (!(S.flags & kSysActive)) return;
+    const int i = it.t0 * kBlock + threadIdx.x;
+    const bool valid = i < S.N;
+    const double4 r = valid ? S.cart[i] : make_double4(0, 0, 0, 0);
+    const int q0 = it.t1 * S.kchunk_len;
+    const int q1 = min(S.M, q0 + S.kchunk_len);
+    const double2 *coef = reinterpret_cast<const double2 *>(S.coef);
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0, f3 = 0.0;
+    for (int q = q0; q < q1; q++) {
+        const double4 k = S.kvec[q];
+        const double th = k.x * r.x + k.y * r.y + k.z * r.z + k.w * r.w;
+        double sn, cs;
+        sincos_rad(th, sn, cs);
+        const double2 ab = coef[q];
+        const double c = __builtin_fma(ab.x, cs, -ab.y * sn);
+        f0 = __builtin_fma(c, k.x, f0);
+        f1 = __builtin_fma(c, k.y, f1);
+        f2 = __builtin_fma(c, k.z, f2);
+        f3 = __builtin_fma(c, k.w, f3);
+    }
+    if (valid) reinterpret_cast<double4 *>(S.fpart)[(int64_t)it.t1 * S.N + i] = make_double4(f0, f1, f2, f3);
+}
+
+// ---------------------------------------------------------------------------------------------
+// force epilogue: sum chunk partials (chunk order), F = -sum_j G_j b_j (lattice) or -sum c k (general),
+// plus the soft-core force; gradient g[i*d+j] = (-1) F_i . a_j / rescale.  items: (sys, particle block)
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_force_final(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items,
+                                                        const double *__restrict__ sc_force_unused)
+{
+    (void)sc_force_unused;
+    const WorkItem it = items[blockIdx.x];
+    const SysDesc &S = sys[it.sys];
+    if (!(S.flags & kSysActive)) return;
+    const int i = it.t0 * kBlock + threadIdx.x;
+    if (i >= S.N) return;
+    const int d = S.d;
+    double G[4] = {0, 0, 0, 0};
+    const double4 *fp = reinterpret_cast<const double4 *>(S.fpart);
+    for (int c = 0; c < S.kchunks; c++) {
+        const double4 v = fp[(int64_t)c * S.N + i];
+        G[0] += v.x;
+        G[1] += v.y;
+        G[2] += v.z;
+        G[3] += v.w;
+    }
+    double F[4] = {0, 0, 0, 0};
+    if (S.flags & kSysLattice) {
+        for (int j = 0; j < d; j++)
+            for (int c = 0; c < d; c++) F[c] = __builtin_fma(-G[j], S.b[j * d + c], F[c]);
+    } else {
+        for (int c = 0; c < d; c++) F[c] = -G[c];
+    }
+    if (S.flags & kSysSoftCore) {
+        // soft-core force of k_sc_pair, stored after the per-particle energies (N x 4)
+        const double *fs = S.sc_epart + S.N + (int64_t)i * 4;
+        for (int c = 0; c < d; c++) F[c] = F[c] + fs[c];
+    }
+    for (int c = 0; c < d; c++) S.F[(int64_t)i * d + c] = F[c];
+    if (S.grad) {
+        for (int j = 0; j < d; j++) {
+            double dot = 0.0;
+            for (int c = 0; c < d; c++) dot += F[c] * S.a[j * d + c];
+            S.grad[(int64_t)i * d + j] = ((-1.0) * dot) / S.rescale;
+        }
+    }
+}
+
+// explicit instantiations of the segment lengths the autotuner may choose
+#define CC_INSTANTIATE_SEG(SEG)                                                                            \
+    template __global__ void k_rho_lattice<SEG>(const SysDesc *__restrict__, const WorkItem *__restrict__); \
+    template __global__ void k_force_lattice<SEG>(const SysDesc *__restrict__, const WorkItem *__restrict__);
+CC_INSTANTIATE_SEG(4)
+CC_INSTANTIATE_SEG(8)
+CC_INSTANTIATE_SEG(16)
+CC_INSTANTIATE_SEG(24)
+CC_INSTANTIATE_SEG(32)
+
+}  // namespace ccgpu
+
+// ---------------------------------------------------------------------------------------------
+// host launchers (the runtime lives in another translation unit)
+// ---------------------------------------------------------------------------------------------
+namespace ccgpu {
+
+#define CC_GRID(n) dim3((unsigned)(n)), dim3(kBlock), 0, stream
+
+void launch_prepare(hipStream_t stream, const SysDesc *sys, const WorkItem *items, int n)
+{
+    if (n > 0) hipLaunchKernelGGL(k_prepare, CC_GRID(n), sys, items);
+}
+
+void launch_rho_lattice(hipStream_t stream, int seg, const SysDesc *sys, const WorkItem *items, int n)
+{
+    if (n <= 0) return;
+    switch (seg) {
+        case 4: hipLaunchKernelGGL(k_rho_lattice<4>, CC_GRID(n), sys, items); break;
+        case 8: hipLaunchKernelGGL(k_rho_lattice<8>, CC_GRID(n), sys, items); break;
+        case 16: hipLaunchKernelGGL(k_rho_lattice<16>, CC_GRID(n), sys, items); break;
+        case 24: hipLaunchKernelGGL(k_rho_lattice<24>, CC_GRID(n), sys, items); break;
+        default: hipLaunchKernelGGL(k_rho_lattice<32>, CC_GRID(n), sys, items); break;
+    }
+}
+
+void launch_force_lattice(hipStream_t stream, int seg, const SysDesc *sys, const WorkItem *items, int n)
+{
+    if (n <= 0) return;
+    switch (seg) {
+        case 4: hipLaunchKernelGGL(k_force_lattice<4>, CC_GRID(n), sys, items); break;
+        case 8: hipLaunchKernelGGL(k_force_lattice<8>, CC_GRID(n), sys, items); break;
+        case 16: hipLaunchKernelGGL(k_force_lattice<16>, CC_GRID(n), sys, items); break;
+        case 24: hipLaunchKernelGGL(k_force_lattice<24>, CC_GRID(n), sys, items); break;
+        default: hipLaunchKernelGGL(k_force_lattice<32>, CC_GRID(n), sys, items); break;
+    }
+}
+
+void launch_rho_general(hipStream_t stream, const SysDesc *sys, const WorkItem *items, int n)
+{
+    if (n > 0) hipLaunchKernelGGL(k_rho_general, CC_GRID(n), sys, items);
+}
+
+void launch_rho_reduce(hipStream_t stream, const SysDesc *sys, const WorkItem *items, int n)
+{
+    if (n > 0) hipLaunchKernelGGL(k_rho_reduce, CC_GRID(n), sys, items);
+}
+
+void launch_energy_final(hipStream_t stream, const SysDesc *sys, const WorkItem *items, int n)
+{
+    if (n > 0) hipLaunchKernelGGL(k_energy_final, CC_GRID(n), sys, items);
+}
+
+void launch_force_general(hipStream_t stream, const SysDesc *sys, const WorkItem *items, int n)
+{
+    if (n > 0) hipLaunchKernelGGL(k_force_general, CC_GRID(n), sys, items);
+}
+
+void launch_force_final(hipStream_t stream, const SysDesc *sys, const WorkItem *items, int n)
+{
+    if (n > 0) hipLaunchKernelGGL(k_force_final, CC_GRID(n), sys, items, (const double *)nullptr);
+}
+
+}  // namespace ccgpu

@@ -1,0 +1,69 @@
+// cc_math.h -- device math for the CC kernels (gfx950).
+//
+// sincos_rev(u): sin(2*pi*u) and cos(2*pi*u) in FP64 for an argument already reduced to revolutions.
+// Method (textbook, written from the mathematics, no vendor library code):
+//   1. f = u - rint(u)            (exact for |u| < 2^52)          -> f in [-1/2, 1/2]
+//   2. q = rint(4 f), g = f - q/4                                  -> g in [-1/8, 1/8] revolutions
+//   3. x = 2 pi g (two-term 2*pi so the product keeps ~106 bits)   -> x in [-pi/4, pi/4]
+//   4. Taylor series: sin x = x (1 - x^2/3! + ... + x^16/17!), cos x = 1 - x^2/2! + ... + x^16/16!
+//      truncation error below 1e-19 relative on |x| <= pi/4 (next terms x^19/19!, x^18/18!)
+//   5. quadrant q mod 4 rotates (sin, cos) by q quarter turns.
+// Error: a few ulp against the correctly rounded value (measured in tests against libm).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace ccgpu {
+
+// 2*pi split as hi + lo with hi exactly representable, lo the rounding remainder.
+#define CC_TWO_PI_HI 6.283185307179586232
+#define CC_TWO_PI_LO 2.4492935982947064e-16
+
+__device__ __forceinline__ void sincos_rev(double u, double &s, double &c)
+{
+    double f = u - __builtin_rint(u);
+    double qd = __builtin_rint(4.0 * f);
+    double g = __builtin_fma(-0.25, qd, f);
+    // x = 2 pi g with the low part of 2 pi folded in
+    double x = __builtin_fma(g, CC_TWO_PI_HI, g * CC_TWO_PI_LO);
+    double x2 = x * x;
+    // sin: x * P(x^2), P = sum_{m=0..8} (-1)^m x^(2m)/(2m+1)!
+    double ps = 2.8114572543455207632e-15;          //  1/17!
+    ps = __builtin_fma(ps, x2, -7.6471637318198164759e-13); // -1/15!
+    ps = __builtin_fma(ps, x2, 1.6059043836821614599e-10);  //  1/13!
+    ps = __builtin_fma(ps, x2, -2.5052108385441718775e-08); // -1/11!
+    ps = __builtin_fma(ps, x2, 2.7557319223985890653e-06);  //  1/9!
+    ps = __builtin_fma(ps, x2, -1.9841269841269841270e-04); // -1/7!
+    ps = __builtin_fma(ps, x2, 8.3333333333333333333e-03);  //  1/5!
+    ps = __builtin_fma(ps, x2, -1.6666666666666666667e-01); // -1/3!
+    double sx = __builtin_fma(ps * x2, x, x);
+    // cos: Q(x^2) = sum_{m=0..8} (-1)^m x^(2m)/(2m)!
+    double pc = 4.7794773323873852974e-14;           //  1/16!
+    pc = __builtin_fma(pc, x2, -1.1470745597729724714e-11); // -1/14!
+    pc = __builtin_fma(pc, x2, 2.0876756987868098979e-09);  //  1/12!
+    pc = __builtin_fma(pc, x2, -2.7557319223985890653e-07); // -1/10!
+    pc = __builtin_fma(pc, x2, 2.4801587301587301587e-05);  //  1/8!
+    pc = __builtin_fma(pc, x2, -1.3888888888888888889e-03); // -1/6!
+    pc = __builtin_fma(pc, x2, 4.1666666666666666667e-02);  //  1/4!
+    pc = __builtin_fma(pc, x2, -0.5);                        // -1/2!
+    double cx = __builtin_fma(pc, x2, 1.0);
+    int q = (int)qd;
+    double ss = (q & 1) ? cx : sx;
+    double cc = (q & 1) ? sx : cx;
+    s = (q & 2) ? -ss : ss;
+    c = ((q + 1) & 2) ? -cc : cc;
+}
+
+// Radian argument: u = theta / (2 pi) formed in double-double before the reduction, so the conversion
+// adds no error beyond the ulp of theta itself.
+__device__ __forceinline__ void sincos_rad(double theta, double &s, double &c)
+{
+    const double inv_hi = 0.15915494309189534561;   // 1/(2 pi) rounded
+    const double inv_lo = -9.8393383375912435e-18;  // remainder
+    double uh = theta * inv_hi;
+    double ul = __builtin_fma(theta, inv_hi, -uh);
+    ul = __builtin_fma(theta, inv_lo, ul);
+    double r = __builtin_rint(uh);
+    sincos_rev((uh - r) + ul, s, c);
+}
+
+}  // namespace ccgpu

@@ -1,0 +1,392 @@
+// cc_runtime.cpp -- host runtime of the CC engine (see cc_runtime.h).
+#include "cc_runtime.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace ccgpu {
+
+void CheckHip(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(e == hipErrorOutOfMemory ? kErrMemory : kErrDevice,
+                    std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+
+Tiles DefaultTiles(int d, int64_t N, int64_t M)
+{
+    (void)d;
+    (void)M;
+    Tiles t;
+    t.seg = 16;
+    // rho: about 16 particle chunks per system, 64..512 particles each
+    int64_t c = (N + 15) / 16;
+    c = ((c + 31) / 32) * 32;
+    t.rho_chunk = (int)std::min<int64_t>(512, std::max<int64_t>(64, c));
+    t.force_chunk = 0;  // resolved in Prepare from the segment count
+    return t;
+}
+
+System::System(int dim) : d(dim) {}
+
+void System::SetCell(const double *basis)
+{
+    bool same = has_cell;
+    for (int i = 0; i < d * d; i++) same = same && a[i] == basis[i];
+    if (same) return;
+    for (int i = 0; i < d * d; i++) a[i] = basis[i];
+    ReciprocalBasis(d, a, b);
+    vol = CellVolume(d, a);
+    if (!(vol > 0.0) || !std::isfinite(vol)) throw Error(kErrArg, "cell basis is singular or not finite");
+    has_cell = true;
+    layout_dirty = true;
+}
+
+void System::SetConstraints(int64_t m, const double *kk, const double *vv)
+{
+    if (m < 0) throw Error(kErrArg, "negative constraint count");
+    for (int64_t i = 0; i < m * d; i++)
+        if (!std::isfinite(kk[i])) throw Error(kErrArg, "non-finite wavevector");
+    k.assign(kk, kk + m * d);
+    v.assign(vv, vv + m);
+    M = m;
+    layout_dirty = true;
+}
+
+void System::AddConstraint(const double *kk, double V)
+{
+    for (int c = 0; c < d; c++)
+        if (!std::isfinite(kk[c])) throw Error(kErrArg, "non-finite wavevector");
+    k.insert(k.end(), kk, kk + d);
+    v.push_back(V);
+    M++;
+    layout_dirty = true;
+}
+
+size_t System::DeviceBytes() const
+{
+    return x.bytes() + pos.bytes() + cart.bytes() + step.bytes() + seg_head.bytes() + slot_k.bytes() +
+           slot_v.bytes() + kvec.bytes() + rpart.bytes() + rho.bytes() + coef.bytes() + epart.bytes() +
+           energy.bytes() + fpart.bytes() + F.bytes() + grad.bytes() + cell_of.bytes() + cell_start.bytes() +
+           cell_items.bytes() + sc_epart.bytes() + sc_F.bytes();
+}
+
+void System::Prepare(hipStream_t stream)
+{
+    if (!has_cell) throw Error(kErrState, "no configuration bound (SetConfiguration first)");
+    Tiles def = DefaultTiles(d, N, M);
+    used.seg = tiles.seg > 0 ? tiles.seg : def.seg;
+    if (used.seg != 4 && used.seg != 8 && used.seg != 16 && used.seg != 24 && used.seg != 32)
+        throw Error(kErrArg, "segment length must be 4, 8, 16, 24 or 32");
+    used.rho_chunk = tiles.rho_chunk > 0 ? std::min(tiles.rho_chunk, 512) : def.rho_chunk;
+
+    if (layout_dirty || plan.S != used.seg) {
+        lattice = M > 0 && LatticeIndices(d, a, M, k.data(), n);
+        int64_t nslots;
+        if (lattice) {
+            plan = BuildSegments(d, M, n.data(), used.seg);
+            nslots = plan.nseg * used.seg;
+            seg_head.Reserve(plan.nseg);
+            slot_k.Reserve(nslots);
+            slot_v.Reserve(nslots);
+            std::vector<double> sv(nslots, 0.0);
+            for (int64_t q = 0; q < nslots; q++)
+                if (plan.slot_k[q] >= 0) sv[q] = v[plan.slot_k[q]];
+            CC_HIP(hipMemcpyAsync(seg_head.get(), plan.head.data(), plan.nseg * sizeof(int4), hipMemcpyHostToDevice, stream));
+            CC_HIP(hipMemcpyAsync(slot_k.get(), plan.slot_k.data(), nslots * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+            CC_HIP(hipMemcpyAsync(slot_v.get(), sv.data(), nslots * sizeof(double), hipMemcpyHostToDevice, stream));
+            CC_HIP(hipStreamSynchronize(stream));
+        } else {
+            plan = SegmentPlan();
+            plan.S = used.seg;
+            nslots = M;
+            kvec.Reserve(std::max<int64_t>(M, 1));
+            slot_v.Reserve(std::max<int64_t>(M, 1));
+            std::vector<double4> kv(M);
+            for (int64_t q = 0; q < M; q++) {
+                double c4[4] = {0, 0, 0, 0};
+                for (int c = 0; c < d; c++) c4[c] = k[q * d + c];
+                kv[q] = make_double4(c4[0], c4[1], c4[2], c4[3]);
+            }
+            if (M > 0) {
+                CC_HIP(hipMemcpyAsync(kvec.get(), kv.data(), M * sizeof(double4), hipMemcpyHostToDevice, stream));
+                CC_HIP(hipMemcpyAsync(slot_v.get(), v.data(), M * sizeof(double), hipMemcpyHostToDevice, stream));
+                CC_HIP(hipStreamSynchronize(stream));
+            }
+        }
+        layout_dirty = false;
+    }
+    const int64_t nslots = lattice ? plan.nseg * used.seg : M;
+    const int64_t nunits = lattice ? plan.nseg : M;  // force-pass loop units
+
+    // chunking of the two contractions
+    pchunks = (int)std::max<int64_t>(1, (N + used.rho_chunk - 1) / used.rho_chunk);
+    const int64_t pblocks = std::max<int64_t>(1, (N + kBlock - 1) / kBlock);
+    if (tiles.force_chunk > 0) {
+        used.force_chunk = tiles.force_chunk;
+    } else {
+        // about 16 blocks per system in the force pass, at least 4 units per chunk
+        int64_t want = std::max<int64_t>(1, (16 + pblocks - 1) / pblocks);
+        want = std::min<int64_t>(want, std::max<int64_t>(1, nunits / 4));
+        used.force_chunk = (int)std::max<int64_t>(1, (nunits + want - 1) / want);
+    }
+    kchunks = (int)std::max<int64_t>(1, (nunits + used.force_chunk - 1) / used.force_chunk);
+
+    // soft-core cell grid: side max(sigma, (V/N)^(1/d)) (apps/CCOptimization/main.cpp:235-240)
+    ncells = 1;
+    for (int c = 0; c < 4; c++) ncell_side[c] = 1;
+    if (softcore() && N > 0) {
+        const double cell = std::max(sigma, std::pow(vol / (double)N, 1.0 / d));
+        for (int c = 0; c < d; c++) {
+            double bn = 0;
+            for (int q = 0; q < d; q++) bn += b[c * d + q] * b[c * d + q];
+            const double h = 2 * kPi / std::sqrt(bn);
+            ncell_side[c] = (int)std::max(1.0, std::min(std::floor(h / cell), 4096.0));
+            ncells *= ncell_side[c];
+        }
+        if (ncells > 4 * N + 64) throw Error(kErrArg, "soft-core cell grid too fine");
+    }
+
+    const int64_t Nn = std::max<int64_t>(N, 1);
+    x.Reserve(Nn * d);
+    pos.Reserve(Nn);
+    cart.Reserve(Nn);
+    step.Reserve(Nn);
+    rpart.Reserve(std::max<int64_t>(1, (int64_t)pchunks * nslots * 2));
+    rho.Reserve(std::max<int64_t>(1, 2 * M));
+    coef.Reserve(std::max<int64_t>(1, 2 * nslots));
+    epart.Reserve(std::max<int64_t>(1, (nslots + kBlock - 1) / kBlock));
+    energy.Reserve(4);
+    fpart.Reserve((int64_t)kchunks * Nn * 4);
+    F.Reserve(Nn * d);
+    grad.Reserve(Nn * d);
+    if (softcore()) {
+        cell_of.Reserve(Nn);
+        cell_start.Reserve(2 * (int64_t)ncells + 1);
+        cell_items.Reserve(Nn);
+        sc_epart.Reserve(5 * Nn);
+    }
+
+    SysDesc &D = desc;
+    std::memset(&D, 0, sizeof(D));
+    D.d = d;
+    D.N = (int32_t)N;
+    D.M = (int32_t)M;
+    D.flags = kSysActive | (lattice ? kSysLattice : 0) | (softcore() ? kSysSoftCore : 0);
+    D.nseg = (int32_t)(lattice ? plan.nseg : 0);
+    D.nslots = (int32_t)nslots;
+    D.pchunks = N > 0 ? pchunks : 0;  // no particles: rho sums nothing
+    D.kchunks = kchunks;
+    D.pchunk_len = used.rho_chunk;
+    D.kchunk_len = used.force_chunk;
+    for (int c = 0; c < 4; c++) D.ncell_side[c] = ncell_side[c];
+    D.ncells = ncells;
+    D.x = x.get();
+    D.pos = pos.get();
+    D.cart = cart.get();
+    D.step = step.get();
+    D.seg_head = seg_head.get();
+    D.slot_k = slot_k.get();
+    D.slot_v = slot_v.get();
+    D.kvec = kvec.get();
+    D.rpart = rpart.get();
+    D.rho = rho.get();
+    D.coef = coef.get();
+    D.epart = epart.get();
+    D.energy = energy.get();
+    D.fpart = fpart.get();
+    D.F = F.get();
+    D.grad = grad.get();
+    D.cell_of = cell_of.get();
+    D.cell_start = cell_start.get();
+    D.cell_items = cell_items.get();
+    D.sc_epart = sc_epart.get();
+    for (int i = 0; i < d * d; i++) {
+        D.a[i] = a[i];
+        D.b[i] = b[i];
+    }
+    D.vol = vol;
+    D.inv_vol = 1.0 / vol;
+    D.shift = shift;
+    D.rescale = rescale;
+    D.sigma = sigma;
+    D.e0 = e0;
+}
+
+// ------------------------------------------------------------------------------------------------
+Batch::Batch() { CC_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)); }
+
+Batch::~Batch()
+{
+    for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+hipEvent_t Batch::NextEvent()
+{
+    if (ev_used_ == ev_pool_.size()) {
+        hipEvent_t e;
+        CC_HIP(hipEventCreate(&e));
+        ev_pool_.push_back(e);
+    }
+    return ev_pool_[ev_used_++];
+}
+
+void Batch::CollectTiming()
+{
+    for (const auto &m : ev_marks_) {
+        float ms = 0.f;
+        CC_HIP(hipEventElapsedTime(&ms, ev_pool_[m.second], ev_pool_[m.second + 1]));
+        if (m.first == 0) {
+            t_rho_ms += ms;
+            n_rho++;
+        } else {
+            t_force_ms += ms;
+            n_force++;
+        }
+    }
+    ev_marks_.clear();
+    ev_used_ = 0;
+}
+
+void Batch::Build()
+{
+    h_items.clear();
+    seg_groups.clear();
+    std::vector<SysDesc> descs;
+    descs.reserve(systems.size());
+    for (System *s : systems) {
+        s->Prepare(stream);
+        descs.push_back(s->desc);
+    }
+    auto add = [&](Range &r, auto &&gen) {
+        r.off = (int)h_items.size();
+        gen();
+        r.n = (int)h_items.size() - r.off;
+    };
+    const int ns = (int)systems.size();
+    auto pblocks = [](const System *s) { return (int)((s->N + kBlock - 1) / kBlock); };
+    add(r_prep, [&] {
+        for (int i = 0; i < ns; i++)
+            for (int b = 0; b < pblocks(systems[i]); b++) h_items.push_back({i, b, 0, 0});
+    });
+    // lattice systems grouped by segment length (one template instance per group)
+    std::vector<int> segs;
+    for (System *s : systems)
+        if (s->lattice && std::find(segs.begin(), segs.end(), s->used.seg) == segs.end()) segs.push_back(s->used.seg);
+    for (int sl : segs) {
+        SegGroup g;
+        g.seg = sl;
+        add(g.rho, [&] {
+            for (int i = 0; i < ns; i++) {
+                System *s = systems[i];
+                if (!s->lattice || s->used.seg != sl || s->N == 0) continue;
+                const int sb = (int)((s->plan.nseg + kBlock - 1) / kBlock);
+                for (int c = 0; c < s->pchunks; c++)
+                    for (int b = 0; b < sb; b++) h_items.push_back({i, b, c, 0});
+            }
+        });
+        add(g.force, [&] {
+            for (int i = 0; i < ns; i++) {
+                System *s = systems[i];
+                if (!s->lattice || s->used.seg != sl) continue;
+                for (int c = 0; c < s->kchunks; c++)
+                    for (int b = 0; b < pblocks(s); b++) h_items.push_back({i, b, c, 0});
+            }
+        });
+        seg_groups.push_back(g);
+    }
+    add(r_rho_gen, [&] {
+        for (int i = 0; i < ns; i++) {
+            System *s = systems[i];
+            if (s->lattice || s->M == 0 || s->N == 0) continue;
+            const int kb = (int)((s->M + kBlock - 1) / kBlock);
+            for (int c = 0; c < s->pchunks; c++)
+                for (int b = 0; b < kb; b++) h_items.push_back({i, b, c, 0});
+        }
+    });
+    add(r_red, [&] {
+        for (int i = 0; i < ns; i++) {
+            const int nb = (int)((systems[i]->desc.nslots + kBlock - 1) / kBlock);
+            for (int b = 0; b < nb; b++) h_items.push_back({i, b, 0, 0});
+        }
+    });
+    add(r_efin, [&] {
+        for (int i = 0; i < ns; i++) h_items.push_back({i, 0, 0, 0});
+    });
+    add(r_f_gen, [&] {
+        for (int i = 0; i < ns; i++) {
+            System *s = systems[i];
+            if (s->lattice) continue;
+            for (int c = 0; c < s->kchunks; c++)
+                for (int b = 0; b < pblocks(s); b++) h_items.push_back({i, b, c, 0});
+        }
+    });
+    add(r_ffin, [&] {
+        for (int i = 0; i < ns; i++)
+            for (int b = 0; b < pblocks(systems[i]); b++) h_items.push_back({i, b, 0, 0});
+    });
+    add(r_sc_part, [&] {
+        for (int i = 0; i < ns; i++)
+            if (systems[i]->softcore())
+                for (int b = 0; b < pblocks(systems[i]); b++) h_items.push_back({i, b, pblocks(systems[i]), 0});
+    });
+    add(r_sc_sys, [&] {
+        for (int i = 0; i < ns; i++)
+            if (systems[i]->softcore()) h_items.push_back({i, 0, 0, 0});
+    });
+    add(r_sc_cell, [&] {
+        for (int i = 0; i < ns; i++)
+            if (systems[i]->softcore())
+                for (int b = 0; b < (systems[i]->ncells + kBlock - 1) / kBlock; b++) h_items.push_back({i, b, 0, 0});
+    });
+    d_desc.Reserve(std::max<size_t>(1, descs.size()));
+    d_items.Reserve(std::max<size_t>(1, h_items.size()));
+    if (!descs.empty())
+        CC_HIP(hipMemcpyAsync(d_desc.get(), descs.data(), descs.size() * sizeof(SysDesc), hipMemcpyHostToDevice, stream));
+    if (!h_items.empty())
+        CC_HIP(hipMemcpyAsync(d_items.get(), h_items.data(), h_items.size() * sizeof(WorkItem), hipMemcpyHostToDevice,
+                              stream));
+    CC_HIP(hipStreamSynchronize(stream));
+    dirty = false;
+}
+
+void Batch::Enqueue(bool rho, bool forces)
+{
+    if (dirty) Build();
+    const SysDesc *D = d_desc.get();
+    const WorkItem *I = d_items.get();
+    if (rho) {
+        launch_prepare(stream, D, I + r_prep.off, r_prep.n);
+        if (r_sc_part.n > 0)
+            launch_softcore(stream, D, I + r_sc_part.off, r_sc_part.n, I + r_sc_sys.off, r_sc_sys.n, I + r_sc_cell.off,
+                            r_sc_cell.n);
+        if (timing) {
+            ev_marks_.push_back({0, ev_used_});
+            CC_HIP(hipEventRecord(NextEvent(), stream));
+        }
+        for (const SegGroup &g : seg_groups) launch_rho_lattice(stream, g.seg, D, I + g.rho.off, g.rho.n);
+        launch_rho_general(stream, D, I + r_rho_gen.off, r_rho_gen.n);
+        if (timing) CC_HIP(hipEventRecord(NextEvent(), stream));
+        launch_rho_reduce(stream, D, I + r_red.off, r_red.n);
+        launch_energy_final(stream, D, I + r_efin.off, r_efin.n);
+    }
+    if (forces) {
+        if (timing) {
+            ev_marks_.push_back({1, ev_used_});
+            CC_HIP(hipEventRecord(NextEvent(), stream));
+        }
+        for (const SegGroup &g : seg_groups) launch_force_lattice(stream, g.seg, D, I + g.force.off, g.force.n);
+        launch_force_general(stream, D, I + r_f_gen.off, r_f_gen.n);
+        if (timing) CC_HIP(hipEventRecord(NextEvent(), stream));
+        launch_force_final(stream, D, I + r_ffin.off, r_ffin.n);
+    }
+    CC_HIP(hipGetLastError());
+}
+
+void Batch::Sync() { CC_HIP(hipStreamSynchronize(stream)); }
+
+}  // namespace ccgpu

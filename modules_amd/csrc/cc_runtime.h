@@ -1,0 +1,179 @@
+// cc_runtime.h -- host runtime of the CC engine: device buffers of one system, batches of systems and
+// the kernel pipeline that evaluates rho, energy and forces for every system of a batch in one set of
+// launches.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "cc_device.h"
+#include "cc_lattice.h"
+
+namespace ccgpu {
+
+// status codes of the C ABI (include/ccgpu.h)
+enum Status { kOk = 0, kErrArg = -1, kErrState = -2, kErrRange = -3, kErrDevice = -4, kErrMemory = -5, kErrNan = -6 };
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void CheckHip(hipError_t e, const char *what);
+#define CC_HIP(expr) ::ccgpu::CheckHip((expr), #expr)
+
+// owning device array, grows on demand (never shrinks)
+template <class T>
+class DevArray {
+public:
+    DevArray() = default;
+    DevArray(const DevArray &) = delete;
+    DevArray &operator=(const DevArray &) = delete;
+    ~DevArray() { Release(); }
+    void Reserve(size_t n)
+    {
+        if (n <= cap_) return;
+        Release();
+        void *p = nullptr;
+        hipError_t e = hipMalloc(&p, (n ? n : 1) * sizeof(T));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error(kErrMemory, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed: " +
+                                        hipGetErrorString(e));
+        }
+        p_ = static_cast<T *>(p);
+        cap_ = n;
+    }
+    void Release()
+    {
+        if (p_) (void)hipFree(p_);
+        p_ = nullptr;
+        cap_ = 0;
+    }
+    T *get() const { return p_; }
+    size_t capacity() const { return cap_; }
+    size_t bytes() const { return cap_ * sizeof(T); }
+
+private:
+    T *p_ = nullptr;
+    size_t cap_ = 0;
+};
+
+// tile shape of one system: segment length of the lattice kernels, particles per rho chunk,
+// segments/constraints per force chunk.  Chosen from the system's own shape so that a system's
+// summation order, and so its bits, do not depend on the other systems of a batch.
+struct Tiles {
+    int seg = 16;
+    int rho_chunk = 0;
+    int force_chunk = 0;
+};
+Tiles DefaultTiles(int d, int64_t N, int64_t M);
+
+// One periodic configuration plus its constraint set, resident in HBM.
+class System {
+public:
+    explicit System(int d);
+
+    int d;
+    int64_t N = 0, M = 0;
+    bool has_cell = false;
+    double a[16] = {}, b[16] = {}, vol = 0;
+    std::vector<double> k, v;  // constraints (host copy): k M x d, V M
+    double shift = 0, e0 = 0, sigma = 0, rescale = 1;
+    Tiles tiles;                // requested (0 = automatic)
+    Tiles used;                 // resolved
+
+    // constraint layout
+    bool lattice = false;
+    std::vector<int32_t> n;
+    SegmentPlan plan;
+    bool layout_dirty = true;
+
+    // device buffers
+    DevArray<double> x;  // optimizer variable / relative coordinates (N*d)
+    DevArray<double4> pos, cart;
+    DevArray<double2> step;
+    DevArray<int4> seg_head;
+    DevArray<int32_t> slot_k;
+    DevArray<double> slot_v;
+    DevArray<double4> kvec;
+    DevArray<double> rpart, rho, coef, epart, energy, fpart, F, grad;
+    DevArray<int32_t> cell_of, cell_start, cell_items;
+    DevArray<double> sc_epart, sc_F;
+
+    SysDesc desc{};
+    int pchunks = 1, kchunks = 1;
+    int ncell_side[4] = {1, 1, 1, 1};
+    int ncells = 1;
+
+    void SetCell(const double *basis);
+    void SetConstraints(int64_t M, const double *k, const double *V);
+    void AddConstraint(const double *k, double V);
+    // (re)build the constraint layout and size every buffer for the current N, M, tiles; fills desc
+    void Prepare(hipStream_t stream);
+    bool softcore() const { return std::abs(e0) > 1e-10 && sigma > 0; }
+    size_t DeviceBytes() const;
+};
+
+// A set of systems evaluated together: one launch per kernel stage covers all of them.
+class Batch {
+public:
+    Batch();
+    ~Batch();
+    Batch(const Batch &) = delete;
+    Batch &operator=(const Batch &) = delete;
+
+    hipStream_t stream = nullptr;
+    std::vector<System *> systems;
+    bool dirty = true;
+
+    // rebuild descriptors and work lists (after systems, shapes or tiles change)
+    void Build();
+    // enqueue prepare -> rho -> reduce -> energy (if rho) and the force pass (if forces) for every system;
+    // forces alone reuse the rho coefficients of the previous rho pass
+    void Enqueue(bool rho, bool forces);
+    void Sync();
+
+    struct Range {
+        int off = 0, n = 0;
+    };
+    struct SegGroup {
+        int seg;
+        Range rho, force;
+    };
+    // optional kernel timing: hipEvents around the rho and force contraction launches of every Enqueue
+    bool timing = false;
+    double t_rho_ms = 0, t_force_ms = 0;
+    int64_t n_rho = 0, n_force = 0;
+    void CollectTiming();  // after a stream sync: fold the recorded event pairs into the totals
+
+    DevArray<SysDesc> d_desc;
+    DevArray<WorkItem> d_items;
+    std::vector<WorkItem> h_items;
+    Range r_prep, r_rho_gen, r_red, r_efin, r_f_gen, r_ffin, r_sc_part, r_sc_sys, r_sc_cell;
+    std::vector<SegGroup> seg_groups;
+
+private:
+    hipEvent_t NextEvent();
+    std::vector<hipEvent_t> ev_pool_;
+    size_t ev_used_ = 0;
+    std::vector<std::pair<int, size_t>> ev_marks_;  // (0 rho / 1 force, index of the start event)
+};
+
+// kernel launchers (cc_kernels.hip)
+void launch_prepare(hipStream_t, const SysDesc *, const WorkItem *, int);
+void launch_rho_lattice(hipStream_t, int seg, const SysDesc *, const WorkItem *, int);
+void launch_rho_general(hipStream_t, const SysDesc *, const WorkItem *, int);
+void launch_rho_reduce(hipStream_t, const SysDesc *, const WorkItem *, int);
+void launch_energy_final(hipStream_t, const SysDesc *, const WorkItem *, int);
+void launch_force_lattice(hipStream_t, int seg, const SysDesc *, const WorkItem *, int);
+void launch_force_general(hipStream_t, const SysDesc *, const WorkItem *, int);
+void launch_force_final(hipStream_t, const SysDesc *, const WorkItem *, int);
+void launch_softcore(hipStream_t, const SysDesc *, const WorkItem *part_items, int npart, const WorkItem *sys_items,
+                     int nsys, const WorkItem *cell_items, int ncell);
+
+}  // namespace ccgpu

@@ -1,0 +1,150 @@
+"""Structure optimisation on the GPU engine, mirroring potential/StructureOptimization.{h,cpp}.
+
+  RelaxStructure_NLOPT / RelaxStructure_NLOPT_Emin   StructureOptimization.cpp:1046-1084 (Switch 0: particles move)
+  LuksanPLIS                                          potential/LuksanPLIS.h:35-36
+  CollectiveCoordinateMultiRun                        apps/CCOptimization/main.cpp:1811-1872 (the solve loop)
+  BatchSolver                                         many RelaxStructure solves resident on the device at once
+                                                      (continuous batching; no counterpart in the reference)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, dp, lib
+from .potential import Configuration, ShiftedCCPotential
+
+MaxEnergy = 1e10  # core/etc.h:8
+StructureOptimization_LBFGSHistory = 40  # StructureOptimization.cpp:17
+
+
+def _p(a):
+    return a.ctypes.data_as(dp)
+
+
+def LuksanPLIS(fun, x0, stopval=-np.inf, maxeval=0, mf=40):
+    """Minimise fun(x) -> (f, grad) with PLIS in NLopt's arithmetic.  Returns (x, minf, status, evaluations)."""
+    x = np.ascontiguousarray(x0, dtype=np.float64).copy()
+    n = len(x)
+
+    @_lib.OBJECTIVE
+    def obj(nn, xp, gp, data):
+        xv = np.ctypeslib.as_array(xp, shape=(nn,))
+        f, g = fun(xv.copy())
+        np.ctypeslib.as_array(gp, shape=(nn,))[:] = g
+        return float(f)
+
+    minf = C.c_double(0)
+    ev = C.c_int(0)
+    st = lib().ccgpu_lbfgs_minimize(n, C.cast(obj, C.c_void_p), None, _p(x), C.byref(minf), stopval, maxeval, mf,
+                                    C.byref(ev))
+    return x, minf.value, st, ev.value
+
+
+def _check_switch(Pressure, Switch, MinDistance):
+    if Switch != 0 or Pressure != 0 or MinDistance > 1e-4:
+        raise NotImplementedError("only Switch 0 (particles move), Pressure 0 and MinDistance 0 are on this path")
+
+
+def RelaxStructure_NLOPT_Emin(List: Configuration, pot: ShiftedCCPotential, Pressure=0.0, Switch=0, MinDistance=0.0,
+                              MaxStep=10000, Emin=-MaxEnergy, history=None):
+    """Relax List in place with three L-BFGS passes (rescale 10000, 1, 0.01), stopping once E < Emin.
+    The L-BFGS runs on the host, each objective evaluation is one GPU E+F pass.  Returns (E, evaluations, status)."""
+    _check_switch(Pressure, Switch, MinDistance)
+    history = history or StructureOptimization_LBFGSHistory
+    pot.SetConfiguration(List)
+    rel = np.ascontiguousarray(List.relative, dtype=np.float64).copy()
+    E = C.c_double(0)
+    ev = C.c_int(0)
+    st = C.c_int(0)
+    check(lib().ccgpu_potential_relax(pot._h, _p(rel), int(MaxStep), float(Emin), int(history), C.byref(E),
+                                      C.byref(ev), C.byref(st)))
+    List.relative = rel.reshape(List.relative.shape)
+    pot.RhoValid = False
+    return E.value, ev.value, st.value
+
+
+def RelaxStructure_NLOPT(List, pot, Pressure=0.0, Switch=0, MinDistance=0.0, MaxStep=10000, history=None):
+    return RelaxStructure_NLOPT_Emin(List, pot, Pressure, Switch, MinDistance, MaxStep, -MaxEnergy, history)
+
+
+class BatchSolver:
+    """Device-resident solver: jobs (RelaxStructure_NLOPT[_Emin] requests) are admitted into slots by HBM budget
+    and advance together, one batched E+F launch and one L-BFGS step launch per evaluation."""
+
+    QUEUED, RESIDENT, DONE, FAILED = 0, 1, 2, 3
+
+    def __init__(self, max_slots=256, hbm_budget=200 << 30, check_interval=8):
+        self._h = C.c_void_p()
+        check(lib().ccgpu_solver_create(int(max_slots), int(hbm_budget), C.byref(self._h)))
+        check(lib().ccgpu_solver_set_check_interval(self._h, int(check_interval)))
+        self._shapes = {}
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib._lib is not None:
+            lib().ccgpu_solver_destroy(h)
+            self._h = None
+
+    def submit(self, config: Configuration, k, V, shift=0.0, epsilon=0.0, sigma=0.0, MaxStep=10000, Emin=-MaxEnergy,
+               history=StructureOptimization_LBFGSHistory):
+        d = config.GetDimension()
+        rel = np.ascontiguousarray(config.relative, dtype=np.float64)
+        k = np.ascontiguousarray(k, dtype=np.float64).reshape(-1, d)
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        job = C.c_int64(0)
+        check(lib().ccgpu_solver_submit(self._h, d, _p(np.ascontiguousarray(config.basis)), rel.shape[0], _p(rel),
+                                        k.shape[0], _p(k), _p(V), shift, epsilon, sigma, int(MaxStep), float(Emin),
+                                        int(history), C.byref(job)))
+        self._shapes[job.value] = rel.shape
+        return job.value
+
+    def step(self, evaluations=64):
+        pending = C.c_int(0)
+        check(lib().ccgpu_solver_step(self._h, int(evaluations), C.byref(pending)))
+        return pending.value
+
+    def run(self, chunk=256, timeout=None):
+        t0 = time.time()
+        while self.step(chunk) > 0:
+            if timeout is not None and time.time() - t0 > timeout:
+                raise TimeoutError("BatchSolver.run: timeout")
+
+    def result(self, job):
+        state = C.c_int(0)
+        E = C.c_double(0)
+        ev = C.c_int(0)
+        dev = C.c_int(0)
+        st = C.c_int(0)
+        t = np.zeros(3)
+        rel = np.zeros(self._shapes[job])
+        check(lib().ccgpu_solver_result(self._h, int(job), C.byref(state), C.byref(E), C.byref(ev), C.byref(dev),
+                                        C.byref(st), _p(t), _p(rel)))
+        return {"state": state.value, "energy": E.value, "evaluations": ev.value, "device_evaluations": dev.value,
+                "status": st.value, "t_submit": t[0], "t_admit": t[1], "t_done": t[2], "relative": rel}
+
+    def release(self, job):
+        check(lib().ccgpu_solver_release(self._h, int(job)))
+        self._shapes.pop(job, None)
+
+    def set_timing(self, on=True):
+        check(lib().ccgpu_solver_set_timing(self._h, 1 if on else 0))
+
+    def timing(self, reset=False):
+        """Accumulated (rho_ms, force_ms, rho_launches, force_launches) of the contraction kernels."""
+        a = C.c_double(0)
+        b = C.c_double(0)
+        na = C.c_int64(0)
+        nb = C.c_int64(0)
+        check(lib().ccgpu_solver_timing(self._h, C.byref(a), C.byref(b), C.byref(na), C.byref(nb), 1 if reset else 0))
+        return a.value, b.value, na.value, nb.value
+
+    def stats(self):
+        r = C.c_int(0)
+        q = C.c_int(0)
+        b = C.c_int64(0)
+        check(lib().ccgpu_solver_stats(self._h, C.byref(r), C.byref(q), C.byref(b)))
+        return {"resident": r.value, "queued": q.value, "bytes_in_use": b.value}

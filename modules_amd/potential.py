@@ -1,0 +1,254 @@
+"""Host mirror of the reference's Potential interface for the collective-coordinate path.
+
+Names, argument meaning and error behaviour follow jaeukk/modules:
+  * Configuration / GetUnitCubicBox            core/PeriodicCellList.h:46, core/PeriodicCellList.cpp:106-118
+  * GetKs, getK, get_chi                        core/PeriodicCellList.cpp:509-735, apps/CCOptimization/main.cpp:2018-2026
+  * ShiftedCCPotential                          potential/CollectiveCoordinatePotential.h:53-1497
+  * RepulsiveCCPotential (soft core + CC)       apps/CCOptimization/RepulsiveCCPotential.h:21-157
+Every evaluation runs in the native engine (modules_amd/libccgpu.so) on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, dp, ip32, lib
+
+pi = 3.1415926535897932384626433832795028841971693993751058209  # core/etc.h:9
+
+
+def _p(a):
+    return a.ctypes.data_as(dp)
+
+
+class Configuration:
+    """Periodic point pattern: basis vectors (rows) and relative coordinates in [0,1)."""
+
+    def __init__(self, basis, relative=None):
+        self.basis = np.array(basis, dtype=np.float64, copy=True)
+        d = self.basis.shape[0]
+        if self.basis.shape != (d, d) or not 1 <= d <= 4:
+            raise ValueError("basis must be d x d with 1 <= d <= 4")
+        self.relative = np.zeros((0, d)) if relative is None else np.array(relative, dtype=np.float64).reshape(-1, d)
+        self.relative = _wrap(self.relative)
+
+    def GetDimension(self):
+        return self.basis.shape[0]
+
+    def NumParticle(self):
+        return self.relative.shape[0]
+
+    def GetBasisVector(self, i):
+        return self.basis[i].copy()
+
+    def GetRelativeCoordinates(self, i):
+        return self.relative[i].copy()
+
+    def GetCartesianCoordinates(self, i):
+        return self.relative[i] @ self.basis
+
+    def PeriodicVolume(self):
+        return lib().ccgpu_cell_volume(self.GetDimension(), _p(np.ascontiguousarray(self.basis)))
+
+    def Insert(self, relative):
+        """Insert (core/PeriodicCellList.h:517-538): wrap the relative coordinate into [0,1) twice."""
+        r = _wrap(np.asarray(relative, dtype=np.float64).reshape(1, -1))
+        self.relative = np.vstack([self.relative, r])
+
+    def MoveParticle(self, i, relative):
+        self.relative[i] = _wrap(np.asarray(relative, dtype=np.float64).reshape(1, -1))[0]
+
+    def Rescale(self, factor):
+        self.basis = self.basis * factor
+
+    def copy(self):
+        return Configuration(self.basis, self.relative)
+
+
+def _wrap(s):
+    y = s - np.floor(s)
+    return y - np.floor(y)
+
+
+def GetUnitCubicBox(d, CellSize=0.1):
+    return Configuration(np.eye(d))
+
+
+def RandomConfiguration(seed, N, d, L=None):
+    """GetUnitCubicBox(d) rescaled by L (default pow(N, 1.0/d), density 1) with N particles drawn from
+    RandomGenerator(seed) (apps/CCOptimization/main.cpp:198, 207-215)."""
+    s = np.zeros((N, d))
+    check(lib().ccgpu_random_configuration(seed & 0xFFFFFFFF, N, d, _p(s)))
+    if L is None:
+        L = math.pow(N, 1.0 / d)
+    return Configuration(np.eye(d) * L, s)
+
+
+def getK(chi, rho, d):
+    return lib().ccgpu_get_k(chi, rho, d)
+
+
+def get_chi(K, rho, d):
+    c = {1: 2.0, 2: pi, 3: 4.1887902047863909846, 4: 4.9348022005446793094}[d]
+    return c * K ** d / (2 * d * (2 * pi) ** d) / rho
+
+
+def GetKs(config, CircularKMax, LinearKMax=None, SampleProbability=1.0, return_indices=False):
+    """GetKs(tempList, CircularKMax, LinearKMax, SampleProbability): the wavevectors with |k| < K (half space).
+    Only the LinearKMax == CircularKMax, SampleProbability >= 1 form used on the stealthy path is offered."""
+    if LinearKMax is None:
+        LinearKMax = CircularKMax
+    if LinearKMax != CircularKMax or SampleProbability < 1.0:
+        raise NotImplementedError("GetKs: only LinearKMax == CircularKMax and SampleProbability >= 1")
+    basis = np.ascontiguousarray(config.basis if isinstance(config, Configuration) else config, dtype=np.float64)
+    d = basis.shape[0]
+    M = C.c_int64(0)
+    check(lib().ccgpu_get_ks(d, _p(basis), CircularKMax, 0, None, None, C.byref(M)))
+    k = np.zeros((max(M.value, 1), d))
+    n = np.zeros((max(M.value, 1), d), dtype=np.int32)
+    check(lib().ccgpu_get_ks(d, _p(basis), CircularKMax, M.value, _p(k), n.ctypes.data_as(ip32), C.byref(M)))
+    k, n = k[: M.value], n[: M.value]
+    return (k, n) if return_indices else k
+
+
+class KPointConstraint:
+    """One constraint: absolute wavevector k and weight V (CollectiveCoordinatePotential.h:84-94)."""
+
+    __slots__ = ("k", "V")
+
+    def __init__(self, k, V):
+        self.k = np.asarray(k, dtype=np.float64).copy()
+        self.V = float(V)
+
+
+class ShiftedCCPotential:
+    """Phi = (1/V) sum_k V_k |rho(k)|^2 + Shift on the GPU (CollectiveCoordinatePotential.h:53-1497)."""
+
+    def __init__(self, dimension):
+        self.Dimension = int(dimension)
+        self.constraints = []
+        self.Shift = 0.0
+        self.ParallelNumThread = 1  # kept for interface parity; the GPU does not use it
+        self.RhoReal = np.zeros(0)
+        self.RhoImag = np.zeros(0)
+        self.RhoValid = False
+        self._h = C.c_void_p()
+        check(lib().ccgpu_potential_create(self.Dimension, C.byref(self._h)))
+        self._pushed_k = None
+        self._pushed_v = None
+        self._pushed_shift = None
+        self._softcore = (0.0, 0.0)
+        self._pushed_softcore = None
+        self._config = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib._lib is not None:
+            lib().ccgpu_potential_destroy(h)
+            self._h = None
+
+    # --- constraints -------------------------------------------------------------------------
+    def AddConstraint(self, k, vals):
+        """AddConstraint(k, vals): vals[0] is the weight; empty vals raises ValueError (std::invalid_argument)."""
+        if len(vals) == 0:
+            raise ValueError("ShiftedCCPotential::AddConstraint: vals must contain at least the weight V(k).")
+        self.constraints.append(KPointConstraint(k, vals[0]))
+        self.RhoValid = False
+
+    def _sync(self):
+        d = self.Dimension
+        M = len(self.constraints)
+        k = np.zeros((M, d))
+        v = np.zeros(M)
+        for i, c in enumerate(self.constraints):
+            k[i] = c.k[:d]
+            v[i] = c.V
+        if self._pushed_k is None or not (np.array_equal(k, self._pushed_k) and np.array_equal(v, self._pushed_v)):
+            check(lib().ccgpu_potential_set_constraints(self._h, M, _p(np.ascontiguousarray(k)), _p(v)))
+            self._pushed_k, self._pushed_v = k, v
+            self.RhoValid = False
+        if self._pushed_shift != self.Shift:
+            check(lib().ccgpu_potential_set_shift(self._h, float(self.Shift)))
+            self._pushed_shift = self.Shift
+        if self._pushed_softcore != self._softcore:
+            check(lib().ccgpu_potential_set_softcore(self._h, *self._softcore))
+            self._pushed_softcore = self._softcore
+            self.RhoValid = False
+
+    # --- binding -----------------------------------------------------------------------------
+    def SetConfiguration(self, config):
+        """Bind a configuration and invalidate rho (CollectiveCoordinatePotential.h:1457-1463)."""
+        if config.GetDimension() != self.Dimension:
+            raise ValueError("dimension mismatch")
+        self._config = config
+        self._sync()
+        rel = np.ascontiguousarray(config.relative, dtype=np.float64)
+        check(lib().ccgpu_potential_set_configuration(self._h, _p(np.ascontiguousarray(config.basis)),
+                                                      rel.shape[0], _p(rel)))
+        self.RhoValid = False
+
+    def _require(self):
+        if self._config is None:
+            raise RuntimeError("no configuration bound: call SetConfiguration first")
+        self._sync()
+
+    def GetRho(self):
+        self._require()
+        M = len(self.constraints)
+        re = np.zeros(max(M, 1))
+        im = np.zeros(max(M, 1))
+        check(lib().ccgpu_potential_get_rho(self._h, _p(re), _p(im)))
+        self.RhoReal, self.RhoImag = re[:M], im[:M]
+        self.RhoValid = True
+
+    def Energy(self):
+        self._require()
+        E = C.c_double(0)
+        check(lib().ccgpu_potential_energy(self._h, C.byref(E)))
+        return E.value
+
+    def Force(self, i):
+        self._require()
+        F = np.zeros(self.Dimension)
+        check(lib().ccgpu_potential_force(self._h, int(i), _p(F)))
+        return F
+
+    def AllForce(self):
+        self._require()
+        F = np.zeros((self._config.NumParticle(), self.Dimension))
+        check(lib().ccgpu_potential_all_force(self._h, _p(F)))
+        return F
+
+    def EnergyAndAllForce(self):
+        """CombinedEnergyForce::EnergyAndAllForce (potential/CombinedEnergyForce.h:17-28)."""
+        self._require()
+        E = C.c_double(0)
+        F = np.zeros((self._config.NumParticle(), self.Dimension))
+        check(lib().ccgpu_potential_energy_and_all_force(self._h, C.byref(E), _p(F)))
+        return E.value, F
+
+    def SetTiles(self, segment_len=0, rho_chunk=0, force_chunk=0):
+        check(lib().ccgpu_potential_set_tiles(self._h, segment_len, rho_chunk, force_chunk))
+
+    def UsesLatticePath(self):
+        v = C.c_int(0)
+        check(lib().ccgpu_potential_uses_lattice_path(self._h, C.byref(v)))
+        return bool(v.value)
+
+
+class RepulsiveCCPotential(ShiftedCCPotential):
+    """Soft core e0 (1 - r/sigma)^2 for r < sigma plus the CC potential
+    (apps/CCOptimization/RepulsiveCCPotential.h:21-157).  Constraints go to .CCPotential, as in the reference."""
+
+    def __init__(self, dim, epsilon, sigma, S0=0.0):
+        if S0 != 0.0:
+            raise NotImplementedError("RepulsiveCCPotential: the S2 kernel (S0 != 0) is not on this path")
+        super().__init__(dim)
+        self.e0 = float(epsilon)
+        self.sigma = float(sigma)
+        self.WithRepulsion = abs(self.e0) > 1e-10
+        self._softcore = (self.e0, self.sigma) if self.WithRepulsion else (0.0, 0.0)
+        self.CCPotential = self  # constraints live on the same engine object

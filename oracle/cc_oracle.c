@@ -415,11 +415,7 @@ void orc_softcore_forces(int d, int64_t N, const double *s, const double *a, dou
 			for (int q = 0; q < nimg; q++) {
 				double sh[ORC_MAXDIM], dv[ORC_MAXDIM];
 				orc_image_shift(d, a, reach, q, sh);
-				if (j == i) {
-					int zero = 1;
-					for (int c = 0; c < ORC_MAXDIM; c++) zero = zero && sh[c] == 0.0;
-					if (zero) continue;
-				}
+				if (j == i) continue; /* ForceLJ skips the particle itself, all images (RepulsiveCCPotential.h:99) */
 				for (int c = 0; c < ORC_MAXDIM; c++) dv[c] = (r[j * ORC_MAXDIM + c] - r[i * ORC_MAXDIM + c]) + sh[c];
 				double m2 = gv_dot(dv, dv);
 				if (!(m2 < sigma * sigma)) continue;

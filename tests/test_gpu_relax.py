@@ -1,0 +1,129 @@
+"""GPU stealthy ground-state solves: the single-solve drop-in path (host L-BFGS, GPU E+F) and the batched
+device-resident solver, checked against the oracle relax driven by the reference LuksanPLIS."""
+import numpy as np
+import pytest
+
+from modules_amd import GetKs, RandomConfiguration, RepulsiveCCPotential, ShiftedCCPotential, getK
+from modules_amd.optimize import BatchSolver, LuksanPLIS, RelaxStructure_NLOPT_Emin
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+EMIN = 1e-18
+
+
+def stealthy(N, d, chi, seed):
+    cfg = RandomConfiguration(seed, N, d)
+    k = GetKs(cfg, getK(chi, 1.0, d))
+    return cfg, k, np.ones(len(k))
+
+
+def test_single_solve_reaches_ground_state_like_reference():
+    cfg, k, v = stealthy(200, 2, 0.3, 0)
+    pot = ShiftedCCPotential(2)
+    for kk in k:
+        pot.AddConstraint(kk, [1.0])
+    E, evals, st = RelaxStructure_NLOPT_Emin(cfg, pot, MaxStep=10000, Emin=EMIN)
+    Eo, so, evo, sto = po.relax_emin(po.make_config(0, 200, 2), cfg.basis, k, v, Emin=EMIN)
+    assert E < EMIN and Eo < EMIN
+    assert st == 2 and sto == 2
+    assert 0.5 * evo < evals < 2.0 * evo
+    # the relaxed configuration is a ground state for the oracle too
+    Ef, Ff = po.total_energy_forces(cfg.relative, cfg.basis, k, v)
+    assert Ef < 10 * EMIN
+
+
+def test_objective_matches_oracle_every_100th_evaluation():
+    # configuration 2 style check (3D, chi 0.45) at a reduced size: E and gradient of the GPU objective
+    # against the oracle at every 100th evaluation of an L-BFGS pass
+    N, d = 400, 3
+    cfg, k, v = stealthy(N, d, 0.45, 3)
+    pot = ShiftedCCPotential(d)
+    for kk in k:
+        pot.AddConstraint(kk, [1.0])
+    a = cfg.basis
+    vol = po.volume(a)
+    escale = len(k) * N * N / vol
+    fscale = 4 * len(k) * N * np.max(np.linalg.norm(k, axis=1)) / vol
+    count = [0]
+    worst = [0.0, 0.0]
+
+    def fun(x):
+        s = x.reshape(N, d) % 1.0
+        from modules_amd import Configuration
+        c = Configuration(a, s)
+        pot.SetConfiguration(c)
+        E, F = pot.EnergyAndAllForce()
+        if count[0] % 100 == 0:
+            Eo, Fo = po.total_energy_forces(c.relative, a, k, v)
+            worst[0] = max(worst[0], abs(E - Eo) / escale)
+            worst[1] = max(worst[1], np.max(np.abs(F - Fo)) / fscale)
+        count[0] += 1
+        return E, -(F @ a.T).ravel()
+
+    LuksanPLIS(fun, cfg.relative.ravel(), stopval=EMIN, maxeval=600)
+    assert count[0] >= 200
+    assert worst[0] <= 1e-12 and worst[1] <= 1e-12
+
+
+def test_batch_solver_mixed_sizes_all_reach_ground_state():
+    solver = BatchSolver(max_slots=16)
+    jobs = []
+    specs = [(150, 2, 0.3, 1), (300, 3, 0.4, 2), (500, 2, 0.2, 3), (120, 3, 0.45, 4), (800, 2, 0.35, 5)]
+    for N, d, chi, seed in specs:
+        cfg, k, v = stealthy(N, d, chi, seed)
+        jobs.append((solver.submit(cfg, k, v, MaxStep=10000, Emin=EMIN), cfg, k, v))
+    solver.run(chunk=128, timeout=300)
+    for job, cfg, k, v in jobs:
+        r = solver.result(job)
+        assert r["state"] == BatchSolver.DONE
+        assert r["energy"] < EMIN
+        Eo, _ = po.total_energy_forces(r["relative"], cfg.basis, k, v)
+        assert Eo < 10 * EMIN
+        assert 10 < r["evaluations"] < 20000
+
+
+def test_batch_result_is_the_same_alone_and_in_the_mix():
+    cfg, k, v = stealthy(400, 3, 0.4, 11)
+    alone = BatchSolver(max_slots=4)
+    j0 = alone.submit(cfg, k, v, Emin=EMIN)
+    alone.run(timeout=300)
+    ra = alone.result(j0)
+    mix = BatchSolver(max_slots=8)
+    others = [stealthy(N, d, chi, s) for N, d, chi, s in ((250, 2, 0.3, 21), (600, 3, 0.25, 22), (90, 2, 0.45, 23))]
+    mix.submit(*others[0], Emin=EMIN)
+    j1 = mix.submit(cfg, k, v, Emin=EMIN)
+    for o in others[1:]:
+        mix.submit(*o, Emin=EMIN)
+    mix.run(timeout=300)
+    rm = mix.result(j1)
+    assert ra["energy"] == rm["energy"]
+    assert ra["evaluations"] == rm["evaluations"]
+    assert np.array_equal(ra["relative"], rm["relative"])
+
+
+def test_batch_admission_by_budget_and_release():
+    solver = BatchSolver(max_slots=2, hbm_budget=64 << 20)
+    ids = [solver.submit(*stealthy(200, 2, 0.3, s), Emin=EMIN) for s in range(5)]
+    st = solver.stats()
+    assert st["resident"] == 0 and st["queued"] == 5
+    solver.step(1)
+    assert solver.stats()["resident"] <= 2
+    solver.run(timeout=300)
+    for j in ids:
+        assert solver.result(j)["energy"] < EMIN
+        solver.release(j)
+    assert solver.stats()["bytes_in_use"] == 0
+
+
+def test_softcore_energy_forces_match_oracle():
+    for d, N, sigma in ((2, 300, 0.4), (3, 250, 0.4 * 1.0)):
+        cfg, k, v = stealthy(N, d, 0.3, 7)
+        pot = RepulsiveCCPotential(d, 2.0, sigma)
+        for kk in k:
+            pot.AddConstraint(kk, [1.0])
+        pot.SetConfiguration(cfg)
+        E, F = pot.EnergyAndAllForce()
+        Eo, Fo = po.total_energy_forces(cfg.relative, cfg.basis, k, v, sigma=sigma, e0=2.0)
+        assert abs(E - Eo) <= 1e-12 * max(1.0, abs(Eo))
+        assert np.max(np.abs(F - Fo)) <= 1e-11 * max(1.0, np.max(np.abs(Fo)))
