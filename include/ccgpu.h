@@ -101,6 +101,10 @@ int ccgpu_potential_energy_and_all_force(ccgpu_potential *p, double *E, double *
  * the autotuner. */
 int ccgpu_potential_set_tiles(ccgpu_potential *p, int segment_len, int rho_chunk, int force_chunk);
 
+/* rho kernel of the lattice path: 0 phase recurrence along each segment, 1 per-particle power tables in LDS,
+ * -1 automatic.  No counterpart in the reference (autotuner hook). */
+int ccgpu_potential_set_rho_kernel(ccgpu_potential *p, int variant);
+
 /* Which kernel family the bound constraint set uses: 1 lattice (segment recurrence), 0 general. */
 int ccgpu_potential_uses_lattice_path(const ccgpu_potential *p, int *lattice);
 

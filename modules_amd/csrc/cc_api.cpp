@@ -384,6 +384,17 @@ int ccgpu_potential_relax(ccgpu_potential *p, double *relative, int maxstep, dou
     });
 }
 
+int ccgpu_potential_set_rho_kernel(ccgpu_potential *p, int variant)
+{
+    return Guard([&] {
+        Require(p != nullptr, CCGPU_ERR_ARG, "null handle");
+        Require(variant >= -1 && variant <= 1, CCGPU_ERR_ARG, "rho kernel variant must be -1, 0 or 1");
+        p->sys.tiles.rho_kernel = variant;
+        p->batch.dirty = true;
+        p->Invalidate();
+    });
+}
+
 int ccgpu_potential_uses_lattice_path(const ccgpu_potential *p, int *lattice)
 {
     return Guard([&] {

@@ -120,6 +120,75 @@ __global__ void __launch_bounds__(kBlock) k_rho_lattice(const SysDesc *__restric
 }
 
 // ---------------------------------------------------------------------------------------------
+// rho, lattice path with per-particle power tables.  items as k_rho_lattice.  The block stages, for
+// sub-chunks of 64 particles, the powers P_t = exp(2 pi i t s_{d-1}), t < SEG, in LDS (one sincos per
+// table entry, amortised over the block's 256 segments).  Each (segment, particle) then needs one base
+// sincos and SEG independent complex multiply-accumulates rho_t += base * P_t (4 FMA per pair, no serial
+// phase chain).
+// ---------------------------------------------------------------------------------------------
+template <int SEG>
+__global__ void __launch_bounds__(kBlock) k_rho_lattice_tab(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
+{
+    constexpr int kSub = 64;
+    __shared__ double2 lpow[kSub * SEG];
+    __shared__ double4 lpos[kSub];
+    const WorkItem it = items[blockIdx.x];
+    const SysDesc &S = sys[it.sys];
+    if (!(S.flags & kSysActive)) return;
+    const int j0 = it.t1 * S.pchunk_len;
+    const int j1 = min(S.N, j0 + S.pchunk_len);
+    const int seg = it.t0 * kBlock + threadIdx.x;
+    const bool valid = seg < S.nseg;
+    const int4 h = valid ? S.seg_head[seg] : make_int4(0, 0, 0, 0);
+    const double h0 = (double)h.x, h1 = (double)h.y, h2 = (double)h.z, h3 = (double)h.w;
+    const int dl = S.d - 1;
+    double re[SEG], im[SEG];
+#pragma unroll
+    for (int t = 0; t < SEG; t++) { re[t] = 0.0; im[t] = 0.0; }
+    for (int jb = j0; jb < j1; jb += kSub) {
+        const int nsub = min(kSub, j1 - jb);
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < nsub * SEG; idx += kBlock) {
+            const int j = idx / SEG, t = idx - j * SEG;
+            const double4 p = S.pos[jb + j];
+            const double sl = dl == 0 ? p.x : dl == 1 ? p.y : dl == 2 ? p.z : p.w;
+            // t * s with its exact rounding residual, reduced to revolutions
+            const double pr = (double)t * sl;
+            const double er = __builtin_fma((double)t, sl, -pr);
+            double sn, cs;
+            sincos_rev((pr - __builtin_rint(pr)) + er, sn, cs);
+            lpow[idx] = make_double2(cs, sn);
+        }
+        for (int idx = threadIdx.x; idx < nsub; idx += kBlock) lpos[idx] = S.pos[jb + idx];
+        __syncthreads();
+        if (valid) {
+            for (int j = 0; j < nsub; j++) {
+                const double4 p = lpos[j];
+                const double u = __builtin_fma(h0, p.x, __builtin_fma(h1, p.y, __builtin_fma(h2, p.z, h3 * p.w)));
+                double bi, br;
+                sincos_rev(u, bi, br);
+                const double2 *P = lpow + j * SEG;
+#pragma unroll
+                for (int t = 0; t < SEG; t++) {
+                    const double2 q = P[t];
+                    re[t] = __builtin_fma(br, q.x, re[t]);
+                    re[t] = __builtin_fma(-bi, q.y, re[t]);
+                    im[t] = __builtin_fma(br, q.y, im[t]);
+                    im[t] = __builtin_fma(bi, q.x, im[t]);
+                }
+            }
+        }
+    }
+    if (!valid) return;
+    double *out = S.rpart + ((int64_t)it.t1 * S.nslots + (int64_t)seg * SEG) * 2;
+#pragma unroll
+    for (int t = 0; t < SEG; t++) {
+        out[2 * t] = re[t];
+        out[2 * t + 1] = im[t];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
 // rho, general path.  items: (sys, constraint block, particle chunk).  One lane = one constraint.
 // ---------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_rho_general(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
@@ -349,6 +418,7 @@ __global__ void __launch_bounds__(kBlock) k_force_final(const SysDesc *__restric
 // explicit instantiations of the segment lengths the autotuner may choose
 #define CC_INSTANTIATE_SEG(SEG)                                                                            \
     template __global__ void k_rho_lattice<SEG>(const SysDesc *__restrict__, const WorkItem *__restrict__); \
+    template __global__ void k_rho_lattice_tab<SEG>(const SysDesc *__restrict__, const WorkItem *__restrict__); \
     template __global__ void k_force_lattice<SEG>(const SysDesc *__restrict__, const WorkItem *__restrict__);
 CC_INSTANTIATE_SEG(4)
 CC_INSTANTIATE_SEG(8)
@@ -370,9 +440,19 @@ void launch_prepare(hipStream_t stream, const SysDesc *sys, const WorkItem *item
     if (n > 0) hipLaunchKernelGGL(k_prepare, CC_GRID(n), sys, items);
 }
 
-void launch_rho_lattice(hipStream_t stream, int seg, const SysDesc *sys, const WorkItem *items, int n)
+void launch_rho_lattice(hipStream_t stream, int seg, const SysDesc *sys, const WorkItem *items, int n, int variant)
 {
     if (n <= 0) return;
+    if (variant == 1) {
+        switch (seg) {
+            case 4: hipLaunchKernelGGL(k_rho_lattice_tab<4>, CC_GRID(n), sys, items); break;
+            case 8: hipLaunchKernelGGL(k_rho_lattice_tab<8>, CC_GRID(n), sys, items); break;
+            case 16: hipLaunchKernelGGL(k_rho_lattice_tab<16>, CC_GRID(n), sys, items); break;
+            case 24: hipLaunchKernelGGL(k_rho_lattice_tab<24>, CC_GRID(n), sys, items); break;
+            default: hipLaunchKernelGGL(k_rho_lattice_tab<32>, CC_GRID(n), sys, items); break;
+        }
+        return;
+    }
     switch (seg) {
         case 4: hipLaunchKernelGGL(k_rho_lattice<4>, CC_GRID(n), sys, items); break;
         case 8: hipLaunchKernelGGL(k_rho_lattice<8>, CC_GRID(n), sys, items); break;

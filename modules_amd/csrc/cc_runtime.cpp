@@ -27,6 +27,7 @@ Tiles DefaultTiles(int d, int64_t N, int64_t M)
     c = ((c + 31) / 32) * 32;
     t.rho_chunk = (int)std::min<int64_t>(512, std::max<int64_t>(64, c));
     t.force_chunk = 0;  // resolved in Prepare from the segment count
+    t.rho_kernel = 1;
     return t;
 }
 
@@ -82,6 +83,7 @@ void System::Prepare(hipStream_t stream)
     if (used.seg != 4 && used.seg != 8 && used.seg != 16 && used.seg != 24 && used.seg != 32)
         throw Error(kErrArg, "segment length must be 4, 8, 16, 24 or 32");
     used.rho_chunk = tiles.rho_chunk > 0 ? std::min(tiles.rho_chunk, 512) : def.rho_chunk;
+    used.rho_kernel = tiles.rho_kernel >= 0 ? tiles.rho_kernel : def.rho_kernel;
 
     if (layout_dirty || plan.S != used.seg) {
         lattice = M > 0 && LatticeIndices(d, a, M, k.data(), n);
@@ -273,17 +275,21 @@ void Batch::Build()
         for (int i = 0; i < ns; i++)
             for (int b = 0; b < pblocks(systems[i]); b++) h_items.push_back({i, b, 0, 0});
     });
-    // lattice systems grouped by segment length (one template instance per group)
-    std::vector<int> segs;
-    for (System *s : systems)
-        if (s->lattice && std::find(segs.begin(), segs.end(), s->used.seg) == segs.end()) segs.push_back(s->used.seg);
-    for (int sl : segs) {
+    // lattice systems grouped by (segment length, rho kernel): one template instance per group
+    std::vector<std::pair<int, int>> segs;
+    for (System *s : systems) {
+        const std::pair<int, int> key(s->used.seg, s->used.rho_kernel);
+        if (s->lattice && std::find(segs.begin(), segs.end(), key) == segs.end()) segs.push_back(key);
+    }
+    for (const auto &key : segs) {
+        const int sl = key.first, rk = key.second;
         SegGroup g;
         g.seg = sl;
+        g.rho_kernel = rk;
         add(g.rho, [&] {
             for (int i = 0; i < ns; i++) {
                 System *s = systems[i];
-                if (!s->lattice || s->used.seg != sl || s->N == 0) continue;
+                if (!s->lattice || s->used.seg != sl || s->used.rho_kernel != rk || s->N == 0) continue;
                 const int sb = (int)((s->plan.nseg + kBlock - 1) / kBlock);
                 for (int c = 0; c < s->pchunks; c++)
                     for (int b = 0; b < sb; b++) h_items.push_back({i, b, c, 0});
@@ -292,7 +298,7 @@ void Batch::Build()
         add(g.force, [&] {
             for (int i = 0; i < ns; i++) {
                 System *s = systems[i];
-                if (!s->lattice || s->used.seg != sl) continue;
+                if (!s->lattice || s->used.seg != sl || s->used.rho_kernel != rk) continue;
                 for (int c = 0; c < s->kchunks; c++)
                     for (int b = 0; b < pblocks(s); b++) h_items.push_back({i, b, c, 0});
             }
@@ -368,7 +374,7 @@ void Batch::Enqueue(bool rho, bool forces)
             ev_marks_.push_back({0, ev_used_});
             CC_HIP(hipEventRecord(NextEvent(), stream));
         }
-        for (const SegGroup &g : seg_groups) launch_rho_lattice(stream, g.seg, D, I + g.rho.off, g.rho.n);
+        for (const SegGroup &g : seg_groups) launch_rho_lattice(stream, g.seg, D, I + g.rho.off, g.rho.n, g.rho_kernel);
         launch_rho_general(stream, D, I + r_rho_gen.off, r_rho_gen.n);
         if (timing) CC_HIP(hipEventRecord(NextEvent(), stream));
         launch_rho_reduce(stream, D, I + r_red.off, r_red.n);

@@ -70,6 +70,7 @@ struct Tiles {
     int seg = 16;
     int rho_chunk = 0;
     int force_chunk = 0;
+    int rho_kernel = -1;  // 0: phase recurrence, 1: LDS power tables; -1 automatic
 };
 Tiles DefaultTiles(int d, int64_t N, int64_t M);
 
@@ -142,7 +143,7 @@ public:
         int off = 0, n = 0;
     };
     struct SegGroup {
-        int seg;
+        int seg, rho_kernel;
         Range rho, force;
     };
     // optional kernel timing: hipEvents around the rho and force contraction launches of every Enqueue
@@ -166,7 +167,7 @@ private:
 
 // kernel launchers (cc_kernels.hip)
 void launch_prepare(hipStream_t, const SysDesc *, const WorkItem *, int);
-void launch_rho_lattice(hipStream_t, int seg, const SysDesc *, const WorkItem *, int);
+void launch_rho_lattice(hipStream_t, int seg, const SysDesc *, const WorkItem *, int, int variant);
 void launch_rho_general(hipStream_t, const SysDesc *, const WorkItem *, int);
 void launch_rho_reduce(hipStream_t, const SysDesc *, const WorkItem *, int);
 void launch_energy_final(hipStream_t, const SysDesc *, const WorkItem *, int);

@@ -233,6 +233,9 @@ class ShiftedCCPotential:
     def SetTiles(self, segment_len=0, rho_chunk=0, force_chunk=0):
         check(lib().ccgpu_potential_set_tiles(self._h, segment_len, rho_chunk, force_chunk))
 
+    def SetRhoKernel(self, variant):
+        check(lib().ccgpu_potential_set_rho_kernel(self._h, int(variant)))
+
     def UsesLatticePath(self):
         v = C.c_int(0)
         check(lib().ccgpu_potential_uses_lattice_path(self._h, C.byref(v)))

@@ -64,10 +64,22 @@ def test_stealthy_configs_match_oracle(N, d, chi, seed):
 
 
 @pytest.mark.parametrize("seg", [4, 8, 16, 24, 32])
-def test_segment_lengths_agree(seg):
+@pytest.mark.parametrize("rho_kernel", [0, 1])
+def test_segment_lengths_agree(seg, rho_kernel):
     cfg = RandomConfiguration(11, 700, 2)
     k = GetKs(cfg, getK(0.4, 1.0, 2))
-    compare(cfg, k, np.ones(len(k)), tiles=(seg, 64, 7))
+    pot = make_pot(2, k, np.ones(len(k)))
+    pot.SetRhoKernel(rho_kernel)
+    pot.SetConfiguration(cfg)
+    pot.SetTiles(seg, 96, 7)
+    pot.GetRho()
+    E, F = pot.EnergyAndAllForce()
+    re, im = po.rho(cfg.relative, cfg.basis, k)
+    vol = po.volume(cfg.basis)
+    sr, se, sf = scales(k, np.ones(len(k)), 700, vol)
+    assert np.max(np.abs(pot.RhoReal - re)) <= RTOL * sr and np.max(np.abs(pot.RhoImag - im)) <= RTOL * sr
+    assert abs(E - po.energy(np.ones(len(k)), re, im, vol)) <= RTOL * se
+    assert np.max(np.abs(F - po.forces(cfg.relative, cfg.basis, k, np.ones(len(k)), re, im))) <= RTOL * sf
 
 
 def test_general_path_random_wavevectors():
