@@ -40,7 +40,22 @@ def parse():
     ap.add_argument("--check-interval", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="run one torch device copy of this size first (calibrates rocprof FETCH/WRITE_SIZE)")
+    ap.add_argument("--graph", action="store_true", help="replay optimizer steps from a captured hipGraph")
+    ap.add_argument("--no-autotune", action="store_true", help="skip the tile autotuner (default tiles)")
     return ap.parse_args()
+
+
+def copy_probe(mib):
+    import torch
+    n = (mib << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"copy probe: {mib} MiB read + {mib} MiB written", file=sys.stderr, flush=True)
 
 
 def main():
@@ -71,7 +86,13 @@ def main():
         except ImportError:
             pass
 
-    solver = BatchSolver(max_slots=max(args.batch, 1), check_interval=args.check_interval)
+    if args.copy_probe_mib > 0:
+        copy_probe(args.copy_probe_mib)
+    tiles = None
+    if not args.no_autotune:
+        from modules_amd import autotune
+        tiles = autotune.autotune(N, d, k=k, basis=proto.basis, batch=args.batch, repeats=3)
+    solver = BatchSolver(max_slots=max(args.batch, 1), check_interval=args.check_interval, graph=args.graph)
 
     def run_step(cfgs):
         ids = [solver.submit(c, k, v, MaxStep=args.maxstep, Emin=args.emin, history=args.history) for c in cfgs]
@@ -122,7 +143,8 @@ def main():
         "data": "synthetic (seeded MT19937 uniform configurations, GetKs constraint set)",
         "config": {"workload": f"batched stealthy relax: {args.batch} seeds x N={N}, d={d}, chi={chi} "
                                f"(M={M}), Emin={args.emin}, 3-pass L-BFGS history {args.history}",
-                   "batch": args.batch, "N": N, "dim": d, "chi": chi, "constraints": M},
+                   "batch": args.batch, "N": N, "dim": d, "chi": chi, "constraints": M, "tiles": tiles,
+                   "graph": bool(args.graph)},
         "solved": solved,
         "solves": len(results),
         "ef_evals_per_s": dev_evals / wall,

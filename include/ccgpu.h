@@ -108,6 +108,17 @@ int ccgpu_potential_set_rho_kernel(ccgpu_potential *p, int variant);
 /* Which kernel family the bound constraint set uses: 1 lattice (segment recurrence), 0 general. */
 int ccgpu_potential_uses_lattice_path(const ccgpu_potential *p, int *lattice);
 
+/* Autotuner (no counterpart in the reference): time candidate tile shapes (segment length, rho kernel,
+ * rho particle chunk, force constraint chunk) on `batch` random configurations of N particles in the cell
+ * `basis` with constraints (k M x dim, V M), `repeats` batched E+F evaluations each, and record the fastest
+ * for systems of this (dim, N, M) shape class (half-octave buckets).  Potentials and solvers prepared later
+ * use the recorded tiles; a system's tiles (and so its bits) depend only on its shape and this table.
+ * Outputs the chosen tiles and the milliseconds of one batched E+F. */
+int ccgpu_autotune(int dim, int64_t N, const double *basis, int64_t M, const double *k, const double *V, int batch,
+                   int repeats, int *segment_len, int *rho_kernel, int *rho_chunk, int *force_chunk, double *ms);
+/* Forget every recorded tile shape. */
+int ccgpu_autotune_clear(void);
+
 /* ---------------------------------------------------------------------------------------------
  * L-BFGS (Luksan's PLIS, the algorithm of NLopt's NLOPT_LD_LBFGS) on the host
  * ------------------------------------------------------------------------------------------- */
@@ -153,6 +164,8 @@ int ccgpu_solver_submit(ccgpu_solver *s, int dim, const double *basis, int64_t N
 int ccgpu_solver_step(ccgpu_solver *s, int evaluations, int *pending);
 /* Evaluations between two host checks for finished jobs (default 8). */
 int ccgpu_solver_set_check_interval(ccgpu_solver *s, int evaluations);
+/* Replay the steps between two checks as one captured hipGraph (1) or launch them on the stream (0, default). */
+int ccgpu_solver_set_graph(ccgpu_solver *s, int on);
 /* State of a job: 0 queued, 1 resident, 2 done, 3 failed.  When done: final energy, objective evaluations,
  * E+F evaluations on the device, NLopt code of the last pass, and times (submit, admit, done; seconds on the
  * solver clock) in t[3]; relative (N x dim, may be NULL) receives the relaxed configuration. */

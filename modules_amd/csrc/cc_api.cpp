@@ -384,6 +384,27 @@ int ccgpu_potential_relax(ccgpu_potential *p, double *relative, int maxstep, dou
     });
 }
 
+int ccgpu_autotune(int dim, int64_t N, const double *basis, int64_t M, const double *k, const double *V, int batch,
+                   int repeats, int *segment_len, int *rho_kernel, int *rho_chunk, int *force_chunk, double *ms)
+{
+    return Guard([&] {
+        Require(dim >= 1 && dim <= kMaxDim && basis && N > 0 && M > 0 && k && V, CCGPU_ERR_ARG, "bad argument");
+        RequireDevice();
+        Tiles t;
+        const double m = Autotune(dim, N, basis, M, k, V, batch, repeats, t);
+        if (segment_len) *segment_len = t.seg;
+        if (rho_kernel) *rho_kernel = t.rho_kernel;
+        if (rho_chunk) *rho_chunk = t.rho_chunk;
+        if (force_chunk) *force_chunk = t.force_chunk;
+        if (ms) *ms = m;
+    });
+}
+
+int ccgpu_autotune_clear(void)
+{
+    return Guard([&] { ClearTiles(); });
+}
+
 int ccgpu_potential_set_rho_kernel(ccgpu_potential *p, int variant)
 {
     return Guard([&] {
@@ -468,6 +489,14 @@ int ccgpu_solver_set_check_interval(ccgpu_solver *s, int evaluations)
     return Guard([&] {
         Require(s != nullptr && evaluations >= 1, CCGPU_ERR_ARG, "bad argument");
         s->solver.SetCheckInterval(evaluations);
+    });
+}
+
+int ccgpu_solver_set_graph(ccgpu_solver *s, int on)
+{
+    return Guard([&] {
+        Require(s != nullptr, CCGPU_ERR_ARG, "null handle");
+        s->solver.SetGraph(on != 0);
     });
 }
 

@@ -36,6 +36,7 @@ struct SysDesc {
 
     // constraints
     const int4 *seg_head;   // lattice: start index of each segment (nseg)
+    const int32_t *seg_len; // lattice: constraints in each segment (nseg), non-increasing
     const int32_t *slot_k;  // lattice: constraint index of each slot (nslots), -1 for padding
     const double *slot_v;   // weight V of each slot (nslots); padding slots hold 0
     const double4 *kvec;    // general: Cartesian k (M)

@@ -92,6 +92,7 @@ __global__ void __launch_bounds__(kBlock) k_rho_lattice(const SysDesc *__restric
     if (seg >= S.nseg) return;
     const int4 h = S.seg_head[seg];
     const double h0 = (double)h.x, h1 = (double)h.y, h2 = (double)h.z, h3 = (double)h.w;
+    const int wlen = __builtin_amdgcn_readfirstlane(S.seg_len[it.t0 * kBlock + (threadIdx.x & ~63)]);
     double re[SEG], im[SEG];
 #pragma unroll
     for (int t = 0; t < SEG; t++) { re[t] = 0.0; im[t] = 0.0; }
@@ -102,13 +103,18 @@ __global__ void __launch_bounds__(kBlock) k_rho_lattice(const SysDesc *__restric
         sincos_rev(u, sn, cs);
         const double2 st = lstep[j];
 #pragma unroll
-        for (int t = 0; t < SEG; t++) {
-            re[t] += cs;
-            im[t] += sn;
-            const double nc = __builtin_fma(cs, st.x, -sn * st.y);
-            const double ns = __builtin_fma(cs, st.y, sn * st.x);
-            cs = nc;
-            sn = ns;
+        for (int t0 = 0; t0 < SEG; t0 += 4) {
+            if (t0 < wlen) {
+#pragma unroll
+                for (int t = t0; t < t0 + 4; t++) {
+                    re[t] += cs;
+                    im[t] += sn;
+                    const double nc = __builtin_fma(cs, st.x, -sn * st.y);
+                    const double ns = __builtin_fma(cs, st.y, sn * st.x);
+                    cs = nc;
+                    sn = ns;
+                }
+            }
         }
     }
     double *out = S.rpart + ((int64_t)it.t1 * S.nslots + (int64_t)seg * SEG) * 2;
@@ -142,6 +148,10 @@ __global__ void __launch_bounds__(kBlock) k_rho_lattice_tab(const SysDesc *__res
     const int4 h = valid ? S.seg_head[seg] : make_int4(0, 0, 0, 0);
     const double h0 = (double)h.x, h1 = (double)h.y, h2 = (double)h.z, h3 = (double)h.w;
     const int dl = S.d - 1;
+    // segments are sorted by length, longest first: the wavefront's first segment bounds the slots any
+    // lane of this wavefront needs, so slot groups beyond it are skipped (uniform branch)
+    const int wseg = it.t0 * kBlock + (threadIdx.x & ~63);
+    const int wlen = __builtin_amdgcn_readfirstlane(wseg < S.nseg ? S.seg_len[wseg] : 0);
     double re[SEG], im[SEG];
 #pragma unroll
     for (int t = 0; t < SEG; t++) { re[t] = 0.0; im[t] = 0.0; }
@@ -169,12 +179,17 @@ __global__ void __launch_bounds__(kBlock) k_rho_lattice_tab(const SysDesc *__res
                 sincos_rev(u, bi, br);
                 const double2 *P = lpow + j * SEG;
 #pragma unroll
-                for (int t = 0; t < SEG; t++) {
-                    const double2 q = P[t];
-                    re[t] = __builtin_fma(br, q.x, re[t]);
-                    re[t] = __builtin_fma(-bi, q.y, re[t]);
-                    im[t] = __builtin_fma(br, q.y, im[t]);
-                    im[t] = __builtin_fma(bi, q.x, im[t]);
+                for (int t0 = 0; t0 < SEG; t0 += 4) {
+                    if (t0 < wlen) {
+#pragma unroll
+                        for (int t = t0; t < t0 + 4; t++) {
+                            const double2 q = P[t];
+                            re[t] = __builtin_fma(br, q.x, re[t]);
+                            re[t] = __builtin_fma(-bi, q.y, re[t]);
+                            im[t] = __builtin_fma(br, q.y, im[t]);
+                            im[t] = __builtin_fma(bi, q.x, im[t]);
+                        }
+                    }
                 }
             }
         }
@@ -302,6 +317,7 @@ __global__ void __launch_bounds__(kBlock) k_force_lattice(const SysDesc *__restr
     const double2 *coef = reinterpret_cast<const double2 *>(S.coef);
     for (int sg = s0; sg < s1; sg++) {
         const int4 h = S.seg_head[sg];
+        const int len = S.seg_len[sg];
         const double h0 = (double)h.x, h1 = (double)h.y, h2 = (double)h.z, h3 = (double)h.w;
         const double u = __builtin_fma(h0, p.x, __builtin_fma(h1, p.y, __builtin_fma(h2, p.z, h3 * p.w)));
         double sn, cs;
@@ -310,15 +326,20 @@ __global__ void __launch_bounds__(kBlock) k_force_lattice(const SysDesc *__restr
         const double hl = (d == 1) ? h0 : (d == 2) ? h1 : (d == 3) ? h2 : h3;
         const double2 *cf = coef + (int64_t)sg * SEG;
 #pragma unroll
-        for (int t = 0; t < SEG; t++) {
-            const double2 ab = cf[t];
-            const double c = __builtin_fma(ab.x, cs, -ab.y * sn);
-            acc += c;
-            accl = __builtin_fma(c, hl + (double)t, accl);
-            const double nc = __builtin_fma(cs, st.x, -sn * st.y);
-            const double ns = __builtin_fma(cs, st.y, sn * st.x);
-            cs = nc;
-            sn = ns;
+        for (int t0 = 0; t0 < SEG; t0 += 4) {
+            if (t0 < len) {  // segment data is uniform across the wavefront
+#pragma unroll
+                for (int t = t0; t < t0 + 4; t++) {
+                    const double2 ab = cf[t];
+                    const double c = __builtin_fma(ab.x, cs, -ab.y * sn);
+                    acc += c;
+                    accl = __builtin_fma(c, hl + (double)t, accl);
+                    const double nc = __builtin_fma(cs, st.x, -sn * st.y);
+                    const double ns = __builtin_fma(cs, st.y, sn * st.x);
+                    cs = nc;
+                    sn = ns;
+                }
+            }
         }
         // components before the last axis are constant along the segment
         if (d == 1) {

@@ -39,6 +39,7 @@ struct RelaxDesc {
     double *s;       // search direction (n)
     double *xo, *go; // history pairs (mf x n each)
     double *uv;      // uo, vo (mf each)
+    double *gram;    // s_a.y_b and y_a.y_b over physical history slots (2 x mf x mf)
     int n, mf;
     int sys;         // index of the job's SysDesc in the batch
     int pad;
@@ -67,7 +68,7 @@ struct Job {
     std::string error;
     // device side (resident only)
     std::unique_ptr<System> sys;
-    DevArray<double> sdir, xo, go, uv;
+    DevArray<double> sdir, xo, go, uv, gram;
     DevArray<RelaxState> st;
     RelaxState host_state{};
 };
@@ -88,12 +89,19 @@ public:
     int Queued() const { return (int)queue_.size(); }
     int64_t BytesInUse() const { return bytes_in_use_; }
     void SetCheckInterval(int k) { check_every_ = k > 0 ? k : 1; }
+    void SetGraph(bool on)
+    {
+        use_graph_ = on;
+        DropGraph();
+    }
 
     Batch batch;
 
 private:
     void Admit();
     void Rebuild();
+    void LaunchBurst(int burst);
+    void DropGraph();
     void Harvest();
     static int64_t EstimateBytes(const Job &j);
 
@@ -102,6 +110,9 @@ private:
     int64_t next_id_ = 1;
     int check_every_ = 8;
     bool dirty_ = true;
+    bool use_graph_ = false;
+    hipGraphExec_t gexec_ = nullptr;
+    int graph_burst_ = 0;
     std::deque<Job *> queue_;
     std::vector<Job *> resident_;
     std::map<int64_t, std::unique_ptr<Job>> jobs_;

@@ -17,6 +17,12 @@ struct BlockPlisOps {
     double *x, *g, *s, *xo, *go;
     double *lds_red;  // kRelaxBlock / 64 doubles
     double *lds_u, *lds_v;
+    double *gsy, *gyy;  // Gram matrices over physical history slots: gsy[a*mf+b] = s_a.y_b, gyy[a*mf+b] = y_a.y_b
+    double *lds_part;   // (kRelaxBlock/64) x kPlisMaxHistory x 5 wavefront partials
+    double *lds_dots;   // kPlisMaxHistory x 5 combined dots
+    double *lds_ab;     // alpha[kPlisMaxHistory], beta[kPlisMaxHistory]
+    bool gs_valid;      // g.s of the last two_loop is cached until g or s changes
+    double gs_val;
 
     __device__ double *vec(int id) const
     {
@@ -47,54 +53,140 @@ struct BlockPlisOps {
         __syncthreads();
         return r;
     }
-    __device__ double dot(int a, int b) const
+    __device__ void touch(int id)
     {
+        if (id == kVecS || id == kVecG) gs_valid = false;
+    }
+    // Element loops visit this thread's elements four at a time, loads first, so that each thread keeps
+    // several independent HBM loads in flight (one workgroup per job leaves few waves per CU).
+    static constexpr int kU = 4;
+    __device__ double dot(int a, int b)
+    {
+        if (gs_valid && ((a == kVecG && b == kVecS) || (a == kVecS && b == kVecG))) return gs_val;
         const double *pa = vec(a), *pb = vec(b);
         double t = 0.0;
-        for (int i = threadIdx.x; i < n; i += kRelaxBlock) t = __builtin_fma(pa[i], pb[i], t);
+        int e = threadIdx.x;
+        for (; e + (kU - 1) * kRelaxBlock < n; e += kU * kRelaxBlock) {
+            double va[kU], vb[kU];
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                va[u] = pa[e + u * kRelaxBlock];
+                vb[u] = pb[e + u * kRelaxBlock];
+            }
+#pragma unroll
+            for (int u = 0; u < kU; u++) t = __builtin_fma(va[u], vb[u], t);
+        }
+        for (; e < n; e += kRelaxBlock) t = __builtin_fma(pa[e], pb[e], t);
         return block_reduce(t, false);
     }
-    __device__ void axpy(int z, double a, int xv, int yv) const
+    __device__ void axpy(int z, double a, int xv, int yv)
     {
+        touch(z);
         double *pz = vec(z);
         const double *px = vec(xv), *py = vec(yv);
-        for (int i = threadIdx.x; i < n; i += kRelaxBlock) pz[i] = __builtin_fma(a, px[i], py[i]);
+        int e = threadIdx.x;
+        for (; e + (kU - 1) * kRelaxBlock < n; e += kU * kRelaxBlock) {
+            double vx[kU], vy[kU];
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                vx[u] = px[e + u * kRelaxBlock];
+                vy[u] = py[e + u * kRelaxBlock];
+            }
+#pragma unroll
+            for (int u = 0; u < kU; u++) pz[e + u * kRelaxBlock] = __builtin_fma(a, vx[u], vy[u]);
+        }
+        for (; e < n; e += kRelaxBlock) pz[e] = __builtin_fma(a, px[e], py[e]);
     }
-    __device__ void neg(int src, int dst) const
+    __device__ void neg(int src, int dst)
     {
+        touch(dst);
         const double *ps = vec(src);
         double *pd = vec(dst);
-        for (int i = threadIdx.x; i < n; i += kRelaxBlock) pd[i] = -ps[i];
+        int e = threadIdx.x;
+        for (; e + (kU - 1) * kRelaxBlock < n; e += kU * kRelaxBlock) {
+            double v[kU];
+#pragma unroll
+            for (int u = 0; u < kU; u++) v[u] = ps[e + u * kRelaxBlock];
+#pragma unroll
+            for (int u = 0; u < kU; u++) pd[e + u * kRelaxBlock] = -v[u];
+        }
+        for (; e < n; e += kRelaxBlock) pd[e] = -ps[e];
     }
-    __device__ void copy(int src, int dst) const
+    __device__ void copy(int src, int dst)
     {
+        touch(dst);
         const double *ps = vec(src);
         double *pd = vec(dst);
-        for (int i = threadIdx.x; i < n; i += kRelaxBlock) pd[i] = ps[i];
+        int e = threadIdx.x;
+        for (; e + (kU - 1) * kRelaxBlock < n; e += kU * kRelaxBlock) {
+            double v[kU];
+#pragma unroll
+            for (int u = 0; u < kU; u++) v[u] = ps[e + u * kRelaxBlock];
+#pragma unroll
+            for (int u = 0; u < kU; u++) pd[e + u * kRelaxBlock] = v[u];
+        }
+        for (; e < n; e += kRelaxBlock) pd[e] = ps[e];
     }
-    __device__ void scal(double a, int xv) const
+    __device__ void scal(double a, int xv)
     {
+        touch(xv);
         double *p = vec(xv);
-        for (int i = threadIdx.x; i < n; i += kRelaxBlock) p[i] = a * p[i];
+        int e = threadIdx.x;
+        for (; e + (kU - 1) * kRelaxBlock < n; e += kU * kRelaxBlock) {
+            double v[kU];
+#pragma unroll
+            for (int u = 0; u < kU; u++) v[u] = p[e + u * kRelaxBlock];
+#pragma unroll
+            for (int u = 0; u < kU; u++) p[e + u * kRelaxBlock] = a * v[u];
+        }
+        for (; e < n; e += kRelaxBlock) p[e] = a * p[e];
     }
-    __device__ void diff(int xv, int yv, int z) const
+    __device__ void diff(int xv, int yv, int z)
     {
+        touch(z);
         const double *px = vec(xv), *py = vec(yv);
         double *pz = vec(z);
-        for (int i = threadIdx.x; i < n; i += kRelaxBlock) pz[i] = px[i] - py[i];
+        int e = threadIdx.x;
+        for (; e + (kU - 1) * kRelaxBlock < n; e += kU * kRelaxBlock) {
+            double vx[kU], vy[kU];
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                vx[u] = px[e + u * kRelaxBlock];
+                vy[u] = py[e + u * kRelaxBlock];
+            }
+#pragma unroll
+            for (int u = 0; u < kU; u++) pz[e + u * kRelaxBlock] = vx[u] - vy[u];
+        }
+        for (; e < n; e += kRelaxBlock) pz[e] = px[e] - py[e];
     }
     __device__ double maxabs(int xv) const
     {
         const double *p = vec(xv);
         double m = 0.0;
-        for (int i = threadIdx.x; i < n; i += kRelaxBlock) m = fmax(m, fabs(p[i]));
+        int e = threadIdx.x;
+        for (; e + (kU - 1) * kRelaxBlock < n; e += kU * kRelaxBlock) {
+            double v[kU];
+#pragma unroll
+            for (int u = 0; u < kU; u++) v[u] = p[e + u * kRelaxBlock];
+#pragma unroll
+            for (int u = 0; u < kU; u++) m = fmax(m, fabs(v[u]));
+        }
+        for (; e < n; e += kRelaxBlock) m = fmax(m, fabs(p[e]));
         return block_reduce(m, true);
     }
     __device__ double l1(int xv) const
     {
         const double *p = vec(xv);
         double t = 0.0;
-        for (int i = threadIdx.x; i < n; i += kRelaxBlock) t += fabs(p[i]);
+        int e = threadIdx.x;
+        for (; e + (kU - 1) * kRelaxBlock < n; e += kU * kRelaxBlock) {
+            double v[kU];
+#pragma unroll
+            for (int u = 0; u < kU; u++) v[u] = p[e + u * kRelaxBlock];
+#pragma unroll
+            for (int u = 0; u < kU; u++) t += fabs(v[u]);
+        }
+        for (; e < n; e += kRelaxBlock) t += fabs(p[e]);
         return block_reduce(t, false);
     }
     __device__ double u(int i) const { return lds_u[i]; }
@@ -118,6 +210,186 @@ struct BlockPlisOps {
             for (int i = k - 1; i >= 1; --i) lds_u[i] = lds_u[i - 1];
         __syncthreads();
     }
+
+    // S = -H g over the k newest pairs, the same L-BFGS direction as the two-loop recursion written through
+    // Gram matrices so that every dot product of one direction is formed in two passes instead of 2k+1
+    // dependent reductions:
+    //   alpha_i = u_i (-s_i.g - sum_{j<i} alpha_j s_i.y_j)                       (backward loop)
+    //   beta_i  = u_i (gamma (-y_i.g - sum_j alpha_j y_i.y_j) + sum_{j>i} (alpha_j - beta_j) y_i.s_j)
+    //   S       = gamma (-g - sum_j alpha_j y_j) + sum_j (alpha_j - beta_j) s_j     (forward loop)
+    // with gamma = b / (y_1.y_1).  Pass 1 forms s_j.g, y_j.g and the Gram row/column of the newest pair
+    // (the only pair added since the last direction); pass 2 forms S, |S|^2 and g.S.
+    __device__ double two_loop(int k, int head, double b)
+    {
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        constexpr int nw = kRelaxBlock / 64;
+        constexpr int kJ = 4;  // history pairs per pass-1 group (20 partial sums per thread)
+        const int p1 = head;
+        const double *s1 = xo + (size_t)p1 * n, *y1 = go + (size_t)p1 * n;
+        for (int j0 = 0; j0 < k; j0 += kJ) {
+            const int nj = min(kJ, k - j0);
+            const double *sj[kJ], *yj[kJ];
+#pragma unroll
+            for (int q = 0; q < kJ; q++) {
+                const int pj = (head + j0 + (q < nj ? q : 0)) % mf;
+                sj[q] = xo + (size_t)pj * n;
+                yj[q] = go + (size_t)pj * n;
+            }
+            double a[kJ][5];
+#pragma unroll
+            for (int q = 0; q < kJ; q++)
+#pragma unroll
+                for (int c = 0; c < 5; c++) a[q][c] = 0.0;
+            auto accumulate = [&](int e, const double *sv, const double *yv, double gv, double s1v, double y1v) {
+#pragma unroll
+                for (int q = 0; q < kJ; q++) {
+                    a[q][0] = __builtin_fma(sv[q], gv, a[q][0]);
+                    a[q][1] = __builtin_fma(yv[q], gv, a[q][1]);
+                    a[q][2] = __builtin_fma(s1v, yv[q], a[q][2]);
+                    a[q][3] = __builtin_fma(sv[q], y1v, a[q][3]);
+                    a[q][4] = __builtin_fma(y1v, yv[q], a[q][4]);
+                }
+            };
+            int e = threadIdx.x;
+            for (; e + kRelaxBlock < n; e += 2 * kRelaxBlock) {  // two elements' loads in flight
+                const int e2 = e + kRelaxBlock;
+                double sv[kJ], yv[kJ], sv2[kJ], yv2[kJ];
+                const double gv = g[e], s1v = s1[e], y1v = y1[e];
+                const double gv2 = g[e2], s1v2 = s1[e2], y1v2 = y1[e2];
+#pragma unroll
+                for (int q = 0; q < kJ; q++) {
+                    sv[q] = sj[q][e];
+                    yv[q] = yj[q][e];
+                    sv2[q] = sj[q][e2];
+                    yv2[q] = yj[q][e2];
+                }
+                accumulate(e, sv, yv, gv, s1v, y1v);
+                accumulate(e2, sv2, yv2, gv2, s1v2, y1v2);
+            }
+            for (; e < n; e += kRelaxBlock) {
+                double sv[kJ], yv[kJ];
+#pragma unroll
+                for (int q = 0; q < kJ; q++) {
+                    sv[q] = sj[q][e];
+                    yv[q] = yj[q][e];
+                }
+                accumulate(e, sv, yv, g[e], s1[e], y1[e]);
+            }
+#pragma unroll
+            for (int q = 0; q < kJ; q++) {
+#pragma unroll
+                for (int c = 0; c < 5; c++) a[q][c] = wave_sum(a[q][c]);
+                if (lane == 0 && q < nj) {
+                    double *w = lds_part + ((size_t)wave * kPlisMaxHistory + j0 + q) * 5;
+#pragma unroll
+                    for (int c = 0; c < 5; c++) w[c] = a[q][c];
+                }
+            }
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < 5 * k; q += kRelaxBlock) {
+            const int j = q / 5, c = q - 5 * j;
+            double t = lds_part[(size_t)j * 5 + c];
+            for (int w = 1; w < nw; w++) t += lds_part[((size_t)w * kPlisMaxHistory + j) * 5 + c];
+            lds_dots[q] = t;
+        }
+        __syncthreads();
+        // Gram row and column of the newest pair, then the k x k window of both Gram matrices in logical
+        // order staged in LDS (lds_part is free again): G_sy[i][j] = s_i.y_j, G_yy[i][j] = y_i.y_j
+        for (int j = threadIdx.x; j < k; j += kRelaxBlock) {
+            const int pj = (head + j) % mf;
+            gsy[p1 * mf + pj] = lds_dots[5 * j + 2];
+            gsy[pj * mf + p1] = lds_dots[5 * j + 3];
+            gyy[p1 * mf + pj] = lds_dots[5 * j + 4];
+            gyy[pj * mf + p1] = lds_dots[5 * j + 4];
+        }
+        __syncthreads();
+        // s_i.y_j of the window for i older than j (the only Gram entries the recursions read), in LDS
+        double *tri = lds_part;  // tri[i(i-1)/2 + j], j < i: at most 64*63/2 entries
+        for (int q = threadIdx.x; q < k * k; q += kRelaxBlock) {
+            const int i = q / k, j = q - i * k;
+            if (j < i) tri[i * (i - 1) / 2 + j] = gsy[((head + i) % mf) * mf + (head + j) % mf];
+        }
+        __syncthreads();
+        double *alpha = lds_ab, *beta = lds_ab + kPlisMaxHistory, *gya = lds_ab + 2 * kPlisMaxHistory;
+        if (threadIdx.x == 0) {
+            for (int i = 0; i < k; i++) {  // backward loop (newest to oldest)
+                double t = -lds_dots[5 * i];
+                for (int j = 0; j < i; j++) t -= alpha[j] * tri[i * (i - 1) / 2 + j];
+                alpha[i] = lds_u[i] * t;
+                lds_v[i] = alpha[i];
+            }
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < k; i += kRelaxBlock) {  // (G_yy alpha)_i
+            const int pi = (head + i) % mf;
+            double t = 0.0;
+            for (int j = 0; j < k; j++) t += alpha[j] * gyy[pi * mf + (head + j) % mf];
+            gya[i] = t;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const double a = gyy[p1 * mf + p1];
+            const double gamma = a > 0. ? b / a : 1.0;
+            for (int i = k - 1; i >= 0; i--) {  // forward loop (oldest to newest)
+                double t = (-lds_dots[5 * i + 1] - gya[i]) * gamma;
+                for (int j = i + 1; j < k; j++) t += (alpha[j] - beta[j]) * tri[j * (j - 1) / 2 + i];
+                beta[i] = lds_u[i] * t;
+            }
+            for (int i = 0; i < k; i++) beta[i] = alpha[i] - beta[i];  // coefficient of s_i
+            lds_ab[3 * kPlisMaxHistory] = gamma;
+        }
+        __syncthreads();
+        const double gamma = lds_ab[3 * kPlisMaxHistory];
+        const double *coef_s = lds_ab + kPlisMaxHistory;
+        double ss = 0.0, gs = 0.0;
+        constexpr int kH = 8;  // history pairs whose elements are loaded together
+        for (int e = threadIdx.x; e < n; e += kRelaxBlock) {
+            const double gv = g[e];
+            double q = -gv, r = 0.0;
+            int j = 0;
+            for (; j + kH <= k; j += kH) {
+                double vy[kH], vs[kH];
+#pragma unroll
+                for (int u = 0; u < kH; u++) {
+                    const size_t off = (size_t)((head + j + u) % mf) * n + e;
+                    vy[u] = go[off];
+                    vs[u] = xo[off];
+                }
+#pragma unroll
+                for (int u = 0; u < kH; u++) {
+                    q = __builtin_fma(-alpha[j + u], vy[u], q);
+                    r = __builtin_fma(coef_s[j + u], vs[u], r);
+                }
+            }
+            for (; j < k; j++) {
+                const int pj = (head + j) % mf;
+                q = __builtin_fma(-alpha[j], go[(size_t)pj * n + e], q);
+                r = __builtin_fma(coef_s[j], xo[(size_t)pj * n + e], r);
+            }
+            const double out = __builtin_fma(gamma, q, r);
+            s[e] = out;
+            ss = __builtin_fma(out, out, ss);
+            gs = __builtin_fma(gv, out, gs);
+        }
+        ss = wave_sum(ss);
+        gs = wave_sum(gs);
+        __syncthreads();
+        if (lane == 0) {
+            lds_red[wave] = ss;
+            lds_dots[wave] = gs;
+        }
+        __syncthreads();
+        double tss = lds_red[0], tgs = lds_dots[0];
+        for (int w = 1; w < nw; w++) {
+            tss += lds_red[w];
+            tgs += lds_dots[w];
+        }
+        __syncthreads();
+        gs_valid = true;
+        gs_val = tgs;
+        return sqrt(tss);
+    }
 };
 
 __device__ __forceinline__ double relax_wrap(double x)
@@ -134,6 +406,9 @@ __global__ void __launch_bounds__(kRelaxBlock) k_relax_step(SysDesc *__restrict_
 {
     __shared__ double lds_red[kRelaxBlock / 64];
     __shared__ double lds_u[kPlisMaxHistory], lds_v[kPlisMaxHistory];
+    __shared__ double lds_part[(kRelaxBlock / 64) * kPlisMaxHistory * 5];
+    __shared__ double lds_dots[kPlisMaxHistory * 5];
+    __shared__ double lds_ab[3 * kPlisMaxHistory + 1];
     const RelaxDesc R = rdv[blockIdx.x];
     SysDesc *D = sysv + R.sys;
     if (!(D->flags & kSysActive)) return;
@@ -143,7 +418,9 @@ __global__ void __launch_bounds__(kRelaxBlock) k_relax_step(SysDesc *__restrict_
         lds_v[i] = R.uv[R.mf + i];
     }
     __syncthreads();
-    BlockPlisOps ops{R.n, R.mf, const_cast<double *>(D->x), D->grad, R.s, R.xo, R.go, lds_red, lds_u, lds_v};
+    BlockPlisOps ops{R.n,     R.mf,      const_cast<double *>(D->x), D->grad,  R.s,   R.xo, R.go, lds_red,
+                     lds_u,   lds_v,     R.gram,                     R.gram + R.mf * R.mf,   lds_part, lds_dots,
+                     lds_ab,  false,     0.0};
     const double E = D->energy[0];
     st.E = E;
     st.device_evals++;
@@ -214,6 +491,7 @@ Solver::Solver(int max_slots, int64_t hbm_budget) : max_slots_(max_slots), budge
 Solver::~Solver()
 {
     (void)hipStreamSynchronize(batch.stream);
+    DropGraph();
     batch.systems.clear();
 }
 
@@ -282,6 +560,7 @@ void Solver::Admit()
             j->xo.Reserve(std::max<int64_t>(n * j->mf, 1));
             j->go.Reserve(std::max<int64_t>(n * j->mf, 1));
             j->uv.Reserve(2 * j->mf);
+            j->gram.Reserve(2 * j->mf * j->mf);
             j->st.Reserve(1);
             std::vector<double> x0(n);
             for (int64_t i = 0; i < n; i++) x0[i] = j->s0[i] * kRelaxRescale[0];  // GetParam (:758-771)
@@ -313,6 +592,7 @@ void Solver::Admit()
 
 void Solver::Rebuild()
 {
+    DropGraph();
     batch.systems.clear();
     std::vector<RelaxDesc> rd;
     for (size_t i = 0; i < resident_.size(); i++) {
@@ -325,6 +605,7 @@ void Solver::Rebuild()
         r.xo = j->xo.get();
         r.go = j->go.get();
         r.uv = j->uv.get();
+        r.gram = j->gram.get();
         r.n = (int)(j->N * j->d);
         r.mf = j->mf;
         r.sys = (int)i;
@@ -381,11 +662,47 @@ void Solver::Harvest()
         j->xo.Release();
         j->go.Release();
         j->uv.Release();
+        j->gram.Release();
         j->st.Release();
         bytes_in_use_ -= j->bytes;
         dirty_ = true;
     }
     resident_.swap(keep);
+}
+
+void Solver::DropGraph()
+{
+    if (gexec_) (void)hipGraphExecDestroy(gexec_);
+    gexec_ = nullptr;
+    graph_burst_ = 0;
+}
+
+void Solver::LaunchBurst(int burst)
+{
+    auto enqueue = [&] {
+        for (int it = 0; it < burst; it++) {
+            batch.Enqueue(true, true);
+            launch_relax_step(batch.stream, batch.d_desc.get(), d_rdesc_.get(), (int)resident_.size());
+        }
+    };
+    // hipGraph replay of `burst` optimizer steps (no memset nodes: every kernel writes what it reads next);
+    // timing mode records events per launch and stays on plain stream launches
+    if (!use_graph_ || batch.timing) {
+        enqueue();
+        return;
+    }
+    if (!gexec_ || graph_burst_ != burst) {
+        DropGraph();
+        hipGraph_t g = nullptr;
+        CC_HIP(hipStreamBeginCapture(batch.stream, hipStreamCaptureModeThreadLocal));
+        enqueue();
+        CC_HIP(hipStreamEndCapture(batch.stream, &g));
+        hipError_t e = hipGraphInstantiate(&gexec_, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        CC_HIP(e);
+        graph_burst_ = burst;
+    }
+    CC_HIP(hipGraphLaunch(gexec_, batch.stream));
 }
 
 int Solver::Step(int evaluations)
@@ -395,10 +712,7 @@ int Solver::Step(int evaluations)
     int done = 0;
     while (done < evaluations && !resident_.empty()) {
         const int burst = std::min(check_every_, evaluations - done);
-        for (int it = 0; it < burst; it++) {
-            batch.Enqueue(true, true);
-            launch_relax_step(batch.stream, batch.d_desc.get(), d_rdesc_.get(), (int)resident_.size());
-        }
+        LaunchBurst(burst);
         CC_HIP(hipGetLastError());
         done += burst;
         Harvest();

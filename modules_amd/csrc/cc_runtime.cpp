@@ -4,6 +4,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <tuple>
 
 namespace ccgpu {
 
@@ -16,10 +20,42 @@ void CheckHip(hipError_t e, const char *what)
     }
 }
 
+namespace {
+std::mutex g_tiles_mu;
+std::map<std::tuple<int, int, int>, Tiles> g_tiles;
+
+std::tuple<int, int, int> TileKey(int d, int64_t N, int64_t M)
+{
+    auto bucket = [](int64_t x) { return x <= 0 ? -1 : (int)std::floor(2.0 * std::log2((double)x)); };
+    return std::make_tuple(d, bucket(N), bucket(M));
+}
+}  // namespace
+
+void RecordTiles(int d, int64_t N, int64_t M, const Tiles &t)
+{
+    std::lock_guard<std::mutex> lk(g_tiles_mu);
+    g_tiles[TileKey(d, N, M)] = t;
+}
+
+bool LookupTiles(int d, int64_t N, int64_t M, Tiles &t)
+{
+    std::lock_guard<std::mutex> lk(g_tiles_mu);
+    auto it = g_tiles.find(TileKey(d, N, M));
+    if (it == g_tiles.end()) return false;
+    t = it->second;
+    return true;
+}
+
+void ClearTiles()
+{
+    std::lock_guard<std::mutex> lk(g_tiles_mu);
+    g_tiles.clear();
+}
+
 Tiles DefaultTiles(int d, int64_t N, int64_t M)
 {
-    (void)d;
-    (void)M;
+    Tiles tuned;
+    if (LookupTiles(d, N, M, tuned)) return tuned;
     Tiles t;
     t.seg = 16;
     // rho: about 16 particle chunks per system, 64..512 particles each
@@ -69,7 +105,7 @@ void System::AddConstraint(const double *kk, double V)
 
 size_t System::DeviceBytes() const
 {
-    return x.bytes() + pos.bytes() + cart.bytes() + step.bytes() + seg_head.bytes() + slot_k.bytes() +
+    return x.bytes() + pos.bytes() + cart.bytes() + step.bytes() + seg_head.bytes() + seg_len.bytes() + slot_k.bytes() +
            slot_v.bytes() + kvec.bytes() + rpart.bytes() + rho.bytes() + coef.bytes() + epart.bytes() +
            energy.bytes() + fpart.bytes() + F.bytes() + grad.bytes() + cell_of.bytes() + cell_start.bytes() +
            cell_items.bytes() + sc_epart.bytes() + sc_F.bytes();
@@ -92,12 +128,14 @@ void System::Prepare(hipStream_t stream)
             plan = BuildSegments(d, M, n.data(), used.seg);
             nslots = plan.nseg * used.seg;
             seg_head.Reserve(plan.nseg);
+            seg_len.Reserve(plan.nseg);
             slot_k.Reserve(nslots);
             slot_v.Reserve(nslots);
             std::vector<double> sv(nslots, 0.0);
             for (int64_t q = 0; q < nslots; q++)
                 if (plan.slot_k[q] >= 0) sv[q] = v[plan.slot_k[q]];
             CC_HIP(hipMemcpyAsync(seg_head.get(), plan.head.data(), plan.nseg * sizeof(int4), hipMemcpyHostToDevice, stream));
+            CC_HIP(hipMemcpyAsync(seg_len.get(), plan.len.data(), plan.nseg * sizeof(int32_t), hipMemcpyHostToDevice, stream));
             CC_HIP(hipMemcpyAsync(slot_k.get(), plan.slot_k.data(), nslots * sizeof(int32_t), hipMemcpyHostToDevice, stream));
             CC_HIP(hipMemcpyAsync(slot_v.get(), sv.data(), nslots * sizeof(double), hipMemcpyHostToDevice, stream));
             CC_HIP(hipStreamSynchronize(stream));
@@ -127,8 +165,9 @@ void System::Prepare(hipStream_t stream)
     // chunking of the two contractions
     pchunks = (int)std::max<int64_t>(1, (N + used.rho_chunk - 1) / used.rho_chunk);
     const int64_t pblocks = std::max<int64_t>(1, (N + kBlock - 1) / kBlock);
-    if (tiles.force_chunk > 0) {
-        used.force_chunk = tiles.force_chunk;
+    const int fc = tiles.force_chunk > 0 ? tiles.force_chunk : def.force_chunk;
+    if (fc > 0) {
+        used.force_chunk = fc;
     } else {
         // about 16 blocks per system in the force pass, at least 4 units per chunk
         int64_t want = std::max<int64_t>(1, (16 + pblocks - 1) / pblocks);
@@ -191,6 +230,7 @@ void System::Prepare(hipStream_t stream)
     D.cart = cart.get();
     D.step = step.get();
     D.seg_head = seg_head.get();
+    D.seg_len = seg_len.get();
     D.slot_k = slot_k.get();
     D.slot_v = slot_v.get();
     D.kvec = kvec.get();
@@ -394,5 +434,72 @@ void Batch::Enqueue(bool rho, bool forces)
 }
 
 void Batch::Sync() { CC_HIP(hipStreamSynchronize(stream)); }
+
+double Autotune(int d, int64_t N, const double *basis, int64_t M, const double *k, const double *v, int batch,
+                int repeats, Tiles &best)
+{
+    if (batch < 1 || repeats < 1 || N < 1) throw Error(kErrArg, "autotune: bad batch, repeats or N");
+    std::vector<std::unique_ptr<System>> sys;
+    Batch b;
+    for (int i = 0; i < batch; i++) {
+        sys.emplace_back(new System(d));
+        System &S = *sys.back();
+        S.SetCell(basis);
+        S.SetConstraints(M, k, v);
+        S.N = N;
+        b.systems.push_back(&S);
+    }
+    // candidate shapes: segment length x rho kernel x rho chunk x force chunk (0 = automatic)
+    std::vector<Tiles> cands;
+    for (int seg : {8, 16, 24})
+        for (int rk : {0, 1})
+            for (int rc : {128, 256, 512})
+                for (int fc : {0, 32}) {
+                    if (rc > 2 * N && rc > 64) continue;
+                    Tiles t;
+                    t.seg = seg;
+                    t.rho_kernel = rk;
+                    t.rho_chunk = rc;
+                    t.force_chunk = fc;
+                    cands.push_back(t);
+                }
+    hipEvent_t e0, e1;
+    CC_HIP(hipEventCreate(&e0));
+    CC_HIP(hipEventCreate(&e1));
+    double best_ms = 1e300;
+    for (const Tiles &t : cands) {
+        for (auto &S : sys) {
+            S->tiles = t;
+            S->layout_dirty = true;
+        }
+        b.dirty = true;
+        b.Build();
+        bool first = true;
+        for (auto &S : sys) {
+            const int64_t n = N * d;
+            std::vector<double> x(n);
+            RandomRelativeConfiguration((uint32_t)(7919 + (&S - &sys[0])), N, d, x.data());
+            if (first) first = false;
+            CC_HIP(hipMemcpyAsync(S->x.get(), x.data(), n * sizeof(double), hipMemcpyHostToDevice, b.stream));
+            CC_HIP(hipStreamSynchronize(b.stream));
+        }
+        b.Enqueue(true, true);  // warm-up
+        CC_HIP(hipEventRecord(e0, b.stream));
+        for (int r = 0; r < repeats; r++) b.Enqueue(true, true);
+        CC_HIP(hipEventRecord(e1, b.stream));
+        CC_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        CC_HIP(hipEventElapsedTime(&ms, e0, e1));
+        const double per = ms / repeats;
+        if (per < best_ms) {
+            best_ms = per;
+            best = t;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    RecordTiles(d, N, M, best);
+    return best_ms;
+}
 
 }  // namespace ccgpu

@@ -74,6 +74,17 @@ struct Tiles {
 };
 Tiles DefaultTiles(int d, int64_t N, int64_t M);
 
+// Process-wide table of tuned tiles keyed by (d, N bucket, M bucket) (half-octave buckets); DefaultTiles
+// consults it.  Filled by Autotune; a system's tiles depend only on its own shape and the table.
+void RecordTiles(int d, int64_t N, int64_t M, const Tiles &t);
+bool LookupTiles(int d, int64_t N, int64_t M, Tiles &t);
+void ClearTiles();
+
+// Time candidate tiles on `batch` copies of a random configuration of this shape (E+F per evaluation,
+// hipEvents on the batch stream) and record the fastest.  Returns its milliseconds per batched E+F.
+double Autotune(int d, int64_t N, const double *basis, int64_t M, const double *k, const double *v, int batch,
+                int repeats, Tiles &best);
+
 // One periodic configuration plus its constraint set, resident in HBM.
 class System {
 public:
@@ -85,7 +96,7 @@ public:
     double a[16] = {}, b[16] = {}, vol = 0;
     std::vector<double> k, v;  // constraints (host copy): k M x d, V M
     double shift = 0, e0 = 0, sigma = 0, rescale = 1;
-    Tiles tiles;                // requested (0 = automatic)
+    Tiles tiles{0, 0, 0, -1};   // requested (0 / -1 = automatic)
     Tiles used;                 // resolved
 
     // constraint layout
@@ -99,6 +110,7 @@ public:
     DevArray<double4> pos, cart;
     DevArray<double2> step;
     DevArray<int4> seg_head;
+    DevArray<int32_t> seg_len;
     DevArray<int32_t> slot_k;
     DevArray<double> slot_v;
     DevArray<double4> kvec;

@@ -264,6 +264,29 @@ L4:
     S.isys = 0;
 }
 
+// The Strang two-loop recursion in NLopt's operation order (luksan_mxdrcb__ / luksan_mxdrcf__ with the
+// gamma = b/a scaling, LuksanPLIS.cpp:1505-1519): S = -H g over logical columns 1..k (1 = newest, physical
+// slot (head + i - 1) mod mf).  Returns |S|.  Policies with sequential arithmetic call this directly.
+template <class Ops>
+PLIS_HD double PlisTwoLoopSequential(Ops &ops, int k, int head, int mf, double b)
+{
+    auto XO = [&](int i) { return kVecXO + (head + i - 1) % mf; };
+    auto GO = [&](int i) { return kVecXO + mf + (head + i - 1) % mf; };
+    ops.neg(kVecG, kVecS);
+    for (int i = 1; i <= k; ++i) {  // backward recursion, newest to oldest
+        const double vi = ops.u(i - 1) * ops.dot(kVecS, XO(i));
+        ops.set_v(i - 1, vi);
+        ops.axpy(kVecS, -vi, GO(i), kVecS);
+    }
+    const double a = ops.dot(GO(1), GO(1));
+    if (a > 0.) ops.scal(b / a, kVecS);
+    for (int i = k; i >= 1; --i) {  // forward recursion, oldest to newest
+        const double t = ops.u(i - 1) * ops.dot(kVecS, GO(i));
+        ops.axpy(kVecS, ops.v(i - 1) - t, XO(i), kVecS);
+    }
+    return sqrt(ops.dot(kVecS, kVecS));
+}
+
 // Reset a state for a new minimisation of n variables with history mf (LuksanPLIS, LuksanPLIS.cpp:1668-1736).
 // The caller must also zero the history vectors (memset of xo).
 PLIS_HD inline void PlisInit(PlisState &S, int n, int mf, int maxeval, double stopval)
@@ -394,19 +417,8 @@ L11130 : {
                 S.irest = S.irest > 1 ? S.irest : 1;
             } else {
                 ops.set_u(0, 1. / b);
-                ops.neg(kVecG, kVecS);
-                for (int i = 1; i <= k; ++i) {  // backward recursion, newest to oldest
-                    const double vi = ops.u(i - 1) * ops.dot(kVecS, XO(i));
-                    ops.set_v(i - 1, vi);
-                    ops.axpy(kVecS, -vi, GO(i), kVecS);
-                }
-                const double a = ops.dot(GO(1), GO(1));
-                if (a > 0.) ops.scal(b / a, kVecS);
-                for (int i = k; i >= 1; --i) {  // forward recursion, oldest to newest
-                    const double t = ops.u(i - 1) * ops.dot(kVecS, GO(i));
-                    ops.axpy(kVecS, ops.v(i - 1) - t, XO(i), kVecS);
-                }
-                S.snorm = sqrt(ops.dot(kVecS, kVecS));
+                // S = -H g by the two-loop recursion over the k newest pairs (the policy decides how)
+                S.snorm = ops.two_loop(k, S.head, b);
                 k = (k + 1 < mf) ? k + 1 : mf;
                 ops.shift_u(k);  // u[i] = u[i-1] for i = k-1 .. 1
                 S.head = (S.head + mf - 1) % mf;

@@ -22,6 +22,10 @@ struct HostPlisOps {
     {
         for (int i = k - 1; i >= 1; --i) uo[i] = uo[i - 1];
     }
+    double two_loop(int k, int head, double b)
+    {
+        return PlisTwoLoopSequential(*this, k, head, mf, b);
+    }
     double *vec(int id) const
     {
         if (id == kVecX) return x;

@@ -77,10 +77,11 @@ class BatchSolver:
 
     QUEUED, RESIDENT, DONE, FAILED = 0, 1, 2, 3
 
-    def __init__(self, max_slots=256, hbm_budget=200 << 30, check_interval=8):
+    def __init__(self, max_slots=256, hbm_budget=200 << 30, check_interval=8, graph=False):
         self._h = C.c_void_p()
         check(lib().ccgpu_solver_create(int(max_slots), int(hbm_budget), C.byref(self._h)))
         check(lib().ccgpu_solver_set_check_interval(self._h, int(check_interval)))
+        check(lib().ccgpu_solver_set_graph(self._h, 1 if graph else 0))
         self._shapes = {}
 
     def __del__(self):

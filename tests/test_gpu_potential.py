@@ -211,3 +211,14 @@ def test_errors_are_raised():
     bad.relative[0, 0] = np.nan
     with pytest.raises(CCGPUError):
         pot.SetConfiguration(bad)
+
+
+def test_autotuned_tiles_are_used_and_keep_parity():
+    from modules_amd import autotune
+    autotune.clear()
+    best = autotune.autotune(600, 3, chi=0.3, batch=4, repeats=2)
+    assert best["segment_len"] in (8, 16, 24, 32) and best["ms_per_batched_ef"] > 0
+    cfg = RandomConfiguration(17, 600, 3)
+    k = GetKs(cfg, getK(0.3, 1.0, 3))
+    compare(cfg, k, np.ones(len(k)), expect_lattice=True)
+    autotune.clear()

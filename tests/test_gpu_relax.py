@@ -127,3 +127,18 @@ def test_softcore_energy_forces_match_oracle():
         Eo, Fo = po.total_energy_forces(cfg.relative, cfg.basis, k, v, sigma=sigma, e0=2.0)
         assert abs(E - Eo) <= 1e-12 * max(1.0, abs(Eo))
         assert np.max(np.abs(F - Fo)) <= 1e-11 * max(1.0, np.max(np.abs(Fo)))
+
+
+def test_graph_replay_matches_stream_launches():
+    # the same solve with the optimizer steps replayed from a captured hipGraph (many replays, host checks and
+    # harvests in between) and with plain stream launches: bitwise the same result
+    specs = [stealthy(300, 2, 0.3, 31), stealthy(200, 3, 0.4, 32)]
+    out = []
+    for graph in (False, True):
+        solver = BatchSolver(max_slots=4, check_interval=8, graph=graph)
+        ids = [solver.submit(*sp, Emin=EMIN) for sp in specs]
+        solver.run(timeout=300)
+        out.append([solver.result(j) for j in ids])
+    for a, b in zip(*out):
+        assert a["energy"] == b["energy"] and a["evaluations"] == b["evaluations"]
+        assert np.array_equal(a["relative"], b["relative"])
