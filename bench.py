@@ -44,6 +44,17 @@ def parse():
                     help="run one torch device copy of this size first (calibrates rocprof FETCH/WRITE_SIZE)")
     ap.add_argument("--graph", action="store_true", help="replay optimizer steps from a captured hipGraph")
     ap.add_argument("--no-autotune", action="store_true", help="skip the tile autotuner (default tiles)")
+    ap.add_argument("--tiles", type=str, default=None,
+                    help="seg,rho_kernel,rho_chunk,force_chunk,force_ppl: use these tiles instead of autotuning")
+    ap.add_argument("--workload", default="batched",
+                    choices=["batched", "single", "mixed256", "large2d", "serve"],
+                    help="batched: the headline (64 seeds, 3D N=2000 chi=0.45); single: the same solves through the "
+                         "single-solve drop-in path; mixed256: 256 configurations of mixed N, d, chi; large2d: 2D "
+                         "N=20000 chi=0.4; serve: Poisson request stream with soft core")
+    ap.add_argument("--requests", type=int, default=48, help="serve: number of requests")
+    ap.add_argument("--rate", type=float, default=20.0, help="serve: arrivals per second")
+    ap.add_argument("--profiler-run", action="store_true",
+                    help="for runs under a counter profiler: no autotuner and no hipEvent timing (roofline null)")
     return ap.parse_args()
 
 
@@ -68,6 +79,10 @@ def main():
     from modules_amd import GetKs, RandomConfiguration, getK
     from modules_amd.optimize import BatchSolver
 
+    if args.workload != "batched":
+        print(json.dumps(OTHER_WORKLOADS[args.workload](args)), flush=True)
+        return
+
     N, d, chi = args.N, args.dim, args.chi
     K = getK(chi, 1.0, d)
     proto = RandomConfiguration(0, N, d)
@@ -89,7 +104,17 @@ def main():
     if args.copy_probe_mib > 0:
         copy_probe(args.copy_probe_mib)
     tiles = None
-    if not args.no_autotune:
+    if args.profiler_run:
+        args.no_autotune = True
+    if args.tiles:
+        from modules_amd import autotune
+        vals = [int(t) for t in args.tiles.split(",")]
+        seg, rk, rc, fc = vals[:4]
+        ppl = vals[4] if len(vals) > 4 else 2
+        autotune.record(N, d, M, seg, rk, rc, fc, ppl)
+        tiles = {"segment_len": seg, "rho_kernel": rk, "rho_chunk": rc, "force_chunk": fc, "force_ppl": ppl,
+                 "source": "--tiles"}
+    elif not args.no_autotune:
         from modules_amd import autotune
         tiles = autotune.autotune(N, d, k=k, basis=proto.basis, batch=args.batch, repeats=3)
     solver = BatchSolver(max_slots=max(args.batch, 1), check_interval=args.check_interval, graph=args.graph)
@@ -107,7 +132,7 @@ def main():
     for cfgs in warm:
         run_step(cfgs)
 
-    solver.set_timing(True)
+    solver.set_timing(not args.profiler_run)
     solver.timing(reset=True)
     results = []
     sync()
@@ -166,6 +191,106 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(proto, k, v, float(np.mean(obj_evals)), args.cpu_seconds)
     print(json.dumps(out), flush=True)
+
+
+def _solve_all(solver, jobs, emin):
+    """Run a solver until its jobs finish; returns results and wall seconds."""
+    t0 = time.perf_counter()
+    solver.run(chunk=1024)
+    wall = time.perf_counter() - t0
+    res = [solver.result(j) for j in jobs]
+    return res, wall
+
+
+def workload_single(args):
+    """The headline solves through the single-solve drop-in path (host L-BFGS in NLopt order, one GPU E+F pass
+    per objective evaluation), on --batch seeds (default 4 here: each solve is sequential)."""
+    from modules_amd import GetKs, RandomConfiguration, ShiftedCCPotential, getK
+    from modules_amd.optimize import RelaxStructure_NLOPT_Emin
+    n_seeds = min(args.batch, 4) if args.batch == 64 else args.batch
+    N, d, chi = args.N, args.dim, args.chi
+    cfgs = [RandomConfiguration(s, N, d) for s in range(n_seeds)]
+    k = GetKs(cfgs[0], getK(chi, 1.0, d))
+    pot = ShiftedCCPotential(d)
+    for kk in k:
+        pot.AddConstraint(kk, [1.0])
+    warm = RandomConfiguration(999999, N, d)
+    RelaxStructure_NLOPT_Emin(warm, pot, MaxStep=args.maxstep, Emin=args.emin)
+    t0 = time.perf_counter()
+    res = [RelaxStructure_NLOPT_Emin(c, pot, MaxStep=args.maxstep, Emin=args.emin) for c in cfgs]
+    wall = time.perf_counter() - t0
+    solved = sum(1 for E, ev, st in res if E < args.emin)
+    evals = sum(ev for E, ev, st in res)
+    return {"metric": "stealthy ground states/s at E<1e-18", "value": solved / wall, "unit": "ground_states/s",
+            "workload": f"single-solve drop-in path: {n_seeds} seeds x N={N}, d={d}, chi={chi} (M={len(k)})",
+            "solved": solved, "solves": n_seeds, "wall_s": wall, "objective_evals_per_s": evals / wall,
+            "latency_mean_s": wall / n_seeds, "dtype": "f64"}
+
+
+def workload_mixed256(args):
+    """256 independent configurations: N log-uniform in [100, 4000], d in {2, 3}, chi uniform in [0.2, 0.45]
+    (one master MT19937 seed draws N, d, chi and the per-configuration seed), all resident in one solver."""
+    from modules_amd import GetKs, RandomConfiguration, getK
+    from modules_amd.optimize import BatchSolver
+    rng = np.random.RandomState(20261203)
+    specs = []
+    for i in range(256):
+        N = int(round(math.exp(rng.uniform(math.log(100), math.log(4000)))))
+        d = 2 if rng.uniform() < 0.5 else 3
+        chi = float(rng.uniform(0.2, 0.45))
+        specs.append((N, d, chi, int(rng.randint(0, 2 ** 31 - 1))))
+    solver = BatchSolver(max_slots=256, check_interval=args.check_interval)
+    jobs, pairs = [], 0
+    for N, d, chi, seed in specs:
+        cfg = RandomConfiguration(seed, N, d)
+        k = GetKs(cfg, getK(chi, 1.0, d))
+        jobs.append(solver.submit(cfg, k, np.ones(len(k)), MaxStep=args.maxstep, Emin=args.emin))
+        pairs += N * len(k)
+    res, wall = _solve_all(solver, jobs, args.emin)
+    solved = sum(1 for r in res if r["energy"] < args.emin)
+    lat = np.array([r["t_done"] - r["t_submit"] for r in res])
+    return {"metric": "stealthy ground states/s at E<1e-18", "value": solved / wall, "unit": "ground_states/s",
+            "workload": "256 configurations, N log-uniform [100,4000], d in {2,3}, chi U[0.2,0.45]",
+            "solved": solved, "solves": len(res), "wall_s": wall,
+            "ef_evals_per_s": sum(r["device_evaluations"] for r in res) / wall,
+            "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
+            "pairs_per_contraction_total": pairs, "dtype": "f64"}
+
+
+def workload_large2d(args):
+    """2D N=20000 at density 1, chi=0.4 (M ~ 16000): one configuration relaxed to E<1e-18 in FP64."""
+    from modules_amd import GetKs, RandomConfiguration, getK
+    from modules_amd.optimize import BatchSolver
+    cfg = RandomConfiguration(4, 20000, 2)
+    k = GetKs(cfg, getK(0.4, 1.0, 2))
+    solver = BatchSolver(max_slots=1, check_interval=args.check_interval)
+    job = solver.submit(cfg, k, np.ones(len(k)), MaxStep=args.maxstep, Emin=args.emin)
+    res, wall = _solve_all(solver, [job], args.emin)
+    r = res[0]
+    return {"metric": "stealthy ground states/s at E<1e-18", "value": (1.0 if r["energy"] < args.emin else 0.0) / wall,
+            "unit": "ground_states/s", "workload": f"2D N=20000 chi=0.4 (M={len(k)}), FP64", "energy": r["energy"],
+            "objective_evals": r["evaluations"], "device_evals": r["device_evaluations"], "wall_s": wall,
+            "ef_evals_per_s": r["device_evaluations"] / wall, "dtype": "f64"}
+
+
+def workload_serve(args):
+    """Poisson request stream (rate --rate/s, --requests requests): N lognormal (median 1000, sigma_ln 0.6,
+    clipped to [100, 50000]), d in {2,3}, chi U[0.1, 0.49], soft core e0 (1 - r/sigma)^2 with e0 = 1 and
+    sigma = 0.4 density^(-1/d); continuous batching on one solver."""
+    from modules_amd.serving import make_stream, serve
+    reqs = make_stream(args.requests, args.rate)
+    reqs, wall = serve(reqs, Emin=args.emin, MaxStep=args.maxstep, max_slots=64)
+    lat = np.array([r.done - r.arrival for r in reqs])
+    solved = sum(1 for r in reqs if r.result.get("energy", 1.0) < args.emin)
+    return {"metric": "stealthy ground states/s at E<1e-18", "value": solved / wall, "unit": "ground_states/s",
+            "workload": f"serving: {args.requests} Poisson requests at {args.rate}/s, soft core e0=1 sigma=0.4",
+            "solved": solved, "requests": len(reqs), "wall_s": wall,
+            "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
+            "N_median": float(np.median([r.N for r in reqs])), "dtype": "f64"}
+
+
+OTHER_WORKLOADS = {"single": workload_single, "mixed256": workload_mixed256, "large2d": workload_large2d,
+                   "serve": workload_serve}
 
 
 def cpu_baseline(cfg, k, v, evals_per_gs, seconds):

@@ -115,9 +115,13 @@ int ccgpu_potential_uses_lattice_path(const ccgpu_potential *p, int *lattice);
  * use the recorded tiles; a system's tiles (and so its bits) depend only on its shape and this table.
  * Outputs the chosen tiles and the milliseconds of one batched E+F. */
 int ccgpu_autotune(int dim, int64_t N, const double *basis, int64_t M, const double *k, const double *V, int batch,
-                   int repeats, int *segment_len, int *rho_kernel, int *rho_chunk, int *force_chunk, double *ms);
+                   int repeats, int *segment_len, int *rho_kernel, int *rho_chunk, int *force_chunk, int *force_ppl,
+                   double *ms);
 /* Forget every recorded tile shape. */
 int ccgpu_autotune_clear(void);
+/* Record a tile shape for the (dim, N, M) shape class without timing (e.g. one found by an earlier run). */
+int ccgpu_autotune_record(int dim, int64_t N, int64_t M, int segment_len, int rho_kernel, int rho_chunk,
+                          int force_chunk, int force_ppl);
 
 /* ---------------------------------------------------------------------------------------------
  * L-BFGS (Luksan's PLIS, the algorithm of NLopt's NLOPT_LD_LBFGS) on the host

@@ -23,13 +23,19 @@ def autotune(N, d, chi=None, k=None, basis=None, batch=64, repeats=5):
         k = GetKs(Configuration(basis), getK(chi, 1.0, d))
     k = np.ascontiguousarray(k, dtype=np.float64)
     v = np.ones(len(k))
-    seg, rk, rc, fc = C.c_int(0), C.c_int(0), C.c_int(0), C.c_int(0)
+    seg, rk, rc, fc, ppl = C.c_int(0), C.c_int(0), C.c_int(0), C.c_int(0), C.c_int(0)
     ms = C.c_double(0)
     check(lib().ccgpu_autotune(d, N, basis.ctypes.data_as(dp), len(k), k.ctypes.data_as(dp), v.ctypes.data_as(dp),
-                               batch, repeats, C.byref(seg), C.byref(rk), C.byref(rc), C.byref(fc), C.byref(ms)))
+                               batch, repeats, C.byref(seg), C.byref(rk), C.byref(rc), C.byref(fc), C.byref(ppl),
+                               C.byref(ms)))
     return {"segment_len": seg.value, "rho_kernel": rk.value, "rho_chunk": rc.value, "force_chunk": fc.value,
-            "ms_per_batched_ef": ms.value}
+            "force_ppl": ppl.value, "ms_per_batched_ef": ms.value}
 
 
 def clear():
     check(lib().ccgpu_autotune_clear())
+
+
+def record(N, d, M, segment_len, rho_kernel, rho_chunk, force_chunk=0, force_ppl=2):
+    """Record a tile shape for the (d, N, M) shape class without timing."""
+    check(lib().ccgpu_autotune_record(d, N, M, segment_len, rho_kernel, rho_chunk, force_chunk, force_ppl))

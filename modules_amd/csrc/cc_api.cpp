@@ -385,7 +385,8 @@ int ccgpu_potential_relax(ccgpu_potential *p, double *relative, int maxstep, dou
 }
 
 int ccgpu_autotune(int dim, int64_t N, const double *basis, int64_t M, const double *k, const double *V, int batch,
-                   int repeats, int *segment_len, int *rho_kernel, int *rho_chunk, int *force_chunk, double *ms)
+                   int repeats, int *segment_len, int *rho_kernel, int *rho_chunk, int *force_chunk, int *force_ppl,
+                   double *ms)
 {
     return Guard([&] {
         Require(dim >= 1 && dim <= kMaxDim && basis && N > 0 && M > 0 && k && V, CCGPU_ERR_ARG, "bad argument");
@@ -396,6 +397,7 @@ int ccgpu_autotune(int dim, int64_t N, const double *basis, int64_t M, const dou
         if (rho_kernel) *rho_kernel = t.rho_kernel;
         if (rho_chunk) *rho_chunk = t.rho_chunk;
         if (force_chunk) *force_chunk = t.force_chunk;
+        if (force_ppl) *force_ppl = t.force_ppl;
         if (ms) *ms = m;
     });
 }
@@ -403,6 +405,26 @@ int ccgpu_autotune(int dim, int64_t N, const double *basis, int64_t M, const dou
 int ccgpu_autotune_clear(void)
 {
     return Guard([&] { ClearTiles(); });
+}
+
+int ccgpu_autotune_record(int dim, int64_t N, int64_t M, int segment_len, int rho_kernel, int rho_chunk,
+                          int force_chunk, int force_ppl)
+{
+    return Guard([&] {
+        Require(dim >= 1 && dim <= kMaxDim && N > 0 && M > 0, CCGPU_ERR_ARG, "bad shape");
+        Require(segment_len == 4 || segment_len == 8 || segment_len == 16 || segment_len == 24 || segment_len == 32,
+                CCGPU_ERR_ARG, "segment length must be 4, 8, 16, 24 or 32");
+        Require(rho_kernel >= 0 && rho_kernel <= 1 && rho_chunk >= 1 && rho_chunk <= 512 && force_chunk >= 0,
+                CCGPU_ERR_ARG, "bad tile values");
+        Require(force_ppl == 1 || force_ppl == 2, CCGPU_ERR_ARG, "force particles per lane must be 1 or 2");
+        Tiles t;
+        t.force_ppl = force_ppl;
+        t.seg = segment_len;
+        t.rho_kernel = rho_kernel;
+        t.rho_chunk = rho_chunk;
+        t.force_chunk = force_chunk;
+        RecordTiles(dim, N, M, t);
+    });
 }
 
 int ccgpu_potential_set_rho_kernel(ccgpu_potential *p, int variant)

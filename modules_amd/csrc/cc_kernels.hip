@@ -300,30 +300,45 @@ __global__ void __launch_bounds__(kBlock) k_energy_final(const SysDesc *__restri
 // segment data is uniform across the wavefront (scalar loads).  Accumulates G_j = sum_k c_k n_kj with
 // c_k = (2V/vol)(Im rho cos - Re rho sin); then F = -sum_j G_j b_j in k_force_final.
 // ---------------------------------------------------------------------------------------------
-template <int SEG>
+template <int SEG, int PPL>
 __global__ void __launch_bounds__(kBlock) k_force_lattice(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
 {
+    // PPL particles per lane (block covers PPL * kBlock particles): independent phase chains for ILP, and
+    // each wavefront-uniform coefficient load serves PPL particles
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
     if (!(S.flags & kSysActive)) return;
-    const int i = it.t0 * kBlock + threadIdx.x;
-    const bool valid = i < S.N;
-    const double4 p = valid ? S.pos[i] : make_double4(0, 0, 0, 0);
-    const double2 st = valid ? S.step[i] : make_double2(1, 0);
+    int idx[PPL];
+    bool valid[PPL];
+    double4 p[PPL];
+    double2 st[PPL];
+#pragma unroll
+    for (int q = 0; q < PPL; q++) {
+        idx[q] = (it.t0 * PPL + q) * kBlock + threadIdx.x;
+        valid[q] = idx[q] < S.N;
+        p[q] = valid[q] ? S.pos[idx[q]] : make_double4(0, 0, 0, 0);
+        st[q] = valid[q] ? S.step[idx[q]] : make_double2(1, 0);
+    }
     const int s0 = it.t1 * S.kchunk_len;
     const int s1 = min(S.nseg, s0 + S.kchunk_len);
     const int d = S.d;
-    double g0 = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0;
+    double g[PPL][4];
+#pragma unroll
+    for (int q = 0; q < PPL; q++) g[q][0] = g[q][1] = g[q][2] = g[q][3] = 0.0;
     const double2 *coef = reinterpret_cast<const double2 *>(S.coef);
     for (int sg = s0; sg < s1; sg++) {
         const int4 h = S.seg_head[sg];
         const int len = S.seg_len[sg];
         const double h0 = (double)h.x, h1 = (double)h.y, h2 = (double)h.z, h3 = (double)h.w;
-        const double u = __builtin_fma(h0, p.x, __builtin_fma(h1, p.y, __builtin_fma(h2, p.z, h3 * p.w)));
-        double sn, cs;
-        sincos_rev(u, sn, cs);
-        double acc = 0.0, accl = 0.0;
         const double hl = (d == 1) ? h0 : (d == 2) ? h1 : (d == 3) ? h2 : h3;
+        double sn[PPL], cs[PPL], acc[PPL], accl[PPL];
+#pragma unroll
+        for (int q = 0; q < PPL; q++) {
+            const double u = __builtin_fma(h0, p[q].x, __builtin_fma(h1, p[q].y, __builtin_fma(h2, p[q].z, h3 * p[q].w)));
+            sincos_rev(u, sn[q], cs[q]);
+            acc[q] = 0.0;
+            accl[q] = 0.0;
+        }
         const double2 *cf = coef + (int64_t)sg * SEG;
 #pragma unroll
         for (int t0 = 0; t0 < SEG; t0 += 4) {
@@ -331,34 +346,45 @@ __global__ void __launch_bounds__(kBlock) k_force_lattice(const SysDesc *__restr
 #pragma unroll
                 for (int t = t0; t < t0 + 4; t++) {
                     const double2 ab = cf[t];
-                    const double c = __builtin_fma(ab.x, cs, -ab.y * sn);
-                    acc += c;
-                    accl = __builtin_fma(c, hl + (double)t, accl);
-                    const double nc = __builtin_fma(cs, st.x, -sn * st.y);
-                    const double ns = __builtin_fma(cs, st.y, sn * st.x);
-                    cs = nc;
-                    sn = ns;
+                    const double ht = hl + (double)t;
+#pragma unroll
+                    for (int q = 0; q < PPL; q++) {
+                        const double c = __builtin_fma(ab.x, cs[q], -ab.y * sn[q]);
+                        acc[q] += c;
+                        accl[q] = __builtin_fma(c, ht, accl[q]);
+                        const double nc = __builtin_fma(cs[q], st[q].x, -sn[q] * st[q].y);
+                        const double ns = __builtin_fma(cs[q], st[q].y, sn[q] * st[q].x);
+                        cs[q] = nc;
+                        sn[q] = ns;
+                    }
                 }
             }
         }
         // components before the last axis are constant along the segment
-        if (d == 1) {
-            g0 += accl;
-        } else if (d == 2) {
-            g0 = __builtin_fma(h0, acc, g0);
-            g1 += accl;
-        } else if (d == 3) {
-            g0 = __builtin_fma(h0, acc, g0);
-            g1 = __builtin_fma(h1, acc, g1);
-            g2 += accl;
-        } else {
-            g0 = __builtin_fma(h0, acc, g0);
-            g1 = __builtin_fma(h1, acc, g1);
-            g2 = __builtin_fma(h2, acc, g2);
-            g3 += accl;
+#pragma unroll
+        for (int q = 0; q < PPL; q++) {
+            if (d == 1) {
+                g[q][0] += accl[q];
+            } else if (d == 2) {
+                g[q][0] = __builtin_fma(h0, acc[q], g[q][0]);
+                g[q][1] += accl[q];
+            } else if (d == 3) {
+                g[q][0] = __builtin_fma(h0, acc[q], g[q][0]);
+                g[q][1] = __builtin_fma(h1, acc[q], g[q][1]);
+                g[q][2] += accl[q];
+            } else {
+                g[q][0] = __builtin_fma(h0, acc[q], g[q][0]);
+                g[q][1] = __builtin_fma(h1, acc[q], g[q][1]);
+                g[q][2] = __builtin_fma(h2, acc[q], g[q][2]);
+                g[q][3] += accl[q];
+            }
         }
     }
-    if (valid) reinterpret_cast<double4 *>(S.fpart)[(int64_t)it.t1 * S.N + i] = make_double4(g0, g1, g2, g3);
+#pragma unroll
+    for (int q = 0; q < PPL; q++)
+        if (valid[q])
+            reinterpret_cast<double4 *>(S.fpart)[(int64_t)it.t1 * S.N + idx[q]] =
+                make_double4(g[q][0], g[q][1], g[q][2], g[q][3]);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -440,7 +466,8 @@ __global__ void __launch_bounds__(kBlock) k_force_final(const SysDesc *__restric
 #define CC_INSTANTIATE_SEG(SEG)                                                                            \
     template __global__ void k_rho_lattice<SEG>(const SysDesc *__restrict__, const WorkItem *__restrict__); \
     template __global__ void k_rho_lattice_tab<SEG>(const SysDesc *__restrict__, const WorkItem *__restrict__); \
-    template __global__ void k_force_lattice<SEG>(const SysDesc *__restrict__, const WorkItem *__restrict__);
+    template __global__ void k_force_lattice<SEG, 1>(const SysDesc *__restrict__, const WorkItem *__restrict__); \
+    template __global__ void k_force_lattice<SEG, 2>(const SysDesc *__restrict__, const WorkItem *__restrict__);
 CC_INSTANTIATE_SEG(4)
 CC_INSTANTIATE_SEG(8)
 CC_INSTANTIATE_SEG(16)
@@ -483,16 +510,22 @@ void launch_rho_lattice(hipStream_t stream, int seg, const SysDesc *sys, const W
     }
 }
 
-void launch_force_lattice(hipStream_t stream, int seg, const SysDesc *sys, const WorkItem *items, int n)
+void launch_force_lattice(hipStream_t stream, int seg, const SysDesc *sys, const WorkItem *items, int n, int ppl)
 {
     if (n <= 0) return;
+#define CC_FORCE_CASE(S)                                                                      \
+    case S:                                                                                   \
+        if (ppl == 2) hipLaunchKernelGGL((k_force_lattice<S, 2>), CC_GRID(n), sys, items);   \
+        else hipLaunchKernelGGL((k_force_lattice<S, 1>), CC_GRID(n), sys, items);            \
+        break;
     switch (seg) {
-        case 4: hipLaunchKernelGGL(k_force_lattice<4>, CC_GRID(n), sys, items); break;
-        case 8: hipLaunchKernelGGL(k_force_lattice<8>, CC_GRID(n), sys, items); break;
-        case 16: hipLaunchKernelGGL(k_force_lattice<16>, CC_GRID(n), sys, items); break;
-        case 24: hipLaunchKernelGGL(k_force_lattice<24>, CC_GRID(n), sys, items); break;
-        default: hipLaunchKernelGGL(k_force_lattice<32>, CC_GRID(n), sys, items); break;
+        CC_FORCE_CASE(4)
+        CC_FORCE_CASE(8)
+        CC_FORCE_CASE(16)
+        CC_FORCE_CASE(24)
+        default: CC_FORCE_CASE(32)
     }
+#undef CC_FORCE_CASE
 }
 
 void launch_rho_general(hipStream_t stream, const SysDesc *sys, const WorkItem *items, int n)

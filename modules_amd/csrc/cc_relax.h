@@ -17,6 +17,7 @@
 namespace ccgpu {
 
 constexpr int kRelaxBlock = 512;
+constexpr int kRelaxChunk = 1024;  // vector elements per block of the wide L-BFGS passes
 constexpr double kRelaxRescale[3] = {10000.0, 1.0, 0.01};  // StructureOptimization.cpp:1055-1063
 constexpr double kMaxEnergy = 1e10;                          // core/etc.h:8
 
@@ -40,12 +41,16 @@ struct RelaxDesc {
     double *xo, *go; // history pairs (mf x n each)
     double *uv;      // uo, vo (mf each)
     double *gram;    // s_a.y_b and y_a.y_b over physical history slots (2 x mf x mf)
+    double *part1;   // pass-1 partial dots: nchunks x kPlisMaxHistory x 5
+    double *part2;   // pass-2 partial |S|^2, g.S: nchunks x 2
+    double *lcoef;   // alpha, (alpha - beta), gamma of the pending direction
     int n, mf;
     int sys;         // index of the job's SysDesc in the batch
-    int pad;
+    int nchunks;     // blocks of kRelaxChunk elements in the wide passes
 };
 
-void launch_relax_step(hipStream_t stream, SysDesc *sys, const RelaxDesc *rd, int n);
+void launch_relax_step(hipStream_t stream, SysDesc *sys, const RelaxDesc *rd, int n, const WorkItem *wide_items,
+                       int nwide);
 
 // One minimisation request and, while resident, its device buffers.
 struct Job {
@@ -68,7 +73,7 @@ struct Job {
     std::string error;
     // device side (resident only)
     std::unique_ptr<System> sys;
-    DevArray<double> sdir, xo, go, uv, gram;
+    DevArray<double> sdir, xo, go, uv, gram, part1, part2, lcoef;
     DevArray<RelaxState> st;
     RelaxState host_state{};
 };
@@ -117,6 +122,8 @@ private:
     std::vector<Job *> resident_;
     std::map<int64_t, std::unique_ptr<Job>> jobs_;
     DevArray<RelaxDesc> d_rdesc_;
+    DevArray<WorkItem> d_wide_;
+    int nwide_ = 0;
     double t0_;
 };
 

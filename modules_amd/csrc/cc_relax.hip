@@ -17,11 +17,7 @@ struct BlockPlisOps {
     double *x, *g, *s, *xo, *go;
     double *lds_red;  // kRelaxBlock / 64 doubles
     double *lds_u, *lds_v;
-    double *gsy, *gyy;  // Gram matrices over physical history slots: gsy[a*mf+b] = s_a.y_b, gyy[a*mf+b] = y_a.y_b
-    double *lds_part;   // (kRelaxBlock/64) x kPlisMaxHistory x 5 wavefront partials
-    double *lds_dots;   // kPlisMaxHistory x 5 combined dots
-    double *lds_ab;     // alpha[kPlisMaxHistory], beta[kPlisMaxHistory]
-    bool gs_valid;      // g.s of the last two_loop is cached until g or s changes
+    bool gs_valid;      // g.S of the last direction is cached until g or S changes
     double gs_val;
 
     __device__ double *vec(int id) const
@@ -211,185 +207,11 @@ struct BlockPlisOps {
         __syncthreads();
     }
 
-    // S = -H g over the k newest pairs, the same L-BFGS direction as the two-loop recursion written through
-    // Gram matrices so that every dot product of one direction is formed in two passes instead of 2k+1
-    // dependent reductions:
-    //   alpha_i = u_i (-s_i.g - sum_{j<i} alpha_j s_i.y_j)                       (backward loop)
-    //   beta_i  = u_i (gamma (-y_i.g - sum_j alpha_j y_i.y_j) + sum_{j>i} (alpha_j - beta_j) y_i.s_j)
-    //   S       = gamma (-g - sum_j alpha_j y_j) + sum_j (alpha_j - beta_j) s_j     (forward loop)
-    // with gamma = b / (y_1.y_1).  Pass 1 forms s_j.g, y_j.g and the Gram row/column of the newest pair
-    // (the only pair added since the last direction); pass 2 forms S, |S|^2 and g.S.
-    __device__ double two_loop(int k, int head, double b)
-    {
-        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        constexpr int nw = kRelaxBlock / 64;
-        constexpr int kJ = 4;  // history pairs per pass-1 group (20 partial sums per thread)
-        const int p1 = head;
-        const double *s1 = xo + (size_t)p1 * n, *y1 = go + (size_t)p1 * n;
-        for (int j0 = 0; j0 < k; j0 += kJ) {
-            const int nj = min(kJ, k - j0);
-            const double *sj[kJ], *yj[kJ];
-#pragma unroll
-            for (int q = 0; q < kJ; q++) {
-                const int pj = (head + j0 + (q < nj ? q : 0)) % mf;
-                sj[q] = xo + (size_t)pj * n;
-                yj[q] = go + (size_t)pj * n;
-            }
-            double a[kJ][5];
-#pragma unroll
-            for (int q = 0; q < kJ; q++)
-#pragma unroll
-                for (int c = 0; c < 5; c++) a[q][c] = 0.0;
-            auto accumulate = [&](int e, const double *sv, const double *yv, double gv, double s1v, double y1v) {
-#pragma unroll
-                for (int q = 0; q < kJ; q++) {
-                    a[q][0] = __builtin_fma(sv[q], gv, a[q][0]);
-                    a[q][1] = __builtin_fma(yv[q], gv, a[q][1]);
-                    a[q][2] = __builtin_fma(s1v, yv[q], a[q][2]);
-                    a[q][3] = __builtin_fma(sv[q], y1v, a[q][3]);
-                    a[q][4] = __builtin_fma(y1v, yv[q], a[q][4]);
-                }
-            };
-            int e = threadIdx.x;
-            for (; e + kRelaxBlock < n; e += 2 * kRelaxBlock) {  // two elements' loads in flight
-                const int e2 = e + kRelaxBlock;
-                double sv[kJ], yv[kJ], sv2[kJ], yv2[kJ];
-                const double gv = g[e], s1v = s1[e], y1v = y1[e];
-                const double gv2 = g[e2], s1v2 = s1[e2], y1v2 = y1[e2];
-#pragma unroll
-                for (int q = 0; q < kJ; q++) {
-                    sv[q] = sj[q][e];
-                    yv[q] = yj[q][e];
-                    sv2[q] = sj[q][e2];
-                    yv2[q] = yj[q][e2];
-                }
-                accumulate(e, sv, yv, gv, s1v, y1v);
-                accumulate(e2, sv2, yv2, gv2, s1v2, y1v2);
-            }
-            for (; e < n; e += kRelaxBlock) {
-                double sv[kJ], yv[kJ];
-#pragma unroll
-                for (int q = 0; q < kJ; q++) {
-                    sv[q] = sj[q][e];
-                    yv[q] = yj[q][e];
-                }
-                accumulate(e, sv, yv, g[e], s1[e], y1[e]);
-            }
-#pragma unroll
-            for (int q = 0; q < kJ; q++) {
-#pragma unroll
-                for (int c = 0; c < 5; c++) a[q][c] = wave_sum(a[q][c]);
-                if (lane == 0 && q < nj) {
-                    double *w = lds_part + ((size_t)wave * kPlisMaxHistory + j0 + q) * 5;
-#pragma unroll
-                    for (int c = 0; c < 5; c++) w[c] = a[q][c];
-                }
-            }
-        }
-        __syncthreads();
-        for (int q = threadIdx.x; q < 5 * k; q += kRelaxBlock) {
-            const int j = q / 5, c = q - 5 * j;
-            double t = lds_part[(size_t)j * 5 + c];
-            for (int w = 1; w < nw; w++) t += lds_part[((size_t)w * kPlisMaxHistory + j) * 5 + c];
-            lds_dots[q] = t;
-        }
-        __syncthreads();
-        // Gram row and column of the newest pair, then the k x k window of both Gram matrices in logical
-        // order staged in LDS (lds_part is free again): G_sy[i][j] = s_i.y_j, G_yy[i][j] = y_i.y_j
-        for (int j = threadIdx.x; j < k; j += kRelaxBlock) {
-            const int pj = (head + j) % mf;
-            gsy[p1 * mf + pj] = lds_dots[5 * j + 2];
-            gsy[pj * mf + p1] = lds_dots[5 * j + 3];
-            gyy[p1 * mf + pj] = lds_dots[5 * j + 4];
-            gyy[pj * mf + p1] = lds_dots[5 * j + 4];
-        }
-        __syncthreads();
-        // s_i.y_j of the window for i older than j (the only Gram entries the recursions read), in LDS
-        double *tri = lds_part;  // tri[i(i-1)/2 + j], j < i: at most 64*63/2 entries
-        for (int q = threadIdx.x; q < k * k; q += kRelaxBlock) {
-            const int i = q / k, j = q - i * k;
-            if (j < i) tri[i * (i - 1) / 2 + j] = gsy[((head + i) % mf) * mf + (head + j) % mf];
-        }
-        __syncthreads();
-        double *alpha = lds_ab, *beta = lds_ab + kPlisMaxHistory, *gya = lds_ab + 2 * kPlisMaxHistory;
-        if (threadIdx.x == 0) {
-            for (int i = 0; i < k; i++) {  // backward loop (newest to oldest)
-                double t = -lds_dots[5 * i];
-                for (int j = 0; j < i; j++) t -= alpha[j] * tri[i * (i - 1) / 2 + j];
-                alpha[i] = lds_u[i] * t;
-                lds_v[i] = alpha[i];
-            }
-        }
-        __syncthreads();
-        for (int i = threadIdx.x; i < k; i += kRelaxBlock) {  // (G_yy alpha)_i
-            const int pi = (head + i) % mf;
-            double t = 0.0;
-            for (int j = 0; j < k; j++) t += alpha[j] * gyy[pi * mf + (head + j) % mf];
-            gya[i] = t;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const double a = gyy[p1 * mf + p1];
-            const double gamma = a > 0. ? b / a : 1.0;
-            for (int i = k - 1; i >= 0; i--) {  // forward loop (oldest to newest)
-                double t = (-lds_dots[5 * i + 1] - gya[i]) * gamma;
-                for (int j = i + 1; j < k; j++) t += (alpha[j] - beta[j]) * tri[j * (j - 1) / 2 + i];
-                beta[i] = lds_u[i] * t;
-            }
-            for (int i = 0; i < k; i++) beta[i] = alpha[i] - beta[i];  // coefficient of s_i
-            lds_ab[3 * kPlisMaxHistory] = gamma;
-        }
-        __syncthreads();
-        const double gamma = lds_ab[3 * kPlisMaxHistory];
-        const double *coef_s = lds_ab + kPlisMaxHistory;
-        double ss = 0.0, gs = 0.0;
-        constexpr int kH = 8;  // history pairs whose elements are loaded together
-        for (int e = threadIdx.x; e < n; e += kRelaxBlock) {
-            const double gv = g[e];
-            double q = -gv, r = 0.0;
-            int j = 0;
-            for (; j + kH <= k; j += kH) {
-                double vy[kH], vs[kH];
-#pragma unroll
-                for (int u = 0; u < kH; u++) {
-                    const size_t off = (size_t)((head + j + u) % mf) * n + e;
-                    vy[u] = go[off];
-                    vs[u] = xo[off];
-                }
-#pragma unroll
-                for (int u = 0; u < kH; u++) {
-                    q = __builtin_fma(-alpha[j + u], vy[u], q);
-                    r = __builtin_fma(coef_s[j + u], vs[u], r);
-                }
-            }
-            for (; j < k; j++) {
-                const int pj = (head + j) % mf;
-                q = __builtin_fma(-alpha[j], go[(size_t)pj * n + e], q);
-                r = __builtin_fma(coef_s[j], xo[(size_t)pj * n + e], r);
-            }
-            const double out = __builtin_fma(gamma, q, r);
-            s[e] = out;
-            ss = __builtin_fma(out, out, ss);
-            gs = __builtin_fma(gv, out, gs);
-        }
-        ss = wave_sum(ss);
-        gs = wave_sum(gs);
-        __syncthreads();
-        if (lane == 0) {
-            lds_red[wave] = ss;
-            lds_dots[wave] = gs;
-        }
-        __syncthreads();
-        double tss = lds_red[0], tgs = lds_dots[0];
-        for (int w = 1; w < nw; w++) {
-            tss += lds_red[w];
-            tgs += lds_dots[w];
-        }
-        __syncthreads();
-        gs_valid = true;
-        gs_val = tgs;
-        return sqrt(tss);
-    }
+    // the two-loop direction is formed by the wide kernels (k_lbfgs_pass1 / solve / pass2) between the two
+    // launches of k_relax_step; the second launch resumes PlisAdvance with |S| and g.S from them
+    static constexpr bool kDeferTwoLoop = true;
+    double res_snorm;
+    __device__ double resume_two_loop(int, int, double) const { return res_snorm; }
 };
 
 __device__ __forceinline__ double relax_wrap(double x)
@@ -399,54 +221,66 @@ __device__ __forceinline__ double relax_wrap(double x)
     return y;
 }
 
-// One optimizer step for every resident job: consume the evaluation just made (energy in SysDesc.energy,
-// gradient in SysDesc.grad), advance PLIS until it needs the next point, handle pass boundaries.
-// Grid: one block of kRelaxBlock threads per job.
-__global__ void __launch_bounds__(kRelaxBlock) k_relax_step(SysDesc *__restrict__ sysv, const RelaxDesc *__restrict__ rdv)
+// One optimizer step for every resident job.  Stage 0 consumes the evaluation just made (energy in
+// SysDesc.energy, gradient in SysDesc.grad) and advances PLIS until it needs the next point, a new two-loop
+// direction (left to the wide kernels), or the end of a pass; stage 1 resumes the jobs that asked for a
+// direction.  Grid: one block of kRelaxBlock threads per job.
+__global__ void __launch_bounds__(kRelaxBlock) k_relax_step(SysDesc *__restrict__ sysv, const RelaxDesc *__restrict__ rdv,
+                                                           int stage)
 {
     __shared__ double lds_red[kRelaxBlock / 64];
     __shared__ double lds_u[kPlisMaxHistory], lds_v[kPlisMaxHistory];
-    __shared__ double lds_part[(kRelaxBlock / 64) * kPlisMaxHistory * 5];
-    __shared__ double lds_dots[kPlisMaxHistory * 5];
-    __shared__ double lds_ab[3 * kPlisMaxHistory + 1];
     const RelaxDesc R = rdv[blockIdx.x];
     SysDesc *D = sysv + R.sys;
     if (!(D->flags & kSysActive)) return;
+    if (stage == 1 && R.st->plis.phase != kPhDirection) return;
     RelaxState st = *R.st;
     for (int i = threadIdx.x; i < R.mf; i += kRelaxBlock) {
         lds_u[i] = R.uv[i];
         lds_v[i] = R.uv[R.mf + i];
     }
     __syncthreads();
-    BlockPlisOps ops{R.n,     R.mf,      const_cast<double *>(D->x), D->grad,  R.s,   R.xo, R.go, lds_red,
-                     lds_u,   lds_v,     R.gram,                     R.gram + R.mf * R.mf,   lds_part, lds_dots,
-                     lds_ab,  false,     0.0};
-    const double E = D->energy[0];
-    st.E = E;
-    st.device_evals++;
+    BlockPlisOps ops{R.n, R.mf, const_cast<double *>(D->x), D->grad, R.s, R.xo, R.go, lds_red, lds_u, lds_v,
+                     false, 0.0, 0.0};
     int act = kPlisNeedEval;
     bool finish = false;
-    if (st.boundary) {
-        // RelaxStructure_NLOPT_inner starts with "if (pot.Energy() < fmin) return" and _Emin stops between
-        // passes once E < Emin; after the third pass the solve ends.
-        if (st.pass >= 3 || E < st.fmin) {
-            finish = true;
-        } else {
-            for (int i = threadIdx.x; i < 2 * R.mf * R.n; i += kRelaxBlock) {
-                if (i < R.mf * R.n) R.xo[i] = 0.0;
-                else R.go[i - R.mf * R.n] = 0.0;
+    if (stage == 1) {
+        // |S|^2 and g.S of the direction from the pass-2 partials, in chunk order
+        double ss = 0.0, gs = 0.0;
+        for (int c = 0; c < R.nchunks; c++) {
+            ss += R.part2[2 * c];
+            gs += R.part2[2 * c + 1];
+        }
+        ops.res_snorm = sqrt(ss);
+        ops.gs_valid = true;
+        ops.gs_val = gs;
+        act = PlisAdvance(st.plis, ops);
+    } else {
+        const double E = D->energy[0];
+        st.E = E;
+        st.device_evals++;
+        if (st.boundary) {
+            // RelaxStructure_NLOPT_inner starts with "if (pot.Energy() < fmin) return" and _Emin stops between
+            // passes once E < Emin; after the third pass the solve ends.
+            if (st.pass >= 3 || E < st.fmin) {
+                finish = true;
+            } else {
+                for (int i = threadIdx.x; i < 2 * R.mf * R.n; i += kRelaxBlock) {
+                    if (i < R.mf * R.n) R.xo[i] = 0.0;
+                    else R.go[i - R.mf * R.n] = 0.0;
+                }
+                for (int i = threadIdx.x; i < R.mf; i += kRelaxBlock) lds_u[i] = lds_v[i] = 0.0;
+                __syncthreads();
+                PlisInit(st.plis, R.n, R.mf, st.maxstep, st.fmin);
+                PlisAdvance(st.plis, ops);  // asks for the first evaluation: the one just made
+                st.boundary = 0;
+                st.plis.f = E;
+                act = PlisAdvance(st.plis, ops);
             }
-            for (int i = threadIdx.x; i < R.mf; i += kRelaxBlock) lds_u[i] = lds_v[i] = 0.0;
-            __syncthreads();
-            PlisInit(st.plis, R.n, R.mf, st.maxstep, st.fmin);
-            PlisAdvance(st.plis, ops);  // asks for the first evaluation: the one just made
-            st.boundary = 0;
+        } else {
             st.plis.f = E;
             act = PlisAdvance(st.plis, ops);
         }
-    } else {
-        st.plis.f = E;
-        act = PlisAdvance(st.plis, ops);
     }
     if (!finish && act == kPlisDone) {
         // end of a pass: s = wrap(x / rescale) (GetConfig), then x = s * next rescale (GetParam)
@@ -475,9 +309,248 @@ __global__ void __launch_bounds__(kRelaxBlock) k_relax_step(SysDesc *__restrict_
     }
 }
 
-void launch_relax_step(hipStream_t stream, SysDesc *sys, const RelaxDesc *rd, int n)
+// ------------------------------------------------------------------------------------------------
+// Two-loop direction of the jobs that asked for one, written through Gram matrices so that it takes two
+// passes over the history instead of 2k+1 dependent reductions:
+//   alpha_i = u_i (-s_i.g - sum_{j<i} alpha_j s_i.y_j)                                (backward loop)
+//   beta_i  = u_i (gamma (-y_i.g - sum_j alpha_j y_i.y_j) + sum_{j>i} (alpha_j - beta_j) s_j.y_i)
+//   S       = gamma (-g - sum_j alpha_j y_j) + sum_j (alpha_j - beta_j) s_j               (forward loop)
+// with gamma = b / (y_1.y_1) and logical column 1 the newest pair; the same direction as the two-loop
+// recursion in exact arithmetic.  Pass 1 (many blocks per job) forms s_j.g, y_j.g and the Gram row and
+// column of the newest pair (the only pair added since the last direction); the solve kernel (one block
+// per job) reduces them, updates the Gram matrices and runs the two scalar recursions; pass 2 (many blocks
+// per job) forms S and the partial sums of |S|^2 and g.S.
+// ------------------------------------------------------------------------------------------------
+constexpr int kWideBlock = 256;
+constexpr int kWideChunk = kRelaxChunk;  // elements per pass block
+
+__device__ __forceinline__ double wsum64(double v)
 {
-    if (n > 0) hipLaunchKernelGGL(k_relax_step, dim3(n), dim3(kRelaxBlock), 0, stream, sys, rd);
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+__global__ void __launch_bounds__(kWideBlock) k_lbfgs_pass1(const SysDesc *__restrict__ sysv, const RelaxDesc *__restrict__ rdv,
+                                                            const WorkItem *__restrict__ items)
+{
+    constexpr int kJ = 4;
+    constexpr int nw = kWideBlock / 64;
+    __shared__ double lds[nw * kJ * 5];
+    const WorkItem it = items[blockIdx.x];
+    const RelaxDesc R = rdv[it.sys];
+    const SysDesc &D = sysv[R.sys];
+    if (!(D.flags & kSysActive) || R.st->plis.phase != kPhDirection) return;
+    const int k = R.st->plis.kdir, head = R.st->plis.head, mf = R.mf, n = R.n;
+    const int e0 = it.t0 * kWideChunk, e1 = min(n, e0 + kWideChunk);
+    const double *g = D.grad;
+    const double *s1 = R.xo + (size_t)head * n, *y1 = R.go + (size_t)head * n;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int j0 = 0; j0 < k; j0 += kJ) {
+        const double *sj[kJ], *yj[kJ];
+#pragma unroll
+        for (int q = 0; q < kJ; q++) {
+            const int pj = (head + j0 + (j0 + q < k ? q : 0)) % mf;
+            sj[q] = R.xo + (size_t)pj * n;
+            yj[q] = R.go + (size_t)pj * n;
+        }
+        double a[kJ][5];
+#pragma unroll
+        for (int q = 0; q < kJ; q++)
+#pragma unroll
+            for (int c = 0; c < 5; c++) a[q][c] = 0.0;
+        for (int e = e0 + threadIdx.x; e < e1; e += kWideBlock) {
+            const double gv = g[e], s1v = s1[e], y1v = y1[e];
+            double sv[kJ], yv[kJ];
+#pragma unroll
+            for (int q = 0; q < kJ; q++) {
+                sv[q] = sj[q][e];
+                yv[q] = yj[q][e];
+            }
+#pragma unroll
+            for (int q = 0; q < kJ; q++) {
+                a[q][0] = __builtin_fma(sv[q], gv, a[q][0]);
+                a[q][1] = __builtin_fma(yv[q], gv, a[q][1]);
+                a[q][2] = __builtin_fma(s1v, yv[q], a[q][2]);
+                a[q][3] = __builtin_fma(sv[q], y1v, a[q][3]);
+                a[q][4] = __builtin_fma(y1v, yv[q], a[q][4]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kJ; q++)
+#pragma unroll
+            for (int c = 0; c < 5; c++) {
+                const double v = wsum64(a[q][c]);
+                if (lane == 0) lds[(wave * kJ + q) * 5 + c] = v;
+            }
+        __syncthreads();
+        if (threadIdx.x < kJ * 5) {
+            const int q = threadIdx.x / 5, c = threadIdx.x - 5 * q;
+            if (j0 + q < k) {
+                double t = lds[q * 5 + c];
+                for (int w = 1; w < nw; w++) t += lds[(w * kJ + q) * 5 + c];
+                R.part1[((size_t)it.t0 * kPlisMaxHistory + j0 + q) * 5 + c] = t;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ int tri(int i, int j) { return (i * (i + 1) >> 1) + j; }
+
+__global__ void __launch_bounds__(kWideBlock) k_lbfgs_solve(const SysDesc *__restrict__ sysv, const RelaxDesc *__restrict__ rdv)
+{
+    constexpr int H = kPlisMaxHistory;
+    __shared__ double dots[H * 5];
+    // lower triangles (j <= i, packed at i(i+1)/2 + j) of the window s_i.y_j and y_i.y_j in logical order
+    // (i = 0 newest); the recursions read s_i.y_j only for j < i and y_i.y_j is symmetric
+    __shared__ double gsy_w[H * (H + 1) / 2];
+    __shared__ double gyy_w[H * (H + 1) / 2];
+    __shared__ double uu[H];
+    const RelaxDesc R = rdv[blockIdx.x];
+    const SysDesc &D = sysv[R.sys];
+    if (!(D.flags & kSysActive) || R.st->plis.phase != kPhDirection) return;
+    const int k = R.st->plis.kdir, head = R.st->plis.head, mf = R.mf;
+    const double b = R.st->plis.bdir;
+    double *gsy = R.gram, *gyy = R.gram + mf * mf;
+    const int p1 = head;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int q = threadIdx.x; q < 5 * k; q += kWideBlock) {
+        double t = 0.0;
+        for (int c = 0; c < R.nchunks; c++) t += R.part1[(size_t)c * H * 5 + q];
+        dots[q] = t;
+    }
+    for (int i = threadIdx.x; i < k; i += kWideBlock) uu[i] = R.uv[i];
+    __syncthreads();
+    for (int j = threadIdx.x; j < k; j += kWideBlock) {  // Gram row and column of the newest pair
+        const int pj = (head + j) % mf;
+        gsy[p1 * mf + pj] = dots[5 * j + 2];
+        gsy[pj * mf + p1] = dots[5 * j + 3];
+        gyy[p1 * mf + pj] = dots[5 * j + 4];
+        gyy[pj * mf + p1] = dots[5 * j + 4];
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < k * k; q += kWideBlock) {
+        const int i = q / k, j = q - i * k;
+        if (j > i) continue;
+        const int pi = (head + i) % mf, pj = (head + j) % mf;
+        gsy_w[tri(i, j)] = gsy[pi * mf + pj];
+        gyy_w[tri(i, j)] = gyy[pi * mf + pj];
+    }
+    __syncthreads();
+    // the two recursions run in wavefront 0: lane j holds the j-th term of each inner sum, the butterfly
+    // sum gives every lane the same total (fixed order), k <= 64 terms
+    if (wave == 0) {
+        const double a = gyy_w[0];
+        const double gamma = a > 0. ? b / a : 1.0;
+        double aj = 0.0;  // alpha of this lane's pair once known
+        for (int i = 0; i < k; i++) {  // backward loop (newest to oldest): alpha_i needs alpha_j, j < i
+            const double term = (lane < i) ? aj * gsy_w[tri(i, lane)] : 0.0;
+            const double t = -dots[5 * i] - wsum64(term);
+            const double ai = uu[i] * t;
+            if (lane == i) aj = ai;
+        }
+        // (G_yy alpha)_i for this lane's i
+        double gya = 0.0;
+        for (int j = 0; j < k; j++) {
+            const double alj = __shfl(aj, j, 64);
+            if (lane < k) gya = __builtin_fma(alj, gyy_w[j <= lane ? tri(lane, j) : tri(j, lane)], gya);
+        }
+        double cj = 0.0;  // (alpha_j - beta_j) of this lane's pair once known
+        for (int i = k - 1; i >= 0; i--) {  // forward loop (oldest to newest): beta_i needs j > i
+            const double term = (lane > i && lane < k) ? cj * gsy_w[tri(lane, i)] : 0.0;
+            const double gyai = __shfl(gya, i, 64);
+            const double t = (-dots[5 * i + 1] - gyai) * gamma + wsum64(term);
+            const double bi = uu[i] * t;
+            const double ali = __shfl(aj, i, 64);
+            if (lane == i) cj = ali - bi;
+        }
+        if (lane < k) {
+            R.lcoef[lane] = aj;
+            R.lcoef[H + lane] = cj;  // coefficient of s_i
+        }
+        if (lane == 0) R.lcoef[2 * H] = gamma;
+    }
+}
+
+__global__ void __launch_bounds__(kWideBlock) k_lbfgs_pass2(const SysDesc *__restrict__ sysv, const RelaxDesc *__restrict__ rdv,
+                                                            const WorkItem *__restrict__ items)
+{
+    constexpr int nw = kWideBlock / 64;
+    constexpr int kH = 8;
+    __shared__ double al[kPlisMaxHistory], cs[kPlisMaxHistory];
+    __shared__ double red[2 * nw];
+    const WorkItem it = items[blockIdx.x];
+    const RelaxDesc R = rdv[it.sys];
+    const SysDesc &D = sysv[R.sys];
+    if (!(D.flags & kSysActive) || R.st->plis.phase != kPhDirection) return;
+    const int k = R.st->plis.kdir, head = R.st->plis.head, mf = R.mf, n = R.n;
+    for (int i = threadIdx.x; i < k; i += kWideBlock) {
+        al[i] = R.lcoef[i];
+        cs[i] = R.lcoef[kPlisMaxHistory + i];
+    }
+    const double gamma = R.lcoef[2 * kPlisMaxHistory];
+    __syncthreads();
+    const int e0 = it.t0 * kWideChunk, e1 = min(n, e0 + kWideChunk);
+    const double *g = D.grad;
+    double ss = 0.0, gs = 0.0;
+    for (int e = e0 + threadIdx.x; e < e1; e += kWideBlock) {
+        const double gv = g[e];
+        double q = -gv, r = 0.0;
+        int j = 0;
+        for (; j + kH <= k; j += kH) {
+            double vy[kH], vs[kH];
+#pragma unroll
+            for (int v = 0; v < kH; v++) {
+                const size_t off = (size_t)((head + j + v) % mf) * n + e;
+                vy[v] = R.go[off];
+                vs[v] = R.xo[off];
+            }
+#pragma unroll
+            for (int v = 0; v < kH; v++) {
+                q = __builtin_fma(-al[j + v], vy[v], q);
+                r = __builtin_fma(cs[j + v], vs[v], r);
+            }
+        }
+        for (; j < k; j++) {
+            const size_t off = (size_t)((head + j) % mf) * n + e;
+            q = __builtin_fma(-al[j], R.go[off], q);
+            r = __builtin_fma(cs[j], R.xo[off], r);
+        }
+        const double out = __builtin_fma(gamma, q, r);
+        R.s[e] = out;
+        ss = __builtin_fma(out, out, ss);
+        gs = __builtin_fma(gv, out, gs);
+    }
+    ss = wsum64(ss);
+    gs = wsum64(gs);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[2 * wave] = ss;
+        red[2 * wave + 1] = gs;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = red[0], c = red[1];
+        for (int w = 1; w < nw; w++) {
+            a += red[2 * w];
+            c += red[2 * w + 1];
+        }
+        R.part2[2 * it.t0] = a;
+        R.part2[2 * it.t0 + 1] = c;
+    }
+}
+
+void launch_relax_step(hipStream_t stream, SysDesc *sys, const RelaxDesc *rd, int n, const WorkItem *wide_items,
+                       int nwide)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_relax_step, dim3(n), dim3(kRelaxBlock), 0, stream, sys, rd, 0);
+    if (nwide > 0) {
+        hipLaunchKernelGGL(k_lbfgs_pass1, dim3(nwide), dim3(kWideBlock), 0, stream, sys, rd, wide_items);
+        hipLaunchKernelGGL(k_lbfgs_solve, dim3(n), dim3(kWideBlock), 0, stream, sys, rd);
+        hipLaunchKernelGGL(k_lbfgs_pass2, dim3(nwide), dim3(kWideBlock), 0, stream, sys, rd, wide_items);
+    }
+    hipLaunchKernelGGL(k_relax_step, dim3(n), dim3(kRelaxBlock), 0, stream, sys, rd, 1);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -561,6 +634,10 @@ void Solver::Admit()
             j->go.Reserve(std::max<int64_t>(n * j->mf, 1));
             j->uv.Reserve(2 * j->mf);
             j->gram.Reserve(2 * j->mf * j->mf);
+            const int nch = (int)std::max<int64_t>(1, (n + kRelaxChunk - 1) / kRelaxChunk);
+            j->part1.Reserve((size_t)nch * kPlisMaxHistory * 5);
+            j->part2.Reserve((size_t)nch * 2);
+            j->lcoef.Reserve(2 * kPlisMaxHistory + 1);
             j->st.Reserve(1);
             std::vector<double> x0(n);
             for (int64_t i = 0; i < n; i++) x0[i] = j->s0[i] * kRelaxRescale[0];  // GetParam (:758-771)
@@ -595,6 +672,7 @@ void Solver::Rebuild()
     DropGraph();
     batch.systems.clear();
     std::vector<RelaxDesc> rd;
+    std::vector<WorkItem> wide;
     for (size_t i = 0; i < resident_.size(); i++) {
         Job *j = resident_[i];
         batch.systems.push_back(j->sys.get());
@@ -606,6 +684,11 @@ void Solver::Rebuild()
         r.go = j->go.get();
         r.uv = j->uv.get();
         r.gram = j->gram.get();
+        r.part1 = j->part1.get();
+        r.part2 = j->part2.get();
+        r.lcoef = j->lcoef.get();
+        r.nchunks = (int)std::max<int64_t>(1, (j->N * j->d + kRelaxChunk - 1) / kRelaxChunk);
+        for (int c = 0; c < r.nchunks; c++) wide.push_back({(int32_t)i, c, 0, 0});
         r.n = (int)(j->N * j->d);
         r.mf = j->mf;
         r.sys = (int)i;
@@ -616,6 +699,11 @@ void Solver::Rebuild()
     d_rdesc_.Reserve(std::max<size_t>(1, rd.size()));
     if (!rd.empty())
         CC_HIP(hipMemcpyAsync(d_rdesc_.get(), rd.data(), rd.size() * sizeof(RelaxDesc), hipMemcpyHostToDevice,
+                              batch.stream));
+    d_wide_.Reserve(std::max<size_t>(1, wide.size()));
+    nwide_ = (int)wide.size();
+    if (!wide.empty())
+        CC_HIP(hipMemcpyAsync(d_wide_.get(), wide.data(), wide.size() * sizeof(WorkItem), hipMemcpyHostToDevice,
                               batch.stream));
     CC_HIP(hipStreamSynchronize(batch.stream));
     dirty_ = false;
@@ -663,6 +751,9 @@ void Solver::Harvest()
         j->go.Release();
         j->uv.Release();
         j->gram.Release();
+        j->part1.Release();
+        j->part2.Release();
+        j->lcoef.Release();
         j->st.Release();
         bytes_in_use_ -= j->bytes;
         dirty_ = true;
@@ -682,7 +773,8 @@ void Solver::LaunchBurst(int burst)
     auto enqueue = [&] {
         for (int it = 0; it < burst; it++) {
             batch.Enqueue(true, true);
-            launch_relax_step(batch.stream, batch.d_desc.get(), d_rdesc_.get(), (int)resident_.size());
+            launch_relax_step(batch.stream, batch.d_desc.get(), d_rdesc_.get(), (int)resident_.size(), d_wide_.get(),
+                              nwide_);
         }
     };
     // hipGraph replay of `burst` optimizer steps (no memset nodes: every kernel writes what it reads next);

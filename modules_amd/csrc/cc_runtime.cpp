@@ -63,7 +63,8 @@ Tiles DefaultTiles(int d, int64_t N, int64_t M)
     c = ((c + 31) / 32) * 32;
     t.rho_chunk = (int)std::min<int64_t>(512, std::max<int64_t>(64, c));
     t.force_chunk = 0;  // resolved in Prepare from the segment count
-    t.rho_kernel = 1;
+    t.rho_kernel = 0;
+    t.force_ppl = 2;
     return t;
 }
 
@@ -120,6 +121,8 @@ void System::Prepare(hipStream_t stream)
         throw Error(kErrArg, "segment length must be 4, 8, 16, 24 or 32");
     used.rho_chunk = tiles.rho_chunk > 0 ? std::min(tiles.rho_chunk, 512) : def.rho_chunk;
     used.rho_kernel = tiles.rho_kernel >= 0 ? tiles.rho_kernel : def.rho_kernel;
+    used.force_ppl = tiles.force_ppl > 0 ? tiles.force_ppl : (def.force_ppl > 0 ? def.force_ppl : 2);
+    if (used.force_ppl != 1 && used.force_ppl != 2) throw Error(kErrArg, "force particles per lane must be 1 or 2");
 
     if (layout_dirty || plan.S != used.seg) {
         lattice = M > 0 && LatticeIndices(d, a, M, k.data(), n);
@@ -164,7 +167,7 @@ void System::Prepare(hipStream_t stream)
 
     // chunking of the two contractions
     pchunks = (int)std::max<int64_t>(1, (N + used.rho_chunk - 1) / used.rho_chunk);
-    const int64_t pblocks = std::max<int64_t>(1, (N + kBlock - 1) / kBlock);
+    const int64_t pblocks = std::max<int64_t>(1, (N + kBlock * used.force_ppl - 1) / (kBlock * used.force_ppl));
     const int fc = tiles.force_chunk > 0 ? tiles.force_chunk : def.force_chunk;
     if (fc > 0) {
         used.force_chunk = fc;
@@ -316,20 +319,23 @@ void Batch::Build()
             for (int b = 0; b < pblocks(systems[i]); b++) h_items.push_back({i, b, 0, 0});
     });
     // lattice systems grouped by (segment length, rho kernel): one template instance per group
-    std::vector<std::pair<int, int>> segs;
+    std::vector<std::tuple<int, int, int>> segs;
     for (System *s : systems) {
-        const std::pair<int, int> key(s->used.seg, s->used.rho_kernel);
+        const std::tuple<int, int, int> key(s->used.seg, s->used.rho_kernel, s->used.force_ppl);
         if (s->lattice && std::find(segs.begin(), segs.end(), key) == segs.end()) segs.push_back(key);
     }
     for (const auto &key : segs) {
-        const int sl = key.first, rk = key.second;
+        const int sl = std::get<0>(key), rk = std::get<1>(key), ppl = std::get<2>(key);
         SegGroup g;
         g.seg = sl;
         g.rho_kernel = rk;
+        g.force_ppl = ppl;
         add(g.rho, [&] {
             for (int i = 0; i < ns; i++) {
                 System *s = systems[i];
-                if (!s->lattice || s->used.seg != sl || s->used.rho_kernel != rk || s->N == 0) continue;
+                if (!s->lattice || s->used.seg != sl || s->used.rho_kernel != rk || s->used.force_ppl != ppl ||
+                    s->N == 0)
+                    continue;
                 const int sb = (int)((s->plan.nseg + kBlock - 1) / kBlock);
                 for (int c = 0; c < s->pchunks; c++)
                     for (int b = 0; b < sb; b++) h_items.push_back({i, b, c, 0});
@@ -338,9 +344,10 @@ void Batch::Build()
         add(g.force, [&] {
             for (int i = 0; i < ns; i++) {
                 System *s = systems[i];
-                if (!s->lattice || s->used.seg != sl || s->used.rho_kernel != rk) continue;
+                if (!s->lattice || s->used.seg != sl || s->used.rho_kernel != rk || s->used.force_ppl != ppl) continue;
+                const int fb = (int)((s->N + kBlock * ppl - 1) / (kBlock * ppl));
                 for (int c = 0; c < s->kchunks; c++)
-                    for (int b = 0; b < pblocks(s); b++) h_items.push_back({i, b, c, 0});
+                    for (int b = 0; b < fb; b++) h_items.push_back({i, b, c, 0});
             }
         });
         seg_groups.push_back(g);
@@ -425,7 +432,7 @@ void Batch::Enqueue(bool rho, bool forces)
             ev_marks_.push_back({1, ev_used_});
             CC_HIP(hipEventRecord(NextEvent(), stream));
         }
-        for (const SegGroup &g : seg_groups) launch_force_lattice(stream, g.seg, D, I + g.force.off, g.force.n);
+        for (const SegGroup &g : seg_groups) launch_force_lattice(stream, g.seg, D, I + g.force.off, g.force.n, g.force_ppl);
         launch_force_general(stream, D, I + r_f_gen.off, r_f_gen.n);
         if (timing) CC_HIP(hipEventRecord(NextEvent(), stream));
         launch_force_final(stream, D, I + r_ffin.off, r_ffin.n);
@@ -454,15 +461,17 @@ double Autotune(int d, int64_t N, const double *basis, int64_t M, const double *
     for (int seg : {8, 16, 24})
         for (int rk : {0, 1})
             for (int rc : {128, 256, 512})
-                for (int fc : {0, 32}) {
-                    if (rc > 2 * N && rc > 64) continue;
-                    Tiles t;
-                    t.seg = seg;
-                    t.rho_kernel = rk;
-                    t.rho_chunk = rc;
-                    t.force_chunk = fc;
-                    cands.push_back(t);
-                }
+                for (int fc : {0, 32})
+                    for (int ppl : {1, 2}) {
+                        if (rc > 2 * N && rc > 128) continue;
+                        Tiles t;
+                        t.seg = seg;
+                        t.rho_kernel = rk;
+                        t.rho_chunk = rc;
+                        t.force_chunk = fc;
+                        t.force_ppl = ppl;
+                        cands.push_back(t);
+                    }
     hipEvent_t e0, e1;
     CC_HIP(hipEventCreate(&e0));
     CC_HIP(hipEventCreate(&e1));

@@ -71,6 +71,7 @@ struct Tiles {
     int rho_chunk = 0;
     int force_chunk = 0;
     int rho_kernel = -1;  // 0: phase recurrence, 1: LDS power tables; -1 automatic
+    int force_ppl = 0;    // particles per lane of the force kernel (1 or 2); 0 automatic
 };
 Tiles DefaultTiles(int d, int64_t N, int64_t M);
 
@@ -96,7 +97,7 @@ public:
     double a[16] = {}, b[16] = {}, vol = 0;
     std::vector<double> k, v;  // constraints (host copy): k M x d, V M
     double shift = 0, e0 = 0, sigma = 0, rescale = 1;
-    Tiles tiles{0, 0, 0, -1};   // requested (0 / -1 = automatic)
+    Tiles tiles{0, 0, 0, -1, 0};  // requested (0 / -1 = automatic)
     Tiles used;                 // resolved
 
     // constraint layout
@@ -155,7 +156,7 @@ public:
         int off = 0, n = 0;
     };
     struct SegGroup {
-        int seg, rho_kernel;
+        int seg, rho_kernel, force_ppl;
         Range rho, force;
     };
     // optional kernel timing: hipEvents around the rho and force contraction launches of every Enqueue
@@ -183,7 +184,7 @@ void launch_rho_lattice(hipStream_t, int seg, const SysDesc *, const WorkItem *,
 void launch_rho_general(hipStream_t, const SysDesc *, const WorkItem *, int);
 void launch_rho_reduce(hipStream_t, const SysDesc *, const WorkItem *, int);
 void launch_energy_final(hipStream_t, const SysDesc *, const WorkItem *, int);
-void launch_force_lattice(hipStream_t, int seg, const SysDesc *, const WorkItem *, int);
+void launch_force_lattice(hipStream_t, int seg, const SysDesc *, const WorkItem *, int, int ppl);
 void launch_force_general(hipStream_t, const SysDesc *, const WorkItem *, int);
 void launch_force_final(hipStream_t, const SysDesc *, const WorkItem *, int);
 void launch_softcore(hipStream_t, const SysDesc *, const WorkItem *part_items, int npart, const WorkItem *sys_items,

@@ -31,9 +31,12 @@ constexpr int kPlisMaxHistory = 64;
 enum PlisVec : int { kVecX = 0, kVecG = 1, kVecS = 2, kVecXO = 3 /* + slot */ };
 // history slot j of the difference pairs: XO = kVecXO + j, GO = kVecXO + mf + j
 
-enum PlisAction : int { kPlisNeedEval = 1, kPlisDone = 2 };
+// kPlisNeedDirection: the policy computes the two-loop direction outside PlisAdvance (Ops::kDeferTwoLoop);
+// the caller forms S = -H g over the S.kdir newest pairs and calls PlisAdvance again with |S| available
+// from ops.resume_two_loop().
+enum PlisAction : int { kPlisNeedEval = 1, kPlisDone = 2, kPlisNeedDirection = 3 };
 
-enum PlisPhase : int { kPhInit = 0, kPhFirstEval = 1, kPhLineSearchEval = 2, kPhDone = 3 };
+enum PlisPhase : int { kPhInit = 0, kPhFirstEval = 1, kPhLineSearchEval = 2, kPhDone = 3, kPhDirection = 4 };
 
 struct PlisState {
     // problem
@@ -50,6 +53,9 @@ struct PlisState {
     // ps1l01 state
     double fl, fu, pl, rl, pu, ru;
     int mes1, mes2, mes3, mode, mtyp;
+    // pending direction: number of pairs and b = s_1.y_1
+    int kdir;
+    double bdir;
     // the mf scalars uo (1/(s.y) of each pair) and vo (two-loop coefficients) live with the Ops policy
 };
 
@@ -319,6 +325,8 @@ PLIS_HD inline void PlisInit(PlisState &S, int n, int mf, int maxeval, double st
     S.gnorm = S.snorm = S.gmax = S.umax = S.par1 = S.par2 = 0;
     S.fl = S.fu = S.pl = S.rl = S.pu = S.ru = 0;
     S.mes1 = S.mes2 = S.mes3 = S.mode = S.mtyp = 0;
+    S.kdir = 0;
+    S.bdir = 0.0;
 }
 
 // NLopt result code of a finished state (LuksanPLIS.cpp:1727-1735)
@@ -355,6 +363,9 @@ PLIS_HD int PlisAdvance(PlisState &S, Ops &ops)
             S.nevals++;
             S.p = ops.dot(kVecG, kVecS);
             goto L11170;
+        case kPhDirection:
+            if constexpr (Ops::kDeferTwoLoop) S.snorm = ops.resume_two_loop(S.kdir, S.head, S.bdir);
+            goto Ldir_done;
         default: return kPlisDone;
     }
 
@@ -401,30 +412,37 @@ L11120:
     if (n > 0 && S.nit - S.kit >= K::ires1 * n + K::ires2) S.irest = S.irest > 1 ? S.irest : 1;
     ++S.nit;
 
-L11130 : {
+L11130:
     // direction determination
     S.gnorm = sqrt(ops.dot(kVecG, kVecG));
-    int iterd = 0;
-    (void)iterd;
     if (S.irest == 0) {
-        int k = S.nit - S.kit;
-        k = k < mf ? k : mf;
-        if (k <= 0) {
+        S.kdir = S.nit - S.kit;
+        if (S.kdir > mf) S.kdir = mf;
+        if (S.kdir <= 0) {
             S.irest = S.irest > 1 ? S.irest : 1;
         } else {
-            const double b = ops.dot(XO(1), GO(1));
-            if (b <= 0.) {
+            S.bdir = ops.dot(XO(1), GO(1));
+            if (S.bdir <= 0.) {
                 S.irest = S.irest > 1 ? S.irest : 1;
             } else {
-                ops.set_u(0, 1. / b);
-                // S = -H g by the two-loop recursion over the k newest pairs (the policy decides how)
-                S.snorm = ops.two_loop(k, S.head, b);
-                k = (k + 1 < mf) ? k + 1 : mf;
-                ops.shift_u(k);  // u[i] = u[i-1] for i = k-1 .. 1
-                S.head = (S.head + mf - 1) % mf;
+                ops.set_u(0, 1. / S.bdir);
+                // S = -H g by the two-loop recursion over the kdir newest pairs
+                if constexpr (Ops::kDeferTwoLoop) {
+                    S.phase = kPhDirection;
+                    return kPlisNeedDirection;
+                } else {
+                    S.snorm = ops.two_loop(S.kdir, S.head, S.bdir);
+                }
+                goto Ldir_done;
             }
         }
     }
+    goto Ldir_after;
+Ldir_done:
+    S.kdir = (S.kdir + 1 < mf) ? S.kdir + 1 : mf;
+    ops.shift_u(S.kdir);  // u[i] = u[i-1] for i = kdir-1 .. 1
+    S.head = (S.head + mf - 1) % mf;
+Ldir_after:
     if (S.irest != 0) {  // steepest descent
         ops.neg(kVecG, kVecS);
         S.snorm = S.gnorm;
@@ -460,7 +478,6 @@ L11130 : {
     ops.copy(kVecX, XO(1));
     ops.copy(kVecG, GO(1));
     if (S.rmax == 0.) goto L11120;
-}
 
 L11170:
     plis_ps1l01(S);

@@ -11,6 +11,7 @@ namespace ccgpu {
 #pragma clang fp contract(off)
 // Vector operations over host arrays in the order of luksan_mxudot__/mxudir__/... (LuksanPLIS.cpp:106-370).
 struct HostPlisOps {
+    static constexpr bool kDeferTwoLoop = false;
     int n, mf;
     double *x, *g, *s, *xo, *go;  // xo, go: mf x n
     double *uo, *vo;              // mf each
