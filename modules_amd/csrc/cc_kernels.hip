@@ -69,54 +69,93 @@ __global__ void __launch_bounds__(kBlock) k_prepare(const SysDesc *__restrict__ 
 
 // ---------------------------------------------------------------------------------------------
 // rho, lattice path.  items: (sys, segment block of kBlock segments, particle chunk)
-// One lane = one segment of up to SEG consecutive constraints; the particles of the chunk are staged in
-// LDS and read by all lanes at the same address (broadcast).
+// One lane = one segment of up to SEG consecutive constraints.  The chunk's particles are staged in LDS in
+// sub-chunks of kSub, each with its powers P_b = exp(2 pi i b s_{d-1}), b = 1..G, and read by all lanes at
+// one address (broadcast).  Per (segment, particle): one table sincos starts the segment at e_0; within a
+// group of G slots rho_{t0+b} += e_{t0} P_b (4 FMA per slot, independent), and e_{t0+G} = e_{t0} P_G
+// steps to the next group, so a slot costs (4G + 2) / G operations and the phase chain is SEG / G long.
 // ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+    return make_double2(__builtin_fma(a.x, b.x, -a.y * b.y), __builtin_fma(a.x, b.y, a.y * b.x));
+}
+
 template <int SEG>
 __global__ void __launch_bounds__(kBlock) k_rho_lattice(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
 {
-    constexpr int kMaxChunk = 512;
-    __shared__ double4 lpos[kMaxChunk];
-    __shared__ double2 lstep[kMaxChunk];
+    constexpr int G = (SEG % 8 == 0) ? 8 : 4;
+    constexpr int kSub = 128;
+    __shared__ double4 lpos[kSub];
+    __shared__ double2 lpow[kSub * G];  // particle j: P_1 .. P_G at j*G + b - 1
+    __shared__ double2 ltab[64];
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
     if (!(S.flags & kSysActive)) return;
+    sincos_table_fill(ltab);
     const int j0 = it.t1 * S.pchunk_len;
     const int j1 = min(S.N, j0 + S.pchunk_len);
-    for (int j = j0 + (int)threadIdx.x; j < j1; j += kBlock) {
-        lpos[j - j0] = S.pos[j];
-        lstep[j - j0] = S.step[j];
-    }
-    __syncthreads();
     const int seg = it.t0 * kBlock + threadIdx.x;
-    if (seg >= S.nseg) return;
-    const int4 h = S.seg_head[seg];
+    const bool valid = seg < S.nseg;
+    const int4 h = valid ? S.seg_head[seg] : make_int4(0, 0, 0, 0);
     const double h0 = (double)h.x, h1 = (double)h.y, h2 = (double)h.z, h3 = (double)h.w;
-    const int wlen = __builtin_amdgcn_readfirstlane(S.seg_len[it.t0 * kBlock + (threadIdx.x & ~63)]);
+    // segments are sorted longest first: the wavefront's first segment bounds the slots any lane needs
+    const int wseg = it.t0 * kBlock + (threadIdx.x & ~63);
+    const int wlen = __builtin_amdgcn_readfirstlane(wseg < S.nseg ? S.seg_len[wseg] : 0);
     double re[SEG], im[SEG];
 #pragma unroll
     for (int t = 0; t < SEG; t++) { re[t] = 0.0; im[t] = 0.0; }
-    for (int j = 0; j < j1 - j0; j++) {
-        const double4 p = lpos[j];
-        const double u = __builtin_fma(h0, p.x, __builtin_fma(h1, p.y, __builtin_fma(h2, p.z, h3 * p.w)));
-        double sn, cs;
-        sincos_rev(u, sn, cs);
-        const double2 st = lstep[j];
+    for (int jb = j0; jb < j1; jb += kSub) {
+        const int nsub = min(kSub, j1 - jb);
+        __syncthreads();
+        if ((int)threadIdx.x < nsub) {
+            const int j = jb + threadIdx.x;
+            lpos[threadIdx.x] = S.pos[j];
+            double2 pw[G];
+            pw[0] = S.step[j];
+            pw[1] = cmul(pw[0], pw[0]);
+            pw[2] = cmul(pw[1], pw[0]);
+            pw[3] = cmul(pw[1], pw[1]);
+            if (G == 8) {
+                pw[4] = cmul(pw[3], pw[0]);
+                pw[5] = cmul(pw[3], pw[1]);
+                pw[6] = cmul(pw[3], pw[2]);
+                pw[7] = cmul(pw[3], pw[3]);
+            }
 #pragma unroll
-        for (int t0 = 0; t0 < SEG; t0 += 4) {
-            if (t0 < wlen) {
+            for (int b = 0; b < G; b++) lpow[threadIdx.x * G + b] = pw[b];
+        }
+        __syncthreads();
+        if (!valid) continue;
+        for (int j = 0; j < nsub; j++) {
+            const double4 p = lpos[j];
+            const double u = __builtin_fma(h0, p.x, __builtin_fma(h1, p.y, __builtin_fma(h2, p.z, h3 * p.w)));
+            double sn, cs;
+            sincos_rev_tab(u, ltab, sn, cs);
+            double2 P[G];
 #pragma unroll
-                for (int t = t0; t < t0 + 4; t++) {
-                    re[t] += cs;
-                    im[t] += sn;
-                    const double nc = __builtin_fma(cs, st.x, -sn * st.y);
-                    const double ns = __builtin_fma(cs, st.y, sn * st.x);
-                    cs = nc;
-                    sn = ns;
+            for (int b = 0; b < G; b++) P[b] = lpow[j * G + b];
+#pragma unroll
+            for (int t0 = 0; t0 < SEG; t0 += G) {
+                if (t0 < wlen) {
+                    re[t0] += cs;
+                    im[t0] += sn;
+#pragma unroll
+                    for (int b = 1; b < G; b++) {
+                        if (b == 4 && t0 + 4 >= wlen) break;  // uniform: the wavefront needs no more slots
+                        re[t0 + b] = __builtin_fma(cs, P[b - 1].x, __builtin_fma(-sn, P[b - 1].y, re[t0 + b]));
+                        im[t0 + b] = __builtin_fma(cs, P[b - 1].y, __builtin_fma(sn, P[b - 1].x, im[t0 + b]));
+                    }
+                    if (t0 + G < wlen) {
+                        const double nc = __builtin_fma(cs, P[G - 1].x, -sn * P[G - 1].y);
+                        const double ns = __builtin_fma(cs, P[G - 1].y, sn * P[G - 1].x);
+                        cs = nc;
+                        sn = ns;
+                    }
                 }
             }
         }
     }
+    if (!valid) return;
     double *out = S.rpart + ((int64_t)it.t1 * S.nslots + (int64_t)seg * SEG) * 2;
 #pragma unroll
     for (int t = 0; t < SEG; t++) {
@@ -305,9 +344,12 @@ __global__ void __launch_bounds__(kBlock) k_force_lattice(const SysDesc *__restr
 {
     // PPL particles per lane (block covers PPL * kBlock particles): independent phase chains for ILP, and
     // each wavefront-uniform coefficient load serves PPL particles
+    __shared__ double2 ltab[64];
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
     if (!(S.flags & kSysActive)) return;
+    sincos_table_fill(ltab);
+    __syncthreads();
     int idx[PPL];
     bool valid[PPL];
     double4 p[PPL];
@@ -335,7 +377,7 @@ __global__ void __launch_bounds__(kBlock) k_force_lattice(const SysDesc *__restr
 #pragma unroll
         for (int q = 0; q < PPL; q++) {
             const double u = __builtin_fma(h0, p[q].x, __builtin_fma(h1, p[q].y, __builtin_fma(h2, p[q].z, h3 * p[q].w)));
-            sincos_rev(u, sn[q], cs[q]);
+            sincos_rev_tab(u, ltab, sn[q], cs[q]);
             acc[q] = 0.0;
             accl[q] = 0.0;
         }

@@ -53,6 +53,40 @@ __device__ __forceinline__ void sincos_rev(double u, double &s, double &c)
     c = ((q + 1) & 2) ? -cc : cc;
 }
 
+// Table of exp(2 pi i t / 64), t = 0..63, for sincos_rev_tab: threads 0..63 of the block each write one
+// entry (sincos_rev, a few ulp); the caller synchronises before use.
+__device__ __forceinline__ void sincos_table_fill(double2 *tab)
+{
+    if (threadIdx.x < 64) {
+        double s, c;
+        sincos_rev((double)threadIdx.x * 0.015625, s, c);
+        tab[threadIdx.x] = make_double2(c, s);
+    }
+}
+
+// sin and cos of 2 pi u through the 64-entry table: u = j/64 + f_lo with |f_lo| <= 1/128 (exact split), then
+// exp(2 pi i u) = T[j mod 64] * exp(2 pi i f_lo) with short Taylor series on |x| = |2 pi f_lo| <= pi/64:
+// sin to x^7 (next term below 5e-18), cos to x^8 (next term below 3e-20).  About 22 VALU operations and one
+// LDS read, against about 36 operations for sincos_rev.
+__device__ __forceinline__ void sincos_rev_tab(double u, const double2 *tab, double &s, double &c)
+{
+    const double f = u - __builtin_rint(u);
+    const double jd = __builtin_rint(64.0 * f);
+    const double fl = __builtin_fma(-0.015625, jd, f);  // exact (Sterbenz)
+    const double x = fl * CC_TWO_PI_HI;
+    const double x2 = x * x;
+    double ps = __builtin_fma(x2, -1.9841269841269841270e-04, 8.3333333333333333333e-03);
+    ps = __builtin_fma(ps, x2, -1.6666666666666666667e-01);
+    const double sl = __builtin_fma(ps * x2, x, x);
+    double pc = __builtin_fma(x2, 2.4801587301587301587e-05, -1.3888888888888888889e-03);
+    pc = __builtin_fma(pc, x2, 4.1666666666666666667e-02);
+    pc = __builtin_fma(pc, x2, -0.5);
+    const double cl = __builtin_fma(pc, x2, 1.0);
+    const double2 t = tab[((int)jd) & 63];
+    c = __builtin_fma(t.x, cl, -t.y * sl);
+    s = __builtin_fma(t.y, cl, t.x * sl);
+}
+
 // Radian argument: u = theta / (2 pi) formed in double-double before the reduction, so the conversion
 // adds no error beyond the ulp of theta itself.
 __device__ __forceinline__ void sincos_rad(double theta, double &s, double &c)

@@ -16,7 +16,7 @@
 
 namespace ccgpu {
 
-constexpr int kRelaxBlock = 512;
+constexpr int kRelaxBlock = 256;
 constexpr int kRelaxChunk = 1024;  // vector elements per block of the wide L-BFGS passes
 constexpr double kRelaxRescale[3] = {10000.0, 1.0, 0.01};  // StructureOptimization.cpp:1055-1063
 constexpr double kMaxEnergy = 1e10;                          // core/etc.h:8

@@ -4,6 +4,7 @@
 #include <cstring>
 
 #include "cc_relax.h"
+#include "cc_wave.h"
 
 namespace ccgpu {
 
@@ -15,7 +16,7 @@ namespace ccgpu {
 struct BlockPlisOps {
     int n, mf;
     double *x, *g, *s, *xo, *go;
-    double *lds_red;  // kRelaxBlock / 64 doubles
+    double *lds_red;  // 2 x kRelaxBlock / 64 doubles
     double *lds_u, *lds_v;
     bool gs_valid;      // g.S of the last direction is cached until g or S changes
     double gs_val;
@@ -28,25 +29,20 @@ struct BlockPlisOps {
         id -= kVecXO;
         return id < mf ? xo + (size_t)id * n : go + (size_t)(id - mf) * n;
     }
-    __device__ double wave_sum(double v) const
-    {
-        for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-        return v;
-    }
-    __device__ double wave_max(double v) const
-    {
-        for (int m = 32; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, 64));
-        return v;
-    }
+    // block-wide sum or maximum: DPP within each wavefront, then the wavefront totals in fixed order.  Two
+    // alternating LDS slots let one barrier per reduction suffice: a slot is rewritten only two reductions
+    // later, after every thread has passed the barrier of the reduction in between.
+    mutable int red_slot;
     __device__ double block_reduce(double v, bool is_max) const
     {
-        v = is_max ? wave_max(v) : wave_sum(v);
+        v = is_max ? wave_max_dpp(v) : wave_sum_dpp(v);
+        double *slot = lds_red + red_slot * (kRelaxBlock / 64);
+        red_slot ^= 1;
         const int w = threadIdx.x >> 6;
-        if ((threadIdx.x & 63) == 0) lds_red[w] = v;
+        if ((threadIdx.x & 63) == 0) slot[w] = v;
         __syncthreads();
-        double r = lds_red[0];
-        for (int i = 1; i < kRelaxBlock / 64; i++) r = is_max ? fmax(r, lds_red[i]) : r + lds_red[i];
-        __syncthreads();
+        double r = slot[0];
+        for (int i = 1; i < kRelaxBlock / 64; i++) r = is_max ? fmax(r, slot[i]) : r + slot[i];
         return r;
     }
     __device__ void touch(int id)
@@ -228,7 +224,7 @@ __device__ __forceinline__ double relax_wrap(double x)
 __global__ void __launch_bounds__(kRelaxBlock) k_relax_step(SysDesc *__restrict__ sysv, const RelaxDesc *__restrict__ rdv,
                                                            int stage)
 {
-    __shared__ double lds_red[kRelaxBlock / 64];
+    __shared__ double lds_red[2 * kRelaxBlock / 64];
     __shared__ double lds_u[kPlisMaxHistory], lds_v[kPlisMaxHistory];
     const RelaxDesc R = rdv[blockIdx.x];
     SysDesc *D = sysv + R.sys;
@@ -241,9 +237,11 @@ __global__ void __launch_bounds__(kRelaxBlock) k_relax_step(SysDesc *__restrict_
     }
     __syncthreads();
     BlockPlisOps ops{R.n, R.mf, const_cast<double *>(D->x), D->grad, R.s, R.xo, R.go, lds_red, lds_u, lds_v,
-                     false, 0.0, 0.0};
+                     false, 0.0, 0, 0.0};
     int act = kPlisNeedEval;
     bool finish = false;
+    int calls = 0;  // PlisAdvance runs from one call site (inlined once, state in registers)
+    double E = 0.0;
     if (stage == 1) {
         // |S|^2 and g.S of the direction from the pass-2 partials, in chunk order
         double ss = 0.0, gs = 0.0;
@@ -254,9 +252,9 @@ __global__ void __launch_bounds__(kRelaxBlock) k_relax_step(SysDesc *__restrict_
         ops.res_snorm = sqrt(ss);
         ops.gs_valid = true;
         ops.gs_val = gs;
-        act = PlisAdvance(st.plis, ops);
+        calls = 1;
     } else {
-        const double E = D->energy[0];
+        E = D->energy[0];
         st.E = E;
         st.device_evals++;
         if (st.boundary) {
@@ -272,15 +270,16 @@ __global__ void __launch_bounds__(kRelaxBlock) k_relax_step(SysDesc *__restrict_
                 for (int i = threadIdx.x; i < R.mf; i += kRelaxBlock) lds_u[i] = lds_v[i] = 0.0;
                 __syncthreads();
                 PlisInit(st.plis, R.n, R.mf, st.maxstep, st.fmin);
-                PlisAdvance(st.plis, ops);  // asks for the first evaluation: the one just made
                 st.boundary = 0;
-                st.plis.f = E;
-                act = PlisAdvance(st.plis, ops);
+                calls = 2;  // the first call asks for the first evaluation: the one just made
             }
         } else {
-            st.plis.f = E;
-            act = PlisAdvance(st.plis, ops);
+            calls = 1;
         }
+    }
+    for (int c = 0; c < calls; c++) {
+        if (stage == 0 && c == calls - 1) st.plis.f = E;
+        act = PlisAdvance(st.plis, ops);
     }
     if (!finish && act == kPlisDone) {
         // end of a pass: s = wrap(x / rescale) (GetConfig), then x = s * next rescale (GetParam)
@@ -324,74 +323,65 @@ __global__ void __launch_bounds__(kRelaxBlock) k_relax_step(SysDesc *__restrict_
 constexpr int kWideBlock = 256;
 constexpr int kWideChunk = kRelaxChunk;  // elements per pass block
 
-__device__ __forceinline__ double wsum64(double v)
-{
-    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-    return v;
-}
-
+// Pass 1: one block per (job, chunk of kWideChunk elements).  Each wavefront keeps the chunk's g, s_1, y_1 in
+// registers (kWideChunk / 64 elements per lane, coalesced) and takes every nw-th history pair: five dot
+// partials per pair, each summed over the wavefront with DPP and written once.
 __global__ void __launch_bounds__(kWideBlock) k_lbfgs_pass1(const SysDesc *__restrict__ sysv, const RelaxDesc *__restrict__ rdv,
                                                             const WorkItem *__restrict__ items)
 {
-    constexpr int kJ = 4;
+    constexpr int kPer = kWideChunk / 64;
     constexpr int nw = kWideBlock / 64;
-    __shared__ double lds[nw * kJ * 5];
     const WorkItem it = items[blockIdx.x];
     const RelaxDesc R = rdv[it.sys];
     const SysDesc &D = sysv[R.sys];
     if (!(D.flags & kSysActive) || R.st->plis.phase != kPhDirection) return;
     const int k = R.st->plis.kdir, head = R.st->plis.head, mf = R.mf, n = R.n;
-    const int e0 = it.t0 * kWideChunk, e1 = min(n, e0 + kWideChunk);
+    const int e0 = it.t0 * kWideChunk;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const double *g = D.grad;
     const double *s1 = R.xo + (size_t)head * n, *y1 = R.go + (size_t)head * n;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int j0 = 0; j0 < k; j0 += kJ) {
-        const double *sj[kJ], *yj[kJ];
+    double gv[kPer], s1v[kPer], y1v[kPer];
 #pragma unroll
-        for (int q = 0; q < kJ; q++) {
-            const int pj = (head + j0 + (j0 + q < k ? q : 0)) % mf;
-            sj[q] = R.xo + (size_t)pj * n;
-            yj[q] = R.go + (size_t)pj * n;
+    for (int m = 0; m < kPer; m++) {
+        const int e = e0 + m * 64 + lane;
+        const bool in = e < n;
+        gv[m] = in ? g[e] : 0.0;
+        s1v[m] = in ? s1[e] : 0.0;
+        y1v[m] = in ? y1[e] : 0.0;
+    }
+    for (int j = wave; j < k; j += nw) {
+        const size_t pj = (size_t)((head + j) % mf) * n;
+        const double *sj = R.xo + pj, *yj = R.go + pj;
+        double sv[kPer], yv[kPer];
+#pragma unroll
+        for (int m = 0; m < kPer; m++) {
+            const int e = e0 + m * 64 + lane;
+            const bool in = e < n;
+            sv[m] = in ? sj[e] : 0.0;
+            yv[m] = in ? yj[e] : 0.0;
         }
-        double a[kJ][5];
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;
 #pragma unroll
-        for (int q = 0; q < kJ; q++)
-#pragma unroll
-            for (int c = 0; c < 5; c++) a[q][c] = 0.0;
-        for (int e = e0 + threadIdx.x; e < e1; e += kWideBlock) {
-            const double gv = g[e], s1v = s1[e], y1v = y1[e];
-            double sv[kJ], yv[kJ];
-#pragma unroll
-            for (int q = 0; q < kJ; q++) {
-                sv[q] = sj[q][e];
-                yv[q] = yj[q][e];
-            }
-#pragma unroll
-            for (int q = 0; q < kJ; q++) {
-                a[q][0] = __builtin_fma(sv[q], gv, a[q][0]);
-                a[q][1] = __builtin_fma(yv[q], gv, a[q][1]);
-                a[q][2] = __builtin_fma(s1v, yv[q], a[q][2]);
-                a[q][3] = __builtin_fma(sv[q], y1v, a[q][3]);
-                a[q][4] = __builtin_fma(y1v, yv[q], a[q][4]);
-            }
+        for (int m = 0; m < kPer; m++) {
+            a0 = __builtin_fma(sv[m], gv[m], a0);
+            a1 = __builtin_fma(yv[m], gv[m], a1);
+            a2 = __builtin_fma(s1v[m], yv[m], a2);
+            a3 = __builtin_fma(sv[m], y1v[m], a3);
+            a4 = __builtin_fma(y1v[m], yv[m], a4);
         }
-#pragma unroll
-        for (int q = 0; q < kJ; q++)
-#pragma unroll
-            for (int c = 0; c < 5; c++) {
-                const double v = wsum64(a[q][c]);
-                if (lane == 0) lds[(wave * kJ + q) * 5 + c] = v;
-            }
-        __syncthreads();
-        if (threadIdx.x < kJ * 5) {
-            const int q = threadIdx.x / 5, c = threadIdx.x - 5 * q;
-            if (j0 + q < k) {
-                double t = lds[q * 5 + c];
-                for (int w = 1; w < nw; w++) t += lds[(w * kJ + q) * 5 + c];
-                R.part1[((size_t)it.t0 * kPlisMaxHistory + j0 + q) * 5 + c] = t;
-            }
+        a0 = wave_sum_dpp(a0);
+        a1 = wave_sum_dpp(a1);
+        a2 = wave_sum_dpp(a2);
+        a3 = wave_sum_dpp(a3);
+        a4 = wave_sum_dpp(a4);
+        if (lane == 0) {
+            double *out = R.part1 + ((size_t)it.t0 * kPlisMaxHistory + j) * 5;
+            out[0] = a0;
+            out[1] = a1;
+            out[2] = a2;
+            out[3] = a3;
+            out[4] = a4;
         }
-        __syncthreads();
     }
 }
 
@@ -437,36 +427,35 @@ __global__ void __launch_bounds__(kWideBlock) k_lbfgs_solve(const SysDesc *__res
         gyy_w[tri(i, j)] = gyy[pi * mf + pj];
     }
     __syncthreads();
-    // the two recursions run in wavefront 0: lane j holds the j-th term of each inner sum, the butterfly
-    // sum gives every lane the same total (fixed order), k <= 64 terms
+    // The two recursions are triangular solves; wavefront 0 runs them as column sweeps.  Lane i holds the
+    // running right-hand side of pair i; at step j lane j's coefficient is final, readlane broadcasts it and
+    // every later lane subtracts its Gram term, so each step is one FMA per lane and no cross-lane sum.
     if (wave == 0) {
-        const double a = gyy_w[0];
-        const double gamma = a > 0. ? b / a : 1.0;
-        double aj = 0.0;  // alpha of this lane's pair once known
-        for (int i = 0; i < k; i++) {  // backward loop (newest to oldest): alpha_i needs alpha_j, j < i
-            const double term = (lane < i) ? aj * gsy_w[tri(i, lane)] : 0.0;
-            const double t = -dots[5 * i] - wsum64(term);
-            const double ai = uu[i] * t;
-            if (lane == i) aj = ai;
+        const double gamma = gyy_w[0] > 0. ? b / gyy_w[0] : 1.0;
+        const bool live = lane < k;
+        const double ul = live ? uu[lane] : 0.0;
+        double r = live ? -dots[5 * lane] : 0.0;  // -s_i.g - sum_{j<i} alpha_j s_i.y_j
+        double gya = 0.0;                         // sum_j alpha_j y_i.y_j
+        double al = 0.0;                          // alpha of this lane's pair
+        for (int j = 0; j < k; j++) {             // backward loop, newest pair first
+            const double aj = readlane_d(ul * r, j);
+            if (lane == j) al = aj;
+            if (live) {
+                if (lane > j) r = __builtin_fma(-aj, gsy_w[tri(lane, j)], r);
+                gya = __builtin_fma(aj, gyy_w[lane >= j ? tri(lane, j) : tri(j, lane)], gya);
+            }
         }
-        // (G_yy alpha)_i for this lane's i
-        double gya = 0.0;
-        for (int j = 0; j < k; j++) {
-            const double alj = __shfl(aj, j, 64);
-            if (lane < k) gya = __builtin_fma(alj, gyy_w[j <= lane ? tri(lane, j) : tri(j, lane)], gya);
+        // gamma (-y_i.g - (G_yy alpha)_i) + sum_{j>i} (alpha_j - beta_j) s_j.y_i
+        double t = live ? gamma * (-dots[5 * lane + 1] - gya) : 0.0;
+        double cl = 0.0;  // alpha - beta of this lane's pair
+        for (int i = k - 1; i >= 0; i--) {  // forward loop, oldest pair first
+            const double ci = readlane_d(al - ul * t, i);
+            if (lane == i) cl = ci;
+            if (lane < i) t = __builtin_fma(ci, gsy_w[tri(i, lane)], t);
         }
-        double cj = 0.0;  // (alpha_j - beta_j) of this lane's pair once known
-        for (int i = k - 1; i >= 0; i--) {  // forward loop (oldest to newest): beta_i needs j > i
-            const double term = (lane > i && lane < k) ? cj * gsy_w[tri(lane, i)] : 0.0;
-            const double gyai = __shfl(gya, i, 64);
-            const double t = (-dots[5 * i + 1] - gyai) * gamma + wsum64(term);
-            const double bi = uu[i] * t;
-            const double ali = __shfl(aj, i, 64);
-            if (lane == i) cj = ali - bi;
-        }
-        if (lane < k) {
-            R.lcoef[lane] = aj;
-            R.lcoef[H + lane] = cj;  // coefficient of s_i
+        if (live) {
+            R.lcoef[lane] = al;
+            R.lcoef[H + lane] = cl;  // coefficient of s_i
         }
         if (lane == 0) R.lcoef[2 * H] = gamma;
     }
@@ -521,8 +510,8 @@ __global__ void __launch_bounds__(kWideBlock) k_lbfgs_pass2(const SysDesc *__res
         ss = __builtin_fma(out, out, ss);
         gs = __builtin_fma(gv, out, gs);
     }
-    ss = wsum64(ss);
-    gs = wsum64(gs);
+    ss = wave_sum_dpp(ss);
+    gs = wave_sum_dpp(gs);
     const int wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
         red[2 * wave] = ss;
@@ -613,6 +602,8 @@ void Solver::Release(int64_t id)
 
 void Solver::Admit()
 {
+    // uploads of every job admitted in this call are queued back to back and synchronised once at the end
+    std::vector<std::vector<double>> staged;
     while (!queue_.empty() && (int)resident_.size() < max_slots_ && bytes_in_use_ + queue_.front()->bytes <= budget_) {
         Job *j = queue_.front();
         queue_.pop_front();
@@ -639,20 +630,21 @@ void Solver::Admit()
             j->part2.Reserve((size_t)nch * 2);
             j->lcoef.Reserve(2 * kPlisMaxHistory + 1);
             j->st.Reserve(1);
-            std::vector<double> x0(n);
+            // x0 then the zero uv block in one host buffer
+            staged.emplace_back(n + 2 * j->mf, 0.0);
+            std::vector<double> &x0 = staged.back();
             for (int64_t i = 0; i < n; i++) x0[i] = j->s0[i] * kRelaxRescale[0];  // GetParam (:758-771)
             if (n > 0)
                 CC_HIP(hipMemcpyAsync(S.x.get(), x0.data(), n * sizeof(double), hipMemcpyHostToDevice, batch.stream));
-            RelaxState rs;
+            CC_HIP(hipMemcpyAsync(j->uv.get(), x0.data() + n, 2 * j->mf * sizeof(double), hipMemcpyHostToDevice,
+                                  batch.stream));
+            RelaxState &rs = j->host_state;
             std::memset(&rs, 0, sizeof(rs));
             rs.pass = 0;
             rs.boundary = 1;
             rs.maxstep = j->maxstep;
             rs.fmin = j->Emin;
-            j->host_state = rs;
             CC_HIP(hipMemcpyAsync(j->st.get(), &rs, sizeof(rs), hipMemcpyHostToDevice, batch.stream));
-            CC_HIP(hipMemsetAsync(j->uv.get(), 0, 2 * j->mf * sizeof(double), batch.stream));
-            CC_HIP(hipStreamSynchronize(batch.stream));
             j->t_admit = Now();
             j->state = Job::kResident;
             bytes_in_use_ += j->bytes;
@@ -665,6 +657,7 @@ void Solver::Admit()
             j->t_done = Now();
         }
     }
+    if (!staged.empty()) CC_HIP(hipStreamSynchronize(batch.stream));
 }
 
 void Solver::Rebuild()
@@ -711,7 +704,7 @@ void Solver::Rebuild()
 
 void Solver::Harvest()
 {
-    std::vector<Job *> keep;
+    std::vector<Job *> keep, fin;
     // one read of the descriptor table tells which jobs finished (kSysActive cleared) and the current
     // rescale of the others (the step kernel changes it at pass boundaries)
     std::vector<SysDesc> dev(resident_.size());
@@ -723,40 +716,49 @@ void Solver::Harvest()
     for (size_t q = 0; q < resident_.size(); q++) {
         Job *j = resident_[q];
         j->sys->rescale = dev[q].rescale;
-        if (dev[q].flags & kSysActive) {
-            keep.push_back(j);
-            continue;
+        if (dev[q].flags & kSysActive) keep.push_back(j);
+        else fin.push_back(j);
+    }
+    if (!fin.empty()) {
+        // final states and coordinates of every finished job: copies queued back to back, one synchronisation
+        std::vector<std::vector<double>> xs(fin.size());
+        for (size_t i = 0; i < fin.size(); i++) {
+            Job *j = fin[i];
+            const int64_t n = j->N * j->d;
+            CC_HIP(hipMemcpyAsync(&j->host_state, j->st.get(), sizeof(RelaxState), hipMemcpyDeviceToHost, batch.stream));
+            xs[i].resize(n);
+            if (n > 0)
+                CC_HIP(hipMemcpyAsync(xs[i].data(), j->sys->x.get(), n * sizeof(double), hipMemcpyDeviceToHost,
+                                      batch.stream));
         }
-        CC_HIP(hipMemcpyAsync(&j->host_state, j->st.get(), sizeof(RelaxState), hipMemcpyDeviceToHost, batch.stream));
         CC_HIP(hipStreamSynchronize(batch.stream));
-        const RelaxState &hs = j->host_state;
-        const int64_t n = j->N * j->d;
-        std::vector<double> x(n);
-        if (n > 0)
-            CC_HIP(hipMemcpyAsync(x.data(), j->sys->x.get(), n * sizeof(double), hipMemcpyDeviceToHost, batch.stream));
-        CC_HIP(hipStreamSynchronize(batch.stream));
-        const double r = hs.pass <= 2 ? kRelaxRescale[hs.pass] : 1.0;
-        j->s_final.resize(n);
-        for (int64_t i = 0; i < n; i++) j->s_final[i] = WrapUnit(x[i] / r);
-        j->E = hs.E;
-        j->evals = hs.evals;
-        j->device_evals = hs.device_evals;
-        j->status = hs.status;
-        j->passes = hs.pass;
-        j->t_done = Now();
-        j->state = Job::kDone;
-        j->sys.reset();
-        j->sdir.Release();
-        j->xo.Release();
-        j->go.Release();
-        j->uv.Release();
-        j->gram.Release();
-        j->part1.Release();
-        j->part2.Release();
-        j->lcoef.Release();
-        j->st.Release();
-        bytes_in_use_ -= j->bytes;
-        dirty_ = true;
+        for (size_t i = 0; i < fin.size(); i++) {
+            Job *j = fin[i];
+            const RelaxState &hs = j->host_state;
+            const int64_t n = j->N * j->d;
+            const double r = hs.pass <= 2 ? kRelaxRescale[hs.pass] : 1.0;
+            j->s_final.resize(n);
+            for (int64_t e = 0; e < n; e++) j->s_final[e] = WrapUnit(xs[i][e] / r);
+            j->E = hs.E;
+            j->evals = hs.evals;
+            j->device_evals = hs.device_evals;
+            j->status = hs.status;
+            j->passes = hs.pass;
+            j->t_done = Now();
+            j->state = Job::kDone;
+            j->sys.reset();
+            j->sdir.Release();
+            j->xo.Release();
+            j->go.Release();
+            j->uv.Release();
+            j->gram.Release();
+            j->part1.Release();
+            j->part2.Release();
+            j->lcoef.Release();
+            j->st.Release();
+            bytes_in_use_ -= j->bytes;
+            dirty_ = true;
+        }
     }
     resident_.swap(keep);
 }

@@ -21,6 +21,96 @@ void CheckHip(hipError_t e, const char *what)
 }
 
 namespace {
+struct Pool {
+    std::mutex mu;
+    std::map<std::pair<int, size_t>, std::vector<void *>> free;  // (device, class bytes) -> blocks
+    size_t cached = 0;
+};
+Pool &GetPool()
+{
+    static Pool *p = new Pool();  // never destroyed: blocks are released with the process
+    return *p;
+}
+size_t SizeClass(size_t bytes)
+{
+    if (bytes <= 256) return 256;
+    size_t p = 1;
+    while (p <= bytes / 2) p <<= 1;  // largest power of two <= bytes
+    const size_t step = p / 8;
+    return (bytes + step - 1) / step * step;
+}
+}  // namespace
+
+void *PoolAlloc(size_t bytes, size_t *cap_bytes)
+{
+    const size_t cls = SizeClass(bytes);
+    int dev = 0;
+    CC_HIP(hipGetDevice(&dev));
+    Pool &P = GetPool();
+    {
+        std::lock_guard<std::mutex> lk(P.mu);
+        auto it = P.free.find({dev, cls});
+        if (it != P.free.end() && !it->second.empty()) {
+            void *p = it->second.back();
+            it->second.pop_back();
+            P.cached -= cls;
+            *cap_bytes = cls;
+            return p;
+        }
+    }
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, cls);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        PoolTrim();
+        e = hipMalloc(&p, cls);
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(kErrMemory, "hipMalloc of " + std::to_string(cls) + " bytes failed: " + hipGetErrorString(e));
+    }
+    *cap_bytes = cls;
+    return p;
+}
+
+void PoolFree(void *p, size_t cap_bytes)
+{
+    if (!p) return;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return;
+    }
+    Pool &P = GetPool();
+    std::lock_guard<std::mutex> lk(P.mu);
+    P.free[{dev, cap_bytes}].push_back(p);
+    P.cached += cap_bytes;
+}
+
+void PoolTrim()
+{
+    Pool &P = GetPool();
+    std::lock_guard<std::mutex> lk(P.mu);
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    for (auto &kv : P.free) {
+        if (kv.second.empty()) continue;
+        (void)hipSetDevice(kv.first.first);
+        for (void *p : kv.second) (void)hipFree(p);
+        kv.second.clear();
+    }
+    (void)hipSetDevice(cur);
+    P.cached = 0;
+}
+
+size_t PoolCachedBytes()
+{
+    Pool &P = GetPool();
+    std::lock_guard<std::mutex> lk(P.mu);
+    return P.cached;
+}
+
+namespace {
 std::mutex g_tiles_mu;
 std::map<std::tuple<int, int, int>, Tiles> g_tiles;
 
@@ -68,6 +158,96 @@ Tiles DefaultTiles(int d, int64_t N, int64_t M)
     return t;
 }
 
+namespace {
+struct LayoutCache {
+    std::mutex mu;
+    std::map<uint64_t, std::vector<std::weak_ptr<const ConstraintLayout>>> m;
+};
+LayoutCache &GetLayoutCache()
+{
+    static LayoutCache *c = new LayoutCache();
+    return *c;
+}
+uint64_t Fnv(uint64_t h, const void *p, size_t n)
+{
+    const unsigned char *b = static_cast<const unsigned char *>(p);
+    for (size_t i = 0; i < n; i++) h = (h ^ b[i]) * 1099511628211ull;
+    return h;
+}
+}  // namespace
+
+std::shared_ptr<const ConstraintLayout> GetLayout(int d, const double *a, int64_t M, const double *k, const double *v,
+                                                  int S, hipStream_t stream)
+{
+    int dev = 0;
+    CC_HIP(hipGetDevice(&dev));
+    uint64_t h = 1469598103934665603ull;
+    h = Fnv(h, &d, sizeof(d));
+    h = Fnv(h, &S, sizeof(S));
+    h = Fnv(h, &dev, sizeof(dev));
+    h = Fnv(h, a, sizeof(double) * d * d);
+    h = Fnv(h, k, sizeof(double) * M * d);
+    h = Fnv(h, v, sizeof(double) * M);
+    LayoutCache &C = GetLayoutCache();
+    std::lock_guard<std::mutex> lk(C.mu);
+    auto &bucket = C.m[h];
+    for (auto it = bucket.begin(); it != bucket.end();) {
+        auto sp = it->lock();
+        if (!sp) {
+            it = bucket.erase(it);
+            continue;
+        }
+        if (sp->d == d && sp->S == S && sp->device == dev && (int64_t)sp->v.size() == M &&
+            std::equal(a, a + d * d, sp->a.begin()) && std::equal(k, k + M * d, sp->k.begin()) &&
+            std::equal(v, v + M, sp->v.begin()))
+            return sp;
+        ++it;
+    }
+    auto L = std::make_shared<ConstraintLayout>();
+    L->d = d;
+    L->S = S;
+    L->device = dev;
+    L->a.assign(a, a + d * d);
+    L->k.assign(k, k + M * d);
+    L->v.assign(v, v + M);
+    std::vector<int32_t> n;
+    L->lattice = M > 0 && LatticeIndices(d, a, M, k, n);
+    if (L->lattice) {
+        L->plan = BuildSegments(d, M, n.data(), S);
+        const int64_t nslots = L->plan.nseg * S;
+        L->seg_head.Reserve(L->plan.nseg);
+        L->seg_len.Reserve(L->plan.nseg);
+        L->slot_k.Reserve(nslots);
+        L->slot_v.Reserve(nslots);
+        std::vector<double> sv(nslots, 0.0);
+        for (int64_t q = 0; q < nslots; q++)
+            if (L->plan.slot_k[q] >= 0) sv[q] = v[L->plan.slot_k[q]];
+        CC_HIP(hipMemcpyAsync(L->seg_head.get(), L->plan.head.data(), L->plan.nseg * sizeof(int4), hipMemcpyHostToDevice, stream));
+        CC_HIP(hipMemcpyAsync(L->seg_len.get(), L->plan.len.data(), L->plan.nseg * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+        CC_HIP(hipMemcpyAsync(L->slot_k.get(), L->plan.slot_k.data(), nslots * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+        CC_HIP(hipMemcpyAsync(L->slot_v.get(), sv.data(), nslots * sizeof(double), hipMemcpyHostToDevice, stream));
+        CC_HIP(hipStreamSynchronize(stream));
+    } else {
+        L->plan = SegmentPlan();
+        L->plan.S = S;
+        L->kvec.Reserve(std::max<int64_t>(M, 1));
+        L->slot_v.Reserve(std::max<int64_t>(M, 1));
+        std::vector<double4> kv(M);
+        for (int64_t q = 0; q < M; q++) {
+            double c4[4] = {0, 0, 0, 0};
+            for (int c = 0; c < d; c++) c4[c] = k[q * d + c];
+            kv[q] = make_double4(c4[0], c4[1], c4[2], c4[3]);
+        }
+        if (M > 0) {
+            CC_HIP(hipMemcpyAsync(L->kvec.get(), kv.data(), M * sizeof(double4), hipMemcpyHostToDevice, stream));
+            CC_HIP(hipMemcpyAsync(L->slot_v.get(), v, M * sizeof(double), hipMemcpyHostToDevice, stream));
+            CC_HIP(hipStreamSynchronize(stream));
+        }
+    }
+    bucket.push_back(L);
+    return L;
+}
+
 System::System(int dim) : d(dim) {}
 
 void System::SetCell(const double *basis)
@@ -106,8 +286,7 @@ void System::AddConstraint(const double *kk, double V)
 
 size_t System::DeviceBytes() const
 {
-    return x.bytes() + pos.bytes() + cart.bytes() + step.bytes() + seg_head.bytes() + seg_len.bytes() + slot_k.bytes() +
-           slot_v.bytes() + kvec.bytes() + rpart.bytes() + rho.bytes() + coef.bytes() + epart.bytes() +
+    return x.bytes() + pos.bytes() + cart.bytes() + step.bytes() + (layout ? layout->DeviceBytes() : 0) + rpart.bytes() + rho.bytes() + coef.bytes() + epart.bytes() +
            energy.bytes() + fpart.bytes() + F.bytes() + grad.bytes() + cell_of.bytes() + cell_start.bytes() +
            cell_items.bytes() + sc_epart.bytes() + sc_F.bytes();
 }
@@ -115,6 +294,9 @@ size_t System::DeviceBytes() const
 void System::Prepare(hipStream_t stream)
 {
     if (!has_cell) throw Error(kErrState, "no configuration bound (SetConfiguration first)");
+    // buffers may be regrown below and their old blocks handed to other users through the pool: nothing of
+    // this system may still be in flight (a fresh system owns no buffers yet)
+    if (x.get()) CC_HIP(hipStreamSynchronize(stream));
     Tiles def = DefaultTiles(d, N, M);
     used.seg = tiles.seg > 0 ? tiles.seg : def.seg;
     if (used.seg != 4 && used.seg != 8 && used.seg != 16 && used.seg != 24 && used.seg != 32)
@@ -124,46 +306,13 @@ void System::Prepare(hipStream_t stream)
     used.force_ppl = tiles.force_ppl > 0 ? tiles.force_ppl : (def.force_ppl > 0 ? def.force_ppl : 2);
     if (used.force_ppl != 1 && used.force_ppl != 2) throw Error(kErrArg, "force particles per lane must be 1 or 2");
 
-    if (layout_dirty || plan.S != used.seg) {
-        lattice = M > 0 && LatticeIndices(d, a, M, k.data(), n);
-        int64_t nslots;
-        if (lattice) {
-            plan = BuildSegments(d, M, n.data(), used.seg);
-            nslots = plan.nseg * used.seg;
-            seg_head.Reserve(plan.nseg);
-            seg_len.Reserve(plan.nseg);
-            slot_k.Reserve(nslots);
-            slot_v.Reserve(nslots);
-            std::vector<double> sv(nslots, 0.0);
-            for (int64_t q = 0; q < nslots; q++)
-                if (plan.slot_k[q] >= 0) sv[q] = v[plan.slot_k[q]];
-            CC_HIP(hipMemcpyAsync(seg_head.get(), plan.head.data(), plan.nseg * sizeof(int4), hipMemcpyHostToDevice, stream));
-            CC_HIP(hipMemcpyAsync(seg_len.get(), plan.len.data(), plan.nseg * sizeof(int32_t), hipMemcpyHostToDevice, stream));
-            CC_HIP(hipMemcpyAsync(slot_k.get(), plan.slot_k.data(), nslots * sizeof(int32_t), hipMemcpyHostToDevice, stream));
-            CC_HIP(hipMemcpyAsync(slot_v.get(), sv.data(), nslots * sizeof(double), hipMemcpyHostToDevice, stream));
-            CC_HIP(hipStreamSynchronize(stream));
-        } else {
-            plan = SegmentPlan();
-            plan.S = used.seg;
-            nslots = M;
-            kvec.Reserve(std::max<int64_t>(M, 1));
-            slot_v.Reserve(std::max<int64_t>(M, 1));
-            std::vector<double4> kv(M);
-            for (int64_t q = 0; q < M; q++) {
-                double c4[4] = {0, 0, 0, 0};
-                for (int c = 0; c < d; c++) c4[c] = k[q * d + c];
-                kv[q] = make_double4(c4[0], c4[1], c4[2], c4[3]);
-            }
-            if (M > 0) {
-                CC_HIP(hipMemcpyAsync(kvec.get(), kv.data(), M * sizeof(double4), hipMemcpyHostToDevice, stream));
-                CC_HIP(hipMemcpyAsync(slot_v.get(), v.data(), M * sizeof(double), hipMemcpyHostToDevice, stream));
-                CC_HIP(hipStreamSynchronize(stream));
-            }
-        }
+    if (layout_dirty || !layout || layout->S != used.seg) {
+        layout = GetLayout(d, a, M, k.data(), v.data(), used.seg, stream);
+        lattice = layout->lattice;
         layout_dirty = false;
     }
-    const int64_t nslots = lattice ? plan.nseg * used.seg : M;
-    const int64_t nunits = lattice ? plan.nseg : M;  // force-pass loop units
+    const int64_t nslots = layout->nslots();
+    const int64_t nunits = lattice ? layout->plan.nseg : M;  // force-pass loop units
 
     // chunking of the two contractions
     pchunks = (int)std::max<int64_t>(1, (N + used.rho_chunk - 1) / used.rho_chunk);
@@ -220,7 +369,7 @@ void System::Prepare(hipStream_t stream)
     D.N = (int32_t)N;
     D.M = (int32_t)M;
     D.flags = kSysActive | (lattice ? kSysLattice : 0) | (softcore() ? kSysSoftCore : 0);
-    D.nseg = (int32_t)(lattice ? plan.nseg : 0);
+    D.nseg = (int32_t)nseg();
     D.nslots = (int32_t)nslots;
     D.pchunks = N > 0 ? pchunks : 0;  // no particles: rho sums nothing
     D.kchunks = kchunks;
@@ -232,11 +381,11 @@ void System::Prepare(hipStream_t stream)
     D.pos = pos.get();
     D.cart = cart.get();
     D.step = step.get();
-    D.seg_head = seg_head.get();
-    D.seg_len = seg_len.get();
-    D.slot_k = slot_k.get();
-    D.slot_v = slot_v.get();
-    D.kvec = kvec.get();
+    D.seg_head = layout->seg_head.get();
+    D.seg_len = layout->seg_len.get();
+    D.slot_k = layout->slot_k.get();
+    D.slot_v = layout->slot_v.get();
+    D.kvec = layout->kvec.get();
     D.rpart = rpart.get();
     D.rho = rho.get();
     D.coef = coef.get();
@@ -266,6 +415,8 @@ Batch::Batch() { CC_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking))
 
 Batch::~Batch()
 {
+    // the systems' buffers return to the pool after this: let the stream drain first
+    if (stream) (void)hipStreamSynchronize(stream);
     for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -336,7 +487,7 @@ void Batch::Build()
                 if (!s->lattice || s->used.seg != sl || s->used.rho_kernel != rk || s->used.force_ppl != ppl ||
                     s->N == 0)
                     continue;
-                const int sb = (int)((s->plan.nseg + kBlock - 1) / kBlock);
+                const int sb = (int)((s->nseg() + kBlock - 1) / kBlock);
                 for (int c = 0; c < s->pchunks; c++)
                     for (int b = 0; b < sb; b++) h_items.push_back({i, b, c, 0});
             }

@@ -6,6 +6,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -26,7 +27,17 @@ struct Error : std::runtime_error {
 void CheckHip(hipError_t e, const char *what);
 #define CC_HIP(expr) ::ccgpu::CheckHip((expr), #expr)
 
-// owning device array, grows on demand (never shrinks)
+// Device memory pool: blocks are rounded up to size classes (powers of two split in eighths, at least 256 B)
+// and kept per device on release, so that jobs admitted and retired at a high rate, and potentials rebound
+// to new shapes, reuse memory instead of calling hipMalloc / hipFree (which synchronise with the device).
+// Cached blocks stay allocated until PoolTrim or the end of the process; a failed hipMalloc trims the pool
+// and retries once.
+void *PoolAlloc(size_t bytes, size_t *cap_bytes);
+void PoolFree(void *p, size_t cap_bytes);
+void PoolTrim();
+size_t PoolCachedBytes();
+
+// owning device array from the pool, grows on demand (never shrinks)
 template <class T>
 class DevArray {
 public:
@@ -38,21 +49,17 @@ public:
     {
         if (n <= cap_) return;
         Release();
-        void *p = nullptr;
-        hipError_t e = hipMalloc(&p, (n ? n : 1) * sizeof(T));
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            throw Error(kErrMemory, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed: " +
-                                        hipGetErrorString(e));
-        }
-        p_ = static_cast<T *>(p);
-        cap_ = n;
+        size_t cap_bytes = 0;
+        p_ = static_cast<T *>(PoolAlloc((n ? n : 1) * sizeof(T), &cap_bytes));
+        cap_bytes_ = cap_bytes;
+        cap_ = cap_bytes / sizeof(T);
     }
     void Release()
     {
-        if (p_) (void)hipFree(p_);
+        if (p_) PoolFree(p_, cap_bytes_);
         p_ = nullptr;
         cap_ = 0;
+        cap_bytes_ = 0;
     }
     T *get() const { return p_; }
     size_t capacity() const { return cap_; }
@@ -61,6 +68,7 @@ public:
 private:
     T *p_ = nullptr;
     size_t cap_ = 0;
+    size_t cap_bytes_ = 0;
 };
 
 // tile shape of one system: segment length of the lattice kernels, particles per rho chunk,
@@ -86,6 +94,28 @@ void ClearTiles();
 double Autotune(int d, int64_t N, const double *basis, int64_t M, const double *k, const double *v, int batch,
                 int repeats, Tiles &best);
 
+// Device layout of one constraint set for one cell and segment length: lattice segments (head, length,
+// slot -> constraint, slot weights) or Cartesian wavevectors.  Systems with the same cell, constraints,
+// weights and segment length share one layout (GetLayout), so a batch of many seeds of one shape uploads
+// and stores its constraint tables once.
+struct ConstraintLayout {
+    int d = 0, S = 0, device = 0;
+    std::vector<double> a, k, v;  // the key: cell basis, wavevectors, weights
+    bool lattice = false;
+    SegmentPlan plan;
+    DevArray<int4> seg_head;
+    DevArray<int32_t> seg_len, slot_k;
+    DevArray<double> slot_v;
+    DevArray<double4> kvec;
+    int64_t nslots() const { return lattice ? plan.nseg * S : (int64_t)v.size(); }
+    size_t DeviceBytes() const
+    {
+        return seg_head.bytes() + seg_len.bytes() + slot_k.bytes() + slot_v.bytes() + kvec.bytes();
+    }
+};
+std::shared_ptr<const ConstraintLayout> GetLayout(int d, const double *a, int64_t M, const double *k, const double *v,
+                                                  int S, hipStream_t stream);
+
 // One periodic configuration plus its constraint set, resident in HBM.
 class System {
 public:
@@ -100,21 +130,16 @@ public:
     Tiles tiles{0, 0, 0, -1, 0};  // requested (0 / -1 = automatic)
     Tiles used;                 // resolved
 
-    // constraint layout
+    // constraint layout (shared between systems with the same cell, constraints and segment length)
     bool lattice = false;
-    std::vector<int32_t> n;
-    SegmentPlan plan;
+    std::shared_ptr<const ConstraintLayout> layout;
     bool layout_dirty = true;
+    int64_t nseg() const { return lattice && layout ? layout->plan.nseg : 0; }
 
     // device buffers
     DevArray<double> x;  // optimizer variable / relative coordinates (N*d)
     DevArray<double4> pos, cart;
     DevArray<double2> step;
-    DevArray<int4> seg_head;
-    DevArray<int32_t> seg_len;
-    DevArray<int32_t> slot_k;
-    DevArray<double> slot_v;
-    DevArray<double4> kvec;
     DevArray<double> rpart, rho, coef, epart, energy, fpart, F, grad;
     DevArray<int32_t> cell_of, cell_start, cell_items;
     DevArray<double> sc_epart, sc_F;

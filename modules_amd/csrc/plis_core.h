@@ -19,8 +19,10 @@
 
 #ifdef __HIPCC__
 #define PLIS_HD __host__ __device__
+#define PLIS_INLINE __attribute__((always_inline))
 #else
 #define PLIS_HD
+#define PLIS_INLINE __attribute__((always_inline))
 #endif
 
 namespace ccgpu {
@@ -79,7 +81,7 @@ PLIS_HD inline bool plis_relstop(double vold, double vnew, double reltol)
 }
 
 // pnint1: new trial step by extrapolation (mode 1) or interpolation (mode 2) inside [rl, ru]
-PLIS_HD inline void plis_pnint1(double rl, double ru, double fl, double fu, double pl, double pu, double &r, int mode,
+PLIS_HD PLIS_INLINE inline void plis_pnint1(double rl, double ru, double fl, double fu, double pl, double pu, double &r, int mode,
                                 int mtyp, int &merr)
 {
     merr = 0;
@@ -143,7 +145,7 @@ PLIS_HD inline void plis_pnint1(double rl, double ru, double fl, double fu, doub
 
 // ps1l01: one step of the line search.  On return isys == 1 asks for an evaluation at step r; isys == 0
 // ends the search with iters > 0 (accepted) or iters <= 0 (failed).
-PLIS_HD inline void plis_ps1l01(PlisState &S)
+PLIS_HD PLIS_INLINE inline void plis_ps1l01(PlisState &S)
 {
     typedef PlisConst K;
     bool l1 = false, l2 = false, l3 = false, l5 = false, l7 = false, m3 = false;
@@ -348,7 +350,7 @@ PLIS_HD inline int PlisResult(const PlisState &S)
 // evaluation with S.f and the gradient vector G filled in.  Returns kPlisNeedEval (x holds the next trial
 // point) or kPlisDone (x holds the final iterate, S.f its objective).
 template <class Ops>
-PLIS_HD int PlisAdvance(PlisState &S, Ops &ops)
+PLIS_HD PLIS_INLINE inline int PlisAdvance(PlisState &S, Ops &ops)
 {
     typedef PlisConst K;
     const int mf = S.mf;

@@ -71,6 +71,86 @@ def RelaxStructure_NLOPT(List, pot, Pressure=0.0, Switch=0, MinDistance=0.0, Max
     return RelaxStructure_NLOPT_Emin(List, pot, Pressure, Switch, MinDistance, MaxStep, -MaxEnergy, history)
 
 
+def _potential_terms(pot):
+    """Constraint set, shift and soft-core parameters of a potential, as the solver takes them."""
+    d = pot.Dimension
+    k = np.array([c.k[:d] for c in pot.constraints], dtype=np.float64).reshape(-1, d)
+    V = np.array([c.V for c in pot.constraints], dtype=np.float64)
+    e0, sigma = getattr(pot, "_softcore", (0.0, 0.0))
+    return k, V, float(pot.Shift), float(e0), float(sigma)
+
+
+def CollectiveCoordinateMultiRun(pConfig: Configuration, pPotential: ShiftedCCPotential, gen, Prefix="",
+                                 SampleNumber=1, TimeLimit=None, AlgorithmName="LBFGS", E_max=1e-14, max_num=10000,
+                                 onAccepted=None, max_slots=64, verbose=False):
+    """CollectiveCoordinateMultiRun (apps/CCOptimization/main.cpp:1811-1872) on the batched device solver.
+
+    Sample i starts from pConfig for i = 0 and from the next N*d draws of `gen` (a RandomGenerator) after
+    sample i-1, exactly the starts of the reference's serial loop; each is relaxed with RelaxStructure_NLOPT
+    (MaxStep = max_num), its energy is re-evaluated with pPotential and it is accepted when E < E_max.  All
+    samples are submitted at once and relax side by side.  Results are handed out in sample order, so
+    onAccepted(config, success_index) sees the reference's order.  TimeLimit (absolute time.time() seconds, as
+    the reference's time_t) is checked after each sample; reaching it ends the run without re-randomizing
+    pConfig.  Returns {"init": [...], "relaxed": [...], "success": [...], "energies": [...]}; the lists stand in
+    for the ConfigurationPack files Prefix, Prefix_InitConfig and Prefix_Success.
+    """
+    if AlgorithmName != "LBFGS":
+        raise NotImplementedError(f"CollectiveCoordinateMultiRun: only LBFGS is on this path (got {AlgorithmName})")
+    N, d = pConfig.NumParticle(), pConfig.GetDimension()
+    k, V, shift, e0, sigma = _potential_terms(pPotential)
+    starts = [pConfig.copy()]
+    pos0 = getattr(gen, "_pos", None)
+    for _ in range(1, SampleNumber):
+        starts.append(Configuration(pConfig.basis, gen.RandomDoubles(N * d).reshape(N, d)))
+    out = {"init": [], "relaxed": [], "success": [], "energies": []}
+    if SampleNumber <= 0:
+        return out
+    solver = BatchSolver(max_slots=max(1, min(max_slots, SampleNumber)))
+    ids = [solver.submit(c, k, V, shift=shift, epsilon=e0, sigma=sigma, MaxStep=max_num) for c in starts]
+    nxt = 0
+    while nxt < SampleNumber:
+        pending = solver.step(256)
+        while nxt < SampleNumber:
+            r = solver.result(ids[nxt])
+            if r["state"] == BatchSolver.FAILED:
+                raise RuntimeError(f"CollectiveCoordinateMultiRun: sample {nxt} failed")
+            if r["state"] != BatchSolver.DONE:
+                break
+            init = starts[nxt]
+            pPotential.SetConfiguration(init)
+            E_init = pPotential.Energy()
+            result = Configuration(pConfig.basis, r["relative"])
+            pPotential.SetConfiguration(result)
+            E = pPotential.Energy()
+            if verbose:
+                print(f"generating config{nxt}\n \tE_init={E_init} \n \tE_relax={E} ")
+            out["init"].append(init)
+            out["relaxed"].append(result)
+            out["energies"].append(E)
+            if E < E_max:
+                out["success"].append(result)
+                if onAccepted is not None:
+                    onAccepted(result, len(out["success"]) - 1)
+            solver.release(ids[nxt])
+            nxt += 1
+            if TimeLimit is not None and time.time() > TimeLimit:
+                # the serial loop returns here before re-randomizing: pConfig still holds this sample's start and
+                # the generator has given out only the draws of the samples started so far
+                pConfig.relative = starts[nxt - 1].relative.copy()
+                if pos0 is not None:
+                    gen._pos = pos0 + (nxt - 1) * N * d
+                return out
+        if pending == 0 and nxt < SampleNumber:
+            raise RuntimeError("CollectiveCoordinateMultiRun: solver drained with samples unfinished")
+    pConfig.relative = _wrap_rel(gen.RandomDoubles(N * d).reshape(N, d))
+    return out
+
+
+def _wrap_rel(s):
+    y = s - np.floor(s)
+    return y - np.floor(y)
+
+
 class BatchSolver:
     """Device-resident solver: jobs (RelaxStructure_NLOPT[_Emin] requests) are admitted into slots by HBM budget
     and advance together, one batched E+F launch and one L-BFGS step launch per evaluation."""

@@ -18,6 +18,7 @@ from . import _lib
 from ._lib import check, dp, ip32, lib
 
 pi = 3.1415926535897932384626433832795028841971693993751058209  # core/etc.h:9
+Potential_Check_Delta = 1e-7  # potential/Potential.cpp:5
 
 
 def _p(a):
@@ -53,6 +54,18 @@ class Configuration:
     def PeriodicVolume(self):
         return lib().ccgpu_cell_volume(self.GetDimension(), _p(np.ascontiguousarray(self.basis)))
 
+    def GetReciprocalBasisVector(self, i):
+        """GetReciprocalBasisVector (core/PeriodicCellList.h:999-1031): b_i with a_j . b_i = 2 pi delta_ij."""
+        d = self.GetDimension()
+        b = np.zeros((d, d))
+        check(lib().ccgpu_reciprocal_basis(d, _p(np.ascontiguousarray(self.basis)), _p(b)))
+        return b[i]
+
+    def CartesianCoord2RelativeCoord(self, r):
+        """core/PeriodicCellList.h:1223-1230: s_i = (r . b_i) * (0.5 / pi)."""
+        r = np.asarray(r, dtype=np.float64)
+        return np.array([np.dot(r, self.GetReciprocalBasisVector(i)) for i in range(self.GetDimension())]) * (0.5 / pi)
+
     def Insert(self, relative):
         """Insert (core/PeriodicCellList.h:517-538): wrap the relative coordinate into [0,1) twice."""
         r = _wrap(np.asarray(relative, dtype=np.float64).reshape(1, -1))
@@ -77,11 +90,47 @@ def GetUnitCubicBox(d, CellSize=0.1):
     return Configuration(np.eye(d))
 
 
+class RandomGenerator:
+    """RandomGenerator(seed) (core/RandomGenerator.h:10-30): MT19937 with RandomDouble() = u32 / 2^32
+    (core/RandomGenerator.cpp:24-27).  The draws come from the engine's MT19937 (ccgpu_random_configuration
+    with d = 1); the generator keeps its position, so successive calls continue one stream."""
+
+    def __init__(self, seed=0):
+        self.seed = int(seed) & 0xFFFFFFFF
+        self._buf = np.zeros(0)
+        self._pos = 0
+
+    def _ensure(self, n):
+        need = self._pos + n
+        if need > self._buf.size:
+            size = max(need, 2 * self._buf.size, 4096)
+            buf = np.zeros(size)
+            check(lib().ccgpu_random_configuration(self.seed, size, 1, _p(buf)))
+            self._buf = buf
+
+    def RandomDouble(self):
+        self._ensure(1)
+        v = float(self._buf[self._pos])
+        self._pos += 1
+        return v
+
+    def RandomDoubles(self, n):
+        """The next n RandomDouble() values, in draw order."""
+        self._ensure(n)
+        v = self._buf[self._pos:self._pos + n].copy()
+        self._pos += n
+        return v
+
+
 def RandomConfiguration(seed, N, d, L=None):
     """GetUnitCubicBox(d) rescaled by L (default pow(N, 1.0/d), density 1) with N particles drawn from
-    RandomGenerator(seed) (apps/CCOptimization/main.cpp:198, 207-215)."""
-    s = np.zeros((N, d))
-    check(lib().ccgpu_random_configuration(seed & 0xFFFFFFFF, N, d, _p(s)))
+    RandomGenerator(seed) (apps/CCOptimization/main.cpp:198, 207-215).  `seed` may also be a RandomGenerator,
+    whose stream then continues (particle i takes its next draws i*d .. i*d+d-1)."""
+    if isinstance(seed, RandomGenerator):
+        s = seed.RandomDoubles(N * d).reshape(N, d)
+    else:
+        s = np.zeros((N, d))
+        check(lib().ccgpu_random_configuration(seed & 0xFFFFFFFF, N, d, _p(s)))
     if L is None:
         L = math.pow(N, 1.0 / d)
     return Configuration(np.eye(d) * L, s)
@@ -229,6 +278,23 @@ class ShiftedCCPotential:
         F = np.zeros((self._config.NumParticle(), self.Dimension))
         check(lib().ccgpu_potential_energy_and_all_force(self._h, C.byref(E), _p(F)))
         return E.value, F
+
+    def Check(self, c, Nbr):
+        """Potential::Check (potential/Potential.cpp:5-19): one-sided finite difference of the energy along x
+        for particle Nbr, delta 1e-7; prints (and returns) (E0 - E1) / delta / F_x, close to 1 when Force is
+        consistent with Energy and c is not a minimum."""
+        c = c.copy()
+        self.SetConfiguration(c)
+        force = self.Force(Nbr)
+        E0 = self.Energy()
+        temp = c.GetCartesianCoordinates(Nbr).copy()
+        temp[0] += Potential_Check_Delta
+        c.MoveParticle(Nbr, c.CartesianCoord2RelativeCoord(temp))
+        self.SetConfiguration(c)
+        E1 = self.Energy()
+        ratio = (E0 - E1) / Potential_Check_Delta / force[0]
+        print(ratio)
+        return ratio
 
     def SetTiles(self, segment_len=0, rho_chunk=0, force_chunk=0):
         check(lib().ccgpu_potential_set_tiles(self._h, segment_len, rho_chunk, force_chunk))

@@ -142,3 +142,41 @@ def test_graph_replay_matches_stream_launches():
     for a, b in zip(*out):
         assert a["energy"] == b["energy"] and a["evaluations"] == b["evaluations"]
         assert np.array_equal(a["relative"], b["relative"])
+
+
+def test_multirun_starts_acceptance_and_generator_state():
+    # CollectiveCoordinateMultiRun (apps/CCOptimization/main.cpp:1811-1872): sample i starts from the i-th block
+    # of N*d draws of the generator, every relaxed sample is re-evaluated and accepted below E_max, acceptance is
+    # reported in sample order and pConfig ends on the next block of draws
+    from modules_amd.optimize import CollectiveCoordinateMultiRun
+    from modules_amd.potential import RandomGenerator
+    N, d, S = 120, 2, 5
+    gen = RandomGenerator(42)
+    cfg = RandomConfiguration(gen, N, d)
+    k = GetKs(cfg, getK(0.3, 1.0, d))
+    pot = ShiftedCCPotential(d)
+    for kk in k:
+        pot.AddConstraint(kk, [1.0])
+    seen = []
+    out = CollectiveCoordinateMultiRun(cfg, pot, gen, SampleNumber=S, E_max=1e-14, max_num=10000,
+                                       onAccepted=lambda c, i: seen.append(i))
+    stream = RandomConfiguration(42, (S + 1) * N, d).relative
+    for i, c in enumerate(out["init"]):
+        assert np.array_equal(c.relative, stream[i * N:(i + 1) * N])
+    assert np.array_equal(cfg.relative, stream[S * N:])
+    assert len(out["relaxed"]) == S
+    assert seen == list(range(len(out["success"])))
+    for c, E in zip(out["relaxed"], out["energies"]):
+        Eo, _ = po.total_energy_forces(c.relative, c.basis, k, np.ones(len(k)))
+        assert abs(E - Eo) <= 1e-12 * max(1.0, len(k) * N * N / po.volume(c.basis)) + 1e-22
+    assert len(out["success"]) == sum(1 for E in out["energies"] if E < 1e-14) >= S - 1
+
+
+def test_potential_check_ratio_is_one():
+    # Potential::Check (potential/Potential.cpp:5-19): one-sided difference, delta 1e-7
+    cfg, k, v = stealthy(300, 3, 0.3, 13)
+    pot = ShiftedCCPotential(3)
+    for kk in k:
+        pot.AddConstraint(kk, [1.0])
+    for i in (0, 123):
+        assert abs(pot.Check(cfg, i) - 1.0) < 1e-3
