@@ -37,20 +37,26 @@ def parse():
     ap.add_argument("--emin", type=float, default=1e-18)
     ap.add_argument("--maxstep", type=int, default=10000)
     ap.add_argument("--history", type=int, default=40)
-    ap.add_argument("--check-interval", type=int, default=8)
+    ap.add_argument("--check-interval", type=int, default=16)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one torch device copy of this size first (calibrates rocprof FETCH/WRITE_SIZE)")
     ap.add_argument("--graph", action="store_true", help="replay optimizer steps from a captured hipGraph")
+    ap.add_argument("--lanes", type=int, default=2, choices=[1, 2],
+                    help="solver streams: the L-BFGS kernels of one overlap the E+F kernels of the other")
     ap.add_argument("--no-autotune", action="store_true", help="skip the tile autotuner (default tiles)")
     ap.add_argument("--tiles", type=str, default=None,
                     help="seg,rho_kernel,rho_chunk,force_chunk,force_ppl: use these tiles instead of autotuning")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
+                    help="batched / mixed256: FP64, or FP32 phases and force terms with FP64 sums and an FP64 "
+                         "polish below 1e-8")
     ap.add_argument("--workload", default="batched",
-                    choices=["batched", "single", "mixed256", "large2d", "serve"],
+                    choices=["batched", "single", "mixed256", "large2d", "mixed2d", "serve"],
                     help="batched: the headline (64 seeds, 3D N=2000 chi=0.45); single: the same solves through the "
                          "single-solve drop-in path; mixed256: 256 configurations of mixed N, d, chi; large2d: 2D "
-                         "N=20000 chi=0.4; serve: Poisson request stream with soft core")
+                         "N=20000 chi=0.4; mixed2d: the same solved in FP64 and in mixed precision, with the "
+                         "mixed-precision error; serve: Poisson request stream with soft core")
     ap.add_argument("--requests", type=int, default=48, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=20.0, help="serve: arrivals per second")
     ap.add_argument("--profiler-run", action="store_true",
@@ -61,15 +67,17 @@ def parse():
     return ap.parse_args()
 
 
-def copy_probe(mib):
-    import torch
-    n = (mib << 20) // 4
-    a = torch.ones(n, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    torch.cuda.synchronize()
-    b.copy_(a)
-    torch.cuda.synchronize()
-    print(f"copy probe: {mib} MiB read + {mib} MiB written", file=sys.stderr, flush=True)
+def copy_probe(mib, repeats=1):
+    """The library's plain copy kernel (k_copy_probe) over `mib` MiB: known traffic for the memory counters of a
+    profile (mib MiB read and written per launch) and an HBM bandwidth figure.  Returns GB/s (read + write)."""
+    import ctypes as C
+    from modules_amd._lib import check, lib
+    ms = C.c_double(0)
+    check(lib().ccgpu_copy_probe(mib << 20, int(repeats), C.byref(ms)))
+    gbs = 2 * (mib << 20) / (ms.value * 1e-3) / 1e9
+    print(f"copy probe: {mib} MiB read + {mib} MiB written per launch, {ms.value:.3f} ms, {gbs:.0f} GB/s",
+          file=sys.stderr, flush=True)
+    return gbs
 
 
 def main():
@@ -104,8 +112,6 @@ def main():
         except ImportError:
             pass
 
-    if args.copy_probe_mib > 0:
-        copy_probe(args.copy_probe_mib)
     tiles = None
     if args.profile_evals > 0:
         args.profiler_run = True
@@ -123,7 +129,8 @@ def main():
     elif not args.no_autotune:
         from modules_amd import autotune
         tiles = autotune.autotune(N, d, k=k, basis=proto.basis, batch=args.batch, repeats=3)
-    solver = BatchSolver(max_slots=max(args.batch, 1), check_interval=args.check_interval, graph=args.graph)
+    solver = BatchSolver(max_slots=max(args.batch, 1), check_interval=args.check_interval, graph=args.graph,
+                         precision=args.precision, lanes=args.lanes)
 
     def run_step(cfgs, clock=None):
         """Submit a batch, admit it (inputs uploaded and resident in HBM), then solve and read the results back.
@@ -146,10 +153,14 @@ def main():
                for c in configs(0, args.batch)]
         pending = solver.step(args.profile_evals)
         sync()
+        if args.copy_probe_mib > 0:
+            copy_probe(args.copy_probe_mib)
         print(json.dumps({"profile_evals": args.profile_evals, "batch": args.batch, "N": N, "dim": d, "M": M,
                           "pending": pending, "tiles": tiles}), flush=True)
         return
 
+    if args.copy_probe_mib > 0:
+        copy_probe(args.copy_probe_mib)
     warm = [configs(1_000_000 + 1000 * w, args.batch) for w in range(args.warmup)]
     timed = [configs(1000 * s, args.batch) for s in range(args.steps)]
     for cfgs in warm:
@@ -194,12 +205,12 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32 phases / f64 sums",
         "data": "synthetic (seeded MT19937 uniform configurations, GetKs constraint set)",
         "config": {"workload": f"batched stealthy relax: {args.batch} seeds x N={N}, d={d}, chi={chi} "
                                f"(M={M}), Emin={args.emin}, 3-pass L-BFGS history {args.history}",
                    "batch": args.batch, "N": N, "dim": d, "chi": chi, "constraints": M, "tiles": tiles,
-                   "graph": bool(args.graph)},
+                   "graph": bool(args.graph), "precision": args.precision, "lanes": args.lanes},
         "solved": solved,
         "solves": len(results),
         "ef_evals_per_s": dev_evals / wall,
@@ -277,7 +288,7 @@ def workload_mixed256(args):
         d = 2 if rng.uniform() < 0.5 else 3
         chi = float(rng.uniform(0.2, 0.45))
         specs.append((N, d, chi, int(rng.randint(0, 2 ** 31 - 1))))
-    solver = BatchSolver(max_slots=256, check_interval=args.check_interval)
+    solver = BatchSolver(max_slots=256, check_interval=args.check_interval, precision=args.precision)
     jobs, pairs = [], 0
     for N, d, chi, seed in specs:
         cfg = RandomConfiguration(seed, N, d)
@@ -311,6 +322,42 @@ def workload_large2d(args):
             "ef_evals_per_s": r["device_evaluations"] / wall, "dtype": "f64"}
 
 
+def workload_mixed2d(args):
+    """2D N=20000 at density 1, chi=0.4: the mixed-precision error at the start configuration (E and forces of
+    the mixed path against the FP64 path, which the tests pin to the oracle at 1e-12), then the same solve in
+    FP64 and in mixed precision (FP32 phases and force terms, FP64 sums, FP64 polish once E < 1e-8)."""
+    from modules_amd import GetKs, RandomConfiguration, ShiftedCCPotential, getK
+    from modules_amd.optimize import BatchSolver
+    cfg = RandomConfiguration(4, 20000, 2)
+    k = GetKs(cfg, getK(0.4, 1.0, 2))
+    v = np.ones(len(k))
+    pot = ShiftedCCPotential(2)
+    for kk in k:
+        pot.AddConstraint(kk, [1.0])
+    pot.SetConfiguration(cfg)
+    E64, F64 = pot.EnergyAndAllForce()
+    pot.SetPrecision("mixed")
+    pot.SetConfiguration(cfg)
+    E32, F32 = pot.EnergyAndAllForce()
+    out = {"metric": "stealthy ground states/s at E<1e-18", "unit": "ground_states/s",
+           "workload": f"2D N=20000 chi=0.4 (M={len(k)}): mixed precision against FP64",
+           "start_config_error": {"dE_over_E": abs(E32 - E64) / abs(E64),
+                                  "max_dF_over_max_F": float(np.max(np.abs(F32 - F64)) / np.max(np.abs(F64)))}}
+    for prec in ("fp64", "mixed"):
+        solver = BatchSolver(max_slots=1, check_interval=args.check_interval, precision=prec)
+        job = solver.submit(cfg, k, v, MaxStep=args.maxstep, Emin=args.emin)
+        res, wall = _solve_all(solver, [job], args.emin)
+        r = res[0]
+        out[prec] = {"value": (1.0 if r["energy"] < args.emin else 0.0) / wall, "wall_s": wall,
+                     "energy": r["energy"], "objective_evals": r["evaluations"],
+                     "device_evals": r["device_evaluations"], "switch_eval": r["switch_eval"],
+                     "ef_evals_per_s": r["device_evaluations"] / wall}
+    out["value"] = out["mixed"]["value"]
+    out["speedup_mixed_over_fp64"] = out["mixed"]["value"] / out["fp64"]["value"] if out["fp64"]["value"] else None
+    out["dtype"] = "f32 phases / f64 sums"
+    return out
+
+
 def workload_serve(args):
     """Poisson request stream (rate --rate/s, --requests requests): N lognormal (median 1000, sigma_ln 0.6,
     clipped to [100, 50000]), d in {2,3}, chi U[0.1, 0.49], soft core e0 (1 - r/sigma)^2 with e0 = 1 and
@@ -328,7 +375,7 @@ def workload_serve(args):
 
 
 OTHER_WORKLOADS = {"single": workload_single, "mixed256": workload_mixed256, "large2d": workload_large2d,
-                   "serve": workload_serve}
+                   "mixed2d": workload_mixed2d, "serve": workload_serve}
 
 
 def cpu_baseline(cfg, k, v, evals_per_gs, seconds):

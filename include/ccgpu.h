@@ -105,6 +105,13 @@ int ccgpu_potential_set_tiles(ccgpu_potential *p, int segment_len, int rho_chunk
  * -1 automatic.  No counterpart in the reference (autotuner hook). */
 int ccgpu_potential_set_rho_kernel(ccgpu_potential *p, int variant);
 
+/* Arithmetic of the lattice contractions: 0 FP64 (default); 1 mixed -- each phase is reduced to [-1/2, 1/2]
+ * revolutions in FP64, then the sincos, the phase steps and the force terms run in FP32 and are summed in
+ * FP64 (rho over runs of 32 particles, forces per segment).  The reference computes in FP64 only
+ * (CollectiveCoordinatePotential.h:1255-1394); the mixed mode has no counterpart and its error against FP64
+ * is reported by the caller.  The general (non-lattice) path is FP64 in either mode. */
+int ccgpu_potential_set_precision(ccgpu_potential *p, int mode);
+
 /* Which kernel family the bound constraint set uses: 1 lattice (segment recurrence), 0 general. */
 int ccgpu_potential_uses_lattice_path(const ccgpu_potential *p, int *lattice);
 
@@ -122,6 +129,11 @@ int ccgpu_autotune_clear(void);
 /* Record a tile shape for the (dim, N, M) shape class without timing (e.g. one found by an earlier run). */
 int ccgpu_autotune_record(int dim, int64_t N, int64_t M, int segment_len, int rho_kernel, int rho_chunk,
                           int force_chunk, int force_ppl);
+
+/* Device copy of `bytes` (rounded down to 32-byte elements) between two fresh buffers, `repeats` times on a
+ * stream of its own; *ms_per_copy: hipEvent milliseconds per copy.  No counterpart in the reference: the
+ * known-traffic kernel that scales the memory counters of a profile, and an HBM bandwidth probe. */
+int ccgpu_copy_probe(int64_t bytes, int repeats, double *ms_per_copy);
 
 /* ---------------------------------------------------------------------------------------------
  * L-BFGS (Luksan's PLIS, the algorithm of NLopt's NLOPT_LD_LBFGS) on the host
@@ -168,8 +180,19 @@ int ccgpu_solver_submit(ccgpu_solver *s, int dim, const double *basis, int64_t N
 int ccgpu_solver_step(ccgpu_solver *s, int evaluations, int *pending);
 /* Evaluations between two host checks for finished jobs (default 8). */
 int ccgpu_solver_set_check_interval(ccgpu_solver *s, int evaluations);
+/* Streams the resident jobs are spread over (1 or 2, default 2; only while no job is resident).  Each stream
+ * runs its jobs' E+F and L-BFGS kernels, so the latency-bound L-BFGS kernels of one stream overlap the E+F
+ * kernels of the other.  Results do not depend on the lane count.  No counterpart in the reference. */
+int ccgpu_solver_set_lanes(ccgpu_solver *s, int lanes);
 /* Replay the steps between two checks as one captured hipGraph (1) or launch them on the stream (0, default). */
 int ccgpu_solver_set_graph(ccgpu_solver *s, int on);
+/* Precision of jobs submitted after the call (no counterpart in the reference): mode 0 FP64; mode 1 mixed
+ * (as ccgpu_potential_set_precision) until the energy first falls below polish_below or the first L-BFGS pass
+ * ends, then FP64 for the rest of the solve: the configuration at the switch is evaluated again in FP64 and
+ * the pass restarts from it, so the final energy and every stop test are FP64. */
+int ccgpu_solver_set_precision(ccgpu_solver *s, int mode, double polish_below);
+/* Device evaluation at which a mixed-precision job switched to FP64 (0: FP64 throughout). */
+int ccgpu_solver_switch_eval(ccgpu_solver *s, int64_t job, int *evaluation);
 /* State of a job: 0 queued, 1 resident, 2 done, 3 failed.  When done: final energy, objective evaluations,
  * E+F evaluations on the device, NLopt code of the last pass, and times (submit, admit, done; seconds on the
  * solver clock) in t[3]; relative (N x dim, may be NULL) receives the relaxed configuration. */

@@ -54,6 +54,8 @@ SIGNATURES = {
     "ccgpu_potential_set_tiles": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
     "ccgpu_potential_uses_lattice_path": (C.c_int, [vp, C.POINTER(C.c_int)]),
     "ccgpu_potential_set_rho_kernel": (C.c_int, [vp, C.c_int]),
+    "ccgpu_potential_set_precision": (C.c_int, [vp, C.c_int]),
+    "ccgpu_copy_probe": (C.c_int, [C.c_int64, C.c_int, dp]),
     "ccgpu_lbfgs_minimize": (C.c_int, [C.c_int, vp, vp, dp, dp, C.c_double, C.c_int, C.c_int,
                                        C.POINTER(C.c_int)]),
     "ccgpu_potential_relax": (C.c_int, [vp, dp, C.c_int, C.c_double, C.c_int, dp, C.POINTER(C.c_int),
@@ -65,6 +67,9 @@ SIGNATURES = {
     "ccgpu_solver_step": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int)]),
     "ccgpu_solver_set_check_interval": (C.c_int, [vp, C.c_int]),
     "ccgpu_solver_set_graph": (C.c_int, [vp, C.c_int]),
+    "ccgpu_solver_set_lanes": (C.c_int, [vp, C.c_int]),
+    "ccgpu_solver_set_precision": (C.c_int, [vp, C.c_int, C.c_double]),
+    "ccgpu_solver_switch_eval": (C.c_int, [vp, C.c_int64, C.POINTER(C.c_int)]),
     "ccgpu_solver_result": (C.c_int, [vp, C.c_int64, C.POINTER(C.c_int), dp, C.POINTER(C.c_int),
                                       C.POINTER(C.c_int), C.POINTER(C.c_int), dp, dp]),
     "ccgpu_solver_release": (C.c_int, [vp, C.c_int64]),

@@ -402,6 +402,42 @@ int ccgpu_autotune(int dim, int64_t N, const double *basis, int64_t M, const dou
     });
 }
 
+int ccgpu_copy_probe(int64_t bytes, int repeats, double *ms_per_copy)
+{
+    return Guard([&] {
+        Require(bytes >= 32 && repeats >= 1 && ms_per_copy != nullptr, CCGPU_ERR_ARG, "bad argument");
+        const int64_t n = bytes / 32;
+        DevArray<double4> a, b;
+        a.Reserve(n);
+        b.Reserve(n);
+        hipStream_t st = nullptr;
+        CC_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        try {
+            CC_HIP(hipEventCreate(&e0));
+            CC_HIP(hipEventCreate(&e1));
+            launch_copy_probe(st, b.get(), a.get(), n);  // first touch
+            CC_HIP(hipEventRecord(e0, st));
+            for (int r = 0; r < repeats; r++) launch_copy_probe(st, a.get(), b.get(), n);
+            CC_HIP(hipEventRecord(e1, st));
+            CC_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            CC_HIP(hipEventElapsedTime(&ms, e0, e1));
+            *ms_per_copy = ms / repeats;
+        } catch (...) {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            (void)hipStreamSynchronize(st);
+            (void)hipStreamDestroy(st);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        CC_HIP(hipStreamSynchronize(st));
+        CC_HIP(hipStreamDestroy(st));
+    });
+}
+
 int ccgpu_autotune_clear(void)
 {
     return Guard([&] { ClearTiles(); });
@@ -435,6 +471,18 @@ int ccgpu_potential_set_rho_kernel(ccgpu_potential *p, int variant)
         p->sys.tiles.rho_kernel = variant;
         p->batch.dirty = true;
         p->Invalidate();
+    });
+}
+
+int ccgpu_potential_set_precision(ccgpu_potential *p, int mode)
+{
+    return Guard([&] {
+        Require(p != nullptr && (mode == 0 || mode == 1), CCGPU_ERR_ARG, "precision mode must be 0 (FP64) or 1 (mixed)");
+        if (p->sys.precision != mode) {
+            p->sys.precision = mode;
+            p->batch.dirty = true;
+            p->Invalidate();
+        }
     });
 }
 
@@ -493,6 +541,8 @@ int ccgpu_solver_submit(ccgpu_solver *s, int dim, const double *basis, int64_t N
         j->maxstep = maxstep;
         j->Emin = Emin;
         j->mf = history;
+        j->precision = s->solver.precision();
+        j->polish = s->solver.polish();
         *job = s->solver.Submit(std::move(j));
     });
 }
@@ -511,6 +561,33 @@ int ccgpu_solver_set_check_interval(ccgpu_solver *s, int evaluations)
     return Guard([&] {
         Require(s != nullptr && evaluations >= 1, CCGPU_ERR_ARG, "bad argument");
         s->solver.SetCheckInterval(evaluations);
+    });
+}
+
+int ccgpu_solver_set_precision(ccgpu_solver *s, int mode, double polish_below)
+{
+    return Guard([&] {
+        Require(s != nullptr && (mode == 0 || mode == 1), CCGPU_ERR_ARG, "precision mode must be 0 (FP64) or 1 (mixed)");
+        Require(std::isfinite(polish_below) && polish_below >= 0, CCGPU_ERR_ARG, "polish threshold must be >= 0");
+        s->solver.SetPrecision(mode, polish_below);
+    });
+}
+
+int ccgpu_solver_switch_eval(ccgpu_solver *s, int64_t job, int *evaluation)
+{
+    return Guard([&] {
+        Require(s != nullptr && evaluation != nullptr, CCGPU_ERR_ARG, "bad argument");
+        Job *j = s->solver.Find(job);
+        Require(j != nullptr, CCGPU_ERR_ARG, "unknown job");
+        *evaluation = j->switch_eval;
+    });
+}
+
+int ccgpu_solver_set_lanes(ccgpu_solver *s, int lanes)
+{
+    return Guard([&] {
+        Require(s != nullptr, CCGPU_ERR_ARG, "null handle");
+        s->solver.SetLanes(lanes);
     });
 }
 
@@ -567,7 +644,7 @@ int ccgpu_solver_set_timing(ccgpu_solver *s, int on)
 {
     return Guard([&] {
         Require(s != nullptr, CCGPU_ERR_ARG, "null handle");
-        s->solver.batch.timing = on != 0;
+        s->solver.SetTiming(on != 0);
     });
 }
 
@@ -576,17 +653,13 @@ int ccgpu_solver_timing(ccgpu_solver *s, double *rho_ms, double *force_ms, int64
 {
     return Guard([&] {
         Require(s != nullptr, CCGPU_ERR_ARG, "null handle");
-        Batch &b = s->solver.batch;
-        b.Sync();
-        b.CollectTiming();
-        if (rho_ms) *rho_ms = b.t_rho_ms;
-        if (force_ms) *force_ms = b.t_force_ms;
-        if (rho_launches) *rho_launches = b.n_rho;
-        if (force_launches) *force_launches = b.n_force;
-        if (reset) {
-            b.t_rho_ms = b.t_force_ms = 0;
-            b.n_rho = b.n_force = 0;
-        }
+        double r = 0, f = 0;
+        int64_t nr = 0, nf = 0;
+        s->solver.Timing(r, f, nr, nf, reset != 0);
+        if (rho_ms) *rho_ms = r;
+        if (force_ms) *force_ms = f;
+        if (rho_launches) *rho_launches = nr;
+        if (force_launches) *force_launches = nf;
     });
 }
 

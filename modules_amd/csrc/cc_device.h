@@ -9,12 +9,14 @@
 
 namespace ccgpu {
 
-constexpr int kBlock = 256;  // threads per work item in every CC kernel (4 wavefronts of 64)
+constexpr int kBlock = 256;    // threads per work item of the CC kernels (4 wavefronts of 64) ...
+constexpr int kRhoBlock = 64;  // ... except the grouped-power rho kernels: one wavefront of 64 segments
 
 enum SysFlags : int32_t {
     kSysActive = 1,    // evaluate this system
     kSysLattice = 2,   // constraints are reciprocal lattice vectors: segment kernels (phase recurrence)
     kSysSoftCore = 4,  // add the soft-core pair term e0 (1 - r/sigma)^2
+    kSysMixed = 8,     // lattice phases and force terms in FP32 (FP64 range reduction and accumulation)
 };
 
 struct SysDesc {
@@ -45,6 +47,7 @@ struct SysDesc {
     double *rpart;   // pchunks x nslots x 2 partial sums
     double *rho;     // 2 x M: RhoReal then RhoImag in constraint order
     double *coef;    // nslots x 2: (2 V/vol) * (Im rho, Re rho)
+    float *coef32;   // nslots x 2: the same rounded to FP32 (force terms of the mixed-precision path)
     double *epart;   // per reduce block: sum V |rho|^2
     double *energy;  // [0] total, [1] CC part, [2] soft-core part
 

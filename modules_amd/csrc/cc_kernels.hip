@@ -15,6 +15,8 @@
 //   general  -- arbitrary Cartesian k: one sincos of k.r per pair.
 // Every sum has a fixed order (chunk partials reduced in chunk order, block trees), so results are
 // bitwise repeatable and do not depend on which other systems share the launch.
+#include <type_traits>
+
 #include "cc_device.h"
 #include "cc_math.h"
 
@@ -68,7 +70,7 @@ __global__ void __launch_bounds__(kBlock) k_prepare(const SysDesc *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------------------------
-// rho, lattice path.  items: (sys, segment block of kBlock segments, particle chunk)
+// rho, lattice path.  items: (sys, segment block of kRhoBlock segments, particle chunk)
 // One lane = one segment of up to SEG consecutive constraints.  The chunk's particles are staged in LDS in
 // sub-chunks of kSub, each with its powers P_b = exp(2 pi i b s_{d-1}), b = 1..G, and read by all lanes at
 // one address (broadcast).  Per (segment, particle): one table sincos starts the segment at e_0; within a
@@ -81,35 +83,35 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b)
 }
 
 template <int SEG>
-__global__ void __launch_bounds__(kBlock) k_rho_lattice(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
+struct RhoLatticeLds {
+    static constexpr int G = (SEG % 8 == 0) ? 8 : 4;
+    static constexpr int kSub = 64;
+    double4 pos[kSub];
+    double2 pow[kSub * G];  // particle j: P_1 .. P_G at j*G + b - 1 (float2 in the first half when mixed)
+    double2 tab[64];
+    float2 tab32[64];
+};
+
+// The particle loop of k_rho_lattice.  MIXED: the phase of each (segment, particle) is reduced to [-1/2, 1/2]
+// revolutions in FP64, then the table sincos, the group powers and the per-slot products run in FP32 and are
+// summed in FP32 over runs of kFlush particles; each run's sums are added to the FP64 accumulators.
+template <int SEG, bool MIXED>
+__device__ __forceinline__ void rho_lattice_body(const SysDesc &S, const WorkItem &it, RhoLatticeLds<SEG> &L,
+                                                 double (&re)[SEG], double (&im)[SEG], bool valid, int wlen,
+                                                 double h0, double h1, double h2, double h3)
 {
-    constexpr int G = (SEG % 8 == 0) ? 8 : 4;
-    constexpr int kSub = 128;
-    __shared__ double4 lpos[kSub];
-    __shared__ double2 lpow[kSub * G];  // particle j: P_1 .. P_G at j*G + b - 1
-    __shared__ double2 ltab[64];
-    const WorkItem it = items[blockIdx.x];
-    const SysDesc &S = sys[it.sys];
-    if (!(S.flags & kSysActive)) return;
-    sincos_table_fill(ltab);
+    constexpr int G = RhoLatticeLds<SEG>::G;
+    constexpr int kSub = RhoLatticeLds<SEG>::kSub;
+    constexpr int kFlush = 32;
+    float2 *pow32 = reinterpret_cast<float2 *>(L.pow);
     const int j0 = it.t1 * S.pchunk_len;
     const int j1 = min(S.N, j0 + S.pchunk_len);
-    const int seg = it.t0 * kBlock + threadIdx.x;
-    const bool valid = seg < S.nseg;
-    const int4 h = valid ? S.seg_head[seg] : make_int4(0, 0, 0, 0);
-    const double h0 = (double)h.x, h1 = (double)h.y, h2 = (double)h.z, h3 = (double)h.w;
-    // segments are sorted longest first: the wavefront's first segment bounds the slots any lane needs
-    const int wseg = it.t0 * kBlock + (threadIdx.x & ~63);
-    const int wlen = __builtin_amdgcn_readfirstlane(wseg < S.nseg ? S.seg_len[wseg] : 0);
-    double re[SEG], im[SEG];
-#pragma unroll
-    for (int t = 0; t < SEG; t++) { re[t] = 0.0; im[t] = 0.0; }
     for (int jb = j0; jb < j1; jb += kSub) {
         const int nsub = min(kSub, j1 - jb);
         __syncthreads();
-        if ((int)threadIdx.x < nsub) {
-            const int j = jb + threadIdx.x;
-            lpos[threadIdx.x] = S.pos[j];
+        for (int q = threadIdx.x; q < nsub; q += kRhoBlock) {
+            const int j = jb + q;
+            L.pos[q] = S.pos[j];
             double2 pw[G];
             pw[0] = S.step[j];
             pw[1] = cmul(pw[0], pw[0]);
@@ -122,39 +124,112 @@ __global__ void __launch_bounds__(kBlock) k_rho_lattice(const SysDesc *__restric
                 pw[7] = cmul(pw[3], pw[3]);
             }
 #pragma unroll
-            for (int b = 0; b < G; b++) lpow[threadIdx.x * G + b] = pw[b];
+            for (int b = 0; b < G; b++) {
+                if (MIXED) pow32[q * G + b] = make_float2((float)pw[b].x, (float)pw[b].y);
+                else L.pow[q * G + b] = pw[b];
+            }
         }
         __syncthreads();
         if (!valid) continue;
-        for (int j = 0; j < nsub; j++) {
-            const double4 p = lpos[j];
-            const double u = __builtin_fma(h0, p.x, __builtin_fma(h1, p.y, __builtin_fma(h2, p.z, h3 * p.w)));
-            double sn, cs;
-            sincos_rev_tab(u, ltab, sn, cs);
-            double2 P[G];
+        if (!MIXED) {
+            for (int j = 0; j < nsub; j++) {
+                const double4 p = L.pos[j];
+                const double u = __builtin_fma(h0, p.x, __builtin_fma(h1, p.y, __builtin_fma(h2, p.z, h3 * p.w)));
+                double sn, cs;
+                sincos_rev_tab(u, L.tab, sn, cs);
+                double2 P[G];
 #pragma unroll
-            for (int b = 0; b < G; b++) P[b] = lpow[j * G + b];
+                for (int b = 0; b < G; b++) P[b] = L.pow[j * G + b];
 #pragma unroll
-            for (int t0 = 0; t0 < SEG; t0 += G) {
-                if (t0 < wlen) {
-                    re[t0] += cs;
-                    im[t0] += sn;
+                for (int t0 = 0; t0 < SEG; t0 += G) {
+                    if (t0 < wlen) {
+                        re[t0] += cs;
+                        im[t0] += sn;
 #pragma unroll
-                    for (int b = 1; b < G; b++) {
-                        if (b == 4 && t0 + 4 >= wlen) break;  // uniform: the wavefront needs no more slots
-                        re[t0 + b] = __builtin_fma(cs, P[b - 1].x, __builtin_fma(-sn, P[b - 1].y, re[t0 + b]));
-                        im[t0 + b] = __builtin_fma(cs, P[b - 1].y, __builtin_fma(sn, P[b - 1].x, im[t0 + b]));
+                        for (int b = 1; b < G; b++) {
+                            if (b == 4 && t0 + 4 >= wlen) break;  // uniform: the wavefront needs no more slots
+                            re[t0 + b] = __builtin_fma(cs, P[b - 1].x, __builtin_fma(-sn, P[b - 1].y, re[t0 + b]));
+                            im[t0 + b] = __builtin_fma(cs, P[b - 1].y, __builtin_fma(sn, P[b - 1].x, im[t0 + b]));
+                        }
+                        if (t0 + G < wlen) {
+                            const double nc = __builtin_fma(cs, P[G - 1].x, -sn * P[G - 1].y);
+                            const double ns = __builtin_fma(cs, P[G - 1].y, sn * P[G - 1].x);
+                            cs = nc;
+                            sn = ns;
+                        }
                     }
-                    if (t0 + G < wlen) {
-                        const double nc = __builtin_fma(cs, P[G - 1].x, -sn * P[G - 1].y);
-                        const double ns = __builtin_fma(cs, P[G - 1].y, sn * P[G - 1].x);
-                        cs = nc;
-                        sn = ns;
+                }
+            }
+        } else {
+            for (int jf = 0; jf < nsub; jf += kFlush) {
+                float r32[SEG], i32[SEG];
+#pragma unroll
+                for (int t = 0; t < SEG; t++) { r32[t] = 0.0f; i32[t] = 0.0f; }
+                const int je = min(nsub, jf + kFlush);
+                for (int j = jf; j < je; j++) {
+                    const double4 p = L.pos[j];
+                    const double u = __builtin_fma(h0, p.x, __builtin_fma(h1, p.y, __builtin_fma(h2, p.z, h3 * p.w)));
+                    float sn, cs;
+                    sincos_rev_tab32((float)(u - __builtin_rint(u)), L.tab32, sn, cs);
+                    float2 P[G];
+#pragma unroll
+                    for (int b = 0; b < G; b++) P[b] = pow32[j * G + b];
+#pragma unroll
+                    for (int t0 = 0; t0 < SEG; t0 += G) {
+                        if (t0 < wlen) {
+                            r32[t0] += cs;
+                            i32[t0] += sn;
+#pragma unroll
+                            for (int b = 1; b < G; b++) {
+                                if (b == 4 && t0 + 4 >= wlen) break;
+                                r32[t0 + b] = __builtin_fmaf(cs, P[b - 1].x, __builtin_fmaf(-sn, P[b - 1].y, r32[t0 + b]));
+                                i32[t0 + b] = __builtin_fmaf(cs, P[b - 1].y, __builtin_fmaf(sn, P[b - 1].x, i32[t0 + b]));
+                            }
+                            if (t0 + G < wlen) {
+                                const float nc = __builtin_fmaf(cs, P[G - 1].x, -sn * P[G - 1].y);
+                                const float ns = __builtin_fmaf(cs, P[G - 1].y, sn * P[G - 1].x);
+                                cs = nc;
+                                sn = ns;
+                            }
+                        }
                     }
+                }
+#pragma unroll
+                for (int t = 0; t < SEG; t++) {
+                    re[t] += (double)r32[t];
+                    im[t] += (double)i32[t];
                 }
             }
         }
     }
+}
+
+// k_rho_lattice serves the FP64 systems of a launch and k_rho_lattice_mixed the mixed-precision ones (each
+// block returns at once for a system of the other kind): two kernels keep the register budget of the FP64
+// body free of the FP32 partial sums.
+template <int SEG, bool MIXED>
+__device__ __forceinline__ void rho_lattice_kernel(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items,
+                                                   RhoLatticeLds<SEG> &L)
+{
+    const WorkItem it = items[blockIdx.x];
+    const SysDesc &S = sys[it.sys];
+    if (!(S.flags & kSysActive) || ((S.flags & kSysMixed) != 0) != MIXED) return;
+    sincos_table_fill(L.tab);
+    if (MIXED) {
+        __syncthreads();
+        sincos_table_fill32(L.tab32, L.tab);
+    }
+    const int seg = it.t0 * kRhoBlock + threadIdx.x;
+    const bool valid = seg < S.nseg;
+    const int4 h = valid ? S.seg_head[seg] : make_int4(0, 0, 0, 0);
+    const double h0 = (double)h.x, h1 = (double)h.y, h2 = (double)h.z, h3 = (double)h.w;
+    // segments are sorted longest first: the block's (wavefront's) first segment bounds the slots any lane needs
+    const int wseg = it.t0 * kRhoBlock;
+    const int wlen = __builtin_amdgcn_readfirstlane(wseg < S.nseg ? S.seg_len[wseg] : 0);
+    double re[SEG], im[SEG];
+#pragma unroll
+    for (int t = 0; t < SEG; t++) { re[t] = 0.0; im[t] = 0.0; }
+    rho_lattice_body<SEG, MIXED>(S, it, L, re, im, valid, wlen, h0, h1, h2, h3);
     if (!valid) return;
     double *out = S.rpart + ((int64_t)it.t1 * S.nslots + (int64_t)seg * SEG) * 2;
 #pragma unroll
@@ -162,6 +237,21 @@ __global__ void __launch_bounds__(kBlock) k_rho_lattice(const SysDesc *__restric
         out[2 * t] = re[t];
         out[2 * t + 1] = im[t];
     }
+}
+
+template <int SEG>
+__global__ void __launch_bounds__(kRhoBlock) k_rho_lattice(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
+{
+    __shared__ RhoLatticeLds<SEG> L;
+    rho_lattice_kernel<SEG, false>(sys, items, L);
+}
+
+template <int SEG>
+__global__ void __launch_bounds__(kRhoBlock) k_rho_lattice_mixed(const SysDesc *__restrict__ sys,
+                                                              const WorkItem *__restrict__ items)
+{
+    __shared__ RhoLatticeLds<SEG> L;
+    rho_lattice_kernel<SEG, true>(sys, items, L);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -300,8 +390,11 @@ __global__ void __launch_bounds__(kBlock) k_rho_reduce(const SysDesc *__restrict
             e = v * (re * re + im * im);
         }
         const double w = (2.0 * v) * S.inv_vol;
-        S.coef[2 * q] = kidx >= 0 ? w * im : 0.0;
-        S.coef[2 * q + 1] = kidx >= 0 ? w * re : 0.0;
+        const double ca = kidx >= 0 ? w * im : 0.0, cb = kidx >= 0 ? w * re : 0.0;
+        S.coef[2 * q] = ca;
+        S.coef[2 * q + 1] = cb;
+        S.coef32[2 * q] = (float)ca;
+        S.coef32[2 * q + 1] = (float)cb;
     }
     const double tot = block_sum(e, lds);
     if (threadIdx.x == 0) S.epart[it.t0] = tot;
@@ -339,16 +432,109 @@ __global__ void __launch_bounds__(kBlock) k_energy_final(const SysDesc *__restri
 // segment data is uniform across the wavefront (scalar loads).  Accumulates G_j = sum_k c_k n_kj with
 // c_k = (2V/vol)(Im rho cos - Re rho sin); then F = -sum_j G_j b_j in k_force_final.
 // ---------------------------------------------------------------------------------------------
+template <int SEG, int PPL, bool MIXED>
+__device__ __forceinline__ void force_lattice_body(const SysDesc &S, const WorkItem &it, const double2 *ltab,
+                                                   const float2 *ltab32, const double4 (&p)[PPL],
+                                                   const double2 (&st)[PPL], double (&g)[PPL][4])
+{
+    using Real = typename std::conditional<MIXED, float, double>::type;
+    const int s0 = it.t1 * S.kchunk_len;
+    const int s1 = min(S.nseg, s0 + S.kchunk_len);
+    const int d = S.d;
+    Real stx[PPL], sty[PPL];
+#pragma unroll
+    for (int q = 0; q < PPL; q++) {
+        stx[q] = (Real)st[q].x;
+        sty[q] = (Real)st[q].y;
+    }
+    for (int sg = s0; sg < s1; sg++) {
+        const int4 h = S.seg_head[sg];
+        const int len = S.seg_len[sg];
+        const double h0 = (double)h.x, h1 = (double)h.y, h2 = (double)h.z, h3 = (double)h.w;
+        const double hl = (d == 1) ? h0 : (d == 2) ? h1 : (d == 3) ? h2 : h3;
+        Real sn[PPL], cs[PPL], acc[PPL], accl[PPL];
+#pragma unroll
+        for (int q = 0; q < PPL; q++) {
+            const double u = __builtin_fma(h0, p[q].x, __builtin_fma(h1, p[q].y, __builtin_fma(h2, p[q].z, h3 * p[q].w)));
+            if (MIXED) {
+                float fs, fc;
+                sincos_rev_tab32((float)(u - __builtin_rint(u)), ltab32, fs, fc);
+                sn[q] = fs;
+                cs[q] = fc;
+            } else {
+                double ds, dc;
+                sincos_rev_tab(u, ltab, ds, dc);
+                sn[q] = ds;
+                cs[q] = dc;
+            }
+            acc[q] = 0;
+            accl[q] = 0;
+        }
+#pragma unroll
+        for (int t0 = 0; t0 < SEG; t0 += 4) {
+            if (t0 < len) {  // segment data is uniform across the wavefront
+#pragma unroll
+                for (int t = t0; t < t0 + 4; t++) {
+                    Real ax, ay;
+                    if (MIXED) {
+                        const float2 ab = reinterpret_cast<const float2 *>(S.coef32)[(int64_t)sg * SEG + t];
+                        ax = ab.x;
+                        ay = ab.y;
+                    } else {
+                        const double2 ab = reinterpret_cast<const double2 *>(S.coef)[(int64_t)sg * SEG + t];
+                        ax = ab.x;
+                        ay = ab.y;
+                    }
+#pragma unroll
+                    for (int q = 0; q < PPL; q++) {
+                        const Real c = ax * cs[q] - ay * sn[q];
+                        acc[q] += c;
+                        accl[q] += (Real)t * c;  // t is a compile-time constant: one FMA
+                        const Real nc = cs[q] * stx[q] - sn[q] * sty[q];
+                        const Real ns = cs[q] * sty[q] + sn[q] * stx[q];
+                        cs[q] = nc;
+                        sn[q] = ns;
+                    }
+                }
+            }
+        }
+        // sum_t (hl + t) c_t = hl * sum c + sum t c; components before the last axis are constant along the segment
+#pragma unroll
+        for (int q = 0; q < PPL; q++) {
+            const double A = (double)acc[q];
+            const double Al = __builtin_fma(hl, A, (double)accl[q]);
+            if (d == 1) {
+                g[q][0] += Al;
+            } else if (d == 2) {
+                g[q][0] = __builtin_fma(h0, A, g[q][0]);
+                g[q][1] += Al;
+            } else if (d == 3) {
+                g[q][0] = __builtin_fma(h0, A, g[q][0]);
+                g[q][1] = __builtin_fma(h1, A, g[q][1]);
+                g[q][2] += Al;
+            } else {
+                g[q][0] = __builtin_fma(h0, A, g[q][0]);
+                g[q][1] = __builtin_fma(h1, A, g[q][1]);
+                g[q][2] = __builtin_fma(h2, A, g[q][2]);
+                g[q][3] += Al;
+            }
+        }
+    }
+}
+
 template <int SEG, int PPL>
 __global__ void __launch_bounds__(kBlock) k_force_lattice(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
 {
     // PPL particles per lane (block covers PPL * kBlock particles): independent phase chains for ILP, and
     // each wavefront-uniform coefficient load serves PPL particles
     __shared__ double2 ltab[64];
+    __shared__ float2 ltab32[64];
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
     if (!(S.flags & kSysActive)) return;
     sincos_table_fill(ltab);
+    __syncthreads();
+    sincos_table_fill32(ltab32, ltab);
     __syncthreads();
     int idx[PPL];
     bool valid[PPL];
@@ -361,67 +547,11 @@ __global__ void __launch_bounds__(kBlock) k_force_lattice(const SysDesc *__restr
         p[q] = valid[q] ? S.pos[idx[q]] : make_double4(0, 0, 0, 0);
         st[q] = valid[q] ? S.step[idx[q]] : make_double2(1, 0);
     }
-    const int s0 = it.t1 * S.kchunk_len;
-    const int s1 = min(S.nseg, s0 + S.kchunk_len);
-    const int d = S.d;
     double g[PPL][4];
 #pragma unroll
     for (int q = 0; q < PPL; q++) g[q][0] = g[q][1] = g[q][2] = g[q][3] = 0.0;
-    const double2 *coef = reinterpret_cast<const double2 *>(S.coef);
-    for (int sg = s0; sg < s1; sg++) {
-        const int4 h = S.seg_head[sg];
-        const int len = S.seg_len[sg];
-        const double h0 = (double)h.x, h1 = (double)h.y, h2 = (double)h.z, h3 = (double)h.w;
-        const double hl = (d == 1) ? h0 : (d == 2) ? h1 : (d == 3) ? h2 : h3;
-        double sn[PPL], cs[PPL], acc[PPL], accl[PPL];
-#pragma unroll
-        for (int q = 0; q < PPL; q++) {
-            const double u = __builtin_fma(h0, p[q].x, __builtin_fma(h1, p[q].y, __builtin_fma(h2, p[q].z, h3 * p[q].w)));
-            sincos_rev_tab(u, ltab, sn[q], cs[q]);
-            acc[q] = 0.0;
-            accl[q] = 0.0;
-        }
-        const double2 *cf = coef + (int64_t)sg * SEG;
-#pragma unroll
-        for (int t0 = 0; t0 < SEG; t0 += 4) {
-            if (t0 < len) {  // segment data is uniform across the wavefront
-#pragma unroll
-                for (int t = t0; t < t0 + 4; t++) {
-                    const double2 ab = cf[t];
-                    const double ht = hl + (double)t;
-#pragma unroll
-                    for (int q = 0; q < PPL; q++) {
-                        const double c = __builtin_fma(ab.x, cs[q], -ab.y * sn[q]);
-                        acc[q] += c;
-                        accl[q] = __builtin_fma(c, ht, accl[q]);
-                        const double nc = __builtin_fma(cs[q], st[q].x, -sn[q] * st[q].y);
-                        const double ns = __builtin_fma(cs[q], st[q].y, sn[q] * st[q].x);
-                        cs[q] = nc;
-                        sn[q] = ns;
-                    }
-                }
-            }
-        }
-        // components before the last axis are constant along the segment
-#pragma unroll
-        for (int q = 0; q < PPL; q++) {
-            if (d == 1) {
-                g[q][0] += accl[q];
-            } else if (d == 2) {
-                g[q][0] = __builtin_fma(h0, acc[q], g[q][0]);
-                g[q][1] += accl[q];
-            } else if (d == 3) {
-                g[q][0] = __builtin_fma(h0, acc[q], g[q][0]);
-                g[q][1] = __builtin_fma(h1, acc[q], g[q][1]);
-                g[q][2] += accl[q];
-            } else {
-                g[q][0] = __builtin_fma(h0, acc[q], g[q][0]);
-                g[q][1] = __builtin_fma(h1, acc[q], g[q][1]);
-                g[q][2] = __builtin_fma(h2, acc[q], g[q][2]);
-                g[q][3] += accl[q];
-            }
-        }
-    }
+    if (S.flags & kSysMixed) force_lattice_body<SEG, PPL, true>(S, it, ltab, ltab32, p, st, g);
+    else force_lattice_body<SEG, PPL, false>(S, it, ltab, ltab32, p, st, g);
 #pragma unroll
     for (int q = 0; q < PPL; q++)
         if (valid[q])
@@ -523,7 +653,20 @@ CC_INSTANTIATE_SEG(32)
 // ---------------------------------------------------------------------------------------------
 namespace ccgpu {
 
+// plain device copy of n double4 (16-byte-aligned 32-byte elements), grid-stride: the known-traffic reference
+// for the memory counters and a bandwidth probe
+__global__ void __launch_bounds__(kBlock) k_copy_probe(const double4 *__restrict__ src, double4 *__restrict__ dst, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) dst[i] = src[i];
+}
+
+void launch_copy_probe(hipStream_t stream, const double4 *src, double4 *dst, int64_t n)
+{
+    if (n > 0) hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(kBlock), 0, stream, src, dst, n);
+}
+
 #define CC_GRID(n) dim3((unsigned)(n)), dim3(kBlock), 0, stream
+#define CC_RHO_GRID(n) dim3((unsigned)(n)), dim3(kRhoBlock), 0, stream
 
 void launch_prepare(hipStream_t stream, const SysDesc *sys, const WorkItem *items, int n)
 {
@@ -543,12 +686,22 @@ void launch_rho_lattice(hipStream_t stream, int seg, const SysDesc *sys, const W
         }
         return;
     }
+    if (variant == 2) {  // mixed-precision systems of the group
+        switch (seg) {
+            case 4: hipLaunchKernelGGL(k_rho_lattice_mixed<4>, CC_RHO_GRID(n), sys, items); break;
+            case 8: hipLaunchKernelGGL(k_rho_lattice_mixed<8>, CC_RHO_GRID(n), sys, items); break;
+            case 16: hipLaunchKernelGGL(k_rho_lattice_mixed<16>, CC_RHO_GRID(n), sys, items); break;
+            case 24: hipLaunchKernelGGL(k_rho_lattice_mixed<24>, CC_RHO_GRID(n), sys, items); break;
+            default: hipLaunchKernelGGL(k_rho_lattice_mixed<32>, CC_RHO_GRID(n), sys, items); break;
+        }
+        return;
+    }
     switch (seg) {
-        case 4: hipLaunchKernelGGL(k_rho_lattice<4>, CC_GRID(n), sys, items); break;
-        case 8: hipLaunchKernelGGL(k_rho_lattice<8>, CC_GRID(n), sys, items); break;
-        case 16: hipLaunchKernelGGL(k_rho_lattice<16>, CC_GRID(n), sys, items); break;
-        case 24: hipLaunchKernelGGL(k_rho_lattice<24>, CC_GRID(n), sys, items); break;
-        default: hipLaunchKernelGGL(k_rho_lattice<32>, CC_GRID(n), sys, items); break;
+        case 4: hipLaunchKernelGGL(k_rho_lattice<4>, CC_RHO_GRID(n), sys, items); break;
+        case 8: hipLaunchKernelGGL(k_rho_lattice<8>, CC_RHO_GRID(n), sys, items); break;
+        case 16: hipLaunchKernelGGL(k_rho_lattice<16>, CC_RHO_GRID(n), sys, items); break;
+        case 24: hipLaunchKernelGGL(k_rho_lattice<24>, CC_RHO_GRID(n), sys, items); break;
+        default: hipLaunchKernelGGL(k_rho_lattice<32>, CC_RHO_GRID(n), sys, items); break;
     }
 }
 

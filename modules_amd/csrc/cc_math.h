@@ -87,6 +87,28 @@ __device__ __forceinline__ void sincos_rev_tab(double u, const double2 *tab, dou
     s = __builtin_fma(t.y, cl, t.x * sl);
 }
 
+// FP32 counterparts for the mixed-precision path: the table as float2 (rounded from the FP64 table), and
+// sin/cos of 2 pi f for an argument already reduced to f in [-1/2, 1/2] revolutions in FP64 (the
+// per-particle range reduction) and rounded to FP32.  Same split as sincos_rev_tab; sin to x^5, cos to x^4
+// (next terms below 7e-13 and 2e-11, under FP32 rounding).
+__device__ __forceinline__ void sincos_table_fill32(float2 *tab32, const double2 *tab)
+{
+    if (threadIdx.x < 64) tab32[threadIdx.x] = make_float2((float)tab[threadIdx.x].x, (float)tab[threadIdx.x].y);
+}
+
+__device__ __forceinline__ void sincos_rev_tab32(float f, const float2 *tab, float &s, float &c)
+{
+    const float jd = __builtin_rintf(64.0f * f);
+    const float fl = __builtin_fmaf(-0.015625f, jd, f);  // exact (Sterbenz)
+    const float x = fl * 6.28318530717958648f;
+    const float x2 = x * x;
+    const float sl = __builtin_fmaf(x * x2, __builtin_fmaf(x2, 8.33333333e-3f, -1.66666667e-1f), x);
+    const float cl = __builtin_fmaf(x2, __builtin_fmaf(x2, 4.16666667e-2f, -0.5f), 1.0f);
+    const float2 t = tab[((int)jd) & 63];
+    c = __builtin_fmaf(t.x, cl, -t.y * sl);
+    s = __builtin_fmaf(t.y, cl, t.x * sl);
+}
+
 // Radian argument: u = theta / (2 pi) formed in double-double before the reduction, so the conversion
 // adds no error beyond the ulp of theta itself.
 __device__ __forceinline__ void sincos_rad(double theta, double &s, double &c)

@@ -30,9 +30,10 @@ struct RelaxState {
     int evals;       // objective evaluations counted by the passes (Aux::NumEvaluation)
     int device_evals;  // every E+F evaluation made for this job
     int maxstep;
-    int pad;
+    int switch_eval;   // mixed precision: device evaluation at which the job went to FP64 (0: never mixed)
     double fmin;     // stop value: Emin, or -MaxEnergy for RelaxStructure_NLOPT
     double E;        // energy of the last evaluation
+    double polish;   // mixed precision: switch to FP64 once E falls below this
 };
 
 struct RelaxDesc {
@@ -61,6 +62,8 @@ struct Job {
     double shift = 0, e0 = 0, sigma = 0, Emin = -kMaxEnergy;
     int maxstep = 10000;
     int mf = 40;
+    int precision = 0;      // 0: FP64; 1: mixed until E < polish or the end of the first pass
+    double polish = 1e-8;
     int Tiles_seg = 0;
     // lifecycle
     enum State { kQueued, kResident, kDone, kFailed } state = kQueued;
@@ -69,15 +72,50 @@ struct Job {
     // results
     std::vector<double> s_final;
     double E = 0;
-    int evals = 0, device_evals = 0, status = 0, passes = 0;
+    int evals = 0, device_evals = 0, status = 0, passes = 0, switch_eval = 0;
     std::string error;
     // device side (resident only)
+    int lane = 0;  // solver lane (stream) the job is resident on
     std::unique_ptr<System> sys;
     DevArray<double> sdir, xo, go, uv, gram, part1, part2, lcoef;
     DevArray<RelaxState> st;
     RelaxState host_state{};
 };
 
+// One stream of the solver with the resident jobs it evaluates: its own batch (descriptor table, work lists),
+// L-BFGS descriptors and, in graph mode, its captured burst.
+struct SolverLane {
+    Batch batch;
+    std::vector<Job *> jobs;   // resident, unfinished jobs
+    std::vector<Job *> slots;  // the batch's systems in descriptor order; nullptr once harvested (the kernels
+                               // skip a finished system, so the work lists are rebuilt only on admission or
+                               // when half the slots are idle)
+    SysDesc *h_desc = nullptr; // pinned host copy of the descriptor table (harvest reads)
+    size_t h_desc_cap = 0;
+    DevArray<RelaxDesc> d_rdesc;
+    DevArray<WorkItem> d_wide;
+    int nwide = 0;
+    hipGraphExec_t gexec = nullptr;
+    int graph_burst = 0;
+    bool dirty = true;
+    int64_t work = 0;  // sum of N (M + 2 history) of its jobs: the load-balancing weight
+    void DropGraph()
+    {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        gexec = nullptr;
+        graph_burst = 0;
+    }
+    ~SolverLane()
+    {
+        if (h_desc) (void)hipHostFree(h_desc);
+    }
+};
+
+// Continuous-batching solver.  Resident jobs are spread over `lanes` streams (default 2); every optimizer step
+// enqueues each lane's batched E+F and L-BFGS kernels on that lane's stream, so the latency-bound L-BFGS
+// kernels of one lane run beside the compute-bound E+F kernels of the other.  A job's arithmetic does not
+// depend on its lane or on the other jobs (per-system tiles, fixed-order sums), so results are the same bits
+// for any lane count and mix.
 class Solver {
 public:
     Solver(int max_slots, int64_t hbm_budget);
@@ -97,16 +135,26 @@ public:
     void SetGraph(bool on)
     {
         use_graph_ = on;
-        DropGraph();
+        for (auto &l : lanes_) l->DropGraph();
     }
-
-    Batch batch;
+    // precision of jobs submitted from now on: 0 FP64, 1 mixed with the FP64 switch below `polish`
+    void SetPrecision(int mode, double polish)
+    {
+        precision_ = mode;
+        polish_ = polish;
+    }
+    int precision() const { return precision_; }
+    double polish() const { return polish_; }
+    // number of streams (1 or 2); only while no job is resident
+    void SetLanes(int n);
+    int Lanes() const { return (int)lanes_.size(); }
+    void SetTiming(bool on);
+    void Timing(double &rho_ms, double &force_ms, int64_t &n_rho, int64_t &n_force, bool reset);
 
 private:
     void Admit();
-    void Rebuild();
+    void Rebuild(SolverLane &L);
     void LaunchBurst(int burst);
-    void DropGraph();
     void Harvest();
     static int64_t EstimateBytes(const Job &j);
 
@@ -114,16 +162,14 @@ private:
     int64_t budget_, bytes_in_use_ = 0;
     int64_t next_id_ = 1;
     int check_every_ = 8;
-    bool dirty_ = true;
     bool use_graph_ = false;
-    hipGraphExec_t gexec_ = nullptr;
-    int graph_burst_ = 0;
+    bool timing_ = false;
+    int precision_ = 0;
+    double polish_ = 1e-8;
     std::deque<Job *> queue_;
     std::vector<Job *> resident_;
     std::map<int64_t, std::unique_ptr<Job>> jobs_;
-    DevArray<RelaxDesc> d_rdesc_;
-    DevArray<WorkItem> d_wide_;
-    int nwide_ = 0;
+    std::vector<std::unique_ptr<SolverLane>> lanes_;
     double t0_;
 };
 

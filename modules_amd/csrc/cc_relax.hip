@@ -257,7 +257,19 @@ __global__ void __launch_bounds__(kRelaxBlock) k_relax_step(SysDesc *__restrict_
         E = D->energy[0];
         st.E = E;
         st.device_evals++;
-        if (st.boundary) {
+        // Mixed precision: the FP32-phase objective drives the descent until E first falls below `polish` or
+        // the first pass ends; then the system goes to FP64 and the same x is evaluated again as the start of
+        // the interrupted (or next) pass, so the polish iterations and every convergence test see FP64.
+        const bool to_fp64 = (D->flags & kSysMixed) && (E < st.polish || (st.boundary && st.pass > 0));
+        if (to_fp64) {
+            if (!st.boundary) {
+                st.evals += st.plis.nevals;
+                st.boundary = 1;
+            }
+            st.switch_eval = st.device_evals;
+            __syncthreads();
+            if (threadIdx.x == 0) D->flags &= ~kSysMixed;
+        } else if (st.boundary) {
             // RelaxStructure_NLOPT_inner starts with "if (pot.Energy() < fmin) return" and _Emin stops between
             // passes once E < Emin; after the third pass the solve ends.
             if (st.pass >= 3 || E < st.fmin) {
@@ -548,13 +560,52 @@ void launch_relax_step(hipStream_t stream, SysDesc *sys, const RelaxDesc *rd, in
 Solver::Solver(int max_slots, int64_t hbm_budget) : max_slots_(max_slots), budget_(hbm_budget)
 {
     t0_ = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+    SetLanes(2);
 }
 
 Solver::~Solver()
 {
-    (void)hipStreamSynchronize(batch.stream);
-    DropGraph();
-    batch.systems.clear();
+    for (auto &l : lanes_) {
+        (void)hipStreamSynchronize(l->batch.stream);
+        l->DropGraph();
+        l->batch.systems.clear();
+    }
+}
+
+void Solver::SetLanes(int n)
+{
+    if (n < 1 || n > 2) throw Error(kErrArg, "lanes must be 1 or 2");
+    if (!resident_.empty()) throw Error(kErrState, "lanes can change only while no job is resident");
+    while ((int)lanes_.size() > n) lanes_.pop_back();
+    while ((int)lanes_.size() < n) {
+        lanes_.emplace_back(new SolverLane());
+        lanes_.back()->batch.timing = timing_;
+    }
+}
+
+void Solver::SetTiming(bool on)
+{
+    timing_ = on;
+    for (auto &l : lanes_) l->batch.timing = on;
+}
+
+void Solver::Timing(double &rho_ms, double &force_ms, int64_t &n_rho, int64_t &n_force, bool reset)
+{
+    rho_ms = force_ms = 0;
+    n_rho = n_force = 0;
+    for (auto &l : lanes_) {
+        Batch &b = l->batch;
+        b.Sync();
+        b.CollectTiming();
+        rho_ms += b.t_rho_ms;
+        force_ms += b.t_force_ms;
+        n_rho += b.n_rho;
+        n_force += b.n_force;
+        if (reset) {
+            b.t_rho_ms = b.t_force_ms = 0;
+            b.n_rho = b.n_force = 0;
+        }
+    }
 }
 
 double Solver::Now() const
@@ -607,6 +658,11 @@ void Solver::Admit()
     while (!queue_.empty() && (int)resident_.size() < max_slots_ && bytes_in_use_ + queue_.front()->bytes <= budget_) {
         Job *j = queue_.front();
         queue_.pop_front();
+        // the lane with the least work (ties to the lower index): balanced streams
+        int li = 0;
+        for (int q = 1; q < (int)lanes_.size(); q++)
+            if (lanes_[q]->work < lanes_[li]->work) li = q;
+        SolverLane &L = *lanes_[li];
         try {
             j->sys.reset(new System(j->d));
             System &S = *j->sys;
@@ -618,7 +674,8 @@ void Solver::Admit()
             S.sigma = j->sigma;
             S.rescale = kRelaxRescale[0];
             if (j->Tiles_seg) S.tiles.seg = j->Tiles_seg;
-            S.Prepare(batch.stream);
+            S.precision = j->precision;
+            S.Prepare(L.batch.stream);
             const int64_t n = j->N * j->d;
             j->sdir.Reserve(std::max<int64_t>(n, 1));
             j->xo.Reserve(std::max<int64_t>(n * j->mf, 1));
@@ -635,21 +692,25 @@ void Solver::Admit()
             std::vector<double> &x0 = staged.back();
             for (int64_t i = 0; i < n; i++) x0[i] = j->s0[i] * kRelaxRescale[0];  // GetParam (:758-771)
             if (n > 0)
-                CC_HIP(hipMemcpyAsync(S.x.get(), x0.data(), n * sizeof(double), hipMemcpyHostToDevice, batch.stream));
+                CC_HIP(hipMemcpyAsync(S.x.get(), x0.data(), n * sizeof(double), hipMemcpyHostToDevice, L.batch.stream));
             CC_HIP(hipMemcpyAsync(j->uv.get(), x0.data() + n, 2 * j->mf * sizeof(double), hipMemcpyHostToDevice,
-                                  batch.stream));
+                                  L.batch.stream));
             RelaxState &rs = j->host_state;
             std::memset(&rs, 0, sizeof(rs));
             rs.pass = 0;
             rs.boundary = 1;
             rs.maxstep = j->maxstep;
             rs.fmin = j->Emin;
-            CC_HIP(hipMemcpyAsync(j->st.get(), &rs, sizeof(rs), hipMemcpyHostToDevice, batch.stream));
+            rs.polish = j->polish;
+            CC_HIP(hipMemcpyAsync(j->st.get(), &rs, sizeof(rs), hipMemcpyHostToDevice, L.batch.stream));
             j->t_admit = Now();
             j->state = Job::kResident;
+            j->lane = li;
             bytes_in_use_ += j->bytes;
             resident_.push_back(j);
-            dirty_ = true;
+            L.jobs.push_back(j);
+            L.work += j->N * ((int64_t)j->v.size() + 2 * j->mf);
+            L.dirty = true;
         } catch (const Error &e) {
             j->state = Job::kFailed;
             j->error = e.what();
@@ -657,17 +718,20 @@ void Solver::Admit()
             j->t_done = Now();
         }
     }
-    if (!staged.empty()) CC_HIP(hipStreamSynchronize(batch.stream));
+    if (!staged.empty())
+        for (auto &l : lanes_) CC_HIP(hipStreamSynchronize(l->batch.stream));
 }
 
-void Solver::Rebuild()
+void Solver::Rebuild(SolverLane &L)
 {
-    DropGraph();
+    L.DropGraph();
+    Batch &batch = L.batch;
     batch.systems.clear();
     std::vector<RelaxDesc> rd;
     std::vector<WorkItem> wide;
-    for (size_t i = 0; i < resident_.size(); i++) {
-        Job *j = resident_[i];
+    L.slots = L.jobs;
+    for (size_t i = 0; i < L.jobs.size(); i++) {
+        Job *j = L.jobs[i];
         batch.systems.push_back(j->sys.get());
         RelaxDesc r;
         std::memset(&r, 0, sizeof(r));
@@ -689,120 +753,156 @@ void Solver::Rebuild()
     }
     batch.dirty = true;
     batch.Build();
-    d_rdesc_.Reserve(std::max<size_t>(1, rd.size()));
+    L.d_rdesc.Reserve(std::max<size_t>(1, rd.size()));
     if (!rd.empty())
-        CC_HIP(hipMemcpyAsync(d_rdesc_.get(), rd.data(), rd.size() * sizeof(RelaxDesc), hipMemcpyHostToDevice,
+        CC_HIP(hipMemcpyAsync(L.d_rdesc.get(), rd.data(), rd.size() * sizeof(RelaxDesc), hipMemcpyHostToDevice,
                               batch.stream));
-    d_wide_.Reserve(std::max<size_t>(1, wide.size()));
-    nwide_ = (int)wide.size();
+    L.d_wide.Reserve(std::max<size_t>(1, wide.size()));
+    L.nwide = (int)wide.size();
     if (!wide.empty())
-        CC_HIP(hipMemcpyAsync(d_wide_.get(), wide.data(), wide.size() * sizeof(WorkItem), hipMemcpyHostToDevice,
+        CC_HIP(hipMemcpyAsync(L.d_wide.get(), wide.data(), wide.size() * sizeof(WorkItem), hipMemcpyHostToDevice,
                               batch.stream));
+    if (L.h_desc_cap < L.slots.size()) {
+        if (L.h_desc) CC_HIP(hipHostFree(L.h_desc));
+        L.h_desc = nullptr;
+        L.h_desc_cap = 0;
+        CC_HIP(hipHostMalloc((void **)&L.h_desc, std::max<size_t>(1, L.slots.size()) * sizeof(SysDesc)));
+        L.h_desc_cap = L.slots.size();
+    }
     CC_HIP(hipStreamSynchronize(batch.stream));
-    dirty_ = false;
+    L.dirty = false;
 }
 
 void Solver::Harvest()
 {
-    std::vector<Job *> keep, fin;
-    // one read of the descriptor table tells which jobs finished (kSysActive cleared) and the current
-    // rescale of the others (the step kernel changes it at pass boundaries)
-    std::vector<SysDesc> dev(resident_.size());
-    if (!dev.empty())
-        CC_HIP(hipMemcpyAsync(dev.data(), batch.d_desc.get(), dev.size() * sizeof(SysDesc), hipMemcpyDeviceToHost,
-                              batch.stream));
-    CC_HIP(hipStreamSynchronize(batch.stream));
-    if (batch.timing) batch.CollectTiming();
-    for (size_t q = 0; q < resident_.size(); q++) {
-        Job *j = resident_[q];
-        j->sys->rescale = dev[q].rescale;
-        if (dev[q].flags & kSysActive) keep.push_back(j);
-        else fin.push_back(j);
+    // one read of each lane's descriptor table (into pinned memory) tells which jobs finished (kSysActive
+    // cleared) and the current rescale of the others (the step kernel changes it at pass boundaries)
+    for (auto &l : lanes_) {
+        SolverLane &L = *l;
+        if (!L.jobs.empty())
+            CC_HIP(hipMemcpyAsync(L.h_desc, L.batch.d_desc.get(), L.slots.size() * sizeof(SysDesc),
+                                  hipMemcpyDeviceToHost, L.batch.stream));
     }
-    if (!fin.empty()) {
-        // final states and coordinates of every finished job: copies queued back to back, one synchronisation
-        std::vector<std::vector<double>> xs(fin.size());
-        for (size_t i = 0; i < fin.size(); i++) {
-            Job *j = fin[i];
-            const int64_t n = j->N * j->d;
-            CC_HIP(hipMemcpyAsync(&j->host_state, j->st.get(), sizeof(RelaxState), hipMemcpyDeviceToHost, batch.stream));
-            xs[i].resize(n);
-            if (n > 0)
-                CC_HIP(hipMemcpyAsync(xs[i].data(), j->sys->x.get(), n * sizeof(double), hipMemcpyDeviceToHost,
-                                      batch.stream));
+    std::vector<Job *> fin;
+    for (auto &l : lanes_) {
+        SolverLane &L = *l;
+        if (L.jobs.empty()) continue;
+        CC_HIP(hipStreamSynchronize(L.batch.stream));
+        if (L.batch.timing) L.batch.CollectTiming();
+        std::vector<Job *> keep;
+        size_t idle = 0;
+        for (size_t q = 0; q < L.slots.size(); q++) {
+            Job *j = L.slots[q];
+            if (!j) {
+                idle++;
+                continue;
+            }
+            const SysDesc &D = L.h_desc[q];
+            j->sys->rescale = D.rescale;
+            if (!(D.flags & kSysMixed)) j->sys->precision = 0;  // switched to FP64 on the device
+            if (D.flags & kSysActive) {
+                keep.push_back(j);
+            } else {
+                fin.push_back(j);
+                L.slots[q] = nullptr;
+                idle++;
+                L.work -= j->N * ((int64_t)j->v.size() + 2 * j->mf);
+            }
         }
-        CC_HIP(hipStreamSynchronize(batch.stream));
-        for (size_t i = 0; i < fin.size(); i++) {
-            Job *j = fin[i];
-            const RelaxState &hs = j->host_state;
-            const int64_t n = j->N * j->d;
-            const double r = hs.pass <= 2 ? kRelaxRescale[hs.pass] : 1.0;
-            j->s_final.resize(n);
-            for (int64_t e = 0; e < n; e++) j->s_final[e] = WrapUnit(xs[i][e] / r);
-            j->E = hs.E;
-            j->evals = hs.evals;
-            j->device_evals = hs.device_evals;
-            j->status = hs.status;
-            j->passes = hs.pass;
-            j->t_done = Now();
-            j->state = Job::kDone;
-            j->sys.reset();
-            j->sdir.Release();
-            j->xo.Release();
-            j->go.Release();
-            j->uv.Release();
-            j->gram.Release();
-            j->part1.Release();
-            j->part2.Release();
-            j->lcoef.Release();
-            j->st.Release();
-            bytes_in_use_ -= j->bytes;
-            dirty_ = true;
-        }
+        L.jobs.swap(keep);
+        if (2 * idle > L.slots.size()) L.dirty = true;  // compact the work lists
     }
+    if (fin.empty()) return;
+    // final states and coordinates of every finished job: copies queued back to back per lane, then one
+    // synchronisation per lane
+    std::vector<std::vector<double>> xs(fin.size());
+    for (size_t i = 0; i < fin.size(); i++) {
+        Job *j = fin[i];
+        hipStream_t st = lanes_[j->lane]->batch.stream;
+        const int64_t n = j->N * j->d;
+        CC_HIP(hipMemcpyAsync(&j->host_state, j->st.get(), sizeof(RelaxState), hipMemcpyDeviceToHost, st));
+        xs[i].resize(n);
+        if (n > 0) CC_HIP(hipMemcpyAsync(xs[i].data(), j->sys->x.get(), n * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
+    for (auto &l : lanes_) CC_HIP(hipStreamSynchronize(l->batch.stream));
+    for (size_t i = 0; i < fin.size(); i++) {
+        Job *j = fin[i];
+        const RelaxState &hs = j->host_state;
+        const int64_t n = j->N * j->d;
+        const double r = hs.pass <= 2 ? kRelaxRescale[hs.pass] : 1.0;
+        j->s_final.resize(n);
+        for (int64_t e = 0; e < n; e++) j->s_final[e] = WrapUnit(xs[i][e] / r);
+        j->E = hs.E;
+        j->evals = hs.evals;
+        j->device_evals = hs.device_evals;
+        j->status = hs.status;
+        j->passes = hs.pass;
+        j->switch_eval = hs.switch_eval;
+        j->t_done = Now();
+        j->state = Job::kDone;
+        j->sys.reset();
+        j->sdir.Release();
+        j->xo.Release();
+        j->go.Release();
+        j->uv.Release();
+        j->gram.Release();
+        j->part1.Release();
+        j->part2.Release();
+        j->lcoef.Release();
+        j->st.Release();
+        bytes_in_use_ -= j->bytes;
+    }
+    std::vector<Job *> keep;
+    for (Job *j : resident_)
+        if (j->state == Job::kResident) keep.push_back(j);
     resident_.swap(keep);
-}
-
-void Solver::DropGraph()
-{
-    if (gexec_) (void)hipGraphExecDestroy(gexec_);
-    gexec_ = nullptr;
-    graph_burst_ = 0;
 }
 
 void Solver::LaunchBurst(int burst)
 {
-    auto enqueue = [&] {
-        for (int it = 0; it < burst; it++) {
-            batch.Enqueue(true, true);
-            launch_relax_step(batch.stream, batch.d_desc.get(), d_rdesc_.get(), (int)resident_.size(), d_wide_.get(),
-                              nwide_);
+    // hipGraph replay of `burst` optimizer steps per lane (no memset nodes: every kernel writes what it reads
+    // next); timing mode records events per launch and stays on plain stream launches.  Lanes are enqueued
+    // step by step in turn, so the two streams advance side by side.
+    auto enqueue = [&](SolverLane &L, int steps) {
+        for (int it = 0; it < steps; it++) {
+            L.batch.Enqueue(true, true);
+            launch_relax_step(L.batch.stream, L.batch.d_desc.get(), L.d_rdesc.get(), (int)L.slots.size(),
+                              L.d_wide.get(), L.nwide);
         }
     };
-    // hipGraph replay of `burst` optimizer steps (no memset nodes: every kernel writes what it reads next);
-    // timing mode records events per launch and stays on plain stream launches
-    if (!use_graph_ || batch.timing) {
-        enqueue();
+    if (!use_graph_ || timing_) {
+        for (int it = 0; it < burst; it++)
+            for (auto &l : lanes_)
+                if (!l->jobs.empty()) enqueue(*l, 1);
         return;
     }
-    if (!gexec_ || graph_burst_ != burst) {
-        DropGraph();
-        hipGraph_t g = nullptr;
-        CC_HIP(hipStreamBeginCapture(batch.stream, hipStreamCaptureModeThreadLocal));
-        enqueue();
-        CC_HIP(hipStreamEndCapture(batch.stream, &g));
-        hipError_t e = hipGraphInstantiate(&gexec_, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        CC_HIP(e);
-        graph_burst_ = burst;
+    for (auto &l : lanes_) {
+        SolverLane &L = *l;
+        if (L.jobs.empty()) continue;
+        if (!L.gexec || L.graph_burst != burst) {
+            L.DropGraph();
+            hipGraph_t g = nullptr;
+            CC_HIP(hipStreamBeginCapture(L.batch.stream, hipStreamCaptureModeThreadLocal));
+            enqueue(L, burst);
+            CC_HIP(hipStreamEndCapture(L.batch.stream, &g));
+            hipError_t e = hipGraphInstantiate(&L.gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            CC_HIP(e);
+            L.graph_burst = burst;
+        }
     }
-    CC_HIP(hipGraphLaunch(gexec_, batch.stream));
+    for (auto &l : lanes_)
+        if (!l->jobs.empty()) CC_HIP(hipGraphLaunch(l->gexec, l->batch.stream));
 }
 
 int Solver::Step(int evaluations)
 {
-    Admit();
-    if (dirty_) Rebuild();
+    auto refresh = [&] {
+        Admit();
+        for (auto &l : lanes_)
+            if (l->dirty && !l->jobs.empty()) Rebuild(*l);
+    };
+    refresh();
     int done = 0;
     while (done < evaluations && !resident_.empty()) {
         const int burst = std::min(check_every_, evaluations - done);
@@ -810,10 +910,7 @@ int Solver::Step(int evaluations)
         CC_HIP(hipGetLastError());
         done += burst;
         Harvest();
-        if (dirty_) {
-            Admit();
-            if (!resident_.empty()) Rebuild();
-        }
+        refresh();
     }
     return (int)(resident_.size() + queue_.size());
 }

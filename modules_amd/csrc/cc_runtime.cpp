@@ -286,7 +286,7 @@ void System::AddConstraint(const double *kk, double V)
 
 size_t System::DeviceBytes() const
 {
-    return x.bytes() + pos.bytes() + cart.bytes() + step.bytes() + (layout ? layout->DeviceBytes() : 0) + rpart.bytes() + rho.bytes() + coef.bytes() + epart.bytes() +
+    return x.bytes() + pos.bytes() + cart.bytes() + step.bytes() + (layout ? layout->DeviceBytes() : 0) + rpart.bytes() + rho.bytes() + coef.bytes() + coef32.bytes() + epart.bytes() +
            energy.bytes() + fpart.bytes() + F.bytes() + grad.bytes() + cell_of.bytes() + cell_start.bytes() +
            cell_items.bytes() + sc_epart.bytes() + sc_F.bytes();
 }
@@ -303,6 +303,7 @@ void System::Prepare(hipStream_t stream)
         throw Error(kErrArg, "segment length must be 4, 8, 16, 24 or 32");
     used.rho_chunk = tiles.rho_chunk > 0 ? std::min(tiles.rho_chunk, 512) : def.rho_chunk;
     used.rho_kernel = tiles.rho_kernel >= 0 ? tiles.rho_kernel : def.rho_kernel;
+    if (precision == 1) used.rho_kernel = 0;  // the mixed-precision body lives in the grouped-power kernel
     used.force_ppl = tiles.force_ppl > 0 ? tiles.force_ppl : (def.force_ppl > 0 ? def.force_ppl : 2);
     if (used.force_ppl != 1 && used.force_ppl != 2) throw Error(kErrArg, "force particles per lane must be 1 or 2");
 
@@ -351,6 +352,7 @@ void System::Prepare(hipStream_t stream)
     rpart.Reserve(std::max<int64_t>(1, (int64_t)pchunks * nslots * 2));
     rho.Reserve(std::max<int64_t>(1, 2 * M));
     coef.Reserve(std::max<int64_t>(1, 2 * nslots));
+    coef32.Reserve(std::max<int64_t>(1, 2 * nslots));
     epart.Reserve(std::max<int64_t>(1, (nslots + kBlock - 1) / kBlock));
     energy.Reserve(4);
     fpart.Reserve((int64_t)kchunks * Nn * 4);
@@ -368,7 +370,8 @@ void System::Prepare(hipStream_t stream)
     D.d = d;
     D.N = (int32_t)N;
     D.M = (int32_t)M;
-    D.flags = kSysActive | (lattice ? kSysLattice : 0) | (softcore() ? kSysSoftCore : 0);
+    D.flags = kSysActive | (lattice ? kSysLattice : 0) | (softcore() ? kSysSoftCore : 0) |
+              (lattice && precision == 1 ? kSysMixed : 0);
     D.nseg = (int32_t)nseg();
     D.nslots = (int32_t)nslots;
     D.pchunks = N > 0 ? pchunks : 0;  // no particles: rho sums nothing
@@ -389,6 +392,7 @@ void System::Prepare(hipStream_t stream)
     D.rpart = rpart.get();
     D.rho = rho.get();
     D.coef = coef.get();
+    D.coef32 = coef32.get();
     D.epart = epart.get();
     D.energy = energy.get();
     D.fpart = fpart.get();
@@ -481,13 +485,18 @@ void Batch::Build()
         g.seg = sl;
         g.rho_kernel = rk;
         g.force_ppl = ppl;
+        g.any_mixed = false;
+        for (System *s : systems)
+            if (s->lattice && s->precision == 1 && s->used.seg == sl && s->used.rho_kernel == rk && s->used.force_ppl == ppl)
+                g.any_mixed = true;
         add(g.rho, [&] {
             for (int i = 0; i < ns; i++) {
                 System *s = systems[i];
                 if (!s->lattice || s->used.seg != sl || s->used.rho_kernel != rk || s->used.force_ppl != ppl ||
                     s->N == 0)
                     continue;
-                const int sb = (int)((s->nseg() + kBlock - 1) / kBlock);
+                const int bs = rk == 1 ? kBlock : kRhoBlock;  // segments per rho block
+                const int sb = (int)((s->nseg() + bs - 1) / bs);
                 for (int c = 0; c < s->pchunks; c++)
                     for (int b = 0; b < sb; b++) h_items.push_back({i, b, c, 0});
             }
@@ -572,7 +581,10 @@ void Batch::Enqueue(bool rho, bool forces)
             ev_marks_.push_back({0, ev_used_});
             CC_HIP(hipEventRecord(NextEvent(), stream));
         }
-        for (const SegGroup &g : seg_groups) launch_rho_lattice(stream, g.seg, D, I + g.rho.off, g.rho.n, g.rho_kernel);
+        for (const SegGroup &g : seg_groups) {
+            launch_rho_lattice(stream, g.seg, D, I + g.rho.off, g.rho.n, g.rho_kernel);
+            if (g.any_mixed) launch_rho_lattice(stream, g.seg, D, I + g.rho.off, g.rho.n, 2);
+        }
         launch_rho_general(stream, D, I + r_rho_gen.off, r_rho_gen.n);
         if (timing) CC_HIP(hipEventRecord(NextEvent(), stream));
         launch_rho_reduce(stream, D, I + r_red.off, r_red.n);

@@ -141,6 +141,8 @@ public:
     DevArray<double4> pos, cart;
     DevArray<double2> step;
     DevArray<double> rpart, rho, coef, epart, energy, fpart, F, grad;
+    DevArray<float> coef32;
+    int precision = 0;  // 0: FP64; 1: mixed (lattice phases and force terms in FP32)
     DevArray<int32_t> cell_of, cell_start, cell_items;
     DevArray<double> sc_epart, sc_F;
 
@@ -182,6 +184,7 @@ public:
     };
     struct SegGroup {
         int seg, rho_kernel, force_ppl;
+        bool any_mixed;  // some system of the group computes in mixed precision: launch the mixed rho kernel too
         Range rho, force;
     };
     // optional kernel timing: hipEvents around the rho and force contraction launches of every Enqueue
@@ -204,6 +207,7 @@ private:
 };
 
 // kernel launchers (cc_kernels.hip)
+void launch_copy_probe(hipStream_t, const double4 *src, double4 *dst, int64_t n);
 void launch_prepare(hipStream_t, const SysDesc *, const WorkItem *, int);
 void launch_rho_lattice(hipStream_t, int seg, const SysDesc *, const WorkItem *, int, int variant);
 void launch_rho_general(hipStream_t, const SysDesc *, const WorkItem *, int);

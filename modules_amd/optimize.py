@@ -157,12 +157,21 @@ class BatchSolver:
 
     QUEUED, RESIDENT, DONE, FAILED = 0, 1, 2, 3
 
-    def __init__(self, max_slots=256, hbm_budget=200 << 30, check_interval=8, graph=False):
+    def __init__(self, max_slots=256, hbm_budget=200 << 30, check_interval=8, graph=False, precision="fp64",
+                 polish=1e-8, lanes=2):
         self._h = C.c_void_p()
         check(lib().ccgpu_solver_create(int(max_slots), int(hbm_budget), C.byref(self._h)))
         check(lib().ccgpu_solver_set_check_interval(self._h, int(check_interval)))
         check(lib().ccgpu_solver_set_graph(self._h, 1 if graph else 0))
+        check(lib().ccgpu_solver_set_lanes(self._h, int(lanes)))
+        self.set_precision(precision, polish)
         self._shapes = {}
+
+    def set_precision(self, precision="fp64", polish=1e-8):
+        """Precision of jobs submitted from now on: "fp64", or "mixed" (FP32 phases and force terms, FP64
+        sums) until E < polish or the end of the first pass, FP64 afterwards."""
+        from .potential import _precision_code
+        check(lib().ccgpu_solver_set_precision(self._h, _precision_code(precision), float(polish)))
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -204,8 +213,11 @@ class BatchSolver:
         rel = np.zeros(self._shapes[job])
         check(lib().ccgpu_solver_result(self._h, int(job), C.byref(state), C.byref(E), C.byref(ev), C.byref(dev),
                                         C.byref(st), _p(t), _p(rel)))
+        sw = C.c_int(0)
+        check(lib().ccgpu_solver_switch_eval(self._h, int(job), C.byref(sw)))
         return {"state": state.value, "energy": E.value, "evaluations": ev.value, "device_evaluations": dev.value,
-                "status": st.value, "t_submit": t[0], "t_admit": t[1], "t_done": t[2], "relative": rel}
+                "status": st.value, "t_submit": t[0], "t_admit": t[1], "t_done": t[2], "relative": rel,
+                "switch_eval": sw.value}
 
     def release(self, job):
         check(lib().ccgpu_solver_release(self._h, int(job)))

@@ -25,6 +25,13 @@ def _p(a):
     return a.ctypes.data_as(dp)
 
 
+def _precision_code(mode):
+    codes = {"fp64": 0, "mixed": 1, 0: 0, 1: 1}
+    if mode not in codes:
+        raise ValueError('precision must be "fp64" or "mixed"')
+    return codes[mode]
+
+
 class Configuration:
     """Periodic point pattern: basis vectors (rows) and relative coordinates in [0,1)."""
 
@@ -301,6 +308,12 @@ class ShiftedCCPotential:
 
     def SetRhoKernel(self, variant):
         check(lib().ccgpu_potential_set_rho_kernel(self._h, int(variant)))
+
+    def SetPrecision(self, mode):
+        """"fp64" (default) or "mixed": lattice phases and force terms in FP32 after an FP64 range reduction,
+        sums in FP64 (no counterpart in the reference; its error against FP64 is the caller's to report)."""
+        check(lib().ccgpu_potential_set_precision(self._h, _precision_code(mode)))
+        self.RhoValid = False
 
     def UsesLatticePath(self):
         v = C.c_int(0)

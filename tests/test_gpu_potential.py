@@ -222,3 +222,34 @@ def test_autotuned_tiles_are_used_and_keep_parity():
     k = GetKs(cfg, getK(0.3, 1.0, 3))
     compare(cfg, k, np.ones(len(k)), expect_lattice=True)
     autotune.clear()
+
+
+@pytest.mark.parametrize("N,d,chi,seed", [(2000, 2, 0.4, 4), (2000, 3, 0.45, 1), (700, 2, 0.3, 11)])
+def test_mixed_precision_error_against_oracle(N, d, chi, seed):
+    # mixed mode: phases reduced in FP64, sincos / phase steps / force terms in FP32, sums in FP64.  FP32 has a
+    # unit roundoff of 6e-8; per-term phase errors of a few 1e-7 summed over N particles bound the error at
+    # 1e-5 of the scales (rho ~ N, E ~ sum|V| N^2 / vol, F ~ 4 sum|V| N kmax / vol)
+    cfg = RandomConfiguration(seed, N, d)
+    k = GetKs(cfg, getK(chi, 1.0, d))
+    v = np.ones(len(k))
+    pot = make_pot(d, k, v)
+    pot.SetPrecision("mixed")
+    pot.SetConfiguration(cfg)
+    pot.GetRho()
+    E, F = pot.EnergyAndAllForce()
+    re, im = po.rho(cfg.relative, cfg.basis, k)
+    vol = po.volume(cfg.basis)
+    Eo = po.energy(v, re, im, vol)
+    Fo = po.forces(cfg.relative, cfg.basis, k, v, re, im)
+    sr, se, sf = scales(k, v, N, vol)
+    drho = max(np.max(np.abs(pot.RhoReal - re)), np.max(np.abs(pot.RhoImag - im))) / sr
+    dE = abs(E - Eo) / se
+    dF = np.max(np.abs(F - Fo)) / sf
+    print(f"mixed N={N} d={d}: rho {drho:.2e}, E {dE:.2e}, F {dF:.2e} (relative to the scales)")
+    assert drho <= 1e-5 and dE <= 1e-5 and dF <= 1e-5
+    # and it is not the FP64 path: the FP32 rounding shows
+    assert drho > 1e-13
+    pot.SetPrecision("fp64")
+    pot.SetConfiguration(cfg)
+    E64, F64 = pot.EnergyAndAllForce()
+    assert abs(E64 - Eo) <= RTOL * se

@@ -180,3 +180,39 @@ def test_potential_check_ratio_is_one():
         pot.AddConstraint(kk, [1.0])
     for i in (0, 123):
         assert abs(pot.Check(cfg, i) - 1.0) < 1e-3
+
+
+def test_mixed_precision_solves_end_in_fp64_ground_states():
+    # FP32-phase descent, FP64 polish: every solve switches to FP64 and ends below Emin in FP64 (checked by the
+    # oracle); the same job alone and in a mix gives the same bits
+    specs = [stealthy(400, 2, 0.35, 41), stealthy(300, 3, 0.4, 42), stealthy(900, 2, 0.25, 43)]
+    solver = BatchSolver(max_slots=8, precision="mixed", polish=1e-8)
+    ids = [solver.submit(*sp, Emin=EMIN) for sp in specs]
+    solver.run(timeout=300)
+    for j, (cfg, k, v) in zip(ids, specs):
+        r = solver.result(j)
+        assert r["state"] == BatchSolver.DONE and r["energy"] < EMIN
+        assert 0 < r["switch_eval"] <= r["device_evaluations"]
+        Eo, _ = po.total_energy_forces(r["relative"], cfg.basis, k, v)
+        assert Eo < 10 * EMIN
+    alone = BatchSolver(max_slots=2, precision="mixed")
+    ja = alone.submit(*specs[1], Emin=EMIN)
+    alone.run(timeout=300)
+    ra, rm = alone.result(ja), solver.result(ids[1])
+    assert ra["energy"] == rm["energy"] and ra["switch_eval"] == rm["switch_eval"]
+    assert np.array_equal(ra["relative"], rm["relative"])
+
+
+def test_one_and_two_lanes_give_the_same_bits():
+    # jobs spread over two streams (E+F of one beside L-BFGS of the other) or on one stream: same results
+    specs = [stealthy(300, 2, 0.3, 51), stealthy(250, 3, 0.4, 52), stealthy(500, 2, 0.2, 53), stealthy(150, 3, 0.45, 54)]
+    out = []
+    for lanes in (1, 2):
+        solver = BatchSolver(max_slots=8, lanes=lanes)
+        ids = [solver.submit(*sp, Emin=EMIN) for sp in specs]
+        solver.run(timeout=300)
+        out.append([solver.result(j) for j in ids])
+    for a, b in zip(*out):
+        assert a["state"] == b["state"] == BatchSolver.DONE
+        assert a["energy"] == b["energy"] and a["evaluations"] == b["evaluations"]
+        assert np.array_equal(a["relative"], b["relative"])
