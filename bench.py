@@ -24,6 +24,11 @@ sys.path.insert(0, ROOT)
 
 FP64_VALU_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak (AMD data sheet)
 
+# HBM bytes per launch of the force kernel at the headline shape (64 x 3D N=2000, M=2724, tiles 24,0,128,32,1),
+# from rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by the library copy kernel's counter factors; the
+# passes, the commands and the factors are in profiles/counters_ef_lbfgs_batched64_3d_N2000_chi045.md
+FORCE_TRAFFIC_BYTES = {(3, 2000, 2724, 64): {"read": 57.6e6, "write": 24.2e6}}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -43,8 +48,9 @@ def parse():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one torch device copy of this size first (calibrates rocprof FETCH/WRITE_SIZE)")
     ap.add_argument("--graph", action="store_true", help="replay optimizer steps from a captured hipGraph")
-    ap.add_argument("--lanes", type=int, default=2, choices=[1, 2],
-                    help="solver streams: the L-BFGS kernels of one overlap the E+F kernels of the other")
+    ap.add_argument("--lanes", type=int, default=1, choices=[1, 2],
+                    help="solver streams (2: the L-BFGS kernels of one overlap the E+F kernels of the other; the "
+                         "kernels then share the GPU and their event durations no longer give a per-kernel roofline)")
     ap.add_argument("--no-autotune", action="store_true", help="skip the tile autotuner (default tiles)")
     ap.add_argument("--tiles", type=str, default=None,
                     help="seg,rho_kernel,rho_chunk,force_chunk,force_ppl: use these tiles instead of autotuning")
@@ -225,7 +231,8 @@ def main():
             "peak": FP64_VALU_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_VALU_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": (_force_traffic(d, N, M, args.batch) or {}).get("bytes_per_launch"),
+            "traffic_detail": _force_traffic(d, N, M, args.batch),
             "avg_launch_ms": force_ms / max(n_force, 1),
             "flops_per_pair": 4 * d + 2,
             "rho_kernel": {"kernel": "k_rho_lattice", "achieved": achieved_rho,
@@ -238,6 +245,14 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(proto, k, v, float(np.mean(obj_evals)), args.cpu_seconds)
     print(json.dumps(out), flush=True)
+
+
+def _force_traffic(d, N, M, batch):
+    t = FORCE_TRAFFIC_BYTES.get((d, N, M, batch))
+    if t is None:
+        return None
+    return {"bytes_per_launch": t["read"] + t["write"], "read": t["read"], "write": t["write"],
+            "source": "profiles/counters_ef_lbfgs_batched64_3d_N2000_chi045.md"}
 
 
 def _solve_all(solver, jobs, emin):

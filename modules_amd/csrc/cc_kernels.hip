@@ -432,6 +432,17 @@ __global__ void __launch_bounds__(kBlock) k_energy_final(const SysDesc *__restri
 // segment data is uniform across the wavefront (scalar loads).  Accumulates G_j = sum_k c_k n_kj with
 // c_k = (2V/vol)(Im rho cos - Re rho sin); then F = -sum_j G_j b_j in k_force_final.
 // ---------------------------------------------------------------------------------------------
+// Read-only tables that are the same for every lane of a wavefront (segment heads, lengths, force
+// coefficients) are read through constant-address-space pointers: the compiler may then use scalar loads
+// (one fetch per wavefront, operands from SGPRs) although the kernel writes other global memory.
+template <class T>
+using ConstPtr = const __attribute__((address_space(4))) T *;
+template <class T>
+__device__ __forceinline__ ConstPtr<T> as_const(const T *p)
+{
+    return (ConstPtr<T>)p;
+}
+
 template <int SEG, int PPL, bool MIXED>
 __device__ __forceinline__ void force_lattice_body(const SysDesc &S, const WorkItem &it, const double2 *ltab,
                                                    const float2 *ltab32, const double4 (&p)[PPL],
@@ -448,8 +459,9 @@ __device__ __forceinline__ void force_lattice_body(const SysDesc &S, const WorkI
         sty[q] = (Real)st[q].y;
     }
     for (int sg = s0; sg < s1; sg++) {
-        const int4 h = S.seg_head[sg];
-        const int len = S.seg_len[sg];
+        const ConstPtr<int> hp = as_const(reinterpret_cast<const int *>(S.seg_head)) + 4 * (int64_t)sg;
+        const int4 h = make_int4(hp[0], hp[1], hp[2], hp[3]);
+        const int len = as_const(S.seg_len)[sg];
         const double h0 = (double)h.x, h1 = (double)h.y, h2 = (double)h.z, h3 = (double)h.w;
         const double hl = (d == 1) ? h0 : (d == 2) ? h1 : (d == 3) ? h2 : h3;
         Real sn[PPL], cs[PPL], acc[PPL], accl[PPL];
@@ -477,13 +489,13 @@ __device__ __forceinline__ void force_lattice_body(const SysDesc &S, const WorkI
                 for (int t = t0; t < t0 + 4; t++) {
                     Real ax, ay;
                     if (MIXED) {
-                        const float2 ab = reinterpret_cast<const float2 *>(S.coef32)[(int64_t)sg * SEG + t];
-                        ax = ab.x;
-                        ay = ab.y;
+                        const ConstPtr<float> ab = as_const(S.coef32) + 2 * ((int64_t)sg * SEG + t);
+                        ax = ab[0];
+                        ay = ab[1];
                     } else {
-                        const double2 ab = reinterpret_cast<const double2 *>(S.coef)[(int64_t)sg * SEG + t];
-                        ax = ab.x;
-                        ay = ab.y;
+                        const ConstPtr<double> ab = as_const(S.coef) + 2 * ((int64_t)sg * SEG + t);
+                        ax = ab[0];
+                        ay = ab[1];
                     }
 #pragma unroll
                     for (int q = 0; q < PPL; q++) {
