@@ -247,6 +247,14 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def _tune(args, cfg, k):
+    """Autotuned tiles for one configuration of this shape (unless --no-autotune)."""
+    if args.no_autotune:
+        return None
+    from modules_amd import autotune
+    return autotune.autotune(cfg.NumParticle(), cfg.GetDimension(), k=k, basis=cfg.basis, batch=1, repeats=2)
+
+
 def _force_traffic(d, N, M, batch):
     t = FORCE_TRAFFIC_BYTES.get((d, N, M, batch))
     if t is None:
@@ -327,12 +335,14 @@ def workload_large2d(args):
     from modules_amd.optimize import BatchSolver
     cfg = RandomConfiguration(4, 20000, 2)
     k = GetKs(cfg, getK(0.4, 1.0, 2))
+    tiles = _tune(args, cfg, k)
     solver = BatchSolver(max_slots=1, check_interval=args.check_interval)
     job = solver.submit(cfg, k, np.ones(len(k)), MaxStep=args.maxstep, Emin=args.emin)
     res, wall = _solve_all(solver, [job], args.emin)
     r = res[0]
     return {"metric": "stealthy ground states/s at E<1e-18", "value": (1.0 if r["energy"] < args.emin else 0.0) / wall,
-            "unit": "ground_states/s", "workload": f"2D N=20000 chi=0.4 (M={len(k)}), FP64", "energy": r["energy"],
+            "unit": "ground_states/s", "workload": f"2D N=20000 chi=0.4 (M={len(k)}), FP64", "tiles": tiles,
+            "energy": r["energy"],
             "objective_evals": r["evaluations"], "device_evals": r["device_evaluations"], "wall_s": wall,
             "ef_evals_per_s": r["device_evaluations"] / wall, "dtype": "f64"}
 
@@ -346,6 +356,7 @@ def workload_mixed2d(args):
     cfg = RandomConfiguration(4, 20000, 2)
     k = GetKs(cfg, getK(0.4, 1.0, 2))
     v = np.ones(len(k))
+    tiles = _tune(args, cfg, k)
     pot = ShiftedCCPotential(2)
     for kk in k:
         pot.AddConstraint(kk, [1.0])
@@ -355,7 +366,7 @@ def workload_mixed2d(args):
     pot.SetConfiguration(cfg)
     E32, F32 = pot.EnergyAndAllForce()
     out = {"metric": "stealthy ground states/s at E<1e-18", "unit": "ground_states/s",
-           "workload": f"2D N=20000 chi=0.4 (M={len(k)}): mixed precision against FP64",
+           "workload": f"2D N=20000 chi=0.4 (M={len(k)}): mixed precision against FP64", "tiles": tiles,
            "start_config_error": {"dE_over_E": abs(E32 - E64) / abs(E64),
                                   "max_dF_over_max_F": float(np.max(np.abs(F32 - F64)) / np.max(np.abs(F64)))}}
     for prec in ("fp64", "mixed"):

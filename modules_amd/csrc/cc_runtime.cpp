@@ -148,11 +148,8 @@ Tiles DefaultTiles(int d, int64_t N, int64_t M)
     if (LookupTiles(d, N, M, tuned)) return tuned;
     Tiles t;
     t.seg = 16;
-    // rho: about 16 particle chunks per system, 64..512 particles each
-    int64_t c = (N + 15) / 16;
-    c = ((c + 31) / 32) * 32;
-    t.rho_chunk = (int)std::min<int64_t>(512, std::max<int64_t>(64, c));
-    t.force_chunk = 0;  // resolved in Prepare from the segment count
+    t.rho_chunk = 0;    // resolved in Prepare from the pair count and the segment count
+    t.force_chunk = 0;  // likewise
     t.rho_kernel = 0;
     t.force_ppl = 2;
     return t;
@@ -301,7 +298,6 @@ void System::Prepare(hipStream_t stream)
     used.seg = tiles.seg > 0 ? tiles.seg : def.seg;
     if (used.seg != 4 && used.seg != 8 && used.seg != 16 && used.seg != 24 && used.seg != 32)
         throw Error(kErrArg, "segment length must be 4, 8, 16, 24 or 32");
-    used.rho_chunk = tiles.rho_chunk > 0 ? std::min(tiles.rho_chunk, 512) : def.rho_chunk;
     used.rho_kernel = tiles.rho_kernel >= 0 ? tiles.rho_kernel : def.rho_kernel;
     if (precision == 1) used.rho_kernel = 0;  // the mixed-precision body lives in the grouped-power kernel
     used.force_ppl = tiles.force_ppl > 0 ? tiles.force_ppl : (def.force_ppl > 0 ? def.force_ppl : 2);
@@ -315,16 +311,32 @@ void System::Prepare(hipStream_t stream)
     const int64_t nslots = layout->nslots();
     const int64_t nunits = lattice ? layout->plan.nseg : M;  // force-pass loop units
 
-    // chunking of the two contractions
+    // Chunking of the two contractions.  Untuned shapes get enough blocks for their work: one block per ~2^18
+    // (particle, constraint) pairs, at least 48 rho blocks and 24 force blocks per system and at most 4096, so a
+    // lone large system fills the chip and a small one in a big batch does not split its sums needlessly.  The
+    // rule reads only the system's own shape (so do its bits).
+    const int64_t pairs = std::max<int64_t>(N, 1) * std::max<int64_t>(M, 1);
+    const int rc = tiles.rho_chunk > 0 ? std::min(tiles.rho_chunk, 512) : def.rho_chunk;
+    if (rc > 0) {
+        used.rho_chunk = rc;
+    } else {
+        const int64_t want = std::max<int64_t>(48, std::min<int64_t>(4096, pairs >> 18));
+        const int64_t per = lattice ? (used.rho_kernel == 1 ? kBlock : kRhoBlock) : kBlock;
+        const int64_t sblocks = std::max<int64_t>(1, ((lattice ? layout->plan.nseg : M) + per - 1) / per);
+        const int64_t pch = std::max<int64_t>(1, (want + sblocks - 1) / sblocks);
+        int64_t c = (N + pch - 1) / pch;
+        c = ((c + 31) / 32) * 32;
+        used.rho_chunk = (int)std::min<int64_t>(512, std::max<int64_t>(64, c));
+    }
     pchunks = (int)std::max<int64_t>(1, (N + used.rho_chunk - 1) / used.rho_chunk);
     const int64_t pblocks = std::max<int64_t>(1, (N + kBlock * used.force_ppl - 1) / (kBlock * used.force_ppl));
     const int fc = tiles.force_chunk > 0 ? tiles.force_chunk : def.force_chunk;
     if (fc > 0) {
         used.force_chunk = fc;
     } else {
-        // about 16 blocks per system in the force pass, at least 4 units per chunk
-        int64_t want = std::max<int64_t>(1, (16 + pblocks - 1) / pblocks);
-        want = std::min<int64_t>(want, std::max<int64_t>(1, nunits / 4));
+        const int64_t want_blocks = std::max<int64_t>(24, std::min<int64_t>(4096, pairs >> 18));
+        int64_t want = std::max<int64_t>(1, (want_blocks + pblocks - 1) / pblocks);  // chunks
+        want = std::min<int64_t>(want, std::max<int64_t>(1, nunits / 4));           // at least 4 units each
         used.force_chunk = (int)std::max<int64_t>(1, (nunits + want - 1) / want);
     }
     kchunks = (int)std::max<int64_t>(1, (nunits + used.force_chunk - 1) / used.force_chunk);
