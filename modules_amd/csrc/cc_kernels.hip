@@ -117,7 +117,7 @@ __device__ __forceinline__ void rho_lattice_body(const SysDesc &S, const WorkIte
             pw[1] = cmul(pw[0], pw[0]);
             pw[2] = cmul(pw[1], pw[0]);
             pw[3] = cmul(pw[1], pw[1]);
-            if (G == 8) {
+            if constexpr (G == 8) {
                 pw[4] = cmul(pw[3], pw[0]);
                 pw[5] = cmul(pw[3], pw[1]);
                 pw[6] = cmul(pw[3], pw[2]);
@@ -482,21 +482,23 @@ __device__ __forceinline__ void force_lattice_body(const SysDesc &S, const WorkI
             acc[q] = 0;
             accl[q] = 0;
         }
+        // coefficients of a group of 4 slots are scalar loads; the next group's are issued before the current
+        // group is computed, so the wait for them overlaps a group of FMAs (scalar loads complete out of order,
+        // so every wait is for all of them)
+        const ConstPtr<Real> cf = as_const(MIXED ? (const Real *)S.coef32 : (const Real *)S.coef) + 2 * (int64_t)sg * SEG;
+        Real cur[8], nxt[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) cur[u] = cf[u];
 #pragma unroll
         for (int t0 = 0; t0 < SEG; t0 += 4) {
             if (t0 < len) {  // segment data is uniform across the wavefront
+                if (t0 + 4 < SEG && t0 + 4 < len) {
+#pragma unroll
+                    for (int u = 0; u < 8; u++) nxt[u] = cf[2 * (t0 + 4) + u];
+                }
 #pragma unroll
                 for (int t = t0; t < t0 + 4; t++) {
-                    Real ax, ay;
-                    if (MIXED) {
-                        const ConstPtr<float> ab = as_const(S.coef32) + 2 * ((int64_t)sg * SEG + t);
-                        ax = ab[0];
-                        ay = ab[1];
-                    } else {
-                        const ConstPtr<double> ab = as_const(S.coef) + 2 * ((int64_t)sg * SEG + t);
-                        ax = ab[0];
-                        ay = ab[1];
-                    }
+                    const Real ax = cur[2 * (t - t0)], ay = cur[2 * (t - t0) + 1];
 #pragma unroll
                     for (int q = 0; q < PPL; q++) {
                         const Real c = ax * cs[q] - ay * sn[q];
@@ -508,6 +510,8 @@ __device__ __forceinline__ void force_lattice_body(const SysDesc &S, const WorkI
                         sn[q] = ns;
                     }
                 }
+#pragma unroll
+                for (int u = 0; u < 8; u++) cur[u] = nxt[u];
             }
         }
         // sum_t (hl + t) c_t = hl * sum c + sum t c; components before the last axis are constant along the segment

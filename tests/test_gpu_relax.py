@@ -216,3 +216,23 @@ def test_one_and_two_lanes_give_the_same_bits():
         assert a["state"] == b["state"] == BatchSolver.DONE
         assert a["energy"] == b["energy"] and a["evaluations"] == b["evaluations"]
         assert np.array_equal(a["relative"], b["relative"])
+
+
+def test_serving_stream_results_match_alone():
+    # continuous batching: requests arrive while others are on the GPU and join the running batch; each result is
+    # the same bits as the same request solved alone
+    from modules_amd.serving import make_stream, serve
+    reqs = make_stream(6, rate=200.0, seed=7, median_N=300, N_clip=(100, 700))
+    reqs, wall = serve(reqs, Emin=EMIN, max_slots=4, timeout=300)
+    assert all(r.result["state"] == BatchSolver.DONE and r.result["energy"] < EMIN for r in reqs)
+    # some requests were on the GPU together (one arrived before another finished)
+    assert any(a.submitted < b.done and b.submitted < a.done for a in reqs for b in reqs if a is not b)
+    for r in reqs[:3]:
+        cfg = RandomConfiguration(r.seed, r.N, r.d)
+        k = GetKs(cfg, getK(r.chi, 1.0, r.d))
+        alone = BatchSolver(max_slots=1)
+        j = alone.submit(cfg, k, np.ones(len(k)), epsilon=r.epsilon, sigma=r.sigma, Emin=EMIN)
+        alone.run(timeout=300)
+        a = alone.result(j)
+        assert a["energy"] == r.result["energy"] and a["evaluations"] == r.result["evaluations"]
+        assert np.array_equal(a["relative"], r.result["relative"])
