@@ -134,12 +134,13 @@ __device__ __forceinline__ void rho_lattice_body(const SysDesc &S, const WorkIte
         if (!MIXED) {
             for (int j = 0; j < nsub; j++) {
                 const double4 p = L.pos[j];
-                const double u = __builtin_fma(h0, p.x, __builtin_fma(h1, p.y, __builtin_fma(h2, p.z, h3 * p.w)));
-                double sn, cs;
-                sincos_rev_tab(u, L.tab, sn, cs);
+                // the group powers are read before the sincos, whose arithmetic covers their LDS latency
                 double2 P[G];
 #pragma unroll
                 for (int b = 0; b < G; b++) P[b] = L.pow[j * G + b];
+                const double u = __builtin_fma(h0, p.x, __builtin_fma(h1, p.y, __builtin_fma(h2, p.z, h3 * p.w)));
+                double sn, cs;
+                sincos_rev_tab(u, L.tab, sn, cs);
 #pragma unroll
                 for (int t0 = 0; t0 < SEG; t0 += G) {
                     if (t0 < wlen) {
@@ -168,12 +169,12 @@ __device__ __forceinline__ void rho_lattice_body(const SysDesc &S, const WorkIte
                 const int je = min(nsub, jf + kFlush);
                 for (int j = jf; j < je; j++) {
                     const double4 p = L.pos[j];
-                    const double u = __builtin_fma(h0, p.x, __builtin_fma(h1, p.y, __builtin_fma(h2, p.z, h3 * p.w)));
-                    float sn, cs;
-                    sincos_rev_tab32((float)(u - __builtin_rint(u)), L.tab32, sn, cs);
                     float2 P[G];
 #pragma unroll
                     for (int b = 0; b < G; b++) P[b] = pow32[j * G + b];
+                    const double u = __builtin_fma(h0, p.x, __builtin_fma(h1, p.y, __builtin_fma(h2, p.z, h3 * p.w)));
+                    float sn, cs;
+                    sincos_rev_tab32((float)(u - __builtin_rint(u)), L.tab32, sn, cs);
 #pragma unroll
                     for (int t0 = 0; t0 < SEG; t0 += G) {
                         if (t0 < wlen) {

@@ -72,6 +72,7 @@ __device__ __forceinline__ void sincos_rev_tab(double u, const double2 *tab, dou
 {
     const double f = u - __builtin_rint(u);
     const double jd = __builtin_rint(64.0 * f);
+    const double2 t = tab[((int)jd) & 63];  // read first: the polynomials below cover its latency
     const double fl = __builtin_fma(-0.015625, jd, f);  // exact (Sterbenz)
     const double x = fl * CC_TWO_PI_HI;
     const double x2 = x * x;
@@ -82,7 +83,6 @@ __device__ __forceinline__ void sincos_rev_tab(double u, const double2 *tab, dou
     pc = __builtin_fma(pc, x2, 4.1666666666666666667e-02);
     pc = __builtin_fma(pc, x2, -0.5);
     const double cl = __builtin_fma(pc, x2, 1.0);
-    const double2 t = tab[((int)jd) & 63];
     c = __builtin_fma(t.x, cl, -t.y * sl);
     s = __builtin_fma(t.y, cl, t.x * sl);
 }
@@ -99,12 +99,12 @@ __device__ __forceinline__ void sincos_table_fill32(float2 *tab32, const double2
 __device__ __forceinline__ void sincos_rev_tab32(float f, const float2 *tab, float &s, float &c)
 {
     const float jd = __builtin_rintf(64.0f * f);
+    const float2 t = tab[((int)jd) & 63];
     const float fl = __builtin_fmaf(-0.015625f, jd, f);  // exact (Sterbenz)
     const float x = fl * 6.28318530717958648f;
     const float x2 = x * x;
     const float sl = __builtin_fmaf(x * x2, __builtin_fmaf(x2, 8.33333333e-3f, -1.66666667e-1f), x);
     const float cl = __builtin_fmaf(x2, __builtin_fmaf(x2, 4.16666667e-2f, -0.5f), 1.0f);
-    const float2 t = tab[((int)jd) & 63];
     c = __builtin_fmaf(t.x, cl, -t.y * sl);
     s = __builtin_fmaf(t.y, cl, t.x * sl);
 }
