@@ -418,6 +418,7 @@ __global__ void __launch_bounds__(kWideBlock) k_lbfgs_solve(const SysDesc *__res
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int q = threadIdx.x; q < 5 * k; q += kWideBlock) {
         double t = 0.0;
+#pragma unroll 8
         for (int c = 0; c < R.nchunks; c++) t += R.part1[(size_t)c * H * 5 + q];
         dots[q] = t;
     }
@@ -431,12 +432,36 @@ __global__ void __launch_bounds__(kWideBlock) k_lbfgs_solve(const SysDesc *__res
         gyy[pj * mf + p1] = dots[5 * j + 4];
     }
     __syncthreads();
-    for (int q = threadIdx.x; q < k * k; q += kWideBlock) {
-        const int i = q / k, j = q - i * k;
-        if (j > i) continue;
-        const int pi = (head + i) % mf, pj = (head + j) % mf;
-        gsy_w[tri(i, j)] = gsy[pi * mf + pj];
-        gyy_w[tri(i, j)] = gyy[pi * mf + pj];
+    // the window's lower triangle, k(k+1)/2 <= 2080 entries: a fixed trip count per thread so that all of its
+    // loads are in flight at once
+    {
+        constexpr int kPer = (H * (H + 1) / 2 + kWideBlock - 1) / kWideBlock;
+        const int ntri = k * (k + 1) / 2;
+        double vs[kPer], vy[kPer];
+        int at[kPer];
+#pragma unroll
+        for (int m = 0; m < kPer; m++) {
+            const int q = threadIdx.x + m * kWideBlock;
+            at[m] = -1;
+            vs[m] = vy[m] = 0.0;
+            if (q < ntri) {
+                // q = i(i+1)/2 + j with j <= i
+                int i = (int)((sqrt(8.0 * q + 1.0) - 1.0) * 0.5);
+                while ((i + 1) * (i + 2) / 2 <= q) i++;
+                while (i * (i + 1) / 2 > q) i--;
+                const int j = q - i * (i + 1) / 2;
+                const int pi = (head + i) % mf, pj = (head + j) % mf;
+                vs[m] = gsy[pi * mf + pj];
+                vy[m] = gyy[pi * mf + pj];
+                at[m] = q;
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < kPer; m++)
+            if (at[m] >= 0) {
+                gsy_w[at[m]] = vs[m];
+                gyy_w[at[m]] = vy[m];
+            }
     }
     __syncthreads();
     // The two recursions are triangular solves; wavefront 0 runs them as column sweeps.  Lane i holds the
