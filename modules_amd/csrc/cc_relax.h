@@ -92,6 +92,9 @@ struct SolverLane {
                                // when half the slots are idle)
     SysDesc *h_desc = nullptr; // pinned host copy of the descriptor table (harvest reads)
     size_t h_desc_cap = 0;
+    hipEvent_t desc_ev = nullptr;  // recorded after the descriptor read of a harvest
+    bool desc_pending = false;
+    size_t marks_at_read = 0;      // timing marks recorded before that read (complete once it is)
     DevArray<RelaxDesc> d_rdesc;
     DevArray<WorkItem> d_wide;
     int nwide = 0;
@@ -108,6 +111,7 @@ struct SolverLane {
     ~SolverLane()
     {
         if (h_desc) (void)hipHostFree(h_desc);
+        if (desc_ev) (void)hipEventDestroy(desc_ev);
     }
 };
 
@@ -155,7 +159,10 @@ private:
     void Admit();
     void Rebuild(SolverLane &L);
     void LaunchBurst(int burst);
-    void Harvest();
+    void EnqueueDescRead();  // queue each lane's descriptor-table read behind its last burst
+    void FinishHarvest();    // wait for those reads, retire the jobs they show finished
+    void SyncLanes();
+    bool NeedsRefresh() const;
     static int64_t EstimateBytes(const Job &j);
 
     int max_slots_;
@@ -170,6 +177,7 @@ private:
     std::vector<Job *> resident_;
     std::map<int64_t, std::unique_ptr<Job>> jobs_;
     std::vector<std::unique_ptr<SolverLane>> lanes_;
+    hipStream_t copy_stream_ = nullptr;  // result read-backs of finished jobs, beside the next burst
     double t0_;
 };
 

@@ -585,6 +585,7 @@ void launch_relax_step(hipStream_t stream, SysDesc *sys, const RelaxDesc *rd, in
 Solver::Solver(int max_slots, int64_t hbm_budget) : max_slots_(max_slots), budget_(hbm_budget)
 {
     t0_ = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+    CC_HIP(hipStreamCreateWithFlags(&copy_stream_, hipStreamNonBlocking));
     SetLanes(2);
 }
 
@@ -594,6 +595,10 @@ Solver::~Solver()
         (void)hipStreamSynchronize(l->batch.stream);
         l->DropGraph();
         l->batch.systems.clear();
+    }
+    if (copy_stream_) {
+        (void)hipStreamSynchronize(copy_stream_);
+        (void)hipStreamDestroy(copy_stream_);
     }
 }
 
@@ -798,24 +803,34 @@ void Solver::Rebuild(SolverLane &L)
     L.dirty = false;
 }
 
-void Solver::Harvest()
+void Solver::EnqueueDescRead()
 {
-    // one read of each lane's descriptor table (into pinned memory) tells which jobs finished (kSysActive
+    // one read of each lane's descriptor table (into pinned memory) will tell which jobs finished (kSysActive
     // cleared) and the current rescale of the others (the step kernel changes it at pass boundaries)
     for (auto &l : lanes_) {
         SolverLane &L = *l;
-        if (!L.jobs.empty())
-            CC_HIP(hipMemcpyAsync(L.h_desc, L.batch.d_desc.get(), L.slots.size() * sizeof(SysDesc),
-                                  hipMemcpyDeviceToHost, L.batch.stream));
+        if (L.jobs.empty()) continue;
+        if (!L.desc_ev) CC_HIP(hipEventCreateWithFlags(&L.desc_ev, hipEventDisableTiming));
+        CC_HIP(hipMemcpyAsync(L.h_desc, L.batch.d_desc.get(), L.slots.size() * sizeof(SysDesc), hipMemcpyDeviceToHost,
+                              L.batch.stream));
+        CC_HIP(hipEventRecord(L.desc_ev, L.batch.stream));
+        L.desc_pending = true;
+        L.marks_at_read = L.batch.TimingMarks();
     }
+}
+
+void Solver::FinishHarvest()
+{
     std::vector<Job *> fin;
     for (auto &l : lanes_) {
         SolverLane &L = *l;
-        if (L.jobs.empty()) continue;
-        CC_HIP(hipStreamSynchronize(L.batch.stream));
-        if (L.batch.timing) L.batch.CollectTiming();
+        if (!L.desc_pending) continue;
+        L.desc_pending = false;
+        CC_HIP(hipEventSynchronize(L.desc_ev));
+        if (L.batch.timing) L.batch.CollectTiming(L.marks_at_read);
         std::vector<Job *> keep;
         size_t idle = 0;
+        bool any = false;
         for (size_t q = 0; q < L.slots.size(); q++) {
             Job *j = L.slots[q];
             if (!j) {
@@ -831,25 +846,27 @@ void Solver::Harvest()
                 fin.push_back(j);
                 L.slots[q] = nullptr;
                 idle++;
+                any = true;
                 L.work -= j->N * ((int64_t)j->v.size() + 2 * j->mf);
             }
         }
         L.jobs.swap(keep);
         if (2 * idle > L.slots.size()) L.dirty = true;  // compact the work lists
+        // a finished job's state and coordinates are final once the burst before the read has run; the next
+        // burst, already queued, skips the job, so its results are read on the copy stream beside that burst
+        if (any) CC_HIP(hipStreamWaitEvent(copy_stream_, L.desc_ev, 0));
     }
     if (fin.empty()) return;
-    // final states and coordinates of every finished job: copies queued back to back per lane, then one
-    // synchronisation per lane
     std::vector<std::vector<double>> xs(fin.size());
     for (size_t i = 0; i < fin.size(); i++) {
         Job *j = fin[i];
-        hipStream_t st = lanes_[j->lane]->batch.stream;
         const int64_t n = j->N * j->d;
-        CC_HIP(hipMemcpyAsync(&j->host_state, j->st.get(), sizeof(RelaxState), hipMemcpyDeviceToHost, st));
+        CC_HIP(hipMemcpyAsync(&j->host_state, j->st.get(), sizeof(RelaxState), hipMemcpyDeviceToHost, copy_stream_));
         xs[i].resize(n);
-        if (n > 0) CC_HIP(hipMemcpyAsync(xs[i].data(), j->sys->x.get(), n * sizeof(double), hipMemcpyDeviceToHost, st));
+        if (n > 0)
+            CC_HIP(hipMemcpyAsync(xs[i].data(), j->sys->x.get(), n * sizeof(double), hipMemcpyDeviceToHost, copy_stream_));
     }
-    for (auto &l : lanes_) CC_HIP(hipStreamSynchronize(l->batch.stream));
+    CC_HIP(hipStreamSynchronize(copy_stream_));
     for (size_t i = 0; i < fin.size(); i++) {
         Job *j = fin[i];
         const RelaxState &hs = j->host_state;
@@ -865,6 +882,8 @@ void Solver::Harvest()
         j->switch_eval = hs.switch_eval;
         j->t_done = Now();
         j->state = Job::kDone;
+        // the buffers go back to the pool; kernels still running skip this job (its flag is clear) and the pool
+        // hands them out again only at an admission, which first drains every lane
         j->sys.reset();
         j->sdir.Release();
         j->xo.Release();
@@ -881,6 +900,22 @@ void Solver::Harvest()
     for (Job *j : resident_)
         if (j->state == Job::kResident) keep.push_back(j);
     resident_.swap(keep);
+}
+
+void Solver::SyncLanes()
+{
+    for (auto &l : lanes_) CC_HIP(hipStreamSynchronize(l->batch.stream));
+    for (auto &l : lanes_)
+        if (l->batch.timing) l->batch.CollectTiming();
+}
+
+bool Solver::NeedsRefresh() const
+{
+    if (!queue_.empty() && (int)resident_.size() < max_slots_ && bytes_in_use_ + queue_.front()->bytes <= budget_)
+        return true;
+    for (auto &l : lanes_)
+        if (l->dirty && !l->jobs.empty()) return true;
+    return false;
 }
 
 void Solver::LaunchBurst(int burst)
@@ -927,16 +962,32 @@ int Solver::Step(int evaluations)
         for (auto &l : lanes_)
             if (l->dirty && !l->jobs.empty()) Rebuild(*l);
     };
+    // The harvest is pipelined: each burst of `check_interval` steps is queued before the descriptor read of
+    // the previous burst is examined, so the GPU runs while the host retires finished jobs.  Work lists change
+    // (admission, compaction) only after every lane has drained.
+    SyncLanes();
     refresh();
     int done = 0;
-    while (done < evaluations && !resident_.empty()) {
-        const int burst = std::min(check_every_, evaluations - done);
-        LaunchBurst(burst);
-        CC_HIP(hipGetLastError());
-        done += burst;
-        Harvest();
-        refresh();
+    while (true) {
+        const bool launch = done < evaluations && !resident_.empty();
+        if (launch) {
+            const int burst = std::min(check_every_, evaluations - done);
+            LaunchBurst(burst);
+            CC_HIP(hipGetLastError());
+            done += burst;
+        }
+        FinishHarvest();  // the read queued behind the previous burst, if any
+        if (!launch) break;
+        if (NeedsRefresh()) {
+            EnqueueDescRead();
+            FinishHarvest();
+            SyncLanes();
+            refresh();
+            continue;
+        }
+        EnqueueDescRead();
     }
+    SyncLanes();
     return (int)(resident_.size() + queue_.size());
 }
 

@@ -437,31 +437,37 @@ Batch::~Batch()
     if (stream) (void)hipStreamDestroy(stream);
 }
 
-hipEvent_t Batch::NextEvent()
+int Batch::NextEvent()
 {
-    if (ev_used_ == ev_pool_.size()) {
-        hipEvent_t e;
-        CC_HIP(hipEventCreate(&e));
-        ev_pool_.push_back(e);
+    if (!ev_free_.empty()) {
+        const int i = ev_free_.back();
+        ev_free_.pop_back();
+        return i;
     }
-    return ev_pool_[ev_used_++];
+    hipEvent_t e = nullptr;
+    CC_HIP(hipEventCreate(&e));
+    ev_pool_.push_back(e);
+    return (int)ev_pool_.size() - 1;
 }
 
-void Batch::CollectTiming()
+void Batch::CollectTiming(size_t count)
 {
-    for (const auto &m : ev_marks_) {
+    count = std::min(count, ev_marks_.size());
+    for (size_t q = 0; q < count; q++) {
+        const Mark &m = ev_marks_[q];
         float ms = 0.f;
-        CC_HIP(hipEventElapsedTime(&ms, ev_pool_[m.second], ev_pool_[m.second + 1]));
-        if (m.first == 0) {
+        CC_HIP(hipEventElapsedTime(&ms, ev_pool_[m.e0], ev_pool_[m.e1]));
+        if (m.kind == 0) {
             t_rho_ms += ms;
             n_rho++;
         } else {
             t_force_ms += ms;
             n_force++;
         }
+        ev_free_.push_back(m.e0);
+        ev_free_.push_back(m.e1);
     }
-    ev_marks_.clear();
-    ev_used_ = 0;
+    ev_marks_.erase(ev_marks_.begin(), ev_marks_.begin() + count);
 }
 
 void Batch::Build()
@@ -589,27 +595,37 @@ void Batch::Enqueue(bool rho, bool forces)
         if (r_sc_part.n > 0)
             launch_softcore(stream, D, I + r_sc_part.off, r_sc_part.n, I + r_sc_sys.off, r_sc_sys.n, I + r_sc_cell.off,
                             r_sc_cell.n);
+        Mark mk{0, -1, -1};
         if (timing) {
-            ev_marks_.push_back({0, ev_used_});
-            CC_HIP(hipEventRecord(NextEvent(), stream));
+            mk.e0 = NextEvent();
+            CC_HIP(hipEventRecord(ev_pool_[mk.e0], stream));
         }
         for (const SegGroup &g : seg_groups) {
             launch_rho_lattice(stream, g.seg, D, I + g.rho.off, g.rho.n, g.rho_kernel);
             if (g.any_mixed) launch_rho_lattice(stream, g.seg, D, I + g.rho.off, g.rho.n, 2);
         }
         launch_rho_general(stream, D, I + r_rho_gen.off, r_rho_gen.n);
-        if (timing) CC_HIP(hipEventRecord(NextEvent(), stream));
+        if (timing) {
+            mk.e1 = NextEvent();
+            CC_HIP(hipEventRecord(ev_pool_[mk.e1], stream));
+            ev_marks_.push_back(mk);
+        }
         launch_rho_reduce(stream, D, I + r_red.off, r_red.n);
         launch_energy_final(stream, D, I + r_efin.off, r_efin.n);
     }
     if (forces) {
+        Mark mk{1, -1, -1};
         if (timing) {
-            ev_marks_.push_back({1, ev_used_});
-            CC_HIP(hipEventRecord(NextEvent(), stream));
+            mk.e0 = NextEvent();
+            CC_HIP(hipEventRecord(ev_pool_[mk.e0], stream));
         }
         for (const SegGroup &g : seg_groups) launch_force_lattice(stream, g.seg, D, I + g.force.off, g.force.n, g.force_ppl);
         launch_force_general(stream, D, I + r_f_gen.off, r_f_gen.n);
-        if (timing) CC_HIP(hipEventRecord(NextEvent(), stream));
+        if (timing) {
+            mk.e1 = NextEvent();
+            CC_HIP(hipEventRecord(ev_pool_[mk.e1], stream));
+            ev_marks_.push_back(mk);
+        }
         launch_force_final(stream, D, I + r_ffin.off, r_ffin.n);
     }
     CC_HIP(hipGetLastError());

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <climits>
 #include <cstdint>
 #include <memory>
 #include <stdexcept>
@@ -191,7 +192,10 @@ public:
     bool timing = false;
     double t_rho_ms = 0, t_force_ms = 0;
     int64_t n_rho = 0, n_force = 0;
-    void CollectTiming();  // after a stream sync: fold the recorded event pairs into the totals
+    // fold the first `count` recorded event pairs (default: all) into the totals; the caller knows they have
+    // completed (a stream sync, or an event recorded after them); their events are reused afterwards
+    void CollectTiming(size_t count = SIZE_MAX);
+    size_t TimingMarks() const { return ev_marks_.size(); }
 
     DevArray<SysDesc> d_desc;
     DevArray<WorkItem> d_items;
@@ -200,10 +204,14 @@ public:
     std::vector<SegGroup> seg_groups;
 
 private:
-    hipEvent_t NextEvent();
+    int NextEvent();  // index of a free event of the pool
     std::vector<hipEvent_t> ev_pool_;
-    size_t ev_used_ = 0;
-    std::vector<std::pair<int, size_t>> ev_marks_;  // (0 rho / 1 force, index of the start event)
+    std::vector<int> ev_free_;
+    struct Mark {
+        int kind;    // 0 rho, 1 force
+        int e0, e1;  // start and end events
+    };
+    std::vector<Mark> ev_marks_;
 };
 
 // kernel launchers (cc_kernels.hip)

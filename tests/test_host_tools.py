@@ -1,0 +1,58 @@
+"""Host-side pieces that need no GPU: the bench's one-GPU contract, the serving request stream, the work-aware
+parts of the Python mirror."""
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout.strip() == ""
+
+
+def test_bench_help_lists_the_workloads():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--help"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0
+    for w in ("batched", "single", "mixed256", "large2d", "mixed2d", "serve"):
+        assert w in r.stdout
+
+
+def test_request_stream_is_seeded_and_within_its_ranges():
+    from modules_amd.serving import make_stream
+    a = make_stream(200, rate=20.0, seed=5)
+    b = make_stream(200, rate=20.0, seed=5)
+    assert [(r.arrival, r.N, r.d, r.chi, r.seed) for r in a] == [(r.arrival, r.N, r.d, r.chi, r.seed) for r in b]
+    arr = np.array([r.arrival for r in a])
+    assert np.all(np.diff(arr) > 0)
+    assert 5.0 < arr[-1] < 15.0  # 200 arrivals at 20/s
+    assert all(100 <= r.N <= 50000 and r.d in (2, 3) and 0.1 <= r.chi <= 0.49 for r in a)
+    assert all(r.sigma == 0.4 and r.epsilon == 1.0 for r in a)
+    med = np.median([r.N for r in a])
+    assert 700 < med < 1400  # lognormal around a median of 1000
+
+
+def test_potential_terms_follow_the_constraints():
+    import ctypes as C
+    from modules_amd import _lib
+    h = C.c_void_p()
+    if _lib.lib().ccgpu_potential_create(2, C.byref(h)) != 0:
+        import pytest
+        pytest.skip("potential objects need a GPU here")
+    _lib.lib().ccgpu_potential_destroy(h)
+    from modules_amd import RepulsiveCCPotential
+    from modules_amd.optimize import _potential_terms
+    pot = RepulsiveCCPotential(2, 2.0, 0.4)
+    pot.AddConstraint([1.0, 0.0], [0.5])
+    pot.AddConstraint([0.0, 2.0], [1.5])
+    pot.Shift = 0.25
+    k, V, shift, e0, sigma = _potential_terms(pot)
+    assert np.array_equal(k, [[1.0, 0.0], [0.0, 2.0]]) and np.array_equal(V, [0.5, 1.5])
+    assert (shift, e0, sigma) == (0.25, 2.0, 0.4)
