@@ -404,27 +404,59 @@ OTHER_WORKLOADS = {"single": workload_single, "mixed256": workload_mixed256, "la
                    "mixed2d": workload_mixed2d, "serve": workload_serve}
 
 
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _time_ef(po, cfg, k, v, threads, seconds, max_n=2000, min_n=1):
+    """Warm E+F evaluations of one configuration on `threads` threads for about `seconds`; per-call times."""
+    po.total_energy_forces(cfg.relative, cfg.basis, k, v, nthreads=threads)  # warm
+    times = []
+    t0 = time.perf_counter()
+    while True:
+        t = time.perf_counter()
+        po.total_energy_forces(cfg.relative, cfg.basis, k, v, nthreads=threads)
+        times.append(time.perf_counter() - t)
+        if (time.perf_counter() - t0 >= seconds and len(times) >= min_n) or len(times) >= max_n:
+            break
+    return np.array(times)
+
+
 def cpu_baseline(cfg, k, v, evals_per_gs, seconds):
     """The oracle's E+F (the reference direct path restated in C with OpenMP, libm sin/cos per pair) timed on
     this host for a bounded number of evaluations of the same configuration; ground states/s = E+F/s divided
-    by the objective evaluations one ground state takes (the GPU run's mean; the optimizer is the same)."""
+    by the objective evaluations one ground state takes (the GPU run's mean; the optimizer is the same).  Also
+    a 1-thread sample and the reference's published sanity anchor (2D N=500 chi=0.2996 on 2 threads:
+    4.0535 ms per E+F, docs/finufft/reports/one-component-benchmark.md:37)."""
+    os.environ.setdefault("OMP_WAIT_POLICY", "ACTIVE")
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    from modules_amd import GetKs, RandomConfiguration, getK
     from oracle import pyoracle as po
 
     threads = min(16, os.cpu_count() or 1)
-    po.total_energy_forces(cfg.relative, cfg.basis, k[:8], v[:8], nthreads=threads)  # warm the library
-    n = 0
-    t0 = time.perf_counter()
-    while True:
-        po.total_energy_forces(cfg.relative, cfg.basis, k, v, nthreads=threads)
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= seconds or n >= 2000:
-            break
-    ef_per_s = n / el
+    t = _time_ef(po, cfg, k, v, threads, seconds)
+    ef_per_s = len(t) / t.sum()
+    t1 = _time_ef(po, cfg, k, v, 1, min(3.0, seconds / 5))
+    acfg = RandomConfiguration(0, 500, 2)
+    ak = GetKs(acfg, getK(0.2996, 1.0, 2))
+    ta = _time_ef(po, acfg, ak, np.ones(len(ak)), 2, 1.0, min_n=30)
     return {"value": ef_per_s / evals_per_gs, "unit": "ground_states/s", "cores": threads, "kind": "port",
-            "sample": f"{n} oracle E+F evaluations of one N={cfg.NumParticle()} d={cfg.GetDimension()} configuration "
-                      f"in {el:.1f} s ({ef_per_s:.3g} E+F/s) divided by {evals_per_gs:.0f} evaluations per ground state",
-            "ef_evals_per_s": ef_per_s}
+            "sample": f"{len(t)} oracle E+F evaluations of one N={cfg.NumParticle()} d={cfg.GetDimension()} "
+                      f"configuration on {threads} threads in {t.sum():.1f} s ({ef_per_s:.3g} E+F/s, median "
+                      f"{1e3 * np.median(t):.1f} ms) divided by {evals_per_gs:.0f} evaluations per ground state",
+            "ef_evals_per_s": ef_per_s, "median_ms_per_ef": 1e3 * float(np.median(t)),
+            "one_thread_median_ms_per_ef": 1e3 * float(np.median(t1)),
+            "anchor_2d_N500_chi0.2996_2threads": {"median_ms_per_ef": 1e3 * float(np.median(ta)), "samples": len(ta),
+                                                  "constraints": len(ak), "published_ms": 4.0535},
+            "cpu_model": _cpu_model(), "host_logical_cpus": os.cpu_count(),
+            "omp": {"OMP_WAIT_POLICY": os.environ.get("OMP_WAIT_POLICY"),
+                    "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND")}}
 
 
 if __name__ == "__main__":
