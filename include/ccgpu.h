@@ -136,6 +136,72 @@ int ccgpu_autotune_record(int dim, int64_t N, int64_t M, int segment_len, int rh
 int ccgpu_copy_probe(int64_t bytes, int repeats, double *ms_per_copy);
 
 /* ---------------------------------------------------------------------------------------------
+ * ccgpu_multi: the multi-species potential MultiStealthyPotential (apps/CCOptimization/MultiStealthyPotential.h:
+ * 117-478, MultiStealthyPotential.cpp:139-985) -- one ShiftedCCPotential per species over the species' particles
+ * plus a soft-core contact term with per-particle radii:
+ *   E   = u0 sum_s (N_s / N) E_s + e0 sum_{i<j, |r| < r_i + r_j} (1 - |r| / (r_i + r_j))^2
+ *   F_i = contact force + (u0 N_s / N) x (species CC force)
+ * Particle indices are the caller's; species index lists keep assignment order, as species_constraints[s].idx.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct ccgpu_multi ccgpu_multi;
+
+/* MultiStealthyPotential(dimension, epsilon) -- MultiStealthyPotential.h:318-332: e0 = epsilon, contact on when
+ * |epsilon| > 1e-10 (fixed at creation, as WithRepulsion); u0 = 1. */
+int ccgpu_multi_create(int dim, double epsilon, ccgpu_multi **out);
+int ccgpu_multi_destroy(ccgpu_multi *p);
+
+/* SetNumSpecies(num) -- MultiStealthyPotential.h:355-360: appends num empty species. */
+int ccgpu_multi_set_num_species(ccgpu_multi *p, int64_t num);
+int ccgpu_multi_num_species(const ccgpu_multi *p, int64_t *num);
+
+/* SetSpeciesOfParticle(i, species) -- MultiStealthyPotential.cpp:139-170: moves particle i to the end of the
+ * species' index list (no-op if already there).  Needs a bound configuration. */
+int ccgpu_multi_set_species_of_particle(ccgpu_multi *p, int64_t i, int64_t species);
+/* idx2species[].species for every particle (N int32, -1 unassigned) -- MultiStealthyPotential.h:143. */
+int ccgpu_multi_species_of(const ccgpu_multi *p, int32_t *species);
+/* species_constraints[species].idx -- MultiStealthyPotential.h:70: writes min(cap, count) indices. */
+int ccgpu_multi_species_indices(const ccgpu_multi *p, int64_t species, int64_t cap, int64_t *idx, int64_t *count);
+
+/* AddConstraint(species, k, {V}) -- MultiStealthyPotential.cpp:176-180 (k: dim doubles, absolute). */
+int ccgpu_multi_add_constraint(ccgpu_multi *p, int64_t species, const double *k, double V);
+/* Replace a species' constraint vector: k M x dim, V M. */
+int ccgpu_multi_set_species_constraints(ccgpu_multi *p, int64_t species, int64_t M, const double *k, const double *V);
+int ccgpu_multi_num_constraints(const ccgpu_multi *p, int64_t species, int64_t *M);
+
+/* public members u0, e0 -- MultiStealthyPotential.h:306-307. */
+int ccgpu_multi_set_scales(ccgpu_multi *p, double u0, double e0);
+
+/* SetConfiguration(Position) -- MultiStealthyPotential.cpp:236-278.  The first call leaves every particle
+ * unassigned with radius 0.5 and contact cutoff 1; later calls keep species and radii (same N required). */
+int ccgpu_multi_set_configuration(ccgpu_multi *p, const double *basis, int64_t N, const double *relative);
+/* SetSecureRadii(radii) -- MultiStealthyPotential.cpp:448-458: finite positive radii, cutoff 2.01 max r. */
+int ccgpu_multi_set_radii(ccgpu_multi *p, int64_t N, const double *radii);
+/* SetPacking(packing) -- MultiStealthyPotential.cpp:461-489: positions and radii, cutoff 2.01 max r. */
+int ccgpu_multi_set_packing(ccgpu_multi *p, const double *basis, int64_t N, const double *relative, const double *radii);
+int ccgpu_multi_get_radii(const ccgpu_multi *p, double *radii);
+/* GetRc() -- MultiStealthyPotential.h:431. */
+int ccgpu_multi_get_rc(const ccgpu_multi *p, double *rc);
+
+/* Energy() -- MultiStealthyPotential.cpp:682-748. */
+int ccgpu_multi_energy(ccgpu_multi *p, double *E);
+/* Force(result, i) -- MultiStealthyPotential.cpp:763-797.  F: dim doubles. */
+int ccgpu_multi_force(ccgpu_multi *p, int64_t i, double *F);
+/* AllForce(results) -- MultiStealthyPotential.cpp:881-930.  F: N x dim. */
+int ccgpu_multi_all_force(ccgpu_multi *p, double *F);
+/* EnergyAndAllForce(results) -- MultiStealthyPotential.cpp:932-984: Energy() followed by AllForce(). */
+int ccgpu_multi_energy_and_all_force(ccgpu_multi *p, double *E, double *F);
+/* GetRho(species) + species_constraints[species].CCPotential->RhoReal / RhoImag -- MultiStealthyPotential.cpp:
+ * 184-190: the species' collective densities over its particles (re, im: M_s doubles each). */
+int ccgpu_multi_get_rho(ccgpu_multi *p, int64_t species, double *re, double *im);
+/* species_constraints[species].CCPotential->Energy() -- the species' own CC energy E_s. */
+int ccgpu_multi_species_energy(ccgpu_multi *p, int64_t species, double *E);
+
+/* RelaxStructure_NLOPT[_Emin] with this potential (the relax of GetMultiCCO's ground mode,
+ * apps/CCOptimization/main.cpp:1565-1590 via CollectiveCoordinateMultiRun), as ccgpu_potential_relax. */
+int ccgpu_multi_relax(ccgpu_multi *p, double *relative, int maxstep, double Emin, int history, double *energy,
+                      int *evaluations, int *status);
+
+/* ---------------------------------------------------------------------------------------------
  * L-BFGS (Luksan's PLIS, the algorithm of NLopt's NLOPT_LD_LBFGS) on the host
  * ------------------------------------------------------------------------------------------- */
 

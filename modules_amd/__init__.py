@@ -6,3 +6,4 @@ reference's Potential / StructureOptimization interface over that ABI.
 """
 from .potential import (Configuration, GetKs, GetUnitCubicBox, KPointConstraint, RandomConfiguration,  # noqa: F401
                         RepulsiveCCPotential, ShiftedCCPotential, get_chi, getK, pi)
+from .multi import MultiStealthyPotential, SpherePacking  # noqa: F401

@@ -81,6 +81,30 @@ SIGNATURES = {
     "ccgpu_autotune_record": (C.c_int, [C.c_int, C.c_int64, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int,
                                         C.c_int]),
     "ccgpu_solver_timing": (C.c_int, [vp, dp, dp, i64p, i64p, C.c_int]),
+    "ccgpu_multi_create": (C.c_int, [C.c_int, C.c_double, C.POINTER(vp)]),
+    "ccgpu_multi_destroy": (C.c_int, [vp]),
+    "ccgpu_multi_set_num_species": (C.c_int, [vp, C.c_int64]),
+    "ccgpu_multi_num_species": (C.c_int, [vp, i64p]),
+    "ccgpu_multi_set_species_of_particle": (C.c_int, [vp, C.c_int64, C.c_int64]),
+    "ccgpu_multi_species_of": (C.c_int, [vp, ip32]),
+    "ccgpu_multi_species_indices": (C.c_int, [vp, C.c_int64, C.c_int64, i64p, i64p]),
+    "ccgpu_multi_add_constraint": (C.c_int, [vp, C.c_int64, dp, C.c_double]),
+    "ccgpu_multi_set_species_constraints": (C.c_int, [vp, C.c_int64, C.c_int64, dp, dp]),
+    "ccgpu_multi_num_constraints": (C.c_int, [vp, C.c_int64, i64p]),
+    "ccgpu_multi_set_scales": (C.c_int, [vp, C.c_double, C.c_double]),
+    "ccgpu_multi_set_configuration": (C.c_int, [vp, dp, C.c_int64, dp]),
+    "ccgpu_multi_set_radii": (C.c_int, [vp, C.c_int64, dp]),
+    "ccgpu_multi_set_packing": (C.c_int, [vp, dp, C.c_int64, dp, dp]),
+    "ccgpu_multi_get_radii": (C.c_int, [vp, dp]),
+    "ccgpu_multi_get_rc": (C.c_int, [vp, dp]),
+    "ccgpu_multi_energy": (C.c_int, [vp, dp]),
+    "ccgpu_multi_force": (C.c_int, [vp, C.c_int64, dp]),
+    "ccgpu_multi_all_force": (C.c_int, [vp, dp]),
+    "ccgpu_multi_energy_and_all_force": (C.c_int, [vp, dp, dp]),
+    "ccgpu_multi_get_rho": (C.c_int, [vp, C.c_int64, dp, dp]),
+    "ccgpu_multi_species_energy": (C.c_int, [vp, C.c_int64, dp]),
+    "ccgpu_multi_relax": (C.c_int, [vp, dp, C.c_int, C.c_double, C.c_int, dp, C.POINTER(C.c_int),
+                                    C.POINTER(C.c_int)]),
 }
 
 OBJECTIVE = C.CFUNCTYPE(C.c_double, C.c_uint, dp, dp, C.c_void_p)

@@ -7,50 +7,17 @@
 #include <string>
 #include <vector>
 
-#include "cc_lattice.h"
-#include "cc_relax.h"
-#include "cc_runtime.h"
-#include "plis_host_ops.h"
+#include "cc_abi.h"
 
 using namespace ccgpu;
 
-namespace {
-thread_local std::string g_last_error;
-
-template <class F>
-int Guard(F &&f)
+namespace ccgpu {
+std::string &LastError()
 {
-    try {
-        g_last_error.clear();
-        f();
-        return CCGPU_OK;
-    } catch (const Error &e) {
-        g_last_error = e.what();
-        return e.code;
-    } catch (const std::bad_alloc &) {
-        g_last_error = "host allocation failed";
-        return CCGPU_ERR_MEMORY;
-    } catch (const std::exception &e) {
-        g_last_error = e.what();
-        return CCGPU_ERR_DEVICE;
-    }
+    thread_local std::string msg;
+    return msg;
 }
-
-void Require(bool ok, int code, const char *msg)
-{
-    if (!ok) throw Error(code, msg);
-}
-
-void RequireDevice()
-{
-    int n = 0;
-    hipError_t e = hipGetDeviceCount(&n);
-    if (e != hipSuccess || n < 1) {
-        (void)hipGetLastError();
-        throw Error(CCGPU_ERR_DEVICE, "no HIP device available");
-    }
-}
-}  // namespace
+}  // namespace ccgpu
 
 // one ShiftedCCPotential (optionally RepulsiveCCPotential) bound to one configuration on the GPU
 struct ccgpu_potential {
@@ -106,7 +73,7 @@ struct ccgpu_potential {
 
 extern "C" {
 
-const char *ccgpu_last_error(void) { return g_last_error.c_str(); }
+const char *ccgpu_last_error(void) { return LastError().c_str(); }
 
 double ccgpu_get_k(double chi, double rho, int d) { return GetK(chi, rho, d); }
 
@@ -326,7 +293,6 @@ int ccgpu_potential_relax(ccgpu_potential *p, double *relative, int maxstep, dou
         System &S = p->sys;
         const int d = S.d;
         const int64_t N = S.N, n = N * d;
-        const double fmin = Emin;
         // Objective (StructureOptimization.cpp:447-578, Switch 0): x -> s = wrap(x / rescale) in the prepare
         // kernel, E + F on the GPU, gradient -F.a_j / rescale in the force epilogue
         auto evaluate = [&](const double *x, double rescale, double *g) -> double {
@@ -344,38 +310,14 @@ int ccgpu_potential_relax(ccgpu_potential *p, double *relative, int maxstep, dou
             p->batch.Sync();
             return e[0];
         };
-        std::vector<double> s(relative, relative + n), x(n), g(n), dir(n), xo(n * history), go(n * history),
-            uo(history), vo(history);
-        int total = 0, last = 0;
         double E = 0;
-        for (int pass = 0; pass < 3; pass++) {
-            const double rescale = kRelaxRescale[pass];
-            // RelaxStructure_NLOPT_inner (:881-979): skip if the energy is already below fmin
-            E = evaluate(s.data(), 1.0, nullptr);
-            if (E < fmin) break;
-            for (int64_t i = 0; i < n; i++) x[i] = s[i] * rescale;
-            std::fill(xo.begin(), xo.end(), 0.0);
-            std::fill(go.begin(), go.end(), 0.0);
-            HostPlisOps ops{(int)n, history, x.data(), g.data(), dir.data(), xo.data(), go.data(), uo.data(), vo.data()};
-            PlisState st;
-            PlisInit(st, (int)n, history, maxstep, fmin);
-            int act = PlisAdvance(st, ops);
-            while (act == kPlisNeedEval) {
-                st.f = evaluate(x.data(), rescale, g.data());
-                act = PlisAdvance(st, ops);
-            }
-            total += st.nevals;
-            last = PlisResult(st);
-            for (int64_t i = 0; i < n; i++) s[i] = WrapUnit(x[i] / rescale);
-            E = evaluate(s.data(), 1.0, nullptr);
-            if (Emin > -kMaxEnergy && E < Emin) break;
-        }
+        int total = 0, last = 0;
+        HostRelax(n, relative, maxstep, Emin, history, evaluate, E, total, last);
         S.rescale = 1.0;
         p->batch.dirty = true;
-        std::copy(s.begin(), s.end(), relative);
         // leave the relaxed configuration bound, as RelaxStructure leaves List relaxed
         p->EnsureBuilt();
-        if (n > 0) CC_HIP(hipMemcpyAsync(S.x.get(), s.data(), n * sizeof(double), hipMemcpyHostToDevice, p->batch.stream));
+        if (n > 0) CC_HIP(hipMemcpyAsync(S.x.get(), relative, n * sizeof(double), hipMemcpyHostToDevice, p->batch.stream));
         p->batch.Sync();
         p->Invalidate();
         *energy = E;

@@ -17,6 +17,8 @@ enum SysFlags : int32_t {
     kSysLattice = 2,   // constraints are reciprocal lattice vectors: segment kernels (phase recurrence)
     kSysSoftCore = 4,  // add the soft-core pair term e0 (1 - r/sigma)^2
     kSysMixed = 8,     // lattice phases and force terms in FP32 (FP64 range reduction and accumulation)
+    kSysView = 16,     // species of a mixture: particle arrays are a range of the parent's (no prepare pass)
+    kSysMixture = 32,  // mixture: no CC term of its own; energy from its species and the radius contact term
 };
 
 struct SysDesc {
@@ -62,12 +64,28 @@ struct SysDesc {
     int32_t *cell_items;  // N (particle indices sorted by cell)
     double *sc_epart;     // per particle soft-core energy (i<j pairs)
 
+    // multi-species (MultiStealthyPotential): a mixture holds every particle, stored species by species, and
+    // the contact term; each species is a view system over its range of the mixture's particle arrays
+    const double *radius;     // mixture: per-particle radii (contact pair diameter r_i + r_j); null: uniform sigma
+    const double *ext_force;  // view: contact force of its particles (N x 4, already times e0), added to F
+    const int32_t *pid;       // mixture: caller's particle index of each stored particle (orients coincident pairs)
+    int32_t parent;           // view: descriptor index of its mixture in the batch; -1 otherwise
+    int32_t child0, nchild;   // mixture: descriptor range of its species
+    int32_t pad0;
+    double fscale;            // scale of the CC force (view: u0 N_s / N; 1 otherwise)
+    double u0;                // mixture: scale of the species energies
+
     double a[16];   // basis rows, d x d
     double b[16];   // reciprocal basis rows, d x d
     double vol, inv_vol, shift, rescale;
-    double sigma, e0;
-    int32_t pad[2];
+    double sigma, e0;  // soft core: sigma (mixture: the cell-list cutoff rc) and the energy scale
 };
+
+// A system takes part in a launch while it is active and, for a species view, while its mixture is.
+__device__ __forceinline__ bool sys_active(const SysDesc *sys, const SysDesc &S)
+{
+    return (S.flags & kSysActive) && (S.parent < 0 || (sys[S.parent].flags & kSysActive));
+}
 
 // one work item of a launch: the system index and up to three tile coordinates
 struct WorkItem {

@@ -53,7 +53,7 @@ __global__ void __launch_bounds__(kBlock) k_prepare(const SysDesc *__restrict__ 
 {
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
-    if (!(S.flags & kSysActive)) return;
+    if (!sys_active(sys, S)) return;
     const int i = it.t0 * kBlock + threadIdx.x;
     if (i >= S.N) return;
     const int d = S.d;
@@ -214,7 +214,7 @@ __device__ __forceinline__ void rho_lattice_kernel(const SysDesc *__restrict__ s
 {
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
-    if (!(S.flags & kSysActive) || ((S.flags & kSysMixed) != 0) != MIXED) return;
+    if (!sys_active(sys, S) || ((S.flags & kSysMixed) != 0) != MIXED) return;
     sincos_table_fill(L.tab);
     if (MIXED) {
         __syncthreads();
@@ -270,7 +270,7 @@ __global__ void __launch_bounds__(kBlock) k_rho_lattice_tab(const SysDesc *__res
     __shared__ double4 lpos[kSub];
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
-    if (!(S.flags & kSysActive)) return;
+    if (!sys_active(sys, S)) return;
     const int j0 = it.t1 * S.pchunk_len;
     const int j1 = min(S.N, j0 + S.pchunk_len);
     const int seg = it.t0 * kBlock + threadIdx.x;
@@ -342,7 +342,7 @@ __global__ void __launch_bounds__(kBlock) k_rho_general(const SysDesc *__restric
     __shared__ double4 lr[kMaxChunk];
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
-    if (!(S.flags & kSysActive)) return;
+    if (!sys_active(sys, S)) return;
     const int j0 = it.t1 * S.pchunk_len;
     const int j1 = min(S.N, j0 + S.pchunk_len);
     for (int j = j0 + (int)threadIdx.x; j < j1; j += kBlock) lr[j - j0] = S.cart[j];
@@ -373,7 +373,7 @@ __global__ void __launch_bounds__(kBlock) k_rho_reduce(const SysDesc *__restrict
     __shared__ double lds[kBlock];
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
-    if (!(S.flags & kSysActive)) return;
+    if (!sys_active(sys, S)) return;
     const int q = it.t0 * kBlock + threadIdx.x;
     double e = 0.0;
     if (q < S.nslots) {
@@ -409,7 +409,7 @@ __global__ void __launch_bounds__(kBlock) k_energy_final(const SysDesc *__restri
     __shared__ double lds[kBlock];
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
-    if (!(S.flags & kSysActive)) return;
+    if (!sys_active(sys, S)) return;
     const int nb = (S.nslots + kBlock - 1) / kBlock;
     double acc = 0.0;
     for (int b = threadIdx.x; b < nb; b += kBlock) acc += S.epart[b];
@@ -425,6 +425,33 @@ __global__ void __launch_bounds__(kBlock) k_energy_final(const SysDesc *__restri
         S.energy[0] = sc + cc;
         S.energy[1] = cc;
         S.energy[2] = sc;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// mixture energy: u0 sum_s (N_s / N) E_s + e0 sum_i contact_i (MultiStealthyPotential::TotalEnergy,
+// apps/CCOptimization/MultiStealthyPotential.cpp:750-761), species in index order.  items: (mixture)
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_mixture_final(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
+{
+    __shared__ double lds[kBlock];
+    const WorkItem it = items[blockIdx.x];
+    const SysDesc &S = sys[it.sys];
+    if (!sys_active(sys, S)) return;
+    double a = 0.0;
+    if (S.flags & kSysSoftCore)
+        for (int i = threadIdx.x; i < S.N; i += kBlock) a += S.sc_epart[i];
+    const double cp = block_sum(a, lds);
+    if (threadIdx.x == 0) {
+        double cc = 0.0;
+        for (int c = 0; c < S.nchild; c++) {
+            const SysDesc &C = sys[S.child0 + c];
+            cc += ((double)C.N / (double)S.N) * C.energy[0];
+        }
+        const double e_cc = S.u0 * cc, e_cp = S.e0 * cp;
+        S.energy[0] = e_cc + e_cp;
+        S.energy[1] = e_cc;
+        S.energy[2] = e_cp;
     }
 }
 
@@ -548,7 +575,7 @@ __global__ void __launch_bounds__(kBlock) k_force_lattice(const SysDesc *__restr
     __shared__ float2 ltab32[64];
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
-    if (!(S.flags & kSysActive)) return;
+    if (!sys_active(sys, S)) return;
     sincos_table_fill(ltab);
     __syncthreads();
     sincos_table_fill32(ltab32, ltab);
@@ -583,7 +610,7 @@ __global__ void __launch_bounds__(kBlock) k_force_general(const SysDesc *__restr
 {
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
-    if (!(S.flags & kSysActive)) return;
+    if (!sys_active(sys, S)) return;
     const int i = it.t0 * kBlock + threadIdx.x;
     const bool valid = i < S.N;
     const double4 r = valid ? S.cart[i] : make_double4(0, 0, 0, 0);
@@ -616,7 +643,7 @@ __global__ void __launch_bounds__(kBlock) k_force_final(const SysDesc *__restric
     (void)sc_force_unused;
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
-    if (!(S.flags & kSysActive)) return;
+    if (!sys_active(sys, S)) return;
     const int i = it.t0 * kBlock + threadIdx.x;
     if (i >= S.N) return;
     const int d = S.d;
@@ -641,12 +668,23 @@ __global__ void __launch_bounds__(kBlock) k_force_final(const SysDesc *__restric
         const double *fs = S.sc_epart + S.N + (int64_t)i * 4;
         for (int c = 0; c < d; c++) F[c] = F[c] + fs[c];
     }
+    if (S.flags & kSysView) {
+        // species of a mixture: contact force + (u0 N_s / N) x species CC force
+        // (MultiStealthyPotential::WithSpeciesForce, MultiStealthyPotential.cpp:667-678)
+        for (int c = 0; c < d; c++) F[c] = F[c] * S.fscale;
+        if (S.ext_force) {
+            const double *fc = S.ext_force + (int64_t)i * 4;
+            for (int c = 0; c < d; c++) F[c] = fc[c] + F[c];
+        }
+    }
     for (int c = 0; c < d; c++) S.F[(int64_t)i * d + c] = F[c];
     if (S.grad) {
+        // a species view writes its range of the mixture's gradient, at the mixture's current rescale
+        const double rescale = S.parent >= 0 ? sys[S.parent].rescale : S.rescale;
         for (int j = 0; j < d; j++) {
             double dot = 0.0;
             for (int c = 0; c < d; c++) dot += F[c] * S.a[j * d + c];
-            S.grad[(int64_t)i * d + j] = ((-1.0) * dot) / S.rescale;
+            S.grad[(int64_t)i * d + j] = ((-1.0) * dot) / rescale;
         }
     }
 }
@@ -753,6 +791,11 @@ void launch_rho_reduce(hipStream_t stream, const SysDesc *sys, const WorkItem *i
 void launch_energy_final(hipStream_t stream, const SysDesc *sys, const WorkItem *items, int n)
 {
     if (n > 0) hipLaunchKernelGGL(k_energy_final, CC_GRID(n), sys, items);
+}
+
+void launch_mixture_final(hipStream_t stream, const SysDesc *sys, const WorkItem *items, int n)
+{
+    if (n > 0) hipLaunchKernelGGL(k_mixture_final, CC_GRID(n), sys, items);
 }
 
 void launch_force_general(hipStream_t stream, const SysDesc *sys, const WorkItem *items, int n)

@@ -293,7 +293,9 @@ void System::Prepare(hipStream_t stream)
     if (!has_cell) throw Error(kErrState, "no configuration bound (SetConfiguration first)");
     // buffers may be regrown below and their old blocks handed to other users through the pool: nothing of
     // this system may still be in flight (a fresh system owns no buffers yet)
-    if (x.get()) CC_HIP(hipStreamSynchronize(stream));
+    if (energy.get()) CC_HIP(hipStreamSynchronize(stream));
+    if (view() && (parent->d != d || view_off < 0 || view_off + N > parent->N))
+        throw Error(kErrState, "species range outside its mixture");
     Tiles def = DefaultTiles(d, N, M);
     used.seg = tiles.seg > 0 ? tiles.seg : def.seg;
     if (used.seg != 4 && used.seg != 8 && used.seg != 16 && used.seg != 24 && used.seg != 32)
@@ -357,10 +359,14 @@ void System::Prepare(hipStream_t stream)
     }
 
     const int64_t Nn = std::max<int64_t>(N, 1);
-    x.Reserve(Nn * d);
-    pos.Reserve(Nn);
-    cart.Reserve(Nn);
-    step.Reserve(Nn);
+    if (!view()) {  // a species reads and writes its mixture's particle arrays
+        x.Reserve(Nn * d);
+        pos.Reserve(Nn);
+        cart.Reserve(Nn);
+        step.Reserve(Nn);
+        F.Reserve(Nn * d);
+        grad.Reserve(Nn * d);
+    }
     rpart.Reserve(std::max<int64_t>(1, (int64_t)pchunks * nslots * 2));
     rho.Reserve(std::max<int64_t>(1, 2 * M));
     coef.Reserve(std::max<int64_t>(1, 2 * nslots));
@@ -368,8 +374,10 @@ void System::Prepare(hipStream_t stream)
     epart.Reserve(std::max<int64_t>(1, (nslots + kBlock - 1) / kBlock));
     energy.Reserve(4);
     fpart.Reserve((int64_t)kchunks * Nn * 4);
-    F.Reserve(Nn * d);
-    grad.Reserve(Nn * d);
+    if (mixture) {
+        radius.Reserve(Nn);
+        pid.Reserve(Nn);
+    }
     if (softcore()) {
         cell_of.Reserve(Nn);
         cell_start.Reserve(2 * (int64_t)ncells + 1);
@@ -383,7 +391,7 @@ void System::Prepare(hipStream_t stream)
     D.N = (int32_t)N;
     D.M = (int32_t)M;
     D.flags = kSysActive | (lattice ? kSysLattice : 0) | (softcore() ? kSysSoftCore : 0) |
-              (lattice && precision == 1 ? kSysMixed : 0);
+              (lattice && precision == 1 ? kSysMixed : 0) | (view() ? kSysView : 0) | (mixture ? kSysMixture : 0);
     D.nseg = (int32_t)nseg();
     D.nslots = (int32_t)nslots;
     D.pchunks = N > 0 ? pchunks : 0;  // no particles: rho sums nothing
@@ -392,10 +400,12 @@ void System::Prepare(hipStream_t stream)
     D.kchunk_len = used.force_chunk;
     for (int c = 0; c < 4; c++) D.ncell_side[c] = ncell_side[c];
     D.ncells = ncells;
-    D.x = x.get();
-    D.pos = pos.get();
-    D.cart = cart.get();
-    D.step = step.get();
+    System *owner = view() ? parent : this;  // owner of the particle arrays
+    const int64_t off = view() ? view_off : 0;
+    D.x = owner->x.get() + off * d;
+    D.pos = owner->pos.get() + off;
+    D.cart = owner->cart.get() + off;
+    D.step = owner->step.get() + off;
     D.seg_head = layout->seg_head.get();
     D.seg_len = layout->seg_len.get();
     D.slot_k = layout->slot_k.get();
@@ -408,8 +418,16 @@ void System::Prepare(hipStream_t stream)
     D.epart = epart.get();
     D.energy = energy.get();
     D.fpart = fpart.get();
-    D.F = F.get();
-    D.grad = grad.get();
+    D.F = owner->F.get() + off * d;
+    D.grad = owner->grad.get() + off * d;
+    D.radius = mixture ? radius.get() : nullptr;
+    D.pid = mixture ? pid.get() : nullptr;
+    D.ext_force = view() && parent->softcore() ? parent->sc_epart.get() + parent->N + off * 4 : nullptr;
+    D.parent = -1;  // descriptor links are set by the batch
+    D.child0 = 0;
+    D.nchild = 0;
+    D.fscale = view() ? fscale : 1.0;
+    D.u0 = u0;
     D.cell_of = cell_of.get();
     D.cell_start = cell_start.get();
     D.cell_items = cell_items.get();
@@ -474,26 +492,43 @@ void Batch::Build()
 {
     h_items.clear();
     seg_groups.clear();
+    // the systems in order, then the species of every mixture among them (each mixture's contiguous): the
+    // descriptor index of systems[i] stays i, and a mixture is prepared before the views of its arrays
+    all.assign(systems.begin(), systems.end());
+    for (System *s : systems)
+        for (auto &c : s->children) {
+            if (c->parent != s) throw Error(kErrState, "species not linked to its mixture");
+            all.push_back(c.get());
+        }
     std::vector<SysDesc> descs;
-    descs.reserve(systems.size());
-    for (System *s : systems) {
+    descs.reserve(all.size());
+    for (System *s : all) {
         s->Prepare(stream);
         descs.push_back(s->desc);
+    }
+    for (size_t i = 0, c = systems.size(); i < systems.size(); i++) {
+        System *s = all[i];
+        if (s->children.empty()) continue;
+        descs[i].child0 = (int32_t)c;
+        descs[i].nchild = (int32_t)s->children.size();
+        for (size_t q = 0; q < s->children.size(); q++) descs[c + q].parent = (int32_t)i;
+        c += s->children.size();
     }
     auto add = [&](Range &r, auto &&gen) {
         r.off = (int)h_items.size();
         gen();
         r.n = (int)h_items.size() - r.off;
     };
-    const int ns = (int)systems.size();
+    const int ns = (int)all.size();
     auto pblocks = [](const System *s) { return (int)((s->N + kBlock - 1) / kBlock); };
     add(r_prep, [&] {
         for (int i = 0; i < ns; i++)
-            for (int b = 0; b < pblocks(systems[i]); b++) h_items.push_back({i, b, 0, 0});
+            if (!all[i]->view())
+                for (int b = 0; b < pblocks(all[i]); b++) h_items.push_back({i, b, 0, 0});
     });
     // lattice systems grouped by (segment length, rho kernel): one template instance per group
     std::vector<std::tuple<int, int, int>> segs;
-    for (System *s : systems) {
+    for (System *s : all) {
         const std::tuple<int, int, int> key(s->used.seg, s->used.rho_kernel, s->used.force_ppl);
         if (s->lattice && std::find(segs.begin(), segs.end(), key) == segs.end()) segs.push_back(key);
     }
@@ -504,12 +539,12 @@ void Batch::Build()
         g.rho_kernel = rk;
         g.force_ppl = ppl;
         g.any_mixed = false;
-        for (System *s : systems)
+        for (System *s : all)
             if (s->lattice && s->precision == 1 && s->used.seg == sl && s->used.rho_kernel == rk && s->used.force_ppl == ppl)
                 g.any_mixed = true;
         add(g.rho, [&] {
             for (int i = 0; i < ns; i++) {
-                System *s = systems[i];
+                System *s = all[i];
                 if (!s->lattice || s->used.seg != sl || s->used.rho_kernel != rk || s->used.force_ppl != ppl ||
                     s->N == 0)
                     continue;
@@ -521,7 +556,7 @@ void Batch::Build()
         });
         add(g.force, [&] {
             for (int i = 0; i < ns; i++) {
-                System *s = systems[i];
+                System *s = all[i];
                 if (!s->lattice || s->used.seg != sl || s->used.rho_kernel != rk || s->used.force_ppl != ppl) continue;
                 const int fb = (int)((s->N + kBlock * ppl - 1) / (kBlock * ppl));
                 for (int c = 0; c < s->kchunks; c++)
@@ -532,7 +567,7 @@ void Batch::Build()
     }
     add(r_rho_gen, [&] {
         for (int i = 0; i < ns; i++) {
-            System *s = systems[i];
+            System *s = all[i];
             if (s->lattice || s->M == 0 || s->N == 0) continue;
             const int kb = (int)((s->M + kBlock - 1) / kBlock);
             for (int c = 0; c < s->pchunks; c++)
@@ -541,38 +576,44 @@ void Batch::Build()
     });
     add(r_red, [&] {
         for (int i = 0; i < ns; i++) {
-            const int nb = (int)((systems[i]->desc.nslots + kBlock - 1) / kBlock);
+            const int nb = (int)((all[i]->desc.nslots + kBlock - 1) / kBlock);
             for (int b = 0; b < nb; b++) h_items.push_back({i, b, 0, 0});
         }
     });
     add(r_efin, [&] {
-        for (int i = 0; i < ns; i++) h_items.push_back({i, 0, 0, 0});
+        for (int i = 0; i < ns; i++)
+            if (!all[i]->mixture) h_items.push_back({i, 0, 0, 0});
+    });
+    add(r_mix, [&] {
+        for (int i = 0; i < ns; i++)
+            if (all[i]->mixture) h_items.push_back({i, 0, 0, 0});
     });
     add(r_f_gen, [&] {
         for (int i = 0; i < ns; i++) {
-            System *s = systems[i];
-            if (s->lattice) continue;
+            System *s = all[i];
+            if (s->lattice || s->mixture) continue;
             for (int c = 0; c < s->kchunks; c++)
                 for (int b = 0; b < pblocks(s); b++) h_items.push_back({i, b, c, 0});
         }
     });
     add(r_ffin, [&] {
         for (int i = 0; i < ns; i++)
-            for (int b = 0; b < pblocks(systems[i]); b++) h_items.push_back({i, b, 0, 0});
+            if (!all[i]->mixture)  // a mixture's forces are written by its species
+                for (int b = 0; b < pblocks(all[i]); b++) h_items.push_back({i, b, 0, 0});
     });
     add(r_sc_part, [&] {
         for (int i = 0; i < ns; i++)
-            if (systems[i]->softcore())
-                for (int b = 0; b < pblocks(systems[i]); b++) h_items.push_back({i, b, pblocks(systems[i]), 0});
+            if (all[i]->softcore())
+                for (int b = 0; b < pblocks(all[i]); b++) h_items.push_back({i, b, pblocks(all[i]), 0});
     });
     add(r_sc_sys, [&] {
         for (int i = 0; i < ns; i++)
-            if (systems[i]->softcore()) h_items.push_back({i, 0, 0, 0});
+            if (all[i]->softcore()) h_items.push_back({i, 0, 0, 0});
     });
     add(r_sc_cell, [&] {
         for (int i = 0; i < ns; i++)
-            if (systems[i]->softcore())
-                for (int b = 0; b < (systems[i]->ncells + kBlock - 1) / kBlock; b++) h_items.push_back({i, b, 0, 0});
+            if (all[i]->softcore())
+                for (int b = 0; b < (all[i]->ncells + kBlock - 1) / kBlock; b++) h_items.push_back({i, b, 0, 0});
     });
     d_desc.Reserve(std::max<size_t>(1, descs.size()));
     d_items.Reserve(std::max<size_t>(1, h_items.size()));
@@ -612,6 +653,7 @@ void Batch::Enqueue(bool rho, bool forces)
         }
         launch_rho_reduce(stream, D, I + r_red.off, r_red.n);
         launch_energy_final(stream, D, I + r_efin.off, r_efin.n);
+        launch_mixture_final(stream, D, I + r_mix.off, r_mix.n);
     }
     if (forces) {
         Mark mk{1, -1, -1};

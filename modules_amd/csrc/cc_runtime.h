@@ -147,6 +147,19 @@ public:
     DevArray<int32_t> cell_of, cell_start, cell_items;
     DevArray<double> sc_epart, sc_F;
 
+    // multi-species (MultiStealthyPotential): a mixture system holds every particle (stored species by species),
+    // the contact term with per-particle radii and the total energy; each species is a child system whose
+    // particle arrays are views of its contiguous range of the mixture's (x, positions, forces, gradient)
+    bool mixture = false;
+    System *parent = nullptr;  // species: its mixture
+    int64_t view_off = 0;      // species: first particle of its range in the mixture
+    double fscale = 1.0;       // species: u0 N_s / N on its CC force
+    double u0 = 1.0;           // mixture: scale of the species energies
+    std::vector<std::unique_ptr<System>> children;  // mixture: its species, in species order
+    DevArray<double> radius;   // mixture: per-particle radii in storage order
+    DevArray<int32_t> pid;     // mixture: caller's particle index of each storage slot
+    bool view() const { return parent != nullptr; }
+
     SysDesc desc{};
     int pchunks = 1, kchunks = 1;
     int ncell_side[4] = {1, 1, 1, 1};
@@ -200,7 +213,10 @@ public:
     DevArray<SysDesc> d_desc;
     DevArray<WorkItem> d_items;
     std::vector<WorkItem> h_items;
-    Range r_prep, r_rho_gen, r_red, r_efin, r_f_gen, r_ffin, r_sc_part, r_sc_sys, r_sc_cell;
+    Range r_prep, r_rho_gen, r_red, r_efin, r_f_gen, r_ffin, r_sc_part, r_sc_sys, r_sc_cell, r_mix;
+    // every system of the launches: `systems` in order (their descriptor indices), then the species of each
+    // mixture among them
+    std::vector<System *> all;
     std::vector<SegGroup> seg_groups;
 
 private:
@@ -221,6 +237,7 @@ void launch_rho_lattice(hipStream_t, int seg, const SysDesc *, const WorkItem *,
 void launch_rho_general(hipStream_t, const SysDesc *, const WorkItem *, int);
 void launch_rho_reduce(hipStream_t, const SysDesc *, const WorkItem *, int);
 void launch_energy_final(hipStream_t, const SysDesc *, const WorkItem *, int);
+void launch_mixture_final(hipStream_t, const SysDesc *, const WorkItem *, int);
 void launch_force_lattice(hipStream_t, int seg, const SysDesc *, const WorkItem *, int, int ppl);
 void launch_force_general(hipStream_t, const SysDesc *, const WorkItem *, int);
 void launch_force_final(hipStream_t, const SysDesc *, const WorkItem *, int);

@@ -6,6 +6,10 @@
 //   E   = e0 sum_{i<j, |r|<sigma} (1 - |r|/sigma)^2       over all periodic images of j
 //   F_i = -(2 e0/sigma) sum_{j != i, |r|<sigma} (1/|r| - 1/sigma) r ,  r = r_j + image - r_i
 //         (coincident points, |r| < 1e-15, push by -(2 e0/sigma) along +-x)
+// Per-particle radii (MultiStealthyPotential, apps/CCOptimization/MultiStealthyPotential.cpp:682-750, 799-879):
+// the pair diameter is sigma_ij = r_i + r_j, the cells and the reach follow the cutoff rc = 2.01 max r_i
+// (SetSecureRadii, :448-458), the energy share is (1 - |r|/sigma_ij)^2 and the force 2/sigma_ij (1/|r| -
+// 1/sigma_ij)(-r), both scaled by e0 after the sum (TotalEnergy, ContactForce).
 // Build: cell index from the relative coordinates, integer counting sort (atomic counts + scan), then
 // each cell's list sorted by particle index so the pair sums have a fixed order.  Pair pass: one lane
 // per particle visits the (2 r_c + 1)^d neighbour cells in a fixed order.
@@ -24,7 +28,7 @@ __global__ void __launch_bounds__(kBlock) k_sc_zero(const SysDesc *__restrict__ 
 {
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
-    if (!(S.flags & kSysActive)) return;
+    if (!sys_active(sys, S)) return;
     for (int c = it.t0 * kBlock + threadIdx.x; c <= S.ncells; c += kBlock * it.t1) S.cell_start[c] = 0;
 }
 
@@ -33,7 +37,7 @@ __global__ void __launch_bounds__(kBlock) k_sc_count(const SysDesc *__restrict__
 {
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
-    if (!(S.flags & kSysActive)) return;
+    if (!sys_active(sys, S)) return;
     const int i = it.t0 * kBlock + threadIdx.x;
     if (i >= S.N) return;
     const double4 p = S.pos[i];
@@ -52,7 +56,7 @@ __global__ void __launch_bounds__(kBlock) k_sc_scan(const SysDesc *__restrict__ 
     __shared__ int32_t part[kBlock];
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
-    if (!(S.flags & kSysActive)) return;
+    if (!sys_active(sys, S)) return;
     const int n = S.ncells;
     const int per = (n + kBlock - 1) / kBlock;
     const int c0 = threadIdx.x * per, c1 = min(n, c0 + per);
@@ -85,7 +89,7 @@ __global__ void __launch_bounds__(kBlock) k_sc_scatter(const SysDesc *__restrict
 {
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
-    if (!(S.flags & kSysActive)) return;
+    if (!sys_active(sys, S)) return;
     const int i = it.t0 * kBlock + threadIdx.x;
     if (i >= S.N) return;
     int32_t *cursor = S.cell_start + (S.ncells + 1);
@@ -98,7 +102,7 @@ __global__ void __launch_bounds__(kBlock) k_sc_sort(const SysDesc *__restrict__ 
 {
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
-    if (!(S.flags & kSysActive)) return;
+    if (!sys_active(sys, S)) return;
     const int c = it.t0 * kBlock + threadIdx.x;
     if (c >= S.ncells) return;
     const int b = S.cell_start[c], e = S.cell_start[c + 1];
@@ -118,7 +122,7 @@ __global__ void __launch_bounds__(kBlock) k_sc_pair(const SysDesc *__restrict__ 
 {
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
-    if (!(S.flags & kSysActive)) return;
+    if (!sys_active(sys, S)) return;
     const int i = it.t0 * kBlock + threadIdx.x;
     if (i >= S.N) return;
     const int d = S.d;
@@ -137,8 +141,11 @@ __global__ void __launch_bounds__(kBlock) k_sc_pair(const SysDesc *__restrict__ 
         span[c] = 2 * reach[c] + 1;
         total *= span[c];
     }
-    const double sig = S.sigma, e0 = S.e0, sig2 = sig * sig;
-    const double pref = 2.0 * e0 / sig;
+    const double e0 = S.e0;
+    const double *__restrict__ radius = S.radius;
+    const double ri = radius ? radius[i] : 0.0;
+    // a coincident pair is pushed apart along x, oriented by the particles' indices in the caller's order
+    const int gi = S.pid ? S.pid[i] : i;
     double E = 0.0;
     double f[4] = {0, 0, 0, 0};
     for (int o = 0; o < total; o++) {
@@ -166,14 +173,19 @@ __global__ void __launch_bounds__(kBlock) k_sc_pair(const SysDesc *__restrict__ 
                 for (int c = 0; c < d; c++) r[c] = __builtin_fma(S.a[q * d + c], ds[q], r[c]);
             double m2 = 0.0;
             for (int c = 0; c < d; c++) m2 = __builtin_fma(r[c], r[c], m2);
-            if (!(m2 < sig2)) continue;
+            const double sig = radius ? ri + radius[j] : S.sigma;
+            if (!(m2 < sig * sig)) continue;
             const double rr = sqrt(m2);
+            // uniform sigma: e0 folded into every term; radii: e0 applied to the sums
+            const double pref = radius ? 2.0 / sig : 2.0 * e0 / sig;
             if (j > i) {
                 const double u = 1.0 - rr / sig;
-                E = __builtin_fma(e0 * u, u, E);
+                if (!radius) E = __builtin_fma(e0 * u, u, E);
+                else if (u > 0.0) E = __builtin_fma(u, u, E);
             }
             if (rr < 1e-15) {
-                f[0] -= pref * ((i > j) ? -1.0 : 1.0);
+                const int gj = S.pid ? S.pid[j] : j;
+                f[0] -= pref * ((gi > gj) ? -1.0 : 1.0);
                 for (int c = 1; c < d; c++) f[c] -= pref * r[c];
             } else {
                 const double cf = pref * (1.0 / rr - 1.0 / sig);
@@ -181,6 +193,8 @@ __global__ void __launch_bounds__(kBlock) k_sc_pair(const SysDesc *__restrict__ 
             }
         }
     }
+    if (radius)
+        for (int c = 0; c < 4; c++) f[c] *= e0;
     S.sc_epart[i] = E;
     // the force is added to the CC force in k_force_final through sc_F (stored as N x 4)
     double *out = S.sc_epart + S.N + (int64_t)i * 4;

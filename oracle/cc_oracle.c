@@ -14,6 +14,8 @@
  *   - ShiftedCCPotential::GetRho_Direct / Energy / Force    potential/CollectiveCoordinatePotential.h:1255-1394
  *   - RepulsiveCCPotential soft core energy / forces         apps/CCOptimization/RepulsiveCCPotential.h:61-145
  *     (brute force over periodic images instead of the cell-list walk; same pair terms)
+ *   - MultiStealthyPotential energy / forces (species CC      apps/CCOptimization/MultiStealthyPotential.cpp:682-833
+ *     terms over index lists, radius contact term)
  *   - StructureOptimization Objective / FillParticles / GetParam and the 3-pass drivers
  *                                                           potential/StructureOptimization.cpp:339-350, 447-578,
  *                                                           710-776, 881-979, 1046-1084
@@ -432,6 +434,96 @@ void orc_softcore_forces(int d, int64_t N, const double *s, const double *a, dou
 		for (int c = 0; c < d; c++) F[i * d + c] = f[c];
 	}
 	free(r);
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * Multi-species potential (apps/CCOptimization/MultiStealthyPotential.cpp).
+ * Contact term with per-particle radii, sigma_ij = r_i + r_j, brute force over the periodic images that can
+ * lie within the cutoff: per-particle energy slots E_i = sum_{j > i, images, |r|/sigma_ij < 1} (1 - |r|/sigma_ij)^2
+ * (Energy's contact lambda, :705-720), and forces e0 * sum_{j != i, images} of ContactForce's pair terms
+ * (:799-833): coincident points (|r| < 1e-15) push by -2/sigma along +-x, else 2/sigma (1/|r| - 1/sigma) (-r).
+ * ---------------------------------------------------------------------------------------------- */
+void orc_contact_radii(int d, int64_t N, const double *s, const double *a, const double *radii, double rc, double e0,
+                       double *E_slots, double *F, int nthreads)
+{
+	double rmax = 0.0;
+	for (int64_t i = 0; i < N; i++) rmax = radii[i] > rmax ? radii[i] : rmax;
+	int reach[ORC_MAXDIM];
+	int nimg = orc_image_reach(d, a, (2.0 * rmax > rc ? 2.0 * rmax : rc), reach);
+	double *r = (double *)malloc(sizeof(double) * ORC_MAXDIM * (N > 0 ? N : 1));
+	for (int64_t j = 0; j < N; j++) orc_cart(d, a, s + j * d, r + j * ORC_MAXDIM);
+	if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 16)
+	for (int64_t i = 0; i < N; i++) {
+		double E = 0.0, f[ORC_MAXDIM] = {0, 0, 0, 0};
+		for (int64_t j = 0; j < N; j++) {
+			if (j == i) continue; /* no self pair, no self image */
+			const double sigma = radii[i] + radii[j];
+			for (int q = 0; q < nimg; q++) {
+				double sh[ORC_MAXDIM], dv[ORC_MAXDIM];
+				orc_image_shift(d, a, reach, q, sh);
+				for (int c = 0; c < ORC_MAXDIM; c++) dv[c] = (r[j * ORC_MAXDIM + c] - r[i * ORC_MAXDIM + c]) + sh[c];
+				const double rr = sqrt(gv_dot(dv, dv));
+				if (i < j) {
+					const double rs = rr / sigma;
+					if (rs < 1.0) E += (1.0 - rs) * (1.0 - rs);
+				}
+				if (rr < 1e-15) {
+					dv[0] = (i > j) ? -1.0 : 1.0;
+					for (int c = 0; c < ORC_MAXDIM; c++) f[c] = f[c] - 2.0 / sigma * dv[c];
+				} else if (rr < sigma) {
+					const double c0 = 2.0 / sigma * (1.0 / rr - 1.0 / sigma);
+					for (int c = 0; c < ORC_MAXDIM; c++) f[c] = f[c] + c0 * (-1. * dv[c]);
+				}
+			}
+		}
+		E_slots[i] = E;
+		for (int c = 0; c < d; c++) F[i * d + c] = f[c] * e0;
+	}
+	free(r);
+}
+
+/* Energy() and AllForce() of MultiStealthyPotential (:682-761, 763-797): species s has particles idx[idx_off[s] ..
+ * idx_off[s+1]) (its own order) and constraints k/v[k_off[s] .. k_off[s+1]); each species is a ShiftedCCPotential
+ * over its particles in the full cell (Shift 0).  E = u0 sum_s (N_s/N) E_s + e0 sum_i E_i (TotalEnergy, :750-761);
+ * F_i = contact_i + (u0 N_s/N) F_s(i) (Force, :763-797).  species_E (nspecies, may be NULL) receives the E_s. */
+double orc_multi_energy_forces(int d, int64_t N, const double *s, const double *a, int nspecies, const int64_t *idx_off,
+                               const int64_t *idx, const int64_t *k_off, const double *k, const double *v,
+                               const double *radii, double rc, double u0, double e0, int repulsion, double *F,
+                               double *species_E, int nthreads)
+{
+	const double V = orc_volume(d, a);
+	double CC = 0.0, CP = 0.0;
+	double *slots = (double *)calloc((size_t)(N > 0 ? N : 1), sizeof(double));
+	double *fc = (double *)calloc((size_t)(N > 0 ? N : 1) * d, sizeof(double));
+	if (repulsion) orc_contact_radii(d, N, s, a, radii, rc, e0, slots, fc, nthreads);
+	for (int sp = 0; sp < nspecies; sp++) {
+		const int64_t n_s = idx_off[sp + 1] - idx_off[sp], M = k_off[sp + 1] - k_off[sp];
+		double *sub = (double *)malloc(sizeof(double) * (size_t)(n_s > 0 ? n_s : 1) * d);
+		for (int64_t q = 0; q < n_s; q++)
+			for (int c = 0; c < d; c++) sub[q * d + c] = s[idx[idx_off[sp] + q] * d + c];
+		double *re = (double *)malloc(sizeof(double) * (size_t)(M > 0 ? M : 1));
+		double *im = (double *)malloc(sizeof(double) * (size_t)(M > 0 ? M : 1));
+		double *fs = (double *)malloc(sizeof(double) * (size_t)(n_s > 0 ? n_s : 1) * d);
+		orc_rho(d, n_s, sub, a, M, k + k_off[sp] * d, re, im, nthreads);
+		const double E_s = orc_energy(M, v + k_off[sp], re, im, V, 0.0);
+		if (species_E) species_E[sp] = E_s;
+		CC += ((double)n_s / (double)N) * E_s;
+		orc_forces(d, n_s, sub, a, M, k + k_off[sp] * d, v + k_off[sp], re, im, V, fs, nthreads);
+		const double w = u0 * (double)n_s / (double)N;
+		for (int64_t q = 0; q < n_s; q++) {
+			const int64_t i = idx[idx_off[sp] + q];
+			for (int c = 0; c < d; c++) F[i * d + c] = fc[i * d + c] + fs[q * d + c] * w;
+		}
+		free(sub);
+		free(re);
+		free(im);
+		free(fs);
+	}
+	for (int64_t i = 0; i < N; i++) CP += slots[i];
+	free(slots);
+	free(fc);
+	return u0 * CC + e0 * CP;
 }
 
 /* ------------------------------------------------------------------------------------------------

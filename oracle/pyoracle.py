@@ -56,6 +56,12 @@ def lib():
                                      C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.orc_relax_emin.restype = C.c_double
         L.orc_max_threads.restype = C.c_int
+        _i64p = C.POINTER(C.c_int64)
+        L.orc_contact_radii.argtypes = [C.c_int, C.c_int64, _dp, _dp, _dp, C.c_double, C.c_double, _dp, _dp,
+                                        C.c_int]
+        L.orc_multi_energy_forces.argtypes = [C.c_int, C.c_int64, _dp, _dp, C.c_int, _i64p, _i64p, _i64p, _dp, _dp,
+                                              _dp, C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp, C.c_int]
+        L.orc_multi_energy_forces.restype = C.c_double
         _lib = L
     return _lib
 
@@ -184,6 +190,45 @@ def total_energy_forces(s, a, k, v, shift=0.0, sigma=0.0, e0=0.0, nthreads=0):
                                       _p(v if M else np.zeros(1)), shift, sigma, e0, _p(F),
                                       nthreads or lib().orc_max_threads())
     return E, F
+
+
+def contact_radii(s, a, radii, rc, e0, nthreads=0):
+    """MultiStealthyPotential's contact term: per-particle energy slots (pairs i < j, before e0) and e0 x forces."""
+    s, a, radii = _c(s), _c(a), _c(radii)
+    N, d = s.shape
+    slots = np.zeros(max(N, 1))
+    F = np.zeros((max(N, 1), d))
+    lib().orc_contact_radii(d, N, _p(s), _p(a), _p(radii), rc, e0, _p(slots), _p(F),
+                            nthreads or lib().orc_max_threads())
+    return slots[:N], F[:N]
+
+
+def multi_energy_forces(s, a, species_idx, species_k, species_v, radii, rc, u0=1.0, e0=1.0, repulsion=True,
+                        nthreads=0):
+    """MultiStealthyPotential Energy() and AllForce(): species_idx[s] lists the species' particles in its own
+    order, species_k[s] / species_v[s] its constraints.  Returns (E, F, species energies)."""
+    s, a, radii = _c(s), _c(a), _c(radii)
+    N, d = s.shape
+    S = len(species_idx)
+    idx_off = np.zeros(S + 1, dtype=np.int64)
+    k_off = np.zeros(S + 1, dtype=np.int64)
+    for q in range(S):
+        idx_off[q + 1] = idx_off[q] + len(species_idx[q])
+        k_off[q + 1] = k_off[q] + len(species_v[q])
+    idx = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.int64) for x in species_idx] +
+                                              [np.zeros(1, dtype=np.int64)]))
+    k = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64).reshape(-1, d) for x in species_k] +
+                                            [np.zeros((1, d))]))
+    v = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64).reshape(-1) for x in species_v] +
+                                            [np.zeros(1)]))
+    F = np.zeros((max(N, 1), d))
+    ES = np.zeros(max(S, 1))
+    ip = C.POINTER(C.c_int64)
+    E = lib().orc_multi_energy_forces(d, N, _p(s), _p(a), S, idx_off.ctypes.data_as(ip), idx.ctypes.data_as(ip),
+                                      k_off.ctypes.data_as(ip), _p(k), _p(v), _p(radii if N else np.zeros(1)),
+                                      rc, u0, e0, int(bool(repulsion)), _p(F), _p(ES),
+                                      nthreads or lib().orc_max_threads())
+    return E, F[:N], ES[:S]
 
 
 def relax_emin(s, a, k, v, shift=0.0, sigma=0.0, e0=0.0, maxstep=10000, Emin=-1e10, mf=40, nthreads=0):
