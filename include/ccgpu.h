@@ -241,6 +241,15 @@ int ccgpu_solver_destroy(ccgpu_solver *s);
 int ccgpu_solver_submit(ccgpu_solver *s, int dim, const double *basis, int64_t N, const double *relative, int64_t M,
                         const double *k, const double *V, double shift, double epsilon, double sigma, int maxstep,
                         double Emin, int history, int64_t *job);
+/* A RelaxStructure_NLOPT[_Emin] request with the multi-species potential (MultiStealthyPotential, as
+ * ccgpu_multi_*): species q holds particles idx[idx_off[q] .. idx_off[q+1]) (its own order; every particle in
+ * exactly one species) and constraints k/V[k_off[q] .. k_off[q+1]); radii N, contact cutoff rc (>= the largest
+ * pair diameter), scales u0 and epsilon (contact on when |epsilon| > 1e-10).  Species contractions run in FP64.
+ * The relaxed configuration of ccgpu_solver_result is in the caller's particle order. */
+int ccgpu_solver_submit_multi(ccgpu_solver *s, int dim, const double *basis, int64_t N, const double *relative,
+                              int64_t nspecies, const int64_t *idx_off, const int64_t *idx, const int64_t *k_off,
+                              const double *k, const double *V, const double *radii, double rc, double u0,
+                              double epsilon, int maxstep, double Emin, int history, int64_t *job);
 /* Admit queued jobs and advance resident ones by up to `evaluations` batched evaluations.
  * *pending: jobs still queued or resident afterwards. */
 int ccgpu_solver_step(ccgpu_solver *s, int evaluations, int *pending);

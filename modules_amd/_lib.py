@@ -64,6 +64,8 @@ SIGNATURES = {
     "ccgpu_solver_destroy": (C.c_int, [vp]),
     "ccgpu_solver_submit": (C.c_int, [vp, C.c_int, dp, C.c_int64, dp, C.c_int64, dp, dp, C.c_double, C.c_double,
                                       C.c_double, C.c_int, C.c_double, C.c_int, i64p]),
+    "ccgpu_solver_submit_multi": (C.c_int, [vp, C.c_int, dp, C.c_int64, dp, C.c_int64, i64p, i64p, i64p, dp, dp, dp,
+                                            C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, i64p]),
     "ccgpu_solver_step": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int)]),
     "ccgpu_solver_set_check_interval": (C.c_int, [vp, C.c_int]),
     "ccgpu_solver_set_graph": (C.c_int, [vp, C.c_int]),

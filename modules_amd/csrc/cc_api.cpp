@@ -489,6 +489,58 @@ int ccgpu_solver_submit(ccgpu_solver *s, int dim, const double *basis, int64_t N
     });
 }
 
+int ccgpu_solver_submit_multi(ccgpu_solver *s, int dim, const double *basis, int64_t N, const double *relative,
+                              int64_t nspecies, const int64_t *idx_off, const int64_t *idx, const int64_t *k_off,
+                              const double *k, const double *V, const double *radii, double rc, double u0,
+                              double epsilon, int maxstep, double Emin, int history, int64_t *job)
+{
+    return Guard([&] {
+        Require(s && basis && job && N >= 0 && nspecies >= 1 && idx_off && k_off && (N == 0 || (relative && radii && idx)),
+                CCGPU_ERR_ARG, "bad argument");
+        Require(dim >= 1 && dim <= kMaxDim, CCGPU_ERR_ARG, "dimension must be 1..4");
+        Require(history >= 1 && history <= kPlisMaxHistory, CCGPU_ERR_ARG, "history must be 1..64");
+        Require(N * dim < (int64_t)1 << 30, CCGPU_ERR_ARG, "too many particles");
+        Require(std::isfinite(u0) && std::isfinite(epsilon) && std::isfinite(rc) && rc >= 0, CCGPU_ERR_ARG,
+                "bad energy scales or cutoff");
+        Require(idx_off[0] == 0 && idx_off[nspecies] == N && k_off[0] == 0, CCGPU_ERR_ARG, "bad species offsets");
+        for (int64_t i = 0; i < N * dim; i++)
+            if (!std::isfinite(relative[i])) throw Error(CCGPU_ERR_NAN, "non-finite particle coordinate");
+        std::unique_ptr<Job> j(new Job);
+        // every particle in exactly one species (MultiStealthyPotential::Energy rejects unassigned particles)
+        std::vector<char> seen((size_t)N, 0);
+        double rmax = 0.0;
+        for (int64_t q = 0; q < nspecies; q++) {
+            Require(idx_off[q + 1] >= idx_off[q] && k_off[q + 1] >= k_off[q], CCGPU_ERR_ARG, "bad species offsets");
+            j->spec_idx.emplace_back(idx + idx_off[q], idx + idx_off[q + 1]);
+            for (int64_t i : j->spec_idx.back()) {
+                if (i < 0 || i >= N) throw Error(CCGPU_ERR_RANGE, "particle index out of range");
+                if (seen[(size_t)i]++) throw Error(CCGPU_ERR_ARG, "particle listed in two species");
+            }
+            j->spec_k.emplace_back(k + k_off[q] * dim, k + k_off[q + 1] * dim);
+            j->spec_v.emplace_back(V + k_off[q], V + k_off[q + 1]);
+        }
+        for (int64_t i = 0; i < N; i++) {
+            Require(std::isfinite(radii[i]) && radii[i] >= 0, CCGPU_ERR_ARG, "bad radius");
+            rmax = std::max(rmax, radii[i]);
+        }
+        Require(rc >= 2.0 * rmax || std::abs(epsilon) <= 1e-10, CCGPU_ERR_ARG, "contact cutoff below the largest pair diameter");
+        j->d = dim;
+        j->N = N;
+        j->basis.assign(basis, basis + dim * dim);
+        j->s0.resize(N * dim);
+        for (int64_t i = 0; i < N * dim; i++) j->s0[i] = WrapUnit(relative[i]);
+        j->radii.assign(radii, radii + N);
+        j->rc = rc;
+        j->u0 = u0;
+        j->e0 = std::abs(epsilon) > 1e-10 ? epsilon : 0.0;
+        j->maxstep = maxstep;
+        j->Emin = Emin;
+        j->mf = history;
+        j->precision = 0;  // the species contractions of a mixture run in FP64
+        *job = s->solver.Submit(std::move(j));
+    });
+}
+
 int ccgpu_solver_step(ccgpu_solver *s, int evaluations, int *pending)
 {
     return Guard([&] {

@@ -65,6 +65,16 @@ struct Job {
     int precision = 0;      // 0: FP64; 1: mixed until E < polish or the end of the first pass
     double polish = 1e-8;
     int Tiles_seg = 0;
+    // multi-species request (MultiStealthyPotential): each species' particle list (caller's indices, its own
+    // order), constraints, the radii, u0 and the contact cutoff; e0 and the history as above.  The particles are
+    // stored species by species (perm: storage slot -> particle); empty for a single-species request.
+    std::vector<std::vector<int64_t>> spec_idx;
+    std::vector<std::vector<double>> spec_k, spec_v;
+    std::vector<double> radii;
+    std::vector<int64_t> perm;
+    double u0 = 1.0, rc = 0.0;
+    bool mixture() const { return !spec_idx.empty(); }
+    int64_t Work() const;  // pair visits of one evaluation plus the history passes (lane balancing weight)
     // lifecycle
     enum State { kQueued, kResident, kDone, kFailed } state = kQueued;
     double t_submit = 0, t_admit = 0, t_done = 0;  // seconds on the solver clock

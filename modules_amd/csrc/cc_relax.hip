@@ -643,15 +643,35 @@ double Solver::Now() const
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count() - t0_;
 }
 
+int64_t Job::Work() const
+{
+    int64_t w = 2 * (int64_t)mf * N;
+    if (mixture()) {
+        for (size_t q = 0; q < spec_idx.size(); q++) w += (int64_t)spec_idx[q].size() * (int64_t)spec_v[q].size();
+    } else {
+        w += N * (int64_t)v.size();
+    }
+    return w;
+}
+
 int64_t Solver::EstimateBytes(const Job &j)
 {
     const int64_t n = j.N * j.d;
-    const int64_t M = (int64_t)j.v.size();
-    const int64_t slots = M + M / 2 + 64;          // segment padding
-    const int64_t pchunks = (j.N + 63) / 64;       // upper bound of the rho chunks
-    const int64_t kchunks = 16;
-    return 8 * (n * (2 * j.mf + 6) + j.N * 10 + slots * (2 * std::min<int64_t>(pchunks, 64) + 8) + kchunks * j.N * 4 +
-                j.N * 8) + 4096;
+    // constraint-side buffers of one system of Np particles and M constraints
+    auto cc = [](int64_t Np, int64_t M) {
+        const int64_t slots = M + M / 2 + 64;        // segment padding
+        const int64_t pchunks = (Np + 63) / 64;      // upper bound of the rho chunks
+        const int64_t kchunks = 16;
+        return 8 * (slots * (2 * std::min<int64_t>(pchunks, 64) + 8) + kchunks * Np * 4) + 4096;
+    };
+    int64_t b = 8 * (n * (2 * j.mf + 6) + j.N * 10 + j.N * 8) + 4096;  // positions, history, forces, cell list
+    if (j.mixture()) {
+        b += 12 * j.N;  // radii and particle ids
+        for (size_t q = 0; q < j.spec_idx.size(); q++) b += cc((int64_t)j.spec_idx[q].size(), (int64_t)j.spec_v[q].size());
+    } else {
+        b += cc(j.N, (int64_t)j.v.size());
+    }
+    return b;
 }
 
 int64_t Solver::Submit(std::unique_ptr<Job> job)
@@ -684,7 +704,9 @@ void Solver::Release(int64_t id)
 void Solver::Admit()
 {
     // uploads of every job admitted in this call are queued back to back and synchronised once at the end
-    std::vector<std::vector<double>> staged;
+    // (deques: references to staged buffers stay valid while more are added)
+    std::deque<std::vector<double>> staged;
+    std::deque<std::vector<int32_t>> staged_ids;
     while (!queue_.empty() && (int)resident_.size() < max_slots_ && bytes_in_use_ + queue_.front()->bytes <= budget_) {
         Job *j = queue_.front();
         queue_.pop_front();
@@ -697,14 +719,34 @@ void Solver::Admit()
             j->sys.reset(new System(j->d));
             System &S = *j->sys;
             S.SetCell(j->basis.data());
-            S.SetConstraints((int64_t)j->v.size(), j->k.data(), j->v.data());
             S.N = j->N;
-            S.shift = j->shift;
             S.e0 = j->e0;
             S.sigma = j->sigma;
             S.rescale = kRelaxRescale[0];
-            if (j->Tiles_seg) S.tiles.seg = j->Tiles_seg;
-            S.precision = j->precision;
+            if (j->mixture()) {
+                // the mixture holds the particles species by species; each species is a view system over its range
+                S.mixture = true;
+                S.u0 = j->u0;
+                S.sigma = j->rc;
+                j->perm.clear();
+                for (size_t q = 0; q < j->spec_idx.size(); q++) {
+                    std::unique_ptr<System> c(new System(j->d));
+                    c->parent = &S;
+                    c->SetCell(j->basis.data());
+                    c->SetConstraints((int64_t)j->spec_v[q].size(), j->spec_k[q].data(), j->spec_v[q].data());
+                    c->N = (int64_t)j->spec_idx[q].size();
+                    c->view_off = (int64_t)j->perm.size();
+                    c->fscale = j->N > 0 ? j->u0 * (double)c->N / (double)j->N : 0.0;
+                    if (j->Tiles_seg) c->tiles.seg = j->Tiles_seg;
+                    j->perm.insert(j->perm.end(), j->spec_idx[q].begin(), j->spec_idx[q].end());
+                    S.children.push_back(std::move(c));
+                }
+            } else {
+                S.SetConstraints((int64_t)j->v.size(), j->k.data(), j->v.data());
+                S.shift = j->shift;
+                if (j->Tiles_seg) S.tiles.seg = j->Tiles_seg;
+                S.precision = j->precision;
+            }
             S.Prepare(L.batch.stream);
             const int64_t n = j->N * j->d;
             j->sdir.Reserve(std::max<int64_t>(n, 1));
@@ -720,7 +762,25 @@ void Solver::Admit()
             // x0 then the zero uv block in one host buffer
             staged.emplace_back(n + 2 * j->mf, 0.0);
             std::vector<double> &x0 = staged.back();
-            for (int64_t i = 0; i < n; i++) x0[i] = j->s0[i] * kRelaxRescale[0];  // GetParam (:758-771)
+            if (j->mixture()) {
+                for (int64_t p = 0; p < j->N; p++)
+                    for (int c = 0; c < j->d; c++)
+                        x0[p * j->d + c] = j->s0[j->perm[p] * j->d + c] * kRelaxRescale[0];  // GetParam (:758-771)
+                if (j->N > 0) {
+                    staged.emplace_back(j->N);
+                    staged_ids.emplace_back(j->N);
+                    for (int64_t p = 0; p < j->N; p++) {
+                        staged.back()[p] = j->radii[j->perm[p]];
+                        staged_ids.back()[p] = (int32_t)j->perm[p];
+                    }
+                    CC_HIP(hipMemcpyAsync(S.radius.get(), staged.back().data(), j->N * sizeof(double),
+                                          hipMemcpyHostToDevice, L.batch.stream));
+                    CC_HIP(hipMemcpyAsync(S.pid.get(), staged_ids.back().data(), j->N * sizeof(int32_t),
+                                          hipMemcpyHostToDevice, L.batch.stream));
+                }
+            } else {
+                for (int64_t i = 0; i < n; i++) x0[i] = j->s0[i] * kRelaxRescale[0];  // GetParam (:758-771)
+            }
             if (n > 0)
                 CC_HIP(hipMemcpyAsync(S.x.get(), x0.data(), n * sizeof(double), hipMemcpyHostToDevice, L.batch.stream));
             CC_HIP(hipMemcpyAsync(j->uv.get(), x0.data() + n, 2 * j->mf * sizeof(double), hipMemcpyHostToDevice,
@@ -739,7 +799,7 @@ void Solver::Admit()
             bytes_in_use_ += j->bytes;
             resident_.push_back(j);
             L.jobs.push_back(j);
-            L.work += j->N * ((int64_t)j->v.size() + 2 * j->mf);
+            L.work += j->Work();
             L.dirty = true;
         } catch (const Error &e) {
             j->state = Job::kFailed;
@@ -847,7 +907,7 @@ void Solver::FinishHarvest()
                 L.slots[q] = nullptr;
                 idle++;
                 any = true;
-                L.work -= j->N * ((int64_t)j->v.size() + 2 * j->mf);
+                L.work -= j->Work();
             }
         }
         L.jobs.swap(keep);
@@ -873,7 +933,13 @@ void Solver::FinishHarvest()
         const int64_t n = j->N * j->d;
         const double r = hs.pass <= 2 ? kRelaxRescale[hs.pass] : 1.0;
         j->s_final.resize(n);
-        for (int64_t e = 0; e < n; e++) j->s_final[e] = WrapUnit(xs[i][e] / r);
+        if (j->mixture()) {  // storage order back to the caller's particle order
+            for (int64_t p = 0; p < j->N; p++)
+                for (int c = 0; c < j->d; c++)
+                    j->s_final[j->perm[p] * j->d + c] = WrapUnit(xs[i][p * j->d + c] / r);
+        } else {
+            for (int64_t e = 0; e < n; e++) j->s_final[e] = WrapUnit(xs[i][e] / r);
+        }
         j->E = hs.E;
         j->evals = hs.evals;
         j->device_evals = hs.device_evals;

@@ -80,7 +80,30 @@ def _potential_terms(pot):
     return k, V, float(pot.Shift), float(e0), float(sigma)
 
 
-def CollectiveCoordinateMultiRun(pConfig: Configuration, pPotential: ShiftedCCPotential, gen, Prefix="",
+def _multi_terms(pot):
+    """Species lists, constraints, radii and scales of a MultiStealthyPotential, as the solver takes them."""
+    d = pot.Dimension
+    pot._require()
+    lists = [sp.idx for sp in pot.species_constraints]
+    idx_off = np.zeros(len(lists) + 1, dtype=np.int64)
+    k_off = np.zeros(len(lists) + 1, dtype=np.int64)
+    ks, vs = [], []
+    for q, sp in enumerate(pot.species_constraints):
+        idx_off[q + 1] = idx_off[q] + len(lists[q])
+        cons = sp.CCPotential.constraints
+        k_off[q + 1] = k_off[q] + len(cons)
+        ks.append(np.array([c.k[:d] for c in cons], dtype=np.float64).reshape(-1, d))
+        vs.append(np.array([c.V for c in cons], dtype=np.float64))
+    return {"idx_off": idx_off, "k_off": k_off,
+            "idx": np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.int64) for x in lists] +
+                                                       [np.zeros(1, dtype=np.int64)])),
+            "k": np.ascontiguousarray(np.concatenate(ks + [np.zeros((1, d))])),
+            "V": np.ascontiguousarray(np.concatenate(vs + [np.zeros(1)])),
+            "radii": np.ascontiguousarray(pot.GetRadii(), dtype=np.float64), "rc": float(pot.GetRc()),
+            "u0": float(pot.u0), "e0": float(pot.e0) if pot.WithRepulsion else 0.0}
+
+
+def CollectiveCoordinateMultiRun(pConfig: Configuration, pPotential, gen, Prefix="",
                                  SampleNumber=1, TimeLimit=None, AlgorithmName="LBFGS", E_max=1e-14, max_num=10000,
                                  onAccepted=None, max_slots=64, verbose=False):
     """CollectiveCoordinateMultiRun (apps/CCOptimization/main.cpp:1811-1872) on the batched device solver.
@@ -88,7 +111,8 @@ def CollectiveCoordinateMultiRun(pConfig: Configuration, pPotential: ShiftedCCPo
     Sample i starts from pConfig for i = 0 and from the next N*d draws of `gen` (a RandomGenerator) after
     sample i-1, exactly the starts of the reference's serial loop; each is relaxed with RelaxStructure_NLOPT
     (MaxStep = max_num), its energy is re-evaluated with pPotential and it is accepted when E < E_max.  All
-    samples are submitted at once and relax side by side.  Results are handed out in sample order, so
+    samples are submitted at once and relax side by side.  pPotential is a ShiftedCCPotential (optionally with the
+    soft core) or a MultiStealthyPotential (GetMultiCCO's ground mode); a mixture keeps its species and radii.  Results are handed out in sample order, so
     onAccepted(config, success_index) sees the reference's order.  TimeLimit (absolute time.time() seconds, as
     the reference's time_t) is checked after each sample; reaching it ends the run without re-randomizing
     pConfig.  Returns {"init": [...], "relaxed": [...], "success": [...], "energies": [...]}; the lists stand in
@@ -97,7 +121,9 @@ def CollectiveCoordinateMultiRun(pConfig: Configuration, pPotential: ShiftedCCPo
     if AlgorithmName != "LBFGS":
         raise NotImplementedError(f"CollectiveCoordinateMultiRun: only LBFGS is on this path (got {AlgorithmName})")
     N, d = pConfig.NumParticle(), pConfig.GetDimension()
-    k, V, shift, e0, sigma = _potential_terms(pPotential)
+    multi = hasattr(pPotential, "species_constraints")  # MultiStealthyPotential (GetMultiCCO, main.cpp:1565-1590)
+    if not multi:
+        k, V, shift, e0, sigma = _potential_terms(pPotential)
     starts = [pConfig.copy()]
     pos0 = getattr(gen, "_pos", None)
     for _ in range(1, SampleNumber):
@@ -106,7 +132,10 @@ def CollectiveCoordinateMultiRun(pConfig: Configuration, pPotential: ShiftedCCPo
     if SampleNumber <= 0:
         return out
     solver = BatchSolver(max_slots=max(1, min(max_slots, SampleNumber)))
-    ids = [solver.submit(c, k, V, shift=shift, epsilon=e0, sigma=sigma, MaxStep=max_num) for c in starts]
+    if multi:
+        ids = [solver.submit_multi(c, pPotential, MaxStep=max_num) for c in starts]
+    else:
+        ids = [solver.submit(c, k, V, shift=shift, epsilon=e0, sigma=sigma, MaxStep=max_num) for c in starts]
     nxt = 0
     while nxt < SampleNumber:
         pending = solver.step(256)
@@ -189,6 +218,26 @@ class BatchSolver:
         check(lib().ccgpu_solver_submit(self._h, d, _p(np.ascontiguousarray(config.basis)), rel.shape[0], _p(rel),
                                         k.shape[0], _p(k), _p(V), shift, epsilon, sigma, int(MaxStep), float(Emin),
                                         int(history), C.byref(job)))
+        self._shapes[job.value] = rel.shape
+        return job.value
+
+    def submit_multi(self, config: Configuration, pot, MaxStep=10000, Emin=-MaxEnergy,
+                     history=StructureOptimization_LBFGSHistory):
+        """A relax request with a MultiStealthyPotential `pot` (its species lists, constraints, radii, u0, e0 and
+        contact cutoff) from the positions of `config` (same particles as pot's configuration)."""
+        d = config.GetDimension()
+        rel = np.ascontiguousarray(config.relative, dtype=np.float64)
+        N = rel.shape[0]
+        terms = _multi_terms(pot)
+        if len(terms["radii"]) != N:
+            raise ValueError("configuration and potential hold different particle numbers")
+        i64 = C.POINTER(C.c_int64)
+        job = C.c_int64(0)
+        check(lib().ccgpu_solver_submit_multi(
+            self._h, d, _p(np.ascontiguousarray(config.basis)), N, _p(rel), len(terms["idx_off"]) - 1,
+            terms["idx_off"].ctypes.data_as(i64), terms["idx"].ctypes.data_as(i64), terms["k_off"].ctypes.data_as(i64),
+            _p(terms["k"]), _p(terms["V"]), _p(terms["radii"]), terms["rc"], terms["u0"], terms["e0"], int(MaxStep),
+            float(Emin), int(history), C.byref(job)))
         self._shapes[job.value] = rel.shape
         return job.value
 

@@ -215,3 +215,64 @@ def test_relax_two_species_lowers_energy_to_a_ground_state():
     assert p.Energy() == E
     Eo, Fo, _ = oracle(p._config, radii, idx, ks, vs, e0)
     assert abs(E - Eo) <= 1e-12 * energy_scale(p._config, idx, vs, e0)
+
+
+def mixture_job(seed, sizes=(60, 60), chi=0.1, d=2, phi=0.2):
+    cfg, radii, idx, ks, vs, e0 = build((d, False, sizes, chi, 1.0, phi, False, 0), seed_shift=seed)
+    return cfg, make_potential(cfg, radii, idx, ks, vs, e0), (radii, idx, ks, vs, e0)
+
+
+def test_solver_mixture_jobs_reach_ground_states_beside_single_species_jobs():
+    from modules_amd import GetKs, RandomConfiguration, getK
+    from modules_amd.optimize import BatchSolver
+    jobs = [mixture_job(s) for s in range(3)] + [mixture_job(7, sizes=(40, 50, 30), d=3)]
+    solver = BatchSolver(max_slots=8, check_interval=8)
+    ids = [solver.submit_multi(cfg, p, Emin=1e-20) for cfg, p, _ in jobs]
+    single = RandomConfiguration(3, 200, 2)
+    ks = GetKs(single, getK(0.3, 1.0, 2))
+    sid = solver.submit(single, ks, np.ones(len(ks)), Emin=1e-20)
+    solver.run(timeout=300)
+    for (cfg, p, (radii, idx, kk, vv, e0)), j in zip(jobs, ids):
+        r = solver.result(j)
+        assert r["state"] == BatchSolver.DONE
+        out = Configuration(cfg.basis, r["relative"])
+        p.SetConfiguration(out)
+        E = p.Energy()
+        print(f"solver mixture job: E={r['energy']:.4g} evaluations={r['evaluations']} status={r['status']}",
+              flush=True)
+        assert abs(E - r["energy"]) <= 1e-12 * energy_scale(out, idx, vv, e0)
+        assert r["energy"] < 1e-14
+        Eo, _, _ = oracle(out, radii, idx, kk, vv, e0)
+        assert abs(Eo - r["energy"]) <= 1e-12 * energy_scale(out, idx, vv, e0)
+    assert solver.result(sid)["energy"] < 1e-18
+
+
+def test_solver_mixture_job_same_bits_alone_and_in_a_mix():
+    from modules_amd.optimize import BatchSolver
+    cfg, p, _ = mixture_job(11, sizes=(50, 40, 30))
+    alone = BatchSolver(max_slots=2, check_interval=8, lanes=1)
+    a = alone.submit_multi(cfg, p, MaxStep=300)
+    alone.run(timeout=300)
+    ra = alone.result(a)
+    mix = BatchSolver(max_slots=6, check_interval=8, lanes=2)
+    others = [mixture_job(s)[0:2] for s in (12, 13)]
+    for c, q in others[:1]:
+        mix.submit_multi(c, q, MaxStep=300)
+    b = mix.submit_multi(cfg, p, MaxStep=300)
+    for c, q in others[1:]:
+        mix.submit_multi(c, q, MaxStep=300)
+    mix.run(timeout=300)
+    rb = mix.result(b)
+    assert ra["energy"] == rb["energy"] and ra["evaluations"] == rb["evaluations"]
+    assert np.array_equal(ra["relative"], rb["relative"])
+
+
+def test_multirun_with_a_mixture():
+    from modules_amd.optimize import CollectiveCoordinateMultiRun
+    from modules_amd.potential import RandomGenerator
+    cfg, p, _ = mixture_job(21)
+    gen = RandomGenerator(5)
+    out = CollectiveCoordinateMultiRun(cfg, p, gen, SampleNumber=3, E_max=1e-14, max_num=10000)
+    print("multirun mixture energies:", out["energies"], flush=True)
+    assert len(out["relaxed"]) == 3
+    assert len(out["success"]) == sum(e < 1e-14 for e in out["energies"]) >= 2
