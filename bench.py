@@ -58,11 +58,12 @@ def parse():
                     help="batched / mixed256: FP64, or FP32 phases and force terms with FP64 sums and an FP64 "
                          "polish below 1e-8")
     ap.add_argument("--workload", default="batched",
-                    choices=["batched", "single", "mixed256", "large2d", "mixed2d", "serve"],
+                    choices=["batched", "single", "mixed256", "large2d", "mixed2d", "serve", "five"],
                     help="batched: the headline (64 seeds, 3D N=2000 chi=0.45); single: the same solves through the "
                          "single-solve drop-in path; mixed256: 256 configurations of mixed N, d, chi; large2d: 2D "
                          "N=20000 chi=0.4; mixed2d: the same solved in FP64 and in mixed precision, with the "
-                         "mixed-precision error; serve: Poisson request stream with soft core")
+                         "mixed-precision error; serve: Poisson request stream with soft core; five: multi-species "
+                         "ground states of the reference's five-component study (2D N=2000, 5 x 400)")
     ap.add_argument("--requests", type=int, default=48, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=20.0, help="serve: arrivals per second")
     ap.add_argument("--profiler-run", action="store_true",
@@ -400,8 +401,84 @@ def workload_serve(args):
             "N_median": float(np.median([r.N for r in reqs])), "dtype": "f64"}
 
 
+def _shell_kset(cfg, target):
+    """Wavevectors of the complete |k| shells whose count is nearest to `target` (GetKs over a K between two
+    shells)."""
+    from modules_amd import GetKs
+    k_all = GetKs(cfg, 2.0 * math.pi * math.sqrt(2.0 * target / math.pi) / cfg.basis[0, 0] * 4.0)
+    m2 = np.round(np.sum(k_all ** 2, axis=1), 9)
+    shells = np.unique(m2)
+    counts = np.array([np.sum(m2 <= q) for q in shells])
+    i = int(np.argmin(np.abs(counts - target)))
+    K = math.sqrt(0.5 * (shells[i] + shells[i + 1]))
+    k = GetKs(cfg, K)
+    assert len(k) == counts[i]
+    return k
+
+
+def five_component_setup(seed, N=2000, species=5, chi=0.10, phi=0.2):
+    """The reference's five-component study (docs/finufft/reports/five-component-original.md): 2D, N=2000 (400
+    per species, contiguous blocks as GetMultiCCO's random mode assigns them, apps/CCOptimization/main.cpp:
+    1195-1212), equal radii sqrt(phi/pi) (total packing fraction 0.2 at density 1), u0 = e0 = 1, every species
+    constrained on the complete shells nearest chi_i = M / (2 (N_s - 1))."""
+    from modules_amd import MultiStealthyPotential, RandomConfiguration
+    cfg = RandomConfiguration(seed, N, 2)
+    n_s = N // species
+    k = _shell_kset(cfg, chi * 2 * (n_s - 1))
+    pot = MultiStealthyPotential(2, 1.0)
+    pot.SetConfiguration(cfg)
+    pot.SetNumSpecies(species)
+    pot.SetSecureRadii(np.full(N, math.sqrt(phi / math.pi)))
+    for q in range(species):
+        for i in range(q * n_s, (q + 1) * n_s):
+            pot.SetSpeciesOfParticle(i, q)
+        for kk in k:
+            pot.AddConstraint(q, kk, [1.0])
+    return cfg, pot, k
+
+
+def workload_five(args):
+    """Multi-species ground states (MultiStealthyPotential): the reference's five-component study shape, --batch
+    random starts relaxed side by side with RelaxStructure_NLOPT (three passes, at most 2000 evaluations each, the
+    study's limits) and accepted at E <= 1e-11 (its acceptance); plus the single-solve drop-in path on a few
+    starts.  The study's direct-sum solves took 6.46, 6.61 and 5.77 s per configuration on 4 threads
+    (docs/finufft/reports/five-component-original.md:17-21).  (The oracle is no CPU baseline here: its contact
+    term is an all-pairs image sum, not the reference's cell-list walk.)"""
+    from modules_amd.optimize import BatchSolver
+    B = args.batch
+    maxstep = 2000
+    setups = [five_component_setup(s) for s in range(B)]
+    M = len(setups[0][2])
+    solver = BatchSolver(max_slots=max(1, B), check_interval=args.check_interval, lanes=args.lanes)
+    jobs = [solver.submit_multi(cfg, pot, MaxStep=maxstep) for cfg, pot, _ in setups]
+    res, wall = _solve_all(solver, jobs, 1e-11)
+    energies = np.array([r["energy"] for r in res])
+    accepted = int(np.sum(energies <= 1e-11))
+    lat = np.array([r["t_done"] - r["t_admit"] for r in res])
+    # single-solve drop-in path (host L-BFGS, one GPU E+F per objective evaluation) on the first starts
+    n_single = min(3, B)
+    single = []
+    for cfg, pot, _ in setups[:n_single]:
+        pot.SetConfiguration(cfg)
+        t0 = time.perf_counter()
+        E, ev, st = pot.Relax(maxstep=maxstep)
+        single.append((time.perf_counter() - t0, E, ev))
+    evals = np.array([r["device_evaluations"] for r in res])
+    return {"metric": "multi-species ground states/s at E<=1e-11", "value": accepted / wall,
+            "unit": "ground_states/s",
+            "workload": f"five-component 2D N=2000 (5 x 400), chi_i={M}/798, phi=0.2, u0=e0=1: {B} starts "
+                        "relaxed side by side",
+            "accepted": accepted, "solves": B, "wall_s": wall, "energy_max": float(energies.max()),
+            "latency_mean_s": float(lat.mean()), "latency_p99_s": float(np.percentile(lat, 99)),
+            "device_evals_mean": float(evals.mean()), "ef_evals_per_s": float(evals.sum() / wall),
+            "single_solve": {"seconds": [x[0] for x in single], "energy": [x[1] for x in single],
+                             "evaluations": [x[2] for x in single]},
+            "published_cpu_s_per_config_4_threads": [6.459, 6.608, 5.772],
+            "vs_published_4_thread_rate": (accepted / wall) / (1.0 / 6.28), "dtype": "f64"}
+
+
 OTHER_WORKLOADS = {"single": workload_single, "mixed256": workload_mixed256, "large2d": workload_large2d,
-                   "mixed2d": workload_mixed2d, "serve": workload_serve}
+                   "mixed2d": workload_mixed2d, "serve": workload_serve, "five": workload_five}
 
 
 def _cpu_model():
