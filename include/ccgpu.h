@@ -112,6 +112,16 @@ int ccgpu_potential_set_rho_kernel(ccgpu_potential *p, int variant);
  * is reported by the caller.  The general (non-lattice) path is FP64 in either mode. */
 int ccgpu_potential_set_precision(ccgpu_potential *p, int mode);
 
+/* Collective-density backend -- SetRhoBackend / SetFinufftTolerance / SetFinufftMaxModes
+ * (potential/CollectiveCoordinatePotential.h:160-212): 0 the direct sums (default); 1 the NUFFT (a Type 1
+ * transform for rho on a fine grid and a Type 2 transform for the forces, requested tolerance eps, default 1e-12
+ * in the reference) when the constraint set is eligible -- lattice vectors in 1-3 dimensions, a mode box
+ * prod (2 K_c + 1) of at most max_modes (2^22 in the reference), no constraint repeating another's mode or its
+ * conjugate; otherwise the direct sums are used (GetLastRhoBackendUsed tells which).  Invalidates rho. */
+int ccgpu_potential_set_rho_backend(ccgpu_potential *p, int backend, double eps, int64_t max_modes);
+/* GetLastRhoBackendUsed (CollectiveCoordinatePotential.h:268): 1 if the bound system evaluates by the NUFFT. */
+int ccgpu_potential_rho_backend_used(ccgpu_potential *p, int *backend);
+
 /* Which kernel family the bound constraint set uses: 1 lattice (segment recurrence), 0 general. */
 int ccgpu_potential_uses_lattice_path(const ccgpu_potential *p, int *lattice);
 
@@ -266,6 +276,9 @@ int ccgpu_solver_set_graph(ccgpu_solver *s, int on);
  * ends, then FP64 for the rest of the solve: the configuration at the switch is evaluated again in FP64 and
  * the pass restarts from it, so the final energy and every stop test are FP64. */
 int ccgpu_solver_set_precision(ccgpu_solver *s, int mode, double polish_below);
+/* Collective-density backend of jobs submitted after the call (as ccgpu_potential_set_rho_backend; the NUFFT only
+ * for eligible single-species jobs, FP64).  No counterpart in the reference's solve loop. */
+int ccgpu_solver_set_rho_backend(ccgpu_solver *s, int backend, double eps);
 /* Device evaluation at which a mixed-precision job switched to FP64 (0: FP64 throughout). */
 int ccgpu_solver_switch_eval(ccgpu_solver *s, int64_t job, int *evaluation);
 /* State of a job: 0 queued, 1 resident, 2 done, 3 failed.  When done: final energy, objective evaluations,

@@ -283,6 +283,30 @@ int ccgpu_potential_set_tiles(ccgpu_potential *p, int segment_len, int rho_chunk
     });
 }
 
+int ccgpu_potential_set_rho_backend(ccgpu_potential *p, int backend, double eps, int64_t max_modes)
+{
+    return Guard([&] {
+        Require(p != nullptr, CCGPU_ERR_ARG, "null handle");
+        Require(backend == 0 || backend == 1, CCGPU_ERR_ARG, "backend must be 0 (direct) or 1 (NUFFT)");
+        Require(eps > 0.0 && std::isfinite(eps), CCGPU_ERR_ARG, "tolerance must be positive and finite");
+        Require(max_modes >= 1, CCGPU_ERR_ARG, "max modes must be >= 1");
+        p->sys.rho_backend = backend;
+        p->sys.nufft_eps = eps;
+        p->sys.nufft_max_modes = max_modes;
+        p->batch.dirty = true;
+        p->Invalidate();
+    });
+}
+
+int ccgpu_potential_rho_backend_used(ccgpu_potential *p, int *backend)
+{
+    return Guard([&] {
+        Require(p && backend, CCGPU_ERR_ARG, "null argument");
+        p->EnsureBuilt();
+        *backend = p->sys.nufft_on() ? 1 : 0;
+    });
+}
+
 int ccgpu_potential_relax(ccgpu_potential *p, double *relative, int maxstep, double Emin, int history, double *energy,
                           int *evaluations, int *status)
 {
@@ -485,6 +509,8 @@ int ccgpu_solver_submit(ccgpu_solver *s, int dim, const double *basis, int64_t N
         j->mf = history;
         j->precision = s->solver.precision();
         j->polish = s->solver.polish();
+        j->rho_backend = s->solver.rho_backend();
+        j->nufft_eps = s->solver.nufft_eps();
         *job = s->solver.Submit(std::move(j));
     });
 }
@@ -564,6 +590,16 @@ int ccgpu_solver_set_precision(ccgpu_solver *s, int mode, double polish_below)
         Require(s != nullptr && (mode == 0 || mode == 1), CCGPU_ERR_ARG, "precision mode must be 0 (FP64) or 1 (mixed)");
         Require(std::isfinite(polish_below) && polish_below >= 0, CCGPU_ERR_ARG, "polish threshold must be >= 0");
         s->solver.SetPrecision(mode, polish_below);
+    });
+}
+
+int ccgpu_solver_set_rho_backend(ccgpu_solver *s, int backend, double eps)
+{
+    return Guard([&] {
+        Require(s != nullptr, CCGPU_ERR_ARG, "null handle");
+        Require(backend == 0 || backend == 1, CCGPU_ERR_ARG, "backend must be 0 (direct) or 1 (NUFFT)");
+        Require(eps > 0.0 && std::isfinite(eps), CCGPU_ERR_ARG, "tolerance must be positive and finite");
+        s->solver.SetRhoBackend(backend, eps);
     });
 }
 

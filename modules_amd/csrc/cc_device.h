@@ -7,6 +7,8 @@
 #include <cstdint>
 #include <hip/hip_runtime.h>
 
+#include "cc_nufft.h"
+
 namespace ccgpu {
 
 constexpr int kBlock = 256;    // threads per work item of the CC kernels (4 wavefronts of 64) ...
@@ -19,6 +21,7 @@ enum SysFlags : int32_t {
     kSysMixed = 8,     // lattice phases and force terms in FP32 (FP64 range reduction and accumulation)
     kSysView = 16,     // species of a mixture: particle arrays are a range of the parent's (no prepare pass)
     kSysMixture = 32,  // mixture: no CC term of its own; energy from its species and the radius contact term
+    kSysNufft = 64,    // lattice constraints evaluated by the NUFFT backend (rho and forces on a fine grid)
 };
 
 struct SysDesc {
@@ -79,6 +82,8 @@ struct SysDesc {
     double b[16];   // reciprocal basis rows, d x d
     double vol, inv_vol, shift, rescale;
     double sigma, e0;  // soft core: sigma (mixture: the cell-list cutoff rc) and the energy scale
+
+    NufftDesc nu;      // NUFFT backend (kSysNufft)
 };
 
 // A system takes part in a launch while it is active and, for a species view, while its mixture is.

@@ -63,6 +63,8 @@ struct Job {
     int maxstep = 10000;
     int mf = 40;
     int precision = 0;      // 0: FP64; 1: mixed until E < polish or the end of the first pass
+    int rho_backend = 0;    // 0 direct, 1 NUFFT (when the constraint set is eligible)
+    double nufft_eps = 1e-12;
     double polish = 1e-8;
     int Tiles_seg = 0;
     // multi-species request (MultiStealthyPotential): each species' particle list (caller's indices, its own
@@ -112,6 +114,9 @@ struct SolverLane {
     int graph_burst = 0;
     bool dirty = true;
     int64_t work = 0;  // sum of N (M + 2 history) of its jobs: the load-balancing weight
+    // systems of jobs harvested since the last rebuild: the batch still lists them as idle slots (their kernels
+    // skip them on the device, the host-driven FFT stage skips them by their host flag), so they live until then
+    std::vector<std::unique_ptr<System>> retired;
     void DropGraph()
     {
         if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -157,6 +162,14 @@ public:
         precision_ = mode;
         polish_ = polish;
     }
+    // rho backend of jobs submitted from now on (0 direct, 1 NUFFT with tolerance eps)
+    void SetRhoBackend(int backend, double eps)
+    {
+        rho_backend_ = backend;
+        nufft_eps_ = eps;
+    }
+    int rho_backend() const { return rho_backend_; }
+    double nufft_eps() const { return nufft_eps_; }
     int precision() const { return precision_; }
     double polish() const { return polish_; }
     // number of streams (1 or 2); only while no job is resident
@@ -183,6 +196,8 @@ private:
     bool timing_ = false;
     int precision_ = 0;
     double polish_ = 1e-8;
+    int rho_backend_ = 0;
+    double nufft_eps_ = 1e-12;
     std::deque<Job *> queue_;
     std::vector<Job *> resident_;
     std::map<int64_t, std::unique_ptr<Job>> jobs_;

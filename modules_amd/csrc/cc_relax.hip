@@ -746,6 +746,8 @@ void Solver::Admit()
                 S.shift = j->shift;
                 if (j->Tiles_seg) S.tiles.seg = j->Tiles_seg;
                 S.precision = j->precision;
+                S.rho_backend = j->rho_backend;
+                S.nufft_eps = j->nufft_eps;
             }
             S.Prepare(L.batch.stream);
             const int64_t n = j->N * j->d;
@@ -817,6 +819,7 @@ void Solver::Rebuild(SolverLane &L)
     L.DropGraph();
     Batch &batch = L.batch;
     batch.systems.clear();
+    L.retired.clear();  // the lane has drained (every rebuild follows a synchronisation)
     std::vector<RelaxDesc> rd;
     std::vector<WorkItem> wide;
     L.slots = L.jobs;
@@ -948,9 +951,12 @@ void Solver::FinishHarvest()
         j->switch_eval = hs.switch_eval;
         j->t_done = Now();
         j->state = Job::kDone;
-        // the buffers go back to the pool; kernels still running skip this job (its flag is clear) and the pool
-        // hands them out again only at an admission, which first drains every lane
-        j->sys.reset();
+        // the system stays listed in its lane's batch until the next rebuild: retire it (host flag cleared, so the
+        // host-driven FFT stage skips it too) and free it at that rebuild; the other buffers go back to the pool
+        // now -- kernels still running skip this job (its flag is clear) and the pool hands them out again only at
+        // an admission, which first drains every lane
+        j->sys->desc.flags &= ~kSysActive;
+        lanes_[j->lane]->retired.push_back(std::move(j->sys));
         j->sdir.Release();
         j->xo.Release();
         j->go.Release();
@@ -996,7 +1002,9 @@ void Solver::LaunchBurst(int burst)
                               L.d_wide.get(), L.nwide);
         }
     };
-    if (!use_graph_ || timing_) {
+    bool nufft = false;  // hipFFT calls stay on plain stream launches
+    for (auto &l : lanes_) nufft = nufft || l->batch.has_nufft;
+    if (!use_graph_ || timing_ || nufft) {
         for (int it = 0; it < burst; it++)
             for (auto &l : lanes_)
                 if (!l->jobs.empty()) enqueue(*l, 1);
@@ -1023,10 +1031,20 @@ void Solver::LaunchBurst(int burst)
 
 int Solver::Step(int evaluations)
 {
-    auto refresh = [&] {
+    auto refresh = [&] {  // every call follows SyncLanes: no lane has work in flight
         Admit();
-        for (auto &l : lanes_)
-            if (l->dirty && !l->jobs.empty()) Rebuild(*l);
+        for (auto &l : lanes_) {
+            if (l->dirty && !l->jobs.empty()) {
+                Rebuild(*l);
+            } else if (l->jobs.empty() && !l->retired.empty()) {
+                // an emptied lane launches nothing until its next rebuild: drop its list and free the systems
+                l->batch.systems.clear();
+                l->slots.clear();
+                l->retired.clear();
+                l->DropGraph();
+                l->dirty = true;
+            }
+        }
     };
     // The harvest is pipelined: each burst of `check_interval` steps is queued before the descriptor read of
     // the previous burst is examined, so the GPU runs while the host retires finished jobs.  Work lists change

@@ -312,6 +312,8 @@ void System::Prepare(hipStream_t stream)
     }
     const int64_t nslots = layout->nslots();
     const int64_t nunits = lattice ? layout->plan.nseg : M;  // force-pass loop units
+    if (rho_backend == 1 && lattice && !view()) PrepareNufft(*this, stream);
+    else if (nu) nu->active = false, nu->layout.reset(), nu->why = "direct backend selected";
 
     // Chunking of the two contractions.  Untuned shapes get enough blocks for their work: one block per ~2^18
     // (particle, constraint) pairs, at least 48 rho blocks and 24 force blocks per system and at most 4096, so a
@@ -331,6 +333,7 @@ void System::Prepare(hipStream_t stream)
         used.rho_chunk = (int)std::min<int64_t>(512, std::max<int64_t>(64, c));
     }
     pchunks = (int)std::max<int64_t>(1, (N + used.rho_chunk - 1) / used.rho_chunk);
+    if (nufft_on()) pchunks = 1;  // the NUFFT writes rho once per slot
     const int64_t pblocks = std::max<int64_t>(1, (N + kBlock * used.force_ppl - 1) / (kBlock * used.force_ppl));
     const int fc = tiles.force_chunk > 0 ? tiles.force_chunk : def.force_chunk;
     if (fc > 0) {
@@ -342,6 +345,7 @@ void System::Prepare(hipStream_t stream)
         used.force_chunk = (int)std::max<int64_t>(1, (nunits + want - 1) / want);
     }
     kchunks = (int)std::max<int64_t>(1, (nunits + used.force_chunk - 1) / used.force_chunk);
+    if (nufft_on()) kchunks = 1;  // ... and the index-space force weights once per particle
 
     // soft-core cell grid: side max(sigma, (V/N)^(1/d)) (apps/CCOptimization/main.cpp:235-240)
     ncells = 1;
@@ -391,7 +395,8 @@ void System::Prepare(hipStream_t stream)
     D.N = (int32_t)N;
     D.M = (int32_t)M;
     D.flags = kSysActive | (lattice ? kSysLattice : 0) | (softcore() ? kSysSoftCore : 0) |
-              (lattice && precision == 1 ? kSysMixed : 0) | (view() ? kSysView : 0) | (mixture ? kSysMixture : 0);
+              (lattice && precision == 1 && !nufft_on() ? kSysMixed : 0) | (view() ? kSysView : 0) |
+              (mixture ? kSysMixture : 0) | (nufft_on() ? kSysNufft : 0);
     D.nseg = (int32_t)nseg();
     D.nslots = (int32_t)nslots;
     D.pchunks = N > 0 ? pchunks : 0;  // no particles: rho sums nothing
@@ -428,6 +433,30 @@ void System::Prepare(hipStream_t stream)
     D.nchild = 0;
     D.fscale = view() ? fscale : 1.0;
     D.u0 = u0;
+    if (nufft_on()) {
+        NufftDesc &X = D.nu;
+        const NufftState &U = *nu;
+        X.grid = U.grid.get();
+        X.kv = U.kv.get();
+        X.start = U.start.get();
+        X.bin_of = U.bin_of.get();
+        X.bin_start = U.bin_start.get();
+        X.bin_items = U.bin_items.get();
+        X.p = U.p.get();
+        for (int c = 0; c < 3; c++) {
+            X.nf[c] = U.nf[c];
+            X.K[c] = U.K[c];
+            X.tile[c] = U.tile[c];
+            X.nbins[c] = U.nbins[c];
+            X.reach[c] = U.reach[c];
+            X.poff[c] = U.poff[c];
+        }
+        X.w = U.w;
+        X.nbin = U.nbin;
+        X.npts = (int32_t)U.npts;
+        X.ntile = U.nbin;
+        X.beta = U.beta;
+    }
     D.cell_of = cell_of.get();
     D.cell_start = cell_start.get();
     D.cell_items = cell_items.get();
@@ -530,7 +559,7 @@ void Batch::Build()
     std::vector<std::tuple<int, int, int>> segs;
     for (System *s : all) {
         const std::tuple<int, int, int> key(s->used.seg, s->used.rho_kernel, s->used.force_ppl);
-        if (s->lattice && std::find(segs.begin(), segs.end(), key) == segs.end()) segs.push_back(key);
+        if (s->lattice && !s->nufft_on() && std::find(segs.begin(), segs.end(), key) == segs.end()) segs.push_back(key);
     }
     for (const auto &key : segs) {
         const int sl = std::get<0>(key), rk = std::get<1>(key), ppl = std::get<2>(key);
@@ -540,13 +569,14 @@ void Batch::Build()
         g.force_ppl = ppl;
         g.any_mixed = false;
         for (System *s : all)
-            if (s->lattice && s->precision == 1 && s->used.seg == sl && s->used.rho_kernel == rk && s->used.force_ppl == ppl)
+            if (s->lattice && !s->nufft_on() && s->precision == 1 && s->used.seg == sl && s->used.rho_kernel == rk &&
+                s->used.force_ppl == ppl)
                 g.any_mixed = true;
         add(g.rho, [&] {
             for (int i = 0; i < ns; i++) {
                 System *s = all[i];
-                if (!s->lattice || s->used.seg != sl || s->used.rho_kernel != rk || s->used.force_ppl != ppl ||
-                    s->N == 0)
+                if (!s->lattice || s->nufft_on() || s->used.seg != sl || s->used.rho_kernel != rk ||
+                    s->used.force_ppl != ppl || s->N == 0)
                     continue;
                 const int bs = rk == 1 ? kBlock : kRhoBlock;  // segments per rho block
                 const int sb = (int)((s->nseg() + bs - 1) / bs);
@@ -557,7 +587,9 @@ void Batch::Build()
         add(g.force, [&] {
             for (int i = 0; i < ns; i++) {
                 System *s = all[i];
-                if (!s->lattice || s->used.seg != sl || s->used.rho_kernel != rk || s->used.force_ppl != ppl) continue;
+                if (!s->lattice || s->nufft_on() || s->used.seg != sl || s->used.rho_kernel != rk ||
+                    s->used.force_ppl != ppl)
+                    continue;
                 const int fb = (int)((s->N + kBlock * ppl - 1) / (kBlock * ppl));
                 for (int c = 0; c < s->kchunks; c++)
                     for (int b = 0; b < fb; b++) h_items.push_back({i, b, c, 0});
@@ -615,6 +647,43 @@ void Batch::Build()
             if (all[i]->softcore())
                 for (int b = 0; b < (all[i]->ncells + kBlock - 1) / kBlock; b++) h_items.push_back({i, b, 0, 0});
     });
+    // NUFFT systems: particle blocks (with their count, the zeroing stride), one item per system, bin blocks,
+    // output tiles, slot blocks and grid blocks (with their count)
+    has_nufft = false;
+    add(r_nu_part, [&] {
+        for (int i = 0; i < ns; i++)
+            if (all[i]->nufft_on()) {
+                has_nufft = true;
+                for (int b = 0; b < pblocks(all[i]); b++) h_items.push_back({i, b, pblocks(all[i]), 0});
+            }
+    });
+    add(r_nu_sys, [&] {
+        for (int i = 0; i < ns; i++)
+            if (all[i]->nufft_on()) h_items.push_back({i, 0, 0, 0});
+    });
+    add(r_nu_bins, [&] {
+        for (int i = 0; i < ns; i++)
+            if (all[i]->nufft_on())
+                for (int b = 0; b < (all[i]->nu->nbin + kBlock - 1) / kBlock; b++) h_items.push_back({i, b, 0, 0});
+    });
+    add(r_nu_tiles, [&] {
+        for (int i = 0; i < ns; i++)
+            if (all[i]->nufft_on())
+                for (int t = 0; t < all[i]->nu->nbin; t++) h_items.push_back({i, t, 0, 0});
+    });
+    add(r_nu_slots, [&] {
+        for (int i = 0; i < ns; i++)
+            if (all[i]->nufft_on())
+                for (int b = 0; b < (all[i]->desc.nslots + kBlock - 1) / kBlock; b++) h_items.push_back({i, b, 0, 0});
+    });
+    add(r_nu_grid, [&] {
+        for (int i = 0; i < ns; i++)
+            if (all[i]->nufft_on()) {
+                const int64_t total = all[i]->nu->npts * (all[i]->d == 3 ? 2 : 1);
+                const int nb = (int)std::min<int64_t>(2048, (total + kBlock - 1) / kBlock);
+                for (int b = 0; b < nb; b++) h_items.push_back({i, b, nb, 0});
+            }
+    });
     d_desc.Reserve(std::max<size_t>(1, descs.size()));
     d_items.Reserve(std::max<size_t>(1, h_items.size()));
     if (!descs.empty())
@@ -646,6 +715,12 @@ void Batch::Enqueue(bool rho, bool forces)
             if (g.any_mixed) launch_rho_lattice(stream, g.seg, D, I + g.rho.off, g.rho.n, 2);
         }
         launch_rho_general(stream, D, I + r_rho_gen.off, r_rho_gen.n);
+        if (has_nufft) {
+            launch_nufft_rho(stream, D, I + r_nu_part.off, r_nu_part.n, I + r_nu_sys.off, r_nu_sys.n, I + r_nu_bins.off,
+                             r_nu_bins.n, I + r_nu_tiles.off, r_nu_tiles.n);
+            NufftFft(true);
+            launch_nufft_extract(stream, D, I + r_nu_slots.off, r_nu_slots.n);
+        }
         if (timing) {
             mk.e1 = NextEvent();
             CC_HIP(hipEventRecord(ev_pool_[mk.e1], stream));
@@ -663,6 +738,11 @@ void Batch::Enqueue(bool rho, bool forces)
         }
         for (const SegGroup &g : seg_groups) launch_force_lattice(stream, g.seg, D, I + g.force.off, g.force.n, g.force_ppl);
         launch_force_general(stream, D, I + r_f_gen.off, r_f_gen.n);
+        if (has_nufft) {
+            launch_nufft_coef(stream, D, I + r_nu_grid.off, r_nu_grid.n, I + r_nu_slots.off, r_nu_slots.n);
+            NufftFft(false);
+            launch_nufft_interp(stream, D, I + r_nu_part.off, r_nu_part.n);
+        }
         if (timing) {
             mk.e1 = NextEvent();
             CC_HIP(hipEventRecord(ev_pool_[mk.e1], stream));
@@ -674,6 +754,21 @@ void Batch::Enqueue(bool rho, bool forces)
 }
 
 void Batch::Sync() { CC_HIP(hipStreamSynchronize(stream)); }
+
+void Batch::NufftFft(bool type1)
+{
+    // in place on each NUFFT system's grid(s): Type 1 with exp(+i) (hipFFT's backward), Type 2 with exp(-i)
+    for (System *s : all) {
+        if (!s->nufft_on() || !(s->desc.flags & kSysActive)) continue;
+        NufftState &U = *s->nu;
+        hipfftHandle plan = type1 ? U.plan1 : U.plan2;
+        if (hipfftSetStream(plan, stream) != HIPFFT_SUCCESS ||
+            hipfftExecZ2Z(plan, reinterpret_cast<hipfftDoubleComplex *>(U.grid.get()),
+                          reinterpret_cast<hipfftDoubleComplex *>(U.grid.get()),
+                          type1 ? HIPFFT_BACKWARD : HIPFFT_FORWARD) != HIPFFT_SUCCESS)
+            throw Error(kErrDevice, "hipFFT execution failed");
+    }
+}
 
 double Autotune(int d, int64_t N, const double *basis, int64_t M, const double *k, const double *v, int batch,
                 int repeats, Tiles &best)

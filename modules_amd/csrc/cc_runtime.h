@@ -3,6 +3,7 @@
 // launches.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
 
 #include <cmath>
 #include <climits>
@@ -117,6 +118,36 @@ struct ConstraintLayout {
 std::shared_ptr<const ConstraintLayout> GetLayout(int d, const double *a, int64_t M, const double *k, const double *v,
                                                   int S, hipStream_t stream);
 
+// NUFFT plan of one system (cc_nufft.h): grid shape, kernel, deconvolution tables, device buffers and the two
+// hipFFT plans (Type 1 exp(+i) on one grid; Type 2 exp(-i) on one grid, two in 3D).
+struct NufftState {
+    bool active = false;  // eligible: the system evaluates through the NUFFT
+    std::string why;      // reason when not eligible (the direct path is used)
+    std::shared_ptr<const ConstraintLayout> layout;  // plan key (held, so its address is not reused meanwhile)
+    double eps = 0;
+    int64_t N = -1, max_modes = 0;
+    int w = 0;
+    double beta = 0;
+    int nf[3] = {1, 1, 1}, K[3] = {0, 0, 0}, tile[3] = {1, 1, 1}, nbins[3] = {1, 1, 1}, reach[3] = {0, 0, 0},
+        poff[3] = {0, 0, 0};
+    int nbin = 1;
+    int64_t npts = 1;
+    DevArray<double2> grid;
+    DevArray<double> kv, p;
+    DevArray<int32_t> start, bin_of, bin_start, bin_items;
+    hipfftHandle plan1 = nullptr, plan2 = nullptr;
+    NufftState() = default;
+    NufftState(const NufftState &) = delete;
+    NufftState &operator=(const NufftState &) = delete;
+    ~NufftState() { DropPlans(); }
+    void DropPlans()
+    {
+        if (plan1) (void)hipfftDestroy(plan1);
+        if (plan2) (void)hipfftDestroy(plan2);
+        plan1 = plan2 = nullptr;
+    }
+};
+
 // One periodic configuration plus its constraint set, resident in HBM.
 class System {
 public:
@@ -144,6 +175,14 @@ public:
     DevArray<double> rpart, rho, coef, epart, energy, fpart, F, grad;
     DevArray<float> coef32;
     int precision = 0;  // 0: FP64; 1: mixed (lattice phases and force terms in FP32)
+    // rho backend (SetRhoBackend / SetFinufftTolerance / SetFinufftMaxModes,
+    // potential/CollectiveCoordinatePotential.h:160-212): 0 direct sums, 1 NUFFT when the constraint set is
+    // eligible (lattice, d <= 3, mode box within the cap, no constraint sharing a mode with another's conjugate)
+    int rho_backend = 0;
+    double nufft_eps = 1e-12;
+    int64_t nufft_max_modes = (int64_t)1 << 22;
+    std::unique_ptr<NufftState> nu;
+    bool nufft_on() const { return nu && nu->active; }
     DevArray<int32_t> cell_of, cell_start, cell_items;
     DevArray<double> sc_epart, sc_F;
 
@@ -192,6 +231,7 @@ public:
     // forces alone reuse the rho coefficients of the previous rho pass
     void Enqueue(bool rho, bool forces);
     void Sync();
+    void NufftFft(bool type1);  // the FFT stage of every NUFFT system (hipFFT on the batch stream)
 
     struct Range {
         int off = 0, n = 0;
@@ -214,6 +254,8 @@ public:
     DevArray<WorkItem> d_items;
     std::vector<WorkItem> h_items;
     Range r_prep, r_rho_gen, r_red, r_efin, r_f_gen, r_ffin, r_sc_part, r_sc_sys, r_sc_cell, r_mix;
+    Range r_nu_part, r_nu_sys, r_nu_bins, r_nu_tiles, r_nu_slots, r_nu_grid;  // NUFFT systems
+    bool has_nufft = false;
     // every system of the launches: `systems` in order (their descriptor indices), then the species of each
     // mixture among them
     std::vector<System *> all;
@@ -243,5 +285,13 @@ void launch_force_general(hipStream_t, const SysDesc *, const WorkItem *, int);
 void launch_force_final(hipStream_t, const SysDesc *, const WorkItem *, int);
 void launch_softcore(hipStream_t, const SysDesc *, const WorkItem *part_items, int npart, const WorkItem *sys_items,
                      int nsys, const WorkItem *cell_items, int ncell);
+// NUFFT backend (cc_nufft.hip); the FFTs between the stages run through each system's hipFFT plans
+void launch_nufft_rho(hipStream_t, const SysDesc *, const WorkItem *part, int npart, const WorkItem *one, int nsys,
+                      const WorkItem *bins, int nbins, const WorkItem *tiles, int ntiles);
+void launch_nufft_extract(hipStream_t, const SysDesc *, const WorkItem *slots, int nslots);
+void launch_nufft_coef(hipStream_t, const SysDesc *, const WorkItem *grid, int ngrid, const WorkItem *slots, int nslots);
+void launch_nufft_interp(hipStream_t, const SysDesc *, const WorkItem *part, int npart);
+// (re)build the NUFFT plan of a lattice system for its layout, tolerance and N; sets nu->active (cc_nufft_host.cpp)
+void PrepareNufft(System &s, hipStream_t stream);
 
 }  // namespace ccgpu

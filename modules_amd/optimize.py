@@ -187,13 +187,14 @@ class BatchSolver:
     QUEUED, RESIDENT, DONE, FAILED = 0, 1, 2, 3
 
     def __init__(self, max_slots=256, hbm_budget=200 << 30, check_interval=8, graph=False, precision="fp64",
-                 polish=1e-8, lanes=2):
+                 polish=1e-8, lanes=2, rho_backend="direct", nufft_eps=1e-12):
         self._h = C.c_void_p()
         check(lib().ccgpu_solver_create(int(max_slots), int(hbm_budget), C.byref(self._h)))
         check(lib().ccgpu_solver_set_check_interval(self._h, int(check_interval)))
         check(lib().ccgpu_solver_set_graph(self._h, 1 if graph else 0))
         check(lib().ccgpu_solver_set_lanes(self._h, int(lanes)))
         self.set_precision(precision, polish)
+        self.set_rho_backend(rho_backend, nufft_eps)
         self._shapes = {}
 
     def set_precision(self, precision="fp64", polish=1e-8):
@@ -201,6 +202,11 @@ class BatchSolver:
         sums) until E < polish or the end of the first pass, FP64 afterwards."""
         from .potential import _precision_code
         check(lib().ccgpu_solver_set_precision(self._h, _precision_code(precision), float(polish)))
+
+    def set_rho_backend(self, backend="direct", eps=1e-12):
+        """Collective-density backend of single-species jobs submitted from now on: "direct" or "nufft"
+        (eligible lattice constraint sets; tolerance eps)."""
+        check(lib().ccgpu_solver_set_rho_backend(self._h, 1 if backend == "nufft" else 0, float(eps)))
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -315,6 +315,43 @@ class ShiftedCCPotential:
         check(lib().ccgpu_potential_set_precision(self._h, _precision_code(mode)))
         self.RhoValid = False
 
+    def SetRhoBackend(self, backend):
+        """SetRhoBackend (CollectiveCoordinatePotential.h:162-169): "direct" (default) or "nufft" -- the Type 1 /
+        Type 2 NUFFT on a fine grid (the reference's FinufftType1 with Type 2 forces) when the constraint set is
+        eligible, the direct sums otherwise (GetLastRhoBackendUsed tells which)."""
+        codes = {"direct": 0, "nufft": 1, "finufft": 1, "FinufftType1": 1, 0: 0, 1: 1}
+        if backend not in codes:
+            raise ValueError('ShiftedCCPotential::SetRhoBackend: backend must be "direct" or "nufft"')
+        self._rho_backend = codes[backend]
+        self._push_backend()
+
+    def SetFinufftTolerance(self, eps):
+        """SetFinufftTolerance (:180-186): requested NUFFT tolerance, positive and finite (default 1e-12)."""
+        if not (eps > 0.0) or not math.isfinite(eps):
+            raise ValueError("ShiftedCCPotential::SetFinufftTolerance: eps must be positive and finite.")
+        self._nufft_eps = float(eps)
+        self._push_backend()
+
+    def SetFinufftMaxModes(self, maxModes):
+        """SetFinufftMaxModes (:199-206): cap on the mode box prod(2 K_c + 1) (default 2^22)."""
+        if int(maxModes) < 1:
+            raise ValueError("ShiftedCCPotential::SetFinufftMaxModes: maxModes must be >= 1.")
+        self._nufft_max_modes = int(maxModes)
+        self._push_backend()
+
+    def _push_backend(self):
+        check(lib().ccgpu_potential_set_rho_backend(self._h, getattr(self, "_rho_backend", 0),
+                                                    getattr(self, "_nufft_eps", 1e-12),
+                                                    getattr(self, "_nufft_max_modes", 1 << 22)))
+        self.RhoValid = False
+
+    def GetLastRhoBackendUsed(self):
+        """"nufft" if the bound system evaluates through the NUFFT, else "direct"."""
+        self._require()
+        v = C.c_int(0)
+        check(lib().ccgpu_potential_rho_backend_used(self._h, C.byref(v)))
+        return "nufft" if v.value else "direct"
+
     def UsesLatticePath(self):
         v = C.c_int(0)
         check(lib().ccgpu_potential_uses_lattice_path(self._h, C.byref(v)))
