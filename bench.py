@@ -58,12 +58,13 @@ def parse():
                     help="batched / mixed256: FP64, or FP32 phases and force terms with FP64 sums and an FP64 "
                          "polish below 1e-8")
     ap.add_argument("--workload", default="batched",
-                    choices=["batched", "single", "mixed256", "large2d", "mixed2d", "serve", "five"],
+                    choices=["batched", "single", "mixed256", "large2d", "mixed2d", "serve", "five", "nufft2d"],
                     help="batched: the headline (64 seeds, 3D N=2000 chi=0.45); single: the same solves through the "
                          "single-solve drop-in path; mixed256: 256 configurations of mixed N, d, chi; large2d: 2D "
                          "N=20000 chi=0.4; mixed2d: the same solved in FP64 and in mixed precision, with the "
                          "mixed-precision error; serve: Poisson request stream with soft core; five: multi-species "
-                         "ground states of the reference's five-component study (2D N=2000, 5 x 400)")
+                         "ground states of the reference's five-component study (2D N=2000, 5 x 400); nufft2d: "
+                         "large 2D systems through the NUFFT backend against the direct sums")
     ap.add_argument("--requests", type=int, default=48, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=20.0, help="serve: arrivals per second")
     ap.add_argument("--profiler-run", action="store_true",
@@ -477,8 +478,63 @@ def workload_five(args):
             "vs_published_4_thread_rate": (accepted / wall) / (1.0 / 6.28), "dtype": "f64"}
 
 
+def workload_nufft2d(args):
+    """Large 2D systems through the NUFFT backend (Type 1 rho, Type 2 forces) against the direct sums: 2D N=20000
+    chi=0.4 solved both ways (with the NUFFT error at the start configuration against the direct path, which the
+    tests pin to the oracle), then 2D N=100000 chi=0.4 (M ~ 80000) solved through the NUFFT, with the direct path's
+    E+F time per evaluation beside it."""
+    from modules_amd import GetKs, RandomConfiguration, ShiftedCCPotential, getK
+    from modules_amd.optimize import BatchSolver
+
+    def solve(cfg, k, backend):
+        solver = BatchSolver(max_slots=1, check_interval=args.check_interval, rho_backend=backend)
+        job = solver.submit(cfg, k, np.ones(len(k)), MaxStep=args.maxstep, Emin=args.emin)
+        res, wall = _solve_all(solver, [job], args.emin)
+        r = res[0]
+        return {"wall_s": wall, "energy": r["energy"], "objective_evals": r["evaluations"],
+                "device_evals": r["device_evaluations"], "ms_per_eval": 1000 * wall / max(r["device_evaluations"], 1)}
+
+    def ef_ms(cfg, k, backend, reps):
+        pot = ShiftedCCPotential(2)
+        for kk in k:
+            pot.AddConstraint(kk, [1.0])
+        pot.SetRhoBackend(backend)
+        pot.SetConfiguration(cfg)
+        E, F = pot.EnergyAndAllForce()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            pot.SetConfiguration(cfg)
+            pot.EnergyAndAllForce()
+        return 1000 * (time.perf_counter() - t0) / reps, E, F, pot.GetLastRhoBackendUsed()
+
+    out = {"metric": "stealthy ground states/s at E<1e-18", "unit": "ground_states/s", "dtype": "f64"}
+    cfg = RandomConfiguration(4, 20000, 2)
+    k = GetKs(cfg, getK(0.4, 1.0, 2))
+    tiles = _tune(args, cfg, k)
+    ms_d, E_d, F_d, _ = ef_ms(cfg, k, "direct", 20)
+    ms_n, E_n, F_n, used = ef_ms(cfg, k, "nufft", 20)
+    out["N20000"] = {"M": len(k), "tiles": tiles, "backend_used": used,
+                     "ef_ms_direct": ms_d, "ef_ms_nufft": ms_n,
+                     "nufft_vs_direct_start": {"dE_over_E": abs(E_n - E_d) / abs(E_d),
+                                               "max_dF_over_max_F": float(np.max(np.abs(F_n - F_d)) / np.max(np.abs(F_d)))},
+                     "direct": solve(cfg, k, "direct"), "nufft": solve(cfg, k, "nufft")}
+    cfg = RandomConfiguration(5, 100000, 2)
+    k = GetKs(cfg, getK(0.4, 1.0, 2))
+    ms_d, E_d, F_d, _ = ef_ms(cfg, k, "direct", 2)
+    ms_n, E_n, F_n, used = ef_ms(cfg, k, "nufft", 10)
+    r = solve(cfg, k, "nufft")
+    out["N100000"] = {"M": len(k), "backend_used": used, "ef_ms_direct": ms_d, "ef_ms_nufft": ms_n,
+                      "nufft_vs_direct_start": {"dE_over_E": abs(E_n - E_d) / abs(E_d),
+                                                "max_dF_over_max_F": float(np.max(np.abs(F_n - F_d)) / np.max(np.abs(F_d)))},
+                      "nufft": r, "direct_solve_estimate_s": ms_d * r["device_evals"] / 1000}
+    out["value"] = (1.0 if r["energy"] < args.emin else 0.0) / r["wall_s"]
+    out["workload"] = "2D N=100000 chi=0.4 through the NUFFT (and N=20000 both ways)"
+    return out
+
+
 OTHER_WORKLOADS = {"single": workload_single, "mixed256": workload_mixed256, "large2d": workload_large2d,
-                   "mixed2d": workload_mixed2d, "serve": workload_serve, "five": workload_five}
+                   "mixed2d": workload_mixed2d, "serve": workload_serve, "five": workload_five,
+                   "nufft2d": workload_nufft2d}
 
 
 def _cpu_model():

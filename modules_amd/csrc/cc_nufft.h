@@ -15,8 +15,9 @@
 // coefficients f_c(n) = n_c c_n / 2 (c_{-n} = conj c_n) make sum_n f_c(n) exp(-i n.x) = i G_c, so one complex
 // transform of f_0 + i f_1 gives G_0 = Im u and G_1 = -Re u (a second transform carries f_2 in 3D).
 //
-// Determinism: particles are binned by output tile (counting sort, each bin sorted by particle index) and each
-// tile gathers its points' contributions bin by bin in a fixed order, so every sum has a fixed order.
+// Determinism: particles are binned by output tile (counting sort, each bin sorted by particle index); each tile
+// spreads its own particles, in index order, into a padded tile (the tile plus the kernel's reach), and each grid
+// point then sums the padded tiles that cover it in a fixed tile order -- every sum has a fixed order.
 #pragma once
 #include <cstdint>
 #include <hip/hip_runtime.h>
@@ -24,7 +25,7 @@
 namespace ccgpu {
 
 constexpr int kNufftMaxW = 16;   // kernel width cap (eps down to ~1e-15)
-constexpr int kNufftChunk = 64;  // particles staged in LDS per spreading step
+constexpr int kNufftMaxPad = 17 * 21 * 21;  // largest padded tile in LDS (3D tiles 4 x 8 x 8 with w = 13)
 
 struct NufftDesc {
     double2 *grid;            // fine grid(s): npts complex (Type 1), then the Type 2 coefficient grid(s) (x 2 in 3D)
@@ -34,13 +35,16 @@ struct NufftDesc {
     int32_t *bin_start;       // nbin + 1 starts, then nbin scatter cursors
     int32_t *bin_items;       // N particle indices, by bin, index order within a bin
     const double *p;          // deconvolution factors: axis c at p + poff[c], mode m at index m + K[c]
+    double *sub;              // ntile padded tiles of psize points: each tile's own particles spread locally
     int32_t nf[3];            // fine grid points per axis (axis 0 slowest; 1 beyond the dimension)
     int32_t K[3];             // mode half-width per axis
     int32_t tile[3];          // output tile (one block) per axis
     int32_t nbins[3];         // nf / tile per axis
     int32_t reach[3];         // neighbouring bins whose particles can touch a tile, per side
     int32_t poff[3];
+    int32_t pad[3];           // padded tile extent per axis: tile + w (1 beyond the dimension)
     int32_t w, nbin, npts, ntile;
+    int32_t hlo, psize;       // padding below a tile (floor(w/2)); points of one padded tile
     double beta;
 };
 

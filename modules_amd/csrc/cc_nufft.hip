@@ -2,7 +2,8 @@
 //
 // Per evaluation of a NUFFT system:
 //   k_nu_zero, k_nu_prep (kernel values, support starts, bin counts), k_nu_scan, k_nu_scatter, k_nu_sort   bins
-//   k_nu_spread   one block per output tile: gathers its points' contributions from the neighbouring bins
+//   k_nu_spread   one block per output tile: its own particles into a padded tile (scratch)
+//   k_nu_gather   every grid point sums the padded tiles that cover it
 //   FFT (exp(+i), in place)
 //   k_nu_extract  rho at every constraint slot -> rpart (one particle chunk), then the usual reduce / energy
 //   k_nu_zero_grid, k_nu_coef   Type 2 coefficient grid(s) from the force coefficients
@@ -119,96 +120,125 @@ __global__ void __launch_bounds__(kBlock) k_nu_scatter(const SysDesc *__restrict
     U.bin_items[slot] = i;
 }
 
-// items (sys, bin block): each bin's list in particle order (insertion sort; bins hold a few dozen)
+// items (sys, bin block of 4 bins): each bin's list in particle order.  One wavefront per bin: the list is read
+// into LDS and every lane places its entries at their rank (the count of smaller indices; indices are distinct).
+constexpr int kSortCap = 1024;  // bins longer than this (far denser than a fine-grid tile holds) sort serially
 __global__ void __launch_bounds__(kBlock) k_nu_sort(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
 {
+    __shared__ int32_t lv[4][kSortCap];
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
     if (!sys_active(sys, S)) return;
     const NufftDesc &U = S.nu;
-    const int b = it.t0 * kBlock + threadIdx.x;
-    if (b >= U.nbin) return;
-    const int lo = U.bin_start[b], hi = U.bin_start[b + 1];
-    for (int x = lo + 1; x < hi; x++) {
-        const int32_t v = U.bin_items[x];
-        int y = x - 1;
-        while (y >= lo && U.bin_items[y] > v) {
-            U.bin_items[y + 1] = U.bin_items[y];
-            y--;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int b = it.t0 * 4 + wv;
+    if (b >= U.nbin) return;  // wavefront-uniform: no block barrier below
+    const int lo = U.bin_start[b], hi = U.bin_start[b + 1], n = hi - lo;
+    int32_t *v = lv[wv];
+    if (n <= kSortCap) {
+        for (int x = lane; x < n; x += 64) v[x] = U.bin_items[lo + x];
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        for (int x = lane; x < n; x += 64) {
+            const int32_t me = v[x];
+            int rank = 0;
+            for (int y = 0; y < n; y++) rank += v[y] < me;
+            U.bin_items[lo + rank] = me;
         }
-        U.bin_items[y + 1] = v;
+    } else if (lane == 0) {
+        for (int x = lo + 1; x < hi; x++) {
+            const int32_t val = U.bin_items[x];
+            int y = x - 1;
+            while (y >= lo && U.bin_items[y] > val) {
+                U.bin_items[y + 1] = U.bin_items[y];
+                y--;
+            }
+            U.bin_items[y + 1] = val;
+        }
     }
 }
 
-// items (sys, tile): the tile's grid points gather the kernel contributions of the particles in the bins within
-// reach, bin by bin (fixed order) and in particle order inside a bin; every grid point is written once
+// items (sys, tile): the tile's own particles (its bin, in particle order) spread into a padded tile in LDS -- the
+// tile plus floor(w/2) points below and ceil(w/2) above per axis, which holds every footprint of those particles --
+// one thread per footprint point and a barrier between particles, so each point adds in particle order; the padded
+// tile goes to the scratch array for k_nu_gather.
 __global__ void __launch_bounds__(kBlock) k_nu_spread(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
 {
-    __shared__ double lkv[kNufftChunk * 3 * kNufftMaxW];
-    __shared__ int32_t lst[kNufftChunk * 4];
+    extern __shared__ double acc[];
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
     if (!sys_active(sys, S)) return;
     const NufftDesc &U = S.nu;
     const int d = S.d, w = U.w;
-    // tile coordinates (axis 2 fastest) and this thread's point
     const int T2 = it.t0 % U.nbins[2];
     const int T1 = (it.t0 / U.nbins[2]) % U.nbins[1];
     const int T0 = it.t0 / (U.nbins[2] * U.nbins[1]);
-    const int t = threadIdx.x;
-    const int a2 = t % U.tile[2], a1 = (t / U.tile[2]) % U.tile[1], a0 = t / (U.tile[2] * U.tile[1]);
-    const int l[3] = {T0 * U.tile[0] + a0, T1 * U.tile[1] + a1, T2 * U.tile[2] + a2};
-    const int T[3] = {T0, T1, T2};
-    int lo[3], cnt[3];
-    for (int c = 0; c < 3; c++) {
-        if (2 * U.reach[c] + 1 >= U.nbins[c]) {
-            lo[c] = 0;
-            cnt[c] = U.nbins[c];
-        } else {
-            lo[c] = T[c] - U.reach[c];
-            cnt[c] = 2 * U.reach[c] + 1;
+    const int org[3] = {T0 * U.tile[0] - (d > 0 ? U.hlo : 0), T1 * U.tile[1] - (d > 1 ? U.hlo : 0),
+                        T2 * U.tile[2] - (d > 2 ? U.hlo : 0)};
+    for (int x = threadIdx.x; x < U.psize; x += kBlock) acc[x] = 0.0;
+    const int fw1 = d > 1 ? w : 1, fw2 = d > 2 ? w : 1;
+    const int nfoot = w * fw1 * fw2;
+    const int beg = U.bin_start[it.t0], end = U.bin_start[it.t0 + 1];
+    for (int q = beg; q < end; q++) {
+        __syncthreads();  // the previous particle's adds (and the zeroing) are done
+        const int j = U.bin_items[q];
+        const int32_t *st = U.start + (int64_t)j * 4;
+        const double *kv = U.kv + (int64_t)j * 3 * kNufftMaxW;
+        const int b0 = st[0] - org[0], b1 = d > 1 ? st[1] - org[1] : 0, b2 = d > 2 ? st[2] - org[2] : 0;
+        for (int f = threadIdx.x; f < nfoot; f += kBlock) {
+            const int f2 = f % fw2, f1 = (f / fw2) % fw1, f0 = f / (fw2 * fw1);
+            double v = kv[f0];
+            if (d > 1) v *= kv[kNufftMaxW + f1];
+            if (d > 2) v *= kv[2 * kNufftMaxW + f2];
+            acc[((b0 + f0) * U.pad[1] + (b1 + f1)) * U.pad[2] + (b2 + f2)] += v;
         }
     }
-    const int dw = d * w;
-    double acc = 0.0;
-    for (int o0 = 0; o0 < cnt[0]; o0++)
-        for (int o1 = 0; o1 < cnt[1]; o1++)
-            for (int o2 = 0; o2 < cnt[2]; o2++) {
-                const int b0 = wrap_index(lo[0] + o0, U.nbins[0]);
-                const int b1 = wrap_index(lo[1] + o1, U.nbins[1]);
-                const int b2 = wrap_index(lo[2] + o2, U.nbins[2]);
-                const int b = (b0 * U.nbins[1] + b1) * U.nbins[2] + b2;
-                const int beg = U.bin_start[b], end = U.bin_start[b + 1];
-                for (int p0 = beg; p0 < end; p0 += kNufftChunk) {
-                    const int np = min(kNufftChunk, end - p0);
-                    __syncthreads();  // the previous chunk is consumed
-                    for (int x = t; x < np * dw; x += kBlock) {
-                        const int pp = x / dw, r = x - pp * dw, c = r / w, o = r - c * w;
-                        const int j = U.bin_items[p0 + pp];
-                        lkv[(pp * 3 + c) * kNufftMaxW + o] = U.kv[((int64_t)j * 3 + c) * kNufftMaxW + o];
-                    }
-                    for (int x = t; x < np * 4; x += kBlock) {
-                        const int j = U.bin_items[p0 + (x >> 2)];
-                        lst[x] = U.start[(int64_t)j * 4 + (x & 3)];
-                    }
-                    __syncthreads();
-                    for (int pp = 0; pp < np; pp++) {
-                        double prod = 1.0;
-                        bool in = true;
-                        for (int c = 0; c < d; c++) {
-                            const int o = wrap_index(l[c] - lst[pp * 4 + c], U.nf[c]);
-                            if (o >= w) {
-                                in = false;
-                                break;
-                            }
-                            prod *= lkv[(pp * 3 + c) * kNufftMaxW + o];
-                        }
-                        if (in) acc += prod;
-                    }
+    __syncthreads();
+    double *out = U.sub + (int64_t)it.t0 * U.psize;
+    for (int x = threadIdx.x; x < U.psize; x += kBlock) out[x] = acc[x];
+}
+
+// items (sys, grid block, number of grid blocks): each grid point sums the padded tiles that cover it (tiles in
+// increasing order per axis, periodic), and the fine grid gets the real spread values
+__global__ void __launch_bounds__(kBlock) k_nu_gather(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
+{
+    const WorkItem it = items[blockIdx.x];
+    const SysDesc &S = sys[it.sys];
+    if (!sys_active(sys, S)) return;
+    const NufftDesc &U = S.nu;
+    const int d = S.d;
+    for (int64_t g = (int64_t)it.t0 * kBlock + threadIdx.x; g < U.npts; g += (int64_t)kBlock * it.t1) {
+        const int l2 = (int)(g % U.nf[2]);
+        const int l1 = (int)((g / U.nf[2]) % U.nf[1]);
+        const int l0 = (int)(g / ((int64_t)U.nf[2] * U.nf[1]));
+        const int l[3] = {l0, l1, l2};
+        // per axis: tiles T (unwrapped) whose padded extent [T tile - hlo, T tile - hlo + pad) holds l
+        int tlo[3], thi[3];
+        for (int c = 0; c < 3; c++) {
+            const int h = c < d ? U.hlo : 0;
+            // T tile - h <= l  and  l < T tile - h + pad  <=>  (l + h - pad) / tile < T <= (l + h) / tile
+            const int up = l[c] + h;
+            thi[c] = up >= 0 ? up / U.tile[c] : -((-up + U.tile[c] - 1) / U.tile[c]);
+            const int dn = l[c] + h - U.pad[c];
+            tlo[c] = (dn >= 0 ? dn / U.tile[c] : -((-dn + U.tile[c] - 1) / U.tile[c])) + 1;
+        }
+        double sum = 0.0;
+        for (int t0 = tlo[0]; t0 <= thi[0]; t0++) {
+            const int w0 = ((t0 % U.nbins[0]) + U.nbins[0]) % U.nbins[0];
+            const int L0 = l[0] - (t0 * U.tile[0] - (d > 0 ? U.hlo : 0));
+            for (int t1 = tlo[1]; t1 <= thi[1]; t1++) {
+                const int w1 = ((t1 % U.nbins[1]) + U.nbins[1]) % U.nbins[1];
+                const int L1 = l[1] - (t1 * U.tile[1] - (d > 1 ? U.hlo : 0));
+                for (int t2 = tlo[2]; t2 <= thi[2]; t2++) {
+                    const int w2 = ((t2 % U.nbins[2]) + U.nbins[2]) % U.nbins[2];
+                    const int L2 = l[2] - (t2 * U.tile[2] - (d > 2 ? U.hlo : 0));
+                    const int64_t tile = ((int64_t)w0 * U.nbins[1] + w1) * U.nbins[2] + w2;
+                    sum += U.sub[tile * U.psize + ((int64_t)L0 * U.pad[1] + L1) * U.pad[2] + L2];
                 }
             }
-    const int64_t g = ((int64_t)l[0] * U.nf[1] + l[1]) * U.nf[2] + l[2];
-    U.grid[g] = make_double2(acc, 0.0);
+        }
+        U.grid[g] = make_double2(sum, 0.0);
+    }
 }
 
 // integer mode n of lattice slot q (segment head plus the slot's offset along the last axis)
@@ -302,9 +332,10 @@ __global__ void __launch_bounds__(kBlock) k_nu_interp(const SysDesc *__restrict_
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
     if (!sys_active(sys, S)) return;
-    const int i = it.t0 * kBlock + threadIdx.x;
-    if (i >= S.N) return;
+    const int x = it.t0 * kBlock + threadIdx.x;
+    if (x >= S.N) return;
     const NufftDesc &U = S.nu;
+    const int i = U.bin_items[x];  // particles in bin order: neighbouring lanes read neighbouring grid rows
     const int d = S.d, w = U.w;
     const double *kv = U.kv + (int64_t)i * 3 * kNufftMaxW;
     const int32_t *st = U.start + (int64_t)i * 4;
@@ -362,7 +393,8 @@ __global__ void __launch_bounds__(kBlock) k_nu_interp(const SysDesc *__restrict_
 #define CC_NU_GRID(n) dim3((unsigned)(n)), dim3(kBlock), 0, stream
 
 void launch_nufft_rho(hipStream_t stream, const SysDesc *sys, const WorkItem *part, int npart, const WorkItem *one,
-                      int nsys, const WorkItem *bins, int nbins, const WorkItem *tiles, int ntiles)
+                      int nsys, const WorkItem *bins, int nbins, const WorkItem *tiles, int ntiles,
+                      const WorkItem *grid, int ngrid, int max_psize)
 {
     if (npart > 0) {
         hipLaunchKernelGGL(k_nu_zero, CC_NU_GRID(npart), sys, part);
@@ -371,7 +403,10 @@ void launch_nufft_rho(hipStream_t stream, const SysDesc *sys, const WorkItem *pa
         hipLaunchKernelGGL(k_nu_scatter, CC_NU_GRID(npart), sys, part);
         if (nbins > 0) hipLaunchKernelGGL(k_nu_sort, CC_NU_GRID(nbins), sys, bins);
     }
-    if (ntiles > 0) hipLaunchKernelGGL(k_nu_spread, CC_NU_GRID(ntiles), sys, tiles);
+    if (ntiles > 0)
+        hipLaunchKernelGGL(k_nu_spread, dim3((unsigned)ntiles), dim3(kBlock), max_psize * sizeof(double), stream, sys,
+                           tiles);
+    if (ngrid > 0) hipLaunchKernelGGL(k_nu_gather, CC_NU_GRID(ngrid), sys, grid);
 }
 
 void launch_nufft_extract(hipStream_t stream, const SysDesc *sys, const WorkItem *slots, int nslots)

@@ -145,8 +145,15 @@ void PrepareNufft(System &S, hipStream_t stream)
             U.nbins[c] = 1;
             U.reach[c] = 0;
         }
+        U.pad[c] = c < d ? U.tile[c] + w : 1;
         U.npts *= U.nf[c];
         U.nbin *= U.nbins[c];
+    }
+    U.hlo = w / 2;
+    U.psize = U.pad[0] * U.pad[1] * U.pad[2];
+    if (U.psize > kNufftMaxPad) {
+        U.why = "kernel too wide for the padded tiles";
+        return;
     }
     // deconvolution factors p_c(m) = (w/2) int_{-1}^{1} phi(z) cos(pi w m z / nf_c) dz
     std::vector<double> gx, gw;
@@ -168,6 +175,7 @@ void PrepareNufft(System &S, hipStream_t stream)
     U.bin_of.Reserve((size_t)N);
     U.bin_items.Reserve((size_t)N);
     U.bin_start.Reserve(2 * (size_t)U.nbin + 1);
+    U.sub.Reserve((size_t)U.nbin * U.psize);
     U.p.Reserve(ptab.size());
     CC_HIP(hipMemcpyAsync(U.p.get(), ptab.data(), ptab.size() * sizeof(double), hipMemcpyHostToDevice, stream));
     CC_HIP(hipStreamSynchronize(stream));

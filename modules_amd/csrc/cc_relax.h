@@ -17,6 +17,11 @@
 namespace ccgpu {
 
 constexpr int kRelaxBlock = 256;
+// Jobs of at least kRelaxBigN optimizer variables step with kRelaxBlockBig threads: the step kernel's vector
+// loops are bound by the loads one workgroup keeps in flight, and 512 threads (two waves per SIMD at the kernel's
+// 256 VGPRs) double them.  The choice reads only the job's own size, so its bits do not depend on the mix.
+constexpr int kRelaxBlockBig = 512;
+constexpr int64_t kRelaxBigN = 32768;
 constexpr int kRelaxChunk = 1024;  // vector elements per block of the wide L-BFGS passes
 constexpr double kRelaxRescale[3] = {10000.0, 1.0, 0.01};  // StructureOptimization.cpp:1055-1063
 constexpr double kMaxEnergy = 1e10;                          // core/etc.h:8
@@ -50,8 +55,8 @@ struct RelaxDesc {
     int nchunks;     // blocks of kRelaxChunk elements in the wide passes
 };
 
-void launch_relax_step(hipStream_t stream, SysDesc *sys, const RelaxDesc *rd, int n, const WorkItem *wide_items,
-                       int nwide);
+void launch_relax_step(hipStream_t stream, SysDesc *sys, const RelaxDesc *rd, int n, int n_big,
+                       const WorkItem *wide_items, int nwide);
 
 // One minimisation request and, while resident, its device buffers.
 struct Job {
@@ -107,7 +112,8 @@ struct SolverLane {
     hipEvent_t desc_ev = nullptr;  // recorded after the descriptor read of a harvest
     bool desc_pending = false;
     size_t marks_at_read = 0;      // timing marks recorded before that read (complete once it is)
-    DevArray<RelaxDesc> d_rdesc;
+    DevArray<RelaxDesc> d_rdesc;  // jobs stepped by kRelaxBlock threads, then the n_big large ones
+    int n_big = 0;
     DevArray<WorkItem> d_wide;
     int nwide = 0;
     hipGraphExec_t gexec = nullptr;

@@ -13,10 +13,11 @@ namespace ccgpu {
 // vector, so element-wise operations need no barrier; reductions are butterfly sums within each
 // wavefront and a fixed-order sum over wavefronts, so every thread gets the same bits in every run.
 // ------------------------------------------------------------------------------------------------
+template <int BS>
 struct BlockPlisOps {
     int n, mf;
     double *x, *g, *s, *xo, *go;
-    double *lds_red;  // 2 x kRelaxBlock / 64 doubles
+    double *lds_red;  // 2 x BS / 64 doubles
     double *lds_u, *lds_v;
     bool gs_valid;      // g.S of the last direction is cached until g or S changes
     double gs_val;
@@ -36,13 +37,13 @@ struct BlockPlisOps {
     __device__ double block_reduce(double v, bool is_max) const
     {
         v = is_max ? wave_max_dpp(v) : wave_sum_dpp(v);
-        double *slot = lds_red + red_slot * (kRelaxBlock / 64);
+        double *slot = lds_red + red_slot * (BS / 64);
         red_slot ^= 1;
         const int w = threadIdx.x >> 6;
         if ((threadIdx.x & 63) == 0) slot[w] = v;
         __syncthreads();
         double r = slot[0];
-        for (int i = 1; i < kRelaxBlock / 64; i++) r = is_max ? fmax(r, slot[i]) : r + slot[i];
+        for (int i = 1; i < BS / 64; i++) r = is_max ? fmax(r, slot[i]) : r + slot[i];
         return r;
     }
     __device__ void touch(int id)
@@ -58,17 +59,17 @@ struct BlockPlisOps {
         const double *pa = vec(a), *pb = vec(b);
         double t = 0.0;
         int e = threadIdx.x;
-        for (; e + (kU - 1) * kRelaxBlock < n; e += kU * kRelaxBlock) {
+        for (; e + (kU - 1) * BS < n; e += kU * BS) {
             double va[kU], vb[kU];
 #pragma unroll
             for (int u = 0; u < kU; u++) {
-                va[u] = pa[e + u * kRelaxBlock];
-                vb[u] = pb[e + u * kRelaxBlock];
+                va[u] = pa[e + u * BS];
+                vb[u] = pb[e + u * BS];
             }
 #pragma unroll
             for (int u = 0; u < kU; u++) t = __builtin_fma(va[u], vb[u], t);
         }
-        for (; e < n; e += kRelaxBlock) t = __builtin_fma(pa[e], pb[e], t);
+        for (; e < n; e += BS) t = __builtin_fma(pa[e], pb[e], t);
         return block_reduce(t, false);
     }
     __device__ void axpy(int z, double a, int xv, int yv)
@@ -77,17 +78,17 @@ struct BlockPlisOps {
         double *pz = vec(z);
         const double *px = vec(xv), *py = vec(yv);
         int e = threadIdx.x;
-        for (; e + (kU - 1) * kRelaxBlock < n; e += kU * kRelaxBlock) {
+        for (; e + (kU - 1) * BS < n; e += kU * BS) {
             double vx[kU], vy[kU];
 #pragma unroll
             for (int u = 0; u < kU; u++) {
-                vx[u] = px[e + u * kRelaxBlock];
-                vy[u] = py[e + u * kRelaxBlock];
+                vx[u] = px[e + u * BS];
+                vy[u] = py[e + u * BS];
             }
 #pragma unroll
-            for (int u = 0; u < kU; u++) pz[e + u * kRelaxBlock] = __builtin_fma(a, vx[u], vy[u]);
+            for (int u = 0; u < kU; u++) pz[e + u * BS] = __builtin_fma(a, vx[u], vy[u]);
         }
-        for (; e < n; e += kRelaxBlock) pz[e] = __builtin_fma(a, px[e], py[e]);
+        for (; e < n; e += BS) pz[e] = __builtin_fma(a, px[e], py[e]);
     }
     __device__ void neg(int src, int dst)
     {
@@ -95,14 +96,14 @@ struct BlockPlisOps {
         const double *ps = vec(src);
         double *pd = vec(dst);
         int e = threadIdx.x;
-        for (; e + (kU - 1) * kRelaxBlock < n; e += kU * kRelaxBlock) {
+        for (; e + (kU - 1) * BS < n; e += kU * BS) {
             double v[kU];
 #pragma unroll
-            for (int u = 0; u < kU; u++) v[u] = ps[e + u * kRelaxBlock];
+            for (int u = 0; u < kU; u++) v[u] = ps[e + u * BS];
 #pragma unroll
-            for (int u = 0; u < kU; u++) pd[e + u * kRelaxBlock] = -v[u];
+            for (int u = 0; u < kU; u++) pd[e + u * BS] = -v[u];
         }
-        for (; e < n; e += kRelaxBlock) pd[e] = -ps[e];
+        for (; e < n; e += BS) pd[e] = -ps[e];
     }
     __device__ void copy(int src, int dst)
     {
@@ -110,28 +111,28 @@ struct BlockPlisOps {
         const double *ps = vec(src);
         double *pd = vec(dst);
         int e = threadIdx.x;
-        for (; e + (kU - 1) * kRelaxBlock < n; e += kU * kRelaxBlock) {
+        for (; e + (kU - 1) * BS < n; e += kU * BS) {
             double v[kU];
 #pragma unroll
-            for (int u = 0; u < kU; u++) v[u] = ps[e + u * kRelaxBlock];
+            for (int u = 0; u < kU; u++) v[u] = ps[e + u * BS];
 #pragma unroll
-            for (int u = 0; u < kU; u++) pd[e + u * kRelaxBlock] = v[u];
+            for (int u = 0; u < kU; u++) pd[e + u * BS] = v[u];
         }
-        for (; e < n; e += kRelaxBlock) pd[e] = ps[e];
+        for (; e < n; e += BS) pd[e] = ps[e];
     }
     __device__ void scal(double a, int xv)
     {
         touch(xv);
         double *p = vec(xv);
         int e = threadIdx.x;
-        for (; e + (kU - 1) * kRelaxBlock < n; e += kU * kRelaxBlock) {
+        for (; e + (kU - 1) * BS < n; e += kU * BS) {
             double v[kU];
 #pragma unroll
-            for (int u = 0; u < kU; u++) v[u] = p[e + u * kRelaxBlock];
+            for (int u = 0; u < kU; u++) v[u] = p[e + u * BS];
 #pragma unroll
-            for (int u = 0; u < kU; u++) p[e + u * kRelaxBlock] = a * v[u];
+            for (int u = 0; u < kU; u++) p[e + u * BS] = a * v[u];
         }
-        for (; e < n; e += kRelaxBlock) p[e] = a * p[e];
+        for (; e < n; e += BS) p[e] = a * p[e];
     }
     __device__ void diff(int xv, int yv, int z)
     {
@@ -139,31 +140,31 @@ struct BlockPlisOps {
         const double *px = vec(xv), *py = vec(yv);
         double *pz = vec(z);
         int e = threadIdx.x;
-        for (; e + (kU - 1) * kRelaxBlock < n; e += kU * kRelaxBlock) {
+        for (; e + (kU - 1) * BS < n; e += kU * BS) {
             double vx[kU], vy[kU];
 #pragma unroll
             for (int u = 0; u < kU; u++) {
-                vx[u] = px[e + u * kRelaxBlock];
-                vy[u] = py[e + u * kRelaxBlock];
+                vx[u] = px[e + u * BS];
+                vy[u] = py[e + u * BS];
             }
 #pragma unroll
-            for (int u = 0; u < kU; u++) pz[e + u * kRelaxBlock] = vx[u] - vy[u];
+            for (int u = 0; u < kU; u++) pz[e + u * BS] = vx[u] - vy[u];
         }
-        for (; e < n; e += kRelaxBlock) pz[e] = px[e] - py[e];
+        for (; e < n; e += BS) pz[e] = px[e] - py[e];
     }
     __device__ double maxabs(int xv) const
     {
         const double *p = vec(xv);
         double m = 0.0;
         int e = threadIdx.x;
-        for (; e + (kU - 1) * kRelaxBlock < n; e += kU * kRelaxBlock) {
+        for (; e + (kU - 1) * BS < n; e += kU * BS) {
             double v[kU];
 #pragma unroll
-            for (int u = 0; u < kU; u++) v[u] = p[e + u * kRelaxBlock];
+            for (int u = 0; u < kU; u++) v[u] = p[e + u * BS];
 #pragma unroll
             for (int u = 0; u < kU; u++) m = fmax(m, fabs(v[u]));
         }
-        for (; e < n; e += kRelaxBlock) m = fmax(m, fabs(p[e]));
+        for (; e < n; e += BS) m = fmax(m, fabs(p[e]));
         return block_reduce(m, true);
     }
     __device__ double l1(int xv) const
@@ -171,14 +172,14 @@ struct BlockPlisOps {
         const double *p = vec(xv);
         double t = 0.0;
         int e = threadIdx.x;
-        for (; e + (kU - 1) * kRelaxBlock < n; e += kU * kRelaxBlock) {
+        for (; e + (kU - 1) * BS < n; e += kU * BS) {
             double v[kU];
 #pragma unroll
-            for (int u = 0; u < kU; u++) v[u] = p[e + u * kRelaxBlock];
+            for (int u = 0; u < kU; u++) v[u] = p[e + u * BS];
 #pragma unroll
             for (int u = 0; u < kU; u++) t += fabs(v[u]);
         }
-        for (; e < n; e += kRelaxBlock) t += fabs(p[e]);
+        for (; e < n; e += BS) t += fabs(p[e]);
         return block_reduce(t, false);
     }
     __device__ double u(int i) const { return lds_u[i]; }
@@ -221,22 +222,23 @@ __device__ __forceinline__ double relax_wrap(double x)
 // SysDesc.energy, gradient in SysDesc.grad) and advances PLIS until it needs the next point, a new two-loop
 // direction (left to the wide kernels), or the end of a pass; stage 1 resumes the jobs that asked for a
 // direction.  Grid: one block of kRelaxBlock threads per job.
-__global__ void __launch_bounds__(kRelaxBlock) k_relax_step(SysDesc *__restrict__ sysv, const RelaxDesc *__restrict__ rdv,
+template <int BS>
+__global__ void __launch_bounds__(BS) k_relax_step(SysDesc *__restrict__ sysv, const RelaxDesc *__restrict__ rdv,
                                                            int stage)
 {
-    __shared__ double lds_red[2 * kRelaxBlock / 64];
+    __shared__ double lds_red[2 * BS / 64];
     __shared__ double lds_u[kPlisMaxHistory], lds_v[kPlisMaxHistory];
     const RelaxDesc R = rdv[blockIdx.x];
     SysDesc *D = sysv + R.sys;
     if (!(D->flags & kSysActive)) return;
     if (stage == 1 && R.st->plis.phase != kPhDirection) return;
     RelaxState st = *R.st;
-    for (int i = threadIdx.x; i < R.mf; i += kRelaxBlock) {
+    for (int i = threadIdx.x; i < R.mf; i += BS) {
         lds_u[i] = R.uv[i];
         lds_v[i] = R.uv[R.mf + i];
     }
     __syncthreads();
-    BlockPlisOps ops{R.n, R.mf, const_cast<double *>(D->x), D->grad, R.s, R.xo, R.go, lds_red, lds_u, lds_v,
+    BlockPlisOps<BS> ops{R.n, R.mf, const_cast<double *>(D->x), D->grad, R.s, R.xo, R.go, lds_red, lds_u, lds_v,
                      false, 0.0, 0, 0.0};
     int act = kPlisNeedEval;
     bool finish = false;
@@ -275,11 +277,11 @@ __global__ void __launch_bounds__(kRelaxBlock) k_relax_step(SysDesc *__restrict_
             if (st.pass >= 3 || E < st.fmin) {
                 finish = true;
             } else {
-                for (int i = threadIdx.x; i < 2 * R.mf * R.n; i += kRelaxBlock) {
+                for (int i = threadIdx.x; i < 2 * R.mf * R.n; i += BS) {
                     if (i < R.mf * R.n) R.xo[i] = 0.0;
                     else R.go[i - R.mf * R.n] = 0.0;
                 }
-                for (int i = threadIdx.x; i < R.mf; i += kRelaxBlock) lds_u[i] = lds_v[i] = 0.0;
+                for (int i = threadIdx.x; i < R.mf; i += BS) lds_u[i] = lds_v[i] = 0.0;
                 __syncthreads();
                 PlisInit(st.plis, R.n, R.mf, st.maxstep, st.fmin);
                 st.boundary = 0;
@@ -300,7 +302,7 @@ __global__ void __launch_bounds__(kRelaxBlock) k_relax_step(SysDesc *__restrict_
         const double r_old = kRelaxRescale[st.pass];
         const double r_new = st.pass + 1 <= 2 ? kRelaxRescale[st.pass + 1] : 1.0;
         double *x = const_cast<double *>(D->x);
-        for (int i = threadIdx.x; i < R.n; i += kRelaxBlock) x[i] = relax_wrap(x[i] / r_old) * r_new;
+        for (int i = threadIdx.x; i < R.n; i += BS) x[i] = relax_wrap(x[i] / r_old) * r_new;
         st.pass++;
         st.boundary = 1;
         __syncthreads();
@@ -314,7 +316,7 @@ __global__ void __launch_bounds__(kRelaxBlock) k_relax_step(SysDesc *__restrict_
         }
         *R.st = st;
     }
-    for (int i = threadIdx.x; i < R.mf; i += kRelaxBlock) {
+    for (int i = threadIdx.x; i < R.mf; i += BS) {
         R.uv[i] = lds_u[i];
         R.uv[R.mf + i] = lds_v[i];
     }
@@ -566,17 +568,24 @@ __global__ void __launch_bounds__(kWideBlock) k_lbfgs_pass2(const SysDesc *__res
     }
 }
 
-void launch_relax_step(hipStream_t stream, SysDesc *sys, const RelaxDesc *rd, int n, const WorkItem *wide_items,
-                       int nwide)
+void launch_relax_step(hipStream_t stream, SysDesc *sys, const RelaxDesc *rd, int n, int n_big,
+                       const WorkItem *wide_items, int nwide)
 {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_relax_step, dim3(n), dim3(kRelaxBlock), 0, stream, sys, rd, 0);
+    // jobs [0, n - n_big) step with kRelaxBlock threads, the large ones after them with kRelaxBlockBig
+    auto step = [&](int stage) {
+        if (n > n_big) hipLaunchKernelGGL(k_relax_step<kRelaxBlock>, dim3(n - n_big), dim3(kRelaxBlock), 0, stream, sys, rd, stage);
+        if (n_big > 0)
+            hipLaunchKernelGGL(k_relax_step<kRelaxBlockBig>, dim3(n_big), dim3(kRelaxBlockBig), 0, stream, sys,
+                               rd + (n - n_big), stage);
+    };
+    step(0);
     if (nwide > 0) {
         hipLaunchKernelGGL(k_lbfgs_pass1, dim3(nwide), dim3(kWideBlock), 0, stream, sys, rd, wide_items);
         hipLaunchKernelGGL(k_lbfgs_solve, dim3(n), dim3(kWideBlock), 0, stream, sys, rd);
         hipLaunchKernelGGL(k_lbfgs_pass2, dim3(nwide), dim3(kWideBlock), 0, stream, sys, rd, wide_items);
     }
-    hipLaunchKernelGGL(k_relax_step, dim3(n), dim3(kRelaxBlock), 0, stream, sys, rd, 1);
+    step(1);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -823,9 +832,16 @@ void Solver::Rebuild(SolverLane &L)
     std::vector<RelaxDesc> rd;
     std::vector<WorkItem> wide;
     L.slots = L.jobs;
-    for (size_t i = 0; i < L.jobs.size(); i++) {
+    for (Job *j : L.jobs) batch.systems.push_back(j->sys.get());
+    // L-BFGS descriptors: the jobs stepped by kRelaxBlock threads first, the large ones (kRelaxBlockBig) last
+    std::vector<size_t> order;
+    for (int big = 0; big < 2; big++)
+        for (size_t i = 0; i < L.jobs.size(); i++)
+            if ((L.jobs[i]->N * L.jobs[i]->d >= kRelaxBigN) == (big == 1)) order.push_back(i);
+    L.n_big = 0;
+    for (size_t i : order) L.n_big += L.jobs[i]->N * L.jobs[i]->d >= kRelaxBigN;
+    for (size_t i : order) {
         Job *j = L.jobs[i];
-        batch.systems.push_back(j->sys.get());
         RelaxDesc r;
         std::memset(&r, 0, sizeof(r));
         r.st = j->st.get();
@@ -838,7 +854,7 @@ void Solver::Rebuild(SolverLane &L)
         r.part2 = j->part2.get();
         r.lcoef = j->lcoef.get();
         r.nchunks = (int)std::max<int64_t>(1, (j->N * j->d + kRelaxChunk - 1) / kRelaxChunk);
-        for (int c = 0; c < r.nchunks; c++) wide.push_back({(int32_t)i, c, 0, 0});
+        for (int c = 0; c < r.nchunks; c++) wide.push_back({(int32_t)rd.size(), c, 0, 0});
         r.n = (int)(j->N * j->d);
         r.mf = j->mf;
         r.sys = (int)i;
@@ -998,7 +1014,7 @@ void Solver::LaunchBurst(int burst)
     auto enqueue = [&](SolverLane &L, int steps) {
         for (int it = 0; it < steps; it++) {
             L.batch.Enqueue(true, true);
-            launch_relax_step(L.batch.stream, L.batch.d_desc.get(), L.d_rdesc.get(), (int)L.slots.size(),
+            launch_relax_step(L.batch.stream, L.batch.d_desc.get(), L.d_rdesc.get(), (int)L.slots.size(), L.n_big,
                               L.d_wide.get(), L.nwide);
         }
     };

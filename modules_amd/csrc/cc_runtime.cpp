@@ -443,6 +443,9 @@ void System::Prepare(hipStream_t stream)
         X.bin_start = U.bin_start.get();
         X.bin_items = U.bin_items.get();
         X.p = U.p.get();
+        X.sub = U.sub.get();
+        X.hlo = U.hlo;
+        X.psize = U.psize;
         for (int c = 0; c < 3; c++) {
             X.nf[c] = U.nf[c];
             X.K[c] = U.K[c];
@@ -450,6 +453,7 @@ void System::Prepare(hipStream_t stream)
             X.nbins[c] = U.nbins[c];
             X.reach[c] = U.reach[c];
             X.poff[c] = U.poff[c];
+            X.pad[c] = U.pad[c];
         }
         X.w = U.w;
         X.nbin = U.nbin;
@@ -650,10 +654,12 @@ void Batch::Build()
     // NUFFT systems: particle blocks (with their count, the zeroing stride), one item per system, bin blocks,
     // output tiles, slot blocks and grid blocks (with their count)
     has_nufft = false;
+    nu_max_psize = 1;
     add(r_nu_part, [&] {
         for (int i = 0; i < ns; i++)
             if (all[i]->nufft_on()) {
                 has_nufft = true;
+                nu_max_psize = std::max(nu_max_psize, all[i]->nu->psize);
                 for (int b = 0; b < pblocks(all[i]); b++) h_items.push_back({i, b, pblocks(all[i]), 0});
             }
     });
@@ -664,7 +670,7 @@ void Batch::Build()
     add(r_nu_bins, [&] {
         for (int i = 0; i < ns; i++)
             if (all[i]->nufft_on())
-                for (int b = 0; b < (all[i]->nu->nbin + kBlock - 1) / kBlock; b++) h_items.push_back({i, b, 0, 0});
+                for (int b = 0; b < (all[i]->nu->nbin + 3) / 4; b++) h_items.push_back({i, b, 0, 0});  // 4 bins each
     });
     add(r_nu_tiles, [&] {
         for (int i = 0; i < ns; i++)
@@ -675,6 +681,13 @@ void Batch::Build()
         for (int i = 0; i < ns; i++)
             if (all[i]->nufft_on())
                 for (int b = 0; b < (all[i]->desc.nslots + kBlock - 1) / kBlock; b++) h_items.push_back({i, b, 0, 0});
+    });
+    add(r_nu_gath, [&] {  // grid blocks of the gather (one grid)
+        for (int i = 0; i < ns; i++)
+            if (all[i]->nufft_on()) {
+                const int nb = (int)std::min<int64_t>(2048, (all[i]->nu->npts + kBlock - 1) / kBlock);
+                for (int b = 0; b < nb; b++) h_items.push_back({i, b, nb, 0});
+            }
     });
     add(r_nu_grid, [&] {
         for (int i = 0; i < ns; i++)
@@ -717,7 +730,8 @@ void Batch::Enqueue(bool rho, bool forces)
         launch_rho_general(stream, D, I + r_rho_gen.off, r_rho_gen.n);
         if (has_nufft) {
             launch_nufft_rho(stream, D, I + r_nu_part.off, r_nu_part.n, I + r_nu_sys.off, r_nu_sys.n, I + r_nu_bins.off,
-                             r_nu_bins.n, I + r_nu_tiles.off, r_nu_tiles.n);
+                             r_nu_bins.n, I + r_nu_tiles.off, r_nu_tiles.n, I + r_nu_gath.off, r_nu_gath.n,
+                             nu_max_psize);
             NufftFft(true);
             launch_nufft_extract(stream, D, I + r_nu_slots.off, r_nu_slots.n);
         }

@@ -133,7 +133,8 @@ struct NufftState {
     int nbin = 1;
     int64_t npts = 1;
     DevArray<double2> grid;
-    DevArray<double> kv, p;
+    DevArray<double> kv, p, sub;
+    int hlo = 0, pad[3] = {1, 1, 1}, psize = 1;
     DevArray<int32_t> start, bin_of, bin_start, bin_items;
     hipfftHandle plan1 = nullptr, plan2 = nullptr;
     NufftState() = default;
@@ -254,8 +255,9 @@ public:
     DevArray<WorkItem> d_items;
     std::vector<WorkItem> h_items;
     Range r_prep, r_rho_gen, r_red, r_efin, r_f_gen, r_ffin, r_sc_part, r_sc_sys, r_sc_cell, r_mix;
-    Range r_nu_part, r_nu_sys, r_nu_bins, r_nu_tiles, r_nu_slots, r_nu_grid;  // NUFFT systems
+    Range r_nu_part, r_nu_sys, r_nu_bins, r_nu_tiles, r_nu_gath, r_nu_slots, r_nu_grid;  // NUFFT systems
     bool has_nufft = false;
+    int nu_max_psize = 1;  // largest padded tile of the batch's NUFFT systems (dynamic LDS of the spread kernel)
     // every system of the launches: `systems` in order (their descriptor indices), then the species of each
     // mixture among them
     std::vector<System *> all;
@@ -287,7 +289,8 @@ void launch_softcore(hipStream_t, const SysDesc *, const WorkItem *part_items, i
                      int nsys, const WorkItem *cell_items, int ncell);
 // NUFFT backend (cc_nufft.hip); the FFTs between the stages run through each system's hipFFT plans
 void launch_nufft_rho(hipStream_t, const SysDesc *, const WorkItem *part, int npart, const WorkItem *one, int nsys,
-                      const WorkItem *bins, int nbins, const WorkItem *tiles, int ntiles);
+                      const WorkItem *bins, int nbins, const WorkItem *tiles, int ntiles, const WorkItem *grid,
+                      int ngrid, int max_psize);
 void launch_nufft_extract(hipStream_t, const SysDesc *, const WorkItem *slots, int nslots);
 void launch_nufft_coef(hipStream_t, const SysDesc *, const WorkItem *grid, int ngrid, const WorkItem *slots, int nslots);
 void launch_nufft_interp(hipStream_t, const SysDesc *, const WorkItem *part, int npart);
