@@ -211,6 +211,48 @@ int ccgpu_multi_species_energy(ccgpu_multi *p, int64_t species, double *E);
 int ccgpu_multi_relax(ccgpu_multi *p, double *relative, int maxstep, double Emin, int history, double *energy,
                       int *evaluations, int *status);
 
+/* ExportConfiguration(ofile, format) -- MultiStealthyPotential.cpp:28-131: format 0 the text table, 1 the MSPB v1
+ * snapshot, of the stored configuration (caller's particle order), species labels and radii.  Writes *size bytes
+ * into buf when cap is large enough (call with buf NULL to learn the size). */
+int ccgpu_multi_export(ccgpu_multi *p, int format, void *buf, int64_t cap, int64_t *size);
+
+/* ---------------------------------------------------------------------------------------------
+ * On-disk formats of the reference's tools (host only; no device needed)
+ * ------------------------------------------------------------------------------------------- */
+
+/* ConfigurationPack(prefix) (core/PeriodicCellList.h:1598-1640, core/PeriodicCellList.cpp:124-204), or with
+ * spheres != 0 SpherePackingPack(prefix) (core/SpherePacking.h:215-251): files prefix.ConfigPack[Index] /
+ * prefix.SpherePack[Index]; records in PeriodicCellList::WriteBinary layout (core/PeriodicCellList.h:1254-1267).
+ * NumConfig: */
+int ccgpu_pack_count(const char *prefix, int spheres, int64_t *count);
+/* AddConfig: dimension, cell size (ApproximateCellSize), basis (dim x dim), N, relative (N x dim) and the particles'
+ * characteristics -- 4-byte names (AtomName; NULL: "a") or radii (doubles; NULL: 0). */
+int ccgpu_pack_add(const char *prefix, int spheres, int dim, double cell_size, const double *basis, int64_t N,
+                   const double *relative, const void *chars);
+/* GetConfig(i): dimension, cell size, basis, N, and up to cap particles' coordinates and characteristics. */
+int ccgpu_pack_get(const char *prefix, int spheres, int64_t i, int *dim, double *cell_size, double *basis,
+                   int64_t *N, int64_t cap, double *relative, void *chars);
+/* Clear(): discard every configuration of the pack. */
+int ccgpu_pack_clear(const char *prefix, int spheres);
+
+/* MSPB v1 snapshot (apps/CCOptimization/EXPORT_FORMAT.md; MSPBSnapshot::Write/Read, MSPB_Pack.h): encode into buf
+ * (*size bytes; buf NULL or cap too small: size only); ids NULL means 0..N-1.  Decode validates as the reference's
+ * reader: version, flags, dimension 1-4, exact byte count, finite values, positive radii, unique ids, labels below
+ * the species count, nonsingular cell. */
+int ccgpu_mspb_encode(int dim, const double *basis, int64_t N, const uint64_t *ids, const uint32_t *species,
+                      const double *relative, const double *radii, uint64_t nspecies, uint8_t *buf, int64_t cap,
+                      int64_t *size);
+int ccgpu_mspb_decode(const uint8_t *bytes, int64_t size, int *dim, double *basis, int64_t *N, uint64_t *nspecies,
+                      int64_t cap, uint64_t *ids, uint32_t *species, double *relative, double *radii);
+/* The text table of ExportConfiguration (MultiStealthyPotential.cpp:82-131) for given data. */
+int ccgpu_export_text(int dim, const double *basis, int64_t N, const int32_t *species, const double *relative,
+                      const double *radii, char *buf, int64_t cap, int64_t *size);
+/* MSPB archive prefix.MSPBPack (apps/CCOptimization/MSPB_PACK.md; MSPB_Pack): NumConfig, AddConfig (a validated
+ * snapshot, bytes unchanged), GetConfig (the snapshot's bytes). */
+int ccgpu_mspbpack_count(const char *prefix, int64_t *count);
+int ccgpu_mspbpack_add(const char *prefix, const uint8_t *snapshot, int64_t size);
+int ccgpu_mspbpack_get(const char *prefix, int64_t index, uint8_t *buf, int64_t cap, int64_t *size);
+
 /* ---------------------------------------------------------------------------------------------
  * L-BFGS (Luksan's PLIS, the algorithm of NLopt's NLOPT_LD_LBFGS) on the host
  * ------------------------------------------------------------------------------------------- */

@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(_HERE, "libccgpu.so")
 dp = C.POINTER(C.c_double)
 ip32 = C.POINTER(C.c_int32)
 i64p = C.POINTER(C.c_int64)
+u64p = C.POINTER(C.c_uint64)
+u32p = C.POINTER(C.c_uint32)
 vp = C.c_void_p
 
 _lib = None
@@ -110,6 +112,19 @@ SIGNATURES = {
     "ccgpu_multi_species_energy": (C.c_int, [vp, C.c_int64, dp]),
     "ccgpu_multi_relax": (C.c_int, [vp, dp, C.c_int, C.c_double, C.c_int, dp, C.POINTER(C.c_int),
                                     C.POINTER(C.c_int)]),
+    "ccgpu_multi_export": (C.c_int, [vp, C.c_int, vp, C.c_int64, i64p]),
+    "ccgpu_pack_count": (C.c_int, [C.c_char_p, C.c_int, i64p]),
+    "ccgpu_pack_add": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_double, dp, C.c_int64, dp, vp]),
+    "ccgpu_pack_get": (C.c_int, [C.c_char_p, C.c_int, C.c_int64, C.POINTER(C.c_int), dp, dp, i64p, C.c_int64, dp,
+                                 vp]),
+    "ccgpu_pack_clear": (C.c_int, [C.c_char_p, C.c_int]),
+    "ccgpu_mspb_encode": (C.c_int, [C.c_int, dp, C.c_int64, u64p, u32p, dp, dp, C.c_uint64, vp, C.c_int64, i64p]),
+    "ccgpu_mspb_decode": (C.c_int, [vp, C.c_int64, C.POINTER(C.c_int), dp, i64p, u64p, C.c_int64, u64p, u32p, dp,
+                                    dp]),
+    "ccgpu_export_text": (C.c_int, [C.c_int, dp, C.c_int64, ip32, dp, dp, vp, C.c_int64, i64p]),
+    "ccgpu_mspbpack_count": (C.c_int, [C.c_char_p, i64p]),
+    "ccgpu_mspbpack_add": (C.c_int, [C.c_char_p, vp, C.c_int64]),
+    "ccgpu_mspbpack_get": (C.c_int, [C.c_char_p, C.c_int64, vp, C.c_int64, i64p]),
 }
 
 OBJECTIVE = C.CFUNCTYPE(C.c_double, C.c_uint, dp, dp, C.c_void_p)

@@ -457,6 +457,40 @@ int ccgpu_multi_species_energy(ccgpu_multi *p, int64_t species, double *E)
     });
 }
 
+int ccgpu_export_text(int dim, const double *basis, int64_t N, const int32_t *species, const double *relative,
+                      const double *radii, char *buf, int64_t cap, int64_t *size);
+int ccgpu_mspb_encode(int dim, const double *basis, int64_t N, const uint64_t *ids, const uint32_t *species,
+                      const double *relative, const double *radii, uint64_t nspecies, uint8_t *buf, int64_t cap,
+                      int64_t *size);
+
+int ccgpu_multi_export(ccgpu_multi *p, int format, void *buf, int64_t cap, int64_t *size)
+{
+    return Guard([&] {
+        // ExportConfiguration (MultiStealthyPotential.cpp:28-131): the stored configuration, species and radii
+        Require(p && size, CCGPU_ERR_ARG, "null argument");
+        Require(format == 0 || format == 1, CCGPU_ERR_ARG, "ExportConfiguration: unknown format");
+        Require(p->bound, CCGPU_ERR_STATE, "ExportConfiguration: no configuration has been set");
+        Require((int64_t)p->species_of.size() == p->N && (int64_t)p->radii.size() == p->N, CCGPU_ERR_STATE,
+                "ExportConfiguration: inconsistent particle metadata");
+        for (int64_t i = 0; i < p->N; i++) {
+            if (p->species_of[(size_t)i] < 0 || p->species_of[(size_t)i] >= p->NumSpecies())
+                throw Error(CCGPU_ERR_STATE, "ExportConfiguration: particle has no valid species");
+            if (!std::isfinite(p->radii[(size_t)i]) || !(p->radii[(size_t)i] > 0.0))
+                throw Error(CCGPU_ERR_STATE, "ExportConfiguration: particle has an invalid radius");
+        }
+        int st;
+        if (format == 0) {
+            st = ccgpu_export_text(p->d, p->basis.data(), p->N, p->species_of.data(), p->rel.data(), p->radii.data(),
+                                   static_cast<char *>(buf), cap, size);
+        } else {
+            std::vector<uint32_t> lab(p->species_of.begin(), p->species_of.end());
+            st = ccgpu_mspb_encode(p->d, p->basis.data(), p->N, nullptr, lab.data(), p->rel.data(), p->radii.data(),
+                                   (uint64_t)p->NumSpecies(), static_cast<uint8_t *>(buf), cap, size);
+        }
+        if (st != CCGPU_OK) throw Error(st, LastError());
+    });
+}
+
 int ccgpu_multi_relax(ccgpu_multi *p, double *relative, int maxstep, double Emin, int history, double *energy,
                       int *evaluations, int *status)
 {

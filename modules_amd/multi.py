@@ -10,6 +10,7 @@ evaluation runs on the GPU; there is no CPU path.
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -156,8 +157,8 @@ class MultiStealthyPotential:
         self.species_constraints[int(species)].CCPotential.AddConstraint(k, V)
 
     def SetRhoBackend(self, backend):
-        """Only the direct sum exists here (evaluated on the GPU); the reference's FINUFFT backend is not on this
-        path."""
+        """Species of a mixture evaluate with the direct sums (on the GPU); the NUFFT backend is selected per
+        single-species potential (ShiftedCCPotential.SetRhoBackend)."""
         if str(backend).lower() not in ("direct", "0", "rhobackend.direct"):
             raise ValueError("MultiStealthyPotential::SetRhoBackend: only the direct backend is available")
 
@@ -220,6 +221,26 @@ class MultiStealthyPotential:
         out = np.zeros(max(n, 1))
         check(lib().ccgpu_multi_get_radii(self._h, _p(out)))
         return out[:n]
+
+    def ExportConfiguration(self, ofile, format="text"):
+        """ExportConfiguration (MultiStealthyPotential.cpp:28-131): the stored configuration, species labels and radii
+        as the text table ("text") or an MSPB v1 snapshot ("binary"), written to `ofile` (a path, or a file object
+        opened in text / binary mode); returns what was written."""
+        codes = {"text": 0, "binary": 1, "Text": 0, "Binary": 1}
+        if format not in codes:
+            raise ValueError("ExportConfiguration: unknown format.")
+        size = C.c_int64(0)
+        check(lib().ccgpu_multi_export(self._h, codes[format], None, 0, C.byref(size)))
+        buf = C.create_string_buffer(max(size.value, 1))
+        check(lib().ccgpu_multi_export(self._h, codes[format], buf, size.value, C.byref(size)))
+        data = buf.raw[: size.value]
+        out = data.decode() if codes[format] == 0 else data
+        if isinstance(ofile, (str, bytes, os.PathLike)):
+            with open(ofile, "w" if codes[format] == 0 else "wb") as f:
+                f.write(out)
+        elif ofile is not None:
+            ofile.write(out)
+        return out
 
     def GetPackingSnapshot(self):
         """GetPackingSnapshot (MultiStealthyPotential.cpp:15-26): cell, relative coordinates and radii."""

@@ -115,8 +115,10 @@ def CollectiveCoordinateMultiRun(pConfig: Configuration, pPotential, gen, Prefix
     soft core) or a MultiStealthyPotential (GetMultiCCO's ground mode); a mixture keeps its species and radii.  Results are handed out in sample order, so
     onAccepted(config, success_index) sees the reference's order.  TimeLimit (absolute time.time() seconds, as
     the reference's time_t) is checked after each sample; reaching it ends the run without re-randomizing
-    pConfig.  Returns {"init": [...], "relaxed": [...], "success": [...], "energies": [...]}; the lists stand in
-    for the ConfigurationPack files Prefix, Prefix_InitConfig and Prefix_Success.
+    pConfig.  Returns {"init": [...], "relaxed": [...], "success": [...], "energies": [...]}; with a non-empty
+    Prefix the same configurations are also appended to the ConfigurationPack files Prefix (relaxed),
+    Prefix_InitConfig and Prefix_Success, as the reference writes them, and onAccepted gets the success pack's
+    index.
     """
     if AlgorithmName != "LBFGS":
         raise NotImplementedError(f"CollectiveCoordinateMultiRun: only LBFGS is on this path (got {AlgorithmName})")
@@ -129,6 +131,12 @@ def CollectiveCoordinateMultiRun(pConfig: Configuration, pPotential, gen, Prefix
     for _ in range(1, SampleNumber):
         starts.append(Configuration(pConfig.basis, gen.RandomDoubles(N * d).reshape(N, d)))
     out = {"init": [], "relaxed": [], "success": [], "energies": []}
+    packs = None
+    if Prefix:
+        # Prefix (relaxed), Prefix_Success and Prefix_InitConfig ConfigurationPacks (main.cpp:1816-1818)
+        from .formats import ConfigurationPack
+        packs = (ConfigurationPack(Prefix), ConfigurationPack(Prefix + "_Success"),
+                 ConfigurationPack(Prefix + "_InitConfig"))
     if SampleNumber <= 0:
         return out
     solver = BatchSolver(max_slots=max(1, min(max_slots, SampleNumber)))
@@ -156,10 +164,15 @@ def CollectiveCoordinateMultiRun(pConfig: Configuration, pPotential, gen, Prefix
             out["init"].append(init)
             out["relaxed"].append(result)
             out["energies"].append(E)
+            if packs:  # the reference's order: start, relaxed result, then the success pack
+                packs[2].AddConfig(init)
+                packs[0].AddConfig(result)
             if E < E_max:
                 out["success"].append(result)
+                if packs:
+                    packs[1].AddConfig(result)
                 if onAccepted is not None:
-                    onAccepted(result, len(out["success"]) - 1)
+                    onAccepted(result, len(out["success"]) - 1 if not packs else packs[1].NumConfig() - 1)
             solver.release(ids[nxt])
             nxt += 1
             if TimeLimit is not None and time.time() > TimeLimit:

@@ -85,7 +85,11 @@ class Configuration:
         self.basis = self.basis * factor
 
     def copy(self):
-        return Configuration(self.basis, self.relative)
+        c = Configuration(self.basis, self.relative)
+        for attr in ("cell_size", "names"):
+            if hasattr(self, attr):
+                setattr(c, attr, getattr(self, attr))
+        return c
 
 
 def _wrap(s):
@@ -140,7 +144,9 @@ def RandomConfiguration(seed, N, d, L=None):
         check(lib().ccgpu_random_configuration(seed & 0xFFFFFFFF, N, d, _p(s)))
     if L is None:
         L = math.pow(N, 1.0 / d)
-    return Configuration(np.eye(d) * L, s)
+    c = Configuration(np.eye(d) * L, s)
+    c.cell_size = 0.1 * L  # GetUnitCubicBox(d, 0.1) rescaled by L (core/PeriodicCellList.h:1201-1210)
+    return c
 
 
 def getK(chi, rho, d):

@@ -276,3 +276,51 @@ def test_multirun_with_a_mixture():
     print("multirun mixture energies:", out["energies"], flush=True)
     assert len(out["relaxed"]) == 3
     assert len(out["success"]) == sum(e < 1e-14 for e in out["energies"]) >= 2
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_export_configuration_follows_the_reference_export_test(dim, tmp_path):
+    # apps/CCOptimization/tests/test_export.cpp: errors before a configuration and before every particle has a
+    # species; after the reassignments the text and MSPB bytes carry the stored configuration, labels and radii
+    from modules_amd._lib import CCGPUError
+    from modules_amd.formats import ExportText, MSPBSnapshot
+    a = np.zeros((dim, dim))
+    for d in range(dim):
+        a[d, d] = 4.0 + d
+    a[1, 0] = 1.0 / 7.0
+    s = np.array([[(i + d + 1) / 11.0 for d in range(dim)] for i in range(3)])
+    p = MultiStealthyPotential(dim, 1.0)
+    with pytest.raises(CCGPUError):
+        p.ExportConfiguration(None)
+    p.SetConfiguration(Configuration(a, s))
+    p.SetNumSpecies(2)
+    with pytest.raises(CCGPUError):
+        p.ExportConfiguration(None, "binary")
+    for i, q in ((0, 0), (0, 1), (1, 1), (2, 0)):
+        p.SetSpeciesOfParticle(i, q)
+    p.SetSecureRadii([0.125, 0.25, 0.375])
+    text = p.ExportConfiguration(str(tmp_path / "c.txt"))
+    assert text == ExportText(a, s, [1, 1, 0], [0.125, 0.25, 0.375])
+    assert open(tmp_path / "c.txt").read() == text
+    data = p.ExportConfiguration(None, "binary")
+    assert data == MSPBSnapshot(a, s, [1, 1, 0], [0.125, 0.25, 0.375], 2).Write()
+    with pytest.raises(ValueError):
+        p.ExportConfiguration(None, "xml")
+
+
+def test_multirun_writes_the_reference_packs(tmp_path):
+    from modules_amd.formats import ConfigurationPack
+    from modules_amd.optimize import CollectiveCoordinateMultiRun
+    from modules_amd.potential import RandomGenerator
+    cfg, p, _ = mixture_job(31)
+    prefix = str(tmp_path / "run")
+    seen = []
+    out = CollectiveCoordinateMultiRun(cfg, p, RandomGenerator(9), Prefix=prefix, SampleNumber=2, E_max=1e-14,
+                                       onAccepted=lambda c, i: seen.append(i))
+    relaxed, success, init = (ConfigurationPack(prefix), ConfigurationPack(prefix + "_Success"),
+                              ConfigurationPack(prefix + "_InitConfig"))
+    assert relaxed.NumConfig() == init.NumConfig() == 2
+    assert success.NumConfig() == len(out["success"]) and seen == list(range(len(out["success"])))
+    for q in range(2):
+        assert np.array_equal(relaxed.GetConfig(q).relative, out["relaxed"][q].relative)
+        assert np.array_equal(init.GetConfig(q).relative, out["init"][q].relative)
