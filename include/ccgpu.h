@@ -67,6 +67,19 @@ int ccgpu_potential_add_constraint(ccgpu_potential *p, const double *k, double V
 int ccgpu_potential_set_constraints(ccgpu_potential *p, int64_t M, const double *k, const double *V);
 int ccgpu_potential_num_constraints(const ccgpu_potential *p, int64_t *M);
 
+/* ShiftedCCPotential_S2(DimensionType) -- CollectiveCoordinatePotential.h:1642-1748: the target-S(k) form
+ *   Phi = sum_k V(k) (|rho(k)|^2 / N - S0(k))^2 + Shift      (no 1/volume factor)
+ *   F_j = -(4 / N) sum_k V(k) (|rho(k)|^2 / N - S0(k)) (Im rho cos(k.r_j) - Re rho sin(k.r_j)) k
+ * (Energy :1686-1712, Force :1718-1741).  Every other ccgpu_potential_* entry works on it as on the plain form;
+ * ccgpu_potential_add_constraint / set_constraints return CCGPU_ERR_ARG on it (the reference's AddConstraint
+ * throws std::invalid_argument when vals lacks S0, :1663-1668). */
+int ccgpu_potential_create_s2(int dim, ccgpu_potential **out);
+/* AddConstraint(k, {V, S0}) -- CollectiveCoordinatePotential.h:1663-1668 */
+int ccgpu_potential_add_constraint_s2(ccgpu_potential *p, const double *k, double V, double S0);
+/* constraints = ..., S0 = ... (public members, :1645, :1667): k M x dim, V M, S0 M */
+int ccgpu_potential_set_constraints_s2(ccgpu_potential *p, int64_t M, const double *k, const double *V,
+                                       const double *S0);
+
 /* public member Shift -- CollectiveCoordinatePotential.h:99 */
 int ccgpu_potential_set_shift(ccgpu_potential *p, double shift);
 
@@ -279,6 +292,43 @@ int ccgpu_lbfgs_minimize(int n, ccgpu_objective f, void *f_data, double *x, doub
 int ccgpu_potential_relax(ccgpu_potential *p, double *relative, int maxstep, double Emin, int history, double *energy,
                           int *evaluations, int *status);
 
+/* ---------------------------------------------------------------------------------------------
+ * Variable-cell potentials (potential/CCPotential_varBox.h, potential/CCPotential_Kernels.h).
+ * A variable-cell potential is a ccgpu_potential whose constraints follow its cell: each SetConfiguration with
+ * a new cell regenerates the reciprocal lattice points 0 < |k| < rK (one of each +-k pair: the first non-zero
+ * coefficient on the reciprocal basis positive; ordered by |k| then by coefficients), weights them by
+ * tildeV(k) and, for the target-S(k) forms, sets S0(k); the Shift follows the cell and N
+ * (ShiftedCCPotential_varBoxable::SetConfiguration, CCPotential_varBox.h:129-208).
+ *   form 0  ShiftedCCPotential_varBox (:215-291): Phi = (1/V) sum v |rho|^2 + N (N - 1) / (2 V) - (N / V) sum v
+ *   form 1  ShiftedCCPotential_S2_varBox (:300-379): Phi = sum v (|rho|^2 / N - S0)^2
+ *   form 2  ShiftedCCPotential_S2_varBox_noV0 (:387-467): Phi = sum v (|rho|^2 / N - S0)^2 - sum v (S0 - 1)^2
+ *   vk      tildeV: 0 (1 - k)^2 (K2Template), 1 (1/k - 1)^2 (KM2), 2 (1/k - 1)^3 (KM3), 3 (1/k - 1)^4 (KM4),
+ *           4 lemniscate, 2D only, with Rc = rK (LemniscateVkTemplate)          (CCPotential_Kernels.h:18-186)
+ *   s0      target: 0 none (form 0 only), 1 k^2 (S0K2Template), 2 k^n (S0Kn), 3 (k^n + 1) / 2 (S0Klc),
+ *           4 (1/k - 1)^2 (S0KM2), 5 -log k (S0KMLog)                          (CCPotential_Kernels.h:195-340)
+ * The reference's named types are combinations: K2Potential = (0, 0, 0), KM3Potential = (0, 2, 0),
+ * K2K2Potential = (1, 0, 1), K2K2Potential_noV0 = (2, 0, 1), KM3K2Potential_noV0 = (2, 2, 1), KM2K2 / KM3K2 /
+ * KM4K2Potential = (1, 1..3, 1), KM3KM2Potential = (1, 2, 4), KM3KMLogPotential = (1, 2, 5), KM2Kn / KM3Kn /
+ * KM4KnPotential = (1, 1..3, 2), KM2Klc / KM3Klc / KM4KlcPotential = (1, 1..3, 3) (:342-358).
+ * Constraint edits (add/set_constraints[_s2]) return CCGPU_ERR_STATE and the soft core is not available. */
+int ccgpu_potential_create_varbox(int dim, int form, int vk, int s0, double rK, double n, ccgpu_potential **out);
+/* the current constraints: k M x dim, V M, S0 M (target-S(k) forms), n M x dim integer coefficients on the
+ * reciprocal basis (variable-cell potentials); any pointer may be null */
+int ccgpu_potential_get_constraints(const ccgpu_potential *p, double *k, double *V, double *S0, int32_t *n);
+/* EnergyDerivativeToBasisVectors(grad, Pressure) (CCPotential_varBox.h:248-290, 333-378, 410-466):
+ * grad[l dim + m] = d(Phi + P V) / d(component m of basis vector l) at fixed relative coordinates.
+ * CCGPU_ERR_STATE for a fixed-cell potential (EnergyDerivativeToBasisVectors_NotImplemented,
+ * potential/Potential.h:23-37, 84-87). */
+int ccgpu_potential_energy_derivative_to_basis(ccgpu_potential *p, double pressure, double *grad);
+/* RelaxStructure_NLOPT / RelaxStructure_NLOPT_Emin (potential/StructureOptimization.cpp:881-979, 1046-1084) with
+ * Switch (mode) 0: particles, 1: cell, 2: both, minimizing the enthalpy E + P V from the bound configuration.
+ * basis (dim x dim) and relative (N x dim) hold the start (the bound configuration's values) and receive the
+ * result, which is left bound.  min_distance must not exceed 1e-4 (the neighbour check is not provided).
+ * enthalpy: final E + P V. */
+int ccgpu_potential_relax_cell(ccgpu_potential *p, double *basis, double *relative, double pressure, int mode,
+                               double min_distance, int maxstep, double Emin, int history, double *enthalpy,
+                               int *evaluations, int *status);
+
 /* Device-resident batched solver with continuous batching.  Jobs wait in a queue and are admitted into
  * one of max_slots slots while their HBM footprint fits hbm_budget bytes; every resident job advances
  * in the same batched E+F launches and one L-BFGS step launch (one workgroup per job) per evaluation.
@@ -293,6 +343,11 @@ int ccgpu_solver_destroy(ccgpu_solver *s);
 int ccgpu_solver_submit(ccgpu_solver *s, int dim, const double *basis, int64_t N, const double *relative, int64_t M,
                         const double *k, const double *V, double shift, double epsilon, double sigma, int maxstep,
                         double Emin, int history, int64_t *job);
+/* The same request with the target-S(k) potential (ShiftedCCPotential_S2, as ccgpu_potential_create_s2; the
+ * RepulsiveCCPotential with S0 != 0 of apps/CCOptimization/RepulsiveCCPotential.h:40-46): S0 M targets. */
+int ccgpu_solver_submit_s2(ccgpu_solver *s, int dim, const double *basis, int64_t N, const double *relative,
+                           int64_t M, const double *k, const double *V, const double *S0, double shift,
+                           double epsilon, double sigma, int maxstep, double Emin, int history, int64_t *job);
 /* A RelaxStructure_NLOPT[_Emin] request with the multi-species potential (MultiStealthyPotential, as
  * ccgpu_multi_*): species q holds particles idx[idx_off[q] .. idx_off[q+1]) (its own order; every particle in
  * exactly one species) and constraints k/V[k_off[q] .. k_off[q+1]); radii N, contact cutoff rc (>= the largest

@@ -65,10 +65,21 @@ SIGNATURES = {
                                        C.POINTER(C.c_int)]),
     "ccgpu_potential_relax": (C.c_int, [vp, dp, C.c_int, C.c_double, C.c_int, dp, C.POINTER(C.c_int),
                                         C.POINTER(C.c_int)]),
+    "ccgpu_potential_create_s2": (C.c_int, [C.c_int, C.POINTER(vp)]),
+    "ccgpu_potential_add_constraint_s2": (C.c_int, [vp, dp, C.c_double, C.c_double]),
+    "ccgpu_potential_set_constraints_s2": (C.c_int, [vp, C.c_int64, dp, dp, dp]),
+    "ccgpu_potential_create_varbox": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                                C.POINTER(vp)]),
+    "ccgpu_potential_get_constraints": (C.c_int, [vp, dp, dp, dp, ip32]),
+    "ccgpu_potential_energy_derivative_to_basis": (C.c_int, [vp, C.c_double, dp]),
+    "ccgpu_potential_relax_cell": (C.c_int, [vp, dp, dp, C.c_double, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int,
+                                             dp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "ccgpu_solver_create": (C.c_int, [C.c_int, C.c_int64, C.POINTER(vp)]),
     "ccgpu_solver_destroy": (C.c_int, [vp]),
     "ccgpu_solver_submit": (C.c_int, [vp, C.c_int, dp, C.c_int64, dp, C.c_int64, dp, dp, C.c_double, C.c_double,
                                       C.c_double, C.c_int, C.c_double, C.c_int, i64p]),
+    "ccgpu_solver_submit_s2": (C.c_int, [vp, C.c_int, dp, C.c_int64, dp, C.c_int64, dp, dp, dp, C.c_double,
+                                         C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, i64p]),
     "ccgpu_solver_submit_multi": (C.c_int, [vp, C.c_int, dp, C.c_int64, dp, C.c_int64, i64p, i64p, i64p, dp, dp, dp,
                                             C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, i64p]),
     "ccgpu_solver_step": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int)]),

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "cc_abi.h"
+#include "cc_varbox.h"
 
 using namespace ccgpu;
 
@@ -69,6 +70,67 @@ struct ccgpu_potential {
         batch.Sync();
         return e[0];
     }
+
+    // variable-cell form (ccgpu_potential_create_varbox): the constraints, weights, targets and Shift follow the
+    // cell (ShiftedCCPotential_varBoxable, potential/CCPotential_varBox.h:31-209)
+    bool varbox = false;
+    VarBoxSpec vb;
+    std::vector<int32_t> vb_n;  // integer coefficients of each constraint on the reciprocal basis
+    bool vb_have = false;       // vb_a holds the cell the constraints were generated for
+    double vb_a[16] = {};
+    int64_t vb_N = -1;          // particle count the Shift was computed for
+
+    // SetConfiguration of the variable-cell forms (:129-208, :232-245, :320-330, :392-406): regenerate the k-points
+    // with their weights (and targets) when the cell differs from the cached one, and the Shift then or when N
+    // changes.  sys.SetCell has already been given the new cell.
+    void RegenerateVarBox(const double *basis, int64_t N)
+    {
+        const int d = sys.d;
+        const bool same = vb_have && std::equal(basis, basis + d * d, vb_a);
+        if (!same) {
+            std::vector<double> k;
+            VarBoxKPoints(d, sys.b, sys.a, vb.rK, k, vb_n);
+            const int64_t M = (int64_t)(k.size() / d);
+            std::vector<double> v(M), t(M);
+            for (int64_t i = 0; i < M; i++) {
+                v[i] = TildeV(vb, d, k.data() + i * d);
+                if (vb.form != kVarBoxStealthy) t[i] = TargetS0(vb, d, k.data() + i * d);
+            }
+            sys.SetConstraints(M, k.data(), v.data());
+            if (vb.form != kVarBoxStealthy) sys.SetTargets(t.data());
+            std::copy(basis, basis + d * d, vb_a);
+            vb_have = true;
+            batch.dirty = true;
+        }
+        if (!same || vb_N != N) {
+            sys.shift = VarBoxShift(vb, N, sys.vol, sys.M, sys.v.data(), sys.s0.data());
+            vb_N = N;
+            batch.dirty = true;
+        }
+    }
+
+    // SetConfiguration: cell, particle count and relative coordinates
+    void Bind(const double *basis, int64_t N, const double *relative)
+    {
+        const int d = sys.d;
+        sys.SetCell(basis);
+        if (varbox) RegenerateVarBox(basis, N);
+        if (sys.N != N) {
+            sys.N = N;
+            batch.dirty = true;
+        }
+        if (sys.layout_dirty) batch.dirty = true;
+        if (sys.rescale != 1.0) {
+            sys.rescale = 1.0;
+            batch.dirty = true;
+        }
+        bound = true;
+        EnsureBuilt();
+        if (N > 0)
+            CC_HIP(hipMemcpyAsync(sys.x.get(), relative, (size_t)(N * d) * sizeof(double), hipMemcpyHostToDevice,
+                                  batch.stream));
+        Invalidate();
+    }
 };
 
 extern "C" {
@@ -126,6 +188,46 @@ int ccgpu_potential_create(int dim, ccgpu_potential **out)
     });
 }
 
+int ccgpu_potential_create_s2(int dim, ccgpu_potential **out)
+{
+    return Guard([&] {
+        Require(out != nullptr, CCGPU_ERR_ARG, "null output handle");
+        *out = nullptr;
+        Require(dim >= 1 && dim <= kMaxDim, CCGPU_ERR_ARG, "dimension must be 1..4");
+        RequireDevice();
+        auto *p = new ccgpu_potential(dim);
+        p->sys.s2 = true;
+        *out = p;
+    });
+}
+
+int ccgpu_potential_add_constraint_s2(ccgpu_potential *p, const double *k, double V, double S0)
+{
+    return Guard([&] {
+        Require(p && k, CCGPU_ERR_ARG, "null argument");
+        Require(p->sys.s2, CCGPU_ERR_STATE, "not a target-S(k) potential");
+        Require(!p->varbox, CCGPU_ERR_STATE, "the constraints of a variable-cell potential follow its cell");
+        p->sys.AddConstraintS2(k, V, S0);
+        p->batch.dirty = true;
+        p->Invalidate();
+    });
+}
+
+int ccgpu_potential_set_constraints_s2(ccgpu_potential *p, int64_t M, const double *k, const double *V,
+                                       const double *S0)
+{
+    return Guard([&] {
+        Require(p && M >= 0 && (M == 0 || (k && V && S0)), CCGPU_ERR_ARG, "bad argument");
+        Require(p->sys.s2, CCGPU_ERR_STATE, "not a target-S(k) potential");
+        Require(!p->varbox, CCGPU_ERR_STATE, "the constraints of a variable-cell potential follow its cell");
+        for (int64_t i = 0; i < M; i++) Require(std::isfinite(S0[i]), CCGPU_ERR_ARG, "non-finite target S0");
+        p->sys.SetConstraints(M, k, V);
+        p->sys.SetTargets(S0);
+        p->batch.dirty = true;
+        p->Invalidate();
+    });
+}
+
 int ccgpu_potential_destroy(ccgpu_potential *p)
 {
     return Guard([&] { delete p; });
@@ -135,6 +237,8 @@ int ccgpu_potential_add_constraint(ccgpu_potential *p, const double *k, double V
 {
     return Guard([&] {
         Require(p && k, CCGPU_ERR_ARG, "null argument");
+        Require(!p->varbox, CCGPU_ERR_STATE, "the constraints of a variable-cell potential follow its cell");
+        Require(!p->sys.s2, CCGPU_ERR_ARG, "a target-S(k) constraint needs the weight V(k) and the target S0(k)");
         p->sys.AddConstraint(k, V);
         p->batch.dirty = true;
         p->Invalidate();
@@ -145,6 +249,8 @@ int ccgpu_potential_set_constraints(ccgpu_potential *p, int64_t M, const double 
 {
     return Guard([&] {
         Require(p && M >= 0 && (M == 0 || (k && V)), CCGPU_ERR_ARG, "bad argument");
+        Require(!p->varbox, CCGPU_ERR_STATE, "the constraints of a variable-cell potential follow its cell");
+        Require(!p->sys.s2, CCGPU_ERR_ARG, "a target-S(k) constraint needs the weight V(k) and the target S0(k)");
         p->sys.SetConstraints(M, k, V);
         p->batch.dirty = true;
         p->Invalidate();
@@ -175,6 +281,8 @@ int ccgpu_potential_set_softcore(ccgpu_potential *p, double epsilon, double sigm
         Require(p != nullptr, CCGPU_ERR_ARG, "null handle");
         Require(std::isfinite(epsilon) && std::isfinite(sigma) && sigma >= 0, CCGPU_ERR_ARG, "bad soft-core parameters");
         Require(std::abs(epsilon) <= 1e-10 || sigma > 0, CCGPU_ERR_ARG, "soft core needs sigma > 0");
+        Require(!p->varbox || std::abs(epsilon) <= 1e-10, CCGPU_ERR_STATE,
+                "variable-cell potentials have no soft core");
         p->sys.e0 = epsilon;
         p->sys.sigma = sigma;
         p->batch.dirty = true;
@@ -191,22 +299,7 @@ int ccgpu_potential_set_configuration(ccgpu_potential *p, const double *basis, i
         for (int i = 0; i < d * d; i++) Require(std::isfinite(basis[i]), CCGPU_ERR_ARG, "non-finite basis");
         for (int64_t i = 0; i < N * d; i++)
             if (!std::isfinite(relative[i])) throw Error(CCGPU_ERR_NAN, "non-finite particle coordinate");
-        p->sys.SetCell(basis);
-        if (p->sys.N != N) {
-            p->sys.N = N;
-            p->batch.dirty = true;
-        }
-        if (p->sys.layout_dirty) p->batch.dirty = true;
-        if (p->sys.rescale != 1.0) {
-            p->sys.rescale = 1.0;
-            p->batch.dirty = true;
-        }
-        p->bound = true;
-        p->EnsureBuilt();
-        if (N > 0)
-            CC_HIP(hipMemcpyAsync(p->sys.x.get(), relative, (size_t)(N * d) * sizeof(double), hipMemcpyHostToDevice,
-                                  p->batch.stream));
-        p->Invalidate();
+        p->Bind(basis, N, relative);
     });
 }
 
@@ -350,6 +443,192 @@ int ccgpu_potential_relax(ccgpu_potential *p, double *relative, int maxstep, dou
     });
 }
 
+int ccgpu_potential_create_varbox(int dim, int form, int vk, int s0, double rK, double n, ccgpu_potential **out)
+{
+    return Guard([&] {
+        Require(out != nullptr, CCGPU_ERR_ARG, "null output handle");
+        *out = nullptr;
+        Require(dim >= 1 && dim <= kMaxDim, CCGPU_ERR_ARG, "dimension must be 1..4");
+        VarBoxSpec spec;
+        spec.form = form;
+        spec.vk = vk;
+        spec.s0 = s0;
+        spec.rK = rK;
+        spec.n = n;
+        CheckVarBoxSpec(spec, dim);
+        RequireDevice();
+        std::unique_ptr<ccgpu_potential> p(new ccgpu_potential(dim));
+        p->varbox = true;
+        p->vb = spec;
+        p->sys.s2 = form != kVarBoxStealthy;
+        *out = p.release();
+    });
+}
+
+int ccgpu_potential_get_constraints(const ccgpu_potential *p, double *k, double *V, double *S0, int32_t *n)
+{
+    return Guard([&] {
+        Require(p != nullptr, CCGPU_ERR_ARG, "null handle");
+        const System &S = p->sys;
+        const int64_t M = S.M;
+        if (k && M > 0) std::copy(S.k.begin(), S.k.end(), k);
+        if (V && M > 0) std::copy(S.v.begin(), S.v.end(), V);
+        if (S0 && M > 0) {
+            Require(S.s2 && (int64_t)S.s0.size() == M, CCGPU_ERR_STATE, "not a target-S(k) potential");
+            std::copy(S.s0.begin(), S.s0.end(), S0);
+        }
+        if (n && M > 0) {
+            Require(p->varbox && (int64_t)p->vb_n.size() == M * S.d, CCGPU_ERR_STATE,
+                    "coefficients are kept for variable-cell potentials only");
+            std::copy(p->vb_n.begin(), p->vb_n.end(), n);
+        }
+    });
+}
+
+namespace {
+// EnergyDerivativeToBasisVectors of the bound configuration: rho from the GPU, the contraction on the host
+void CellDerivative(ccgpu_potential *p, double pressure, double *grad)
+{
+    System &S = p->sys;
+    p->EnsureRho();
+    const int64_t M = S.M;
+    std::vector<double> rho(2 * std::max<int64_t>(M, 1));
+    if (M > 0)
+        CC_HIP(hipMemcpyAsync(rho.data(), S.rho.get(), 2 * M * sizeof(double), hipMemcpyDeviceToHost, p->batch.stream));
+    const double phi = p->ReadEnergy();  // syncs the stream
+    VarBoxDerivative(p->vb, S.d, S.a, S.b, S.vol, S.N, M, S.k.data(), p->vb_n.data(), S.v.data(),
+                     S.s2 ? S.s0.data() : nullptr, rho.data(), rho.data() + M, phi, pressure, grad);
+}
+}  // namespace
+
+int ccgpu_potential_energy_derivative_to_basis(ccgpu_potential *p, double pressure, double *grad)
+{
+    return Guard([&] {
+        Require(p && grad, CCGPU_ERR_ARG, "null argument");
+        Require(p->varbox, CCGPU_ERR_STATE, "EnergyDerivativeToBasisVectors is not implemented for this class");
+        Require(p->bound, CCGPU_ERR_STATE, "no configuration bound: call SetConfiguration first");
+        Require(std::isfinite(pressure), CCGPU_ERR_ARG, "non-finite pressure");
+        CellDerivative(p, pressure, grad);
+    });
+}
+
+int ccgpu_potential_relax_cell(ccgpu_potential *p, double *basis, double *relative, double pressure, int mode,
+                               double min_distance, int maxstep, double Emin, int history, double *enthalpy,
+                               int *evaluations, int *status)
+{
+    return Guard([&] {
+        Require(p && basis && enthalpy, CCGPU_ERR_ARG, "null argument");
+        Require(mode >= 0 && mode <= 2, CCGPU_ERR_ARG, "Switch must be 0 (particles), 1 (cell) or 2 (both)");
+        Require(p->bound, CCGPU_ERR_STATE, "no configuration bound: call SetConfiguration first");
+        Require(history >= 1 && history <= kPlisMaxHistory, CCGPU_ERR_ARG, "history must be 1..64");
+        Require(std::isfinite(pressure), CCGPU_ERR_ARG, "non-finite pressure");
+        Require(mode == 0 || p->varbox, CCGPU_ERR_STATE,
+                "EnergyDerivativeToBasisVectors is not implemented for this class");
+        // the neighbour check of Objective (StructureOptimization.cpp:476-488) runs only for MinDistance above
+        // LengthPrecision = 1e-4 (core/etc.h:5); that check is not provided here
+        Require(!(min_distance > 1e-4), CCGPU_ERR_ARG, "minimum-distance checks are not supported");
+        System &S = p->sys;
+        const int d = S.d;
+        const int64_t N = S.N;
+        Require(N == 0 || relative, CCGPU_ERR_ARG, "null relative coordinates");
+        const int64_t nb = mode != 0 ? d * d : 0, np = mode != 1 ? N * d : 0, n = nb + np;
+        Require(n > 0, CCGPU_ERR_ARG, "nothing to optimize");
+        std::vector<double> a(basis, basis + d * d), s(relative, relative + N * d), cell(d * d), pos(N * d);
+        int total = 0, last = 0;
+        // Objective (StructureOptimization.cpp:447-578) with GetConfig (:383-437): x = [basis (Switch 1, 2)],
+        // [relative coordinates times rescale (Switch 0, 2)]; the enthalpy E + P V, or MaxEnergy (with a zero
+        // gradient) for a cell with a basis vector longer than 2000, a volume outside [max(0.01, MinDistance^d),
+        // 1e7] (:11-14, :399-407) or below LengthPrecision^d (:466-468)
+        auto evaluate = [&](const double *x, double rescale, double *g) -> double {
+            if (!(x[0] == x[0])) throw Error(CCGPU_ERR_NAN, "NotANumberFound");
+            if (g) std::fill(g, g + n, 0.0);
+            if (mode != 0) std::copy(x, x + d * d, cell.begin());
+            else std::copy(a.begin(), a.end(), cell.begin());
+            if (mode != 0) {
+                for (int i = 0; i < d; i++) {
+                    double l2 = 0.0;
+                    for (int c = 0; c < d; c++) l2 += cell[i * d + c] * cell[i * d + c];
+                    if (l2 > 4000000.0) return kMaxEnergy;
+                }
+                const double v = std::abs(CellVolume(d, cell.data()));
+                if (!(v >= std::pow(std::max(min_distance, 0.0), d)) || v > 10000000.0 || v < 0.01) return kMaxEnergy;
+            }
+            for (int64_t i = 0; i < N * d; i++) pos[i] = mode != 1 ? x[nb + i] / rescale : s[i];
+            for (int64_t i = 0; i < N * d; i++)
+                if (!std::isfinite(pos[i])) throw Error(CCGPU_ERR_NAN, "NotANumberFound");
+            const double vol = std::abs(CellVolume(d, cell.data()));
+            if (vol < std::pow(1e-4, d)) return kMaxEnergy;
+            p->Bind(cell.data(), N, pos.data());
+            double E;
+            if (g && mode != 1) {
+                p->EnsureForces();
+                E = p->ReadEnergy();
+                for (int64_t i = 0; i < N; i++)
+                    for (int j = 0; j < d; j++) {
+                        double fa = 0.0;
+                        for (int c = 0; c < d; c++) fa += p->forces[i * d + c] * cell[j * d + c];
+                        g[nb + i * d + j] = -fa / rescale;
+                    }
+            } else {
+                p->EnsureRho();
+                E = p->ReadEnergy();
+            }
+            if (g && mode != 0) CellDerivative(p, pressure, g);
+            return E + pressure * vol;
+        };
+        std::vector<double> x(n), gx(n), dir(n), xo(n * history), go(n * history), uo(history), vo(history);
+        double H = 0.0;
+        for (int pass = 0; pass < 3; pass++) {
+            const double rescale = kRelaxRescale[pass];
+            // RelaxStructure_NLOPT_inner (:881-931): skip the pass when the energy is already below the stop value
+            p->Bind(a.data(), N, s.data());
+            p->EnsureRho();
+            if (p->ReadEnergy() < Emin) break;
+            if (mode != 0) std::copy(a.begin(), a.end(), x.begin());
+            for (int64_t i = 0; i < np; i++) x[nb + i] = s[i] * rescale;
+            std::fill(xo.begin(), xo.end(), 0.0);
+            std::fill(go.begin(), go.end(), 0.0);
+            HostPlisOps ops{(int)n, history, x.data(), gx.data(), dir.data(), xo.data(), go.data(), uo.data(), vo.data()};
+            PlisState st;
+            PlisInit(st, (int)n, history, maxstep, Emin);
+            int act = PlisAdvance(st, ops);
+            while (act == kPlisNeedEval) {
+                st.f = evaluate(x.data(), rescale, gx.data());
+                act = PlisAdvance(st, ops);
+            }
+            total += st.nevals;
+            last = PlisResult(st);
+            // GetConfig of the result (:923-928): kept unless the final cell is invalid
+            bool valid = true;
+            if (mode != 0) {
+                for (int i = 0; i < d && valid; i++) {
+                    double l2 = 0.0;
+                    for (int c = 0; c < d; c++) l2 += x[i * d + c] * x[i * d + c];
+                    valid = l2 <= 4000000.0;
+                }
+                const double v = std::abs(CellVolume(d, x.data()));
+                valid = valid && v >= std::pow(std::max(min_distance, 0.0), d) && v <= 10000000.0 && v >= 0.01;
+            }
+            if (valid) {
+                if (mode != 0) std::copy(x.begin(), x.begin() + d * d, a.begin());
+                for (int64_t i = 0; i < np; i++) s[i] = WrapUnit(x[nb + i] / rescale);
+            }
+            // RelaxStructure_NLOPT_Emin (:1074-1084): stop between passes once below Emin
+            p->Bind(a.data(), N, s.data());
+            p->EnsureRho();
+            if (Emin > -kMaxEnergy && p->ReadEnergy() < Emin) break;
+        }
+        p->Bind(a.data(), N, s.data());
+        p->EnsureRho();
+        H = p->ReadEnergy() + pressure * std::abs(CellVolume(d, a.data()));
+        std::copy(a.begin(), a.end(), basis);
+        if (N > 0 && relative) std::copy(s.begin(), s.end(), relative);
+        *enthalpy = H;
+        if (evaluations) *evaluations = total;
+        if (status) *status = last;
+    });
+}
+
 int ccgpu_autotune(int dim, int64_t N, const double *basis, int64_t M, const double *k, const double *V, int batch,
                    int repeats, int *segment_len, int *rho_kernel, int *rho_chunk, int *force_chunk, int *force_ppl,
                    double *ms)
@@ -481,9 +760,10 @@ int ccgpu_solver_destroy(ccgpu_solver *s)
     return Guard([&] { delete s; });
 }
 
-int ccgpu_solver_submit(ccgpu_solver *s, int dim, const double *basis, int64_t N, const double *relative, int64_t M,
-                        const double *k, const double *V, double shift, double epsilon, double sigma, int maxstep,
-                        double Emin, int history, int64_t *job)
+namespace {
+int SubmitSingle(ccgpu_solver *s, int dim, const double *basis, int64_t N, const double *relative, int64_t M,
+                 const double *k, const double *V, const double *S0, double shift, double epsilon, double sigma,
+                 int maxstep, double Emin, int history, int64_t *job)
 {
     return Guard([&] {
         Require(s && basis && job && N >= 0 && M >= 0 && (N == 0 || relative) && (M == 0 || (k && V)), CCGPU_ERR_ARG,
@@ -501,6 +781,11 @@ int ccgpu_solver_submit(ccgpu_solver *s, int dim, const double *basis, int64_t N
         for (int64_t i = 0; i < N * dim; i++) j->s0[i] = WrapUnit(relative[i]);
         j->k.assign(k, k + M * dim);
         j->v.assign(V, V + M);
+        if (S0) {
+            for (int64_t i = 0; i < M; i++) Require(std::isfinite(S0[i]), CCGPU_ERR_ARG, "non-finite target S0");
+            j->s2 = true;
+            j->target.assign(S0, S0 + M);
+        }
         j->shift = shift;
         j->e0 = std::abs(epsilon) > 1e-10 ? epsilon : 0.0;
         j->sigma = sigma;
@@ -513,6 +798,25 @@ int ccgpu_solver_submit(ccgpu_solver *s, int dim, const double *basis, int64_t N
         j->nufft_eps = s->solver.nufft_eps();
         *job = s->solver.Submit(std::move(j));
     });
+}
+}  // namespace
+
+int ccgpu_solver_submit(ccgpu_solver *s, int dim, const double *basis, int64_t N, const double *relative, int64_t M,
+                        const double *k, const double *V, double shift, double epsilon, double sigma, int maxstep,
+                        double Emin, int history, int64_t *job)
+{
+    return SubmitSingle(s, dim, basis, N, relative, M, k, V, nullptr, shift, epsilon, sigma, maxstep, Emin, history,
+                        job);
+}
+
+int ccgpu_solver_submit_s2(ccgpu_solver *s, int dim, const double *basis, int64_t N, const double *relative,
+                           int64_t M, const double *k, const double *V, const double *S0, double shift,
+                           double epsilon, double sigma, int maxstep, double Emin, int history, int64_t *job)
+{
+    if (M > 0 && !S0) return Guard([] { throw Error(CCGPU_ERR_ARG, "null targets"); });
+    static const double none = 0.0;
+    return SubmitSingle(s, dim, basis, N, relative, M, k, V, S0 ? S0 : &none, shift, epsilon, sigma, maxstep, Emin,
+                        history, job);
 }
 
 int ccgpu_solver_submit_multi(ccgpu_solver *s, int dim, const double *basis, int64_t N, const double *relative,

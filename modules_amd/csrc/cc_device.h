@@ -22,6 +22,7 @@ enum SysFlags : int32_t {
     kSysView = 16,     // species of a mixture: particle arrays are a range of the parent's (no prepare pass)
     kSysMixture = 32,  // mixture: no CC term of its own; energy from its species and the radius contact term
     kSysNufft = 64,    // lattice constraints evaluated by the NUFFT backend (rho and forces on a fine grid)
+    kSysS2 = 128,      // target-S(k) energy sum V (|rho|^2 / N - S0)^2 + Shift (no 1/vol), per-slot targets slot_s0
 };
 
 struct SysDesc {
@@ -46,6 +47,7 @@ struct SysDesc {
     const int32_t *seg_len; // lattice: constraints in each segment (nseg), non-increasing
     const int32_t *slot_k;  // lattice: constraint index of each slot (nslots), -1 for padding
     const double *slot_v;   // weight V of each slot (nslots); padding slots hold 0
+    const double *slot_s0;  // kSysS2: target S0 of each slot (nslots); null otherwise
     const double4 *kvec;    // general: Cartesian k (M)
 
     // rho and energy
@@ -81,6 +83,7 @@ struct SysDesc {
     double a[16];   // basis rows, d x d
     double b[16];   // reciprocal basis rows, d x d
     double vol, inv_vol, shift, rescale;
+    double inv_n;      // 1 / N (kSysS2: S(k) = |rho|^2 / N)
     double sigma, e0;  // soft core: sigma (mixture: the cell-list cutoff rc) and the energy scale
 
     NufftDesc nu;      // NUFFT backend (kSysNufft)

@@ -366,7 +366,8 @@ __global__ void __launch_bounds__(kBlock) k_rho_general(const SysDesc *__restric
 
 // ---------------------------------------------------------------------------------------------
 // rho reduce: rho_q = sum over chunks (chunk order), scatter to constraint order, energy partial
-// sum V |rho|^2 per block, force coefficients (2V/vol) (Im rho, Re rho).  items: (sys, slot block)
+// sum V |rho|^2 per block, force coefficients (2V/vol) (Im rho, Re rho) -- for a target-S(k) system the
+// partial sum V (S - S0)^2 and coefficients (4 V (S - S0) / N) (Im rho, Re rho).  items: (sys, slot block)
 // ---------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_rho_reduce(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
 {
@@ -385,12 +386,20 @@ __global__ void __launch_bounds__(kBlock) k_rho_reduce(const SysDesc *__restrict
         }
         const double v = S.slot_v[q];
         const int kidx = (S.flags & kSysLattice) ? S.slot_k[q] : q;
+        double w = (2.0 * v) * S.inv_vol;
         if (kidx >= 0) {
             S.rho[kidx] = re;
             S.rho[S.M + kidx] = im;
-            e = v * (re * re + im * im);
+            if (S.flags & kSysS2) {
+                // ShiftedCCPotential_S2::Energy / Force (potential/CollectiveCoordinatePotential.h:1686-1741):
+                // V (S - S0)^2 with S = |rho|^2 / N; force coefficient (4 / N) V (S - S0)
+                const double ds = (re * re + im * im) * S.inv_n - S.slot_s0[q];
+                e = v * ds * ds;
+                w = (4.0 * S.inv_n) * v * ds;
+            } else {
+                e = v * (re * re + im * im);
+            }
         }
-        const double w = (2.0 * v) * S.inv_vol;
         const double ca = kidx >= 0 ? w * im : 0.0, cb = kidx >= 0 ? w * re : 0.0;
         S.coef[2 * q] = ca;
         S.coef[2 * q + 1] = cb;
@@ -402,7 +411,8 @@ __global__ void __launch_bounds__(kBlock) k_rho_reduce(const SysDesc *__restrict
 }
 
 // ---------------------------------------------------------------------------------------------
-// energy: one block per system; Phi = sum(epart)/vol + Shift, plus the soft-core sum.  items: (sys)
+// energy: one block per system; Phi = sum(epart)/vol + Shift (target-S(k) systems: sum(epart) + Shift), plus the
+// soft-core sum.  items: (sys)
 // ---------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_energy_final(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
 {
@@ -421,7 +431,7 @@ __global__ void __launch_bounds__(kBlock) k_energy_final(const SysDesc *__restri
         sc = block_sum(a2, lds);
     }
     if (threadIdx.x == 0) {
-        const double cc = cc_sum / S.vol + S.shift;
+        const double cc = ((S.flags & kSysS2) ? cc_sum : cc_sum / S.vol) + S.shift;
         S.energy[0] = sc + cc;
         S.energy[1] = cc;
         S.energy[2] = sc;

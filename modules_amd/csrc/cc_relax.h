@@ -63,7 +63,9 @@ struct Job {
     int64_t id = 0;
     int d = 0;
     int64_t N = 0;
-    std::vector<double> basis, s0, k, v;
+    std::vector<double> basis, s0, k, v;  // cell, start (relative coordinates), constraints
+    bool s2 = false;                      // target-S(k) energy (ShiftedCCPotential_S2) with ...
+    std::vector<double> target;           // ... one target S0 per constraint
     double shift = 0, e0 = 0, sigma = 0, Emin = -kMaxEnergy;
     int maxstep = 10000;
     int mf = 40;

@@ -752,6 +752,7 @@ void Solver::Admit()
                 }
             } else {
                 S.SetConstraints((int64_t)j->v.size(), j->k.data(), j->v.data());
+                if (j->s2) S.SetTargets(j->target.data());
                 S.shift = j->shift;
                 if (j->Tiles_seg) S.tiles.seg = j->Tiles_seg;
                 S.precision = j->precision;

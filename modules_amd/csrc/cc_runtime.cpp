@@ -174,8 +174,9 @@ uint64_t Fnv(uint64_t h, const void *p, size_t n)
 }  // namespace
 
 std::shared_ptr<const ConstraintLayout> GetLayout(int d, const double *a, int64_t M, const double *k, const double *v,
-                                                  int S, hipStream_t stream)
+                                                  const double *s0, int S, hipStream_t stream)
 {
+    const bool has_s0 = s0 != nullptr;
     int dev = 0;
     CC_HIP(hipGetDevice(&dev));
     uint64_t h = 1469598103934665603ull;
@@ -185,6 +186,7 @@ std::shared_ptr<const ConstraintLayout> GetLayout(int d, const double *a, int64_
     h = Fnv(h, a, sizeof(double) * d * d);
     h = Fnv(h, k, sizeof(double) * M * d);
     h = Fnv(h, v, sizeof(double) * M);
+    if (has_s0) h = Fnv(h, s0, sizeof(double) * M);
     LayoutCache &C = GetLayoutCache();
     std::lock_guard<std::mutex> lk(C.mu);
     auto &bucket = C.m[h];
@@ -196,7 +198,8 @@ std::shared_ptr<const ConstraintLayout> GetLayout(int d, const double *a, int64_
         }
         if (sp->d == d && sp->S == S && sp->device == dev && (int64_t)sp->v.size() == M &&
             std::equal(a, a + d * d, sp->a.begin()) && std::equal(k, k + M * d, sp->k.begin()) &&
-            std::equal(v, v + M, sp->v.begin()))
+            std::equal(v, v + M, sp->v.begin()) && sp->has_s0 == has_s0 &&
+            (!has_s0 || std::equal(s0, s0 + M, sp->s0.begin())))
             return sp;
         ++it;
     }
@@ -207,6 +210,8 @@ std::shared_ptr<const ConstraintLayout> GetLayout(int d, const double *a, int64_
     L->a.assign(a, a + d * d);
     L->k.assign(k, k + M * d);
     L->v.assign(v, v + M);
+    L->has_s0 = has_s0;
+    if (has_s0) L->s0.assign(s0, s0 + M);
     std::vector<int32_t> n;
     L->lattice = M > 0 && LatticeIndices(d, a, M, k, n);
     if (L->lattice) {
@@ -223,6 +228,14 @@ std::shared_ptr<const ConstraintLayout> GetLayout(int d, const double *a, int64_
         CC_HIP(hipMemcpyAsync(L->seg_len.get(), L->plan.len.data(), L->plan.nseg * sizeof(int32_t), hipMemcpyHostToDevice, stream));
         CC_HIP(hipMemcpyAsync(L->slot_k.get(), L->plan.slot_k.data(), nslots * sizeof(int32_t), hipMemcpyHostToDevice, stream));
         CC_HIP(hipMemcpyAsync(L->slot_v.get(), sv.data(), nslots * sizeof(double), hipMemcpyHostToDevice, stream));
+        std::vector<double> ss;
+        if (has_s0) {
+            ss.assign(nslots, 0.0);
+            for (int64_t q = 0; q < nslots; q++)
+                if (L->plan.slot_k[q] >= 0) ss[q] = s0[L->plan.slot_k[q]];
+            L->slot_s0.Reserve(nslots);
+            CC_HIP(hipMemcpyAsync(L->slot_s0.get(), ss.data(), nslots * sizeof(double), hipMemcpyHostToDevice, stream));
+        }
         CC_HIP(hipStreamSynchronize(stream));
     } else {
         L->plan = SegmentPlan();
@@ -238,6 +251,10 @@ std::shared_ptr<const ConstraintLayout> GetLayout(int d, const double *a, int64_
         if (M > 0) {
             CC_HIP(hipMemcpyAsync(L->kvec.get(), kv.data(), M * sizeof(double4), hipMemcpyHostToDevice, stream));
             CC_HIP(hipMemcpyAsync(L->slot_v.get(), v, M * sizeof(double), hipMemcpyHostToDevice, stream));
+            if (has_s0) {
+                L->slot_s0.Reserve(M);
+                CC_HIP(hipMemcpyAsync(L->slot_s0.get(), s0, M * sizeof(double), hipMemcpyHostToDevice, stream));
+            }
             CC_HIP(hipStreamSynchronize(stream));
         }
     }
@@ -269,6 +286,23 @@ void System::SetConstraints(int64_t m, const double *kk, const double *vv)
     v.assign(vv, vv + m);
     M = m;
     layout_dirty = true;
+}
+
+void System::SetTargets(const double *t)
+{
+    for (int64_t i = 0; i < M; i++)
+        if (!std::isfinite(t[i])) throw Error(kErrArg, "non-finite target S0");
+    s0.assign(t, t + M);
+    s2 = true;
+    layout_dirty = true;
+}
+
+void System::AddConstraintS2(const double *kk, double V, double S0)
+{
+    if (!std::isfinite(S0)) throw Error(kErrArg, "non-finite target S0");
+    if (!s2 || (int64_t)s0.size() != M) throw Error(kErrState, "target-S(k) constraints need a target-S(k) potential");
+    AddConstraint(kk, V);
+    s0.push_back(S0);
 }
 
 void System::AddConstraint(const double *kk, double V)
@@ -306,7 +340,8 @@ void System::Prepare(hipStream_t stream)
     if (used.force_ppl != 1 && used.force_ppl != 2) throw Error(kErrArg, "force particles per lane must be 1 or 2");
 
     if (layout_dirty || !layout || layout->S != used.seg) {
-        layout = GetLayout(d, a, M, k.data(), v.data(), used.seg, stream);
+        if (s2 && (int64_t)s0.size() != M) throw Error(kErrState, "target-S(k) potential: one S0 per constraint");
+        layout = GetLayout(d, a, M, k.data(), v.data(), s2 ? s0.data() : nullptr, used.seg, stream);
         lattice = layout->lattice;
         layout_dirty = false;
     }
@@ -396,7 +431,7 @@ void System::Prepare(hipStream_t stream)
     D.M = (int32_t)M;
     D.flags = kSysActive | (lattice ? kSysLattice : 0) | (softcore() ? kSysSoftCore : 0) |
               (lattice && precision == 1 && !nufft_on() ? kSysMixed : 0) | (view() ? kSysView : 0) |
-              (mixture ? kSysMixture : 0) | (nufft_on() ? kSysNufft : 0);
+              (mixture ? kSysMixture : 0) | (nufft_on() ? kSysNufft : 0) | (s2 ? kSysS2 : 0);
     D.nseg = (int32_t)nseg();
     D.nslots = (int32_t)nslots;
     D.pchunks = N > 0 ? pchunks : 0;  // no particles: rho sums nothing
@@ -415,6 +450,7 @@ void System::Prepare(hipStream_t stream)
     D.seg_len = layout->seg_len.get();
     D.slot_k = layout->slot_k.get();
     D.slot_v = layout->slot_v.get();
+    D.slot_s0 = s2 ? layout->slot_s0.get() : nullptr;
     D.kvec = layout->kvec.get();
     D.rpart = rpart.get();
     D.rho = rho.get();
@@ -471,6 +507,7 @@ void System::Prepare(hipStream_t stream)
     }
     D.vol = vol;
     D.inv_vol = 1.0 / vol;
+    D.inv_n = N > 0 ? 1.0 / (double)N : 0.0;
     D.shift = shift;
     D.rescale = rescale;
     D.sigma = sigma;

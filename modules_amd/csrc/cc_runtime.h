@@ -102,21 +102,24 @@ double Autotune(int d, int64_t N, const double *basis, int64_t M, const double *
 // and stores its constraint tables once.
 struct ConstraintLayout {
     int d = 0, S = 0, device = 0;
-    std::vector<double> a, k, v;  // the key: cell basis, wavevectors, weights
+    std::vector<double> a, k, v;  // the key: cell basis, wavevectors, weights ...
+    bool has_s0 = false;
+    std::vector<double> s0;       // ... and the targets S0 of a target-S(k) set
     bool lattice = false;
     SegmentPlan plan;
     DevArray<int4> seg_head;
     DevArray<int32_t> seg_len, slot_k;
-    DevArray<double> slot_v;
+    DevArray<double> slot_v, slot_s0;
     DevArray<double4> kvec;
     int64_t nslots() const { return lattice ? plan.nseg * S : (int64_t)v.size(); }
     size_t DeviceBytes() const
     {
-        return seg_head.bytes() + seg_len.bytes() + slot_k.bytes() + slot_v.bytes() + kvec.bytes();
+        return seg_head.bytes() + seg_len.bytes() + slot_k.bytes() + slot_v.bytes() + slot_s0.bytes() + kvec.bytes();
     }
 };
+// s0: per-constraint targets of a target-S(k) set, or null
 std::shared_ptr<const ConstraintLayout> GetLayout(int d, const double *a, int64_t M, const double *k, const double *v,
-                                                  int S, hipStream_t stream);
+                                                  const double *s0, int S, hipStream_t stream);
 
 // NUFFT plan of one system (cc_nufft.h): grid shape, kernel, deconvolution tables, device buffers and the two
 // hipFFT plans (Type 1 exp(+i) on one grid; Type 2 exp(-i) on one grid, two in 3D).
@@ -159,6 +162,10 @@ public:
     bool has_cell = false;
     double a[16] = {}, b[16] = {}, vol = 0;
     std::vector<double> k, v;  // constraints (host copy): k M x d, V M
+    // target-S(k) form (ShiftedCCPotential_S2, potential/CollectiveCoordinatePotential.h:1642-1748): energy
+    // sum V (|rho|^2 / N - S0)^2 + Shift with one target S0 per constraint
+    bool s2 = false;
+    std::vector<double> s0;
     double shift = 0, e0 = 0, sigma = 0, rescale = 1;
     Tiles tiles{0, 0, 0, -1, 0};  // requested (0 / -1 = automatic)
     Tiles used;                 // resolved
@@ -208,6 +215,8 @@ public:
     void SetCell(const double *basis);
     void SetConstraints(int64_t M, const double *k, const double *V);
     void AddConstraint(const double *k, double V);
+    void SetTargets(const double *S0);  // target-S(k) form: one S0 per current constraint
+    void AddConstraintS2(const double *k, double V, double S0);
     // (re)build the constraint layout and size every buffer for the current N, M, tiles; fills desc
     void Prepare(hipStream_t stream);
     bool softcore() const { return std::abs(e0) > 1e-10 && sigma > 0; }

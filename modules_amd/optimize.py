@@ -45,17 +45,37 @@ def LuksanPLIS(fun, x0, stopval=-np.inf, maxeval=0, mf=40):
 
 
 def _check_switch(Pressure, Switch, MinDistance):
-    if Switch != 0 or Pressure != 0 or MinDistance > 1e-4:
-        raise NotImplementedError("only Switch 0 (particles move), Pressure 0 and MinDistance 0 are on this path")
+    if Switch not in (0, 1, 2):
+        raise ValueError("Switch must be 0 (particles), 1 (cell) or 2 (both)")
+    if MinDistance > 1e-4:
+        raise NotImplementedError("the MinDistance neighbour check (MinDistance > 1e-4) is not on this path")
 
 
 def RelaxStructure_NLOPT_Emin(List: Configuration, pot: ShiftedCCPotential, Pressure=0.0, Switch=0, MinDistance=0.0,
                               MaxStep=10000, Emin=-MaxEnergy, history=None):
     """Relax List in place with three L-BFGS passes (rescale 10000, 1, 0.01), stopping once E < Emin.
-    The L-BFGS runs on the host, each objective evaluation is one GPU E+F pass.  Returns (E, evaluations, status)."""
+    The L-BFGS runs on the host, each objective evaluation is one GPU E+F pass.  Returns (E, evaluations, status);
+    with a Pressure or a moving cell (Switch 1: cell, 2: both; variable-cell potentials) the objective is the
+    enthalpy E + P V and the first value returned is the final enthalpy."""
     _check_switch(Pressure, Switch, MinDistance)
     history = history or StructureOptimization_LBFGSHistory
     pot.SetConfiguration(List)
+    if Switch != 0 or Pressure != 0:
+        basis = np.ascontiguousarray(List.basis, dtype=np.float64).copy()
+        rel = np.ascontiguousarray(List.relative, dtype=np.float64).copy()
+        H = C.c_double(0)
+        ev = C.c_int(0)
+        st = C.c_int(0)
+        check(lib().ccgpu_potential_relax_cell(pot._h, _p(basis), _p(rel), float(Pressure), int(Switch),
+                                               float(MinDistance), int(MaxStep), float(Emin), int(history),
+                                               C.byref(H), C.byref(ev), C.byref(st)))
+        List.basis = basis.reshape(List.basis.shape)
+        List.relative = rel.reshape(List.relative.shape)
+        pot.RhoValid = False
+        if hasattr(pot, "_pull"):
+            pot._config = List
+            pot._pull()
+        return H.value, ev.value, st.value
     rel = np.ascontiguousarray(List.relative, dtype=np.float64).copy()
     E = C.c_double(0)
     ev = C.c_int(0)
@@ -142,6 +162,12 @@ def CollectiveCoordinateMultiRun(pConfig: Configuration, pPotential, gen, Prefix
     solver = BatchSolver(max_slots=max(1, min(max_slots, SampleNumber)))
     if multi:
         ids = [solver.submit_multi(c, pPotential, MaxStep=max_num) for c in starts]
+    elif getattr(pPotential, "_S2", False):
+        if hasattr(pPotential, "_pull"):
+            raise NotImplementedError("CollectiveCoordinateMultiRun: variable-cell potentials relax through "
+                                      "RelaxStructure_NLOPT")
+        S0 = np.asarray(pPotential.S0, dtype=np.float64)
+        ids = [solver.submit_s2(c, k, V, S0, shift=shift, epsilon=e0, sigma=sigma, MaxStep=max_num) for c in starts]
     else:
         ids = [solver.submit(c, k, V, shift=shift, epsilon=e0, sigma=sigma, MaxStep=max_num) for c in starts]
     nxt = 0
@@ -237,6 +263,25 @@ class BatchSolver:
         check(lib().ccgpu_solver_submit(self._h, d, _p(np.ascontiguousarray(config.basis)), rel.shape[0], _p(rel),
                                         k.shape[0], _p(k), _p(V), shift, epsilon, sigma, int(MaxStep), float(Emin),
                                         int(history), C.byref(job)))
+        self._shapes[job.value] = rel.shape
+        return job.value
+
+    def submit_s2(self, config: Configuration, k, V, S0, shift=0.0, epsilon=0.0, sigma=0.0, MaxStep=10000,
+                  Emin=-MaxEnergy, history=StructureOptimization_LBFGSHistory):
+        """A relax request with the target-S(k) potential (ShiftedCCPotential_S2; soft core as
+        RepulsiveCCPotential with S0 != 0): one target S0 per constraint."""
+        d = config.GetDimension()
+        rel = np.ascontiguousarray(config.relative, dtype=np.float64)
+        k = np.ascontiguousarray(k, dtype=np.float64).reshape(-1, d)
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        S0 = np.ascontiguousarray(S0, dtype=np.float64)
+        if S0.shape != V.shape:
+            raise ValueError("one target S0 per constraint")
+        job = C.c_int64(0)
+        check(lib().ccgpu_solver_submit_s2(self._h, d, _p(np.ascontiguousarray(config.basis)), rel.shape[0],
+                                           _p(rel), k.shape[0], _p(k), _p(V), _p(S0 if len(S0) else np.zeros(1)),
+                                           shift, epsilon, sigma, int(MaxStep), float(Emin), int(history),
+                                           C.byref(job)))
         self._shapes[job.value] = rel.shape
         return job.value
 

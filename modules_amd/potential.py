@@ -189,22 +189,29 @@ class KPointConstraint:
 class ShiftedCCPotential:
     """Phi = (1/V) sum_k V_k |rho(k)|^2 + Shift on the GPU (CollectiveCoordinatePotential.h:53-1497)."""
 
+    _S2 = False  # target-S(k) form (ShiftedCCPotential_S2)
+
     def __init__(self, dimension):
         self.Dimension = int(dimension)
         self.constraints = []
+        self.S0 = []
         self.Shift = 0.0
         self.ParallelNumThread = 1  # kept for interface parity; the GPU does not use it
         self.RhoReal = np.zeros(0)
         self.RhoImag = np.zeros(0)
         self.RhoValid = False
         self._h = C.c_void_p()
-        check(lib().ccgpu_potential_create(self.Dimension, C.byref(self._h)))
+        self._create()
         self._pushed_k = None
         self._pushed_v = None
         self._pushed_shift = None
         self._softcore = (0.0, 0.0)
         self._pushed_softcore = None
         self._config = None
+
+    def _create(self):
+        create = lib().ccgpu_potential_create_s2 if self._S2 else lib().ccgpu_potential_create
+        check(create(self.Dimension, C.byref(self._h)))
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -214,11 +221,28 @@ class ShiftedCCPotential:
 
     # --- constraints -------------------------------------------------------------------------
     def AddConstraint(self, k, vals):
-        """AddConstraint(k, vals): vals[0] is the weight; empty vals raises ValueError (std::invalid_argument)."""
-        if len(vals) == 0:
+        """AddConstraint(k, vals): vals[0] is the weight; empty vals raises ValueError (std::invalid_argument).
+        The target-S(k) form takes vals = [V(k), S0(k)] (CollectiveCoordinatePotential.h:1663-1668)."""
+        if self._S2:
+            if len(vals) < 2:
+                raise ValueError("ShiftedCCPotential_S2::AddConstraint: vals must contain the weight V(k) and the "
+                                 "target S0(k).")
+            self.S0.append(float(vals[1]))
+        elif len(vals) == 0:
             raise ValueError("ShiftedCCPotential::AddConstraint: vals must contain at least the weight V(k).")
         self.constraints.append(KPointConstraint(k, vals[0]))
         self.RhoValid = False
+
+    def PrintConstraints(self, ofile=None):
+        """PrintConstraints (CollectiveCoordinatePotential.h:1340-1345, 1672-1677)."""
+        import sys
+        out = ofile or sys.stdout
+        for i, c in enumerate(self.constraints):
+            ks = " ".join("%g" % x for x in c.k[: self.Dimension])
+            if self._S2:
+                out.write("@ k= %s: v=%g, S0(k)=%g\n" % (ks, c.V, self.S0[i]))
+            else:
+                out.write("@ k= %s: v=%g\n" % (ks, c.V))
 
     def _sync(self):
         d = self.Dimension
@@ -228,8 +252,17 @@ class ShiftedCCPotential:
         for i, c in enumerate(self.constraints):
             k[i] = c.k[:d]
             v[i] = c.V
+        if self._S2:
+            if len(self.S0) != M:
+                raise ValueError("ShiftedCCPotential_S2: one target S0 per constraint")
+            v = np.concatenate([v, np.asarray(self.S0, dtype=np.float64)])  # weights then targets
         if self._pushed_k is None or not (np.array_equal(k, self._pushed_k) and np.array_equal(v, self._pushed_v)):
-            check(lib().ccgpu_potential_set_constraints(self._h, M, _p(np.ascontiguousarray(k)), _p(v)))
+            if self._S2:
+                check(lib().ccgpu_potential_set_constraints_s2(self._h, M, _p(np.ascontiguousarray(k)),
+                                                               _p(np.ascontiguousarray(v[:M])),
+                                                               _p(np.ascontiguousarray(v[M:]))))
+            else:
+                check(lib().ccgpu_potential_set_constraints(self._h, M, _p(np.ascontiguousarray(k)), _p(v)))
             self._pushed_k, self._pushed_v = k, v
             self.RhoValid = False
         if self._pushed_shift != self.Shift:
@@ -364,13 +397,27 @@ class ShiftedCCPotential:
         return bool(v.value)
 
 
+class ShiftedCCPotential_S2(ShiftedCCPotential):
+    """Phi = sum_k V_k (|rho(k)|^2 / N - S0_k)^2 + Shift on the GPU, forces
+    -(4/N) sum_k V_k (S_k - S0_k) (Im rho cos(k.r) - Re rho sin(k.r)) k
+    (ShiftedCCPotential_S2, CollectiveCoordinatePotential.h:1642-1748).  Constraints take vals = [V(k), S0(k)];
+    the targets are in .S0, one per constraint."""
+
+    _S2 = True
+
+    @property
+    def N(self):
+        return 0 if self._config is None else self._config.NumParticle()
+
+
 class RepulsiveCCPotential(ShiftedCCPotential):
     """Soft core e0 (1 - r/sigma)^2 for r < sigma plus the CC potential
     (apps/CCOptimization/RepulsiveCCPotential.h:21-157).  Constraints go to .CCPotential, as in the reference."""
 
     def __init__(self, dim, epsilon, sigma, S0=0.0):
-        if S0 != 0.0:
-            raise NotImplementedError("RepulsiveCCPotential: the S2 kernel (S0 != 0) is not on this path")
+        # S0 selects the kernel (RepulsiveCCPotential.h:40-46): nonzero makes the CC part ShiftedCCPotential_S2,
+        # whose constraints then take vals = [V(k), S0(k)]
+        self._S2 = S0 != 0.0
         super().__init__(dim)
         self.e0 = float(epsilon)
         self.sigma = float(sigma)

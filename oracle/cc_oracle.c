@@ -12,6 +12,7 @@
  *   - GetKs (d>=2 lattice walk, half-space rule, sort; d=1) core/PeriodicCellList.cpp:509-735
  *   - getK                                                  apps/CCOptimization/main.cpp:2018-2021
  *   - ShiftedCCPotential::GetRho_Direct / Energy / Force    potential/CollectiveCoordinatePotential.h:1255-1394
+ *   - ShiftedCCPotential_S2::Energy / Force (target S(k))   potential/CollectiveCoordinatePotential.h:1686-1741
  *   - RepulsiveCCPotential soft core energy / forces         apps/CCOptimization/RepulsiveCCPotential.h:61-145
  *     (brute force over periodic images instead of the cell-list walk; same pair terms)
  *   - MultiStealthyPotential energy / forces (species CC      apps/CCOptimization/MultiStealthyPotential.cpp:682-833
@@ -342,6 +343,42 @@ void orc_forces(int d, int64_t N, const double *s, const double *a, int64_t M, c
 		}
 		double inv = 1.0 / volume;
 		for (int q = 0; q < d; q++) F[j * d + q] = res[q] * inv;
+	}
+}
+
+/* ShiftedCCPotential_S2::Energy (CollectiveCoordinatePotential.h:1686-1712) with one thread:
+ * sum_k V_k (S_k - S0_k)^2 + Shift, S_k = (Re^2 + Im^2) / N (no 1/Volume). */
+double orc_s2_energy(int64_t M, const double *v, const double *s0, const double *re, const double *im, int64_t N,
+                     double shift)
+{
+	double result = 0.0;
+	for (int64_t i = 0; i < M; i++) {
+		double s = (re[i] * re[i] + im[i] * im[i]) / (double)N;
+		double ds = s - s0[i];
+		result += v[i] * ds * ds;
+	}
+	return result + shift;
+}
+
+/* ShiftedCCPotential_S2::Force (CollectiveCoordinatePotential.h:1718-1741):
+ * F_j = - sum_k (4/N) V_k (S_k - S0_k) (Im cos(k.r) - Re sin(k.r)) k, parallel over particles. */
+void orc_s2_forces(int d, int64_t N, const double *s, const double *a, int64_t M, const double *k, const double *v,
+                   const double *s0, const double *re, const double *im, double *F, int nthreads)
+{
+	if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+	for (int64_t j = 0; j < N; j++) {
+		double r4[ORC_MAXDIM], res[ORC_MAXDIM] = {0, 0, 0, 0};
+		orc_cart(d, a, s + j * d, r4);
+		for (int64_t i = 0; i < M; i++) {
+			double k4[ORC_MAXDIM];
+			orc_pad(d, k + i * d, k4);
+			double sk = (re[i] * re[i] + im[i] * im[i]) / (double)N;
+			double ds = sk - s0[i];
+			double c = (4.0 / (double)N) * v[i] * ds * (im[i] * cos(gv_dot(k4, r4)) - re[i] * sin(gv_dot(k4, r4)));
+			for (int q = 0; q < ORC_MAXDIM; q++) res[q] -= c * k4[q];
+		}
+		for (int q = 0; q < d; q++) F[j * d + q] = res[q];
 	}
 }
 

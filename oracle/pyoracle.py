@@ -62,6 +62,9 @@ def lib():
         L.orc_multi_energy_forces.argtypes = [C.c_int, C.c_int64, _dp, _dp, C.c_int, _i64p, _i64p, _i64p, _dp, _dp,
                                               _dp, C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp, C.c_int]
         L.orc_multi_energy_forces.restype = C.c_double
+        L.orc_s2_energy.argtypes = [C.c_int64, _dp, _dp, _dp, _dp, C.c_int64, C.c_double]
+        L.orc_s2_energy.restype = C.c_double
+        L.orc_s2_forces.argtypes = [C.c_int, C.c_int64, _dp, _dp, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int]
         _lib = L
     return _lib
 
@@ -164,6 +167,25 @@ def forces(s, a, k, v, re, im, nthreads=0):
     kk = k if M else np.zeros((1, d))
     lib().orc_forces(d, N, _p(s), _p(a), M, _p(kk), _p(v if M else np.zeros(1)), _p(re if M else np.zeros(1)),
                      _p(im if M else np.zeros(1)), volume(a), _p(F), nthreads or lib().orc_max_threads())
+    return F
+
+
+def s2_energy(v, s0, re, im, N, shift=0.0):
+    """ShiftedCCPotential_S2::Energy: sum V (|rho|^2 / N - S0)^2 + Shift."""
+    v, s0, re, im = _c(v), _c(s0), _c(re), _c(im)
+    return lib().orc_s2_energy(len(v), _p(v), _p(s0), _p(re), _p(im), int(N), shift)
+
+
+def s2_forces(s, a, k, v, s0, re, im, nthreads=0):
+    """ShiftedCCPotential_S2::Force for every particle."""
+    s, a, k, v, s0, re, im = _c(s), _c(a), _c(k), _c(v), _c(s0), _c(re), _c(im)
+    N, d = s.shape
+    M = k.shape[0]
+    F = np.zeros((N, d))
+    if N == 0 or M == 0:
+        return F
+    lib().orc_s2_forces(d, N, _p(s), _p(a), M, _p(k), _p(v), _p(s0), _p(re), _p(im), _p(F),
+                        nthreads or lib().orc_max_threads())
     return F
 
 
