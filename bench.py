@@ -58,13 +58,15 @@ def parse():
                     help="batched / mixed256: FP64, or FP32 phases and force terms with FP64 sums and an FP64 "
                          "polish below 1e-8")
     ap.add_argument("--workload", default="batched",
-                    choices=["batched", "single", "mixed256", "large2d", "mixed2d", "serve", "five", "nufft2d"],
+                    choices=["batched", "single", "mixed256", "large2d", "mixed2d", "serve", "five", "nufft2d",
+                             "targets"],
                     help="batched: the headline (64 seeds, 3D N=2000 chi=0.45); single: the same solves through the "
                          "single-solve drop-in path; mixed256: 256 configurations of mixed N, d, chi; large2d: 2D "
                          "N=20000 chi=0.4; mixed2d: the same solved in FP64 and in mixed precision, with the "
                          "mixed-precision error; serve: Poisson request stream with soft core; five: multi-species "
                          "ground states of the reference's five-component study (2D N=2000, 5 x 400); nufft2d: "
-                         "large 2D systems through the NUFFT backend against the direct sums")
+                         "large 2D systems through the NUFFT backend against the direct sums; targets: target-S(k) "
+                         "(equiluminous) relaxes and a variable-cell solve")
     ap.add_argument("--requests", type=int, default=48, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=20.0, help="serve: arrivals per second")
     ap.add_argument("--profiler-run", action="store_true",
@@ -532,9 +534,72 @@ def workload_nufft2d(args):
     return out
 
 
+def workload_targets(args):
+    """Target-S(k) and variable-cell rows.  (1) Equiluminous patterns (RepulsiveCCPotential with S0 != 0,
+    apps/CCOptimization/main.cpp:302-318): 2D N=2000, every constraint with 0 < |k| <= K(chi=0.3) weighted 1 with
+    the target S(k) = 0.5, --batch random starts relaxed side by side (three passes, at most 2000 evaluations each);
+    the oracle's S2 E+F (rho and forces restated in C) timed on the host threads beside it.  (2) A variable-cell
+    stealthy solve (K2Potential, CCPotential_Kernels.h:191): 3D N=2000 with rK = 1 at the density giving M ~ 0.3 d N
+    constraints, RelaxStructure_NLOPT with Switch 2 (cell and particles) at P = 1e-4, each evaluation regenerating
+    the constraints for the new cell; milliseconds per evaluation."""
+    from modules_amd import Configuration, GetKs, RandomConfiguration, getK
+    from modules_amd import varbox as VB
+    from modules_amd.optimize import BatchSolver, RelaxStructure_NLOPT
+    from oracle import pyoracle as po
+    out = {"metric": "equiluminous (target-S(k)) relaxes/s", "unit": "relaxes/s", "dtype": "f64"}
+    B, N, d = args.batch, 2000, 2
+    cfgs = [RandomConfiguration(100 + i, N, d) for i in range(B)]
+    k = GetKs(cfgs[0], getK(0.3, 1.0, d))
+    v = np.ones(len(k))
+    s0 = np.full(len(k), 0.5)
+    solver = BatchSolver(max_slots=max(1, B), check_interval=args.check_interval, lanes=args.lanes)
+    jobs = [solver.submit_s2(c, k, v, s0, MaxStep=2000) for c in cfgs]
+    res, wall = _solve_all(solver, jobs, -1e10)
+    E = np.array([r["energy"] for r in res])
+    evals = np.array([r["device_evaluations"] for r in res])
+    out["value"] = B / wall
+    out["s2"] = {"M": len(k), "solves": B, "wall_s": wall, "energy_max": float(E.max()),
+                 "energy_median": float(np.median(E)), "device_evals_mean": float(evals.mean()),
+                 "ef_evals_per_s": float(evals.sum() / wall)}
+    # oracle S2 E+F on the host threads, bounded sample
+    threads = min(16, os.cpu_count() or 1)
+    c0 = cfgs[0]
+    t = []
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < 8.0 and len(t) < 500:
+        t1 = time.perf_counter()
+        re, im = po.rho(c0.relative, c0.basis, k, nthreads=threads)
+        po.s2_energy(v, s0, re, im, N)
+        po.s2_forces(c0.relative, c0.basis, k, v, s0, re, im, nthreads=threads)
+        t.append(time.perf_counter() - t1)
+    ef = len(t) / sum(t)
+    out["cpu_baseline"] = {"value": ef / max(float(evals.mean()), 1.0), "unit": "relaxes/s", "cores": threads,
+                           "kind": "port", "sample": f"{len(t)} oracle S2 E+F evaluations of one N={N} d={d} "
+                           f"configuration on {threads} threads ({ef:.3g} E+F/s) divided by "
+                           f"{evals.mean():.0f} evaluations per relax"}
+    # variable-cell stealthy solve, Switch 2
+    d3, N3 = 3, 2000
+    L = (0.3 * d3 * N3 * 12 * math.pi ** 2) ** (1.0 / 3)  # M ~ V / (12 pi^2) points with |k| < 1 (one of +-k)
+    c3 = Configuration(np.diag([L, L, L]), RandomConfiguration(7, N3, d3).relative)
+    pot = VB.K2Potential(d3, 1.0)
+    pot.SetConfiguration(c3)
+    M3 = len(pot.constraints)
+    H0 = pot.Energy() + 1e-4 * c3.PeriodicVolume()
+    t0 = time.perf_counter()
+    H, ev, st = RelaxStructure_NLOPT(c3, pot, 1e-4, 2, 0.0, 400)
+    wall3 = time.perf_counter() - t0
+    out["varbox"] = {"workload": f"K2Potential 3D N={N3}, rK=1, start cell {L:.2f}^3 (M={M3}), P=1e-4, Switch 2",
+                     "H0": H0, "H": H, "evaluations": ev, "status": st, "wall_s": wall3,
+                     "ms_per_eval": 1000 * wall3 / max(ev, 1), "final_M": len(pot.constraints),
+                     "final_volume": c3.PeriodicVolume()}
+    out["workload"] = (f"equiluminous S(k)=0.5 relaxes, 2D N={N} chi=0.3 (M={len(k)}), {B} starts; variable-cell "
+                       f"3D N={N3} K2Potential Switch 2")
+    return out
+
+
 OTHER_WORKLOADS = {"single": workload_single, "mixed256": workload_mixed256, "large2d": workload_large2d,
                    "mixed2d": workload_mixed2d, "serve": workload_serve, "five": workload_five,
-                   "nufft2d": workload_nufft2d}
+                   "nufft2d": workload_nufft2d, "targets": workload_targets}
 
 
 def _cpu_model():

@@ -312,6 +312,11 @@ int ccgpu_potential_relax(ccgpu_potential *p, double *relative, int maxstep, dou
  * KM4KnPotential = (1, 1..3, 2), KM2Klc / KM3Klc / KM4KlcPotential = (1, 1..3, 3) (:342-358).
  * Constraint edits (add/set_constraints[_s2]) return CCGPU_ERR_STATE and the soft core is not available. */
 int ccgpu_potential_create_varbox(int dim, int form, int vk, int s0, double rK, double n, ccgpu_potential **out);
+/* The k-point list a variable-cell potential generates for a cell (host only, no device needed): the reciprocal
+ * lattice points 0 < |k| < rK, one of each +-k pair, ordered by |k| then coefficients; writes up to cap points
+ * (k cap x d, n cap x d integer coefficients on the reciprocal basis) and *M = the full count
+ * (ShiftedCCPotential_varBoxable::SetConfiguration, CCPotential_varBox.h:129-208). */
+int ccgpu_varbox_kpoints(int d, const double *basis, double rK, int64_t cap, double *k, int32_t *n, int64_t *M);
 /* the current constraints: k M x dim, V M, S0 M (target-S(k) forms), n M x dim integer coefficients on the
  * reciprocal basis (variable-cell potentials); any pointer may be null */
 int ccgpu_potential_get_constraints(const ccgpu_potential *p, double *k, double *V, double *S0, int32_t *n);

@@ -71,6 +71,7 @@ SIGNATURES = {
     "ccgpu_potential_create_varbox": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                                 C.POINTER(vp)]),
     "ccgpu_potential_get_constraints": (C.c_int, [vp, dp, dp, dp, ip32]),
+    "ccgpu_varbox_kpoints": (C.c_int, [C.c_int, dp, C.c_double, C.c_int64, dp, ip32, i64p]),
     "ccgpu_potential_energy_derivative_to_basis": (C.c_int, [vp, C.c_double, dp]),
     "ccgpu_potential_relax_cell": (C.c_int, [vp, dp, dp, C.c_double, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int,
                                              dp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),

@@ -155,6 +155,27 @@ int ccgpu_get_ks(int d, const double *basis, double K, int64_t cap, double *k, i
     });
 }
 
+int ccgpu_varbox_kpoints(int d, const double *basis, double rK, int64_t cap, double *k, int32_t *n, int64_t *M)
+{
+    return Guard([&] {
+        Require(d >= 1 && d <= kMaxDim, CCGPU_ERR_ARG, "dimension must be 1..4");
+        Require(basis != nullptr && M != nullptr, CCGPU_ERR_ARG, "null argument");
+        Require(rK > 0.0 && std::isfinite(rK), CCGPU_ERR_ARG, "k-point cutoff must be positive");
+        for (int i = 0; i < d * d; i++) Require(std::isfinite(basis[i]), CCGPU_ERR_ARG, "non-finite basis");
+        Require(CellVolume(d, basis) > 0.0, CCGPU_ERR_ARG, "cell basis is singular");
+        double b[16];
+        ReciprocalBasis(d, basis, b);
+        std::vector<double> kk;
+        std::vector<int32_t> nn;
+        VarBoxKPoints(d, b, basis, rK, kk, nn);
+        const int64_t m = (int64_t)(kk.size() / d);
+        *M = m;
+        const int64_t w = std::min<int64_t>(m, std::max<int64_t>(cap, 0));
+        if (k && w > 0) std::memcpy(k, kk.data(), (size_t)(w * d) * sizeof(double));
+        if (n && w > 0) std::memcpy(n, nn.data(), (size_t)(w * d) * sizeof(int32_t));
+    });
+}
+
 int ccgpu_reciprocal_basis(int d, const double *basis, double *b_out)
 {
     return Guard([&] {

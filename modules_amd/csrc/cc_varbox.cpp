@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <numeric>
 
 #include "cc_lattice.h"
@@ -158,22 +159,32 @@ void GradTargetS0(const VarBoxSpec &spec, int d, const double *k, double *g)
 
 // ShiftedCCPotential_varBoxable::SetConfiguration (potential/CCPotential_varBox.h:129-208): the reciprocal
 // lattice points strictly within rK of the origin, one of each +-k pair (the first non-zero coefficient
-// positive).  Written as an enumeration of the coefficient box |n_i| <= rK |a_i| / (2 pi) (since
-// n_i = k.a_i / (2 pi)), sorted by |k|^2 and then n.
+// positive), sorted by |k|^2 and then n.  The coefficients are enumerated level by level inside the ellipsoid
+// n^T G n < rK^2 (G the Gram matrix of the reciprocal basis, in its triangular form
+// q(n) = sum_i q_ii (n_i + sum_{j>i} q_ij n_j)^2, Fincke and Pohst, Math. Comp. 44 (1985) 463), each range widened
+// by one integer on either side, and every candidate is then tested exactly on k = sum_i n_i b_i: the work follows
+// the number of points, not a bounding box (which a sheared cell can make far larger).
 void VarBoxKPoints(int d, const double *b, const double *a, double rK, std::vector<double> &k,
                    std::vector<int32_t> &n)
 {
+    (void)a;
     k.clear();
     n.clear();
-    int64_t lim[kMaxDim] = {0, 0, 0, 0};
-    int64_t box = 1;
+    double Q[kMaxDim][kMaxDim];
+    for (int i = 0; i < d; i++)
+        for (int j = 0; j < d; j++) {
+            double g = 0.0;
+            for (int c = 0; c < d; c++) g += b[i * d + c] * b[j * d + c];
+            Q[i][j] = g;
+        }
     for (int i = 0; i < d; i++) {
-        const double al = std::sqrt(Norm2(d, a + i * d));
-        const double l = std::floor(rK * al / (2.0 * kPi)) + 1.0;
-        if (!(l < 1 << 20)) throw Error(kErrArg, "k-point cutoff too large for the cell");
-        lim[i] = (int64_t)l;
-        box *= 2 * lim[i] + 1;
-        if (box > ((int64_t)1 << 32)) throw Error(kErrArg, "k-point cutoff too large for the cell");
+        if (!(Q[i][i] > 0.0)) throw Error(kErrArg, "cell basis is singular");
+        for (int j = i + 1; j < d; j++) {
+            Q[j][i] = Q[i][j];
+            Q[i][j] = Q[i][j] / Q[i][i];
+        }
+        for (int r = i + 1; r < d; r++)
+            for (int c = r; c < d; c++) Q[r][c] -= Q[r][i] * Q[i][c];
     }
     const double rK2 = rK * rK;
     struct KP {
@@ -182,29 +193,44 @@ void VarBoxKPoints(int d, const double *b, const double *a, double rK, std::vect
         double k[kMaxDim];
     };
     std::vector<KP> out;
-    int64_t c[kMaxDim];
-    for (int i = 0; i < d; i++) c[i] = -lim[i];
-    for (int64_t t = 0; t < box; t++) {
+    int64_t c[kMaxDim] = {0, 0, 0, 0};
+    int64_t visited = 0;
+    const double slack = 1e-9 * rK2;
+    auto leaf = [&]() {
         int first = 0;  // sign of the first non-zero coefficient
         for (int i = 0; i < d && first == 0; i++) first = c[i] > 0 ? 1 : c[i] < 0 ? -1 : 0;
-        if (first > 0) {
-            KP p{};
-            for (int q = 0; q < d; q++) {
-                double s = 0.0;
-                for (int i = 0; i < d; i++) s += (double)c[i] * b[i * d + q];
-                p.k[q] = s;
-            }
-            p.k2 = Norm2(d, p.k);
-            if (p.k2 < rK2) {
-                for (int i = 0; i < d; i++) p.n[i] = (int32_t)c[i];
-                out.push_back(p);
-            }
+        if (first <= 0) return;
+        KP p{};
+        for (int q = 0; q < d; q++) {
+            double s = 0.0;
+            for (int i = 0; i < d; i++) s += (double)c[i] * b[i * d + q];
+            p.k[q] = s;
         }
-        for (int i = d - 1; i >= 0; i--) {  // odometer over the box
-            if (++c[i] <= lim[i]) break;
-            c[i] = -lim[i];
+        p.k2 = Norm2(d, p.k);
+        if (p.k2 < rK2) {
+            for (int i = 0; i < d; i++) p.n[i] = (int32_t)c[i];
+            out.push_back(p);
         }
-    }
+    };
+    // level i chooses n_i given n_{i+1..d-1}; T is what remains of rK^2
+    std::function<void(int, double)> level = [&](int i, double T) {
+        double center = 0.0;
+        for (int j = i + 1; j < d; j++) center -= Q[i][j] * (double)c[j];
+        const double r = std::sqrt(std::max(T, 0.0) / Q[i][i]);
+        const double lo = std::ceil(center - r) - 1.0, hi = std::floor(center + r) + 1.0;
+        if (!(hi - lo < (double)(1 << 30))) throw Error(kErrArg, "k-point cutoff too large for the cell");
+        for (int64_t v = (int64_t)lo; v <= (int64_t)hi; v++) {
+            if (++visited > ((int64_t)1 << 31)) throw Error(kErrArg, "k-point cutoff too large for the cell");
+            const double u = (double)v - center;
+            const double rem = T - Q[i][i] * u * u;
+            if (rem < -slack) continue;
+            c[i] = v;
+            if (i == 0) leaf();
+            else level(i - 1, rem);
+        }
+        c[i] = 0;
+    };
+    level(d - 1, rK2);
     std::sort(out.begin(), out.end(), [d](const KP &x, const KP &y) {
         if (x.k2 != y.k2) return x.k2 < y.k2;
         return std::lexicographical_compare(x.n, x.n + d, y.n, y.n + d);

@@ -23,6 +23,19 @@ S0K = {None: 0, "K2": 1, "Kn": 2, "Klc": 3, "KM2": 4, "KMLog": 5}
 Potential_Check_Delta = 1e-7  # potential/Potential.h
 
 
+def KPoints(basis, rK):
+    """The constrained k-points a variable-cell potential generates for the cell `basis` (rows are basis vectors):
+    (k M x d, integer coefficients M x d), 0 < |k| < rK, one of each +-k pair, ordered by |k| then coefficients."""
+    basis = np.ascontiguousarray(basis, dtype=np.float64)
+    d = basis.shape[0]
+    M = C.c_int64(0)
+    check(lib().ccgpu_varbox_kpoints(d, _p(basis), float(rK), 0, None, None, C.byref(M)))
+    k = np.zeros((max(M.value, 1), d))
+    n = np.zeros((max(M.value, 1), d), dtype=np.int32)
+    check(lib().ccgpu_varbox_kpoints(d, _p(basis), float(rK), M.value, _p(k), n.ctypes.data_as(ip32), C.byref(M)))
+    return k[: M.value], n[: M.value]
+
+
 class VarBoxPotential(ShiftedCCPotential):
     """ShiftedCCPotential_varBoxable<Base> with the weight `vk` and target `s0` mixins
     (CCPotential_varBox.h:31-209, CCPotential_Kernels.h:18-340).  form: "stealthy" (ShiftedCCPotential_varBox),

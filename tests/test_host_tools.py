@@ -6,6 +6,7 @@ import subprocess
 import sys
 
 import numpy as np
+import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -56,3 +57,23 @@ def test_potential_terms_follow_the_constraints():
     k, V, shift, e0, sigma = _potential_terms(pot)
     assert np.array_equal(k, [[1.0, 0.0], [0.0, 2.0]]) and np.array_equal(V, [0.5, 1.5])
     assert (shift, e0, sigma) == (0.25, 2.0, 0.4)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_varbox_kpoints_match_the_oracle(seed):
+    """The library's variable-cell k-point list (host entry) equals the oracle's box enumeration: same points, same
+    coefficients, same order, on random skewed cells in 2D and 3D (some strongly sheared)."""
+    from modules_amd.varbox import KPoints
+    from oracle import varbox as ovb
+    rng = np.random.default_rng(seed)
+    d = 2 + seed % 2
+    a = np.diag(rng.uniform(5.0, 12.0, d))
+    a[np.tril_indices(d, -1)] = rng.uniform(-3.0, 3.0, d * (d - 1) // 2)
+    if seed >= 4:
+        a[1, 0] = 25.0  # strong shear: the bounding box is much larger than the ellipsoid
+    rK = rng.uniform(1.0, 2.5)
+    k, n = KPoints(a, rK)
+    ko, no = ovb.kpoints(a, rK)
+    assert len(k) == len(ko) > 0
+    assert np.array_equal(n, no)
+    assert np.allclose(k, ko, rtol=0, atol=1e-14)
