@@ -490,11 +490,15 @@ __device__ __forceinline__ void force_lattice_body(const SysDesc &S, const WorkI
     const int s0 = it.t1 * S.kchunk_len;
     const int s1 = min(S.nseg, s0 + S.kchunk_len);
     const int d = S.d;
-    Real stx[PPL], sty[PPL];
+    // two phase chains per particle (even and odd slots, each stepped by step^2), each with its own sums: four
+    // independent dependency chains per particle instead of two, so the FMA latency hides behind the other chains
+    Real stx[PPL], sty[PPL], st2x[PPL], st2y[PPL];
 #pragma unroll
     for (int q = 0; q < PPL; q++) {
         stx[q] = (Real)st[q].x;
         sty[q] = (Real)st[q].y;
+        st2x[q] = stx[q] * stx[q] - sty[q] * sty[q];
+        st2y[q] = (Real)2 * stx[q] * sty[q];
     }
     for (int sg = s0; sg < s1; sg++) {
         const ConstPtr<int> hp = as_const(reinterpret_cast<const int *>(S.seg_head)) + 4 * (int64_t)sg;
@@ -502,23 +506,28 @@ __device__ __forceinline__ void force_lattice_body(const SysDesc &S, const WorkI
         const int len = as_const(S.seg_len)[sg];
         const double h0 = (double)h.x, h1 = (double)h.y, h2 = (double)h.z, h3 = (double)h.w;
         const double hl = (d == 1) ? h0 : (d == 2) ? h1 : (d == 3) ? h2 : h3;
-        Real sn[PPL], cs[PPL], acc[PPL], accl[PPL];
+        Real sn[PPL][2], cs[PPL][2], acc[PPL][2], accl[PPL][2];
 #pragma unroll
         for (int q = 0; q < PPL; q++) {
             const double u = __builtin_fma(h0, p[q].x, __builtin_fma(h1, p[q].y, __builtin_fma(h2, p[q].z, h3 * p[q].w)));
+            Real s_, c_;
             if (MIXED) {
                 float fs, fc;
                 sincos_rev_tab32((float)(u - __builtin_rint(u)), ltab32, fs, fc);
-                sn[q] = fs;
-                cs[q] = fc;
+                s_ = fs;
+                c_ = fc;
             } else {
                 double ds, dc;
                 sincos_rev_tab(u, ltab, ds, dc);
-                sn[q] = ds;
-                cs[q] = dc;
+                s_ = ds;
+                c_ = dc;
             }
-            acc[q] = 0;
-            accl[q] = 0;
+            cs[q][0] = c_;
+            sn[q][0] = s_;
+            cs[q][1] = c_ * stx[q] - s_ * sty[q];
+            sn[q][1] = c_ * sty[q] + s_ * stx[q];
+            acc[q][0] = acc[q][1] = 0;
+            accl[q][0] = accl[q][1] = 0;
         }
         // coefficients of a group of 4 slots are scalar loads; the next group's are issued before the current
         // group is computed, so the wait for them overlaps a group of FMAs (scalar loads complete out of order,
@@ -536,16 +545,17 @@ __device__ __forceinline__ void force_lattice_body(const SysDesc &S, const WorkI
                 }
 #pragma unroll
                 for (int t = t0; t < t0 + 4; t++) {
+                    const int ch = (t - t0) & 1;
                     const Real ax = cur[2 * (t - t0)], ay = cur[2 * (t - t0) + 1];
 #pragma unroll
                     for (int q = 0; q < PPL; q++) {
-                        const Real c = ax * cs[q] - ay * sn[q];
-                        acc[q] += c;
-                        accl[q] += (Real)t * c;  // t is a compile-time constant: one FMA
-                        const Real nc = cs[q] * stx[q] - sn[q] * sty[q];
-                        const Real ns = cs[q] * sty[q] + sn[q] * stx[q];
-                        cs[q] = nc;
-                        sn[q] = ns;
+                        const Real c = ax * cs[q][ch] - ay * sn[q][ch];
+                        acc[q][ch] += c;
+                        accl[q][ch] += (Real)t * c;  // t is a compile-time constant: one FMA
+                        const Real nc = cs[q][ch] * st2x[q] - sn[q][ch] * st2y[q];
+                        const Real ns = cs[q][ch] * st2y[q] + sn[q][ch] * st2x[q];
+                        cs[q][ch] = nc;
+                        sn[q][ch] = ns;
                     }
                 }
 #pragma unroll
@@ -555,8 +565,8 @@ __device__ __forceinline__ void force_lattice_body(const SysDesc &S, const WorkI
         // sum_t (hl + t) c_t = hl * sum c + sum t c; components before the last axis are constant along the segment
 #pragma unroll
         for (int q = 0; q < PPL; q++) {
-            const double A = (double)acc[q];
-            const double Al = __builtin_fma(hl, A, (double)accl[q]);
+            const double A = (double)(acc[q][0] + acc[q][1]);
+            const double Al = __builtin_fma(hl, A, (double)(accl[q][0] + accl[q][1]));
             if (d == 1) {
                 g[q][0] += Al;
             } else if (d == 2) {
