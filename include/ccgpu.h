@@ -80,6 +80,15 @@ int ccgpu_potential_add_constraint_s2(ccgpu_potential *p, const double *k, doubl
 int ccgpu_potential_set_constraints_s2(ccgpu_potential *p, int64_t M, const double *k, const double *V,
                                        const double *S0);
 
+/* ShiftedCCPotential_withWeight(pot, weights) (CollectiveCoordinatePotential.h:1498-1600): rho(k) = sum_j w_j
+ * exp(i k.r_j) and particle j's force scaled by w_j (GetRho :1520-1550, Force :1555-1577).  N weights, one per
+ * particle of the configurations bound afterwards (CCGPU_ERR_STATE at evaluation when the count differs);
+ * N = 0 clears them.  A weighted potential evaluates through the direct sums (the NUFFT spreads unit strengths). */
+int ccgpu_potential_set_weights(ccgpu_potential *p, int64_t N, const double *w);
+/* LogCCPotential(pot) (CollectiveCoordinatePotential.h:1604-1634): Energy = log(Phi) of the form it wraps and
+ * Force = F / Phi (on = 1; 0 restores the plain form).  Not with the soft core (CCGPU_ERR_STATE). */
+int ccgpu_potential_set_log_form(ccgpu_potential *p, int on);
+
 /* public member Shift -- CollectiveCoordinatePotential.h:99 */
 int ccgpu_potential_set_shift(ccgpu_potential *p, double shift);
 

@@ -5,6 +5,7 @@ C ABI of include/ccgpu.h (modules_amd/libccgpu.so).  This package is the host-si
 reference's Potential / StructureOptimization interface over that ABI.
 """
 from .potential import (Configuration, GetKs, GetUnitCubicBox, KPointConstraint, RandomConfiguration,  # noqa: F401
-                        RepulsiveCCPotential, ShiftedCCPotential, ShiftedCCPotential_S2, get_chi, getK, pi)
+                        LogCCPotential, RepulsiveCCPotential, ShiftedCCPotential, ShiftedCCPotential_S2,
+                        ShiftedCCPotential_withWeight, get_chi, getK, pi)
 from .multi import MultiStealthyPotential, SpherePacking  # noqa: F401
 from .varbox import ShiftedCCPotential_varBox, VarBoxPotential  # noqa: F401

@@ -286,6 +286,30 @@ int ccgpu_potential_num_constraints(const ccgpu_potential *p, int64_t *M)
     });
 }
 
+int ccgpu_potential_set_weights(ccgpu_potential *p, int64_t N, const double *w)
+{
+    return Guard([&] {
+        Require(p != nullptr && N >= 0 && (N == 0 || w), CCGPU_ERR_ARG, "bad argument");
+        for (int64_t i = 0; i < N; i++) Require(std::isfinite(w[i]), CCGPU_ERR_ARG, "non-finite weight");
+        Require(!p->sys.mixture, CCGPU_ERR_STATE, "weights are per single-species potential");
+        p->sys.weights.assign(w, w + N);
+        p->sys.weights_dirty = true;
+        p->batch.dirty = true;
+        p->Invalidate();
+    });
+}
+
+int ccgpu_potential_set_log_form(ccgpu_potential *p, int on)
+{
+    return Guard([&] {
+        Require(p != nullptr, CCGPU_ERR_ARG, "null handle");
+        Require(!on || !p->sys.softcore(), CCGPU_ERR_STATE, "the log form has no soft core");
+        p->sys.log_form = on != 0;
+        p->batch.dirty = true;
+        p->Invalidate();
+    });
+}
+
 int ccgpu_potential_set_shift(ccgpu_potential *p, double shift)
 {
     return Guard([&] {
@@ -304,6 +328,7 @@ int ccgpu_potential_set_softcore(ccgpu_potential *p, double epsilon, double sigm
         Require(std::abs(epsilon) <= 1e-10 || sigma > 0, CCGPU_ERR_ARG, "soft core needs sigma > 0");
         Require(!p->varbox || std::abs(epsilon) <= 1e-10, CCGPU_ERR_STATE,
                 "variable-cell potentials have no soft core");
+        Require(!p->sys.log_form || std::abs(epsilon) <= 1e-10, CCGPU_ERR_STATE, "the log form has no soft core");
         p->sys.e0 = epsilon;
         p->sys.sigma = sigma;
         p->batch.dirty = true;

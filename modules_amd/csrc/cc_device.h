@@ -23,6 +23,7 @@ enum SysFlags : int32_t {
     kSysMixture = 32,  // mixture: no CC term of its own; energy from its species and the radius contact term
     kSysNufft = 64,    // lattice constraints evaluated by the NUFFT backend (rho and forces on a fine grid)
     kSysS2 = 128,      // target-S(k) energy sum V (|rho|^2 / N - S0)^2 + Shift (no 1/vol), per-slot targets slot_s0
+    kSysLog = 256,     // energy log(Phi) and forces F / Phi (LogCCPotential); energy[1] keeps Phi
 };
 
 struct SysDesc {
@@ -39,6 +40,7 @@ struct SysDesc {
     // positions: the optimizer variable x (N*d) holds relative coordinates times `rescale`
     const double *x;
     double4 *pos;       // wrapped relative coordinates s (N), zero padded to 4
+    const double *weight;  // per-particle weights of rho (ShiftedCCPotential_withWeight); null: all 1
     double4 *cart;      // Cartesian r = sum_i a_i s_i (N)
     double2 *step;      // exp(2 pi i s_{d-1}) (N): one lattice step along the last reciprocal axis
 

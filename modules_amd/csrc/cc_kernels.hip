@@ -87,6 +87,7 @@ struct RhoLatticeLds {
     static constexpr int G = (SEG % 8 == 0) ? 8 : 4;
     static constexpr int kSub = 64;
     double4 pos[kSub];
+    double wgt[kSub];       // particle weights (weighted systems)
     double2 pow[kSub * G];  // particle j: P_1 .. P_G at j*G + b - 1 (float2 in the first half when mixed)
     double2 tab[64];
     float2 tab32[64];
@@ -112,6 +113,7 @@ __device__ __forceinline__ void rho_lattice_body(const SysDesc &S, const WorkIte
         for (int q = threadIdx.x; q < nsub; q += kRhoBlock) {
             const int j = jb + q;
             L.pos[q] = S.pos[j];
+            if (S.weight) L.wgt[q] = S.weight[j];
             double2 pw[G];
             pw[0] = S.step[j];
             pw[1] = cmul(pw[0], pw[0]);
@@ -141,6 +143,10 @@ __device__ __forceinline__ void rho_lattice_body(const SysDesc &S, const WorkIte
                 const double u = __builtin_fma(h0, p.x, __builtin_fma(h1, p.y, __builtin_fma(h2, p.z, h3 * p.w)));
                 double sn, cs;
                 sincos_rev_tab(u, L.tab, sn, cs);
+                if (S.weight) {  // the weight rides along the unit-modulus phase chain
+                    cs *= L.wgt[j];
+                    sn *= L.wgt[j];
+                }
 #pragma unroll
                 for (int t0 = 0; t0 < SEG; t0 += G) {
                     if (t0 < wlen) {
@@ -175,6 +181,10 @@ __device__ __forceinline__ void rho_lattice_body(const SysDesc &S, const WorkIte
                     const double u = __builtin_fma(h0, p.x, __builtin_fma(h1, p.y, __builtin_fma(h2, p.z, h3 * p.w)));
                     float sn, cs;
                     sincos_rev_tab32((float)(u - __builtin_rint(u)), L.tab32, sn, cs);
+                    if (S.weight) {
+                        cs *= (float)L.wgt[j];
+                        sn *= (float)L.wgt[j];
+                    }
 #pragma unroll
                     for (int t0 = 0; t0 < SEG; t0 += G) {
                         if (t0 < wlen) {
@@ -268,6 +278,7 @@ __global__ void __launch_bounds__(kBlock) k_rho_lattice_tab(const SysDesc *__res
     constexpr int kSub = 64;
     __shared__ double2 lpow[kSub * SEG];
     __shared__ double4 lpos[kSub];
+    __shared__ double lwgt[kSub];
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
     if (!sys_active(sys, S)) return;
@@ -299,7 +310,10 @@ __global__ void __launch_bounds__(kBlock) k_rho_lattice_tab(const SysDesc *__res
             sincos_rev((pr - __builtin_rint(pr)) + er, sn, cs);
             lpow[idx] = make_double2(cs, sn);
         }
-        for (int idx = threadIdx.x; idx < nsub; idx += kBlock) lpos[idx] = S.pos[jb + idx];
+        for (int idx = threadIdx.x; idx < nsub; idx += kBlock) {
+            lpos[idx] = S.pos[jb + idx];
+            if (S.weight) lwgt[idx] = S.weight[jb + idx];
+        }
         __syncthreads();
         if (valid) {
             for (int j = 0; j < nsub; j++) {
@@ -307,6 +321,10 @@ __global__ void __launch_bounds__(kBlock) k_rho_lattice_tab(const SysDesc *__res
                 const double u = __builtin_fma(h0, p.x, __builtin_fma(h1, p.y, __builtin_fma(h2, p.z, h3 * p.w)));
                 double bi, br;
                 sincos_rev(u, bi, br);
+                if (S.weight) {
+                    br *= lwgt[j];
+                    bi *= lwgt[j];
+                }
                 const double2 *P = lpow + j * SEG;
 #pragma unroll
                 for (int t0 = 0; t0 < SEG; t0 += 4) {
@@ -340,24 +358,40 @@ __global__ void __launch_bounds__(kBlock) k_rho_general(const SysDesc *__restric
 {
     constexpr int kMaxChunk = 512;
     __shared__ double4 lr[kMaxChunk];
+    __shared__ double lw[kMaxChunk];
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
     if (!sys_active(sys, S)) return;
     const int j0 = it.t1 * S.pchunk_len;
     const int j1 = min(S.N, j0 + S.pchunk_len);
-    for (int j = j0 + (int)threadIdx.x; j < j1; j += kBlock) lr[j - j0] = S.cart[j];
+    for (int j = j0 + (int)threadIdx.x; j < j1; j += kBlock) {
+        lr[j - j0] = S.cart[j];
+        if (S.weight) lw[j - j0] = S.weight[j];
+    }
     __syncthreads();
     const int q = it.t0 * kBlock + threadIdx.x;
     if (q >= S.M) return;
     const double4 k = S.kvec[q];
     double tr = 0.0, ti = 0.0;
-    for (int j = 0; j < j1 - j0; j++) {
-        const double4 r = lr[j];
-        const double th = k.x * r.x + k.y * r.y + k.z * r.z + k.w * r.w;
-        double sn, cs;
-        sincos_rad(th, sn, cs);
-        tr += cs;
-        ti += sn;
+    if (S.weight) {
+        // ShiftedCCPotential_withWeight::GetRho (CollectiveCoordinatePotential.h:1520-1550)
+        for (int j = 0; j < j1 - j0; j++) {
+            const double4 r = lr[j];
+            const double th = k.x * r.x + k.y * r.y + k.z * r.z + k.w * r.w;
+            double sn, cs;
+            sincos_rad(th, sn, cs);
+            tr += cs * lw[j];
+            ti += sn * lw[j];
+        }
+    } else {
+        for (int j = 0; j < j1 - j0; j++) {
+            const double4 r = lr[j];
+            const double th = k.x * r.x + k.y * r.y + k.z * r.z + k.w * r.w;
+            double sn, cs;
+            sincos_rad(th, sn, cs);
+            tr += cs;
+            ti += sn;
+        }
     }
     double *out = S.rpart + ((int64_t)it.t1 * S.nslots + q) * 2;
     out[0] = tr;
@@ -432,7 +466,8 @@ __global__ void __launch_bounds__(kBlock) k_energy_final(const SysDesc *__restri
     }
     if (threadIdx.x == 0) {
         const double cc = ((S.flags & kSysS2) ? cc_sum : cc_sum / S.vol) + S.shift;
-        S.energy[0] = sc + cc;
+        // LogCCPotential::Energy (CollectiveCoordinatePotential.h:1620-1623): log of the plain form's energy
+        S.energy[0] = (S.flags & kSysLog) ? log(cc) : sc + cc;
         S.energy[1] = cc;
         S.energy[2] = sc;
     }
@@ -682,6 +717,14 @@ __global__ void __launch_bounds__(kBlock) k_force_final(const SysDesc *__restric
             for (int c = 0; c < d; c++) F[c] = __builtin_fma(-G[j], S.b[j * d + c], F[c]);
     } else {
         for (int c = 0; c < d; c++) F[c] = -G[c];
+    }
+    // ShiftedCCPotential_withWeight::Force scales particle i's force by its weight (:1560-1577);
+    // LogCCPotential::Force divides by the plain form's energy (:1629-1633)
+    if (S.weight)
+        for (int c = 0; c < d; c++) F[c] = F[c] * S.weight[i];
+    if (S.flags & kSysLog) {
+        const double inv = 1.0 / S.energy[1];
+        for (int c = 0; c < d; c++) F[c] = F[c] * inv;
     }
     if (S.flags & kSysSoftCore) {
         // soft-core force of k_sc_pair, stored after the per-particle energies (N x 4)

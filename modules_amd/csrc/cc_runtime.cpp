@@ -347,7 +347,24 @@ void System::Prepare(hipStream_t stream)
     }
     const int64_t nslots = layout->nslots();
     const int64_t nunits = lattice ? layout->plan.nseg : M;  // force-pass loop units
-    if (rho_backend == 1 && lattice && !view()) PrepareNufft(*this, stream);
+    if (!weights.empty()) {
+        if ((int64_t)weights.size() != N) throw Error(kErrState, "one weight per particle");
+        if (weight.capacity() < (size_t)std::max<int64_t>(N, 1)) {
+            weight.Reserve(std::max<int64_t>(N, 1));  // a new block: its contents must be written
+            weights_dirty = true;
+        }
+        if (weights_dirty) {
+            if (N > 0)
+                CC_HIP(hipMemcpyAsync(weight.get(), weights.data(), N * sizeof(double), hipMemcpyHostToDevice, stream));
+            weights_dirty = false;
+        }
+    }
+    if (rho_backend == 1 && lattice && !view() && weights.empty()) PrepareNufft(*this, stream);
+    else if (rho_backend == 1 && !weights.empty()) {
+        if (nu) nu->active = false, nu->layout.reset();
+        if (!nu) nu.reset(new NufftState());
+        nu->why = "weighted rho";
+    }
     else if (nu) nu->active = false, nu->layout.reset(), nu->why = "direct backend selected";
 
     // Chunking of the two contractions.  Untuned shapes get enough blocks for their work: one block per ~2^18
@@ -431,7 +448,8 @@ void System::Prepare(hipStream_t stream)
     D.M = (int32_t)M;
     D.flags = kSysActive | (lattice ? kSysLattice : 0) | (softcore() ? kSysSoftCore : 0) |
               (lattice && precision == 1 && !nufft_on() ? kSysMixed : 0) | (view() ? kSysView : 0) |
-              (mixture ? kSysMixture : 0) | (nufft_on() ? kSysNufft : 0) | (s2 ? kSysS2 : 0);
+              (mixture ? kSysMixture : 0) | (nufft_on() ? kSysNufft : 0) | (s2 ? kSysS2 : 0) |
+              (log_form ? kSysLog : 0);
     D.nseg = (int32_t)nseg();
     D.nslots = (int32_t)nslots;
     D.pchunks = N > 0 ? pchunks : 0;  // no particles: rho sums nothing
@@ -444,6 +462,7 @@ void System::Prepare(hipStream_t stream)
     const int64_t off = view() ? view_off : 0;
     D.x = owner->x.get() + off * d;
     D.pos = owner->pos.get() + off;
+    D.weight = weights.empty() ? nullptr : weight.get();
     D.cart = owner->cart.get() + off;
     D.step = owner->step.get() + off;
     D.seg_head = layout->seg_head.get();

@@ -166,6 +166,12 @@ public:
     // sum V (|rho|^2 / N - S0)^2 + Shift with one target S0 per constraint
     bool s2 = false;
     std::vector<double> s0;
+    // per-particle weights of rho (ShiftedCCPotential_withWeight, CollectiveCoordinatePotential.h:1498-1600);
+    // empty: unweighted.  Uploaded at Prepare when changed.
+    std::vector<double> weights;
+    bool weights_dirty = false;
+    DevArray<double> weight;
+    bool log_form = false;  // LogCCPotential (:1604-1634): energy log(Phi), forces F / Phi
     double shift = 0, e0 = 0, sigma = 0, rescale = 1;
     Tiles tiles{0, 0, 0, -1, 0};  // requested (0 / -1 = automatic)
     Tiles used;                 // resolved

@@ -410,6 +410,39 @@ class ShiftedCCPotential_S2(ShiftedCCPotential):
         return 0 if self._config is None else self._config.NumParticle()
 
 
+def _copy_terms(dst, src):
+    dst.constraints = [KPointConstraint(c.k, c.V) for c in src.constraints]
+    dst.S0 = list(getattr(src, "S0", []))
+    dst.Shift = src.Shift
+    be = getattr(src, "_rho_backend", None)
+    if be is not None:
+        dst._rho_backend = be
+        dst._nufft_eps = getattr(src, "_nufft_eps", 1e-12)
+        dst._nufft_max_modes = getattr(src, "_nufft_max_modes", 1 << 22)
+        dst._push_backend()
+
+
+class ShiftedCCPotential_withWeight(ShiftedCCPotential):
+    """rho(k) = sum_j w_j exp(i k.r_j), particle j's force times w_j (ShiftedCCPotential_withWeight,
+    CollectiveCoordinatePotential.h:1498-1600): built from a plain potential and one weight per particle."""
+
+    def __init__(self, Pot, weights):
+        super().__init__(Pot.Dimension)
+        _copy_terms(self, Pot)
+        self.weights = np.ascontiguousarray(weights, dtype=np.float64)
+        check(lib().ccgpu_potential_set_weights(self._h, len(self.weights), _p(self.weights)))
+
+
+class LogCCPotential(ShiftedCCPotential):
+    """Energy log(Phi) of the plain form, forces F / Phi (LogCCPotential, CollectiveCoordinatePotential.h:1604-1634):
+    built from a plain potential."""
+
+    def __init__(self, pot):
+        super().__init__(pot.Dimension)
+        _copy_terms(self, pot)
+        check(lib().ccgpu_potential_set_log_form(self._h, 1))
+
+
 class RepulsiveCCPotential(ShiftedCCPotential):
     """Soft core e0 (1 - r/sigma)^2 for r < sigma plus the CC potential
     (apps/CCOptimization/RepulsiveCCPotential.h:21-157).  Constraints go to .CCPotential, as in the reference."""

@@ -13,6 +13,7 @@
  *   - getK                                                  apps/CCOptimization/main.cpp:2018-2021
  *   - ShiftedCCPotential::GetRho_Direct / Energy / Force    potential/CollectiveCoordinatePotential.h:1255-1394
  *   - ShiftedCCPotential_S2::Energy / Force (target S(k))   potential/CollectiveCoordinatePotential.h:1686-1741
+ *   - ShiftedCCPotential_withWeight::GetRho (weighted rho)  potential/CollectiveCoordinatePotential.h:1520-1550
  *   - RepulsiveCCPotential soft core energy / forces         apps/CCOptimization/RepulsiveCCPotential.h:61-145
  *     (brute force over periodic images instead of the cell-list walk; same pair terms)
  *   - MultiStealthyPotential energy / forces (species CC      apps/CCOptimization/MultiStealthyPotential.cpp:682-833
@@ -308,6 +309,28 @@ void orc_rho(int d, int64_t N, const double *s, const double *a, int64_t M, cons
 		for (int64_t j = 0; j < N; j++) {
 			tr += cos(gv_dot(k4, r + j * ORC_MAXDIM));
 			ti += sin(gv_dot(k4, r + j * ORC_MAXDIM));
+		}
+		re[i] = tr;
+		im[i] = ti;
+	}
+	free(r);
+}
+
+/* ShiftedCCPotential_withWeight::GetRho (CollectiveCoordinatePotential.h:1520-1550): rho_k = sum_j w_j cos(k.r_j)
+ * + i w_j sin(k.r_j), parallel over k. */
+void orc_rho_weighted(int d, int64_t N, const double *s, const double *a, int64_t M, const double *k,
+                      const double *w, double *re, double *im, int nthreads)
+{
+	double *r = (double *)malloc(sizeof(double) * ORC_MAXDIM * (N > 0 ? N : 1));
+	for (int64_t j = 0; j < N; j++) orc_cart(d, a, s + j * d, r + j * ORC_MAXDIM);
+	if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+	for (int64_t i = 0; i < M; i++) {
+		double k4[ORC_MAXDIM], tr = 0, ti = 0;
+		orc_pad(d, k + i * d, k4);
+		for (int64_t j = 0; j < N; j++) {
+			tr += cos(gv_dot(k4, r + j * ORC_MAXDIM)) * w[j];
+			ti += sin(gv_dot(k4, r + j * ORC_MAXDIM)) * w[j];
 		}
 		re[i] = tr;
 		im[i] = ti;

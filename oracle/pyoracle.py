@@ -62,6 +62,7 @@ def lib():
         L.orc_multi_energy_forces.argtypes = [C.c_int, C.c_int64, _dp, _dp, C.c_int, _i64p, _i64p, _i64p, _dp, _dp,
                                               _dp, C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp, C.c_int]
         L.orc_multi_energy_forces.restype = C.c_double
+        L.orc_rho_weighted.argtypes = [C.c_int, C.c_int64, _dp, _dp, C.c_int64, _dp, _dp, _dp, _dp, C.c_int]
         L.orc_s2_energy.argtypes = [C.c_int64, _dp, _dp, _dp, _dp, C.c_int64, C.c_double]
         L.orc_s2_energy.restype = C.c_double
         L.orc_s2_forces.argtypes = [C.c_int, C.c_int64, _dp, _dp, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int]
@@ -168,6 +169,25 @@ def forces(s, a, k, v, re, im, nthreads=0):
     lib().orc_forces(d, N, _p(s), _p(a), M, _p(kk), _p(v if M else np.zeros(1)), _p(re if M else np.zeros(1)),
                      _p(im if M else np.zeros(1)), volume(a), _p(F), nthreads or lib().orc_max_threads())
     return F
+
+
+def rho_weighted(s, a, k, w, nthreads=0):
+    """ShiftedCCPotential_withWeight::GetRho: rho = sum_j w_j exp(i k.r_j)."""
+    s, a, k, w = _c(s), _c(a), _c(k), _c(w)
+    N, d = s.shape
+    M = k.shape[0]
+    re = np.zeros(M)
+    im = np.zeros(M)
+    if M:
+        lib().orc_rho_weighted(d, N, _p(s), _p(a), M, _p(k), _p(w if N else np.zeros(1)), _p(re), _p(im),
+                               nthreads or lib().orc_max_threads())
+    return re, im
+
+
+def weighted_forces(s, a, k, v, w, re, im, nthreads=0):
+    """ShiftedCCPotential_withWeight::Force for every particle: the plain form's force over the weighted rho,
+    times the particle's weight (CollectiveCoordinatePotential.h:1555-1577)."""
+    return forces(s, a, k, v, re, im, nthreads) * _c(w)[:, None]
 
 
 def s2_energy(v, s0, re, im, N, shift=0.0):

@@ -275,3 +275,64 @@ def test_variable_cell_relax_matches_oracle(form, vk, s0, d, N, seed, cell, P, s
     assert np.array_equal(pot.kCoeffs, ovb.kpoints(ao, 1.0)[1])
     if switch == 1:
         assert np.array_equal(cfg.relative, s)
+
+
+# ---------------------------------------------------------------------------------------------------------------
+# weighted rho and the log form
+# ---------------------------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("N,d,chi,seed,general", [(400, 2, 0.3, 3, False), (1500, 3, 0.4, 4, False),
+                                                  (300, 2, 0.3, 5, True)])
+def test_weighted_rho_matches_oracle(N, d, chi, seed, general):
+    from modules_amd import ShiftedCCPotential, ShiftedCCPotential_withWeight
+    cfg = RandomConfiguration(seed, N, d)
+    rng = np.random.default_rng(seed)
+    k = rng.normal(size=(80, d)) * 1.5 if general else GetKs(cfg, getK(chi, 1.0, d))
+    v = 0.5 + rng.random(len(k))
+    w = 0.2 + 1.6 * rng.random(N)
+    base = ShiftedCCPotential(d)
+    for kk, vv in zip(k, v):
+        base.AddConstraint(kk, [vv])
+    base.Shift = 0.125
+    base.SetRhoBackend("nufft")  # a weighted potential keeps the direct sums
+    pot = ShiftedCCPotential_withWeight(base, w)
+    pot.SetConfiguration(cfg)
+    assert pot.GetLastRhoBackendUsed() == "direct"
+    assert pot.UsesLatticePath() == (not general)
+    pot.GetRho()
+    E, F = pot.EnergyAndAllForce()
+    re, im = po.rho_weighted(cfg.relative, cfg.basis, k, w)
+    vol = po.volume(cfg.basis)
+    Eo = po.energy(v, re, im, vol, 0.125)
+    Fo = po.weighted_forces(cfg.relative, cfg.basis, k, v, w, re, im)
+    W = np.sum(np.abs(w))
+    kmax = np.max(np.linalg.norm(k, axis=1))
+    assert np.max(np.abs(pot.RhoReal - re)) <= RTOL * W and np.max(np.abs(pot.RhoImag - im)) <= RTOL * W
+    assert abs(E - Eo) <= RTOL * np.sum(v) * W * W / vol
+    assert np.max(np.abs(F - Fo)) <= RTOL * 4 * np.sum(v) * W * kmax * np.max(w) / vol
+    with pytest.raises(CCGPUError):  # one weight per particle
+        pot.SetConfiguration(RandomConfiguration(seed, N + 1, d))
+        pot.Energy()
+
+
+def test_log_form_matches_oracle_and_relaxes():
+    from modules_amd import LogCCPotential, ShiftedCCPotential
+    cfg = RandomConfiguration(12, 300, 2)
+    k = GetKs(cfg, getK(0.3, 1.0, 2))
+    v = np.ones(len(k))
+    base = ShiftedCCPotential(2)
+    for kk in k:
+        base.AddConstraint(kk, [1.0])
+    base.Shift = 0.5
+    pot = LogCCPotential(base)
+    pot.SetConfiguration(cfg)
+    E, F = pot.EnergyAndAllForce()
+    re, im = po.rho(cfg.relative, cfg.basis, k)
+    Ep = po.energy(v, re, im, po.volume(cfg.basis), 0.5)
+    Fp = po.forces(cfg.relative, cfg.basis, k, v, re, im)
+    assert E == pytest.approx(np.log(Ep), rel=1e-13)
+    assert np.allclose(F, Fp / Ep, rtol=1e-11, atol=1e-13 * np.max(np.abs(Fp / Ep)))
+    c1 = cfg.copy()
+    H, ev, _ = RelaxStructure_NLOPT(c1, pot, 0.0, 0, 0.0, 1500)
+    re, im = po.rho(c1.relative, c1.basis, k)
+    assert H == pytest.approx(np.log(po.energy(v, re, im, po.volume(c1.basis), 0.5)), rel=1e-10)
+    assert H < E and ev > 0

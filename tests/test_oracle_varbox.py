@@ -183,3 +183,37 @@ def test_shift_closed_forms():
     assert vb.shift("stealthy", 10, 20.0, v) == pytest.approx(10 * 9 / 20 / 2 - 10 / 20 * 0.875)
     assert vb.shift("S2", 10, 20.0, v, np.ones(3)) == 0.0
     assert vb.shift("S2_noV0", 10, 20.0, v, np.array([0.0, 2.0, 3.0])) == pytest.approx(-(0.5 + 0.25 + 0.5))
+
+
+def test_weighted_rho_pins():
+    """ShiftedCCPotential_withWeight's rho: unit weights give the plain rho; integer weights equal duplicating the
+    particles; forces are minus the gradient of the energy over the weighted rho."""
+    rng = np.random.default_rng(5)
+    a = np.array([[5.0, 0.0], [0.8, 4.5]])
+    s = rng.random((9, 2))
+    k, _ = po.get_ks(a, 3.5)
+    re1, im1 = po.rho_weighted(s, a, k, np.ones(9))
+    re0, im0 = po.rho(s, a, k)
+    assert np.array_equal(re1, re0) and np.array_equal(im1, im0)
+    w = np.array([1, 2, 3, 1, 1, 2, 1, 4, 1], dtype=float)
+    rew, imw = po.rho_weighted(s, a, k, w)
+    dup = np.repeat(s, w.astype(int), axis=0)
+    red, imd = po.rho(dup, a, k)
+    assert np.allclose(rew, red, atol=1e-12) and np.allclose(imw, imd, atol=1e-12)
+    v = 0.5 + rng.random(len(k))
+    vol = po.volume(a)
+    wr = 0.3 + rng.random(9)
+    re, im = po.rho_weighted(s, a, k, wr)
+    F = po.weighted_forces(s, a, k, v, wr, re, im)
+    inv = np.linalg.inv(a)
+    h = 1e-6
+    for j in range(9):
+        for c in range(2):
+            dr = np.zeros(2)
+            dr[c] = h
+            sp, sm = s.copy(), s.copy()
+            sp[j] += dr @ inv
+            sm[j] -= dr @ inv
+            Ep = po.energy(v, *po.rho_weighted(sp, a, k, wr), vol)
+            Em = po.energy(v, *po.rho_weighted(sm, a, k, wr), vol)
+            assert abs(-(Ep - Em) / (2 * h) - F[j, c]) < 1e-6 * max(1.0, abs(F[j, c]))
