@@ -74,6 +74,15 @@ int ccgpu_potential_num_constraints(const ccgpu_potential *p, int64_t *M);
  * ccgpu_potential_add_constraint / set_constraints return CCGPU_ERR_ARG on it (the reference's AddConstraint
  * throws std::invalid_argument when vals lacks S0, :1663-1668). */
 int ccgpu_potential_create_s2(int dim, ccgpu_potential **out);
+/* ShiftedCCPotential_S2_MultiConfig(d, NSubConfig) -- CollectiveCoordinatePotential.h:1756-1901: the bound
+ * configuration is NSubConfig sub-configurations of Np = N / NSubConfig consecutive particles (N must be a
+ * multiple: CCGPU_ERR_ARG otherwise), and
+ *   Phi = sum_k V(k) (<S(k)> - S0(k))^2 + Shift,  <S(k)> = sum_c |rho_c(k)|^2 / (Np NSubConfig)
+ *   F_j = -(4 / (Np NSubConfig)) sum_k V(k) (<S(k)> - S0(k)) (Im rho_c cos(k.r_j) - Re rho_c sin(k.r_j)) k
+ * for particle j of sub-configuration c (Energy :1818-1844, Force :1849-1879).  Constraints through
+ * ccgpu_potential_add_constraint_s2 / set_constraints_s2; ccgpu_potential_get_rho writes NSubConfig x M values
+ * (sub-configuration c at c M, as RhoReal / RhoImag :1785-1800).  No soft core, no NUFFT (direct sums). */
+int ccgpu_potential_create_s2_multiconfig(int dim, int64_t nsub, ccgpu_potential **out);
 /* AddConstraint(k, {V, S0}) -- CollectiveCoordinatePotential.h:1663-1668 */
 int ccgpu_potential_add_constraint_s2(ccgpu_potential *p, const double *k, double V, double S0);
 /* constraints = ..., S0 = ... (public members, :1645, :1667): k M x dim, V M, S0 M */

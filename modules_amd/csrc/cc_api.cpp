@@ -38,7 +38,52 @@ struct ccgpu_potential {
     void EnsureBuilt()
     {
         Require(bound, CCGPU_ERR_STATE, "no configuration bound: call SetConfiguration first");
+        if (sys.nsub > 0) SyncSubConfigs();
         if (batch.dirty || sys.layout_dirty) batch.Build();
+    }
+
+    // ShiftedCCPotential_S2_MultiConfig (CollectiveCoordinatePotential.h:1756-1901): the mixture's children are
+    // the nsub sub-configurations, Np = N / nsub consecutive particles each, with the mixture's constraints and
+    // targets (SetConfiguration, :1882-1898, requires N to be a multiple of nsub)
+    void SyncSubConfigs()
+    {
+        const int64_t C = sys.nsub;
+        Require(sys.N % C == 0, CCGPU_ERR_ARG,
+                "the total number of particles is not multiple of the number of configurations");
+        const int64_t np = sys.N / C;
+        if ((int64_t)sys.children.size() != C) {
+            sys.children.clear();
+            for (int64_t c = 0; c < C; c++) {
+                std::unique_ptr<System> ch(new System(sys.d));
+                ch->parent = &sys;
+                ch->s2 = true;
+                ch->fscale = 1.0;
+                sys.children.push_back(std::move(ch));
+            }
+            batch.dirty = true;
+        }
+        for (int64_t c = 0; c < C; c++) {
+            System &ch = *sys.children[(size_t)c];
+            if (!ch.has_cell || !std::equal(sys.a, sys.a + sys.d * sys.d, ch.a)) {
+                ch.SetCell(sys.a);
+                batch.dirty = true;
+            }
+            if (ch.N != np || ch.view_off != c * np) {
+                ch.N = np;
+                ch.view_off = c * np;
+                batch.dirty = true;
+            }
+            if (ch.k != sys.k || ch.v != sys.v || ch.s0 != sys.s0) {
+                ch.SetConstraints(sys.M, sys.k.data(), sys.v.data());
+                ch.SetTargets(sys.s0.data());
+                batch.dirty = true;
+            }
+            if (ch.tiles.seg != sys.tiles.seg || ch.precision != sys.precision) {
+                ch.tiles = sys.tiles;
+                ch.precision = sys.precision;
+                batch.dirty = true;
+            }
+        }
     }
     void EnsureRho()
     {
@@ -222,6 +267,22 @@ int ccgpu_potential_create_s2(int dim, ccgpu_potential **out)
     });
 }
 
+int ccgpu_potential_create_s2_multiconfig(int dim, int64_t nsub, ccgpu_potential **out)
+{
+    return Guard([&] {
+        Require(out != nullptr, CCGPU_ERR_ARG, "null output handle");
+        *out = nullptr;
+        Require(dim >= 1 && dim <= kMaxDim, CCGPU_ERR_ARG, "dimension must be 1..4");
+        Require(nsub >= 1 && nsub <= (1 << 20), CCGPU_ERR_ARG, "number of sub-configurations must be 1..2^20");
+        RequireDevice();
+        std::unique_ptr<ccgpu_potential> p(new ccgpu_potential(dim));
+        p->sys.s2 = true;
+        p->sys.mixture = true;
+        p->sys.nsub = nsub;
+        *out = p.release();
+    });
+}
+
 int ccgpu_potential_add_constraint_s2(ccgpu_potential *p, const double *k, double V, double S0)
 {
     return Guard([&] {
@@ -304,6 +365,7 @@ int ccgpu_potential_set_log_form(ccgpu_potential *p, int on)
     return Guard([&] {
         Require(p != nullptr, CCGPU_ERR_ARG, "null handle");
         Require(!on || !p->sys.softcore(), CCGPU_ERR_STATE, "the log form has no soft core");
+        Require(!on || !p->sys.mixture, CCGPU_ERR_STATE, "the log form wraps a single-configuration potential");
         p->sys.log_form = on != 0;
         p->batch.dirty = true;
         p->Invalidate();
@@ -329,6 +391,8 @@ int ccgpu_potential_set_softcore(ccgpu_potential *p, double epsilon, double sigm
         Require(!p->varbox || std::abs(epsilon) <= 1e-10, CCGPU_ERR_STATE,
                 "variable-cell potentials have no soft core");
         Require(!p->sys.log_form || std::abs(epsilon) <= 1e-10, CCGPU_ERR_STATE, "the log form has no soft core");
+        Require(p->sys.nsub == 0 || std::abs(epsilon) <= 1e-10, CCGPU_ERR_STATE,
+                "the sub-configuration potential has no soft core");
         p->sys.e0 = epsilon;
         p->sys.sigma = sigma;
         p->batch.dirty = true;
@@ -355,10 +419,17 @@ int ccgpu_potential_get_rho(ccgpu_potential *p, double *re, double *im)
         Require(p != nullptr, CCGPU_ERR_ARG, "null handle");
         p->EnsureRho();
         const int64_t M = p->sys.M;
-        if (M > 0) {
-            if (re) CC_HIP(hipMemcpyAsync(re, p->sys.rho.get(), M * sizeof(double), hipMemcpyDeviceToHost, p->batch.stream));
+        // sub-configurations: RhoReal / RhoImag hold nsub x M values, sub-configuration c at c M (GetRho_MultiConfig,
+        // CollectiveCoordinatePotential.h:1777-1808)
+        const int64_t C = p->sys.nsub > 0 ? p->sys.nsub : 1;
+        for (int64_t c = 0; c < C && M > 0; c++) {
+            const System &src = p->sys.nsub > 0 ? *p->sys.children[(size_t)c] : p->sys;
+            if (re)
+                CC_HIP(hipMemcpyAsync(re + c * M, src.rho.get(), M * sizeof(double), hipMemcpyDeviceToHost,
+                                      p->batch.stream));
             if (im)
-                CC_HIP(hipMemcpyAsync(im, p->sys.rho.get() + M, M * sizeof(double), hipMemcpyDeviceToHost, p->batch.stream));
+                CC_HIP(hipMemcpyAsync(im + c * M, src.rho.get() + M, M * sizeof(double), hipMemcpyDeviceToHost,
+                                      p->batch.stream));
         }
         p->batch.Sync();
     });

@@ -24,6 +24,7 @@ enum SysFlags : int32_t {
     kSysNufft = 64,    // lattice constraints evaluated by the NUFFT backend (rho and forces on a fine grid)
     kSysS2 = 128,      // target-S(k) energy sum V (|rho|^2 / N - S0)^2 + Shift (no 1/vol), per-slot targets slot_s0
     kSysLog = 256,     // energy log(Phi) and forces F / Phi (LogCCPotential); energy[1] keeps Phi
+    kSysMultiConfig = 512,  // mixture of equal sub-configurations sharing one target-S(k) energy of the mean S
 };
 
 struct SysDesc {

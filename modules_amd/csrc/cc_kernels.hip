@@ -474,6 +474,50 @@ __global__ void __launch_bounds__(kBlock) k_energy_final(const SysDesc *__restri
 }
 
 // ---------------------------------------------------------------------------------------------
+// sub-configuration coefficients (ShiftedCCPotential_S2_MultiConfig, CollectiveCoordinatePotential.h:1756-1901):
+// for each slot the mean structure factor over the C sub-configurations (the mixture's children, Np particles
+// each, rho already reduced into each child's rho in constraint order), S = sum_c |rho_c|^2 / (Np C), the energy
+// partial sum V (S - S0)^2 into the mixture's epart, and each child's force coefficients
+// (4 V (S - S0) / (Np C)) (Im rho_c, Re rho_c).  items: (mixture, slot block)
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_multiconfig_coef(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
+{
+    __shared__ double lds[kBlock];
+    const WorkItem it = items[blockIdx.x];
+    const SysDesc &S = sys[it.sys];
+    if (!sys_active(sys, S)) return;
+    const int q = it.t0 * kBlock + threadIdx.x;
+    double e = 0.0;
+    if (q < S.nslots) {
+        const int kidx = (S.flags & kSysLattice) ? S.slot_k[q] : q;
+        const double np = (double)sys[S.child0].N, nc = (double)S.nchild;
+        double snow = 0.0;
+        if (kidx >= 0)
+            for (int c = 0; c < S.nchild; c++) {
+                const SysDesc &Cd = sys[S.child0 + c];
+                const double re = Cd.rho[kidx], im = Cd.rho[Cd.M + kidx];
+                snow += re * re + im * im;
+            }
+        snow /= np * nc;
+        const double v = S.slot_v[q];
+        const double ds = snow - S.slot_s0[q];
+        if (kidx >= 0) e = v * ds * ds;
+        const double w = kidx >= 0 ? (4.0 / np) * v * ds / nc : 0.0;
+        for (int c = 0; c < S.nchild; c++) {
+            const SysDesc &Cd = sys[S.child0 + c];
+            const double re = kidx >= 0 ? Cd.rho[kidx] : 0.0, im = kidx >= 0 ? Cd.rho[Cd.M + kidx] : 0.0;
+            const double ca = w * im, cb = w * re;
+            Cd.coef[2 * q] = ca;
+            Cd.coef[2 * q + 1] = cb;
+            Cd.coef32[2 * q] = (float)ca;
+            Cd.coef32[2 * q + 1] = (float)cb;
+        }
+    }
+    const double tot = block_sum(e, lds);
+    if (threadIdx.x == 0) S.epart[it.t0] = tot;
+}
+
+// ---------------------------------------------------------------------------------------------
 // mixture energy: u0 sum_s (N_s / N) E_s + e0 sum_i contact_i (MultiStealthyPotential::TotalEnergy,
 // apps/CCOptimization/MultiStealthyPotential.cpp:750-761), species in index order.  items: (mixture)
 // ---------------------------------------------------------------------------------------------
@@ -483,6 +527,19 @@ __global__ void __launch_bounds__(kBlock) k_mixture_final(const SysDesc *__restr
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
     if (!sys_active(sys, S)) return;
+    if (S.flags & kSysMultiConfig) {
+        // ShiftedCCPotential_S2_MultiConfig::Energy (:1818-1844): sum V (S - S0)^2 + Shift over the mean S
+        const int nb = (S.nslots + kBlock - 1) / kBlock;
+        double acc = 0.0;
+        for (int b = threadIdx.x; b < nb; b += kBlock) acc += S.epart[b];
+        const double cc = block_sum(acc, lds);
+        if (threadIdx.x == 0) {
+            S.energy[0] = cc + S.shift;
+            S.energy[1] = cc + S.shift;
+            S.energy[2] = 0.0;
+        }
+        return;
+    }
     double a = 0.0;
     if (S.flags & kSysSoftCore)
         for (int i = threadIdx.x; i < S.N; i += kBlock) a += S.sc_epart[i];
@@ -854,6 +911,11 @@ void launch_rho_reduce(hipStream_t stream, const SysDesc *sys, const WorkItem *i
 void launch_energy_final(hipStream_t stream, const SysDesc *sys, const WorkItem *items, int n)
 {
     if (n > 0) hipLaunchKernelGGL(k_energy_final, CC_GRID(n), sys, items);
+}
+
+void launch_multiconfig_coef(hipStream_t stream, const SysDesc *sys, const WorkItem *items, int n)
+{
+    if (n > 0) hipLaunchKernelGGL(k_multiconfig_coef, CC_GRID(n), sys, items);
 }
 
 void launch_mixture_final(hipStream_t stream, const SysDesc *sys, const WorkItem *items, int n)

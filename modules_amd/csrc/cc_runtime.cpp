@@ -359,7 +359,7 @@ void System::Prepare(hipStream_t stream)
             weights_dirty = false;
         }
     }
-    if (rho_backend == 1 && lattice && !view() && weights.empty()) PrepareNufft(*this, stream);
+    if (rho_backend == 1 && lattice && !view() && !mixture && weights.empty()) PrepareNufft(*this, stream);
     else if (rho_backend == 1 && !weights.empty()) {
         if (nu) nu->active = false, nu->layout.reset();
         if (!nu) nu.reset(new NufftState());
@@ -449,7 +449,7 @@ void System::Prepare(hipStream_t stream)
     D.flags = kSysActive | (lattice ? kSysLattice : 0) | (softcore() ? kSysSoftCore : 0) |
               (lattice && precision == 1 && !nufft_on() ? kSysMixed : 0) | (view() ? kSysView : 0) |
               (mixture ? kSysMixture : 0) | (nufft_on() ? kSysNufft : 0) | (s2 ? kSysS2 : 0) |
-              (log_form ? kSysLog : 0);
+              (log_form ? kSysLog : 0) | (mixture && nsub > 0 ? kSysMultiConfig : 0);
     D.nseg = (int32_t)nseg();
     D.nslots = (int32_t)nslots;
     D.pchunks = N > 0 ? pchunks : 0;  // no particles: rho sums nothing
@@ -619,7 +619,8 @@ void Batch::Build()
     std::vector<std::tuple<int, int, int>> segs;
     for (System *s : all) {
         const std::tuple<int, int, int> key(s->used.seg, s->used.rho_kernel, s->used.force_ppl);
-        if (s->lattice && !s->nufft_on() && std::find(segs.begin(), segs.end(), key) == segs.end()) segs.push_back(key);
+        if (s->lattice && !s->nufft_on() && !s->mixture && std::find(segs.begin(), segs.end(), key) == segs.end())
+            segs.push_back(key);
     }
     for (const auto &key : segs) {
         const int sl = std::get<0>(key), rk = std::get<1>(key), ppl = std::get<2>(key);
@@ -629,13 +630,13 @@ void Batch::Build()
         g.force_ppl = ppl;
         g.any_mixed = false;
         for (System *s : all)
-            if (s->lattice && !s->nufft_on() && s->precision == 1 && s->used.seg == sl && s->used.rho_kernel == rk &&
-                s->used.force_ppl == ppl)
+            if (s->lattice && !s->nufft_on() && !s->mixture && s->precision == 1 && s->used.seg == sl &&
+                s->used.rho_kernel == rk && s->used.force_ppl == ppl)
                 g.any_mixed = true;
         add(g.rho, [&] {
             for (int i = 0; i < ns; i++) {
                 System *s = all[i];
-                if (!s->lattice || s->nufft_on() || s->used.seg != sl || s->used.rho_kernel != rk ||
+                if (!s->lattice || s->nufft_on() || s->mixture || s->used.seg != sl || s->used.rho_kernel != rk ||
                     s->used.force_ppl != ppl || s->N == 0)
                     continue;
                 const int bs = rk == 1 ? kBlock : kRhoBlock;  // segments per rho block
@@ -647,7 +648,7 @@ void Batch::Build()
         add(g.force, [&] {
             for (int i = 0; i < ns; i++) {
                 System *s = all[i];
-                if (!s->lattice || s->nufft_on() || s->used.seg != sl || s->used.rho_kernel != rk ||
+                if (!s->lattice || s->nufft_on() || s->mixture || s->used.seg != sl || s->used.rho_kernel != rk ||
                     s->used.force_ppl != ppl)
                     continue;
                 const int fb = (int)((s->N + kBlock * ppl - 1) / (kBlock * ppl));
@@ -660,7 +661,7 @@ void Batch::Build()
     add(r_rho_gen, [&] {
         for (int i = 0; i < ns; i++) {
             System *s = all[i];
-            if (s->lattice || s->M == 0 || s->N == 0) continue;
+            if (s->lattice || s->mixture || s->M == 0 || s->N == 0) continue;
             const int kb = (int)((s->M + kBlock - 1) / kBlock);
             for (int c = 0; c < s->pchunks; c++)
                 for (int b = 0; b < kb; b++) h_items.push_back({i, b, c, 0});
@@ -668,6 +669,14 @@ void Batch::Build()
     });
     add(r_red, [&] {
         for (int i = 0; i < ns; i++) {
+            if (all[i]->mixture) continue;  // a mixture's rho is its children's
+            const int nb = (int)((all[i]->desc.nslots + kBlock - 1) / kBlock);
+            for (int b = 0; b < nb; b++) h_items.push_back({i, b, 0, 0});
+        }
+    });
+    add(r_mc, [&] {
+        for (int i = 0; i < ns; i++) {
+            if (!all[i]->mixture || all[i]->nsub <= 0) continue;
             const int nb = (int)((all[i]->desc.nslots + kBlock - 1) / kBlock);
             for (int b = 0; b < nb; b++) h_items.push_back({i, b, 0, 0});
         }
@@ -797,6 +806,7 @@ void Batch::Enqueue(bool rho, bool forces)
             ev_marks_.push_back(mk);
         }
         launch_rho_reduce(stream, D, I + r_red.off, r_red.n);
+        launch_multiconfig_coef(stream, D, I + r_mc.off, r_mc.n);
         launch_energy_final(stream, D, I + r_efin.off, r_efin.n);
         launch_mixture_final(stream, D, I + r_mix.off, r_mix.n);
     }

@@ -172,6 +172,9 @@ public:
     bool weights_dirty = false;
     DevArray<double> weight;
     bool log_form = false;  // LogCCPotential (:1604-1634): energy log(Phi), forces F / Phi
+    // ShiftedCCPotential_S2_MultiConfig (:1756-1901): a mixture of nsub equal sub-configurations (its children,
+    // views of consecutive particle ranges) whose target-S(k) energy reads the mean S(k); 0 otherwise
+    int64_t nsub = 0;
     double shift = 0, e0 = 0, sigma = 0, rescale = 1;
     Tiles tiles{0, 0, 0, -1, 0};  // requested (0 / -1 = automatic)
     Tiles used;                 // resolved
@@ -269,7 +272,7 @@ public:
     DevArray<SysDesc> d_desc;
     DevArray<WorkItem> d_items;
     std::vector<WorkItem> h_items;
-    Range r_prep, r_rho_gen, r_red, r_efin, r_f_gen, r_ffin, r_sc_part, r_sc_sys, r_sc_cell, r_mix;
+    Range r_prep, r_rho_gen, r_red, r_efin, r_f_gen, r_ffin, r_sc_part, r_sc_sys, r_sc_cell, r_mix, r_mc;
     Range r_nu_part, r_nu_sys, r_nu_bins, r_nu_tiles, r_nu_gath, r_nu_slots, r_nu_grid;  // NUFFT systems
     bool has_nufft = false;
     int nu_max_psize = 1;  // largest padded tile of the batch's NUFFT systems (dynamic LDS of the spread kernel)
@@ -297,6 +300,7 @@ void launch_rho_general(hipStream_t, const SysDesc *, const WorkItem *, int);
 void launch_rho_reduce(hipStream_t, const SysDesc *, const WorkItem *, int);
 void launch_energy_final(hipStream_t, const SysDesc *, const WorkItem *, int);
 void launch_mixture_final(hipStream_t, const SysDesc *, const WorkItem *, int);
+void launch_multiconfig_coef(hipStream_t, const SysDesc *, const WorkItem *, int);
 void launch_force_lattice(hipStream_t, int seg, const SysDesc *, const WorkItem *, int, int ppl);
 void launch_force_general(hipStream_t, const SysDesc *, const WorkItem *, int);
 void launch_force_final(hipStream_t, const SysDesc *, const WorkItem *, int);

@@ -410,6 +410,39 @@ class ShiftedCCPotential_S2(ShiftedCCPotential):
         return 0 if self._config is None else self._config.NumParticle()
 
 
+class ShiftedCCPotential_S2_MultiConfig(ShiftedCCPotential):
+    """The configuration as NSubConfig sub-configurations of Np = N / NSubConfig consecutive particles, with
+    Phi = sum_k V (<S(k)> - S0)^2 + Shift over the mean S(k) (ShiftedCCPotential_S2_MultiConfig,
+    CollectiveCoordinatePotential.h:1756-1901).  Constraints take vals = [V(k), S0(k)]; GetRho fills RhoReal / RhoImag
+    with NSubConfig x M values (sub-configuration c at c M) and Snow with the mean S(k)."""
+
+    _S2 = True
+
+    def __init__(self, d, NSubConfig):
+        self.NSubConfig = int(NSubConfig)
+        super().__init__(d)
+        self.Snow = np.zeros(0)
+
+    def _create(self):
+        check(lib().ccgpu_potential_create_s2_multiconfig(self.Dimension, self.NSubConfig, C.byref(self._h)))
+
+    @property
+    def Np(self):
+        return 0 if self._config is None else self._config.NumParticle() // self.NSubConfig
+
+    def GetRho(self):
+        self._require()
+        M = len(self.constraints)
+        n = max(M * self.NSubConfig, 1)
+        re = np.zeros(n)
+        im = np.zeros(n)
+        check(lib().ccgpu_potential_get_rho(self._h, _p(re), _p(im)))
+        self.RhoReal, self.RhoImag = re[: M * self.NSubConfig], im[: M * self.NSubConfig]
+        s2 = (self.RhoReal ** 2 + self.RhoImag ** 2).reshape(self.NSubConfig, M)
+        self.Snow = s2.sum(axis=0) / (self.Np * self.NSubConfig) if M else np.zeros(0)
+        self.RhoValid = True
+
+
 def _copy_terms(dst, src):
     dst.constraints = [KPointConstraint(c.k, c.V) for c in src.constraints]
     dst.S0 = list(getattr(src, "S0", []))

@@ -14,6 +14,7 @@
  *   - ShiftedCCPotential::GetRho_Direct / Energy / Force    potential/CollectiveCoordinatePotential.h:1255-1394
  *   - ShiftedCCPotential_S2::Energy / Force (target S(k))   potential/CollectiveCoordinatePotential.h:1686-1741
  *   - ShiftedCCPotential_withWeight::GetRho (weighted rho)  potential/CollectiveCoordinatePotential.h:1520-1550
+ *   - ShiftedCCPotential_S2_MultiConfig rho / E / F        potential/CollectiveCoordinatePotential.h:1777-1879
  *   - RepulsiveCCPotential soft core energy / forces         apps/CCOptimization/RepulsiveCCPotential.h:61-145
  *     (brute force over periodic images instead of the cell-list walk; same pair terms)
  *   - MultiStealthyPotential energy / forces (species CC      apps/CCOptimization/MultiStealthyPotential.cpp:682-833
@@ -403,6 +404,46 @@ void orc_s2_forces(int d, int64_t N, const double *s, const double *a, int64_t M
 		}
 		for (int q = 0; q < d; q++) F[j * d + q] = res[q];
 	}
+}
+
+/* ShiftedCCPotential_S2_MultiConfig (CollectiveCoordinatePotential.h:1756-1901): GetRho_MultiConfig (:1777-1808,
+ * rho of every sub-configuration of Np = N / nsub consecutive particles, Snow = sum |rho_c|^2 / (Np nsub)),
+ * Energy (:1818-1844) and Force (:1849-1879) of every particle.  re / im: nsub x M.  Returns the energy. */
+double orc_s2_multiconfig(int d, int64_t N, const double *s, const double *a, int64_t M, const double *k,
+                          const double *v, const double *s0, int64_t nsub, double shift, double *re, double *im,
+                          double *F, int nthreads)
+{
+	const int64_t np = N / nsub;
+	double *snow = (double *)calloc((size_t)(M > 0 ? M : 1), sizeof(double));
+	for (int64_t c = 0; c < nsub; c++) {
+		orc_rho(d, np, s + c * np * d, a, M, k, re + c * M, im + c * M, nthreads);
+		for (int64_t i = 0; i < M; i++) snow[i] += re[c * M + i] * re[c * M + i] + im[c * M + i] * im[c * M + i];
+	}
+	for (int64_t i = 0; i < M; i++) snow[i] /= (double)(np * nsub);
+	double result = 0.0;
+	for (int64_t i = 0; i < M; i++) {
+		double ds = snow[i] - s0[i];
+		result += v[i] * ds * ds;
+	}
+	result += shift;
+	if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+	for (int64_t j = 0; j < N; j++) {
+		const int64_t inc = j / np;
+		double r4[ORC_MAXDIM], res[ORC_MAXDIM] = {0, 0, 0, 0};
+		orc_cart(d, a, s + j * d, r4);
+		for (int64_t i = 0; i < M; i++) {
+			double k4[ORC_MAXDIM];
+			orc_pad(d, k + i * d, k4);
+			double ds = snow[i] - s0[i];
+			double c = (4.0 / (double)np) * v[i] * ds *
+			           (im[i + inc * M] * cos(gv_dot(k4, r4)) - re[i + inc * M] * sin(gv_dot(k4, r4)));
+			for (int q = 0; q < ORC_MAXDIM; q++) res[q] -= c * k4[q];
+		}
+		for (int q = 0; q < d; q++) F[j * d + q] = res[q] * (1.0 / (double)nsub);
+	}
+	free(snow);
+	return result;
 }
 
 /* ------------------------------------------------------------------------------------------------

@@ -62,6 +62,9 @@ def lib():
         L.orc_multi_energy_forces.argtypes = [C.c_int, C.c_int64, _dp, _dp, C.c_int, _i64p, _i64p, _i64p, _dp, _dp,
                                               _dp, C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp, C.c_int]
         L.orc_multi_energy_forces.restype = C.c_double
+        L.orc_s2_multiconfig.argtypes = [C.c_int, C.c_int64, _dp, _dp, C.c_int64, _dp, _dp, _dp, C.c_int64,
+                                         C.c_double, _dp, _dp, _dp, C.c_int]
+        L.orc_s2_multiconfig.restype = C.c_double
         L.orc_rho_weighted.argtypes = [C.c_int, C.c_int64, _dp, _dp, C.c_int64, _dp, _dp, _dp, _dp, C.c_int]
         L.orc_s2_energy.argtypes = [C.c_int64, _dp, _dp, _dp, _dp, C.c_int64, C.c_double]
         L.orc_s2_energy.restype = C.c_double
@@ -188,6 +191,20 @@ def weighted_forces(s, a, k, v, w, re, im, nthreads=0):
     """ShiftedCCPotential_withWeight::Force for every particle: the plain form's force over the weighted rho,
     times the particle's weight (CollectiveCoordinatePotential.h:1555-1577)."""
     return forces(s, a, k, v, re, im, nthreads) * _c(w)[:, None]
+
+
+def s2_multiconfig(s, a, k, v, s0, nsub, shift=0.0, nthreads=0):
+    """ShiftedCCPotential_S2_MultiConfig: (E, F, re nsub x M, im nsub x M)."""
+    s, a, k, v, s0 = _c(s), _c(a), _c(k), _c(v), _c(s0)
+    N, d = s.shape
+    M = k.shape[0]
+    re = np.zeros(max(nsub * M, 1))
+    im = np.zeros(max(nsub * M, 1))
+    F = np.zeros((max(N, 1), d))
+    E = lib().orc_s2_multiconfig(d, N, _p(s), _p(a), M, _p(k if M else np.zeros((1, d))),
+                                 _p(v if M else np.zeros(1)), _p(s0 if M else np.zeros(1)), int(nsub), shift,
+                                 _p(re), _p(im), _p(F), nthreads or lib().orc_max_threads())
+    return E, F[:N], re[: nsub * M].reshape(nsub, M), im[: nsub * M].reshape(nsub, M)
 
 
 def s2_energy(v, s0, re, im, N, shift=0.0):

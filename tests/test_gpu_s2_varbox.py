@@ -336,3 +336,54 @@ def test_log_form_matches_oracle_and_relaxes():
     re, im = po.rho(c1.relative, c1.basis, k)
     assert H == pytest.approx(np.log(po.energy(v, re, im, po.volume(c1.basis), 0.5)), rel=1e-10)
     assert H < E and ev > 0
+
+
+# ---------------------------------------------------------------------------------------------------------------
+# sub-configurations (ShiftedCCPotential_S2_MultiConfig)
+# ---------------------------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("nsub,Np,d,chi,seed,general", [(4, 300, 2, 0.3, 1, False), (3, 500, 3, 0.35, 2, False),
+                                                        (5, 120, 2, 0.3, 3, True), (1, 400, 2, 0.3, 4, False)])
+def test_s2_multiconfig_matches_oracle(nsub, Np, d, chi, seed, general):
+    from modules_amd import ShiftedCCPotential_S2_MultiConfig
+    N = nsub * Np
+    cfg = RandomConfiguration(seed, N, d)
+    rng = np.random.default_rng(seed)
+    k = rng.normal(size=(70, d)) * 1.5 if general else GetKs(cfg, getK(chi, 1.0, d))
+    v = 0.5 + rng.random(len(k))
+    s0 = rng.random(len(k)) * 2
+    pot = ShiftedCCPotential_S2_MultiConfig(d, nsub)
+    for kk, vv, ss in zip(k, v, s0):
+        pot.AddConstraint(kk, [vv, ss])
+    pot.Shift = 0.3
+    pot.SetConfiguration(cfg)
+    assert pot.UsesLatticePath() == (not general)
+    pot.GetRho()
+    E, F = pot.EnergyAndAllForce()
+    Eo, Fo, re, im = po.s2_multiconfig(cfg.relative, cfg.basis, k, v, s0, nsub, 0.3)
+    assert np.max(np.abs(pot.RhoReal.reshape(nsub, -1) - re)) <= RTOL * Np
+    assert np.max(np.abs(pot.RhoImag.reshape(nsub, -1) - im)) <= RTOL * Np
+    snow = (re ** 2 + im ** 2).sum(axis=0) / N
+    assert np.allclose(pot.Snow, snow, rtol=1e-12, atol=1e-12)
+    _, se, sf = s2_scales(k, v, s0, Np)
+    assert abs(E - Eo) <= RTOL * se
+    assert np.max(np.abs(F - Fo)) <= RTOL * sf
+    for i in (0, N // 2, N - 1):
+        assert np.allclose(pot.Force(i), Fo[i], rtol=0, atol=RTOL * sf)
+
+
+def test_s2_multiconfig_requires_a_multiple_and_relaxes():
+    from modules_amd import ShiftedCCPotential_S2_MultiConfig
+    pot = ShiftedCCPotential_S2_MultiConfig(2, 3)
+    cfg = RandomConfiguration(6, 150, 2)
+    k = GetKs(cfg, getK(0.25, 1.0, 2))
+    for kk in k:
+        pot.AddConstraint(kk, [1.0, 0.5])
+    with pytest.raises(CCGPUError):
+        pot.SetConfiguration(RandomConfiguration(6, 151, 2))
+    pot.SetConfiguration(cfg)
+    E0 = pot.Energy()
+    c1 = cfg.copy()
+    E1, ev, _ = RelaxStructure_NLOPT(c1, pot, 0.0, 0, 0.0, 1500)
+    Eo = po.s2_multiconfig(c1.relative, c1.basis, k, np.ones(len(k)), np.full(len(k), 0.5), 3)[0]
+    assert E1 == pytest.approx(Eo, rel=1e-10, abs=1e-14)
+    assert E1 < 0.01 * E0 and ev > 0

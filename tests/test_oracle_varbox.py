@@ -217,3 +217,35 @@ def test_weighted_rho_pins():
             Ep = po.energy(v, *po.rho_weighted(sp, a, k, wr), vol)
             Em = po.energy(v, *po.rho_weighted(sm, a, k, wr), vol)
             assert abs(-(Ep - Em) / (2 * h) - F[j, c]) < 1e-6 * max(1.0, abs(F[j, c]))
+
+
+def test_s2_multiconfig_pins():
+    """ShiftedCCPotential_S2_MultiConfig: one sub-configuration is the plain S2 form; C identical sub-configurations
+    give the S2 energy of one of them with forces divided by C (the mean S equals each S); forces are minus the
+    energy gradient."""
+    rng = np.random.default_rng(9)
+    a = np.array([[6.0, 0.0], [0.9, 5.5]])
+    s = rng.random((8, 2))
+    k, _ = po.get_ks(a, 3.0)
+    v = 0.5 + rng.random(len(k))
+    s0 = rng.random(len(k))
+    E1, F1, re, im = po.s2_multiconfig(s, a, k, v, s0, 1, 0.2)
+    r0, i0 = po.rho(s, a, k)
+    assert E1 == pytest.approx(po.s2_energy(v, s0, r0, i0, 8, 0.2), rel=1e-14)
+    assert np.allclose(F1, po.s2_forces(s, a, k, v, s0, r0, i0), rtol=1e-13, atol=1e-15)
+    E3, F3, re3, im3 = po.s2_multiconfig(np.vstack([s, s, s]), a, k, v, s0, 3, 0.2)
+    assert E3 == pytest.approx(E1, rel=1e-12)
+    assert np.allclose(F3, np.vstack([F1, F1, F1]) / 3, rtol=1e-12, atol=1e-15)
+    s6 = rng.random((12, 2))
+    E, F, _, _ = po.s2_multiconfig(s6, a, k, v, s0, 2)
+    inv = np.linalg.inv(a)
+    h = 1e-6
+    for j in range(12):
+        for c in range(2):
+            dr = np.zeros(2)
+            dr[c] = h
+            sp, sm = s6.copy(), s6.copy()
+            sp[j] += dr @ inv
+            sm[j] -= dr @ inv
+            fd = -(po.s2_multiconfig(sp, a, k, v, s0, 2)[0] - po.s2_multiconfig(sm, a, k, v, s0, 2)[0]) / (2 * h)
+            assert abs(fd - F[j, c]) < 1e-6 * max(1.0, abs(F[j, c]))
