@@ -74,6 +74,18 @@ int ccgpu_potential_num_constraints(const ccgpu_potential *p, int64_t *M);
  * ccgpu_potential_add_constraint / set_constraints return CCGPU_ERR_ARG on it (the reference's AddConstraint
  * throws std::invalid_argument when vals lacks S0, :1663-1668). */
 int ccgpu_potential_create_s2(int dim, ccgpu_potential **out);
+/* ShiftedCCPotential_ForMCMixin::TryMove / AcceptMove (CollectiveCoordinatePotential.h:1906-1958; the
+ * forMCPotential interface of potential/Potential.h:108-120 used by potential/Move.h:96-125, 208-233): the
+ * energy change of moving one particle from prev_cart to after_cart (Cartesian), from rho'_k = rho_k -
+ * exp(i k.r_prev) + exp(i k.r_after) on the GPU (O(M)), for a plain or target-S(k) potential without soft core,
+ * weights or log form.  AcceptMove adopts rho' and its energy (Energy and GetRho return them, as the reference's
+ * cached rho).  i is the moved particle's index, so that accepting also moves it in the bound configuration
+ * (forces then recompute rho from the positions); with i = -1 (the reference's signature carries no index) the
+ * positions are left as bound and a force call returns CCGPU_ERR_STATE until the configuration is bound again. */
+int ccgpu_potential_try_move(ccgpu_potential *p, int64_t i, const double *prev_cart, const double *after_cart,
+                             double *dE);
+int ccgpu_potential_accept_move(ccgpu_potential *p);
+
 /* ShiftedCCPotential_S2_MultiConfig(d, NSubConfig) -- CollectiveCoordinatePotential.h:1756-1901: the bound
  * configuration is NSubConfig sub-configurations of Np = N / NSubConfig consecutive particles (N must be a
  * multiple: CCGPU_ERR_ARG otherwise), and

@@ -6,6 +6,7 @@ reference's Potential / StructureOptimization interface over that ABI.
 """
 from .potential import (Configuration, GetKs, GetUnitCubicBox, KPointConstraint, RandomConfiguration,  # noqa: F401
                         LogCCPotential, RepulsiveCCPotential, ShiftedCCPotential, ShiftedCCPotential_S2,
-                        ShiftedCCPotential_S2_MultiConfig, ShiftedCCPotential_withWeight, get_chi, getK, pi)
+                        ShiftedCCPotential_S2_MultiConfig, ShiftedCCPotential_withWeight,
+                        ShiftedCCPotential_ForMCMixin, get_chi, getK, pi)
 from .multi import MultiStealthyPotential, SpherePacking  # noqa: F401
 from .varbox import ShiftedCCPotential_varBox, VarBoxPotential  # noqa: F401

@@ -67,6 +67,8 @@ SIGNATURES = {
                                         C.POINTER(C.c_int)]),
     "ccgpu_potential_set_weights": (C.c_int, [vp, C.c_int64, dp]),
     "ccgpu_potential_set_log_form": (C.c_int, [vp, C.c_int]),
+    "ccgpu_potential_try_move": (C.c_int, [vp, C.c_int64, dp, dp, dp]),
+    "ccgpu_potential_accept_move": (C.c_int, [vp]),
     "ccgpu_potential_create_s2": (C.c_int, [C.c_int, C.POINTER(vp)]),
     "ccgpu_potential_create_s2_multiconfig": (C.c_int, [C.c_int, C.c_int64, C.POINTER(vp)]),
     "ccgpu_potential_add_constraint_s2": (C.c_int, [vp, dp, C.c_double, C.c_double]),

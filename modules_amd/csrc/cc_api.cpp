@@ -34,7 +34,21 @@ struct ccgpu_potential {
     {
         rho_valid = false;
         forces_valid = false;
+        coef_stale = false;
+        positions_stale = false;
+        mc_pending = false;
     }
+    // after an accepted Monte Carlo move rho is the incrementally updated one, as in the reference, while the force
+    // coefficients are not: the next force pass recomputes rho from the (updated) positions first
+    bool coef_stale = false;
+    bool positions_stale = false;  // a move was accepted without its particle index: positions must be bound again
+    // Monte Carlo trial state (ccgpu_potential_try_move / accept_move)
+    DevArray<double> mc_rho, mc_epart, mc_buf;  // trial rho (2 M), block partials, move (16) + trial energy (1)
+    DevArray<WorkItem> mc_items;
+    int64_t mc_nitems = -1;
+    bool mc_pending = false;
+    int64_t mc_i = -1;
+    double mc_after[4] = {0, 0, 0, 0};
     void EnsureBuilt()
     {
         Require(bound, CCGPU_ERR_STATE, "no configuration bound: call SetConfiguration first");
@@ -97,8 +111,11 @@ struct ccgpu_potential {
     void EnsureForces()
     {
         EnsureBuilt();
+        Require(!positions_stale, CCGPU_ERR_STATE,
+                "a Monte Carlo move was accepted without its particle index: bind the configuration again");
         if (!forces_valid) {
-            batch.Enqueue(!rho_valid, true);
+            batch.Enqueue(!rho_valid || coef_stale, true);
+            coef_stale = false;
             rho_valid = true;
             forces.resize((size_t)sys.N * sys.d);
             if (!forces.empty())
@@ -557,6 +574,90 @@ int ccgpu_potential_relax(ccgpu_potential *p, double *relative, int maxstep, dou
         *energy = E;
         if (evaluations) *evaluations = total;
         if (status) *status = last;
+    });
+}
+
+int ccgpu_potential_try_move(ccgpu_potential *p, int64_t i, const double *prev_cart, const double *after_cart,
+                             double *dE)
+{
+    return Guard([&] {
+        Require(p && prev_cart && after_cart && dE, CCGPU_ERR_ARG, "null argument");
+        System &S = p->sys;
+        Require(!S.mixture && !S.softcore() && S.weights.empty() && !S.log_form, CCGPU_ERR_STATE,
+                "trial moves need a plain or target-S(k) single-configuration potential without soft core");
+        Require(i >= -1 && i < S.N, CCGPU_ERR_RANGE, "particle index out of range");
+        const int d = S.d;
+        for (int c = 0; c < d; c++)
+            Require(std::isfinite(prev_cart[c]) && std::isfinite(after_cart[c]), CCGPU_ERR_NAN, "non-finite position");
+        p->EnsureRho();
+        const double E_prev = p->ReadEnergy();
+        // relative coordinates s_c = r . b_c / (2 pi) (b_c . a_j = 2 pi delta_cj)
+        double mv[17] = {0};
+        for (int c = 0; c < d; c++) {
+            double sp = 0.0, sa = 0.0;
+            for (int q = 0; q < d; q++) {
+                sp += prev_cart[q] * S.b[c * d + q];
+                sa += after_cart[q] * S.b[c * d + q];
+            }
+            mv[c] = sp / (2.0 * kPi);
+            mv[4 + c] = sa / (2.0 * kPi);
+            mv[8 + c] = prev_cart[c];
+            mv[12 + c] = after_cart[c];
+        }
+        const int64_t M = S.M, nslots = S.desc.nslots;
+        const int64_t nb = (nslots + kBlock - 1) / kBlock;
+        p->mc_rho.Reserve(std::max<int64_t>(2 * M, 1));
+        p->mc_epart.Reserve(std::max<int64_t>(nb, 1));
+        p->mc_buf.Reserve(17);
+        if (p->mc_nitems != nb) {
+            std::vector<WorkItem> it((size_t)std::max<int64_t>(nb, 1));
+            for (int64_t b = 0; b < nb; b++) it[(size_t)b] = WorkItem{0, (int32_t)b, 0, 0};
+            p->mc_items.Reserve(it.size());
+            CC_HIP(hipMemcpyAsync(p->mc_items.get(), it.data(), it.size() * sizeof(WorkItem), hipMemcpyHostToDevice,
+                                  p->batch.stream));
+            p->mc_nitems = nb;
+        }
+        CC_HIP(hipMemcpyAsync(p->mc_buf.get(), mv, 16 * sizeof(double), hipMemcpyHostToDevice, p->batch.stream));
+        launch_mc_trial(p->batch.stream, p->batch.d_desc.get(), p->mc_items.get(), (int)nb, p->mc_buf.get(),
+                        p->mc_rho.get(), p->mc_epart.get(), 0, p->mc_buf.get() + 16);
+        double Et = 0.0;
+        CC_HIP(hipMemcpyAsync(&Et, p->mc_buf.get() + 16, sizeof(double), hipMemcpyDeviceToHost, p->batch.stream));
+        p->batch.Sync();
+        p->mc_pending = true;
+        p->mc_i = i;
+        for (int c = 0; c < 4; c++) p->mc_after[c] = c < d ? mv[4 + c] : 0.0;
+        *dE = Et - E_prev;
+    });
+}
+
+int ccgpu_potential_accept_move(ccgpu_potential *p)
+{
+    return Guard([&] {
+        Require(p != nullptr, CCGPU_ERR_ARG, "null handle");
+        Require(p->mc_pending, CCGPU_ERR_STATE, "no trial move to accept");
+        System &S = p->sys;
+        const int64_t M = S.M;
+        if (M > 0)
+            CC_HIP(hipMemcpyAsync(S.rho.get(), p->mc_rho.get(), 2 * M * sizeof(double), hipMemcpyDeviceToDevice,
+                                  p->batch.stream));
+        CC_HIP(hipMemcpyAsync(S.energy.get(), p->mc_buf.get() + 16, sizeof(double), hipMemcpyDeviceToDevice,
+                              p->batch.stream));
+        CC_HIP(hipMemcpyAsync(S.energy.get() + 1, p->mc_buf.get() + 16, sizeof(double), hipMemcpyDeviceToDevice,
+                              p->batch.stream));
+        if (p->mc_i >= 0) {
+            // the moved particle's new relative coordinates (wrapped into [0, 1), as Configuration::MoveParticle)
+            double sw[4];
+            for (int c = 0; c < S.d; c++) sw[c] = WrapUnit(p->mc_after[c]);
+            CC_HIP(hipMemcpyAsync(S.x.get() + p->mc_i * S.d, sw, S.d * sizeof(double), hipMemcpyHostToDevice,
+                                  p->batch.stream));
+        } else {
+            p->positions_stale = true;
+        }
+        p->batch.Sync();
+        p->mc_pending = false;
+        p->rho_valid = true;
+        p->forces_valid = false;
+        p->coef_stale = true;
     });
 }
 

@@ -474,6 +474,75 @@ __global__ void __launch_bounds__(kBlock) k_energy_final(const SysDesc *__restri
 }
 
 // ---------------------------------------------------------------------------------------------
+// Monte Carlo trial move (ShiftedCCPotential_ForMCMixin::TryMove, CollectiveCoordinatePotential.h:1925-1948):
+// rho'_k = rho_k - exp(i k.r_prev) + exp(i k.r_after) for every constraint, written in constraint order to
+// rho_trial, with the energy partial sum of rho' per slot block (V |rho'|^2, or V (|rho'|^2 / N - S0)^2).
+// mv: s_prev, s_after (relative, 4 each), r_prev, r_after (Cartesian, 4 each).  items: (sys, slot block)
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_mc_trial(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items,
+                                                     const double *__restrict__ mv, double *__restrict__ rho_trial,
+                                                     double *__restrict__ epart)
+{
+    __shared__ double lds[kBlock];
+    const WorkItem it = items[blockIdx.x];
+    const SysDesc &S = sys[it.sys];
+    const int q = it.t0 * kBlock + threadIdx.x;
+    double e = 0.0;
+    if (q < S.nslots) {
+        int kidx = q;
+        double sp, cp, sa, ca;
+        if (S.flags & kSysLattice) {
+            kidx = S.slot_k[q];
+            const int segl = S.nslots / S.nseg;
+            const int seg = q / segl, t = q - seg * segl;
+            const int4 h = S.seg_head[seg];
+            double n[4] = {(double)h.x, (double)h.y, (double)h.z, (double)h.w};
+            n[S.d - 1] += (double)t;  // a segment runs along the last reciprocal axis
+            double up = 0.0, ua = 0.0;
+            for (int c = 0; c < 4; c++) {
+                up = __builtin_fma(n[c], mv[c], up);
+                ua = __builtin_fma(n[c], mv[4 + c], ua);
+            }
+            sincos_rev(up, sp, cp);
+            sincos_rev(ua, sa, ca);
+        } else {
+            const double4 k = S.kvec[q];
+            const double tp = k.x * mv[8] + k.y * mv[9] + k.z * mv[10] + k.w * mv[11];
+            const double ta = k.x * mv[12] + k.y * mv[13] + k.z * mv[14] + k.w * mv[15];
+            sincos_rad(tp, sp, cp);
+            sincos_rad(ta, sa, ca);
+        }
+        if (kidx >= 0) {
+            const double re = S.rho[kidx] - cp + ca, im = S.rho[S.M + kidx] - sp + sa;
+            rho_trial[kidx] = re;
+            rho_trial[S.M + kidx] = im;
+            const double v = S.slot_v[q];
+            if (S.flags & kSysS2) {
+                const double ds = (re * re + im * im) * S.inv_n - S.slot_s0[q];
+                e = v * ds * ds;
+            } else {
+                e = v * (re * re + im * im);
+            }
+        }
+    }
+    const double tot = block_sum(e, lds);
+    if (threadIdx.x == 0) epart[it.t0] = tot;
+}
+
+// energy of the trial rho: sum of the block partials, / vol (plain form), + Shift.  One block.
+__global__ void __launch_bounds__(kBlock) k_mc_final(const SysDesc *__restrict__ sys, int sidx,
+                                                     const double *__restrict__ epart, double *__restrict__ out)
+{
+    __shared__ double lds[kBlock];
+    const SysDesc &S = sys[sidx];
+    const int nb = (S.nslots + kBlock - 1) / kBlock;
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kBlock) acc += epart[b];
+    const double tot = block_sum(acc, lds);
+    if (threadIdx.x == 0) out[0] = ((S.flags & kSysS2) ? tot : tot / S.vol) + S.shift;
+}
+
+// ---------------------------------------------------------------------------------------------
 // sub-configuration coefficients (ShiftedCCPotential_S2_MultiConfig, CollectiveCoordinatePotential.h:1756-1901):
 // for each slot the mean structure factor over the C sub-configurations (the mixture's children, Np particles
 // each, rho already reduced into each child's rho in constraint order), S = sum_c |rho_c|^2 / (Np C), the energy
@@ -916,6 +985,13 @@ void launch_energy_final(hipStream_t stream, const SysDesc *sys, const WorkItem 
 void launch_multiconfig_coef(hipStream_t stream, const SysDesc *sys, const WorkItem *items, int n)
 {
     if (n > 0) hipLaunchKernelGGL(k_multiconfig_coef, CC_GRID(n), sys, items);
+}
+
+void launch_mc_trial(hipStream_t stream, const SysDesc *sys, const WorkItem *items, int n, const double *mv,
+                     double *rho_trial, double *epart, int sidx, double *energy_out)
+{
+    if (n > 0) hipLaunchKernelGGL(k_mc_trial, CC_GRID(n), sys, items, mv, rho_trial, epart);
+    hipLaunchKernelGGL(k_mc_final, dim3(1), dim3(kBlock), 0, stream, sys, sidx, epart, energy_out);
 }
 
 void launch_mixture_final(hipStream_t stream, const SysDesc *sys, const WorkItem *items, int n)

@@ -301,6 +301,9 @@ void launch_rho_reduce(hipStream_t, const SysDesc *, const WorkItem *, int);
 void launch_energy_final(hipStream_t, const SysDesc *, const WorkItem *, int);
 void launch_mixture_final(hipStream_t, const SysDesc *, const WorkItem *, int);
 void launch_multiconfig_coef(hipStream_t, const SysDesc *, const WorkItem *, int);
+// Monte Carlo trial move of one system (descriptor sidx): rho_trial (2 M), epart (slot blocks), energy_out (1)
+void launch_mc_trial(hipStream_t, const SysDesc *, const WorkItem *, int, const double *mv, double *rho_trial,
+                     double *epart, int sidx, double *energy_out);
 void launch_force_lattice(hipStream_t, int seg, const SysDesc *, const WorkItem *, int, int ppl);
 void launch_force_general(hipStream_t, const SysDesc *, const WorkItem *, int);
 void launch_force_final(hipStream_t, const SysDesc *, const WorkItem *, int);

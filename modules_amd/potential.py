@@ -476,6 +476,31 @@ class LogCCPotential(ShiftedCCPotential):
         check(lib().ccgpu_potential_set_log_form(self._h, 1))
 
 
+class ShiftedCCPotential_ForMCMixin(ShiftedCCPotential):
+    """A plain potential with Monte Carlo single-particle trial moves (ShiftedCCPotential_ForMCMixin,
+    CollectiveCoordinatePotential.h:1906-1958): TryMove(prevCart, afterCart[, i]) returns the energy change from the
+    incrementally updated rho (on the GPU), AcceptMove adopts it.  Passing the particle index i also moves it in the
+    bound configuration."""
+
+    def __init__(self, src):
+        self._S2 = getattr(src, "_S2", False)
+        super().__init__(src.Dimension)
+        _copy_terms(self, src)
+
+    def TryMove(self, prevCart, afterCart, i=-1):
+        self._require()
+        dE = C.c_double(0)
+        pc = np.ascontiguousarray(prevCart, dtype=np.float64)[: self.Dimension]
+        ac = np.ascontiguousarray(afterCart, dtype=np.float64)[: self.Dimension]
+        check(lib().ccgpu_potential_try_move(self._h, int(i), _p(pc), _p(ac), C.byref(dE)))
+        return dE.value
+
+    def AcceptMove(self):
+        """Adopt the trial rho (the caller moves the particle in its Configuration, as potential/Move.h:224-233)."""
+        check(lib().ccgpu_potential_accept_move(self._h))
+        self.RhoValid = False
+
+
 class RepulsiveCCPotential(ShiftedCCPotential):
     """Soft core e0 (1 - r/sigma)^2 for r < sigma plus the CC potential
     (apps/CCOptimization/RepulsiveCCPotential.h:21-157).  Constraints go to .CCPotential, as in the reference."""

@@ -387,3 +387,63 @@ def test_s2_multiconfig_requires_a_multiple_and_relaxes():
     Eo = po.s2_multiconfig(c1.relative, c1.basis, k, np.ones(len(k)), np.full(len(k), 0.5), 3)[0]
     assert E1 == pytest.approx(Eo, rel=1e-10, abs=1e-14)
     assert E1 < 0.01 * E0 and ev > 0
+
+
+# ---------------------------------------------------------------------------------------------------------------
+# Monte Carlo trial moves (ShiftedCCPotential_ForMCMixin)
+# ---------------------------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("d,N,chi,s2,general", [(2, 300, 0.3, False, False), (3, 400, 0.35, False, False),
+                                                (2, 200, 0.3, True, False), (2, 150, 0.3, False, True)])
+def test_mc_moves_match_recomputed_energies(d, N, chi, s2, general):
+    from modules_amd import ShiftedCCPotential, ShiftedCCPotential_ForMCMixin
+    cfg = RandomConfiguration(40 + d, N, d)
+    rng = np.random.default_rng(d + N)
+    k = rng.normal(size=(60, d)) * 1.4 if general else GetKs(cfg, getK(chi, 1.0, d))
+    v = 0.5 + rng.random(len(k))
+    t = rng.random(len(k))
+    base = ShiftedCCPotential_S2(d) if s2 else ShiftedCCPotential(d)
+    for i in range(len(k)):
+        base.AddConstraint(k[i], [v[i], t[i]] if s2 else [v[i]])
+    base.Shift = 0.25
+    pot = ShiftedCCPotential_ForMCMixin(base)
+    pot.SetConfiguration(cfg)
+
+    def oracle_E(c):
+        re, im = po.rho(c.relative, c.basis, k)
+        if s2:
+            return po.s2_energy(v, t, re, im, N, 0.25), re, im
+        return po.energy(v, re, im, po.volume(c.basis), 0.25), re, im
+
+    E = pot.Energy()
+    accepted = 0
+    for step in range(40):
+        i = int(rng.integers(N))
+        prev = cfg.GetCartesianCoordinates(i).copy()
+        new_rel = cfg.relative[i] + 0.05 * rng.normal(size=d)
+        trial = cfg.copy()
+        trial.MoveParticle(i, new_rel)
+        after = trial.GetCartesianCoordinates(i)
+        dE = pot.TryMove(prev, after, i if step % 2 == 0 else -1)
+        Et = oracle_E(trial)[0]
+        assert dE == pytest.approx(Et - E, rel=1e-9, abs=1e-10 * max(1.0, abs(E)))
+        if dE < 0 or rng.random() < 0.3:
+            pot.AcceptMove()
+            cfg.MoveParticle(i, new_rel)
+            E = E + dE
+            accepted += 1
+            assert pot.Energy() == pytest.approx(E, rel=1e-12, abs=1e-14)
+            if step % 2 == 1:  # accepted without its index: bind again before forces
+                with pytest.raises(CCGPUError):
+                    pot.AllForce()
+                pot.SetConfiguration(cfg)
+                E = pot.Energy()
+    assert accepted > 5
+    Eo, re, im = oracle_E(cfg)
+    assert pot.Energy() == pytest.approx(Eo, rel=1e-10, abs=1e-12)
+    pot.GetRho()
+    assert np.max(np.abs(pot.RhoReal - re)) <= 1e-10 * N and np.max(np.abs(pot.RhoImag - im)) <= 1e-10 * N
+    # forces after the last accepted move (index given or configuration rebound) follow the moved particles
+    F = pot.AllForce()
+    Fo = po.s2_forces(cfg.relative, cfg.basis, k, v, t, re, im) if s2 else \
+        po.forces(cfg.relative, cfg.basis, k, v, re, im)
+    assert np.max(np.abs(F - Fo)) <= 1e-9 * max(1.0, np.max(np.abs(Fo)))
