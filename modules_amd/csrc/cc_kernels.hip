@@ -642,10 +642,21 @@ __device__ __forceinline__ ConstPtr<T> as_const(const T *p)
     return (ConstPtr<T>)p;
 }
 
+// Segments are staged through LDS in groups of kForceStage: heads, lengths and coefficients are loaded once per
+// block with vector loads and read back as wavefront-uniform LDS broadcasts.
+constexpr int kForceStage = 16;
+
+template <int SEG>
+struct ForceLds {
+    double coef[kForceStage * SEG * 2];  // FP64 coefficients, or FP32 ones in the first half (mixed)
+    int4 head[kForceStage];
+    int len[kForceStage];
+};
+
 template <int SEG, int PPL, bool MIXED>
 __device__ __forceinline__ void force_lattice_body(const SysDesc &S, const WorkItem &it, const double2 *ltab,
                                                    const float2 *ltab32, const double4 (&p)[PPL],
-                                                   const double2 (&st)[PPL], double (&g)[PPL][4])
+                                                   const double2 (&st)[PPL], double (&g)[PPL][4], ForceLds<SEG> &L)
 {
     using Real = typename std::conditional<MIXED, float, double>::type;
     const int s0 = it.t1 * S.kchunk_len;
@@ -661,87 +672,88 @@ __device__ __forceinline__ void force_lattice_body(const SysDesc &S, const WorkI
         st2x[q] = stx[q] * stx[q] - sty[q] * sty[q];
         st2y[q] = (Real)2 * stx[q] * sty[q];
     }
-    for (int sg = s0; sg < s1; sg++) {
-        const ConstPtr<int> hp = as_const(reinterpret_cast<const int *>(S.seg_head)) + 4 * (int64_t)sg;
-        const int4 h = make_int4(hp[0], hp[1], hp[2], hp[3]);
-        const int len = as_const(S.seg_len)[sg];
-        const double h0 = (double)h.x, h1 = (double)h.y, h2 = (double)h.z, h3 = (double)h.w;
-        const double hl = (d == 1) ? h0 : (d == 2) ? h1 : (d == 3) ? h2 : h3;
-        Real sn[PPL][2], cs[PPL][2], acc[PPL][2], accl[PPL][2];
-#pragma unroll
-        for (int q = 0; q < PPL; q++) {
-            const double u = __builtin_fma(h0, p[q].x, __builtin_fma(h1, p[q].y, __builtin_fma(h2, p[q].z, h3 * p[q].w)));
-            Real s_, c_;
-            if (MIXED) {
-                float fs, fc;
-                sincos_rev_tab32((float)(u - __builtin_rint(u)), ltab32, fs, fc);
-                s_ = fs;
-                c_ = fc;
-            } else {
-                double ds, dc;
-                sincos_rev_tab(u, ltab, ds, dc);
-                s_ = ds;
-                c_ = dc;
-            }
-            cs[q][0] = c_;
-            sn[q][0] = s_;
-            cs[q][1] = c_ * stx[q] - s_ * sty[q];
-            sn[q][1] = c_ * sty[q] + s_ * stx[q];
-            acc[q][0] = acc[q][1] = 0;
-            accl[q][0] = accl[q][1] = 0;
+    const Real *gcf = MIXED ? (const Real *)S.coef32 : (const Real *)S.coef;
+    Real *lcf = reinterpret_cast<Real *>(L.coef);
+    for (int sb = s0; sb < s1; sb += kForceStage) {
+        const int ns = min(kForceStage, s1 - sb);
+        __syncthreads();  // the previous group is consumed
+        for (int x = threadIdx.x; x < 2 * ns * SEG; x += kBlock) lcf[x] = gcf[2 * (int64_t)sb * SEG + x];
+        if (threadIdx.x < ns) {
+            L.head[threadIdx.x] = S.seg_head[sb + threadIdx.x];
+            L.len[threadIdx.x] = S.seg_len[sb + threadIdx.x];
         }
-        // coefficients of a group of 4 slots are scalar loads; the next group's are issued before the current
-        // group is computed, so the wait for them overlaps a group of FMAs (scalar loads complete out of order,
-        // so every wait is for all of them)
-        const ConstPtr<Real> cf = as_const(MIXED ? (const Real *)S.coef32 : (const Real *)S.coef) + 2 * (int64_t)sg * SEG;
-        Real cur[8], nxt[8];
+        __syncthreads();
+        for (int sl = 0; sl < ns; sl++) {
+            const int4 hv = L.head[sl];
+            const int h0i = __builtin_amdgcn_readfirstlane(hv.x), h1i = __builtin_amdgcn_readfirstlane(hv.y);
+            const int h2i = __builtin_amdgcn_readfirstlane(hv.z), h3i = __builtin_amdgcn_readfirstlane(hv.w);
+            const int len = __builtin_amdgcn_readfirstlane(L.len[sl]);
+            const double h0 = (double)h0i, h1 = (double)h1i, h2 = (double)h2i, h3 = (double)h3i;
+            const double hl = (d == 1) ? h0 : (d == 2) ? h1 : (d == 3) ? h2 : h3;
+            Real sn[PPL][2], cs[PPL][2], acc[PPL][2], accl[PPL][2];
 #pragma unroll
-        for (int u = 0; u < 8; u++) cur[u] = cf[u];
-#pragma unroll
-        for (int t0 = 0; t0 < SEG; t0 += 4) {
-            if (t0 < len) {  // segment data is uniform across the wavefront
-                if (t0 + 4 < SEG && t0 + 4 < len) {
-#pragma unroll
-                    for (int u = 0; u < 8; u++) nxt[u] = cf[2 * (t0 + 4) + u];
+            for (int q = 0; q < PPL; q++) {
+                const double u = __builtin_fma(h0, p[q].x, __builtin_fma(h1, p[q].y, __builtin_fma(h2, p[q].z, h3 * p[q].w)));
+                Real s_, c_;
+                if (MIXED) {
+                    float fs, fc;
+                    sincos_rev_tab32((float)(u - __builtin_rint(u)), ltab32, fs, fc);
+                    s_ = fs;
+                    c_ = fc;
+                } else {
+                    double ds, dc;
+                    sincos_rev_tab(u, ltab, ds, dc);
+                    s_ = ds;
+                    c_ = dc;
                 }
+                cs[q][0] = c_;
+                sn[q][0] = s_;
+                cs[q][1] = c_ * stx[q] - s_ * sty[q];
+                sn[q][1] = c_ * sty[q] + s_ * stx[q];
+                acc[q][0] = acc[q][1] = 0;
+                accl[q][0] = accl[q][1] = 0;
+            }
+            const Real *cf = lcf + 2 * sl * SEG;
 #pragma unroll
-                for (int t = t0; t < t0 + 4; t++) {
-                    const int ch = (t - t0) & 1;
-                    const Real ax = cur[2 * (t - t0)], ay = cur[2 * (t - t0) + 1];
+            for (int t0 = 0; t0 < SEG; t0 += 4) {
+                if (t0 < len) {  // uniform
 #pragma unroll
-                    for (int q = 0; q < PPL; q++) {
-                        const Real c = ax * cs[q][ch] - ay * sn[q][ch];
-                        acc[q][ch] += c;
-                        accl[q][ch] += (Real)t * c;  // t is a compile-time constant: one FMA
-                        const Real nc = cs[q][ch] * st2x[q] - sn[q][ch] * st2y[q];
-                        const Real ns = cs[q][ch] * st2y[q] + sn[q][ch] * st2x[q];
-                        cs[q][ch] = nc;
-                        sn[q][ch] = ns;
+                    for (int t = t0; t < t0 + 4; t++) {
+                        const int ch = (t - t0) & 1;
+                        const Real ax = cf[2 * t], ay = cf[2 * t + 1];
+#pragma unroll
+                        for (int q = 0; q < PPL; q++) {
+                            const Real c = ax * cs[q][ch] - ay * sn[q][ch];
+                            acc[q][ch] += c;
+                            accl[q][ch] += (Real)t * c;  // t is a compile-time constant: one FMA
+                            const Real nc = cs[q][ch] * st2x[q] - sn[q][ch] * st2y[q];
+                            const Real ns2 = cs[q][ch] * st2y[q] + sn[q][ch] * st2x[q];
+                            cs[q][ch] = nc;
+                            sn[q][ch] = ns2;
+                        }
                     }
                 }
-#pragma unroll
-                for (int u = 0; u < 8; u++) cur[u] = nxt[u];
             }
-        }
-        // sum_t (hl + t) c_t = hl * sum c + sum t c; components before the last axis are constant along the segment
+            // sum_t (hl + t) c_t = hl * sum c + sum t c; components before the last axis are constant along the segment
 #pragma unroll
-        for (int q = 0; q < PPL; q++) {
-            const double A = (double)(acc[q][0] + acc[q][1]);
-            const double Al = __builtin_fma(hl, A, (double)(accl[q][0] + accl[q][1]));
-            if (d == 1) {
-                g[q][0] += Al;
-            } else if (d == 2) {
-                g[q][0] = __builtin_fma(h0, A, g[q][0]);
-                g[q][1] += Al;
-            } else if (d == 3) {
-                g[q][0] = __builtin_fma(h0, A, g[q][0]);
-                g[q][1] = __builtin_fma(h1, A, g[q][1]);
-                g[q][2] += Al;
-            } else {
-                g[q][0] = __builtin_fma(h0, A, g[q][0]);
-                g[q][1] = __builtin_fma(h1, A, g[q][1]);
-                g[q][2] = __builtin_fma(h2, A, g[q][2]);
-                g[q][3] += Al;
+            for (int q = 0; q < PPL; q++) {
+                const double A = (double)(acc[q][0] + acc[q][1]);
+                const double Al = __builtin_fma(hl, A, (double)(accl[q][0] + accl[q][1]));
+                if (d == 1) {
+                    g[q][0] += Al;
+                } else if (d == 2) {
+                    g[q][0] = __builtin_fma(h0, A, g[q][0]);
+                    g[q][1] += Al;
+                } else if (d == 3) {
+                    g[q][0] = __builtin_fma(h0, A, g[q][0]);
+                    g[q][1] = __builtin_fma(h1, A, g[q][1]);
+                    g[q][2] += Al;
+                } else {
+                    g[q][0] = __builtin_fma(h0, A, g[q][0]);
+                    g[q][1] = __builtin_fma(h1, A, g[q][1]);
+                    g[q][2] = __builtin_fma(h2, A, g[q][2]);
+                    g[q][3] += Al;
+                }
             }
         }
     }
@@ -754,6 +766,7 @@ __global__ void __launch_bounds__(kBlock) k_force_lattice(const SysDesc *__restr
     // each wavefront-uniform coefficient load serves PPL particles
     __shared__ double2 ltab[64];
     __shared__ float2 ltab32[64];
+    __shared__ ForceLds<SEG> L;
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
     if (!sys_active(sys, S)) return;
@@ -775,8 +788,8 @@ __global__ void __launch_bounds__(kBlock) k_force_lattice(const SysDesc *__restr
     double g[PPL][4];
 #pragma unroll
     for (int q = 0; q < PPL; q++) g[q][0] = g[q][1] = g[q][2] = g[q][3] = 0.0;
-    if (S.flags & kSysMixed) force_lattice_body<SEG, PPL, true>(S, it, ltab, ltab32, p, st, g);
-    else force_lattice_body<SEG, PPL, false>(S, it, ltab, ltab32, p, st, g);
+    if (S.flags & kSysMixed) force_lattice_body<SEG, PPL, true>(S, it, ltab, ltab32, p, st, g, L);
+    else force_lattice_body<SEG, PPL, false>(S, it, ltab, ltab32, p, st, g, L);
 #pragma unroll
     for (int q = 0; q < PPL; q++)
         if (valid[q])
