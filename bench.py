@@ -27,7 +27,7 @@ FP64_VALU_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak (AMD data sheet)
 # HBM bytes per launch of the force kernel at the headline shape (64 x 3D N=2000, M=2724, tiles 24,0,128,32,1),
 # from rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by the library copy kernel's counter factors; the
 # passes, the commands and the factors are in profiles/counters_ef_lbfgs_batched64_3d_N2000_chi045.md
-FORCE_TRAFFIC_BYTES = {(3, 2000, 2724, 64): {"read": 57.6e6, "write": 24.2e6}}
+FORCE_TRAFFIC_BYTES = {(3, 2000, 2724, 64): {"read": 43.5e6, "write": 24.2e6}}
 
 
 def parse():
