@@ -323,6 +323,42 @@ int ccgpu_potential_relax(ccgpu_potential *p, double *relative, int maxstep, dou
                           int *evaluations, int *status);
 
 /* ---------------------------------------------------------------------------------------------
+ * Molecular dynamics: ParticleMolecularDynamics (potential/MD_System.h:23-131, MD_System.cpp).  The handle owns
+ * its configuration (cell, relative positions), velocities and accelerations, all resident on the GPU; each
+ * evolve call binds the given potential to that configuration (potential.SetConfiguration(Position)) and runs
+ * every step on the device: the velocity Verlet of Evolve (MD_System.cpp:135-170), v += dt/2 a, r += dt v
+ * (wrapped into the cell), a = F / m from the potential's force pass, v += dt/2 a.  After a call the potential
+ * stays bound to the final positions.  Random draws (SetRandomSpeed, AndersonEvolve) come from the caller's
+ * RandomDouble() stream: `draws` holds the next values of that stream and *used says how many were consumed, so
+ * the caller's generator advances as the reference's would.  The MDDump binary restart format is not provided.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct ccgpu_md ccgpu_md;
+/* ParticleMolecularDynamics(Position, TimeStep, Mass) (MD_System.cpp:17-29): cell basis (dim x dim), N relative
+ * positions, one mass per particle; velocities and accelerations start at zero. */
+int ccgpu_md_create(int dim, const double *basis, int64_t N, const double *relative, double timestep,
+                    const double *mass, ccgpu_md **out);
+int ccgpu_md_destroy(ccgpu_md *m);
+/* Velocity (N x dim) */
+int ccgpu_md_set_velocities(ccgpu_md *m, const double *v);
+/* Position (relative, N x dim), Velocity, Acceleration, Time, TimeStep and xi; any pointer may be null */
+int ccgpu_md_get_state(ccgpu_md *m, double *relative, double *velocity, double *acceleration, double *time,
+                       double *timestep, double *xi);
+int ccgpu_md_set_timestep(ccgpu_md *m, double timestep);
+/* GetKineticEnergy (MD_System.cpp:370-377) */
+int ccgpu_md_kinetic_energy(ccgpu_md *m, double *Ek);
+/* Evolve(repeat, potential) (:135-170); the accelerations are computed first if never computed (:80-87) */
+int ccgpu_md_evolve(ccgpu_md *m, ccgpu_potential *p, int64_t repeat);
+/* InitNoseHoover(Q) (:10-14) and NoseHooverEvolve(repeat, potential, kT) (:314-368) */
+int ccgpu_md_init_nose_hoover(ccgpu_md *m, double Q);
+int ccgpu_md_nose_hoover_evolve(ccgpu_md *m, ccgpu_potential *p, int64_t repeat, double kT);
+/* AndersonEvolve(repeat, potential, kT, ResetSpeedProbability, gen) (:203-252) */
+int ccgpu_md_anderson_evolve(ccgpu_md *m, ccgpu_potential *p, int64_t repeat, double kT, double prob,
+                             const double *draws, int64_t ndraws, int64_t *used);
+/* SetRandomSpeed(kT, gen) (:379-394) and RemoveOverallMomentum() (:395-407) */
+int ccgpu_md_set_random_speed(ccgpu_md *m, double kT, const double *draws, int64_t ndraws, int64_t *used);
+int ccgpu_md_remove_overall_momentum(ccgpu_md *m);
+
+/* ---------------------------------------------------------------------------------------------
  * Variable-cell potentials (potential/CCPotential_varBox.h, potential/CCPotential_Kernels.h).
  * A variable-cell potential is a ccgpu_potential whose constraints follow its cell: each SetConfiguration with
  * a new cell regenerates the reciprocal lattice points 0 < |k| < rK (one of each +-k pair: the first non-zero

@@ -69,6 +69,18 @@ SIGNATURES = {
     "ccgpu_potential_set_log_form": (C.c_int, [vp, C.c_int]),
     "ccgpu_potential_try_move": (C.c_int, [vp, C.c_int64, dp, dp, dp]),
     "ccgpu_potential_accept_move": (C.c_int, [vp]),
+    "ccgpu_md_create": (C.c_int, [C.c_int, dp, C.c_int64, dp, C.c_double, dp, C.POINTER(vp)]),
+    "ccgpu_md_destroy": (C.c_int, [vp]),
+    "ccgpu_md_set_velocities": (C.c_int, [vp, dp]),
+    "ccgpu_md_get_state": (C.c_int, [vp, dp, dp, dp, dp, dp, dp]),
+    "ccgpu_md_set_timestep": (C.c_int, [vp, C.c_double]),
+    "ccgpu_md_kinetic_energy": (C.c_int, [vp, dp]),
+    "ccgpu_md_evolve": (C.c_int, [vp, vp, C.c_int64]),
+    "ccgpu_md_init_nose_hoover": (C.c_int, [vp, C.c_double]),
+    "ccgpu_md_nose_hoover_evolve": (C.c_int, [vp, vp, C.c_int64, C.c_double]),
+    "ccgpu_md_anderson_evolve": (C.c_int, [vp, vp, C.c_int64, C.c_double, C.c_double, dp, C.c_int64, i64p]),
+    "ccgpu_md_set_random_speed": (C.c_int, [vp, C.c_double, dp, C.c_int64, i64p]),
+    "ccgpu_md_remove_overall_momentum": (C.c_int, [vp]),
     "ccgpu_potential_create_s2": (C.c_int, [C.c_int, C.POINTER(vp)]),
     "ccgpu_potential_create_s2_multiconfig": (C.c_int, [C.c_int, C.c_int64, C.POINTER(vp)]),
     "ccgpu_potential_add_constraint_s2": (C.c_int, [vp, dp, C.c_double, C.c_double]),

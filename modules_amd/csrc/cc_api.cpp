@@ -661,6 +661,320 @@ int ccgpu_potential_accept_move(ccgpu_potential *p)
     });
 }
 
+// ---------------------------------------------------------------------------------------------
+// molecular dynamics (ParticleMolecularDynamics, potential/MD_System.cpp) over a bound potential
+// ---------------------------------------------------------------------------------------------
+}  // extern "C"
+
+struct ccgpu_md {
+    int d = 0;
+    int64_t N = 0;
+    double basis[16] = {}, dt = 0, time = 0, Q = 0;
+    std::vector<double> mass, inv_mass;
+    DevArray<double> x, v, a, half, dmass, dinv, bt, state, resets;
+    hipStream_t stream = nullptr;  // the stream of the potential last evolved with (its batch stream)
+    bool accel = false;  // AccelerationInitialized
+
+    // potential.SetConfiguration(Position): bind the potential to this cell and N and give it the positions
+    void BindTo(ccgpu_potential *p)
+    {
+        Require(p->sys.d == d, CCGPU_ERR_ARG, "dimension mismatch");
+        Require(!p->sys.mixture, CCGPU_ERR_STATE, "molecular dynamics needs a single-configuration potential");
+        std::vector<double> zero((size_t)std::max<int64_t>(N * d, 1), 0.0);
+        p->Bind(basis, N, zero.data());
+        if (N > 0)
+            CC_HIP(hipMemcpyAsync(p->sys.x.get(), x.get(), (size_t)(N * d) * sizeof(double), hipMemcpyDeviceToDevice,
+                                  p->batch.stream));
+        stream = p->batch.stream;
+    }
+    // a = F / m at the current positions (_EnsureInitialAcceleration, MD_System.cpp:80-87)
+    void EnsureAcceleration(ccgpu_potential *p)
+    {
+        if (accel) return;
+        p->batch.Enqueue(true, true);
+        launch_md_accel(p->batch.stream, v.get(), a.get(), p->sys.F.get(), dinv.get(), N, d, dt, 0);
+        accel = true;
+    }
+    // one velocity Verlet step; mode 0: plain second half kick, 1: Nose-Hoover (kT), resets: Anderson entries
+    void Step(ccgpu_potential *p, int mode, double kT, const double *reset_table, int nreset)
+    {
+        hipStream_t st = p->batch.stream;
+        launch_md_kick_drift(st, p->sys.x.get(), v.get(), a.get(), mode == 1 ? half.get() : nullptr, N, d, dt, bt.get());
+        p->batch.Enqueue(true, true);
+        if (mode == 1) {
+            launch_md_accel(st, v.get(), a.get(), p->sys.F.get(), dinv.get(), N, d, dt, 0);
+            launch_md_nose_hoover(st, v.get(), a.get(), half.get(), dmass.get(), N, d, dt, kT, Q, state.get());
+        } else {
+            launch_md_accel(st, v.get(), a.get(), p->sys.F.get(), dinv.get(), N, d, dt, 1);
+        }
+        if (nreset > 0) launch_md_resets(st, v.get(), reset_table, nreset, d);
+        time += dt;
+    }
+    // the positions back into the MD; the potential stays bound to them, its energy and rho those of the final
+    // positions (its host force copy is not)
+    void Finish(ccgpu_potential *p)
+    {
+        if (N > 0)
+            CC_HIP(hipMemcpyAsync(x.get(), p->sys.x.get(), (size_t)(N * d) * sizeof(double), hipMemcpyDeviceToDevice,
+                                  p->batch.stream));
+        p->batch.Sync();
+        p->rho_valid = true;
+        p->forces_valid = false;
+    }
+    void Sync() { CC_HIP(stream ? hipStreamSynchronize(stream) : hipDeviceSynchronize()); }
+    void Download(const DevArray<double> &src, double *dst)
+    {
+        Sync();
+        if (N > 0) CC_HIP(hipMemcpy(dst, src.get(), (size_t)(N * d) * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    void Upload(DevArray<double> &dst, const double *src)
+    {
+        Sync();
+        if (N > 0) CC_HIP(hipMemcpy(dst.get(), src, (size_t)(N * d) * sizeof(double), hipMemcpyHostToDevice));
+    }
+    double Kinetic()  // GetKineticEnergy (MD_System.cpp:370-377)
+    {
+        std::vector<double> vv((size_t)(N * d));
+        Download(v, vv.data());
+        double r = 0;
+        for (int64_t i = 0; i < N; i++) {
+            double m2 = 0;
+            for (int c = 0; c < d; c++) m2 += vv[(size_t)(i * d + c)] * vv[(size_t)(i * d + c)];
+            r += m2 * mass[(size_t)i];
+        }
+        return r * 0.5;
+    }
+};
+
+extern "C" {
+
+int ccgpu_md_create(int dim, const double *basis, int64_t N, const double *relative, double timestep,
+                    const double *mass, ccgpu_md **out)
+{
+    return Guard([&] {
+        Require(out != nullptr, CCGPU_ERR_ARG, "null output handle");
+        *out = nullptr;
+        Require(dim >= 1 && dim <= kMaxDim && basis && N >= 0 && (N == 0 || (relative && mass)), CCGPU_ERR_ARG,
+                "bad argument");
+        Require(std::isfinite(timestep), CCGPU_ERR_ARG, "non-finite time step");
+        for (int i = 0; i < dim * dim; i++) Require(std::isfinite(basis[i]), CCGPU_ERR_ARG, "non-finite basis");
+        Require(CellVolume(dim, basis) > 0.0, CCGPU_ERR_ARG, "cell basis is singular");
+        for (int64_t i = 0; i < N * dim; i++)
+            Require(std::isfinite(relative[i]), CCGPU_ERR_NAN, "non-finite particle coordinate");
+        for (int64_t i = 0; i < N; i++) Require(mass[i] > 0 && std::isfinite(mass[i]), CCGPU_ERR_ARG, "bad mass");
+        RequireDevice();
+        std::unique_ptr<ccgpu_md> m(new ccgpu_md);
+        m->d = dim;
+        m->N = N;
+        m->dt = timestep;
+        std::copy(basis, basis + dim * dim, m->basis);
+        m->mass.assign(mass, mass + N);
+        m->inv_mass.resize((size_t)N);
+        for (int64_t i = 0; i < N; i++) m->inv_mass[(size_t)i] = 1.0 / mass[i];
+        const int64_t n = std::max<int64_t>(N * dim, 1);
+        for (DevArray<double> *b : {&m->x, &m->v, &m->a, &m->half}) b->Reserve(n);
+        m->dmass.Reserve(std::max<int64_t>(N, 1));
+        m->dinv.Reserve(std::max<int64_t>(N, 1));
+        m->bt.Reserve(16);
+        m->state.Reserve(1);
+        double b[16], bt[16] = {};
+        ReciprocalBasis(dim, basis, b);
+        for (int c = 0; c < dim; c++)
+            for (int q = 0; q < dim; q++) bt[c * dim + q] = b[c * dim + q] / (2.0 * kPi);
+        std::vector<double> zero((size_t)n, 0.0), s((size_t)n, 0.0);
+        for (int64_t i = 0; i < N * dim; i++) s[(size_t)i] = WrapUnit(relative[i]);
+        CC_HIP(hipMemcpy(m->x.get(), s.data(), n * sizeof(double), hipMemcpyHostToDevice));
+        CC_HIP(hipMemcpy(m->v.get(), zero.data(), n * sizeof(double), hipMemcpyHostToDevice));
+        CC_HIP(hipMemcpy(m->a.get(), zero.data(), n * sizeof(double), hipMemcpyHostToDevice));
+        if (N > 0) {
+            CC_HIP(hipMemcpy(m->dmass.get(), mass, N * sizeof(double), hipMemcpyHostToDevice));
+            CC_HIP(hipMemcpy(m->dinv.get(), m->inv_mass.data(), N * sizeof(double), hipMemcpyHostToDevice));
+        }
+        CC_HIP(hipMemcpy(m->bt.get(), bt, 16 * sizeof(double), hipMemcpyHostToDevice));
+        CC_HIP(hipMemcpy(m->state.get(), zero.data(), sizeof(double), hipMemcpyHostToDevice));
+        *out = m.release();
+    });
+}
+
+int ccgpu_md_destroy(ccgpu_md *m)
+{
+    return Guard([&] {
+        if (m) m->Sync();
+        delete m;
+    });
+}
+
+int ccgpu_md_set_velocities(ccgpu_md *m, const double *v)
+{
+    return Guard([&] {
+        Require(m && (m->N == 0 || v), CCGPU_ERR_ARG, "null argument");
+        m->Upload(m->v, v);
+    });
+}
+
+int ccgpu_md_get_state(ccgpu_md *m, double *relative, double *velocity, double *acceleration, double *time,
+                       double *timestep, double *xi)
+{
+    return Guard([&] {
+        Require(m != nullptr, CCGPU_ERR_ARG, "null handle");
+        if (relative) m->Download(m->x, relative);
+        if (velocity) m->Download(m->v, velocity);
+        if (acceleration) m->Download(m->a, acceleration);
+        if (time) *time = m->time;
+        if (timestep) *timestep = m->dt;
+        if (xi) {
+            m->Sync();
+            CC_HIP(hipMemcpy(xi, m->state.get(), sizeof(double), hipMemcpyDeviceToHost));
+        }
+    });
+}
+
+int ccgpu_md_set_timestep(ccgpu_md *m, double timestep)
+{
+    return Guard([&] {
+        Require(m && std::isfinite(timestep), CCGPU_ERR_ARG, "bad argument");
+        m->dt = timestep;
+    });
+}
+
+int ccgpu_md_kinetic_energy(ccgpu_md *m, double *Ek)
+{
+    return Guard([&] {
+        Require(m && Ek, CCGPU_ERR_ARG, "null argument");
+        *Ek = m->Kinetic();
+    });
+}
+
+int ccgpu_md_evolve(ccgpu_md *m, ccgpu_potential *p, int64_t repeat)
+{
+    return Guard([&] {
+        Require(m && p && repeat >= 0, CCGPU_ERR_ARG, "bad argument");
+        m->Sync();
+        m->BindTo(p);
+        m->EnsureAcceleration(p);
+        for (int64_t u = 0; u < repeat; u++) m->Step(p, 0, 0.0, nullptr, 0);
+        m->Finish(p);
+    });
+}
+
+int ccgpu_md_init_nose_hoover(ccgpu_md *m, double Q)
+{
+    return Guard([&] {
+        Require(m && std::isfinite(Q), CCGPU_ERR_ARG, "bad argument");
+        m->Q = Q;
+        m->Sync();
+        const double zero = 0.0;
+        CC_HIP(hipMemcpy(m->state.get(), &zero, sizeof(double), hipMemcpyHostToDevice));
+    });
+}
+
+int ccgpu_md_nose_hoover_evolve(ccgpu_md *m, ccgpu_potential *p, int64_t repeat, double kT)
+{
+    return Guard([&] {
+        Require(m && p && repeat >= 0 && std::isfinite(kT), CCGPU_ERR_ARG, "bad argument");
+        Require(m->Q != 0.0, CCGPU_ERR_STATE, "InitNoseHoover first");
+        m->Sync();
+        m->BindTo(p);
+        m->EnsureAcceleration(p);
+        for (int64_t u = 0; u < repeat; u++) m->Step(p, 1, kT, nullptr, 0);
+        m->Finish(p);
+    });
+}
+
+int ccgpu_md_anderson_evolve(ccgpu_md *m, ccgpu_potential *p, int64_t repeat, double kT, double prob,
+                             const double *draws, int64_t ndraws, int64_t *used)
+{
+    return Guard([&] {
+        Require(m && p && repeat >= 0 && std::isfinite(kT) && std::isfinite(prob) && used && (ndraws == 0 || draws),
+                CCGPU_ERR_ARG, "bad argument");
+        const int d = m->d;
+        const int64_t N = m->N;
+        // the resets of every step from the caller's RandomDouble() stream, in the reference's draw order
+        // (AndersonEvolve, MD_System.cpp:233-247): ResetCount += p per step; while ResetCount > 0 a particle
+        // floor(u N) gets d Box-Muller components sqrt(-2 log u1) sin(2 pi u2) sqrt(kT / m), u1 redrawn while <= 0
+        int64_t k = 0;
+        auto draw = [&]() -> double {
+            Require(k < ndraws, CCGPU_ERR_RANGE, "not enough random draws for the Anderson resets");
+            return draws[k++];
+        };
+        std::vector<double> table;
+        std::vector<int64_t> off(1, 0);
+        double count = 0.0;
+        for (int64_t u = 0; u < repeat; u++) {
+            count += prob;
+            for (; count > 0; count -= 1.0) {
+                const int64_t i = (int64_t)std::floor(draw() * (double)N);
+                Require(i >= 0 && i < N, CCGPU_ERR_RANGE, "reset particle out of range");
+                const double sigma = std::sqrt(kT / m->mass[(size_t)i]);
+                double row[5] = {(double)i, 0, 0, 0, 0};
+                for (int c = 0; c < d; c++) {
+                    double t;
+                    do {
+                        t = draw();
+                    } while (t <= 0);
+                    row[1 + c] = std::sqrt(-2 * std::log(t)) * std::sin(2 * kPi * draw()) * sigma;
+                }
+                table.insert(table.end(), row, row + 5);
+            }
+            off.push_back((int64_t)(table.size() / 5));
+        }
+        *used = k;
+        m->Sync();
+        m->resets.Reserve(std::max<size_t>(table.size(), 1));
+        if (!table.empty())
+            CC_HIP(hipMemcpy(m->resets.get(), table.data(), table.size() * sizeof(double), hipMemcpyHostToDevice));
+        m->BindTo(p);
+        m->EnsureAcceleration(p);
+        for (int64_t u = 0; u < repeat; u++)
+            m->Step(p, 0, 0.0, m->resets.get() + off[(size_t)u] * 5, (int)(off[(size_t)u + 1] - off[(size_t)u]));
+        m->Finish(p);
+    });
+}
+
+int ccgpu_md_set_random_speed(ccgpu_md *m, double kT, const double *draws, int64_t ndraws, int64_t *used)
+{
+    return Guard([&] {
+        Require(m && std::isfinite(kT) && used && (ndraws == 0 || draws), CCGPU_ERR_ARG, "bad argument");
+        // SetRandomSpeed (MD_System.cpp:379-394): each component sqrt(-2 log u1) sin(2 pi u2) sqrt(kT / m),
+        // redrawn while its magnitude exceeds 1e30
+        const int d = m->d;
+        int64_t k = 0;
+        auto draw = [&]() -> double {
+            Require(k < ndraws, CCGPU_ERR_RANGE, "not enough random draws for the velocities");
+            return draws[k++];
+        };
+        std::vector<double> vv((size_t)(m->N * d));
+        for (int64_t i = 0; i < m->N; i++) {
+            const double sigma = std::sqrt(kT / m->mass[(size_t)i]);
+            for (int c = 0; c < d; c++) {
+                double x = std::sqrt(-2 * std::log(draw())) * std::sin(2 * kPi * draw()) * sigma;
+                while (std::abs(x) > 1e30) x = std::sqrt(-2 * std::log(draw())) * std::sin(2 * kPi * draw()) * sigma;
+                vv[(size_t)(i * d + c)] = x;
+            }
+        }
+        *used = k;
+        m->Upload(m->v, vv.data());
+    });
+}
+
+int ccgpu_md_remove_overall_momentum(ccgpu_md *m)
+{
+    return Guard([&] {
+        Require(m != nullptr, CCGPU_ERR_ARG, "null handle");
+        // RemoveOverallMomentum (MD_System.cpp:395-407): subtract the mean velocity
+        const int d = m->d;
+        std::vector<double> vv((size_t)(m->N * d));
+        m->Download(m->v, vv.data());
+        double op[4] = {0, 0, 0, 0};
+        for (int64_t i = 0; i < m->N; i++)
+            for (int c = 0; c < d; c++) op[c] += vv[(size_t)(i * d + c)];
+        for (int c = 0; c < d; c++) op[c] *= 1.0 / (double)m->N;
+        for (int64_t i = 0; i < m->N; i++)
+            for (int c = 0; c < d; c++) vv[(size_t)(i * d + c)] -= op[c];
+        m->Upload(m->v, vv.data());
+    });
+}
+
 int ccgpu_potential_create_varbox(int dim, int form, int vk, int s0, double rK, double n, ccgpu_potential **out)
 {
     return Guard([&] {

@@ -301,6 +301,14 @@ void launch_rho_reduce(hipStream_t, const SysDesc *, const WorkItem *, int);
 void launch_energy_final(hipStream_t, const SysDesc *, const WorkItem *, int);
 void launch_mixture_final(hipStream_t, const SysDesc *, const WorkItem *, int);
 void launch_multiconfig_coef(hipStream_t, const SysDesc *, const WorkItem *, int);
+// molecular dynamics steps (cc_md.hip)
+void launch_md_kick_drift(hipStream_t, double *x, double *v, const double *a, double *half, int64_t N, int d,
+                          double dt, const double *bt);
+void launch_md_accel(hipStream_t, double *v, double *a, const double *F, const double *inv_mass, int64_t N, int d,
+                     double dt, int kick);
+void launch_md_nose_hoover(hipStream_t, double *v, const double *a, const double *half, const double *mass,
+                           int64_t N, int d, double dt, double kT, double Q, double *state);
+void launch_md_resets(hipStream_t, double *v, const double *table, int n, int d);
 // Monte Carlo trial move of one system (descriptor sidx): rho_trial (2 M), epart (slot blocks), energy_out (1)
 void launch_mc_trial(hipStream_t, const SysDesc *, const WorkItem *, int, const double *mv, double *rho_trial,
                      double *epart, int sidx, double *energy_out);
