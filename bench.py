@@ -59,14 +59,14 @@ def parse():
                          "polish below 1e-8")
     ap.add_argument("--workload", default="batched",
                     choices=["batched", "single", "mixed256", "large2d", "mixed2d", "serve", "five", "nufft2d",
-                             "targets"],
+                             "targets", "md"],
                     help="batched: the headline (64 seeds, 3D N=2000 chi=0.45); single: the same solves through the "
                          "single-solve drop-in path; mixed256: 256 configurations of mixed N, d, chi; large2d: 2D "
                          "N=20000 chi=0.4; mixed2d: the same solved in FP64 and in mixed precision, with the "
                          "mixed-precision error; serve: Poisson request stream with soft core; five: multi-species "
                          "ground states of the reference's five-component study (2D N=2000, 5 x 400); nufft2d: "
                          "large 2D systems through the NUFFT backend against the direct sums; targets: target-S(k) "
-                         "(equiluminous) relaxes and a variable-cell solve")
+                         "(equiluminous) relaxes and a variable-cell solve; md: molecular dynamics steps")
     ap.add_argument("--requests", type=int, default=48, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=20.0, help="serve: arrivals per second")
     ap.add_argument("--profiler-run", action="store_true",
@@ -597,9 +597,47 @@ def workload_targets(args):
     return out
 
 
+def workload_md(args):
+    """Molecular dynamics (ParticleMolecularDynamics, the MD mode of apps/CCOptimization/main.cpp:1875-2010) on
+    the headline system, 3D N=2000 chi=0.45 (M=2724) with the soft core e0=1, sigma=0.5: velocity Verlet steps
+    resident on the GPU (one E+F each), Anderson steps, and the oracle's E+F on the host threads beside it (one
+    E+F per MD step)."""
+    from modules_amd import GetKs, RandomConfiguration, RepulsiveCCPotential, getK
+    from modules_amd.md import ParticleMolecularDynamics
+    from modules_amd.potential import RandomGenerator
+    from oracle import pyoracle as po
+    cfg = RandomConfiguration(0, 2000, 3)
+    k = GetKs(cfg, getK(0.45, 1.0, 3))
+    pot = RepulsiveCCPotential(3, 1.0, 0.5)
+    for kk in k:
+        pot.CCPotential.AddConstraint(kk, [1.0])
+    pot.SetConfiguration(cfg)
+    md = ParticleMolecularDynamics(cfg, 0.001, 1.0)
+    gen = RandomGenerator(1)
+    md.SetRandomSpeed(0.01, gen)
+    md.Evolve(20, pot)  # warm
+    steps = 2000
+    t0 = time.perf_counter()
+    md.Evolve(steps, pot)
+    nve = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    md.AndersonEvolve(steps, pot, 0.01, 0.01, gen)
+    anderson = time.perf_counter() - t0
+    threads = min(16, os.cpu_count() or 1)
+    t = _time_ef(po, cfg, k, np.ones(len(k)), threads, 8.0)
+    ef = len(t) / t.sum()
+    return {"metric": "MD steps/s", "value": steps / nve, "unit": "steps/s", "dtype": "f64",
+            "workload": f"velocity Verlet MD, 3D N=2000 chi=0.45 (M={len(k)}), soft core e0=1 sigma=0.5, dt=0.001",
+            "nve": {"steps": steps, "wall_s": nve, "ms_per_step": 1000 * nve / steps},
+            "anderson": {"steps": steps, "wall_s": anderson, "ms_per_step": 1000 * anderson / steps},
+            "cpu_baseline": {"value": ef, "unit": "steps/s", "cores": threads, "kind": "port",
+                             "sample": f"{len(t)} oracle CC E+F evaluations (soft core left out) of the start configuration "
+                                       f"on {threads} threads: one per MD step"}}
+
+
 OTHER_WORKLOADS = {"single": workload_single, "mixed256": workload_mixed256, "large2d": workload_large2d,
                    "mixed2d": workload_mixed2d, "serve": workload_serve, "five": workload_five,
-                   "nufft2d": workload_nufft2d, "targets": workload_targets}
+                   "nufft2d": workload_nufft2d, "targets": workload_targets, "md": workload_md}
 
 
 def _cpu_model():

@@ -14,6 +14,7 @@
 // each cell's list sorted by particle index so the pair sums have a fixed order.  Pair pass: one lane
 // per particle visits the (2 r_c + 1)^d neighbour cells in a fixed order.
 #include "cc_device.h"
+#include "cc_wave.h"
 
 namespace ccgpu {
 
@@ -117,14 +118,21 @@ __global__ void __launch_bounds__(kBlock) k_sc_sort(const SysDesc *__restrict__ 
     }
 }
 
-// items (sys, particle block): energy share (pairs i<j) and force of particle i
+// One wavefront per particle: the lanes split the (2 reach + 1)^d neighbour cells (lane o takes cells o, o + 64,
+// ...), each lane walks its cells' particles in index order, and the energy share (pairs i < j) and force are
+// summed over the lanes by the fixed DPP tree, so the bits do not depend on scheduling.  items (sys, particle
+// block of kBlock) as the other soft-core kernels; each item spans kScSub blocks of kBlock / 64 particles.
+constexpr int kScWaves = kBlock / 64;
+constexpr int kScSub = kBlock / kScWaves;
+
 __global__ void __launch_bounds__(kBlock) k_sc_pair(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
 {
-    const WorkItem it = items[blockIdx.x];
+    const WorkItem it = items[blockIdx.x / kScSub];
     const SysDesc &S = sys[it.sys];
     if (!sys_active(sys, S)) return;
-    const int i = it.t0 * kBlock + threadIdx.x;
-    if (i >= S.N) return;
+    const int lane = threadIdx.x & 63;
+    const int i = it.t0 * kBlock + (int)(blockIdx.x % kScSub) * kScWaves + (int)(threadIdx.x >> 6);
+    if (i >= S.N) return;  // uniform per wavefront
     const int d = S.d;
     const double4 pi4 = S.pos[i];
     const double si[4] = {pi4.x, pi4.y, pi4.z, pi4.w};
@@ -148,7 +156,7 @@ __global__ void __launch_bounds__(kBlock) k_sc_pair(const SysDesc *__restrict__ 
     const int gi = S.pid ? S.pid[i] : i;
     double E = 0.0;
     double f[4] = {0, 0, 0, 0};
-    for (int o = 0; o < total; o++) {
+    for (int o = lane; o < total; o += 64) {
         int rem = o, lin = 0, mult = 1;
         double img[4] = {0, 0, 0, 0};
         for (int c = 0; c < d; c++) {
@@ -193,6 +201,9 @@ __global__ void __launch_bounds__(kBlock) k_sc_pair(const SysDesc *__restrict__ 
             }
         }
     }
+    E = wave_sum_dpp(E);
+    for (int c = 0; c < 4; c++) f[c] = wave_sum_dpp(f[c]);
+    if (lane != 0) return;
     if (radius)
         for (int c = 0; c < 4; c++) f[c] *= e0;
     S.sc_epart[i] = E;
@@ -213,7 +224,7 @@ void launch_softcore(hipStream_t stream, const SysDesc *sys, const WorkItem *par
     hipLaunchKernelGGL(k_sc_scan, dim3(nsys), dim3(kBlock), 0, stream, sys, sys_items);
     hipLaunchKernelGGL(k_sc_scatter, dim3(npart), dim3(kBlock), 0, stream, sys, part_items);
     if (ncell > 0) hipLaunchKernelGGL(k_sc_sort, dim3(ncell), dim3(kBlock), 0, stream, sys, cell_items);
-    hipLaunchKernelGGL(k_sc_pair, dim3(npart), dim3(kBlock), 0, stream, sys, part_items);
+    hipLaunchKernelGGL(k_sc_pair, dim3(npart * kScSub), dim3(kBlock), 0, stream, sys, part_items);
 }
 
 }  // namespace ccgpu
