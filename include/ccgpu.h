@@ -67,6 +67,12 @@ int ccgpu_potential_add_constraint(ccgpu_potential *p, const double *k, double V
 int ccgpu_potential_set_constraints(ccgpu_potential *p, int64_t M, const double *k, const double *V);
 int ccgpu_potential_num_constraints(const ccgpu_potential *p, int64_t *M);
 
+/* Soft-core / contact pair kernel for every later launch (no counterpart in the reference): 0 automatic (one
+ * thread per particle for launches of 65536 particles or more, lane groups per particle below), 1 always one
+ * thread per particle, 2 always lane groups.  Both kernels sum each particle's pairs in the same fixed tree, so
+ * the choice changes time, never bits. */
+int ccgpu_set_softcore_kernel(int mode);
+
 /* ShiftedCCPotential_S2(DimensionType) -- CollectiveCoordinatePotential.h:1642-1748: the target-S(k) form
  *   Phi = sum_k V(k) (|rho(k)|^2 / N - S0(k))^2 + Shift      (no 1/volume factor)
  *   F_j = -(4 / N) sum_k V(k) (|rho(k)|^2 / N - S0(k)) (Im rho cos(k.r_j) - Re rho sin(k.r_j)) k

@@ -271,6 +271,14 @@ int ccgpu_potential_create(int dim, ccgpu_potential **out)
     });
 }
 
+int ccgpu_set_softcore_kernel(int mode)
+{
+    return Guard([&] {
+        Require(mode >= 0 && mode <= 2, CCGPU_ERR_ARG, "mode must be 0 (automatic), 1 (thread) or 2 (lane groups)");
+        SetSoftCoreKernel(mode);
+    });
+}
+
 int ccgpu_potential_create_s2(int dim, ccgpu_potential **out)
 {
     return Guard([&] {

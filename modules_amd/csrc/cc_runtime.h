@@ -315,6 +315,9 @@ void launch_mc_trial(hipStream_t, const SysDesc *, const WorkItem *, int, const 
 void launch_force_lattice(hipStream_t, int seg, const SysDesc *, const WorkItem *, int, int ppl);
 void launch_force_general(hipStream_t, const SysDesc *, const WorkItem *, int);
 void launch_force_final(hipStream_t, const SysDesc *, const WorkItem *, int);
+// soft-core pair kernel choice for every later launch: 0 automatic, 1 one thread per particle, 2 lane groups per
+// particle (both give the same bits)
+void SetSoftCoreKernel(int mode);
 void launch_softcore(hipStream_t, const SysDesc *, const WorkItem *part_items, int npart, const WorkItem *sys_items,
                      int nsys, const WorkItem *cell_items, int ncell);
 // NUFFT backend (cc_nufft.hip); the FFTs between the stages run through each system's hipFFT plans

@@ -112,3 +112,24 @@ def test_collective_coordinate_md_writes_samples(tmp_path):
     out = CollectiveCoordinateMD(cfg, pot, RandomGenerator(1), 0.01, 0.1, str(tmp_path / "md"), 3, 20, 2)
     assert len(out) == 3 and all(np.isfinite(e) and ek > 0 for e, ek in out)
     assert ConfigurationPack(str(tmp_path / "md")).NumConfig() == 3
+
+
+@pytest.mark.parametrize("d,N,sigma", [(2, 3000, 0.6), (3, 1500, 0.9), (3, 800, 1.6)])
+def test_softcore_kernels_give_identical_bits(d, N, sigma):
+    """The thread-per-particle and lane-group pair kernels sum every particle's pairs in the same tree."""
+    from modules_amd._lib import lib
+    cfg = RandomConfiguration(21 + d, N, d)
+    pot = RepulsiveCCPotential(d, 1.0, sigma)
+    results = []
+    try:
+        for mode in (1, 2):
+            assert lib().ccgpu_set_softcore_kernel(mode) == 0
+            pot.SetConfiguration(cfg)
+            results.append(pot.EnergyAndAllForce())
+    finally:
+        lib().ccgpu_set_softcore_kernel(0)
+    (E1, F1), (E2, F2) = results
+    assert E1 == E2 and np.array_equal(F1, F2)
+    Eo = po.softcore_energy(cfg.relative, cfg.basis, sigma, 1.0)
+    Fo = po.softcore_forces(cfg.relative, cfg.basis, sigma, 1.0)
+    assert E1 == pytest.approx(Eo, rel=1e-12) and np.allclose(F1, Fo, rtol=1e-11, atol=1e-12)
