@@ -128,6 +128,14 @@ int ccgpu_potential_set_softcore(ccgpu_potential *p, double epsilon, double sigm
  * as Insert does).  Copies the particles to HBM and invalidates rho. */
 int ccgpu_potential_set_configuration(ccgpu_potential *p, const double *basis, int64_t N, const double *relative);
 
+/* SetIndexedConfiguration(full, indices) -- CollectiveCoordinatePotential.h:1465-1496: bind the sub-configuration
+ * whose local particle q is particle indices[q] of `full` (Nfull x dim relative coordinates, cell basis), in that
+ * order -- the same bits as ccgpu_potential_set_configuration on that sub-configuration.  CCGPU_ERR_RANGE for an
+ * index outside [0, Nfull) (std::out_of_range); CCGPU_ERR_STATE for the derived forms (target-S(k), weighted,
+ * log, sub-configuration, variable-cell), as SupportsIndexedView() is false for them (std::logic_error). */
+int ccgpu_potential_set_indexed_configuration(ccgpu_potential *p, const double *basis, int64_t Nfull,
+                                              const double *relative, int64_t n, const int64_t *indices);
+
 /* GetRho() + RhoReal / RhoImag -- CollectiveCoordinatePotential.h:1290-1314.  re, im: M doubles each. */
 int ccgpu_potential_get_rho(ccgpu_potential *p, double *re, double *im);
 

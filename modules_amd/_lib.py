@@ -82,6 +82,7 @@ SIGNATURES = {
     "ccgpu_md_set_random_speed": (C.c_int, [vp, C.c_double, dp, C.c_int64, i64p]),
     "ccgpu_md_remove_overall_momentum": (C.c_int, [vp]),
     "ccgpu_set_softcore_kernel": (C.c_int, [C.c_int]),
+    "ccgpu_potential_set_indexed_configuration": (C.c_int, [vp, dp, C.c_int64, dp, C.c_int64, i64p]),
     "ccgpu_potential_create_s2": (C.c_int, [C.c_int, C.POINTER(vp)]),
     "ccgpu_potential_create_s2_multiconfig": (C.c_int, [C.c_int, C.c_int64, C.POINTER(vp)]),
     "ccgpu_potential_add_constraint_s2": (C.c_int, [vp, dp, C.c_double, C.c_double]),

@@ -438,6 +438,32 @@ int ccgpu_potential_set_configuration(ccgpu_potential *p, const double *basis, i
     });
 }
 
+int ccgpu_potential_set_indexed_configuration(ccgpu_potential *p, const double *basis, int64_t Nfull,
+                                              const double *relative, int64_t n, const int64_t *indices)
+{
+    return Guard([&] {
+        Require(p && basis && Nfull >= 0 && n >= 0 && (Nfull == 0 || relative) && (n == 0 || indices), CCGPU_ERR_ARG,
+                "bad argument");
+        const System &S = p->sys;
+        Require(!S.s2 && !S.mixture && S.weights.empty() && !S.log_form && !p->varbox, CCGPU_ERR_STATE,
+                "ShiftedCCPotential::SetIndexedConfiguration: not supported by this derived class; use "
+                "SetConfiguration.");
+        const int d = S.d;
+        for (int64_t q = 0; q < n; q++)
+            if (indices[q] < 0 || indices[q] >= Nfull)
+                throw Error(CCGPU_ERR_RANGE, "ShiftedCCPotential::SetIndexedConfiguration: particle index out of range.");
+        for (int i = 0; i < d * d; i++) Require(std::isfinite(basis[i]), CCGPU_ERR_ARG, "non-finite basis");
+        std::vector<double> sub((size_t)std::max<int64_t>(n * d, 1));
+        for (int64_t q = 0; q < n; q++)
+            for (int c = 0; c < d; c++) {
+                const double x = relative[indices[q] * d + c];
+                if (!std::isfinite(x)) throw Error(CCGPU_ERR_NAN, "non-finite particle coordinate");
+                sub[(size_t)(q * d + c)] = x;
+            }
+        p->Bind(basis, n, sub.data());
+    });
+}
+
 int ccgpu_potential_get_rho(ccgpu_potential *p, double *re, double *im)
 {
     return Guard([&] {

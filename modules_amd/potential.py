@@ -285,6 +285,29 @@ class ShiftedCCPotential:
                                                       rel.shape[0], _p(rel)))
         self.RhoValid = False
 
+    def SupportsIndexedView(self):
+        """True only for the plain form (CollectiveCoordinatePotential.h:1465-1475)."""
+        return type(self) in (ShiftedCCPotential, RepulsiveCCPotential) and not self._S2 and \
+            getattr(self, "_softcore", (0.0, 0.0)) == (0.0, 0.0)
+
+    def SetIndexedConfiguration(self, full, indices):
+        """Bind the sub-configuration whose particle q is particle indices[q] of `full` (:1477-1496): the same bits
+        as SetConfiguration on that sub-configuration.  RuntimeError (std::logic_error) for the derived forms,
+        IndexError (std::out_of_range) for an index outside full."""
+        if not self.SupportsIndexedView():
+            raise RuntimeError("ShiftedCCPotential::SetIndexedConfiguration: not supported by this derived class; "
+                               "use SetConfiguration.")
+        idx = np.ascontiguousarray(indices, dtype=np.int64)
+        if idx.size and (idx.min() < 0 or idx.max() >= full.NumParticle()):
+            raise IndexError("ShiftedCCPotential::SetIndexedConfiguration: particle index out of range.")
+        self._sync()
+        rel = np.ascontiguousarray(full.relative, dtype=np.float64)
+        check(lib().ccgpu_potential_set_indexed_configuration(
+            self._h, _p(np.ascontiguousarray(full.basis)), rel.shape[0], _p(rel), idx.size,
+            idx.ctypes.data_as(C.POINTER(C.c_int64))))
+        self._config = Configuration(full.basis, rel[idx] if idx.size else np.zeros((0, full.GetDimension())))
+        self.RhoValid = False
+
     def _require(self):
         if self._config is None:
             raise RuntimeError("no configuration bound: call SetConfiguration first")

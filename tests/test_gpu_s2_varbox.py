@@ -447,3 +447,26 @@ def test_mc_moves_match_recomputed_energies(d, N, chi, s2, general):
     Fo = po.s2_forces(cfg.relative, cfg.basis, k, v, t, re, im) if s2 else \
         po.forces(cfg.relative, cfg.basis, k, v, re, im)
     assert np.max(np.abs(F - Fo)) <= 1e-9 * max(1.0, np.max(np.abs(Fo)))
+
+
+def test_indexed_view_equals_the_sub_configuration():
+    """SetIndexedConfiguration(full, indices) gives the bits of SetConfiguration on the sub-configuration
+    (CollectiveCoordinatePotential.h:1477-1496), and refuses what the reference refuses."""
+    from modules_amd import ShiftedCCPotential
+    full = RandomConfiguration(31, 900, 3)
+    k = GetKs(full, getK(0.3, 1.0, 3))
+    idx = np.random.default_rng(2).permutation(900)[:333]
+    a, b = ShiftedCCPotential(3), ShiftedCCPotential(3)
+    for kk in k:
+        a.AddConstraint(kk, [1.0])
+        b.AddConstraint(kk, [1.0])
+    a.SetIndexedConfiguration(full, idx)
+    b.SetConfiguration(Configuration(full.basis, full.relative[idx]))
+    Ea, Fa = a.EnergyAndAllForce()
+    Eb, Fb = b.EnergyAndAllForce()
+    assert Ea == Eb and np.array_equal(Fa, Fb)
+    with pytest.raises(IndexError):
+        a.SetIndexedConfiguration(full, [0, 900])
+    s2 = ShiftedCCPotential_S2(3)
+    with pytest.raises(RuntimeError):
+        s2.SetIndexedConfiguration(full, idx)
