@@ -236,3 +236,62 @@ def ExportText(basis, relative, species, radii):
     buf = C.create_string_buffer(size.value)
     check(lib().ccgpu_export_text(*args, buf, size.value, C.byref(size)))
     return buf.raw[: size.value].decode()
+
+
+_MAX_DIMENSION = 4  # core/etc.h:12: every coordinate row of the text files holds four columns
+
+
+def WriteSpherePacking(pck, OutputName):
+    """WriteConfiguration(SpherePacking, name) (core/ConfigurationIO.cpp:763-785): name.txt holding the dimension,
+    one row per basis vector and one row per sphere (four Cartesian columns, then the radius), 17 digits."""
+    d = pck.GetDimension()
+    basis = np.asarray(pck.basis, dtype=np.float64)
+    cart = np.asarray(pck.relative, dtype=np.float64).reshape(-1, d) @ basis
+    pad = np.zeros(_MAX_DIMENSION)
+
+    def row(v):
+        w = pad.copy()
+        w[: len(v)] = v
+        return "\t".join(repr(float(x)) for x in w)
+
+    lines = [str(d)] + [row(basis[i]) for i in range(d)]
+    for i in range(cart.shape[0]):
+        lines.append(row(cart[i]) + "\t" + repr(float(pck.radii[i])))
+    with open(str(OutputName) + ".txt", "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def ReadSpherePacking(InputName):
+    """ReadConfiguration(SpherePacking, name) (core/ConfigurationIO.cpp:865-908): reads name.txt; the first d
+    numbers of each basis row are the basis vector, then every five numbers (four coordinates, of which the first
+    d are used, and the radius) are one sphere, converted to relative coordinates and wrapped into [0, 1).  A
+    trailing incomplete row is dropped, as the reference's end-of-file test drops it."""
+    from .multi import SpherePacking
+    with open(str(InputName) + ".txt") as f:
+        lines = f.read().split("\n")
+    head = lines[0].split()
+    d = int(head[0])
+    rest = head[1:]
+    basis = np.zeros((d, d))
+    li = 1
+    for i in range(d):
+        # `ifile >> x` skips blank lines, then the rest of the basis row is ignored
+        vals = list(rest)
+        rest = []
+        while len(vals) < d:
+            vals += lines[li].split()
+            li += 1
+        basis[i] = [float(x) for x in vals[:d]]
+    toks = " ".join(lines[li:]).split()
+    per = _MAX_DIMENSION + 1
+    n = len(toks) // per
+    cart = np.zeros((n, d))
+    radii = np.zeros(n)
+    for p in range(n):
+        row = toks[p * per:(p + 1) * per]
+        cart[p] = [float(x) for x in row[:d]]
+        radii[p] = float(row[_MAX_DIMENSION])
+    rel = cart @ np.linalg.inv(basis) if n else np.zeros((0, d))
+    rel = rel - np.floor(rel)
+    rel = rel - np.floor(rel)
+    return SpherePacking(basis, rel, radii)

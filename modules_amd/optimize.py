@@ -18,7 +18,10 @@ from ._lib import check, dp, lib
 from .potential import Configuration, ShiftedCCPotential
 
 MaxEnergy = 1e10  # core/etc.h:8
-StructureOptimization_LBFGSHistory = 40  # StructureOptimization.cpp:17
+StructureOptimization_LBFGSHistory = 40  # potential/StructureOptimization.cpp:17
+# potential/StructureOptimization.cpp:18 (0 nlopt, 1 ring, 3 fast): the selector is kept for the drivers' commands;
+# every value runs the one device L-BFGS (Luksan's limited-memory BFGS, the algorithm of the nlopt choice)
+StructureOptimization_LBFGSImplementation = 0
 
 
 def _p(a):
@@ -254,7 +257,8 @@ class BatchSolver:
             self._h = None
 
     def submit(self, config: Configuration, k, V, shift=0.0, epsilon=0.0, sigma=0.0, MaxStep=10000, Emin=-MaxEnergy,
-               history=StructureOptimization_LBFGSHistory):
+               history=None):
+        history = history or StructureOptimization_LBFGSHistory
         d = config.GetDimension()
         rel = np.ascontiguousarray(config.relative, dtype=np.float64)
         k = np.ascontiguousarray(k, dtype=np.float64).reshape(-1, d)
@@ -267,9 +271,10 @@ class BatchSolver:
         return job.value
 
     def submit_s2(self, config: Configuration, k, V, S0, shift=0.0, epsilon=0.0, sigma=0.0, MaxStep=10000,
-                  Emin=-MaxEnergy, history=StructureOptimization_LBFGSHistory):
+                  Emin=-MaxEnergy, history=None):
         """A relax request with the target-S(k) potential (ShiftedCCPotential_S2; soft core as
         RepulsiveCCPotential with S0 != 0): one target S0 per constraint."""
+        history = history or StructureOptimization_LBFGSHistory
         d = config.GetDimension()
         rel = np.ascontiguousarray(config.relative, dtype=np.float64)
         k = np.ascontiguousarray(k, dtype=np.float64).reshape(-1, d)
@@ -286,9 +291,10 @@ class BatchSolver:
         return job.value
 
     def submit_multi(self, config: Configuration, pot, MaxStep=10000, Emin=-MaxEnergy,
-                     history=StructureOptimization_LBFGSHistory):
+                     history=None):
         """A relax request with a MultiStealthyPotential `pot` (its species lists, constraints, radii, u0, e0 and
         contact cutoff) from the positions of `config` (same particles as pot's configuration)."""
+        history = history or StructureOptimization_LBFGSHistory
         d = config.GetDimension()
         rel = np.ascontiguousarray(config.relative, dtype=np.float64)
         N = rel.shape[0]
