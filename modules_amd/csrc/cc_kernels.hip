@@ -9,8 +9,9 @@
 // Two kernel families:
 //   lattice  -- constraints are reciprocal lattice vectors k = sum_j n_j b_j, so k.r = 2 pi n.s with s the
 //               relative coordinates.  Constraints are grouped into segments of consecutive last-axis
-//               index; one polynomial sincos per (segment, particle) starts the segment and a complex
-//               multiply by exp(2 pi i s_{d-1}) steps along it.  VALU-bound (FP64 FMA), no MFMA: the
+//               index; one polynomial sincos per (segment, particle) starts the segment and steps by
+//               exp(2 pi i s_{d-1}) (complex products, or the three-term recurrence of the force chains)
+//               walk along it.  VALU-bound (FP64 FMA), no MFMA: the
 //               contraction depth is d <= 4.
 //   general  -- arbitrary Cartesian k: one sincos of k.r per pair.
 // Every sum has a fixed order (chunk partials reduced in chunk order, block trees), so results are
@@ -714,6 +715,12 @@ __device__ __forceinline__ void force_lattice_body(const SysDesc &S, const WorkI
                 accl[q][0] = accl[q][1] = 0;
             }
             const Real *cf = lcf + 2 * sl * SEG;
+            // FP64: after its first step each chain advances by the three-term recurrence
+            //   e_{t+2} = 2 cos(2 phi) e_t - e_{t-2}
+            // (cos and sin of an arithmetic progression of angles both satisfy it): two FMAs per slot instead of
+            // the four operations of a complex product.  Its rounding error grows as t^2 eps over a chain, with
+            // sin(n x) / sin(x) <= n bounding the amplification for every step angle.  FP32 keeps the product.
+            Real pc[PPL][2], ps[PPL][2];
 #pragma unroll
             for (int t0 = 0; t0 < SEG; t0 += 4) {
                 if (t0 < len) {  // uniform
@@ -726,10 +733,22 @@ __device__ __forceinline__ void force_lattice_body(const SysDesc &S, const WorkI
                             const Real c = ax * cs[q][ch] - ay * sn[q][ch];
                             acc[q][ch] += c;
                             accl[q][ch] += (Real)t * c;  // t is a compile-time constant: one FMA
-                            const Real nc = cs[q][ch] * st2x[q] - sn[q][ch] * st2y[q];
-                            const Real ns2 = cs[q][ch] * st2y[q] + sn[q][ch] * st2x[q];
-                            cs[q][ch] = nc;
-                            sn[q][ch] = ns2;
+                            if (MIXED || t < 2) {
+                                const Real nc = cs[q][ch] * st2x[q] - sn[q][ch] * st2y[q];
+                                const Real ns2 = cs[q][ch] * st2y[q] + sn[q][ch] * st2x[q];
+                                pc[q][ch] = cs[q][ch];
+                                ps[q][ch] = sn[q][ch];
+                                cs[q][ch] = nc;
+                                sn[q][ch] = ns2;
+                            } else {
+                                const Real k2 = st2x[q] + st2x[q];
+                                const Real nc = k2 * cs[q][ch] - pc[q][ch];
+                                const Real ns2 = k2 * sn[q][ch] - ps[q][ch];
+                                pc[q][ch] = cs[q][ch];
+                                ps[q][ch] = sn[q][ch];
+                                cs[q][ch] = nc;
+                                sn[q][ch] = ns2;
+                            }
                         }
                     }
                 }
