@@ -83,27 +83,60 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b)
     return make_double2(__builtin_fma(a.x, b.x, -a.y * b.y), __builtin_fma(a.x, b.y, a.y * b.x));
 }
 
-template <int SEG>
+template <int SEG, bool MIXED>
 struct RhoLatticeLds {
     static constexpr int G = (SEG % 8 == 0) ? 8 : 4;
     static constexpr int kSub = 64;
+    static constexpr int NP = MIXED ? G : 1;  // staged powers per particle
     double4 pos[kSub];
-    double wgt[kSub];       // particle weights (weighted systems)
-    double2 pow[kSub * G];  // particle j: P_1 .. P_G at j*G + b - 1 (float2 in the first half when mixed)
+    double wgt[kSub];        // particle weights (weighted systems)
+    double2 pow[kSub * NP];  // FP64: the step P_1 of particle j at j; mixed: P_1 .. P_G as float2 at j*G + b - 1
     double2 tab[64];
     float2 tab32[64];
 };
 
-// The particle loop of k_rho_lattice.  MIXED: the phase of each (segment, particle) is reduced to [-1/2, 1/2]
-// revolutions in FP64, then the table sincos, the group powers and the per-slot products run in FP32 and are
-// summed in FP32 over runs of kFlush particles; each run's sums are added to the FP64 accumulators.
+// One (segment, particle) phase chain of the FP64 rho kernel: e_{t-1} = (px, py), e_t = (cx, cy).
+struct RhoChain {
+    double px, py, cx, cy, k1;
+    __device__ __forceinline__ void start(const double4 &p, const double2 &st, double w, double h0, double h1,
+                                          double h2, double h3, const double2 *tab)
+    {
+        const double u = __builtin_fma(h0, p.x, __builtin_fma(h1, p.y, __builtin_fma(h2, p.z, h3 * p.w)));
+        double sn, cs;
+        sincos_rev_tab(u, tab, sn, cs);
+        cs *= w;  // the weight rides along the chain (the recurrence is linear); w = 1 without weights
+        sn *= w;
+        k1 = st.x + st.x;
+        px = cs;
+        py = sn;
+        cx = __builtin_fma(cs, st.x, -sn * st.y);
+        cy = __builtin_fma(cs, st.y, sn * st.x);
+    }
+    __device__ __forceinline__ void step()
+    {
+        const double nx = __builtin_fma(k1, cx, -px);
+        const double ny = __builtin_fma(k1, cy, -py);
+        px = cx;
+        py = cy;
+        cx = nx;
+        cy = ny;
+    }
+};
+
+// The particle loop of k_rho_lattice.  FP64: from e_0 (the table sincos) and e_1 = e_0 P_1, the segment's
+// phases follow the three-term recurrence e_{t+1} = 2 cos(phi) e_t - e_{t-1} (cos and sin of an arithmetic
+// progression of angles both satisfy it): two FMAs and two adds per slot, one staged step per particle.  Its
+// rounding error grows as t^2 eps along the segment, sin(n x) / sin(x) <= n bounding the amplification for every
+// step angle.  MIXED: the phase of each (segment, particle) is reduced to [-1/2, 1/2] revolutions in FP64, then
+// the table sincos, the group powers and the per-slot products run in FP32 and are summed in FP32 over runs of
+// kFlush particles; each run's sums are added to the FP64 accumulators.
 template <int SEG, bool MIXED>
-__device__ __forceinline__ void rho_lattice_body(const SysDesc &S, const WorkItem &it, RhoLatticeLds<SEG> &L,
+__device__ __forceinline__ void rho_lattice_body(const SysDesc &S, const WorkItem &it, RhoLatticeLds<SEG, MIXED> &L,
                                                  double (&re)[SEG], double (&im)[SEG], bool valid, int wlen,
                                                  double h0, double h1, double h2, double h3)
 {
-    constexpr int G = RhoLatticeLds<SEG>::G;
-    constexpr int kSub = RhoLatticeLds<SEG>::kSub;
+    constexpr int G = RhoLatticeLds<SEG, MIXED>::G;
+    constexpr int kSub = RhoLatticeLds<SEG, MIXED>::kSub;
     constexpr int kFlush = 32;
     float2 *pow32 = reinterpret_cast<float2 *>(L.pow);
     const int j0 = it.t1 * S.pchunk_len;
@@ -115,55 +148,40 @@ __device__ __forceinline__ void rho_lattice_body(const SysDesc &S, const WorkIte
             const int j = jb + q;
             L.pos[q] = S.pos[j];
             if (S.weight) L.wgt[q] = S.weight[j];
-            double2 pw[G];
-            pw[0] = S.step[j];
-            pw[1] = cmul(pw[0], pw[0]);
-            pw[2] = cmul(pw[1], pw[0]);
-            pw[3] = cmul(pw[1], pw[1]);
-            if constexpr (G == 8) {
-                pw[4] = cmul(pw[3], pw[0]);
-                pw[5] = cmul(pw[3], pw[1]);
-                pw[6] = cmul(pw[3], pw[2]);
-                pw[7] = cmul(pw[3], pw[3]);
-            }
+            if constexpr (MIXED) {
+                double2 pw[G];
+                pw[0] = S.step[j];
+                pw[1] = cmul(pw[0], pw[0]);
+                pw[2] = cmul(pw[1], pw[0]);
+                pw[3] = cmul(pw[1], pw[1]);
+                if constexpr (G == 8) {
+                    pw[4] = cmul(pw[3], pw[0]);
+                    pw[5] = cmul(pw[3], pw[1]);
+                    pw[6] = cmul(pw[3], pw[2]);
+                    pw[7] = cmul(pw[3], pw[3]);
+                }
 #pragma unroll
-            for (int b = 0; b < G; b++) {
-                if (MIXED) pow32[q * G + b] = make_float2((float)pw[b].x, (float)pw[b].y);
-                else L.pow[q * G + b] = pw[b];
+                for (int b = 0; b < G; b++) pow32[q * G + b] = make_float2((float)pw[b].x, (float)pw[b].y);
+            } else {
+                L.pow[q] = S.step[j];
             }
         }
         __syncthreads();
         if (!valid) continue;
-        if (!MIXED) {
+        if constexpr (!MIXED) {
             for (int j = 0; j < nsub; j++) {
-                const double4 p = L.pos[j];
-                // the group powers are read before the sincos, whose arithmetic covers their LDS latency
-                double2 P[G];
+                RhoChain c;
+                c.start(L.pos[j], L.pow[j], S.weight ? L.wgt[j] : 1.0, h0, h1, h2, h3, L.tab);
+                re[0] += c.px;
+                im[0] += c.py;
 #pragma unroll
-                for (int b = 0; b < G; b++) P[b] = L.pow[j * G + b];
-                const double u = __builtin_fma(h0, p.x, __builtin_fma(h1, p.y, __builtin_fma(h2, p.z, h3 * p.w)));
-                double sn, cs;
-                sincos_rev_tab(u, L.tab, sn, cs);
-                if (S.weight) {  // the weight rides along the unit-modulus phase chain
-                    cs *= L.wgt[j];
-                    sn *= L.wgt[j];
-                }
+                for (int t0 = 0; t0 < SEG; t0 += 4) {
+                    if (t0 < wlen) {  // uniform: the wavefront needs no more slots
 #pragma unroll
-                for (int t0 = 0; t0 < SEG; t0 += G) {
-                    if (t0 < wlen) {
-                        re[t0] += cs;
-                        im[t0] += sn;
-#pragma unroll
-                        for (int b = 1; b < G; b++) {
-                            if (b == 4 && t0 + 4 >= wlen) break;  // uniform: the wavefront needs no more slots
-                            re[t0 + b] = __builtin_fma(cs, P[b - 1].x, __builtin_fma(-sn, P[b - 1].y, re[t0 + b]));
-                            im[t0 + b] = __builtin_fma(cs, P[b - 1].y, __builtin_fma(sn, P[b - 1].x, im[t0 + b]));
-                        }
-                        if (t0 + G < wlen) {
-                            const double nc = __builtin_fma(cs, P[G - 1].x, -sn * P[G - 1].y);
-                            const double ns = __builtin_fma(cs, P[G - 1].y, sn * P[G - 1].x);
-                            cs = nc;
-                            sn = ns;
+                        for (int t = (t0 == 0 ? 1 : t0); t < t0 + 4; t++) {
+                            re[t] += c.cx;
+                            im[t] += c.cy;
+                            if (t + 1 < SEG) c.step();
                         }
                     }
                 }
@@ -221,7 +239,7 @@ __device__ __forceinline__ void rho_lattice_body(const SysDesc &S, const WorkIte
 // body free of the FP32 partial sums.
 template <int SEG, bool MIXED>
 __device__ __forceinline__ void rho_lattice_kernel(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items,
-                                                   RhoLatticeLds<SEG> &L)
+                                                   RhoLatticeLds<SEG, MIXED> &L)
 {
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
@@ -254,7 +272,7 @@ __device__ __forceinline__ void rho_lattice_kernel(const SysDesc *__restrict__ s
 template <int SEG>
 __global__ void __launch_bounds__(kRhoBlock) k_rho_lattice(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
 {
-    __shared__ RhoLatticeLds<SEG> L;
+    __shared__ RhoLatticeLds<SEG, false> L;
     rho_lattice_kernel<SEG, false>(sys, items, L);
 }
 
@@ -262,7 +280,7 @@ template <int SEG>
 __global__ void __launch_bounds__(kRhoBlock) k_rho_lattice_mixed(const SysDesc *__restrict__ sys,
                                                               const WorkItem *__restrict__ items)
 {
-    __shared__ RhoLatticeLds<SEG> L;
+    __shared__ RhoLatticeLds<SEG, true> L;
     rho_lattice_kernel<SEG, true>(sys, items, L);
 }
 
