@@ -431,8 +431,26 @@ __global__ void __launch_bounds__(kBlock) k_rho_reduce(const SysDesc *__restrict
     const int q = it.t0 * kBlock + threadIdx.x;
     double e = 0.0;
     if (q < S.nslots) {
+        // chunk order; the loads of eight chunks are issued before their adds, so a system with many chunks
+        // (few slot blocks, long loops) keeps loads in flight
         double re = 0.0, im = 0.0;
-        for (int c = 0; c < S.pchunks; c++) {
+        constexpr int kLoads = 8;
+        int c = 0;
+        for (; c + kLoads <= S.pchunks; c += kLoads) {
+            double vr[kLoads], vi[kLoads];
+#pragma unroll
+            for (int u = 0; u < kLoads; u++) {
+                const double *p = S.rpart + ((int64_t)(c + u) * S.nslots + q) * 2;
+                vr[u] = p[0];
+                vi[u] = p[1];
+            }
+#pragma unroll
+            for (int u = 0; u < kLoads; u++) {
+                re += vr[u];
+                im += vi[u];
+            }
+        }
+        for (; c < S.pchunks; c++) {
             const double *p = S.rpart + ((int64_t)c * S.nslots + q) * 2;
             re += p[0];
             im += p[1];
