@@ -82,6 +82,22 @@ def test_segment_lengths_agree(seg, rho_kernel):
     assert np.max(np.abs(F - po.forces(cfg.relative, cfg.basis, k, np.ones(len(k)), re, im))) <= RTOL * sf
 
 
+@pytest.mark.parametrize("seg", [16, 24, 32])
+def test_phase_recurrences_at_degenerate_step_angles(seg):
+    # The lattice kernels walk each segment by the three-term recurrence e_{t+1} = 2 cos(phi) e_t - e_{t-1}
+    # (phi = 2 pi s_last).  Its error amplification is sin(n phi) / sin(phi) <= n, largest where sin(phi) -> 0:
+    # particles at s_last = 0, 1e-12, 1/2 -+ 1e-12, 1 - 1e-12 and 1/4, with segments as long as the tile allows
+    # (N = 64 in an 8 x 8 cell, |k| < 17: lattice lines of about 44 points), must still meet the oracle.
+    rng = np.random.default_rng(5)
+    s = rng.random((64, 2))
+    s[:36, 1] = np.repeat([0.0, 1e-12, 0.5 - 1e-12, 0.5, 1.0 - 1e-12, 0.25], 6)
+    cfg = Configuration(np.eye(2) * 8.0, s)
+    k = GetKs(cfg, 17.0)
+    assert len(k) > 700
+    pot, _, _ = compare(cfg, k, np.ones(len(k)), tiles=(seg, 64, 5), expect_lattice=True)
+    assert pot.UsesLatticePath()
+
+
 def test_general_path_random_wavevectors():
     rng = np.random.default_rng(5)
     cfg = Configuration(np.array([[3.0, 0.0, 0.0], [0.5, 2.5, 0.0], [0.2, 0.3, 4.0]]), rng.random((150, 3)))
