@@ -9,10 +9,9 @@
 // Two kernel families:
 //   lattice  -- constraints are reciprocal lattice vectors k = sum_j n_j b_j, so k.r = 2 pi n.s with s the
 //               relative coordinates.  Constraints are grouped into segments of consecutive last-axis
-//               index; one polynomial sincos per (segment, particle) starts the segment and steps by
-//               exp(2 pi i s_{d-1}) (complex products, or the three-term recurrence of the force chains)
-//               walk along it.  VALU-bound (FP64 FMA), no MFMA: the
-//               contraction depth is d <= 4.
+//               index; one polynomial sincos per (segment, particle) starts the segment, and the phases along
+//               it follow from the step exp(2 pi i s_{d-1}) (the three-term recurrence in FP64, complex products
+//               in the FP32 variants).  VALU-bound (FP64 FMA), no MFMA: the contraction depth is d <= 4.
 //   general  -- arbitrary Cartesian k: one sincos of k.r per pair.
 // Every sum has a fixed order (chunk partials reduced in chunk order, block trees), so results are
 // bitwise repeatable and do not depend on which other systems share the launch.
@@ -73,10 +72,10 @@ __global__ void __launch_bounds__(kBlock) k_prepare(const SysDesc *__restrict__ 
 // ---------------------------------------------------------------------------------------------
 // rho, lattice path.  items: (sys, segment block of kRhoBlock segments, particle chunk)
 // One lane = one segment of up to SEG consecutive constraints.  The chunk's particles are staged in LDS in
-// sub-chunks of kSub, each with its powers P_b = exp(2 pi i b s_{d-1}), b = 1..G, and read by all lanes at
-// one address (broadcast).  Per (segment, particle): one table sincos starts the segment at e_0; within a
-// group of G slots rho_{t0+b} += e_{t0} P_b (4 FMA per slot, independent), and e_{t0+G} = e_{t0} P_G
-// steps to the next group, so a slot costs (4G + 2) / G operations and the phase chain is SEG / G long.
+// sub-chunks of kSub with their step P_1 = exp(2 pi i s_{d-1}) and read by all lanes at one address
+// (broadcast).  Per (segment, particle): one table sincos starts the segment at e_0, then the recurrence below
+// gives e_1 .. e_{SEG-1} at 4 operations per slot.  The mixed-precision variant stages P_1 .. P_G (G = 8) in
+// FP32 instead: within a group of G slots rho_{t0+b} += e_{t0} P_b, and e_{t0+G} = e_{t0} P_G.
 // ---------------------------------------------------------------------------------------------
 __device__ __forceinline__ double2 cmul(double2 a, double2 b)
 {
@@ -728,6 +727,7 @@ __device__ __forceinline__ void force_lattice_body(const SysDesc &S, const WorkI
             const double h0 = (double)h0i, h1 = (double)h1i, h2 = (double)h2i, h3 = (double)h3i;
             const double hl = (d == 1) ? h0 : (d == 2) ? h1 : (d == 3) ? h2 : h3;
             Real sn[PPL][2], cs[PPL][2], acc[PPL][2], accl[PPL][2];
+            Real pc[PPL][2], ps[PPL][2];  // FP64: e_{t-2} of each chain
 #pragma unroll
             for (int q = 0; q < PPL; q++) {
                 const double u = __builtin_fma(h0, p[q].x, __builtin_fma(h1, p[q].y, __builtin_fma(h2, p[q].z, h3 * p[q].w)));
@@ -747,16 +747,24 @@ __device__ __forceinline__ void force_lattice_body(const SysDesc &S, const WorkI
                 sn[q][0] = s_;
                 cs[q][1] = c_ * stx[q] - s_ * sty[q];
                 sn[q][1] = c_ * sty[q] + s_ * stx[q];
+                if constexpr (!MIXED) {
+                    // e_{-1} = 2 cos(phi) e_0 - e_1 and e_{-2} = 2 cos(phi) e_{-1} - e_0: the recurrence run backwards
+                    const Real k1 = stx[q] + stx[q];
+                    pc[q][1] = k1 * c_ - cs[q][1];
+                    ps[q][1] = k1 * s_ - sn[q][1];
+                    pc[q][0] = k1 * pc[q][1] - c_;
+                    ps[q][0] = k1 * ps[q][1] - s_;
+                }
                 acc[q][0] = acc[q][1] = 0;
                 accl[q][0] = accl[q][1] = 0;
             }
             const Real *cf = lcf + 2 * sl * SEG;
-            // FP64: after its first step each chain advances by the three-term recurrence
+            // FP64: each chain advances by the three-term recurrence
             //   e_{t+2} = 2 cos(2 phi) e_t - e_{t-2}
-            // (cos and sin of an arithmetic progression of angles both satisfy it): two FMAs per slot instead of
-            // the four operations of a complex product.  Its rounding error grows as t^2 eps over a chain, with
-            // sin(n x) / sin(x) <= n bounding the amplification for every step angle.  FP32 keeps the product.
-            Real pc[PPL][2], ps[PPL][2];
+            // (cos and sin of an arithmetic progression of angles both satisfy it), from e_{-2}, e_{-1} above: two
+            // FMAs per slot instead of the four operations of a complex product.  Its rounding error grows as
+            // t^2 eps over a chain, with sin(n x) / sin(x) <= n bounding the amplification for every step angle.
+            // FP32 keeps the product.
 #pragma unroll
             for (int t0 = 0; t0 < SEG; t0 += 4) {
                 if (t0 < len) {  // uniform
@@ -769,7 +777,7 @@ __device__ __forceinline__ void force_lattice_body(const SysDesc &S, const WorkI
                             const Real c = ax * cs[q][ch] - ay * sn[q][ch];
                             acc[q][ch] += c;
                             accl[q][ch] += (Real)t * c;  // t is a compile-time constant: one FMA
-                            if (MIXED || t < 2) {
+                            if (MIXED) {
                                 const Real nc = cs[q][ch] * st2x[q] - sn[q][ch] * st2y[q];
                                 const Real ns2 = cs[q][ch] * st2y[q] + sn[q][ch] * st2x[q];
                                 pc[q][ch] = cs[q][ch];
