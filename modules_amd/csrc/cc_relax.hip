@@ -50,26 +50,45 @@ struct BlockPlisOps {
     {
         if (id == kVecS || id == kVecG) gs_valid = false;
     }
-    // Element loops visit this thread's elements four at a time, loads first, so that each thread keeps
-    // several independent HBM loads in flight (one workgroup per job leaves few waves per CU).
-    static constexpr int kU = 4;
+    // Element loops visit this thread's elements kU at a time, all loads of a batch before their uses, so that
+    // each thread keeps several independent HBM loads in flight (one workgroup per job leaves few waves per CU);
+    // with 256 threads the remainder is one predicated batch, so no element waits on a round trip of its own.
+    // Sums run in element order either way.
+    static constexpr int kU = BS <= 256 ? 16 : 4;
+    template <class Load, class Use>
+    __device__ __forceinline__ void batches(Load load, Use use) const
+    {
+        int e = threadIdx.x;
+        for (; e + (kU - 1) * BS < n; e += kU * BS) {
+#pragma unroll
+            for (int u = 0; u < kU; u++) load(u, e + u * BS);
+#pragma unroll
+            for (int u = 0; u < kU; u++) use(u, e + u * BS);
+        }
+        if constexpr (BS <= 256) {
+            if (e < n) {
+#pragma unroll
+                for (int u = 0; u < kU; u++)
+                    if (e + u * BS < n) load(u, e + u * BS);
+#pragma unroll
+                for (int u = 0; u < kU; u++)
+                    if (e + u * BS < n) use(u, e + u * BS);
+            }
+        } else {
+            // the 512-thread variant has no registers to spare for a predicated batch: one element at a time
+            for (; e < n; e += BS) {
+                load(0, e);
+                use(0, e);
+            }
+        }
+    }
     __device__ double dot(int a, int b)
     {
         if (gs_valid && ((a == kVecG && b == kVecS) || (a == kVecS && b == kVecG))) return gs_val;
         const double *pa = vec(a), *pb = vec(b);
-        double t = 0.0;
-        int e = threadIdx.x;
-        for (; e + (kU - 1) * BS < n; e += kU * BS) {
-            double va[kU], vb[kU];
-#pragma unroll
-            for (int u = 0; u < kU; u++) {
-                va[u] = pa[e + u * BS];
-                vb[u] = pb[e + u * BS];
-            }
-#pragma unroll
-            for (int u = 0; u < kU; u++) t = __builtin_fma(va[u], vb[u], t);
-        }
-        for (; e < n; e += BS) t = __builtin_fma(pa[e], pb[e], t);
+        double t = 0.0, va[kU], vb[kU];
+        batches([&](int u, int i) { va[u] = pa[i]; vb[u] = pb[i]; },
+                [&](int u, int) { t = __builtin_fma(va[u], vb[u], t); });
         return block_reduce(t, false);
     }
     __device__ void axpy(int z, double a, int xv, int yv)
@@ -77,109 +96,54 @@ struct BlockPlisOps {
         touch(z);
         double *pz = vec(z);
         const double *px = vec(xv), *py = vec(yv);
-        int e = threadIdx.x;
-        for (; e + (kU - 1) * BS < n; e += kU * BS) {
-            double vx[kU], vy[kU];
-#pragma unroll
-            for (int u = 0; u < kU; u++) {
-                vx[u] = px[e + u * BS];
-                vy[u] = py[e + u * BS];
-            }
-#pragma unroll
-            for (int u = 0; u < kU; u++) pz[e + u * BS] = __builtin_fma(a, vx[u], vy[u]);
-        }
-        for (; e < n; e += BS) pz[e] = __builtin_fma(a, px[e], py[e]);
+        double vx[kU], vy[kU];
+        batches([&](int u, int i) { vx[u] = px[i]; vy[u] = py[i]; },
+                [&](int u, int i) { pz[i] = __builtin_fma(a, vx[u], vy[u]); });
     }
     __device__ void neg(int src, int dst)
     {
         touch(dst);
         const double *ps = vec(src);
         double *pd = vec(dst);
-        int e = threadIdx.x;
-        for (; e + (kU - 1) * BS < n; e += kU * BS) {
-            double v[kU];
-#pragma unroll
-            for (int u = 0; u < kU; u++) v[u] = ps[e + u * BS];
-#pragma unroll
-            for (int u = 0; u < kU; u++) pd[e + u * BS] = -v[u];
-        }
-        for (; e < n; e += BS) pd[e] = -ps[e];
+        double v[kU];
+        batches([&](int u, int i) { v[u] = ps[i]; }, [&](int u, int i) { pd[i] = -v[u]; });
     }
     __device__ void copy(int src, int dst)
     {
         touch(dst);
         const double *ps = vec(src);
         double *pd = vec(dst);
-        int e = threadIdx.x;
-        for (; e + (kU - 1) * BS < n; e += kU * BS) {
-            double v[kU];
-#pragma unroll
-            for (int u = 0; u < kU; u++) v[u] = ps[e + u * BS];
-#pragma unroll
-            for (int u = 0; u < kU; u++) pd[e + u * BS] = v[u];
-        }
-        for (; e < n; e += BS) pd[e] = ps[e];
+        double v[kU];
+        batches([&](int u, int i) { v[u] = ps[i]; }, [&](int u, int i) { pd[i] = v[u]; });
     }
     __device__ void scal(double a, int xv)
     {
         touch(xv);
         double *p = vec(xv);
-        int e = threadIdx.x;
-        for (; e + (kU - 1) * BS < n; e += kU * BS) {
-            double v[kU];
-#pragma unroll
-            for (int u = 0; u < kU; u++) v[u] = p[e + u * BS];
-#pragma unroll
-            for (int u = 0; u < kU; u++) p[e + u * BS] = a * v[u];
-        }
-        for (; e < n; e += BS) p[e] = a * p[e];
+        double v[kU];
+        batches([&](int u, int i) { v[u] = p[i]; }, [&](int u, int i) { p[i] = a * v[u]; });
     }
     __device__ void diff(int xv, int yv, int z)
     {
         touch(z);
         const double *px = vec(xv), *py = vec(yv);
         double *pz = vec(z);
-        int e = threadIdx.x;
-        for (; e + (kU - 1) * BS < n; e += kU * BS) {
-            double vx[kU], vy[kU];
-#pragma unroll
-            for (int u = 0; u < kU; u++) {
-                vx[u] = px[e + u * BS];
-                vy[u] = py[e + u * BS];
-            }
-#pragma unroll
-            for (int u = 0; u < kU; u++) pz[e + u * BS] = vx[u] - vy[u];
-        }
-        for (; e < n; e += BS) pz[e] = px[e] - py[e];
+        double vx[kU], vy[kU];
+        batches([&](int u, int i) { vx[u] = px[i]; vy[u] = py[i]; },
+                [&](int u, int i) { pz[i] = vx[u] - vy[u]; });
     }
     __device__ double maxabs(int xv) const
     {
         const double *p = vec(xv);
-        double m = 0.0;
-        int e = threadIdx.x;
-        for (; e + (kU - 1) * BS < n; e += kU * BS) {
-            double v[kU];
-#pragma unroll
-            for (int u = 0; u < kU; u++) v[u] = p[e + u * BS];
-#pragma unroll
-            for (int u = 0; u < kU; u++) m = fmax(m, fabs(v[u]));
-        }
-        for (; e < n; e += BS) m = fmax(m, fabs(p[e]));
+        double m = 0.0, v[kU];
+        batches([&](int u, int i) { v[u] = p[i]; }, [&](int u, int) { m = fmax(m, fabs(v[u])); });
         return block_reduce(m, true);
     }
     __device__ double l1(int xv) const
     {
         const double *p = vec(xv);
-        double t = 0.0;
-        int e = threadIdx.x;
-        for (; e + (kU - 1) * BS < n; e += kU * BS) {
-            double v[kU];
-#pragma unroll
-            for (int u = 0; u < kU; u++) v[u] = p[e + u * BS];
-#pragma unroll
-            for (int u = 0; u < kU; u++) t += fabs(v[u]);
-        }
-        for (; e < n; e += BS) t += fabs(p[e]);
+        double t = 0.0, v[kU];
+        batches([&](int u, int i) { v[u] = p[i]; }, [&](int u, int) { t += fabs(v[u]); });
         return block_reduce(t, false);
     }
     __device__ double u(int i) const { return lds_u[i]; }
