@@ -468,7 +468,9 @@ int ccgpu_solver_switch_eval(ccgpu_solver *s, int64_t job, int *evaluation);
  * solver clock) in t[3]; relative (N x dim, may be NULL) receives the relaxed configuration. */
 int ccgpu_solver_result(ccgpu_solver *s, int64_t job, int *state, double *energy, int *evaluations,
                         int *device_evaluations, int *status, double *t, double *relative);
+/* Forget a finished job and free its slot (no counterpart in the reference). */
 int ccgpu_solver_release(ccgpu_solver *s, int64_t job);
+/* Resident and queued job counts and the HBM the solver holds (no counterpart in the reference). */
 int ccgpu_solver_stats(ccgpu_solver *s, int *resident, int *queued, int64_t *bytes_in_use);
 /* Kernel timing (no counterpart in the reference): when on, hipEvents on the solver's stream bracket the
  * rho and force contraction launches of every evaluation; read the accumulated milliseconds and launch
