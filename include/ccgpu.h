@@ -65,6 +65,7 @@ int ccgpu_potential_destroy(ccgpu_potential *p);
 int ccgpu_potential_add_constraint(ccgpu_potential *p, const double *k, double V);
 /* Replace the whole constraint vector (constraints = ...): k M x dim, V M. */
 int ccgpu_potential_set_constraints(ccgpu_potential *p, int64_t M, const double *k, const double *V);
+/* constraints.size() (public member, CollectiveCoordinatePotential.h:96): M. */
 int ccgpu_potential_num_constraints(const ccgpu_potential *p, int64_t *M);
 
 /* Soft-core / contact pair kernel for every later launch (no counterpart in the reference): 0 automatic (one
@@ -233,6 +234,7 @@ int ccgpu_multi_species_indices(const ccgpu_multi *p, int64_t species, int64_t c
 int ccgpu_multi_add_constraint(ccgpu_multi *p, int64_t species, const double *k, double V);
 /* Replace a species' constraint vector: k M x dim, V M. */
 int ccgpu_multi_set_species_constraints(ccgpu_multi *p, int64_t species, int64_t M, const double *k, const double *V);
+/* species_constraints[species].CCPotential->constraints.size(): M. */
 int ccgpu_multi_num_constraints(const ccgpu_multi *p, int64_t species, int64_t *M);
 
 /* public members u0, e0 -- MultiStealthyPotential.h:306-307. */
@@ -245,6 +247,7 @@ int ccgpu_multi_set_configuration(ccgpu_multi *p, const double *basis, int64_t N
 int ccgpu_multi_set_radii(ccgpu_multi *p, int64_t N, const double *radii);
 /* SetPacking(packing) -- MultiStealthyPotential.cpp:461-489: positions and radii, cutoff 2.01 max r. */
 int ccgpu_multi_set_packing(ccgpu_multi *p, const double *basis, int64_t N, const double *relative, const double *radii);
+/* public member radii (MultiStealthyPotential.h:121) in the caller's particle order: N doubles. */
 int ccgpu_multi_get_radii(const ccgpu_multi *p, double *radii);
 /* GetRc() -- MultiStealthyPotential.h:431. */
 int ccgpu_multi_get_rc(const ccgpu_multi *p, double *rc);
@@ -357,6 +360,7 @@ int ccgpu_md_set_velocities(ccgpu_md *m, const double *v);
 /* Position (relative, N x dim), Velocity, Acceleration, Time, TimeStep and xi; any pointer may be null */
 int ccgpu_md_get_state(ccgpu_md *m, double *relative, double *velocity, double *acceleration, double *time,
                        double *timestep, double *xi);
+/* public member TimeStep (potential/MD_System.h:38): the step of the next Evolve calls. */
 int ccgpu_md_set_timestep(ccgpu_md *m, double timestep);
 /* GetKineticEnergy (MD_System.cpp:370-377) */
 int ccgpu_md_kinetic_energy(ccgpu_md *m, double *Ek);
