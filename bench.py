@@ -93,8 +93,8 @@ def copy_probe(mib, repeats=1):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the path shards by "
-              "configuration with no exchange, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py: measures one GPU only (asked for --gpus {args.gpus}); the path shards by configuration "
+              "(each GPU relaxes its own configurations, with no exchange between GPUs)", file=sys.stderr)
         sys.exit(2)
 
     from modules_amd import GetKs, RandomConfiguration, getK

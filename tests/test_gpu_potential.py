@@ -2,7 +2,7 @@
 
 Tolerances follow the reference test's scale-aware closeness (potential/tests/finufft_type1/
 test_finufft_type1.cpp:197-212): rho ~ N, energy ~ sum|V| N^2 / vol, force ~ 4 sum|V| N kmax / vol,
-with relative tolerance 1e-12 on those scales (north star: FP64 within 1e-12 relative).
+with relative tolerance 1e-12 on those scales (the parity bar: FP64 within 1e-12 relative).
 """
 import numpy as np
 import pytest
