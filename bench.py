@@ -1,14 +1,17 @@
 """Benchmark: stealthy ground states per second on one MI355X.
 
-Workload (one "step"): a batch of 64 seeded configurations of N=2000 uniform-random points in a periodic cube
-at density 1 (3D), stealthy constraint set v(k)=1 for 0 < |k| < K with K = getK(chi=0.45, 1, 3) (GetKs, half
-space), each relaxed with RelaxStructure_NLOPT_Emin (three L-BFGS passes, history 40, at most 10000 evaluations
-per pass, Emin = 1e-18) by the device-resident batched solver.  A solve counts as a ground state when its final
-energy is below 1e-18.  Inputs are synthetic (seeded MT19937), generated before the timed region.
+Headline workload (one "step" = one batch): 256 independent configurations per batch, N log-uniform on [100, 4000],
+d in {2, 3} by a fair coin and chi uniform on [0.2, 0.45] (one MT19937 stream seeded 20261203 draws N, d, chi and a
+configuration seed per member), each a seeded uniform-random configuration at density 1 with the stealthy constraint
+set v(k)=1 for 0 < |k| < K(chi) (GetKs, half space), relaxed with RelaxStructure_NLOPT_Emin (three L-BFGS passes,
+history 40, at most 10000 evaluations per pass, Emin = 1e-18) by the device-resident batched solver.  A solve counts
+as a ground state when its final energy is below 1e-18.  The timed batches use the same shapes with fresh
+configuration seeds; their inputs are generated and admitted (uploaded, resident in HBM) before the timed region.
 
-Prints one JSON line: value = ground states / s over the timed steps, plus E+F evaluations/s, p50/p99 solve
-latency, a roofline record for the E+F contraction kernels (hipEvent-timed on the solver's stream) and a CPU
-baseline (the oracle's OpenMP E+F restatement timed on this host, see cpu_baseline.sample).
+Prints one JSON line: value = ground states / s over the timed batches, plus E+F evaluations/s, a roofline record
+for the E+F contraction kernels (hipEvent-timed on the solver's stream), the other workloads' records (64 seeds of
+3D N=2000; 2D N=20000 in FP64 and mixed precision with the error against the oracle; a Poisson request stream with
+soft core; the single-solve drop-in path) and a CPU baseline (timed oracle ground states, see cpu_baseline.sample).
 """
 import argparse
 import json
@@ -24,53 +27,53 @@ sys.path.insert(0, ROOT)
 
 FP64_VALU_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak (AMD data sheet)
 
-# HBM bytes per launch of the force kernel at the headline shape (64 x 3D N=2000, M=2724, tiles 24,0,128,32,1),
-# from rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by the library copy kernel's counter factors; the
-# passes, the commands and the factors are in profiles/counters_ef_lbfgs_batched64_3d_N2000_chi045.md
-FORCE_TRAFFIC_BYTES = {(3, 2000, 2724, 64): {"read": 43.5e6, "write": 24.2e6}}
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=64, help="configurations per step")
-    ap.add_argument("--N", type=int, default=2000)
-    ap.add_argument("--dim", type=int, default=3)
-    ap.add_argument("--chi", type=float, default=0.45)
+    ap.add_argument("--steps", type=int, default=3, help="timed 256-configuration batches")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed batches before them")
     ap.add_argument("--emin", type=float, default=1e-18)
     ap.add_argument("--maxstep", type=int, default=10000)
     ap.add_argument("--history", type=int, default=40)
     ap.add_argument("--check-interval", type=int, default=16)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound of the CPU-baseline sample")
+    ap.add_argument("--resident", default="all", choices=["all", "step"],
+                    help="all: the timed batches are admitted together and advance in the same launches "
+                         "(continuous batching, one tail at the end); step: one batch at a time")
+    ap.add_argument("--cpu-seconds", type=float, default=30.0,
+                    help="bound of the CPU-baseline sample (the stratified ground states stop once it is spent)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-sub", action="store_true", help="headline only: skip the other workloads' records")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
-                    help="run one torch device copy of this size first (calibrates rocprof FETCH/WRITE_SIZE)")
-    ap.add_argument("--graph", action="store_true", help="replay optimizer steps from a captured hipGraph")
+                    help="run the library's copy kernel over this size first (calibrates rocprof FETCH/WRITE_SIZE)")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay optimizer steps from captured hipGraphs (per-launch kernel timing is then off: the "
+                         "kernel roofline comes from one extra stream-launched batch after the timed ones)")
     ap.add_argument("--lanes", type=int, default=1, choices=[1, 2],
                     help="solver streams (2: the L-BFGS kernels of one overlap the E+F kernels of the other; the "
                          "kernels then share the GPU and their event durations no longer give a per-kernel roofline)")
-    ap.add_argument("--no-autotune", action="store_true", help="skip the tile autotuner (default tiles)")
+    ap.add_argument("--autotune", action="store_true", help="time candidate tiles per shape class first")
     ap.add_argument("--tiles", type=str, default=None,
-                    help="seg,rho_kernel,rho_chunk,force_chunk,force_ppl: use these tiles instead of autotuning")
+                    help="seg2d/seg3d,rho_kernel,rho_chunk,force_chunk,force_ppl for every shape class of the batch "
+                         "(0 chunks: automatic), e.g. 32/24,2,0,0,2, instead of the default rule or the autotuner")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
-                    help="batched / mixed256: FP64, or FP32 phases and force terms with FP64 sums and an FP64 "
-                         "polish below 1e-8")
-    ap.add_argument("--workload", default="batched",
-                    choices=["batched", "single", "mixed256", "large2d", "mixed2d", "serve", "five", "nufft2d",
+                    help="FP64, or FP32 phases and force terms with FP64 sums and an FP64 polish below 1e-8")
+    ap.add_argument("--workload", default="batch256",
+                    choices=["batch256", "seeds64", "single", "large2d", "mixed2d", "serve", "five", "nufft2d",
                              "targets", "md"],
-                    help="batched: the headline (64 seeds, 3D N=2000 chi=0.45); single: the same solves through the "
-                         "single-solve drop-in path; mixed256: 256 configurations of mixed N, d, chi; large2d: 2D "
-                         "N=20000 chi=0.4; mixed2d: the same solved in FP64 and in mixed precision, with the "
+                    help="batch256: the headline (256 configurations of mixed N, d, chi per batch); seeds64: 64 seeds "
+                         "of 3D N=2000 chi=0.45; single: those solves through the single-solve drop-in path; "
+                         "large2d: 2D N=20000 chi=0.4; mixed2d: the same in FP64 and in mixed precision, with the "
                          "mixed-precision error; serve: Poisson request stream with soft core; five: multi-species "
                          "ground states of the reference's five-component study (2D N=2000, 5 x 400); nufft2d: "
                          "large 2D systems through the NUFFT backend against the direct sums; targets: target-S(k) "
                          "(equiluminous) relaxes and a variable-cell solve; md: molecular dynamics steps")
-    ap.add_argument("--requests", type=int, default=48, help="serve: number of requests")
-    ap.add_argument("--rate", type=float, default=20.0, help="serve: arrivals per second")
+    ap.add_argument("--batch", type=int, default=64, help="seeds64 / five / targets: configurations per step")
+    ap.add_argument("--requests", type=int, default=96, help="serve: number of requests")
+    ap.add_argument("--rate", type=float, default=200.0, help="serve: arrivals per second")
     ap.add_argument("--profiler-run", action="store_true",
-                    help="for runs under a counter profiler: no autotuner and no hipEvent timing (roofline null)")
+                    help="for runs under a counter profiler: no hipEvent timing (roofline null), no sub-records")
     ap.add_argument("--profile-evals", type=int, default=0,
                     help="counter-profiler mode: submit one batch, run only this many batched optimizer steps "
                          "(E+F and L-BFGS kernels) and exit; implies --profiler-run and --no-cpu")
@@ -90,164 +93,410 @@ def copy_probe(mib, repeats=1):
     return gbs
 
 
+def _sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except ImportError:
+        pass
+
+
+def _log(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
+def batch_shapes():
+    """The 256-configuration batch: (N, d, chi, seed) per configuration, N log-uniform on [100, 4000], d in {2, 3},
+    chi uniform on [0.2, 0.45], drawn from one MT19937 stream seeded 20261203; with each shape's constraint set."""
+    from modules_amd import GetKs, RandomConfiguration, getK
+    from modules_amd.serving import make_batch
+    reqs = make_batch()
+    ks = [GetKs(RandomConfiguration(0, r.N, r.d), getK(r.chi, 1.0, r.d)) for r in reqs]
+    return reqs, ks
+
+
+def _seed(base, batch_index):
+    """Configuration seed of one member of batch `batch_index` (0 = the batch's own draw)."""
+    return int((base + 1000003 * batch_index) % (2 ** 31 - 1))
+
+
+def _roofline(jobs, rho_ms, force_ms, n_rho, n_force):
+    """Algorithmic FP64 work of the contraction kernels over the timed launches: per (particle, constraint) pair of
+    an E+F evaluation, rho 2d+1 flops and forces 4d+2 (6d+3 in all), summed over every evaluation of every job,
+    divided by the summed hipEvent durations of the rho and force launches (on the solver's stream)."""
+    r_flops = sum(ev * N * M * (2 * d + 1) for ev, N, M, d in jobs)
+    f_flops = sum(ev * N * M * (4 * d + 2) for ev, N, M, d in jobs)
+    pairs = sum(ev * N * M for ev, N, M, d in jobs)
+    if rho_ms <= 0 or force_ms <= 0:
+        return None
+    ach_f = f_flops / (force_ms / 1e3) / 1e12
+    ach_r = r_flops / (rho_ms / 1e3) / 1e12
+    ach_ef = (r_flops + f_flops) / ((rho_ms + force_ms) / 1e3) / 1e12
+    P = FP64_VALU_PEAK_TFLOPS
+    return {
+        "bound": "valu",
+        "kernel": "k_force_lattice (force contraction, FP64 VALU)",
+        "achieved": ach_f, "peak": P, "unit": "TFLOP/s", "frac": ach_f / P,
+        "traffic": None,
+        "traffic_note": "HBM bytes per launch are measured by rocprofv3 FETCH_SIZE / WRITE_SIZE passes, not in this "
+                        "run (profiles/)",
+        "avg_launch_ms": force_ms / max(n_force, 1), "launches": n_force,
+        "flops_per_pair": "4d+2",
+        "rho_kernel": {"kernel": "k_rho_lattice", "achieved": ach_r, "frac": ach_r / P,
+                       "avg_launch_ms": rho_ms / max(n_rho, 1), "launches": n_rho, "flops_per_pair": "2d+1"},
+        "ef_pair": {"achieved": ach_ef, "frac": ach_ef / P, "flops_per_pair": "6d+3"},
+        "pairs_per_ef_mean": pairs / max(n_force, 1),
+    }
+
+
+def headline(args):
+    """The 256-configuration batch relaxed to E < Emin by the device-resident solver."""
+    from modules_amd import RandomConfiguration
+    from modules_amd.optimize import BatchSolver
+    reqs, ks = batch_shapes()
+    B = len(reqs)
+    tiles = None
+    if args.tiles:
+        tiles = record_tiles(reqs, ks, args.tiles)
+    elif args.autotune and not args.profiler_run:
+        tiles = autotune_classes(reqs, ks)
+    K, W = args.steps, args.warmup
+    resident_all = args.resident == "all"
+    slots = B * (max(K, W, 1) if resident_all else 1)
+    solver = BatchSolver(max_slots=slots, check_interval=args.check_interval, graph=args.graph,
+                         precision=args.precision, lanes=args.lanes)
+
+    def submit(batch_index):
+        out = []
+        for r, k in zip(reqs, ks):
+            cfg = RandomConfiguration(_seed(r.seed, batch_index), r.N, r.d)
+            j = solver.submit(cfg, k, np.ones(len(k)), MaxStep=args.maxstep, Emin=args.emin, history=args.history)
+            out.append((j, r, len(k)))
+        return out
+
+    def solve(groups, clock):
+        """Admit the submitted batches (inputs uploaded and resident in HBM), then solve them; only the solve and the
+        result read-back are timed."""
+        solver.step(0)
+        _sync()
+        t0 = time.perf_counter()
+        solver.run(chunk=4096)
+        res = [[(solver.result(j), r, M) for j, r, M in g] for g in groups]
+        _sync()
+        clock.append(time.perf_counter() - t0)
+        for g in groups:
+            for j, _, _ in g:
+                solver.release(j)
+        return res
+
+    if args.profile_evals > 0:
+        # as many batches resident as the timed region holds
+        for s in range(K if resident_all else 1):
+            submit(s)
+        pending = solver.step(args.profile_evals)
+        _sync()
+        if args.copy_probe_mib > 0:
+            copy_probe(args.copy_probe_mib)
+        print(json.dumps({"profile_evals": args.profile_evals, "batch": B, "batches": K if resident_all else 1,
+                          "pending": pending,
+                          "pairs_per_contraction_per_batch": sum(r.N * len(k) for r, k in zip(reqs, ks))}), flush=True)
+        return None
+    if args.copy_probe_mib > 0:
+        copy_probe(args.copy_probe_mib)
+
+    _log(f"warm-up: {W} batch(es) of {B}")
+    warm_clock = []
+    if resident_all and W > 0:
+        solve([submit(1000 + w) for w in range(W)], warm_clock)
+    else:
+        for w in range(W):
+            solve([submit(1000 + w)], warm_clock)
+    timing = not args.profiler_run and not args.graph
+    solver.set_timing(timing)
+    solver.timing(reset=True)
+    clock = []
+    _log(f"timed: {K} batch(es) of {B}, resident={args.resident}, precision={args.precision}, graph={args.graph}")
+    if resident_all:
+        groups = [submit(s) for s in range(K)]
+        results = [x for g in solve(groups, clock) for x in g]
+    else:
+        results = []
+        for s in range(K):
+            results.extend(solve([submit(s)], clock)[0])
+    wall = sum(clock)
+    rho_ms, force_ms, n_rho, n_force = solver.timing(reset=True)
+    launches = solver.launch_stats()
+    roof = _roofline([(res["device_evaluations"], r.N, M, r.d) for res, r, M in results], rho_ms, force_ms, n_rho,
+                     n_force) if timing else None
+    if roof is None and not args.profiler_run:
+        # graph mode: the kernel roofline from one more batch with stream launches and per-launch events
+        solver.set_graph(False)
+        solver.set_timing(True)
+        solver.timing(reset=True)
+        extra = solve([submit(2000)], [])[0]
+        rho_ms, force_ms, n_rho, n_force = solver.timing(reset=True)
+        roof = _roofline([(res["device_evaluations"], r.N, M, r.d) for res, r, M in extra], rho_ms, force_ms, n_rho,
+                         n_force)
+        if roof:
+            roof["measured_on"] = "one extra stream-launched batch after the timed (graph-replayed) batches"
+    solved = sum(1 for res, r, M in results if res["state"] == BatchSolver.DONE and res["energy"] < args.emin)
+    evals = np.array([res["device_evaluations"] for res, r, M in results])
+    lat = np.array([res["t_done"] - res["t_admit"] for res, r, M in results])
+    pairs = sum(r.N * M for res, r, M in results) / max(K, 1)
+    return {
+        "metric": "stealthy ground states/s at E<1e-18",
+        "value": solved / wall,
+        "unit": "ground_states/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": 1e3 * wall / max(K, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32 phases / f64 sums",
+        "data": "synthetic (seeded MT19937 uniform configurations at density 1, GetKs constraint sets, v(k)=1)",
+        "config": {"workload": f"batched stealthy relax: {B} configurations per batch, N log-uniform on [100, 4000], "
+                               f"d in {{2,3}}, chi uniform on [0.2, 0.45] (master seed 20261203), Emin={args.emin}, "
+                               f"3-pass L-BFGS history {args.history}",
+                   "batch": B, "pairs_per_contraction_per_batch": pairs,
+                   "N_range": [min(r.N for r in reqs), max(r.N for r in reqs)],
+                   "resident": args.resident, "graph": bool(args.graph), "precision": args.precision,
+                   "lanes": args.lanes, "tiles": tiles or "default rule per system shape"},
+        "solved": solved,
+        "solves": len(results),
+        "launch_bursts": launches,
+        "ef_evals_per_s": float(evals.sum()) / wall,
+        "device_evals_per_solve_mean": float(evals.mean()),
+        "latency_admit_to_done_p50_s": float(np.percentile(lat, 50)),
+        "latency_admit_to_done_p99_s": float(np.percentile(lat, 99)),
+        "roofline": roof,
+    }
+
+
+def record_tiles(reqs, ks, spec):
+    """Record one tile shape (the --tiles string) for every shape class of the batch."""
+    from modules_amd import autotune
+    segs, rk, rc, fc, ppl = spec.split(",")
+    seg2, seg3 = (segs.split("/") + [segs])[:2] if "/" in segs else (segs, segs)
+    for r, k in zip(reqs, ks):
+        autotune.record(r.N, r.d, len(k), int(seg2 if r.d == 2 else seg3), int(rk), int(rc), int(fc), int(ppl))
+    return {"source": "--tiles", "spec": spec}
+
+
+def autotune_classes(reqs, ks, per_class=1):
+    """Time candidate tiles for one configuration of every (d, N, M) shape class of the batch (the engine's tile
+    table keys on half-octave buckets of N and M) and record them; returns {class: tiles}."""
+    from modules_amd import autotune
+    seen = {}
+    for r, k in zip(reqs, ks):
+        key = (r.d, int(2 * math.log2(r.N)), int(2 * math.log2(max(len(k), 1))))
+        if key in seen:
+            continue
+        seen[key] = autotune.autotune(r.N, r.d, k=k, batch=4, repeats=2)
+    return {str(k): v for k, v in seen.items()}
+
+
+def sub_seeds64(args, check_oracle=True):
+    """64 seeds of 3D N=2000 at chi=0.45 (M=2724), FP64, relaxed side by side; each end point's energy is
+    recomputed by the oracle."""
+    from modules_amd import GetKs, RandomConfiguration, getK
+    from modules_amd.optimize import BatchSolver
+    N, d, chi, B = 2000, 3, 0.45, 64
+    k = GetKs(RandomConfiguration(0, N, d), getK(chi, 1.0, d))
+    v = np.ones(len(k))
+    solver = BatchSolver(max_slots=B, check_interval=args.check_interval, lanes=args.lanes)
+    cfgs = [RandomConfiguration(s, N, d) for s in range(B)]
+    warm = [solver.submit(RandomConfiguration(10 ** 6 + s, N, d), k, v, Emin=args.emin) for s in range(8)]
+    res, _ = _solve_all(solver, warm, args.emin)
+    for j in warm:
+        solver.release(j)
+    jobs = [solver.submit(c, k, v, Emin=args.emin) for c in cfgs]
+    res, wall = _solve_all(solver, jobs, args.emin)
+    solved = sum(1 for r in res if r["energy"] < args.emin)
+    out = {"workload": f"{B} seeds x 3D N={N} chi={chi} (M={len(k)}), FP64", "value": solved / wall,
+           "unit": "ground_states/s", "solved": solved, "solves": B, "wall_s": wall,
+           "device_evals_per_solve_mean": float(np.mean([r["device_evaluations"] for r in res]))}
+    if check_oracle:
+        from oracle import pyoracle as po
+        Eo = [po.total_energy_forces(r["relative"], c.basis, k, v)[0] for r, c in zip(res, cfgs)]
+        out["oracle_energy_max"] = float(max(Eo))
+        out["oracle_accepted"] = int(sum(1 for e in Eo if e < 10 * args.emin))
+    return out
+
+
+def sub_mixed2d(args):
+    """2D N=20000 at chi=0.4: the mixed-precision error at the start configuration against the oracle, then the
+    same solve in FP64 and in mixed precision (FP32 phases and force terms, FP64 sums, FP64 polish below 1e-8)."""
+    from modules_amd import GetKs, RandomConfiguration, ShiftedCCPotential, getK
+    from modules_amd.optimize import BatchSolver
+    from oracle import pyoracle as po
+    cfg = RandomConfiguration(4, 20000, 2)
+    k = GetKs(cfg, getK(0.4, 1.0, 2))
+    v = np.ones(len(k))
+    Eo, Fo = po.total_energy_forces(cfg.relative, cfg.basis, k, v)
+    out = {"workload": f"2D N=20000 chi=0.4 (M={len(k)})", "unit": "ground_states/s"}
+    for prec in ("fp64", "mixed"):
+        pot = ShiftedCCPotential(2)
+        for kk in k:
+            pot.AddConstraint(kk, [1.0])
+        pot.SetPrecision(prec)
+        pot.SetConfiguration(cfg)
+        E, F = pot.EnergyAndAllForce()
+        solver = BatchSolver(max_slots=1, check_interval=args.check_interval, precision=prec)
+        job = solver.submit(cfg, k, v, MaxStep=args.maxstep, Emin=args.emin)
+        res, wall = _solve_all(solver, [job], args.emin)
+        r = res[0]
+        out[prec] = {"start_vs_oracle": {"dE_over_E": abs(E - Eo) / abs(Eo),
+                                         "max_dF_over_max_F": float(np.max(np.abs(F - Fo)) / np.max(np.abs(Fo)))},
+                     "value": (1.0 if r["energy"] < args.emin else 0.0) / wall, "wall_s": wall,
+                     "energy": r["energy"], "objective_evals": r["evaluations"],
+                     "device_evals": r["device_evaluations"], "switch_eval": r["switch_eval"],
+                     "ms_per_eval": 1e3 * wall / max(r["device_evaluations"], 1)}
+        out[prec]["oracle_energy"] = po.total_energy_forces(r["relative"], cfg.basis, k, v)[0]
+    out["speedup_mixed_over_fp64"] = out["fp64"]["wall_s"] / out["mixed"]["wall_s"]
+    return out
+
+
+def sub_serving(args, requests=None, rate=None):
+    """Poisson request stream (N lognormal median 1000, sigma_ln 0.6, clipped to [100, 50000]; d in {2,3}; chi
+    uniform on [0.1, 0.49]; soft core e0 = 1, sigma = 0.4 density^(-1/d)), arrivals faster than one at a time can be
+    served, on one continuous-batching solver; latency = completion - arrival."""
+    from modules_amd.serving import make_stream, serve
+    requests = requests or args.requests
+    rate = rate or args.rate
+    serve(make_stream(8, rate, seed=99), Emin=args.emin, MaxStep=args.maxstep, max_slots=64)  # warm
+    reqs, wall = serve(make_stream(requests, rate), Emin=args.emin, MaxStep=args.maxstep, max_slots=256)
+    lat = np.array([r.done - r.arrival for r in reqs])
+    solved = sum(1 for r in reqs if r.result.get("energy", 1.0) < args.emin)
+    return {"workload": f"serving: {requests} Poisson requests at {rate}/s, soft core e0=1 sigma=0.4",
+            "value": solved / wall, "unit": "ground_states/s", "solved": solved, "requests": len(reqs),
+            "wall_s": wall, "latency_p50_s": float(np.percentile(lat, 50)),
+            "latency_p99_s": float(np.percentile(lat, 99)), "N_median": float(np.median([r.N for r in reqs])),
+            "N_max": int(max(r.N for r in reqs)), "last_arrival_s": float(max(r.arrival for r in reqs))}
+
+
+def sub_single(args, seeds=2):
+    """The single-solve drop-in path (host L-BFGS in NLopt order, one GPU E+F per objective evaluation through the
+    Potential interface) on seeds of 3D N=2000 chi=0.45."""
+    from modules_amd import GetKs, RandomConfiguration, ShiftedCCPotential, getK
+    from modules_amd.optimize import RelaxStructure_NLOPT_Emin
+    N, d, chi = 2000, 3, 0.45
+    cfgs = [RandomConfiguration(s, N, d) for s in range(seeds)]
+    k = GetKs(cfgs[0], getK(chi, 1.0, d))
+    pot = ShiftedCCPotential(d)
+    for kk in k:
+        pot.AddConstraint(kk, [1.0])
+    RelaxStructure_NLOPT_Emin(RandomConfiguration(999999, N, d), pot, MaxStep=200, Emin=args.emin)  # warm
+    t0 = time.perf_counter()
+    res = [RelaxStructure_NLOPT_Emin(c, pot, MaxStep=args.maxstep, Emin=args.emin) for c in cfgs]
+    wall = time.perf_counter() - t0
+    solved = sum(1 for E, ev, st in res if E < args.emin)
+    evals = sum(ev for E, ev, st in res)
+    return {"workload": f"single-solve drop-in path: {seeds} seeds x 3D N={N} chi={chi} (M={len(k)})",
+            "value": solved / wall, "unit": "ground_states/s", "solved": solved, "solves": seeds, "wall_s": wall,
+            "ms_per_objective_eval": 1e3 * wall / max(evals, 1), "latency_mean_s": wall / seeds}
+
+
+def _cpu_topology():
+    """Threads this process may run on, and the physical cores under them (sysfs topology)."""
+    cpus = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else list(range(os.cpu_count() or 1))
+    cores = set()
+    for c in cpus:
+        try:
+            base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+            cores.add((open(base + "physical_package_id").read().strip(), open(base + "core_id").read().strip()))
+        except OSError:
+            cores.add(("?", str(c)))
+    return len(cpus), len(cores)
+
+
+def cpu_baseline_batch(reqs, ks, args, strata=8):
+    """Ground states timed on the host CPU: the oracle (the reference direct path restated in C, OpenMP over
+    constraints and particles, libm sin/cos per pair) driven by the reference's own LuksanPLIS (compiled from
+    potential/LuksanPLIS.cpp into oracle/_ref) through the three-pass RelaxStructure_NLOPT_Emin.  The sample is
+    stratified over the batch: the 256 configurations are sorted by N*M and cut into `strata` equal groups, and the
+    middle configuration of each group is solved, cheapest group first, until --cpu-seconds is spent.  With equal
+    groups the batch's CPU rate is (solved in the sample) / (sample seconds) over the groups covered; the record
+    says how many."""
+    os.environ.setdefault("OMP_WAIT_POLICY", "ACTIVE")
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    from modules_amd import GetKs, RandomConfiguration, getK
+    from oracle import pyoracle as po
+    threads_avail, phys = _cpu_topology()
+    threads = min(16, threads_avail)
+    order = sorted(range(len(reqs)), key=lambda i: reqs[i].N * len(ks[i]))
+    g = len(order) // strata
+    picks = [order[q * g + g // 2] for q in range(strata)]
+    sample, spent = [], 0.0
+    for i in picks:
+        if spent >= args.cpu_seconds:
+            break
+        r, k = reqs[i], ks[i]
+        cfg = RandomConfiguration(r.seed, r.N, r.d)
+        t0 = time.perf_counter()
+        E, s, ev, st = po.relax_emin(cfg.relative, cfg.basis, k, np.ones(len(k)), maxstep=args.maxstep,
+                                     Emin=args.emin, nthreads=threads)
+        dt = time.perf_counter() - t0
+        spent += dt
+        sample.append({"N": r.N, "d": r.d, "chi": round(r.chi, 4), "M": len(k), "seconds": dt, "evaluations": ev,
+                       "energy": E, "solved": bool(E < args.emin)})
+    solved = sum(1 for x in sample if x["solved"])
+    secs = sum(x["seconds"] for x in sample)
+    # anchor: the reference's published 2D N=500 chi=0.2996 E+F on 2 threads (4.0535 ms,
+    # docs/finufft/reports/one-component-benchmark.md:37)
+    acfg = RandomConfiguration(0, 500, 2)
+    ak = GetKs(acfg, getK(0.2996, 1.0, 2))
+    ta = _time_ef(po, acfg, ak, np.ones(len(ak)), 2, 1.0, min_n=30)
+    covered = len(sample)
+    return {"value": solved / secs if secs > 0 else None, "unit": "ground_states/s", "cores": threads, "kind": "port",
+            "sample": f"{covered} of {strata} strata of the 256-configuration batch (sorted by N*M, middle member of "
+                      f"each stratum, cheapest stratum first): timed oracle ground states (RelaxStructure_NLOPT_Emin "
+                      f"with the reference LuksanPLIS, oracle E+F) on {threads} threads, {secs:.1f} s",
+            "strata_covered": covered, "strata": strata,
+            "strata_note": "when fewer strata than all are covered the sample leans to cheap configurations, so the "
+                           "rate overstates the CPU",
+            "solves": sample,
+            "threads": threads, "physical_cores": phys,
+            "anchor_2d_N500_chi0.2996_2threads": {"median_ms_per_ef": 1e3 * float(np.median(ta)), "samples": len(ta),
+                                                  "constraints": len(ak), "published_ms": 4.0535},
+            "cpu_model": _cpu_model(),
+            "omp": {"OMP_WAIT_POLICY": os.environ.get("OMP_WAIT_POLICY"),
+                    "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND")}}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: measures one GPU only (asked for --gpus {args.gpus}); the path shards by configuration "
               "(each GPU relaxes its own configurations, with no exchange between GPUs)", file=sys.stderr)
         sys.exit(2)
-
-    from modules_amd import GetKs, RandomConfiguration, getK
-    from modules_amd.optimize import BatchSolver
-
-    if args.workload != "batched":
-        print(json.dumps(OTHER_WORKLOADS[args.workload](args)), flush=True)
-        return
-
-    N, d, chi = args.N, args.dim, args.chi
-    K = getK(chi, 1.0, d)
-    proto = RandomConfiguration(0, N, d)
-    k = GetKs(proto, K)
-    M = len(k)
-    v = np.ones(M)
-
-    def configs(first_seed, count):
-        return [RandomConfiguration(first_seed + i, N, d) for i in range(count)]
-
-    def sync():
-        try:
-            import torch
-            if torch.cuda.is_available():
-                torch.cuda.synchronize()
-        except ImportError:
-            pass
-
-    tiles = None
     if args.profile_evals > 0:
         args.profiler_run = True
         args.no_cpu = True
-    if args.profiler_run:
-        args.no_autotune = True
-    if args.tiles:
-        from modules_amd import autotune
-        vals = [int(t) for t in args.tiles.split(",")]
-        seg, rk, rc, fc = vals[:4]
-        ppl = vals[4] if len(vals) > 4 else 2
-        autotune.record(N, d, M, seg, rk, rc, fc, ppl)
-        tiles = {"segment_len": seg, "rho_kernel": rk, "rho_chunk": rc, "force_chunk": fc, "force_ppl": ppl,
-                 "source": "--tiles"}
-    elif not args.no_autotune:
-        from modules_amd import autotune
-        tiles = autotune.autotune(N, d, k=k, basis=proto.basis, batch=args.batch, repeats=3)
-    solver = BatchSolver(max_slots=max(args.batch, 1), check_interval=args.check_interval, graph=args.graph,
-                         precision=args.precision, lanes=args.lanes)
-
-    def run_step(cfgs, clock=None):
-        """Submit a batch, admit it (inputs uploaded and resident in HBM), then solve and read the results back.
-        Only the solve and the readback are inside `clock` (a list collecting seconds)."""
-        ids = [solver.submit(c, k, v, MaxStep=args.maxstep, Emin=args.emin, history=args.history) for c in cfgs]
-        solver.step(0)  # admission only: uploads, no evaluation
-        sync()
-        t0 = time.perf_counter()
-        solver.run(chunk=1024)
-        res = [solver.result(j) for j in ids]
-        sync()
-        if clock is not None:
-            clock.append(time.perf_counter() - t0)
-        for j in ids:
-            solver.release(j)
-        return res
-
-    if args.profile_evals > 0:
-        ids = [solver.submit(c, k, v, MaxStep=args.maxstep, Emin=args.emin, history=args.history)
-               for c in configs(0, args.batch)]
-        pending = solver.step(args.profile_evals)
-        sync()
-        if args.copy_probe_mib > 0:
-            copy_probe(args.copy_probe_mib)
-        print(json.dumps({"profile_evals": args.profile_evals, "batch": args.batch, "N": N, "dim": d, "M": M,
-                          "pending": pending, "tiles": tiles}), flush=True)
+    if args.workload != "batch256":
+        print(json.dumps(OTHER_WORKLOADS[args.workload](args)), flush=True)
         return
-
-    if args.copy_probe_mib > 0:
-        copy_probe(args.copy_probe_mib)
-    warm = [configs(1_000_000 + 1000 * w, args.batch) for w in range(args.warmup)]
-    timed = [configs(1000 * s, args.batch) for s in range(args.steps)]
-    for cfgs in warm:
-        run_step(cfgs)
-
-    solver.set_timing(not args.profiler_run)
-    solver.timing(reset=True)
-    results = []
-    clock = []
-    sync()
-    t_all = time.perf_counter()
-    for cfgs in timed:
-        results.extend(run_step(cfgs, clock))
-    sync()
-    wall_with_upload = time.perf_counter() - t_all
-    wall = sum(clock)
-    rho_ms, force_ms, n_rho, n_force = solver.timing(reset=True)
-
-    solved = sum(1 for r in results if r["energy"] < args.emin)
-    gs_per_s = solved / wall
-    dev_evals = sum(r["device_evaluations"] for r in results)
-    obj_evals = [r["evaluations"] for r in results]
-    lat = np.array([r["t_done"] - r["t_submit"] for r in results])
-    pairs = dev_evals * N * M  # (particle, constraint) pairs per contraction, summed over all evaluations
-    flops = pairs * (6 * d + 3)  # algorithmic flops per pair per E+F (rho: 2d+1, forces: 4d+2)
-    # dominant kernel: the force contraction (k_force_lattice, 4d+2 flops per pair); the rho contraction
-    # (k_rho_lattice, 2d+1 flops per pair) is reported beside it
-    f_flops = pairs * (4 * d + 2)
-    r_flops = pairs * (2 * d + 1)
-    achieved = f_flops / (force_ms / 1e3) / 1e12 if force_ms > 0 else 0.0
-    achieved_rho = r_flops / (rho_ms / 1e3) / 1e12 if rho_ms > 0 else 0.0
-    achieved_ef = flops / ((rho_ms + force_ms) / 1e3) / 1e12 if rho_ms + force_ms > 0 else 0.0
-
-    out = {
-        "metric": "stealthy ground states/s at E<1e-18",
-        "value": gs_per_s,
-        "unit": "ground_states/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * wall / max(args.steps, 1),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32 phases / f64 sums",
-        "data": "synthetic (seeded MT19937 uniform configurations, GetKs constraint set)",
-        "config": {"workload": f"batched stealthy relax: {args.batch} seeds x N={N}, d={d}, chi={chi} "
-                               f"(M={M}), Emin={args.emin}, 3-pass L-BFGS history {args.history}",
-                   "batch": args.batch, "N": N, "dim": d, "chi": chi, "constraints": M, "tiles": tiles,
-                   "graph": bool(args.graph), "precision": args.precision, "lanes": args.lanes},
-        "solved": solved,
-        "solves": len(results),
-        "ef_evals_per_s": dev_evals / wall,
-        "objective_evals_per_solve_mean": float(np.mean(obj_evals)),
-        "latency_p50_s": float(np.percentile(lat, 50)),
-        "latency_p99_s": float(np.percentile(lat, 99)),
-        "pcie_inclusive_value": solved / wall_with_upload,
-        "roofline": {
-            "bound": "valu",
-            "kernel": "k_force_lattice (force contraction, FP64 VALU)",
-            "achieved": achieved,
-            "peak": FP64_VALU_PEAK_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved / FP64_VALU_PEAK_TFLOPS,
-            "traffic": (_force_traffic(d, N, M, args.batch) or {}).get("bytes_per_launch"),
-            "traffic_detail": _force_traffic(d, N, M, args.batch),
-            "avg_launch_ms": force_ms / max(n_force, 1),
-            "flops_per_pair": 4 * d + 2,
-            "rho_kernel": {"kernel": "k_rho_lattice", "achieved": achieved_rho,
-                           "frac": achieved_rho / FP64_VALU_PEAK_TFLOPS, "avg_launch_ms": rho_ms / max(n_rho, 1),
-                           "flops_per_pair": 2 * d + 1},
-            "ef_pair": {"achieved": achieved_ef, "frac": achieved_ef / FP64_VALU_PEAK_TFLOPS,
-                        "flops_per_pair": 6 * d + 3},
-        },
-    }
+    out = headline(args)
+    if out is None:
+        return
+    if not args.profiler_run and not args.no_sub:
+        subs = {}
+        for name, fn in (("seeds64_3d_N2000", sub_seeds64), ("mixed_2d_N20000", sub_mixed2d),
+                         ("serving", sub_serving), ("single_solve_3d_N2000", sub_single)):
+            _log(f"sub-record {name}")
+            subs[name] = fn(args)
+        out["workloads"] = subs
+        if subs["single_solve_3d_N2000"]["value"] and subs["seeds64_3d_N2000"]["value"]:
+            out["batched_over_single_solve_3d_N2000"] = (subs["seeds64_3d_N2000"]["value"] /
+                                                         subs["single_solve_3d_N2000"]["value"])
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(proto, k, v, float(np.mean(obj_evals)), args.cpu_seconds)
+        _log("CPU baseline")
+        reqs, ks = batch_shapes()
+        out["cpu_baseline"] = cpu_baseline_batch(reqs, ks, args)
+        cb = out["cpu_baseline"]["value"]
+        out["gpu_over_cpu"] = out["value"] / cb if cb else None
     print(json.dumps(out), flush=True)
 
 
@@ -259,14 +508,6 @@ def _tune(args, cfg, k):
     return autotune.autotune(cfg.NumParticle(), cfg.GetDimension(), k=k, basis=cfg.basis, batch=1, repeats=2)
 
 
-def _force_traffic(d, N, M, batch):
-    t = FORCE_TRAFFIC_BYTES.get((d, N, M, batch))
-    if t is None:
-        return None
-    return {"bytes_per_launch": t["read"] + t["write"], "read": t["read"], "write": t["write"],
-            "source": "profiles/counters_ef_lbfgs_batched64_3d_N2000_chi045.md"}
-
-
 def _solve_all(solver, jobs, emin):
     """Admit the submitted jobs (inputs resident in HBM), then run the solver until they finish; returns results
     and the wall seconds of the solve and readback."""
@@ -276,61 +517,6 @@ def _solve_all(solver, jobs, emin):
     wall = time.perf_counter() - t0
     res = [solver.result(j) for j in jobs]
     return res, wall
-
-
-def workload_single(args):
-    """The headline solves through the single-solve drop-in path (host L-BFGS in NLopt order, one GPU E+F pass
-    per objective evaluation), on --batch seeds (default 4 here: each solve is sequential)."""
-    from modules_amd import GetKs, RandomConfiguration, ShiftedCCPotential, getK
-    from modules_amd.optimize import RelaxStructure_NLOPT_Emin
-    n_seeds = min(args.batch, 4) if args.batch == 64 else args.batch
-    N, d, chi = args.N, args.dim, args.chi
-    cfgs = [RandomConfiguration(s, N, d) for s in range(n_seeds)]
-    k = GetKs(cfgs[0], getK(chi, 1.0, d))
-    pot = ShiftedCCPotential(d)
-    for kk in k:
-        pot.AddConstraint(kk, [1.0])
-    warm = RandomConfiguration(999999, N, d)
-    RelaxStructure_NLOPT_Emin(warm, pot, MaxStep=args.maxstep, Emin=args.emin)
-    t0 = time.perf_counter()
-    res = [RelaxStructure_NLOPT_Emin(c, pot, MaxStep=args.maxstep, Emin=args.emin) for c in cfgs]
-    wall = time.perf_counter() - t0
-    solved = sum(1 for E, ev, st in res if E < args.emin)
-    evals = sum(ev for E, ev, st in res)
-    return {"metric": "stealthy ground states/s at E<1e-18", "value": solved / wall, "unit": "ground_states/s",
-            "workload": f"single-solve drop-in path: {n_seeds} seeds x N={N}, d={d}, chi={chi} (M={len(k)})",
-            "solved": solved, "solves": n_seeds, "wall_s": wall, "objective_evals_per_s": evals / wall,
-            "latency_mean_s": wall / n_seeds, "dtype": "f64"}
-
-
-def workload_mixed256(args):
-    """256 independent configurations: N log-uniform in [100, 4000], d in {2, 3}, chi uniform in [0.2, 0.45]
-    (one master MT19937 seed draws N, d, chi and the per-configuration seed), all resident in one solver."""
-    from modules_amd import GetKs, RandomConfiguration, getK
-    from modules_amd.optimize import BatchSolver
-    rng = np.random.RandomState(20261203)
-    specs = []
-    for i in range(256):
-        N = int(round(math.exp(rng.uniform(math.log(100), math.log(4000)))))
-        d = 2 if rng.uniform() < 0.5 else 3
-        chi = float(rng.uniform(0.2, 0.45))
-        specs.append((N, d, chi, int(rng.randint(0, 2 ** 31 - 1))))
-    solver = BatchSolver(max_slots=256, check_interval=args.check_interval, precision=args.precision)
-    jobs, pairs = [], 0
-    for N, d, chi, seed in specs:
-        cfg = RandomConfiguration(seed, N, d)
-        k = GetKs(cfg, getK(chi, 1.0, d))
-        jobs.append(solver.submit(cfg, k, np.ones(len(k)), MaxStep=args.maxstep, Emin=args.emin))
-        pairs += N * len(k)
-    res, wall = _solve_all(solver, jobs, args.emin)
-    solved = sum(1 for r in res if r["energy"] < args.emin)
-    lat = np.array([r["t_done"] - r["t_submit"] for r in res])
-    return {"metric": "stealthy ground states/s at E<1e-18", "value": solved / wall, "unit": "ground_states/s",
-            "workload": "256 configurations, N log-uniform [100,4000], d in {2,3}, chi U[0.2,0.45]",
-            "solved": solved, "solves": len(res), "wall_s": wall,
-            "ef_evals_per_s": sum(r["device_evaluations"] for r in res) / wall,
-            "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
-            "pairs_per_contraction_total": pairs, "dtype": "f64"}
 
 
 def workload_large2d(args):
@@ -349,59 +535,6 @@ def workload_large2d(args):
             "energy": r["energy"],
             "objective_evals": r["evaluations"], "device_evals": r["device_evaluations"], "wall_s": wall,
             "ef_evals_per_s": r["device_evaluations"] / wall, "dtype": "f64"}
-
-
-def workload_mixed2d(args):
-    """2D N=20000 at density 1, chi=0.4: the mixed-precision error at the start configuration (E and forces of
-    the mixed path against the FP64 path, which the tests pin to the oracle at 1e-12), then the same solve in
-    FP64 and in mixed precision (FP32 phases and force terms, FP64 sums, FP64 polish once E < 1e-8)."""
-    from modules_amd import GetKs, RandomConfiguration, ShiftedCCPotential, getK
-    from modules_amd.optimize import BatchSolver
-    cfg = RandomConfiguration(4, 20000, 2)
-    k = GetKs(cfg, getK(0.4, 1.0, 2))
-    v = np.ones(len(k))
-    tiles = _tune(args, cfg, k)
-    pot = ShiftedCCPotential(2)
-    for kk in k:
-        pot.AddConstraint(kk, [1.0])
-    pot.SetConfiguration(cfg)
-    E64, F64 = pot.EnergyAndAllForce()
-    pot.SetPrecision("mixed")
-    pot.SetConfiguration(cfg)
-    E32, F32 = pot.EnergyAndAllForce()
-    out = {"metric": "stealthy ground states/s at E<1e-18", "unit": "ground_states/s",
-           "workload": f"2D N=20000 chi=0.4 (M={len(k)}): mixed precision against FP64", "tiles": tiles,
-           "start_config_error": {"dE_over_E": abs(E32 - E64) / abs(E64),
-                                  "max_dF_over_max_F": float(np.max(np.abs(F32 - F64)) / np.max(np.abs(F64)))}}
-    for prec in ("fp64", "mixed"):
-        solver = BatchSolver(max_slots=1, check_interval=args.check_interval, precision=prec)
-        job = solver.submit(cfg, k, v, MaxStep=args.maxstep, Emin=args.emin)
-        res, wall = _solve_all(solver, [job], args.emin)
-        r = res[0]
-        out[prec] = {"value": (1.0 if r["energy"] < args.emin else 0.0) / wall, "wall_s": wall,
-                     "energy": r["energy"], "objective_evals": r["evaluations"],
-                     "device_evals": r["device_evaluations"], "switch_eval": r["switch_eval"],
-                     "ef_evals_per_s": r["device_evaluations"] / wall}
-    out["value"] = out["mixed"]["value"]
-    out["speedup_mixed_over_fp64"] = out["mixed"]["value"] / out["fp64"]["value"] if out["fp64"]["value"] else None
-    out["dtype"] = "f32 phases / f64 sums"
-    return out
-
-
-def workload_serve(args):
-    """Poisson request stream (rate --rate/s, --requests requests): N lognormal (median 1000, sigma_ln 0.6,
-    clipped to [100, 50000]), d in {2,3}, chi U[0.1, 0.49], soft core e0 (1 - r/sigma)^2 with e0 = 1 and
-    sigma = 0.4 density^(-1/d); continuous batching on one solver."""
-    from modules_amd.serving import make_stream, serve
-    reqs = make_stream(args.requests, args.rate)
-    reqs, wall = serve(reqs, Emin=args.emin, MaxStep=args.maxstep, max_slots=64)
-    lat = np.array([r.done - r.arrival for r in reqs])
-    solved = sum(1 for r in reqs if r.result.get("energy", 1.0) < args.emin)
-    return {"metric": "stealthy ground states/s at E<1e-18", "value": solved / wall, "unit": "ground_states/s",
-            "workload": f"serving: {args.requests} Poisson requests at {args.rate}/s, soft core e0=1 sigma=0.4",
-            "solved": solved, "requests": len(reqs), "wall_s": wall,
-            "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
-            "N_median": float(np.median([r.N for r in reqs])), "dtype": "f64"}
 
 
 def _shell_kset(cfg, target):
@@ -635,9 +768,9 @@ def workload_md(args):
                                        f"on {threads} threads: one per MD step"}}
 
 
-OTHER_WORKLOADS = {"single": workload_single, "mixed256": workload_mixed256, "large2d": workload_large2d,
-                   "mixed2d": workload_mixed2d, "serve": workload_serve, "five": workload_five,
-                   "nufft2d": workload_nufft2d, "targets": workload_targets, "md": workload_md}
+OTHER_WORKLOADS = {"seeds64": sub_seeds64, "single": sub_single, "large2d": workload_large2d,
+                   "mixed2d": sub_mixed2d, "serve": sub_serving, "five": workload_five, "nufft2d": workload_nufft2d,
+                   "targets": workload_targets, "md": workload_md}
 
 
 def _cpu_model():
@@ -662,37 +795,6 @@ def _time_ef(po, cfg, k, v, threads, seconds, max_n=2000, min_n=1):
         if (time.perf_counter() - t0 >= seconds and len(times) >= min_n) or len(times) >= max_n:
             break
     return np.array(times)
-
-
-def cpu_baseline(cfg, k, v, evals_per_gs, seconds):
-    """The oracle's E+F (the reference direct path restated in C with OpenMP, libm sin/cos per pair) timed on
-    this host for a bounded number of evaluations of the same configuration; ground states/s = E+F/s divided
-    by the objective evaluations one ground state takes (the GPU run's mean; the optimizer is the same).  Also
-    a 1-thread sample and the reference's published sanity anchor (2D N=500 chi=0.2996 on 2 threads:
-    4.0535 ms per E+F, docs/finufft/reports/one-component-benchmark.md:37)."""
-    os.environ.setdefault("OMP_WAIT_POLICY", "ACTIVE")
-    os.environ.setdefault("OMP_PROC_BIND", "close")
-    from modules_amd import GetKs, RandomConfiguration, getK
-    from oracle import pyoracle as po
-
-    threads = min(16, os.cpu_count() or 1)
-    t = _time_ef(po, cfg, k, v, threads, seconds)
-    ef_per_s = len(t) / t.sum()
-    t1 = _time_ef(po, cfg, k, v, 1, min(3.0, seconds / 5))
-    acfg = RandomConfiguration(0, 500, 2)
-    ak = GetKs(acfg, getK(0.2996, 1.0, 2))
-    ta = _time_ef(po, acfg, ak, np.ones(len(ak)), 2, 1.0, min_n=30)
-    return {"value": ef_per_s / evals_per_gs, "unit": "ground_states/s", "cores": threads, "kind": "port",
-            "sample": f"{len(t)} oracle E+F evaluations of one N={cfg.NumParticle()} d={cfg.GetDimension()} "
-                      f"configuration on {threads} threads in {t.sum():.1f} s ({ef_per_s:.3g} E+F/s, median "
-                      f"{1e3 * np.median(t):.1f} ms) divided by {evals_per_gs:.0f} evaluations per ground state",
-            "ef_evals_per_s": ef_per_s, "median_ms_per_ef": 1e3 * float(np.median(t)),
-            "one_thread_median_ms_per_ef": 1e3 * float(np.median(t1)),
-            "anchor_2d_N500_chi0.2996_2threads": {"median_ms_per_ef": 1e3 * float(np.median(ta)), "samples": len(ta),
-                                                  "constraints": len(ak), "published_ms": 4.0535},
-            "cpu_model": _cpu_model(), "host_logical_cpus": os.cpu_count(),
-            "omp": {"OMP_WAIT_POLICY": os.environ.get("OMP_WAIT_POLICY"),
-                    "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND")}}
 
 
 if __name__ == "__main__":

@@ -154,13 +154,16 @@ int ccgpu_potential_all_force(ccgpu_potential *p, double *F);
  * equal to Energy() followed by AllForce(). */
 int ccgpu_potential_energy_and_all_force(ccgpu_potential *p, double *E, double *F);
 
-/* Tile shape of the kernels (segment length 4/8/16/24/32, particles per rho chunk <= 512, constraint
- * chunk of the force pass; 0 keeps the automatic choice).  No counterpart in the reference: the hook of
- * the autotuner. */
+/* Tile shape of the kernels (segment length 4/8/16/24/32, particles per rho chunk (at most 512 for rho kernels 0
+ * and 1, 65536 for kernel 2), segments or constraints per chunk of the force pass; 0 keeps the automatic choice).
+ * No counterpart in the reference: the hook of the autotuner. */
 int ccgpu_potential_set_tiles(ccgpu_potential *p, int segment_len, int rho_chunk, int force_chunk);
 
-/* rho kernel of the lattice path: 0 phase recurrence along each segment, 1 per-particle power tables in LDS,
- * -1 automatic.  No counterpart in the reference (autotuner hook). */
+/* Contraction kernels of the lattice path: 0 one segment per lane with the phase recurrence, 1 per-particle power
+ * tables in LDS (both with the particle-lane force kernel that covers 256 x ppl particles per block); 2 (default) 16
+ * segments x 4 particle rows per wavefront for rho and 64 x ppl particles per block, its four wavefronts sharing the
+ * segments, for the forces; -1 automatic.  Mixed-precision systems use kernel 0.  No counterpart in the reference
+ * (autotuner hook). */
 int ccgpu_potential_set_rho_kernel(ccgpu_potential *p, int variant);
 
 /* Arithmetic of the lattice contractions: 0 FP64 (default); 1 mixed -- each phase is reduced to [-1/2, 1/2]
@@ -194,7 +197,8 @@ int ccgpu_autotune(int dim, int64_t N, const double *basis, int64_t M, const dou
                    double *ms);
 /* Forget every recorded tile shape. */
 int ccgpu_autotune_clear(void);
-/* Record a tile shape for the (dim, N, M) shape class without timing (e.g. one found by an earlier run). */
+/* Record a tile shape for the (dim, N, M) shape class without timing (e.g. one found by an earlier run); a rho or
+ * force chunk of 0 keeps the automatic rule for that chunk. */
 int ccgpu_autotune_record(int dim, int64_t N, int64_t M, int segment_len, int rho_kernel, int rho_chunk,
                           int force_chunk, int force_ppl);
 
@@ -476,6 +480,10 @@ int ccgpu_solver_result(ccgpu_solver *s, int64_t job, int *state, double *energy
 int ccgpu_solver_release(ccgpu_solver *s, int64_t job);
 /* Resident and queued job counts and the HBM the solver holds (no counterpart in the reference). */
 int ccgpu_solver_stats(ccgpu_solver *s, int *resident, int *queued, int64_t *bytes_in_use);
+/* Bursts of optimizer steps (the steps between two host checks) launched so far: replayed from a captured
+ * hipGraph, and enqueued as plain stream launches; graph mode falls back to stream launches while kernel timing
+ * is on or a resident job uses the NUFFT.  No counterpart in the reference. */
+int ccgpu_solver_launch_stats(ccgpu_solver *s, int64_t *graph_bursts, int64_t *stream_bursts);
 /* Kernel timing (no counterpart in the reference): when on, hipEvents on the solver's stream bracket the
  * rho and force contraction launches of every evaluation; read the accumulated milliseconds and launch
  * counts (reset = 1 clears them after reading). */

@@ -534,7 +534,7 @@ int ccgpu_potential_set_tiles(ccgpu_potential *p, int segment_len, int rho_chunk
         Require(segment_len == 0 || segment_len == 4 || segment_len == 8 || segment_len == 16 || segment_len == 24 ||
                     segment_len == 32,
                 CCGPU_ERR_ARG, "segment length must be 0, 4, 8, 16, 24 or 32");
-        Require(rho_chunk >= 0 && rho_chunk <= 512 && force_chunk >= 0, CCGPU_ERR_ARG, "bad chunk size");
+        Require(rho_chunk >= 0 && rho_chunk <= 65536 && force_chunk >= 0, CCGPU_ERR_ARG, "bad chunk size");
         p->sys.tiles.seg = segment_len;
         p->sys.tiles.rho_chunk = rho_chunk;
         p->sys.tiles.force_chunk = force_chunk;
@@ -1261,7 +1261,8 @@ int ccgpu_autotune_record(int dim, int64_t N, int64_t M, int segment_len, int rh
         Require(dim >= 1 && dim <= kMaxDim && N > 0 && M > 0, CCGPU_ERR_ARG, "bad shape");
         Require(segment_len == 4 || segment_len == 8 || segment_len == 16 || segment_len == 24 || segment_len == 32,
                 CCGPU_ERR_ARG, "segment length must be 4, 8, 16, 24 or 32");
-        Require(rho_kernel >= 0 && rho_kernel <= 1 && rho_chunk >= 1 && rho_chunk <= 512 && force_chunk >= 0,
+        Require(rho_kernel >= 0 && rho_kernel <= 2 && rho_chunk >= 0 && rho_chunk <= (rho_kernel == 2 ? 65536 : 512) &&
+                    force_chunk >= 0,
                 CCGPU_ERR_ARG, "bad tile values");
         Require(force_ppl == 1 || force_ppl == 2, CCGPU_ERR_ARG, "force particles per lane must be 1 or 2");
         Tiles t;
@@ -1278,7 +1279,7 @@ int ccgpu_potential_set_rho_kernel(ccgpu_potential *p, int variant)
 {
     return Guard([&] {
         Require(p != nullptr, CCGPU_ERR_ARG, "null handle");
-        Require(variant >= -1 && variant <= 1, CCGPU_ERR_ARG, "rho kernel variant must be -1, 0 or 1");
+        Require(variant >= -1 && variant <= 2, CCGPU_ERR_ARG, "rho kernel variant must be -1, 0, 1 or 2");
         p->sys.tiles.rho_kernel = variant;
         p->batch.dirty = true;
         p->Invalidate();
@@ -1537,6 +1538,15 @@ int ccgpu_solver_stats(ccgpu_solver *s, int *resident, int *queued, int64_t *byt
         if (resident) *resident = s->solver.Resident();
         if (queued) *queued = s->solver.Queued();
         if (bytes_in_use) *bytes_in_use = s->solver.BytesInUse();
+    });
+}
+
+int ccgpu_solver_launch_stats(ccgpu_solver *s, int64_t *graph_bursts, int64_t *stream_bursts)
+{
+    return Guard([&] {
+        Require(s != nullptr, CCGPU_ERR_ARG, "null handle");
+        if (graph_bursts) *graph_bursts = s->solver.GraphBursts();
+        if (stream_bursts) *stream_bursts = s->solver.StreamBursts();
     });
 }
 

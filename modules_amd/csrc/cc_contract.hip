@@ -1,0 +1,405 @@
+// cc_contract.hip -- the FP64 lattice contractions of the CC potential in their batched form (gfx950).
+//
+//   rho_k = sum_j exp(2 pi i n_k . s_j)                   (GetRho_Direct, potential/CollectiveCoordinatePotential.h:1255-1286)
+//   G_j   = sum_k c_k n_k,  c_k = A_k cos - B_k sin        (Force / AllForce, :1377-1448; F = -sum_j G_j b_j)
+// with the constraints grouped into segments: runs of consecutive last-axis index at fixed leading indices, at most
+// SEG long.  Along a segment the phases of one particle follow the three-term recurrence
+// e_{t+1} = 2 cos(phi) e_t - e_{t-1} (cos and sin of an arithmetic progression of angles both satisfy it) from one
+// sincos at the segment's head, so a (particle, constraint) pair costs two FMAs for its phase.
+//
+// rho (k_rho_rows): one wavefront per work item = 16 segments (the 16 lanes of a row) x 4 particle rows (row r takes
+// every fourth particle of the chunk).  Segments are sorted longest first, so the 16 of a wavefront have similar
+// lengths and the slot loop stops at the longest of them.  Each lane accumulates its segment's SEG slots over its
+// particles; the four rows are then summed by two exchange steps (v_permlane16_swap and v_permlane32_swap, which swap
+// rows between two registers), after which each lane holds the complete sums of SEG/4 slots of its segment.  A
+// system whose particles fit one chunk is finished here (slot_finish); otherwise the chunk sums go to rpart.
+//
+// forces (k_force_rows): one block of 4 wavefronts per work item = one group of 64 x PPL particles (lane = particle)
+// and one chunk of segments, which the 4 wavefronts share out (wavefront w takes segments w, w + 4, ...; sorted by
+// length, so the four get about the same work).  The segment's coefficients are staged in LDS and read back as
+// wavefront-uniform broadcasts; every lane walks the same segment, so the slot loop has the segment's own length.
+// The four wavefronts' sums are added in LDS in wavefront order; a system whose segments fit one chunk is finished
+// here (force_finish), otherwise the sums go to fpart.
+//
+// Every sum has a fixed order, so a system's bits do not depend on the other systems of a launch.
+#include "cc_device.h"
+#include "cc_epilogue.h"
+#include "cc_math.h"
+
+namespace ccgpu {
+
+namespace {
+
+// rows 1 and 3 of a swapped with rows 0 and 2 of b (rows are 16 lanes)
+__device__ __forceinline__ void swap_rows16(double &a, double &b)
+{
+    const auto lo = __builtin_amdgcn_permlane16_swap(__double2loint(a), __double2loint(b), false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap(__double2hiint(a), __double2hiint(b), false, false);
+    a = __hiloint2double(hi[0], lo[0]);
+    b = __hiloint2double(hi[1], lo[1]);
+}
+
+// lanes 32..63 of a swapped with lanes 0..31 of b
+__device__ __forceinline__ void swap_halves32(double &a, double &b)
+{
+    const auto lo = __builtin_amdgcn_permlane32_swap(__double2loint(a), __double2loint(b), false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap(__double2hiint(a), __double2hiint(b), false, false);
+    a = __hiloint2double(hi[0], lo[0]);
+    b = __hiloint2double(hi[1], lo[1]);
+}
+
+__device__ __forceinline__ double head_dot(const int4 &h, const double4 &p)
+{
+    return __builtin_fma((double)h.x, p.x, __builtin_fma((double)h.y, p.y, __builtin_fma((double)h.z, p.z, (double)h.w * p.w)));
+}
+
+}  // namespace
+
+constexpr int kRowSegs = 16;  // segments per rho wavefront
+constexpr int kRowSub = 64;   // particles staged in LDS at a time
+
+// ---------------------------------------------------------------------------------------------
+// rho.  items: (sys, block of 16 segments, particle chunk of pchunk_len particles).  One wavefront per block.
+// ---------------------------------------------------------------------------------------------
+struct RhoRowsLds {
+    double2 tab[64];
+    double4 pos[kRowSub];
+    double2 step[kRowSub];
+    double wgt[kRowSub];
+};
+
+// The work item with exactly L slots per segment (L: the block's longest segment rounded up to a multiple of 2; the
+// slots beyond a segment's own length are padding, whose sums nothing reads): the particle loop, the row sums and
+// the output.  A specialisation per L keeps the slot loop free of branches.
+template <int SEG, int L>
+__device__ __forceinline__ void rho_rows_run(const SysDesc &S, const WorkItem &it, RhoRowsLds &lds, int lane, bool valid,
+                                             int seg, int4 h)
+{
+    const int row = lane >> 4;
+    double re[SEG], im[SEG];
+#pragma unroll
+    for (int t = 0; t < SEG; t++) re[t] = im[t] = 0.0;
+    const int j0 = it.t1 * S.pchunk_len;
+    const int j1 = min(S.N, j0 + S.pchunk_len);
+    const bool weighted = S.weight != nullptr;
+    for (int jb = j0; jb < j1; jb += kRowSub) {
+        const int nsub = min(kRowSub, j1 - jb);
+        __syncthreads();
+        if (lane < nsub) {
+            lds.pos[lane] = S.pos[jb + lane];
+            lds.step[lane] = S.step[jb + lane];
+            lds.wgt[lane] = weighted ? S.weight[jb + lane] : 1.0;
+        }
+        __syncthreads();
+        const int iters = (nsub + 3) >> 2;
+        for (int q = 0; q < iters; q++) {
+            const int j = 4 * q + row;
+            const bool ok = j < nsub && valid;
+            const int jj = j < nsub ? j : 0;
+            const double4 p = lds.pos[jj];
+            const double2 st = lds.step[jj];
+            const double w = ok ? lds.wgt[jj] : 0.0;
+            double sn, cs;
+            sincos_rev_tab(head_dot(h, p), lds.tab, sn, cs);
+            // e_t lives in register pair (x0, y0) for even t and (x1, y1) for odd t, each updated in place by
+            // e_{t+1} = 2 cos(phi) e_t - e_{t-1}; the weight rides along the (linear) recurrence
+            double x0 = cs * w, y0 = sn * w;
+            double x1 = __builtin_fma(x0, st.x, -y0 * st.y);
+            double y1 = __builtin_fma(x0, st.y, y0 * st.x);
+            const double k1 = st.x + st.x;
+            re[0] += x0;
+            im[0] += y0;
+#pragma unroll
+            for (int t = 1; t < L; t++) {
+                if (t & 1) {
+                    if (t > 1) {
+                        x1 = __builtin_fma(k1, x0, -x1);
+                        y1 = __builtin_fma(k1, y0, -y1);
+                    }
+                    re[t] += x1;
+                    im[t] += y1;
+                } else {
+                    x0 = __builtin_fma(k1, x1, -x0);
+                    y0 = __builtin_fma(k1, y1, -y0);
+                    re[t] += x0;
+                    im[t] += y0;
+                }
+            }
+        }
+    }
+    // sum the four rows: after the first step rows 0, 2 hold slots t and rows 1, 3 slots t + SEG/2 (pairs of rows
+    // added); after the second, row 0 holds the totals of slots [0, SEG/4), row 2 of [SEG/4, SEG/2), row 1 of
+    // [SEG/2, 3 SEG/4) and row 3 of [3 SEG/4, SEG), in re[0 .. SEG/4) and im[0 .. SEG/4); slots from L on are zero
+    constexpr int H = SEG / 2, Q = SEG / 4;
+#pragma unroll
+    for (int t = 0; t < (H < L ? H : L); t++) {
+        double a = re[t], b = re[t + H];
+        swap_rows16(a, b);
+        re[t] = a + b;
+        a = im[t];
+        b = im[t + H];
+        swap_rows16(a, b);
+        im[t] = a + b;
+    }
+#pragma unroll
+    for (int t = 0; t < (Q < L ? Q : L); t++) {
+        double a = re[t], b = re[t + Q];
+        swap_halves32(a, b);
+        re[t] = a + b;
+        a = im[t];
+        b = im[t + Q];
+        swap_halves32(a, b);
+        im[t] = a + b;
+    }
+    if (!valid) return;
+    const int tq = (row == 0 ? 0 : row == 2 ? 1 : row == 1 ? 2 : 3) * Q;  // first slot this lane holds
+    const int qs = seg * SEG + tq;
+    if (S.flags & kSysRhoFinal) {
+#pragma unroll
+        for (int t = 0; t < Q; t++) slot_finish(S, qs + t, re[t], im[t]);
+    } else {
+        double2 *out = reinterpret_cast<double2 *>(S.rpart) + (int64_t)it.t1 * S.nslots + qs;
+#pragma unroll
+        for (int t = 0; t < Q; t++) out[t] = make_double2(re[t], im[t]);
+    }
+}
+
+template <int SEG, int L = 2>
+__device__ __forceinline__ void rho_rows_dispatch(int wlen, const SysDesc &S, const WorkItem &it, RhoRowsLds &lds,
+                                                  int lane, bool valid, int seg, int4 h)
+{
+    if constexpr (L >= SEG) {
+        rho_rows_run<SEG, SEG>(S, it, lds, lane, valid, seg, h);
+    } else {
+        if (wlen <= L) rho_rows_run<SEG, L>(S, it, lds, lane, valid, seg, h);
+        else rho_rows_dispatch<SEG, L + 2>(wlen, S, it, lds, lane, valid, seg, h);
+    }
+}
+
+template <int SEG>
+__global__ void __launch_bounds__(64) k_rho_rows(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
+{
+    static_assert(SEG % 4 == 0, "segment length must be a multiple of 4");
+    __shared__ RhoRowsLds lds;
+    const WorkItem it = items[blockIdx.x];
+    const SysDesc &S = sys[it.sys];
+    if (!sys_active(sys, S)) return;
+    sincos_table_fill(lds.tab);
+    const int lane = threadIdx.x, col = lane & 15;
+    const int seg0 = it.t0 * kRowSegs;
+    const int seg = seg0 + col;
+    const bool valid = seg < S.nseg;
+    const int4 h = valid ? S.seg_head[seg] : make_int4(0, 0, 0, 0);
+    // sorted longest first: the block's first segment bounds the slots any lane needs
+    const int wlen = __builtin_amdgcn_readfirstlane(S.seg_len[seg0]);
+    rho_rows_dispatch<SEG>(wlen, S, it, lds, lane, valid, seg, h);
+}
+
+// ---------------------------------------------------------------------------------------------
+// forces.  items: (sys, particle group of 64 x PPL particles, segment chunk of kchunk_len segments).  256 threads.
+// ---------------------------------------------------------------------------------------------
+constexpr int kForceStageRows = 16;  // segments staged in LDS at a time (4 per wavefront)
+
+template <int SEG>
+struct ForceRowsLds {
+    double2 tab[64];
+    double2 coef[kForceStageRows * SEG];
+    int4 head[kForceStageRows];
+    int len[kForceStageRows];
+};
+
+// One segment of L slots (its length rounded up to a multiple of 2: a padding slot has zero coefficients) for the
+// lane's PPL particles.  Two chains per particle (even and odd slots), each stepped by the three-term recurrence in
+// 2 phi, e_{t+2} = 2 cos(2 phi) e_t - e_{t-2}, in place in two register pairs, from e_{-2}, e_{-1} (the recurrence in
+// phi run backwards from e_0, e_1).  Per slot and particle: c = A cos - B sin, sum c, sum t c, one step: 6 FP64
+// operations; the slot's coefficients are read once for the PPL particles.
+template <int SEG, int PPL, int L>
+__device__ __forceinline__ void force_rows_segment(const double2 *cf, int4 h, double hl, int d, const double4 (&p)[PPL],
+                                                   const double (&stx)[PPL], const double (&sty)[PPL],
+                                                   const double (&k2)[PPL], const double2 *tab, double (&g)[PPL][4])
+{
+    // chain ch of particle q, register pair r: R[q][ch][r]; at slot t = 2 m + ch the current phase is
+    // R[q][ch][(m + 1) & 1], and R[q][ch][m & 1] then becomes the phase two slots later
+    double Rc[PPL][2][2], Rs[PPL][2][2];
+    double acc[PPL][2], accl[PPL][2];
+#pragma unroll
+    for (int q = 0; q < PPL; q++) {
+        double sn0, cs0;
+        sincos_rev_tab(head_dot(h, p[q]), tab, sn0, cs0);
+        const double cs1 = __builtin_fma(cs0, stx[q], -sn0 * sty[q]);
+        const double sn1 = __builtin_fma(cs0, sty[q], sn0 * stx[q]);
+        const double k1 = stx[q] + stx[q];
+        const double cm1 = __builtin_fma(k1, cs0, -cs1);  // e_{-1} = 2 cos(phi) e_0 - e_1
+        const double sm1 = __builtin_fma(k1, sn0, -sn1);
+        Rc[q][0][0] = __builtin_fma(k1, cm1, -cs0);  // e_{-2} = 2 cos(phi) e_{-1} - e_0
+        Rs[q][0][0] = __builtin_fma(k1, sm1, -sn0);
+        Rc[q][0][1] = cs0;
+        Rs[q][0][1] = sn0;
+        Rc[q][1][0] = cm1;
+        Rs[q][1][0] = sm1;
+        Rc[q][1][1] = cs1;
+        Rs[q][1][1] = sn1;
+        acc[q][0] = acc[q][1] = accl[q][0] = accl[q][1] = 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < L; t++) {
+        const int ch = t & 1, m = t >> 1;
+        const double2 ab = cf[t];
+#pragma unroll
+        for (int q = 0; q < PPL; q++) {
+            const double cc = Rc[q][ch][(m + 1) & 1], ss = Rs[q][ch][(m + 1) & 1];
+            const double c = __builtin_fma(ab.x, cc, -ab.y * ss);
+            acc[q][ch] += c;
+            accl[q][ch] = __builtin_fma((double)t, c, accl[q][ch]);
+            if (t + 2 < L) {
+                Rc[q][ch][m & 1] = __builtin_fma(k2[q], cc, -Rc[q][ch][m & 1]);
+                Rs[q][ch][m & 1] = __builtin_fma(k2[q], ss, -Rs[q][ch][m & 1]);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < PPL; q++) {
+        // sum_t (hl + t) c_t = hl sum c + sum t c; leading components are constant along the segment
+        const double A = acc[q][0] + acc[q][1];
+        const double Al = __builtin_fma(hl, A, accl[q][0] + accl[q][1]);
+        if (d == 1) {
+            g[q][0] += Al;
+        } else if (d == 2) {
+            g[q][0] = __builtin_fma((double)h.x, A, g[q][0]);
+            g[q][1] += Al;
+        } else if (d == 3) {
+            g[q][0] = __builtin_fma((double)h.x, A, g[q][0]);
+            g[q][1] = __builtin_fma((double)h.y, A, g[q][1]);
+            g[q][2] += Al;
+        } else {
+            g[q][0] = __builtin_fma((double)h.x, A, g[q][0]);
+            g[q][1] = __builtin_fma((double)h.y, A, g[q][1]);
+            g[q][2] = __builtin_fma((double)h.z, A, g[q][2]);
+            g[q][3] += Al;
+        }
+    }
+}
+
+template <int SEG, int PPL, int L = 2>
+__device__ __forceinline__ void force_rows_dispatch(int len, const double2 *cf, int4 h, double hl, int d,
+                                                    const double4 (&p)[PPL], const double (&stx)[PPL],
+                                                    const double (&sty)[PPL], const double (&k2)[PPL],
+                                                    const double2 *tab, double (&g)[PPL][4])
+{
+    if constexpr (L >= SEG) {
+        force_rows_segment<SEG, PPL, SEG>(cf, h, hl, d, p, stx, sty, k2, tab, g);
+    } else {
+        if (len <= L) force_rows_segment<SEG, PPL, L>(cf, h, hl, d, p, stx, sty, k2, tab, g);
+        else force_rows_dispatch<SEG, PPL, L + 2>(len, cf, h, hl, d, p, stx, sty, k2, tab, g);
+    }
+}
+
+template <int SEG, int PPL>
+__global__ void __launch_bounds__(kBlock) k_force_rows(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
+{
+    __shared__ ForceRowsLds<SEG> L;
+    __shared__ double4 gsum[4][PPL][64];
+    const WorkItem it = items[blockIdx.x];
+    const SysDesc &S = sys[it.sys];
+    if (!sys_active(sys, S)) return;
+    sincos_table_fill(L.tab);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int d = S.d;
+    double4 p[PPL];
+    double stx[PPL], sty[PPL], k2[PPL];
+    double g[PPL][4];
+#pragma unroll
+    for (int q = 0; q < PPL; q++) {
+        const int i = (it.t0 * PPL + q) * 64 + lane;
+        const bool ok = i < S.N;
+        p[q] = ok ? S.pos[i] : make_double4(0, 0, 0, 0);
+        const double2 st = ok ? S.step[i] : make_double2(1, 0);
+        stx[q] = st.x;
+        sty[q] = st.y;
+        const double c2 = st.x * st.x - st.y * st.y;  // cos(2 phi)
+        k2[q] = c2 + c2;
+        g[q][0] = g[q][1] = g[q][2] = g[q][3] = 0.0;
+    }
+    const int s0 = it.t1 * S.kchunk_len;
+    const int s1 = min(S.nseg, s0 + S.kchunk_len);
+    const double2 *coef = reinterpret_cast<const double2 *>(S.coef);
+    for (int sb = s0; sb < s1; sb += kForceStageRows) {
+        const int ns = min(kForceStageRows, s1 - sb);
+        __syncthreads();  // the previous group is consumed
+        for (int x = threadIdx.x; x < ns * SEG; x += kBlock) L.coef[x] = coef[(int64_t)sb * SEG + x];
+        if (threadIdx.x < ns) {
+            L.head[threadIdx.x] = S.seg_head[sb + threadIdx.x];
+            L.len[threadIdx.x] = S.seg_len[sb + threadIdx.x];
+        }
+        __syncthreads();
+        for (int sl = wave; sl < ns; sl += 4) {
+            const int4 hv = L.head[sl];
+            const int4 h = make_int4(__builtin_amdgcn_readfirstlane(hv.x), __builtin_amdgcn_readfirstlane(hv.y),
+                                     __builtin_amdgcn_readfirstlane(hv.z), __builtin_amdgcn_readfirstlane(hv.w));
+            const int len = __builtin_amdgcn_readfirstlane(L.len[sl]);
+            const double hl = (d == 1) ? (double)h.x : (d == 2) ? (double)h.y : (d == 3) ? (double)h.z : (double)h.w;
+            force_rows_dispatch<SEG, PPL>(len, L.coef + sl * SEG, h, hl, d, p, stx, sty, k2, L.tab, g);
+        }
+    }
+    // the four wavefronts' sums, added in wavefront order
+#pragma unroll
+    for (int q = 0; q < PPL; q++) gsum[wave][q][lane] = make_double4(g[q][0], g[q][1], g[q][2], g[q][3]);
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * PPL; e += kBlock) {
+        const int q = e >> 6, l = e & 63;
+        const int i = (it.t0 * PPL + q) * 64 + l;
+        if (i >= S.N) continue;
+        const double4 a = gsum[0][q][l], b = gsum[1][q][l], c = gsum[2][q][l], w = gsum[3][q][l];
+        const double G[4] = {((a.x + b.x) + c.x) + w.x, ((a.y + b.y) + c.y) + w.y, ((a.z + b.z) + c.z) + w.z,
+                             ((a.w + b.w) + c.w) + w.w};
+        if (S.flags & kSysForceFinal) {
+            force_finish(sys, S, i, G);
+        } else {
+            reinterpret_cast<double4 *>(S.fpart)[(int64_t)it.t1 * S.N + i] = make_double4(G[0], G[1], G[2], G[3]);
+        }
+    }
+}
+
+#define CC_INSTANTIATE_ROWS(SEG)                                                                              \
+    template __global__ void k_rho_rows<SEG>(const SysDesc *__restrict__, const WorkItem *__restrict__);       \
+    template __global__ void k_force_rows<SEG, 1>(const SysDesc *__restrict__, const WorkItem *__restrict__);  \
+    template __global__ void k_force_rows<SEG, 2>(const SysDesc *__restrict__, const WorkItem *__restrict__);
+CC_INSTANTIATE_ROWS(4)
+CC_INSTANTIATE_ROWS(8)
+CC_INSTANTIATE_ROWS(16)
+CC_INSTANTIATE_ROWS(24)
+CC_INSTANTIATE_ROWS(32)
+
+void launch_rho_rows(hipStream_t stream, int seg, const SysDesc *sys, const WorkItem *items, int n)
+{
+    if (n <= 0) return;
+    const dim3 grid((unsigned)n), block(64);
+    switch (seg) {
+        case 4: hipLaunchKernelGGL(k_rho_rows<4>, grid, block, 0, stream, sys, items); break;
+        case 8: hipLaunchKernelGGL(k_rho_rows<8>, grid, block, 0, stream, sys, items); break;
+        case 16: hipLaunchKernelGGL(k_rho_rows<16>, grid, block, 0, stream, sys, items); break;
+        case 24: hipLaunchKernelGGL(k_rho_rows<24>, grid, block, 0, stream, sys, items); break;
+        default: hipLaunchKernelGGL(k_rho_rows<32>, grid, block, 0, stream, sys, items); break;
+    }
+}
+
+void launch_force_rows(hipStream_t stream, int seg, const SysDesc *sys, const WorkItem *items, int n, int ppl)
+{
+    if (n <= 0) return;
+    const dim3 grid((unsigned)n), block(kBlock);
+#define CC_ROWS_CASE(S)                                                                                    \
+    case S:                                                                                                \
+        if (ppl == 2) hipLaunchKernelGGL((k_force_rows<S, 2>), grid, block, 0, stream, sys, items);       \
+        else hipLaunchKernelGGL((k_force_rows<S, 1>), grid, block, 0, stream, sys, items);                \
+        break;
+    switch (seg) {
+        CC_ROWS_CASE(4)
+        CC_ROWS_CASE(8)
+        CC_ROWS_CASE(16)
+        CC_ROWS_CASE(24)
+        default: CC_ROWS_CASE(32)
+    }
+#undef CC_ROWS_CASE
+}
+
+}  // namespace ccgpu

@@ -25,6 +25,8 @@ enum SysFlags : int32_t {
     kSysS2 = 128,      // target-S(k) energy sum V (|rho|^2 / N - S0)^2 + Shift (no 1/vol), per-slot targets slot_s0
     kSysLog = 256,     // energy log(Phi) and forces F / Phi (LogCCPotential); energy[1] keeps Phi
     kSysMultiConfig = 512,  // mixture of equal sub-configurations sharing one target-S(k) energy of the mean S
+    kSysRhoFinal = 1024,    // the rho contraction covers every particle in one chunk and finishes each slot itself
+    kSysForceFinal = 2048,  // the force contraction covers every segment in one chunk and finishes each particle
 };
 
 struct SysDesc {
@@ -58,7 +60,7 @@ struct SysDesc {
     double *rho;     // 2 x M: RhoReal then RhoImag in constraint order
     double *coef;    // nslots x 2: (2 V/vol) * (Im rho, Re rho)
     float *coef32;   // nslots x 2: the same rounded to FP32 (force terms of the mixed-precision path)
-    double *epart;   // per reduce block: sum V |rho|^2
+    double *epart;   // nslots: energy term of each slot (V |rho|^2, or V (S - S0)^2)
     double *energy;  // [0] total, [1] CC part, [2] soft-core part
 
     // forces

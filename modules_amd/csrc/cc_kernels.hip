@@ -18,6 +18,7 @@
 #include <type_traits>
 
 #include "cc_device.h"
+#include "cc_epilogue.h"
 #include "cc_math.h"
 
 namespace ccgpu {
@@ -417,72 +418,47 @@ __global__ void __launch_bounds__(kBlock) k_rho_general(const SysDesc *__restric
 }
 
 // ---------------------------------------------------------------------------------------------
-// rho reduce: rho_q = sum over chunks (chunk order), scatter to constraint order, energy partial
-// sum V |rho|^2 per block, force coefficients (2V/vol) (Im rho, Re rho) -- for a target-S(k) system the
-// partial sum V (S - S0)^2 and coefficients (4 V (S - S0) / N) (Im rho, Re rho).  items: (sys, slot block)
+// rho reduce: rho_q = sum over chunks (chunk order), then the slot epilogue (slot_finish: rho in constraint order,
+// the slot's energy term, the force coefficients).  items: (sys, slot block)
 // ---------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_rho_reduce(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
 {
-    __shared__ double lds[kBlock];
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
     if (!sys_active(sys, S)) return;
     const int q = it.t0 * kBlock + threadIdx.x;
-    double e = 0.0;
-    if (q < S.nslots) {
-        // chunk order; the loads of eight chunks are issued before their adds, so a system with many chunks
-        // (few slot blocks, long loops) keeps loads in flight
-        double re = 0.0, im = 0.0;
-        constexpr int kLoads = 8;
-        int c = 0;
-        for (; c + kLoads <= S.pchunks; c += kLoads) {
-            double vr[kLoads], vi[kLoads];
+    if (q >= S.nslots) return;
+    // chunk order; the loads of eight chunks are issued before their adds, so a system with many chunks (few slot
+    // blocks, long loops) keeps loads in flight
+    double re = 0.0, im = 0.0;
+    constexpr int kLoads = 8;
+    int c = 0;
+    for (; c + kLoads <= S.pchunks; c += kLoads) {
+        double vr[kLoads], vi[kLoads];
 #pragma unroll
-            for (int u = 0; u < kLoads; u++) {
-                const double *p = S.rpart + ((int64_t)(c + u) * S.nslots + q) * 2;
-                vr[u] = p[0];
-                vi[u] = p[1];
-            }
+        for (int u = 0; u < kLoads; u++) {
+            const double *p = S.rpart + ((int64_t)(c + u) * S.nslots + q) * 2;
+            vr[u] = p[0];
+            vi[u] = p[1];
+        }
 #pragma unroll
-            for (int u = 0; u < kLoads; u++) {
-                re += vr[u];
-                im += vi[u];
-            }
+        for (int u = 0; u < kLoads; u++) {
+            re += vr[u];
+            im += vi[u];
         }
-        for (; c < S.pchunks; c++) {
-            const double *p = S.rpart + ((int64_t)c * S.nslots + q) * 2;
-            re += p[0];
-            im += p[1];
-        }
-        const double v = S.slot_v[q];
-        const int kidx = (S.flags & kSysLattice) ? S.slot_k[q] : q;
-        double w = (2.0 * v) * S.inv_vol;
-        if (kidx >= 0) {
-            S.rho[kidx] = re;
-            S.rho[S.M + kidx] = im;
-            if (S.flags & kSysS2) {
-                // ShiftedCCPotential_S2::Energy / Force (potential/CollectiveCoordinatePotential.h:1686-1741):
-                // V (S - S0)^2 with S = |rho|^2 / N; force coefficient (4 / N) V (S - S0)
-                const double ds = (re * re + im * im) * S.inv_n - S.slot_s0[q];
-                e = v * ds * ds;
-                w = (4.0 * S.inv_n) * v * ds;
-            } else {
-                e = v * (re * re + im * im);
-            }
-        }
-        const double ca = kidx >= 0 ? w * im : 0.0, cb = kidx >= 0 ? w * re : 0.0;
-        S.coef[2 * q] = ca;
-        S.coef[2 * q + 1] = cb;
-        S.coef32[2 * q] = (float)ca;
-        S.coef32[2 * q + 1] = (float)cb;
     }
-    const double tot = block_sum(e, lds);
-    if (threadIdx.x == 0) S.epart[it.t0] = tot;
+    for (; c < S.pchunks; c++) {
+        const double *p = S.rpart + ((int64_t)c * S.nslots + q) * 2;
+        re += p[0];
+        im += p[1];
+    }
+    slot_finish(S, q, re, im);
 }
 
 // ---------------------------------------------------------------------------------------------
 // energy: one block per system; Phi = sum(epart)/vol + Shift (target-S(k) systems: sum(epart) + Shift), plus the
-// soft-core sum.  items: (sys)
+// soft-core sum.  epart holds one term per slot; thread t sums slots t, t + 256, ... and the block tree adds the
+// threads in a fixed order.  items: (sys)
 // ---------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_energy_final(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
 {
@@ -490,9 +466,8 @@ __global__ void __launch_bounds__(kBlock) k_energy_final(const SysDesc *__restri
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
     if (!sys_active(sys, S)) return;
-    const int nb = (S.nslots + kBlock - 1) / kBlock;
     double acc = 0.0;
-    for (int b = threadIdx.x; b < nb; b += kBlock) acc += S.epart[b];
+    for (int q = threadIdx.x; q < S.nslots; q += kBlock) acc += S.epart[q];
     const double cc_sum = block_sum(acc, lds);
     double sc = 0.0;
     if (S.flags & kSysSoftCore) {
@@ -587,7 +562,6 @@ __global__ void __launch_bounds__(kBlock) k_mc_final(const SysDesc *__restrict__
 // ---------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_multiconfig_coef(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
 {
-    __shared__ double lds[kBlock];
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
     if (!sys_active(sys, S)) return;
@@ -617,9 +591,8 @@ __global__ void __launch_bounds__(kBlock) k_multiconfig_coef(const SysDesc *__re
             Cd.coef32[2 * q] = (float)ca;
             Cd.coef32[2 * q + 1] = (float)cb;
         }
+        S.epart[q] = e;
     }
-    const double tot = block_sum(e, lds);
-    if (threadIdx.x == 0) S.epart[it.t0] = tot;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -634,9 +607,8 @@ __global__ void __launch_bounds__(kBlock) k_mixture_final(const SysDesc *__restr
     if (!sys_active(sys, S)) return;
     if (S.flags & kSysMultiConfig) {
         // ShiftedCCPotential_S2_MultiConfig::Energy (:1818-1844): sum V (S - S0)^2 + Shift over the mean S
-        const int nb = (S.nslots + kBlock - 1) / kBlock;
         double acc = 0.0;
-        for (int b = threadIdx.x; b < nb; b += kBlock) acc += S.epart[b];
+        for (int q = threadIdx.x; q < S.nslots; q += kBlock) acc += S.epart[q];
         const double cc = block_sum(acc, lds);
         if (threadIdx.x == 0) {
             S.energy[0] = cc + S.shift;
@@ -891,19 +863,16 @@ __global__ void __launch_bounds__(kBlock) k_force_general(const SysDesc *__restr
 }
 
 // ---------------------------------------------------------------------------------------------
-// force epilogue: sum chunk partials (chunk order), F = -sum_j G_j b_j (lattice) or -sum c k (general),
-// plus the soft-core force; gradient g[i*d+j] = (-1) F_i . a_j / rescale.  items: (sys, particle block)
+// force epilogue: sum chunk partials (chunk order), then force_finish (F = -sum_j G_j b_j (lattice) or -sum c k
+// (general), the weight, log-form, soft-core and species terms, the gradient).  items: (sys, particle block)
 // ---------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_force_final(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items,
-                                                        const double *__restrict__ sc_force_unused)
+__global__ void __launch_bounds__(kBlock) k_force_final(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
 {
-    (void)sc_force_unused;
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
     if (!sys_active(sys, S)) return;
     const int i = it.t0 * kBlock + threadIdx.x;
     if (i >= S.N) return;
-    const int d = S.d;
     double G[4] = {0, 0, 0, 0};
     const double4 *fp = reinterpret_cast<const double4 *>(S.fpart);
     for (int c = 0; c < S.kchunks; c++) {
@@ -913,45 +882,7 @@ __global__ void __launch_bounds__(kBlock) k_force_final(const SysDesc *__restric
         G[2] += v.z;
         G[3] += v.w;
     }
-    double F[4] = {0, 0, 0, 0};
-    if (S.flags & kSysLattice) {
-        for (int j = 0; j < d; j++)
-            for (int c = 0; c < d; c++) F[c] = __builtin_fma(-G[j], S.b[j * d + c], F[c]);
-    } else {
-        for (int c = 0; c < d; c++) F[c] = -G[c];
-    }
-    // ShiftedCCPotential_withWeight::Force scales particle i's force by its weight (:1560-1577);
-    // LogCCPotential::Force divides by the plain form's energy (:1629-1633)
-    if (S.weight)
-        for (int c = 0; c < d; c++) F[c] = F[c] * S.weight[i];
-    if (S.flags & kSysLog) {
-        const double inv = 1.0 / S.energy[1];
-        for (int c = 0; c < d; c++) F[c] = F[c] * inv;
-    }
-    if (S.flags & kSysSoftCore) {
-        // soft-core force of k_sc_pair, stored after the per-particle energies (N x 4)
-        const double *fs = S.sc_epart + S.N + (int64_t)i * 4;
-        for (int c = 0; c < d; c++) F[c] = F[c] + fs[c];
-    }
-    if (S.flags & kSysView) {
-        // species of a mixture: contact force + (u0 N_s / N) x species CC force
-        // (MultiStealthyPotential::WithSpeciesForce, MultiStealthyPotential.cpp:667-678)
-        for (int c = 0; c < d; c++) F[c] = F[c] * S.fscale;
-        if (S.ext_force) {
-            const double *fc = S.ext_force + (int64_t)i * 4;
-            for (int c = 0; c < d; c++) F[c] = fc[c] + F[c];
-        }
-    }
-    for (int c = 0; c < d; c++) S.F[(int64_t)i * d + c] = F[c];
-    if (S.grad) {
-        // a species view writes its range of the mixture's gradient, at the mixture's current rescale
-        const double rescale = S.parent >= 0 ? sys[S.parent].rescale : S.rescale;
-        for (int j = 0; j < d; j++) {
-            double dot = 0.0;
-            for (int c = 0; c < d; c++) dot += F[c] * S.a[j * d + c];
-            S.grad[(int64_t)i * d + j] = ((-1.0) * dot) / rescale;
-        }
-    }
+    force_finish(sys, S, i, G);
 }
 
 // explicit instantiations of the segment lengths the autotuner may choose
@@ -1082,7 +1013,7 @@ void launch_force_general(hipStream_t stream, const SysDesc *sys, const WorkItem
 
 void launch_force_final(hipStream_t stream, const SysDesc *sys, const WorkItem *items, int n)
 {
-    if (n > 0) hipLaunchKernelGGL(k_force_final, CC_GRID(n), sys, items, (const double *)nullptr);
+    if (n > 0) hipLaunchKernelGGL(k_force_final, CC_GRID(n), sys, items);
 }
 
 }  // namespace ccgpu

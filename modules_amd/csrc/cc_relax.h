@@ -185,6 +185,10 @@ public:
     int Lanes() const { return (int)lanes_.size(); }
     void SetTiming(bool on);
     void Timing(double &rho_ms, double &force_ms, int64_t &n_rho, int64_t &n_force, bool reset);
+    // bursts of optimizer steps launched so far: replayed from a captured hipGraph, and enqueued as plain stream
+    // launches (graph mode falls back to stream launches while kernel timing is on or a job uses the NUFFT)
+    int64_t GraphBursts() const { return graph_bursts_; }
+    int64_t StreamBursts() const { return stream_bursts_; }
 
 private:
     void Admit();
@@ -202,6 +206,7 @@ private:
     int check_every_ = 8;
     bool use_graph_ = false;
     bool timing_ = false;
+    int64_t graph_bursts_ = 0, stream_bursts_ = 0;
     int precision_ = 0;
     double polish_ = 1e-8;
     int rho_backend_ = 0;

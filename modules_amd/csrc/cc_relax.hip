@@ -986,6 +986,7 @@ void Solver::LaunchBurst(int burst)
     bool nufft = false;  // hipFFT calls stay on plain stream launches
     for (auto &l : lanes_) nufft = nufft || l->batch.has_nufft;
     if (!use_graph_ || timing_ || nufft) {
+        stream_bursts_++;
         for (int it = 0; it < burst; it++)
             for (auto &l : lanes_)
                 if (!l->jobs.empty()) enqueue(*l, 1);
@@ -1006,6 +1007,7 @@ void Solver::LaunchBurst(int burst)
             L.graph_burst = burst;
         }
     }
+    graph_bursts_++;
     for (auto &l : lanes_)
         if (!l->jobs.empty()) CC_HIP(hipGraphLaunch(l->gexec, l->batch.stream));
 }

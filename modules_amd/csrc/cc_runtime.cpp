@@ -147,11 +147,11 @@ Tiles DefaultTiles(int d, int64_t N, int64_t M)
     Tiles tuned;
     if (LookupTiles(d, N, M, tuned)) return tuned;
     Tiles t;
-    t.seg = 16;
+    t.seg = d == 2 ? 32 : 24;  // 2D runs are long (up to twice the k-disk radius in index units)
     t.rho_chunk = 0;    // resolved in Prepare from the pair count and the segment count
     t.force_chunk = 0;  // likewise
-    t.rho_kernel = 0;
-    t.force_ppl = 2;
+    t.rho_kernel = 2;   // the particle-row contractions
+    t.force_ppl = 1;
     return t;
 }
 
@@ -262,6 +262,54 @@ std::shared_ptr<const ConstraintLayout> GetLayout(int d, const double *a, int64_
     return L;
 }
 
+ChunkPlan PlanChunks(int64_t N, int64_t M, bool lattice, int64_t nunits, const Tiles &used, const Tiles &req,
+                     const Tiles &def)
+{
+    // Untuned shapes get enough work items for their work and no more, so a lone large system fills the chip and
+    // a small one in a big batch does not split its sums needlessly.  The rule reads only the system's own shape
+    // (so do its bits).
+    ChunkPlan cp;
+    const int64_t pairs = std::max<int64_t>(N, 1) * std::max<int64_t>(M, 1);
+    const bool rows = lattice && used.rho_kernel == 2;
+    const int rc = req.rho_chunk > 0 ? std::min(req.rho_chunk, rows ? 1 << 16 : 512) : def.rho_chunk;
+    if (rc > 0) {
+        cp.rho_chunk = rc;
+    } else if (rows) {
+        // particle-row rho kernel: one wavefront per 16 segments and chunk, about 2^16 pairs each
+        const int64_t want = std::max<int64_t>(1, std::min<int64_t>(8192, pairs >> 16));
+        const int64_t sblocks = std::max<int64_t>(1, (nunits + 15) / 16);
+        const int64_t pch = std::max<int64_t>(1, (want + sblocks - 1) / sblocks);
+        const int64_t c = (std::max<int64_t>(N, 1) + pch - 1) / pch;
+        cp.rho_chunk = (int)std::max<int64_t>(64, ((c + 63) / 64) * 64);
+    } else {
+        // one block per ~2^18 pairs, at least 48 rho blocks per system and at most 4096
+        const int64_t want = std::max<int64_t>(48, std::min<int64_t>(4096, pairs >> 18));
+        const int64_t per = lattice ? (used.rho_kernel == 1 ? kBlock : kRhoBlock) : kBlock;
+        const int64_t sblocks = std::max<int64_t>(1, (nunits + per - 1) / per);
+        const int64_t pch = std::max<int64_t>(1, (want + sblocks - 1) / sblocks);
+        int64_t c = (N + pch - 1) / pch;
+        c = ((c + 31) / 32) * 32;
+        cp.rho_chunk = (int)std::min<int64_t>(512, std::max<int64_t>(64, c));
+    }
+    cp.pchunks = (int)std::max<int64_t>(1, (N + cp.rho_chunk - 1) / cp.rho_chunk);
+    const int64_t per_block = rows ? 64 * used.force_ppl : (int64_t)kBlock * used.force_ppl;
+    const int64_t pblocks = std::max<int64_t>(1, (N + per_block - 1) / per_block);
+    const int fc = req.force_chunk > 0 ? req.force_chunk : def.force_chunk;
+    if (fc > 0) {
+        cp.force_chunk = fc;
+    } else {
+        // one block per ~2^18 pairs (at least 24 per system with the older kernel), each chunk at least 4 units
+        // (16 for the particle-row kernel, whose four wavefronts share a chunk)
+        const int64_t lo = rows ? 1 : 24;
+        const int64_t want_blocks = std::max<int64_t>(lo, std::min<int64_t>(4096, pairs >> 18));
+        int64_t want = std::max<int64_t>(1, (want_blocks + pblocks - 1) / pblocks);
+        want = std::min<int64_t>(want, std::max<int64_t>(1, nunits / (rows ? 16 : 4)));
+        cp.force_chunk = (int)std::max<int64_t>(1, (nunits + want - 1) / want);
+    }
+    cp.kchunks = (int)std::max<int64_t>(1, (nunits + cp.force_chunk - 1) / cp.force_chunk);
+    return cp;
+}
+
 System::System(int dim) : d(dim) {}
 
 void System::SetCell(const double *basis)
@@ -346,7 +394,7 @@ void System::Prepare(hipStream_t stream)
         layout_dirty = false;
     }
     const int64_t nslots = layout->nslots();
-    const int64_t nunits = lattice ? layout->plan.nseg : M;  // force-pass loop units
+    const int64_t nunits = lattice ? layout->plan.nseg : M;  // force-pass loop units (segments or constraints)
     if (!weights.empty()) {
         if ((int64_t)weights.size() != N) throw Error(kErrState, "one weight per particle");
         if (weight.capacity() < (size_t)std::max<int64_t>(N, 1)) {
@@ -367,36 +415,12 @@ void System::Prepare(hipStream_t stream)
     }
     else if (nu) nu->active = false, nu->layout.reset(), nu->why = "direct backend selected";
 
-    // Chunking of the two contractions.  Untuned shapes get enough blocks for their work: one block per ~2^18
-    // (particle, constraint) pairs, at least 48 rho blocks and 24 force blocks per system and at most 4096, so a
-    // lone large system fills the chip and a small one in a big batch does not split its sums needlessly.  The
-    // rule reads only the system's own shape (so do its bits).
-    const int64_t pairs = std::max<int64_t>(N, 1) * std::max<int64_t>(M, 1);
-    const int rc = tiles.rho_chunk > 0 ? std::min(tiles.rho_chunk, 512) : def.rho_chunk;
-    if (rc > 0) {
-        used.rho_chunk = rc;
-    } else {
-        const int64_t want = std::max<int64_t>(48, std::min<int64_t>(4096, pairs >> 18));
-        const int64_t per = lattice ? (used.rho_kernel == 1 ? kBlock : kRhoBlock) : kBlock;
-        const int64_t sblocks = std::max<int64_t>(1, ((lattice ? layout->plan.nseg : M) + per - 1) / per);
-        const int64_t pch = std::max<int64_t>(1, (want + sblocks - 1) / sblocks);
-        int64_t c = (N + pch - 1) / pch;
-        c = ((c + 31) / 32) * 32;
-        used.rho_chunk = (int)std::min<int64_t>(512, std::max<int64_t>(64, c));
-    }
-    pchunks = (int)std::max<int64_t>(1, (N + used.rho_chunk - 1) / used.rho_chunk);
+    const ChunkPlan cp = PlanChunks(N, M, lattice, nunits, used, tiles, def);
+    used.rho_chunk = cp.rho_chunk;
+    used.force_chunk = cp.force_chunk;
+    pchunks = cp.pchunks;
+    kchunks = cp.kchunks;
     if (nufft_on()) pchunks = 1;  // the NUFFT writes rho once per slot
-    const int64_t pblocks = std::max<int64_t>(1, (N + kBlock * used.force_ppl - 1) / (kBlock * used.force_ppl));
-    const int fc = tiles.force_chunk > 0 ? tiles.force_chunk : def.force_chunk;
-    if (fc > 0) {
-        used.force_chunk = fc;
-    } else {
-        const int64_t want_blocks = std::max<int64_t>(24, std::min<int64_t>(4096, pairs >> 18));
-        int64_t want = std::max<int64_t>(1, (want_blocks + pblocks - 1) / pblocks);  // chunks
-        want = std::min<int64_t>(want, std::max<int64_t>(1, nunits / 4));           // at least 4 units each
-        used.force_chunk = (int)std::max<int64_t>(1, (nunits + want - 1) / want);
-    }
-    kchunks = (int)std::max<int64_t>(1, (nunits + used.force_chunk - 1) / used.force_chunk);
     if (nufft_on()) kchunks = 1;  // ... and the index-space force weights once per particle
 
     // soft-core cell grid: side max(sigma, (V/N)^(1/d)) (apps/CCOptimization/main.cpp:235-240)
@@ -427,7 +451,7 @@ void System::Prepare(hipStream_t stream)
     rho.Reserve(std::max<int64_t>(1, 2 * M));
     coef.Reserve(std::max<int64_t>(1, 2 * nslots));
     coef32.Reserve(std::max<int64_t>(1, 2 * nslots));
-    epart.Reserve(std::max<int64_t>(1, (nslots + kBlock - 1) / kBlock));
+    epart.Reserve(std::max<int64_t>(1, nslots));  // one energy term per slot
     energy.Reserve(4);
     fpart.Reserve((int64_t)kchunks * Nn * 4);
     if (mixture) {
@@ -446,10 +470,12 @@ void System::Prepare(hipStream_t stream)
     D.d = d;
     D.N = (int32_t)N;
     D.M = (int32_t)M;
+    const bool rows = lattice && !nufft_on() && !mixture && used.rho_kernel == 2;
     D.flags = kSysActive | (lattice ? kSysLattice : 0) | (softcore() ? kSysSoftCore : 0) |
               (lattice && precision == 1 && !nufft_on() ? kSysMixed : 0) | (view() ? kSysView : 0) |
               (mixture ? kSysMixture : 0) | (nufft_on() ? kSysNufft : 0) | (s2 ? kSysS2 : 0) |
-              (log_form ? kSysLog : 0) | (mixture && nsub > 0 ? kSysMultiConfig : 0);
+              (log_form ? kSysLog : 0) | (mixture && nsub > 0 ? kSysMultiConfig : 0) |
+              (rows && N > 0 && pchunks == 1 ? kSysRhoFinal : 0) | (rows && kchunks == 1 ? kSysForceFinal : 0);
     D.nseg = (int32_t)nseg();
     D.nslots = (int32_t)nslots;
     D.pchunks = N > 0 ? pchunks : 0;  // no particles: rho sums nothing
@@ -639,7 +665,7 @@ void Batch::Build()
                 if (!s->lattice || s->nufft_on() || s->mixture || s->used.seg != sl || s->used.rho_kernel != rk ||
                     s->used.force_ppl != ppl || s->N == 0)
                     continue;
-                const int bs = rk == 1 ? kBlock : kRhoBlock;  // segments per rho block
+                const int bs = rk == 1 ? kBlock : rk == 2 ? 16 : kRhoBlock;  // segments per rho block
                 const int sb = (int)((s->nseg() + bs - 1) / bs);
                 for (int c = 0; c < s->pchunks; c++)
                     for (int b = 0; b < sb; b++) h_items.push_back({i, b, c, 0});
@@ -651,7 +677,8 @@ void Batch::Build()
                 if (!s->lattice || s->nufft_on() || s->mixture || s->used.seg != sl || s->used.rho_kernel != rk ||
                     s->used.force_ppl != ppl)
                     continue;
-                const int fb = (int)((s->N + kBlock * ppl - 1) / (kBlock * ppl));
+                const int per = (rk == 2 ? 64 : kBlock) * ppl;  // particles per force block
+                const int fb = (int)((s->N + per - 1) / per);
                 for (int c = 0; c < s->kchunks; c++)
                     for (int b = 0; b < fb; b++) h_items.push_back({i, b, c, 0});
             }
@@ -669,7 +696,8 @@ void Batch::Build()
     });
     add(r_red, [&] {
         for (int i = 0; i < ns; i++) {
-            if (all[i]->mixture) continue;  // a mixture's rho is its children's
+            // a mixture's rho is its children's; a system whose rho contraction finishes its slots needs no reduce
+            if (all[i]->mixture || (all[i]->desc.flags & kSysRhoFinal)) continue;
             const int nb = (int)((all[i]->desc.nslots + kBlock - 1) / kBlock);
             for (int b = 0; b < nb; b++) h_items.push_back({i, b, 0, 0});
         }
@@ -699,7 +727,8 @@ void Batch::Build()
     });
     add(r_ffin, [&] {
         for (int i = 0; i < ns; i++)
-            if (!all[i]->mixture)  // a mixture's forces are written by its species
+            // a mixture's forces are written by its species; a finishing force contraction writes its own
+            if (!all[i]->mixture && !(all[i]->desc.flags & kSysForceFinal))
                 for (int b = 0; b < pblocks(all[i]); b++) h_items.push_back({i, b, 0, 0});
     });
     add(r_sc_part, [&] {
@@ -789,6 +818,10 @@ void Batch::Enqueue(bool rho, bool forces)
             CC_HIP(hipEventRecord(ev_pool_[mk.e0], stream));
         }
         for (const SegGroup &g : seg_groups) {
+            if (g.rho_kernel == 2) {
+                launch_rho_rows(stream, g.seg, D, I + g.rho.off, g.rho.n);
+                continue;
+            }
             launch_rho_lattice(stream, g.seg, D, I + g.rho.off, g.rho.n, g.rho_kernel);
             if (g.any_mixed) launch_rho_lattice(stream, g.seg, D, I + g.rho.off, g.rho.n, 2);
         }
@@ -816,7 +849,10 @@ void Batch::Enqueue(bool rho, bool forces)
             mk.e0 = NextEvent();
             CC_HIP(hipEventRecord(ev_pool_[mk.e0], stream));
         }
-        for (const SegGroup &g : seg_groups) launch_force_lattice(stream, g.seg, D, I + g.force.off, g.force.n, g.force_ppl);
+        for (const SegGroup &g : seg_groups) {
+            if (g.rho_kernel == 2) launch_force_rows(stream, g.seg, D, I + g.force.off, g.force.n, g.force_ppl);
+            else launch_force_lattice(stream, g.seg, D, I + g.force.off, g.force.n, g.force_ppl);
+        }
         launch_force_general(stream, D, I + r_f_gen.off, r_f_gen.n);
         if (has_nufft) {
             launch_nufft_coef(stream, D, I + r_nu_grid.off, r_nu_grid.n, I + r_nu_slots.off, r_nu_slots.n);

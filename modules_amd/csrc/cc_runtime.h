@@ -80,10 +80,21 @@ struct Tiles {
     int seg = 16;
     int rho_chunk = 0;
     int force_chunk = 0;
-    int rho_kernel = -1;  // 0: phase recurrence, 1: LDS power tables; -1 automatic
+    int rho_kernel = -1;  // 0: segment lanes with the phase recurrence, 1: LDS power tables, 2: particle rows (and
+                          // the particle-row force kernel); -1 automatic
     int force_ppl = 0;    // particles per lane of the force kernel (1 or 2); 0 automatic
 };
 Tiles DefaultTiles(int d, int64_t N, int64_t M);
+
+// Chunking of the two contractions for one system: particles per rho chunk and their count, force units
+// (segments or constraints) per chunk and their count, from the resolved tiles `used` (seg, rho_kernel, force_ppl),
+// the requested ones `req` (0 = automatic) and the defaults.  Shared by System::Prepare and the solver's HBM
+// estimate.
+struct ChunkPlan {
+    int rho_chunk = 0, pchunks = 1, force_chunk = 0, kchunks = 1;
+};
+ChunkPlan PlanChunks(int64_t N, int64_t M, bool lattice, int64_t nunits, const Tiles &used, const Tiles &req,
+                     const Tiles &def);
 
 // Process-wide table of tuned tiles keyed by (d, N bucket, M bucket) (half-octave buckets); DefaultTiles
 // consults it.  Filled by Autotune; a system's tiles depend only on its own shape and the table.
@@ -313,6 +324,9 @@ void launch_md_resets(hipStream_t, double *v, const double *table, int n, int d)
 void launch_mc_trial(hipStream_t, const SysDesc *, const WorkItem *, int, const double *mv, double *rho_trial,
                      double *epart, int sidx, double *energy_out);
 void launch_force_lattice(hipStream_t, int seg, const SysDesc *, const WorkItem *, int, int ppl);
+// the particle-row contractions (cc_contract.hip)
+void launch_rho_rows(hipStream_t, int seg, const SysDesc *, const WorkItem *, int);
+void launch_force_rows(hipStream_t, int seg, const SysDesc *, const WorkItem *, int, int ppl);
 void launch_force_general(hipStream_t, const SysDesc *, const WorkItem *, int);
 void launch_force_final(hipStream_t, const SysDesc *, const WorkItem *, int);
 // soft-core pair kernel choice for every later launch: 0 automatic, 1 one thread per particle, 2 lane groups per

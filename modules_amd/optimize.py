@@ -239,6 +239,11 @@ class BatchSolver:
         self.set_rho_backend(rho_backend, nufft_eps)
         self._shapes = {}
 
+    def set_graph(self, on=True):
+        """Replay the optimizer steps between two host checks from a captured hipGraph (True) or launch them on
+        the stream (False)."""
+        check(lib().ccgpu_solver_set_graph(self._h, 1 if on else 0))
+
     def set_precision(self, precision="fp64", polish=1e-8):
         """Precision of jobs submitted from now on: "fp64", or "mixed" (FP32 phases and force terms, FP64
         sums) until E < polish or the end of the first pass, FP64 afterwards."""
@@ -353,6 +358,13 @@ class BatchSolver:
         nb = C.c_int64(0)
         check(lib().ccgpu_solver_timing(self._h, C.byref(a), C.byref(b), C.byref(na), C.byref(nb), 1 if reset else 0))
         return a.value, b.value, na.value, nb.value
+
+    def launch_stats(self):
+        """Bursts of optimizer steps launched so far: {"graph": replayed hipGraphs, "stream": stream-launched}."""
+        g = C.c_int64(0)
+        st = C.c_int64(0)
+        check(lib().ccgpu_solver_launch_stats(self._h, C.byref(g), C.byref(st)))
+        return {"graph": g.value, "stream": st.value}
 
     def stats(self):
         r = C.c_int(0)

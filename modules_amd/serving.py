@@ -49,6 +49,21 @@ def make_stream(n_requests, rate, seed=20261203, median_N=1000, sigma_ln=0.6, N_
     return reqs
 
 
+def make_batch(count=256, master_seed=20261203, N_range=(100, 4000), dims=(2, 3), chi_range=(0.2, 0.45)):
+    """A batch of independent stealthy requests, all present at t = 0: N log-uniform on N_range (rounded), d a fair
+    coin over `dims`, chi uniform on chi_range and one configuration seed each, drawn in that order per request from
+    one MT19937 stream (numpy's RandomState) seeded with `master_seed`.  No soft core."""
+    rng = np.random.RandomState(master_seed)
+    reqs = []
+    lo, hi = math.log(N_range[0]), math.log(N_range[1])
+    for i in range(count):
+        N = int(round(math.exp(rng.uniform(lo, hi))))
+        d = dims[0] if rng.uniform() < 0.5 else dims[1]
+        chi = float(rng.uniform(*chi_range))
+        reqs.append(Request(i, 0.0, N, d, chi, int(rng.randint(0, 2 ** 31 - 1))))
+    return reqs
+
+
 def serve(reqs, Emin=1e-18, MaxStep=10000, max_slots=64, hbm_budget=200 << 30, step_evals=16, timeout=600):
     """Run the stream in real time.  Returns the requests with results, and the wall time."""
     solver = BatchSolver(max_slots=max_slots, hbm_budget=hbm_budget, check_interval=step_evals)

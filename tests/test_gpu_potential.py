@@ -64,7 +64,7 @@ def test_stealthy_configs_match_oracle(N, d, chi, seed):
 
 
 @pytest.mark.parametrize("seg", [4, 8, 16, 24, 32])
-@pytest.mark.parametrize("rho_kernel", [0, 1])
+@pytest.mark.parametrize("rho_kernel", [0, 1, 2])
 def test_segment_lengths_agree(seg, rho_kernel):
     cfg = RandomConfiguration(11, 700, 2)
     k = GetKs(cfg, getK(0.4, 1.0, 2))

@@ -1,0 +1,251 @@
+"""The benchmark configurations at their own sizes on the GPU, each result checked against the oracle.
+
+  seeded 2D N=200 chi=0.3, 64 seeds          one E+F against the oracle, every seed to E < 1e-18
+  3D N=2000 chi=0.45, 64 seeds               E and forces against the oracle every 100th objective evaluation,
+                                             every seed to E < 1e-18 in the batched solver
+  256-job batch (N 100..4000, d 2|3,         every job's end point accepted by the oracle, one start E+F per
+    chi 0.2..0.45)                           shape class against the oracle, alone-vs-mix bits
+  2D N=20000 chi=0.4, mixed precision        error of the mixed path against the oracle at the start
+                                             configuration, then a solve to E < 1e-18 in FP64
+  Poisson request stream with soft core      every result accepted by the oracle's soft-core + CC energy,
+                                             a request of N = 10^4 among them
+
+Ground-state acceptance follows the reference's one-component benchmark
+(docs/finufft/reports/one-component-benchmark.md:13): energy, largest constrained S(k) and largest force
+component of the end point, recomputed by the oracle, plus no evaluation-limit stop.  The energy bar here is
+10 x Emin with Emin = 1e-18 (the reference's own acceptance is 1e-11); S(k) <= 1e-11; |F| <= 1e-7."""
+import math
+
+import numpy as np
+import pytest
+
+from modules_amd import GetKs, RandomConfiguration, ShiftedCCPotential, getK
+from modules_amd.optimize import BatchSolver, LuksanPLIS
+from modules_amd.serving import make_batch, make_stream, serve
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+EMIN = 1e-18
+MAXEVAL_STATUS = 5  # NLopt result code of an evaluation-limit stop
+
+
+def stealthy(N, d, chi, seed):
+    cfg = RandomConfiguration(seed, N, d)
+    k = GetKs(cfg, getK(chi, 1.0, d))
+    return cfg, k, np.ones(len(k))
+
+
+def scales(cfg, k):
+    """The reference's scale-aware tolerances (potential/tests/finufft_type1/test_finufft_type1.cpp:197-212)."""
+    N, vol = cfg.NumParticle(), cfg.PeriodicVolume()
+    return len(k) * N * N / vol, 4 * len(k) * N * np.max(np.linalg.norm(k, axis=1)) / vol
+
+
+def accept_ground_state(rel, basis, k, v, sigma=0.0, e0=0.0):
+    """The oracle's energy, max S(k) and max |F| at an end point; asserts the acceptance bars."""
+    E, F = po.total_energy_forces(rel, basis, k, v, sigma=sigma, e0=e0)
+    re, im = po.rho(rel, basis, k)
+    smax = float(np.max((re * re + im * im) / len(rel))) if len(k) else 0.0
+    fmax = float(np.max(np.abs(F)))
+    assert E < 10 * EMIN, E
+    assert smax <= 1e-11, smax
+    assert fmax <= 1e-7, fmax
+    return E
+
+
+def check_ef_against_oracle(cfg, k, v, precision="fp64", tol=1e-12):
+    pot = ShiftedCCPotential(cfg.GetDimension())
+    for kk in k:
+        pot.AddConstraint(kk, [1.0])
+    if precision != "fp64":
+        pot.SetPrecision(precision)
+    pot.SetConfiguration(cfg)
+    E, F = pot.EnergyAndAllForce()
+    Eo, Fo = po.total_energy_forces(cfg.relative, cfg.basis, k, v)
+    es, fs = scales(cfg, k)
+    de, df = abs(E - Eo) / es, float(np.max(np.abs(F - Fo))) / fs
+    assert de <= tol and df <= tol, (de, df)
+    return E, F, Eo, Fo
+
+
+def test_small_2d_64_seeds():
+    # 2D N=200 chi=0.3, seeds 0..63: one E+F each against the oracle, then all 64 relaxed side by side
+    specs = [stealthy(200, 2, 0.3, s) for s in range(64)]
+    for cfg, k, v in specs[:8]:
+        check_ef_against_oracle(cfg, k, v)
+    solver = BatchSolver(max_slots=64)
+    ids = [solver.submit(cfg, k, v, Emin=EMIN) for cfg, k, v in specs]
+    solver.run(chunk=256, timeout=300)
+    for j, (cfg, k, v) in zip(ids, specs):
+        r = solver.result(j)
+        assert r["state"] == BatchSolver.DONE and r["energy"] < EMIN
+        assert r["status"] != MAXEVAL_STATUS
+        accept_ground_state(r["relative"], cfg.basis, k, v)
+
+
+def test_3d_N2000_objective_against_oracle_every_100th_evaluation():
+    # 3D N=2000 chi=0.45 (M=2724): E and the objective gradient of the GPU objective against the oracle at every
+    # 100th evaluation of RelaxStructure_NLOPT_Emin's three passes (x = s * rescale with rescale 10000, 1, 0.01;
+    # potential/StructureOptimization.cpp:1046-1084) driven by the host PLIS, to E < 1e-18
+    N, d = 2000, 3
+    cfg, k, v = stealthy(N, d, 0.45, 0)
+    assert len(k) == 2724
+    pot = ShiftedCCPotential(d)
+    for kk in k:
+        pot.AddConstraint(kk, [1.0])
+    a = cfg.basis
+    es, fs = scales(cfg, k)
+    count = [0]
+    worst = [0.0, 0.0]
+    from modules_amd import Configuration
+
+    s = cfg.relative.copy()
+    E = None
+    for rescale in (10000.0, 1.0, 0.01):
+        def fun(x):
+            y = x.reshape(N, d) / rescale
+            c = Configuration(a, y - np.floor(y))
+            pot.SetConfiguration(c)
+            E, F = pot.EnergyAndAllForce()
+            if count[0] % 100 == 0:
+                Eo, Fo = po.total_energy_forces(c.relative, a, k, v)
+                worst[0] = max(worst[0], abs(E - Eo) / es)
+                worst[1] = max(worst[1], float(np.max(np.abs(F - Fo))) / fs)
+            count[0] += 1
+            return E, -(F @ a.T).ravel() / rescale
+
+        x, minf, status, evals = LuksanPLIS(fun, (s * rescale).ravel(), stopval=EMIN, maxeval=10000)
+        y = x.reshape(N, d) / rescale
+        s = y - np.floor(y)
+        E = minf
+        if E < EMIN:
+            break
+    assert E < EMIN and count[0] >= 300
+    assert worst[0] <= 1e-12 and worst[1] <= 1e-12, worst
+    accept_ground_state(s, a, k, v)
+
+
+def test_3d_N2000_64_seeds_reach_ground_states():
+    specs = [stealthy(2000, 3, 0.45, s) for s in range(64)]
+    solver = BatchSolver(max_slots=64, check_interval=16)
+    ids = [solver.submit(cfg, k, v, Emin=EMIN) for cfg, k, v in specs]
+    solver.run(chunk=1024, timeout=300)
+    for j, (cfg, k, v) in zip(ids, specs):
+        r = solver.result(j)
+        assert r["state"] == BatchSolver.DONE and r["energy"] < EMIN, (j, r["energy"])
+        assert r["status"] != MAXEVAL_STATUS
+        accept_ground_state(r["relative"], cfg.basis, k, v)
+
+
+def test_batch_of_256_mixed_shapes():
+    # the 256-job batch: N log-uniform on [100, 4000], d in {2, 3}, chi in [0.2, 0.45] from master seed 20261203
+    reqs = make_batch()
+    assert max(r.N for r in reqs if r.d == 2) > 3800 and max(r.N for r in reqs if r.d == 3) > 3800
+    specs = [stealthy(r.N, r.d, r.chi, r.seed) for r in reqs]
+    # one start E+F per shape class (d, N octave) against the oracle
+    seen = set()
+    for r, (cfg, k, v) in zip(reqs, specs):
+        key = (r.d, int(math.log2(r.N)))
+        if key in seen:
+            continue
+        seen.add(key)
+        check_ef_against_oracle(cfg, k, v)
+    assert len(seen) >= 10
+    solver = BatchSolver(max_slots=256, check_interval=16)
+    ids = [solver.submit(cfg, k, v, Emin=EMIN) for cfg, k, v in specs]
+    solver.run(chunk=1024, timeout=600)
+    results = [solver.result(j) for j in ids]
+    for r, res, (cfg, k, v) in zip(reqs, results, specs):
+        assert res["state"] == BatchSolver.DONE and res["energy"] < EMIN, (r.rid, r.N, r.d, res["energy"])
+        assert res["status"] != MAXEVAL_STATUS
+        accept_ground_state(res["relative"], cfg.basis, k, v)
+    # the same jobs alone: the same bits (largest 2D, largest 3D, smallest)
+    picks = [max((i for i in range(256) if reqs[i].d == 2), key=lambda i: reqs[i].N),
+             max((i for i in range(256) if reqs[i].d == 3), key=lambda i: reqs[i].N),
+             min(range(256), key=lambda i: reqs[i].N)]
+    for i in picks:
+        alone = BatchSolver(max_slots=1)
+        j = alone.submit(*specs[i], Emin=EMIN)
+        alone.run(chunk=1024, timeout=300)
+        a = alone.result(j)
+        assert a["energy"] == results[i]["energy"] and a["evaluations"] == results[i]["evaluations"]
+        assert np.array_equal(a["relative"], results[i]["relative"])
+
+
+def test_2d_N20000_mixed_precision_error_and_solve():
+    # 2D N=20000 chi=0.4 (M ~ 16000): the mixed path's E and forces against the oracle at the start
+    # configuration, |dE|/E <= 1e-6 and max|dF| / max|F| <= 1e-5 (FP32 phases and force terms), the FP64 path
+    # at the 1e-12 scale bars; then the mixed solve reaches E < 1e-18 in FP64 (the switch to FP64 happens)
+    cfg, k, v = stealthy(20000, 2, 0.4, 4)
+    assert 15900 < len(k) < 16100
+    E64, F64, Eo, Fo = check_ef_against_oracle(cfg, k, v)
+    pot = ShiftedCCPotential(2)
+    for kk in k:
+        pot.AddConstraint(kk, [1.0])
+    pot.SetPrecision("mixed")
+    pot.SetConfiguration(cfg)
+    E32, F32 = pot.EnergyAndAllForce()
+    dE = abs(E32 - Eo) / abs(Eo)
+    dF = float(np.max(np.abs(F32 - Fo)) / np.max(np.abs(Fo)))
+    print(f"mixed vs oracle at the start: |dE|/E = {dE:.3g}, max|dF|/max|F| = {dF:.3g}")
+    assert dE <= 1e-6 and dF <= 1e-5
+    small = stealthy(300, 2, 0.3, 5)
+    solver = BatchSolver(max_slots=2, precision="mixed", polish=1e-8)
+    jb = solver.submit(cfg, k, v, Emin=EMIN)
+    js = solver.submit(*small, Emin=EMIN)
+    solver.run(chunk=1024, timeout=300)
+    r = solver.result(jb)
+    print(f"mixed solve: {r['device_evaluations']} evaluations, FP64 from evaluation {r['switch_eval']}")
+    assert r["state"] == BatchSolver.DONE and r["energy"] < EMIN
+    assert 0 < r["switch_eval"] <= r["device_evaluations"]
+    assert r["status"] != MAXEVAL_STATUS
+    accept_ground_state(r["relative"], cfg.basis, k, v)
+    # the small job beside the large one: the same bits as alone
+    alone = BatchSolver(max_slots=1, precision="mixed", polish=1e-8)
+    ja = alone.submit(*small, Emin=EMIN)
+    alone.run(timeout=300)
+    a, b = alone.result(ja), solver.result(js)
+    assert a["energy"] == b["energy"] and np.array_equal(a["relative"], b["relative"])
+
+
+def test_2d_N20000_fp64_large_job_beside_small():
+    # a job of 40000 optimizer variables (the large-job step path) beside a small job, in FP64
+    cfg, k, v = stealthy(20000, 2, 0.4, 4)
+    small = stealthy(250, 3, 0.35, 6)
+    solver = BatchSolver(max_slots=2)
+    jb = solver.submit(cfg, k, v, Emin=EMIN)
+    js = solver.submit(*small, Emin=EMIN)
+    solver.run(chunk=1024, timeout=300)
+    r = solver.result(jb)
+    assert r["state"] == BatchSolver.DONE and r["energy"] < EMIN
+    accept_ground_state(r["relative"], cfg.basis, k, v)
+    alone = BatchSolver(max_slots=1)
+    ja = alone.submit(*small, Emin=EMIN)
+    alone.run(timeout=300)
+    a, b = alone.result(ja), solver.result(js)
+    assert a["energy"] == b["energy"] and np.array_equal(a["relative"], b["relative"])
+
+
+def test_serving_stream_to_spec_against_oracle():
+    # Poisson stream: N lognormal (median 1000, sigma_ln 0.6, clipped to [100, 50000]), d in {2,3}, chi in
+    # [0.1, 0.49], soft core e0 = 1, sigma = 0.4 density^(-1/d); request 5 is set to N = 10000 so that one large
+    # request shares the GPU with the others
+    reqs = make_stream(16, rate=40.0, seed=11)
+    reqs[5].N = 10000
+    reqs, wall = serve(reqs, Emin=EMIN, max_slots=16, timeout=400)
+    assert any(a.submitted < b.done and b.submitted < a.done for a in reqs for b in reqs if a is not b)
+    for r in reqs:
+        cfg = RandomConfiguration(r.seed, r.N, r.d)
+        k = GetKs(cfg, getK(r.chi, 1.0, r.d))
+        assert r.result["state"] == BatchSolver.DONE and r.result["energy"] < EMIN, (r.rid, r.N, r.d, r.chi)
+        accept_ground_state(r.result["relative"], cfg.basis, k, np.ones(len(k)), sigma=r.sigma, e0=r.epsilon)
+    # the large request alone gives the same bits
+    r = reqs[5]
+    cfg = RandomConfiguration(r.seed, r.N, r.d)
+    k = GetKs(cfg, getK(r.chi, 1.0, r.d))
+    alone = BatchSolver(max_slots=1)
+    j = alone.submit(cfg, k, np.ones(len(k)), epsilon=r.epsilon, sigma=r.sigma, Emin=EMIN)
+    alone.run(timeout=300)
+    a = alone.result(j)
+    assert a["energy"] == r.result["energy"] and np.array_equal(a["relative"], r.result["relative"])

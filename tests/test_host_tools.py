@@ -22,8 +22,23 @@ def test_bench_help_lists_the_workloads():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--help"], capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 0
-    for w in ("batched", "single", "mixed256", "large2d", "mixed2d", "serve"):
+    for w in ("batch256", "seeds64", "single", "large2d", "mixed2d", "serve"):
         assert w in r.stdout
+
+
+def test_batch_draw_is_seeded_and_within_its_ranges():
+    from modules_amd.serving import make_batch
+    a = make_batch()
+    b = make_batch()
+    assert [(r.N, r.d, r.chi, r.seed) for r in a] == [(r.N, r.d, r.chi, r.seed) for r in b]
+    assert len(a) == 256
+    assert all(100 <= r.N <= 4000 and r.d in (2, 3) and 0.2 <= r.chi <= 0.45 for r in a)
+    assert 90 < sum(r.d == 2 for r in a) < 166
+    # log-uniform N: about half of the batch below the geometric mean sqrt(100 * 4000) ~ 632
+    assert 90 < sum(r.N < 632 for r in a) < 166
+    # the first draw follows numpy's legacy MT19937 stream of the master seed
+    rng = np.random.RandomState(20261203)
+    assert a[0].N == int(round(np.exp(rng.uniform(np.log(100), np.log(4000)))))
 
 
 def test_request_stream_is_seeded_and_within_its_ranges():
