@@ -65,6 +65,41 @@ def test_bitwise_equal_to_reference(fun, n, mf, seed):
         assert np.array_equal(a, b) and fa == fb
 
 
+def steep_cosh(x):
+    # the unit first step overshoots to an overflowing objective: bisection below the huge-value bound
+    f = float(np.sum(np.cosh(40.0 * x)))
+    return f, 40.0 * np.sinh(40.0 * x)
+
+
+def rastrigin(x):
+    # negative curvature along many directions: extrapolation, interpolation and failed searches (restarts)
+    f = float(10.0 * len(x) + np.sum(x * x - 10.0 * np.cos(2 * np.pi * x)))
+    return f, 2 * x + 20.0 * np.pi * np.sin(2 * np.pi * x)
+
+
+def quartic(x):
+    # a flat minimum: ends on the relative x or f change tests
+    w = np.arange(1, len(x) + 1, dtype=float)
+    return float(np.sum(w * x ** 4)), 4 * w * x ** 3
+
+
+@pytest.mark.parametrize("fun,n,mf,seed", [(steep_cosh, 3, 4, 5), (steep_cosh, 8, 40, 6), (rastrigin, 6, 5, 7),
+                                           (rastrigin, 12, 40, 8), (quartic, 5, 1, 9), (quartic, 9, 10, 10),
+                                           (rosenbrock, 6, 1, 11)])
+def test_bitwise_equal_on_hard_paths(fun, n, mf, seed):
+    rng = np.random.default_rng(seed)
+    x0 = rng.uniform(-1.0, 1.0, size=n)
+    with np.errstate(over="ignore", invalid="ignore"):
+        xr, fr, sr, er, tr = po.ref_plis(fun, x0, mf=mf, maxeval=5000)
+        xp, fp, sp, ep, tp = product_minimize(fun, x0, mf=mf, maxeval=5000)
+    assert (sr, er) == (sp, ep)
+    assert fr == fp or (np.isnan(fr) and np.isnan(fp))
+    assert np.array_equal(xr, xp, equal_nan=True)
+    assert len(tr) == len(tp)
+    for (a, fa), (b, fb) in zip(tr, tp):
+        assert np.array_equal(a, b, equal_nan=True) and (fa == fb or (np.isnan(fa) and np.isnan(fb)))
+
+
 def test_stopval_and_maxeval_codes():
     x0 = np.array([-1.2, 1.0, -1.2, 1.0])
     for kw in ({"stopval": 1e-3}, {"maxeval": 17}):
