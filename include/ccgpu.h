@@ -472,10 +472,19 @@ int ccgpu_solver_set_rho_backend(ccgpu_solver *s, int backend, double eps);
 /* Device evaluation at which a mixed-precision job switched to FP64 (0: FP64 throughout). */
 int ccgpu_solver_switch_eval(ccgpu_solver *s, int64_t job, int *evaluation);
 /* State of a job: 0 queued, 1 resident, 2 done, 3 failed.  When done: final energy, objective evaluations,
- * E+F evaluations on the device, NLopt code of the last pass, and times (submit, admit, done; seconds on the
- * solver clock) in t[3]; relative (N x dim, may be NULL) receives the relaxed configuration. */
+ * E+F evaluations on the device, NLopt code of the last pass (6 for a job stopped by its time limit), and times
+ * (submit, admit, done; seconds on the solver clock) in t[3]; relative (N x dim, may be NULL) receives the relaxed
+ * configuration.  A failed job fills state, energy, counts and times and returns its error: CCGPU_ERR_NAN when an
+ * evaluation gave a NaN energy or a non-finite gradient (the reference's NotANumberFound,
+ * potential/StructureOptimization.cpp:458-461; the other jobs are unaffected), CCGPU_ERR_MEMORY or
+ * CCGPU_ERR_ARG when admission failed; the message is in ccgpu_last_error. */
 int ccgpu_solver_result(ccgpu_solver *s, int64_t job, int *state, double *energy, int *evaluations,
                         int *device_evaluations, int *status, double *t, double *relative);
+/* Wall-clock limit in seconds from admission for jobs submitted after the call (0: none, the default).  A job past
+ * its limit is stopped at the next host check and returns as done with NLopt code 6 (NLOPT_MAXTIME_REACHED) and
+ * the configuration and energy of its last evaluation: the per-job counterpart of the reference's global
+ * TimeLimit, which writes the current configuration and exits (potential/StructureOptimization.cpp:352-378). */
+int ccgpu_solver_set_time_limit(ccgpu_solver *s, double seconds);
 /* Forget a finished job and free its slot (no counterpart in the reference). */
 int ccgpu_solver_release(ccgpu_solver *s, int64_t job);
 /* Resident and queued job counts and the HBM the solver holds (no counterpart in the reference). */

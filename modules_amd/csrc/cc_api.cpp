@@ -1508,7 +1508,6 @@ int ccgpu_solver_result(ccgpu_solver *s, int64_t job, int *state, double *energy
         Job *j = s->solver.Find(job);
         if (!j) throw Error(CCGPU_ERR_RANGE, "unknown job");
         if (state) *state = (int)j->state;
-        if (j->state == Job::kFailed) throw Error(CCGPU_ERR_DEVICE, "job failed: " + j->error);
         if (energy) *energy = j->E;
         if (evaluations) *evaluations = j->evals;
         if (device_evaluations) *device_evaluations = j->device_evals;
@@ -1520,6 +1519,16 @@ int ccgpu_solver_result(ccgpu_solver *s, int64_t job, int *state, double *energy
         }
         if (relative && j->state == Job::kDone && !j->s_final.empty())
             std::memcpy(relative, j->s_final.data(), j->s_final.size() * sizeof(double));
+        if (j->state == Job::kFailed)
+            throw Error(j->error_code ? j->error_code : CCGPU_ERR_DEVICE, "job failed: " + j->error);
+    });
+}
+
+int ccgpu_solver_set_time_limit(ccgpu_solver *s, double seconds)
+{
+    return Guard([&] {
+        Require(s != nullptr && std::isfinite(seconds) && seconds >= 0, CCGPU_ERR_ARG, "bad time limit");
+        s->solver.SetTimeLimit(seconds);
     });
 }
 

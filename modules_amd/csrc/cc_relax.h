@@ -36,6 +36,7 @@ struct RelaxState {
     int device_evals;  // every E+F evaluation made for this job
     int maxstep;
     int switch_eval;   // mixed precision: device evaluation at which the job went to FP64 (0: never mixed)
+    int failed;      // 1: the last evaluation gave a NaN energy or a non-finite gradient (the job ends)
     double fmin;     // stop value: Emin, or -MaxEnergy for RelaxStructure_NLOPT
     double E;        // energy of the last evaluation
     double polish;   // mixed precision: switch to FP64 once E falls below this
@@ -87,11 +88,14 @@ struct Job {
     // lifecycle
     enum State { kQueued, kResident, kDone, kFailed } state = kQueued;
     double t_submit = 0, t_admit = 0, t_done = 0;  // seconds on the solver clock
+    double time_limit = 0;  // wall seconds from admission after which the job is stopped (0: none)
+    bool timed_out = false;
     int64_t bytes = 0;
     // results
     std::vector<double> s_final;
     double E = 0;
     int evals = 0, device_evals = 0, status = 0, passes = 0, switch_eval = 0;
+    int error_code = 0;  // failed jobs: the C-ABI status (CCGPU_ERR_NAN, CCGPU_ERR_MEMORY, ...)
     std::string error;
     // device side (resident only)
     int lane = 0;  // solver lane (stream) the job is resident on
@@ -115,6 +119,7 @@ struct SolverLane {
     bool desc_pending = false;
     size_t marks_at_read = 0;      // timing marks recorded before that read (complete once it is)
     DevArray<RelaxDesc> d_rdesc;  // jobs stepped by kRelaxBlock threads, then the n_big large ones
+    std::vector<int> rd_of_slot;  // index in d_rdesc of each slot's job
     int n_big = 0;
     DevArray<WorkItem> d_wide;
     int nwide = 0;
@@ -170,6 +175,10 @@ public:
         precision_ = mode;
         polish_ = polish;
     }
+    // wall-clock limit (seconds from admission) of jobs submitted from now on; 0: none.  A job past its limit is
+    // stopped at the next host check and returned with the NLopt code of a time-limit stop (6)
+    void SetTimeLimit(double seconds) { time_limit_ = seconds > 0 ? seconds : 0; }
+    double TimeLimit() const { return time_limit_; }
     // rho backend of jobs submitted from now on (0 direct, 1 NUFFT with tolerance eps)
     void SetRhoBackend(int backend, double eps)
     {
@@ -196,6 +205,7 @@ private:
     void LaunchBurst(int burst);
     void EnqueueDescRead();  // queue each lane's descriptor-table read behind its last burst
     void FinishHarvest();    // wait for those reads, retire the jobs they show finished
+    void StopExpired();      // stop the resident jobs past their wall-clock limit (queued behind the current work)
     void SyncLanes();
     bool NeedsRefresh() const;
     static int64_t EstimateBytes(const Job &j);
@@ -209,6 +219,7 @@ private:
     int64_t graph_bursts_ = 0, stream_bursts_ = 0;
     int precision_ = 0;
     double polish_ = 1e-8;
+    double time_limit_ = 0;
     int rho_backend_ = 0;
     double nufft_eps_ = 1e-12;
     std::deque<Job *> queue_;

@@ -223,6 +223,22 @@ __global__ void __launch_bounds__(BS) k_relax_step(SysDesc *__restrict__ sysv, c
         E = D->energy[0];
         st.E = E;
         st.device_evals++;
+        // A NaN energy or a non-finite gradient ends the job as failed (the reference's Objective throws
+        // NotANumberFound on NaN coordinates, potential/StructureOptimization.cpp:458-461; NaN coordinates follow
+        // from a NaN energy or gradient at the next trial point).  An infinite energy with a finite gradient stays a
+        // valid, if huge, objective value: the line search bisects below its huge-value bound.
+        bool bad = E != E;
+        const double *gr = D->grad;
+        for (int i = threadIdx.x; i < R.n; i += BS) bad = bad || !isfinite(gr[i]);
+        if (__syncthreads_or(bad)) {
+            if (threadIdx.x == 0) {
+                st.failed = 1;
+                st.finished = 1;
+                *R.st = st;
+                D->flags &= ~kSysActive;
+            }
+            return;
+        }
         // Mixed precision: the FP32-phase objective drives the descent until E first falls below `polish` or
         // the first pass ends; then the system goes to FP64 and the same x is evaluated again as the start of
         // the interrupted (or next) pass, so the polish iterations and every convergence test see FP64.
@@ -532,6 +548,27 @@ __global__ void __launch_bounds__(kWideBlock) k_lbfgs_pass2(const SysDesc *__res
     }
 }
 
+// Stop a job past its wall-clock limit: later launches skip its system and the next harvest retires it.  Its
+// result is the last evaluated point: while a line-search trial is pending, x goes back to the search's start point
+// (kept in the newest history slot until the step is accepted) and E to its objective.  One block.
+__global__ void __launch_bounds__(kRelaxBlock) k_relax_stop(SysDesc *sys, const RelaxDesc *rd, int idx)
+{
+    const RelaxDesc R = rd[idx];
+    SysDesc *D = sys + R.sys;
+    if (!(D->flags & kSysActive)) return;
+    const bool pending = R.st->plis.phase == kPhLineSearchEval;
+    if (pending) {
+        double *x = const_cast<double *>(D->x);
+        const double *x0 = R.xo + (size_t)R.st->plis.head * R.n;
+        for (int i = threadIdx.x; i < R.n; i += kRelaxBlock) x[i] = x0[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (pending) R.st->E = R.st->plis.f0;
+        D->flags &= ~kSysActive;
+    }
+}
+
 void launch_relax_step(hipStream_t stream, SysDesc *sys, const RelaxDesc *rd, int n, int n_big,
                        const WorkItem *wide_items, int nwide)
 {
@@ -651,6 +688,7 @@ int64_t Solver::Submit(std::unique_ptr<Job> job)
 {
     job->id = next_id_++;
     job->t_submit = Now();
+    job->time_limit = time_limit_;
     job->bytes = EstimateBytes(*job);
     if (job->bytes > budget_) throw Error(kErrMemory, "job does not fit the HBM budget of the solver");
     Job *p = job.get();
@@ -779,6 +817,7 @@ void Solver::Admit()
             L.dirty = true;
         } catch (const Error &e) {
             j->state = Job::kFailed;
+            j->error_code = e.code;
             j->error = e.what();
             j->sys.reset();
             j->t_done = Now();
@@ -805,6 +844,8 @@ void Solver::Rebuild(SolverLane &L)
             if ((L.jobs[i]->N * L.jobs[i]->d >= kRelaxBigN) == (big == 1)) order.push_back(i);
     L.n_big = 0;
     for (size_t i : order) L.n_big += L.jobs[i]->N * L.jobs[i]->d >= kRelaxBigN;
+    L.rd_of_slot.assign(L.jobs.size(), 0);
+    for (size_t r = 0; r < order.size(); r++) L.rd_of_slot[order[r]] = (int)r;
     for (size_t i : order) {
         Job *j = L.jobs[i];
         RelaxDesc r;
@@ -932,6 +973,15 @@ void Solver::FinishHarvest()
         j->switch_eval = hs.switch_eval;
         j->t_done = Now();
         j->state = Job::kDone;
+        if (hs.failed) {
+            j->state = Job::kFailed;
+            j->error_code = kErrNan;
+            j->error = "NaN energy or non-finite gradient at device evaluation " + std::to_string(hs.device_evals) +
+                       " (NotANumberFound)";
+        } else if (j->timed_out && !hs.finished) {
+            j->status = 6;  // NLOPT_MAXTIME_REACHED: the configuration of the last evaluation
+            j->evals += hs.plis.nevals;
+        }
         // the system stays listed in its lane's batch until the next rebuild: retire it (host flag cleared, so the
         // host-driven FFT stage skips it too) and free it at that rebuild; the other buffers go back to the pool
         // now -- kernels still running skip this job (its flag is clear) and the pool hands them out again only at
@@ -953,6 +1003,22 @@ void Solver::FinishHarvest()
     for (Job *j : resident_)
         if (j->state == Job::kResident) keep.push_back(j);
     resident_.swap(keep);
+}
+
+void Solver::StopExpired()
+{
+    const double now = Now();
+    for (auto &l : lanes_) {
+        SolverLane &L = *l;
+        for (size_t q = 0; q < L.slots.size(); q++) {
+            Job *j = L.slots[q];
+            if (!j || j->timed_out || j->time_limit <= 0 || now - j->t_admit <= j->time_limit) continue;
+            j->timed_out = true;
+            hipLaunchKernelGGL(k_relax_stop, dim3(1), dim3(kRelaxBlock), 0, L.batch.stream, L.batch.d_desc.get(),
+                               L.d_rdesc.get(), L.rd_of_slot[q]);
+            CC_HIP(hipGetLastError());
+        }
+    }
 }
 
 void Solver::SyncLanes()
@@ -1044,6 +1110,7 @@ int Solver::Step(int evaluations)
             done += burst;
         }
         FinishHarvest();  // the read queued behind the previous burst, if any
+        StopExpired();
         if (!launch) break;
         if (NeedsRefresh()) {
             EnqueueDescRead();

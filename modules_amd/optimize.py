@@ -327,7 +327,15 @@ class BatchSolver:
             if timeout is not None and time.time() - t0 > timeout:
                 raise TimeoutError("BatchSolver.run: timeout")
 
+    def set_time_limit(self, seconds):
+        """Wall-clock limit (seconds from admission) of jobs submitted from now on; 0: none.  A job past its limit
+        is stopped and returned with status 6 (NLOPT_MAXTIME_REACHED) and its last evaluated configuration."""
+        check(lib().ccgpu_solver_set_time_limit(self._h, float(seconds)))
+
     def result(self, job):
+        """State, energy, counts, times and relaxed configuration of a job.  A failed job (state FAILED) also
+        carries "error" (the message) and "code" (CCGPU_ERR_NAN for a NaN energy or non-finite gradient,
+        the reference's NotANumberFound; CCGPU_ERR_MEMORY or CCGPU_ERR_ARG when admission failed)."""
         state = C.c_int(0)
         E = C.c_double(0)
         ev = C.c_int(0)
@@ -335,13 +343,21 @@ class BatchSolver:
         st = C.c_int(0)
         t = np.zeros(3)
         rel = np.zeros(self._shapes[job])
-        check(lib().ccgpu_solver_result(self._h, int(job), C.byref(state), C.byref(E), C.byref(ev), C.byref(dev),
-                                        C.byref(st), _p(t), _p(rel)))
+        code = lib().ccgpu_solver_result(self._h, int(job), C.byref(state), C.byref(E), C.byref(ev), C.byref(dev),
+                                         C.byref(st), _p(t), _p(rel))
+        out = {"state": state.value, "energy": E.value, "evaluations": ev.value, "device_evaluations": dev.value,
+               "status": st.value, "t_submit": t[0], "t_admit": t[1], "t_done": t[2], "relative": rel}
+        if code != 0:
+            if state.value != self.FAILED:
+                check(code)
+            out["code"] = code
+            out["error"] = lib().ccgpu_last_error().decode()
+            out["switch_eval"] = 0
+            return out
         sw = C.c_int(0)
         check(lib().ccgpu_solver_switch_eval(self._h, int(job), C.byref(sw)))
-        return {"state": state.value, "energy": E.value, "evaluations": ev.value, "device_evaluations": dev.value,
-                "status": st.value, "t_submit": t[0], "t_admit": t[1], "t_done": t[2], "relative": rel,
-                "switch_eval": sw.value}
+        out["switch_eval"] = sw.value
+        return out
 
     def release(self, job):
         check(lib().ccgpu_solver_release(self._h, int(job)))
