@@ -501,8 +501,8 @@ def main():
 
 
 def _tune(args, cfg, k):
-    """Autotuned tiles for one configuration of this shape (unless --no-autotune)."""
-    if args.no_autotune:
+    """Autotuned tiles for one configuration of this shape (with --autotune)."""
+    if not args.autotune:
         return None
     from modules_amd import autotune
     return autotune.autotune(cfg.NumParticle(), cfg.GetDimension(), k=k, basis=cfg.basis, batch=1, repeats=2)

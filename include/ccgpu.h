@@ -485,6 +485,9 @@ int ccgpu_solver_result(ccgpu_solver *s, int64_t job, int *state, double *energy
  * the configuration and energy of its last evaluation: the per-job counterpart of the reference's global
  * TimeLimit, which writes the current configuration and exits (potential/StructureOptimization.cpp:352-378). */
 int ccgpu_solver_set_time_limit(ccgpu_solver *s, double seconds);
+/* HBM accounting of a job (no counterpart in the reference): the estimate admission charges against the budget, and
+ * the device bytes it holds while resident (0 otherwise; its systems' buffers, shared constraint tables included). */
+int ccgpu_solver_job_bytes(ccgpu_solver *s, int64_t job, int64_t *estimate, int64_t *resident);
 /* Forget a finished job and free its slot (no counterpart in the reference). */
 int ccgpu_solver_release(ccgpu_solver *s, int64_t job);
 /* Resident and queued job counts and the HBM the solver holds (no counterpart in the reference). */

@@ -115,6 +115,7 @@ SIGNATURES = {
     "ccgpu_solver_set_timing": (C.c_int, [vp, C.c_int]),
     "ccgpu_solver_launch_stats": (C.c_int, [vp, i64p, i64p]),
     "ccgpu_solver_set_time_limit": (C.c_int, [vp, C.c_double]),
+    "ccgpu_solver_job_bytes": (C.c_int, [vp, C.c_int64, i64p, i64p]),
     "ccgpu_autotune": (C.c_int, [C.c_int, C.c_int64, dp, C.c_int64, dp, dp, C.c_int, C.c_int, C.POINTER(C.c_int),
                                  C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), dp]),
     "ccgpu_autotune_clear": (C.c_int, []),

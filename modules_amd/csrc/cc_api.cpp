@@ -1524,6 +1524,17 @@ int ccgpu_solver_result(ccgpu_solver *s, int64_t job, int *state, double *energy
     });
 }
 
+int ccgpu_solver_job_bytes(ccgpu_solver *s, int64_t job, int64_t *estimate, int64_t *resident)
+{
+    return Guard([&] {
+        Require(s != nullptr, CCGPU_ERR_ARG, "null handle");
+        Job *j = s->solver.Find(job);
+        if (!j) throw Error(CCGPU_ERR_RANGE, "unknown job");
+        if (estimate) *estimate = j->bytes;
+        if (resident) *resident = j->state == Job::kResident ? j->DeviceBytes() : 0;
+    });
+}
+
 int ccgpu_solver_set_time_limit(ccgpu_solver *s, double seconds)
 {
     return Guard([&] {

@@ -85,6 +85,7 @@ struct Job {
     double u0 = 1.0, rc = 0.0;
     bool mixture() const { return !spec_idx.empty(); }
     int64_t Work() const;  // pair visits of one evaluation plus the history passes (lane balancing weight)
+    int64_t DeviceBytes() const;  // device memory the resident job holds (pool capacities)
     // lifecycle
     enum State { kQueued, kResident, kDone, kFailed } state = kQueued;
     double t_submit = 0, t_admit = 0, t_done = 0;  // seconds on the solver clock

@@ -17,10 +17,15 @@ template <int BS>
 struct BlockPlisOps {
     int n, mf;
     double *x, *g, *s, *xo, *go;
-    double *lds_red;  // 2 x BS / 64 doubles
+    double *lds_red;  // 2 x 5 x BS / 64 doubles
     double *lds_u, *lds_v;
     bool gs_valid;      // g.S of the last direction is cached until g or S changes
     double gs_val;
+    // sums of the fused pair pass (form_pair), valid until the next vector write: |s|_1 and |x|_1, max |g|, g.g and
+    // s.y of the new pair in slots (ps, py)
+    bool pair_valid = false;
+    int ps = -1, py = -1;
+    double l1_s = 0, l1_x = 0, max_g = 0, gg = 0, sy = 0;
 
     __device__ double *vec(int id) const
     {
@@ -46,9 +51,30 @@ struct BlockPlisOps {
         for (int i = 1; i < BS / 64; i++) r = is_max ? fmax(r, slot[i]) : r + slot[i];
         return r;
     }
+    // five block-wide reductions at once (sums, the maximum in position 2): one barrier
+    __device__ void block_reduce5(double (&v)[5]) const
+    {
+        constexpr int nw = BS / 64;
+#pragma unroll
+        for (int q = 0; q < 5; q++) v[q] = q == 2 ? wave_max_dpp(v[q]) : wave_sum_dpp(v[q]);
+        double *slot = lds_red + red_slot * (5 * nw);
+        red_slot ^= 1;
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0)
+#pragma unroll
+            for (int q = 0; q < 5; q++) slot[q * nw + w] = v[q];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+            double r = slot[q * nw];
+            for (int i = 1; i < nw; i++) r = q == 2 ? fmax(r, slot[q * nw + i]) : r + slot[q * nw + i];
+            v[q] = r;
+        }
+    }
     __device__ void touch(int id)
     {
         if (id == kVecS || id == kVecG) gs_valid = false;
+        pair_valid = false;
     }
     // Element loops visit this thread's elements kU at a time, all loads of a batch before their uses, so that
     // each thread keeps several independent HBM loads in flight (one workgroup per job leaves few waves per CU);
@@ -85,6 +111,8 @@ struct BlockPlisOps {
     __device__ double dot(int a, int b)
     {
         if (gs_valid && ((a == kVecG && b == kVecS) || (a == kVecS && b == kVecG))) return gs_val;
+        if (pair_valid && a == kVecG && b == kVecG) return gg;
+        if (pair_valid && ((a == ps && b == py) || (a == py && b == ps))) return sy;
         const double *pa = vec(a), *pb = vec(b);
         double t = 0.0, va[kU], vb[kU];
         batches([&](int u, int i) { va[u] = pa[i]; vb[u] = pb[i]; },
@@ -134,6 +162,7 @@ struct BlockPlisOps {
     }
     __device__ double maxabs(int xv) const
     {
+        if (pair_valid && xv == kVecG) return max_g;
         const double *p = vec(xv);
         double m = 0.0, v[kU];
         batches([&](int u, int i) { v[u] = p[i]; }, [&](int u, int) { m = fmax(m, fabs(v[u])); });
@@ -141,10 +170,82 @@ struct BlockPlisOps {
     }
     __device__ double l1(int xv) const
     {
+        if (pair_valid && xv == ps) return l1_s;
+        if (pair_valid && xv == kVecX) return l1_x;
         const double *p = vec(xv);
         double t = 0.0, v[kU];
         batches([&](int u, int i) { v[u] = p[i]; }, [&](int u, int) { t += fabs(v[u]); });
         return block_reduce(t, false);
+    }
+    // The new difference pair in slots (sv, yv): s = x - s, y = g - y, in one pass that also forms |s|_1, |x|_1,
+    // max |g|, g.g and s.y for the tests and the direction that follow (each summed in this thread's element order,
+    // as the separate operations would).
+    __device__ void form_pair(int sv, int yv)
+    {
+        touch(sv);
+        touch(yv);
+        double *psv = vec(sv), *pyv = vec(yv);
+        constexpr int kP = kU < 8 ? kU : 8;
+        double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // |s|_1, |x|_1, max |g|, g.g, s.y
+        int e = threadIdx.x;
+        for (; e < n; e += kP * BS) {
+            double vx[kP], vs[kP], vg[kP], vy[kP];
+#pragma unroll
+            for (int u = 0; u < kP; u++) {
+                const int i = e + u * BS;
+                const bool in = i < n;
+                vx[u] = in ? x[i] : 0.0;
+                vs[u] = in ? psv[i] : 0.0;
+                vg[u] = in ? g[i] : 0.0;
+                vy[u] = in ? pyv[i] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < kP; u++) {
+                const int i = e + u * BS;
+                if (i < n) {
+                    const double ds = vx[u] - vs[u], dy = vg[u] - vy[u];
+                    psv[i] = ds;
+                    pyv[i] = dy;
+                    a[0] += fabs(ds);
+                    a[1] += fabs(vx[u]);
+                    a[2] = fmax(a[2], fabs(vg[u]));
+                    a[3] = __builtin_fma(vg[u], vg[u], a[3]);
+                    a[4] = __builtin_fma(ds, dy, a[4]);
+                }
+            }
+        }
+        block_reduce5(a);
+        ps = sv;
+        py = yv;
+        l1_s = a[0];
+        l1_x = a[1];
+        max_g = a[2];
+        gg = a[3];
+        sy = a[4];
+        pair_valid = true;
+    }
+    // The start point of a line search into slots (sv, yv): s = x, y = g, and with a trial x = s + r S, in one pass.
+    __device__ void save_start(int sv, int yv, double r, bool trial)
+    {
+        touch(sv);
+        touch(yv);
+        touch(kVecX);
+        double *psv = vec(sv), *pyv = vec(yv);
+        double vx[kU], vg[kU], vd[kU];
+        if (trial) {
+            batches([&](int u, int i) { vx[u] = x[i]; vg[u] = g[i]; vd[u] = s[i]; },
+                    [&](int u, int i) {
+                        psv[i] = vx[u];
+                        pyv[i] = vg[u];
+                        x[i] = __builtin_fma(r, vd[u], vx[u]);
+                    });
+        } else {
+            batches([&](int u, int i) { vx[u] = x[i]; vg[u] = g[i]; },
+                    [&](int u, int i) {
+                        psv[i] = vx[u];
+                        pyv[i] = vg[u];
+                    });
+        }
     }
     __device__ double u(int i) const { return lds_u[i]; }
     __device__ double v(int i) const { return lds_v[i]; }
@@ -185,12 +286,12 @@ __device__ __forceinline__ double relax_wrap(double x)
 // One optimizer step for every resident job.  Stage 0 consumes the evaluation just made (energy in
 // SysDesc.energy, gradient in SysDesc.grad) and advances PLIS until it needs the next point, a new two-loop
 // direction (left to the wide kernels), or the end of a pass; stage 1 resumes the jobs that asked for a
-// direction.  Grid: one block of kRelaxBlock threads per job.
+// direction.  Grid: one block of kRelaxBlock threads per job (kRelaxBlockBig for the large ones).
 template <int BS>
 __global__ void __launch_bounds__(BS) k_relax_step(SysDesc *__restrict__ sysv, const RelaxDesc *__restrict__ rdv,
                                                            int stage)
 {
-    __shared__ double lds_red[2 * BS / 64];
+    __shared__ double lds_red[2 * 5 * BS / 64];
     __shared__ double lds_u[kPlisMaxHistory], lds_v[kPlisMaxHistory];
     const RelaxDesc R = rdv[blockIdx.x];
     SysDesc *D = sysv + R.sys;
@@ -203,7 +304,9 @@ __global__ void __launch_bounds__(BS) k_relax_step(SysDesc *__restrict__ sysv, c
     }
     __syncthreads();
     BlockPlisOps<BS> ops{R.n, R.mf, const_cast<double *>(D->x), D->grad, R.s, R.xo, R.go, lds_red, lds_u, lds_v,
-                     false, 0.0, 0, 0.0};
+                         false, 0.0};
+    ops.red_slot = 0;
+    ops.res_snorm = 0.0;
     int act = kPlisNeedEval;
     bool finish = false;
     int calls = 0;  // PlisAdvance runs from one call site (inlined once, state in registers)
@@ -228,8 +331,11 @@ __global__ void __launch_bounds__(BS) k_relax_step(SysDesc *__restrict__ sysv, c
         // from a NaN energy or gradient at the next trial point).  An infinite energy with a finite gradient stays a
         // valid, if huge, objective value: the line search bisects below its huge-value bound.
         bool bad = E != E;
-        const double *gr = D->grad;
-        for (int i = threadIdx.x; i < R.n; i += BS) bad = bad || !isfinite(gr[i]);
+        {
+            const double *gr = D->grad;
+            double v[BlockPlisOps<BS>::kU];
+            ops.batches([&](int u, int i) { v[u] = gr[i]; }, [&](int u, int) { bad |= !isfinite(v[u]); });
+        }
         if (__syncthreads_or(bad)) {
             if (threadIdx.x == 0) {
                 st.failed = 1;
@@ -667,19 +773,30 @@ int64_t Job::Work() const
 int64_t Solver::EstimateBytes(const Job &j)
 {
     const int64_t n = j.N * j.d;
-    // constraint-side buffers of one system of Np particles and M constraints
-    auto cc = [](int64_t Np, int64_t M) {
-        const int64_t slots = M + M / 2 + 64;        // segment padding
-        const int64_t pchunks = (Np + 63) / 64;      // upper bound of the rho chunks
-        const int64_t kchunks = 16;
-        return 8 * (slots * (2 * std::min<int64_t>(pchunks, 64) + 8) + kchunks * Np * 4) + 4096;
-    };
-    int64_t b = 8 * (n * (2 * j.mf + 6) + j.N * 10 + j.N * 8) + 4096;  // positions, history, forces, cell list
+    Tiles req{0, 0, 0, -1, 0};
+    if (j.Tiles_seg) req.seg = j.Tiles_seg;
+    // positions, history, direction, forces, gradient, cell list and per-job L-BFGS partials
+    const int64_t nch = std::max<int64_t>(1, (n + kRelaxChunk - 1) / kRelaxChunk);
+    int64_t b = 8 * (n * (2 * j.mf + 6) + j.N * 10 + j.N * 8) + 8 * nch * (kPlisMaxHistory * 5 + 2) +
+                8 * (2 * j.mf * j.mf + 2 * j.mf) + 4096;
     if (j.mixture()) {
         b += 12 * j.N;  // radii and particle ids
-        for (size_t q = 0; q < j.spec_idx.size(); q++) b += cc((int64_t)j.spec_idx[q].size(), (int64_t)j.spec_v[q].size());
+        for (size_t q = 0; q < j.spec_idx.size(); q++)
+            b += EstimateSystemBytes(j.d, j.basis.data(), (int64_t)j.spec_idx[q].size(), (int64_t)j.spec_v[q].size(),
+                                     j.spec_k[q].data(), req, 0);
     } else {
-        b += cc(j.N, (int64_t)j.v.size());
+        b += EstimateSystemBytes(j.d, j.basis.data(), j.N, (int64_t)j.v.size(), j.k.data(), req, j.precision);
+    }
+    return b + b / 8;  // the pool rounds each block up to its size class (at most 1/8 more)
+}
+
+int64_t Job::DeviceBytes() const
+{
+    int64_t b = sdir.bytes() + xo.bytes() + go.bytes() + uv.bytes() + gram.bytes() + part1.bytes() + part2.bytes() +
+                lcoef.bytes() + st.bytes();
+    if (sys) {
+        b += sys->DeviceBytes();
+        for (auto &c : sys->children) b += c->DeviceBytes();
     }
     return b;
 }

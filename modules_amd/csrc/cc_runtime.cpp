@@ -262,6 +262,38 @@ std::shared_ptr<const ConstraintLayout> GetLayout(int d, const double *a, int64_
     return L;
 }
 
+Tiles ResolveTiles(const Tiles &req, const Tiles &def, int precision)
+{
+    Tiles used;
+    used.seg = req.seg > 0 ? req.seg : def.seg;
+    if (used.seg != 4 && used.seg != 8 && used.seg != 16 && used.seg != 24 && used.seg != 32)
+        throw Error(kErrArg, "segment length must be 4, 8, 16, 24 or 32");
+    used.rho_kernel = req.rho_kernel >= 0 ? req.rho_kernel : def.rho_kernel;
+    if (precision == 1) used.rho_kernel = 0;  // the mixed-precision body lives in the grouped-power kernel
+    used.force_ppl = req.force_ppl > 0 ? req.force_ppl : (def.force_ppl > 0 ? def.force_ppl : 2);
+    if (used.force_ppl != 1 && used.force_ppl != 2) throw Error(kErrArg, "force particles per lane must be 1 or 2");
+    return used;
+}
+
+int64_t EstimateSystemBytes(int d, const double *a, int64_t N, int64_t M, const double *k, const Tiles &req,
+                            int precision)
+{
+    const Tiles def = DefaultTiles(d, N, M);
+    const Tiles used = ResolveTiles(req, def, precision);
+    std::vector<int32_t> n;
+    const bool lattice = M > 0 && LatticeIndices(d, a, M, k, n);
+    const int64_t nseg = lattice ? BuildSegments(d, M, n.data(), used.seg).nseg : M;
+    const int64_t nslots = lattice ? nseg * used.seg : M;
+    const ChunkPlan cp = PlanChunks(N, M, lattice, nseg, used, req, def);
+    const int64_t Nn = std::max<int64_t>(N, 1);
+    int64_t b = 0;
+    b += 8 * (int64_t)cp.pchunks * nslots * 2;              // rpart
+    b += 8 * (2 * M + 2 * nslots + nslots) + 4 * 2 * nslots;  // rho, coef, epart, coef32
+    b += 8 * (int64_t)cp.kchunks * Nn * 4;                   // fpart
+    b += nslots * (4 + 8 + 8) + nseg * (16 + 4) + M * 32;     // layout tables (an upper bound: shared by equal shapes)
+    return b + 4096;
+}
+
 ChunkPlan PlanChunks(int64_t N, int64_t M, bool lattice, int64_t nunits, const Tiles &used, const Tiles &req,
                      const Tiles &def)
 {
@@ -378,14 +410,8 @@ void System::Prepare(hipStream_t stream)
     if (energy.get()) CC_HIP(hipStreamSynchronize(stream));
     if (view() && (parent->d != d || view_off < 0 || view_off + N > parent->N))
         throw Error(kErrState, "species range outside its mixture");
-    Tiles def = DefaultTiles(d, N, M);
-    used.seg = tiles.seg > 0 ? tiles.seg : def.seg;
-    if (used.seg != 4 && used.seg != 8 && used.seg != 16 && used.seg != 24 && used.seg != 32)
-        throw Error(kErrArg, "segment length must be 4, 8, 16, 24 or 32");
-    used.rho_kernel = tiles.rho_kernel >= 0 ? tiles.rho_kernel : def.rho_kernel;
-    if (precision == 1) used.rho_kernel = 0;  // the mixed-precision body lives in the grouped-power kernel
-    used.force_ppl = tiles.force_ppl > 0 ? tiles.force_ppl : (def.force_ppl > 0 ? def.force_ppl : 2);
-    if (used.force_ppl != 1 && used.force_ppl != 2) throw Error(kErrArg, "force particles per lane must be 1 or 2");
+    const Tiles def = DefaultTiles(d, N, M);
+    used = ResolveTiles(tiles, def, precision);
 
     if (layout_dirty || !layout || layout->S != used.seg) {
         if (s2 && (int64_t)s0.size() != M) throw Error(kErrState, "target-S(k) potential: one S0 per constraint");

@@ -95,6 +95,12 @@ struct ChunkPlan {
 };
 ChunkPlan PlanChunks(int64_t N, int64_t M, bool lattice, int64_t nunits, const Tiles &used, const Tiles &req,
                      const Tiles &def);
+// The tiles a system uses: requested values where given, else the defaults; mixed precision takes rho kernel 0.
+Tiles ResolveTiles(const Tiles &req, const Tiles &def, int precision);
+// Device bytes of the constraint-side buffers of one system (N particles, M constraints k in cell a), with the same
+// segment plan and chunk rule as System::Prepare: the solver's admission estimate.
+int64_t EstimateSystemBytes(int d, const double *a, int64_t N, int64_t M, const double *k, const Tiles &req,
+                            int precision);
 
 // Process-wide table of tuned tiles keyed by (d, N bucket, M bucket) (half-octave buckets); DefaultTiles
 // consults it.  Filled by Autotune; a system's tiles depend only on its own shape and the table.

@@ -10,7 +10,8 @@
 // other start-step rules and line-search modes) have no counterpart here.
 //
 // The caller owns the vectors; an Ops policy performs the vector operations (sequential loops on the host,
-// block-cooperative loops on the device).  PlisAdvance() runs until it needs an objective evaluation at x, a search
+// block-cooperative loops on the device, which may fuse the passes of form_pair and save_start with the reductions
+// that follow them).  PlisAdvance() runs until it needs an objective evaluation at x, a search
 // direction (when the policy forms it outside, Ops::kDeferTwoLoop), or the method ends.  All scalars persist in
 // PlisState, so a batch of solves keeps one state per configuration in HBM and advances them in one launch.
 //
@@ -478,20 +479,21 @@ PLIS_HD PLIS_INLINE inline int PlisAdvance(PlisState &S, Ops &ops)
                     next = kDirection;
                     break;
                 }
-                // the start point of the search becomes the newest history slot (difference formed on acceptance)
+                // the start point of the search becomes the newest history slot (difference formed on acceptance),
+                // and with a trial step x = x0 + r S
                 S.f0 = S.f;
                 S.slope0 = S.slope;
-                ops.copy(kVecX, s1());
-                ops.copy(kVecG, y1());
                 if (S.rmax == 0.) {
+                    ops.save_start(s1(), y1(), 0., false);
                     next = kTests;
                     break;
                 }
                 if (plis_search_begin(S)) {
-                    ops.axpy(kVecX, S.r, kVecS, s1());  // x = x0 + r S
+                    ops.save_start(s1(), y1(), S.r, true);
                     S.phase = kPhLineSearchEval;
                     return kPlisNeedEval;
                 }
+                ops.save_start(s1(), y1(), 0., false);
                 next = kSearch;
                 break;
             }
@@ -512,8 +514,7 @@ PLIS_HD PLIS_INLINE inline int PlisAdvance(PlisState &S, Ops &ops)
                     break;
                 }
                 // accepted: the new pair s = x - x0, y = g - g0, then the relative x-change test
-                ops.diff(kVecX, s1(), s1());
-                ops.diff(kVecG, y1(), y1());
+                ops.form_pair(s1(), y1());
                 S.slope0 = S.r * S.slope0;
                 S.slope = S.r * S.slope;
                 S.x_flat = ops.l1(s1()) < PlisParam::tol_x * ops.l1(kVecX) ? 1 : 0;

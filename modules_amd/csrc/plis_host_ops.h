@@ -66,6 +66,19 @@ struct HostPlisOps {
         double *pz = vec(z);
         for (int i = 0; i < n; i++) pz[i] = px[i] - py[i];
     }
+    // the new difference pair in slots (sv, yv): s = x - s, y = g - y
+    void form_pair(int sv, int yv) const
+    {
+        diff(kVecX, sv, sv);
+        diff(kVecG, yv, yv);
+    }
+    // the start point of a line search into slots (sv, yv), and with a trial x = s + r S
+    void save_start(int sv, int yv, double r, bool trial) const
+    {
+        copy(kVecX, sv);
+        copy(kVecG, yv);
+        if (trial) axpy(kVecX, r, kVecS, sv);
+    }
     double maxabs(int xv) const
     {
         const double *p = vec(xv);

@@ -375,6 +375,13 @@ class BatchSolver:
         check(lib().ccgpu_solver_timing(self._h, C.byref(a), C.byref(b), C.byref(na), C.byref(nb), 1 if reset else 0))
         return a.value, b.value, na.value, nb.value
 
+    def job_bytes(self, job):
+        """(estimate charged against the HBM budget, device bytes held while resident) of a job."""
+        e = C.c_int64(0)
+        r = C.c_int64(0)
+        check(lib().ccgpu_solver_job_bytes(self._h, int(job), C.byref(e), C.byref(r)))
+        return e.value, r.value
+
     def launch_stats(self):
         """Bursts of optimizer steps launched so far: {"graph": replayed hipGraphs, "stream": stream-launched}."""
         g = C.c_int64(0)

@@ -70,3 +70,16 @@ def test_time_limit_returns_the_last_evaluated_point():
     rs = solver.result(js)
     ra = solve_alone(small)
     assert rs["status"] != 6 and rs["energy"] == ra["energy"] and np.array_equal(rs["relative"], ra["relative"])
+
+
+@pytest.mark.parametrize("N,d,chi", [(50000, 3, 0.49), (50000, 2, 0.49), (20000, 2, 0.4), (4000, 3, 0.45), (150, 2, 0.2)])
+def test_admission_estimate_covers_the_resident_buffers(N, d, chi):
+    # the HBM estimate of a job (same segment plan and chunk rule as the system's own preparation) is at least what
+    # the resident job holds, up to the largest serving requests
+    cfg, k, v = stealthy(N, d, chi, 9)
+    solver = BatchSolver(max_slots=1)
+    j = solver.submit(cfg, k, v, Emin=EMIN)
+    solver.step(0)  # admission only
+    est, held = solver.job_bytes(j)
+    assert held > 0 and est >= held, (est, held)
+    assert est <= 2 * held + (1 << 20)  # and not wildly above it
