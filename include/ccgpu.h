@@ -154,6 +154,18 @@ int ccgpu_potential_all_force(ccgpu_potential *p, double *F);
  * equal to Energy() followed by AllForce(). */
 int ccgpu_potential_energy_and_all_force(ccgpu_potential *p, double *E, double *F);
 
+/* No counterpart in the reference: the variant of ccgpu_potential_energy_and_all_force for callers that cannot
+ * allocate device memory (tight serving loops, harnesses that forbid hipMalloc).  Energy and all N x dim forces of
+ * the plain stealthy potential (constraints k M x dim, V M; Shift 0, no soft core, direct sums, FP64) on the
+ * configuration (cell basis dim x dim, relative N x dim), with the same bits as a potential bound to the same
+ * constraints and configuration.  It calls no hipMalloc, hipFree or allocating routine: every device buffer of the
+ * evaluation lives in a fixed 64 MiB array of the library, one copy per GPU, held until the process ends;
+ * CCGPU_ERR_MEMORY (before any kernel runs) when an evaluation needs more.  The calls on one GPU share that
+ * array, which keeps the last call's contents: overlapping calls would race on it, so the library runs the
+ * calls of a process on one GPU one at a time (each returns after its results are copied out). */
+int ccgpu_energy_and_all_force_noalloc(int dim, const double *basis, int64_t N, const double *relative, int64_t M,
+                                       const double *k, const double *V, double *E, double *F);
+
 /* Tile shape of the kernels (segment length 4/8/16/24/32, particles per rho chunk (at most 512 for rho kernels 0
  * and 1, 65536 for kernel 2), segments or constraints per chunk of the force pass; 0 keeps the automatic choice).
  * No counterpart in the reference: the hook of the autotuner. */

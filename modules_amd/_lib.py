@@ -53,6 +53,7 @@ SIGNATURES = {
     "ccgpu_potential_force": (C.c_int, [vp, C.c_int64, dp]),
     "ccgpu_potential_all_force": (C.c_int, [vp, dp]),
     "ccgpu_potential_energy_and_all_force": (C.c_int, [vp, dp, dp]),
+    "ccgpu_energy_and_all_force_noalloc": (C.c_int, [C.c_int, dp, C.c_int64, dp, C.c_int64, dp, dp, dp, dp]),
     "ccgpu_potential_set_tiles": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
     "ccgpu_potential_uses_lattice_path": (C.c_int, [vp, C.POINTER(C.c_int)]),
     "ccgpu_potential_set_rho_kernel": (C.c_int, [vp, C.c_int]),

@@ -41,8 +41,29 @@ size_t SizeClass(size_t bytes)
 }
 }  // namespace
 
-void *PoolAlloc(size_t bytes, size_t *cap_bytes)
+namespace {
+thread_local Arena *tl_arena = nullptr;
+}
+
+ArenaScope::ArenaScope(Arena &a) : prev_(tl_arena) { tl_arena = &a; }
+ArenaScope::~ArenaScope() { tl_arena = prev_; }
+bool ArenaInstalled() { return tl_arena != nullptr; }
+
+void *PoolAlloc(size_t bytes, size_t *cap_bytes, bool *pooled)
 {
+    if (Arena *A = tl_arena) {
+        const size_t need = (std::max<size_t>(bytes, 1) + 255) / 256 * 256;
+        if (need > A->cap - A->used)
+            throw Error(kErrMemory, "the evaluation needs more than the " + std::to_string(A->cap) +
+                                        "-byte scratch region (" + std::to_string(A->used + need) +
+                                        " bytes so far)");
+        void *p = A->base + A->used;
+        A->used += need;
+        *cap_bytes = need;
+        *pooled = false;
+        return p;
+    }
+    *pooled = true;
     const size_t cls = SizeClass(bytes);
     int dev = 0;
     CC_HIP(hipGetDevice(&dev));
@@ -258,7 +279,7 @@ std::shared_ptr<const ConstraintLayout> GetLayout(int d, const double *a, int64_
             CC_HIP(hipStreamSynchronize(stream));
         }
     }
-    bucket.push_back(L);
+    if (!ArenaInstalled()) bucket.push_back(L);  // a layout in a scratch arena ends with its caller
     return L;
 }
 
@@ -588,12 +609,14 @@ void System::Prepare(hipStream_t stream)
 // ------------------------------------------------------------------------------------------------
 Batch::Batch() { CC_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)); }
 
+Batch::Batch(hipStream_t external) : stream(external), owns_stream_(false) {}
+
 Batch::~Batch()
 {
     // the systems' buffers return to the pool after this: let the stream drain first
-    if (stream) (void)hipStreamSynchronize(stream);
+    (void)hipStreamSynchronize(stream);
     for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
-    if (stream) (void)hipStreamDestroy(stream);
+    if (stream && owns_stream_) (void)hipStreamDestroy(stream);
 }
 
 int Batch::NextEvent()

@@ -34,10 +34,30 @@ void CheckHip(hipError_t e, const char *what);
 // to new shapes, reuse memory instead of calling hipMalloc / hipFree (which synchronise with the device).
 // Cached blocks stay allocated until PoolTrim or the end of the process; a failed hipMalloc trims the pool
 // and retries once.
-void *PoolAlloc(size_t bytes, size_t *cap_bytes);
+// *pooled: false for a block carved from the thread's scratch arena (below), which must not be handed to PoolFree.
+void *PoolAlloc(size_t bytes, size_t *cap_bytes, bool *pooled);
 void PoolFree(void *p, size_t cap_bytes);
 void PoolTrim();
 size_t PoolCachedBytes();
+
+// Scratch arena of a thread: while an ArenaScope is live, the blocks that thread reserves through PoolAlloc are
+// carved in order (256-byte aligned) from a device region the caller owns -- no hipMalloc, no pool block -- and a
+// request past the region's end throws kErrMemory.  The blocks are only valid until the scope ends.
+struct Arena {
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+};
+class ArenaScope {
+public:
+    explicit ArenaScope(Arena &a);
+    ~ArenaScope();
+    ArenaScope(const ArenaScope &) = delete;
+    ArenaScope &operator=(const ArenaScope &) = delete;
+
+private:
+    Arena *prev_;
+};
+bool ArenaInstalled();
 
 // owning device array from the pool, grows on demand (never shrinks)
 template <class T>
@@ -52,16 +72,17 @@ public:
         if (n <= cap_) return;
         Release();
         size_t cap_bytes = 0;
-        p_ = static_cast<T *>(PoolAlloc((n ? n : 1) * sizeof(T), &cap_bytes));
+        p_ = static_cast<T *>(PoolAlloc((n ? n : 1) * sizeof(T), &cap_bytes, &pooled_));
         cap_bytes_ = cap_bytes;
         cap_ = cap_bytes / sizeof(T);
     }
     void Release()
     {
-        if (p_) PoolFree(p_, cap_bytes_);
+        if (p_ && pooled_) PoolFree(p_, cap_bytes_);
         p_ = nullptr;
         cap_ = 0;
         cap_bytes_ = 0;
+        pooled_ = true;
     }
     T *get() const { return p_; }
     size_t capacity() const { return cap_; }
@@ -71,6 +92,7 @@ private:
     T *p_ = nullptr;
     size_t cap_ = 0;
     size_t cap_bytes_ = 0;
+    bool pooled_ = true;
 };
 
 // tile shape of one system: segment length of the lattice kernels, particles per rho chunk,
@@ -253,6 +275,8 @@ public:
 class Batch {
 public:
     Batch();
+    // launches on the caller's stream, which the batch does not own (0: the null stream)
+    explicit Batch(hipStream_t external);
     ~Batch();
     Batch(const Batch &) = delete;
     Batch &operator=(const Batch &) = delete;
@@ -299,6 +323,7 @@ public:
     std::vector<SegGroup> seg_groups;
 
 private:
+    bool owns_stream_ = true;
     int NextEvent();  // index of a free event of the pool
     std::vector<hipEvent_t> ev_pool_;
     std::vector<int> ev_free_;
