@@ -190,10 +190,25 @@ int ccgpu_potential_set_precision(ccgpu_potential *p, int mode);
  * transform for rho on a fine grid and a Type 2 transform for the forces, requested tolerance eps, default 1e-12
  * in the reference) when the constraint set is eligible -- lattice vectors in 1-3 dimensions, a mode box
  * prod (2 K_c + 1) of at most max_modes (2^22 in the reference), no constraint repeating another's mode or its
- * conjugate; otherwise the direct sums are used (GetLastRhoBackendUsed tells which).  Invalidates rho. */
+ * conjugate -- and the cost gate picks it: the autotuner's timed choice for the shape class
+ * (ccgpu_autotune_backend) or else the MI355X cost model, which needs a predicted 1.3x win (the reference's
+ * FinufftForceGate, :318-420, gates the Type 2 forces the same way; here both transforms follow the gate, so a
+ * declined system is bitwise the direct path); 2 the NUFFT whenever eligible.  Otherwise the direct sums are used
+ * (GetLastRhoBackendUsed and ccgpu_potential_nufft_status tell which and why).  Invalidates rho. */
 int ccgpu_potential_set_rho_backend(ccgpu_potential *p, int backend, double eps, int64_t max_modes);
 /* GetLastRhoBackendUsed (CollectiveCoordinatePotential.h:268): 1 if the bound system evaluates by the NUFFT. */
 int ccgpu_potential_rho_backend_used(ccgpu_potential *p, int *backend);
+/* GetLastForceBackendUsed (CollectiveCoordinatePotential.h:275-285): 1 if the last pass that computed all forces
+ * (all_force, energy_and_all_force, or force(i), which reads that pass) used the NUFFT's Type 2 transform, 0 the
+ * direct sums (also before any). */
+int ccgpu_potential_force_backend_used(ccgpu_potential *p, int *backend);
+/* GetLastFinufftStatus (CollectiveCoordinatePotential.h:268-273): *status 0 when the last rho pass ran the NUFFT,
+ * -1 (FinufftNotCalled) when it ran the direct sums or none ran yet.  why (cap bytes, NULL to skip) gets the
+ * reason the bound system takes the direct sums ("" when it takes the NUFFT): the backend, an eligibility
+ * failure, or the cost gate with its predictions; pred_direct_us / pred_nufft_us (NULL to skip) the cost model's
+ * microseconds for one E+F of the bound system on each path (0 when not computed). */
+int ccgpu_potential_nufft_status(ccgpu_potential *p, int *status, char *why, int64_t cap, double *pred_direct_us,
+                                 double *pred_nufft_us);
 
 /* Which kernel family the bound constraint set uses: 1 lattice (segment recurrence), 0 general. */
 int ccgpu_potential_uses_lattice_path(const ccgpu_potential *p, int *lattice);
@@ -207,8 +222,15 @@ int ccgpu_potential_uses_lattice_path(const ccgpu_potential *p, int *lattice);
 int ccgpu_autotune(int dim, int64_t N, const double *basis, int64_t M, const double *k, const double *V, int batch,
                    int repeats, int *segment_len, int *rho_kernel, int *rho_chunk, int *force_chunk, int *force_ppl,
                    double *ms);
-/* Forget every recorded tile shape. */
+/* Forget every recorded tile shape and backend choice. */
 int ccgpu_autotune_clear(void);
+/* Backend autotuner (no counterpart in the reference; the measured analogue of the cost constants the reference fits
+ * offline, CollectiveCoordinatePotential.h:1159-1187): time one E+F of a random configuration of N particles in the
+ * cell `basis` with constraints (k M x dim, V M) on the direct sums and through the NUFFT at tolerance eps (hipEvents,
+ * mean of `repeats`), and record the faster for the (dim, N, M) shape class; systems prepared later with backend 1
+ * follow the record.  *ms_nufft is -1 when the constraint set is not eligible. */
+int ccgpu_autotune_backend(int dim, int64_t N, const double *basis, int64_t M, const double *k, const double *V,
+                           double eps, int repeats, int *use_nufft, double *ms_direct, double *ms_nufft);
 /* Record a tile shape for the (dim, N, M) shape class without timing (e.g. one found by an earlier run); a rho or
  * force chunk of 0 keeps the automatic rule for that chunk. */
 int ccgpu_autotune_record(int dim, int64_t N, int64_t M, int segment_len, int rho_kernel, int rho_chunk,
@@ -478,8 +500,8 @@ int ccgpu_solver_set_graph(ccgpu_solver *s, int on);
  * ends, then FP64 for the rest of the solve: the configuration at the switch is evaluated again in FP64 and
  * the pass restarts from it, so the final energy and every stop test are FP64. */
 int ccgpu_solver_set_precision(ccgpu_solver *s, int mode, double polish_below);
-/* Collective-density backend of jobs submitted after the call (as ccgpu_potential_set_rho_backend; the NUFFT only
- * for eligible single-species jobs, FP64).  No counterpart in the reference's solve loop. */
+/* Collective-density backend of jobs submitted after the call (0, 1 or 2 as ccgpu_potential_set_rho_backend; the
+ * NUFFT only for eligible single-species jobs, FP64).  No counterpart in the reference's solve loop. */
 int ccgpu_solver_set_rho_backend(ccgpu_solver *s, int backend, double eps);
 /* Device evaluation at which a mixed-precision job switched to FP64 (0: FP64 throughout). */
 int ccgpu_solver_switch_eval(ccgpu_solver *s, int64_t job, int *evaluation);

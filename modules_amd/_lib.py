@@ -60,6 +60,10 @@ SIGNATURES = {
     "ccgpu_potential_set_precision": (C.c_int, [vp, C.c_int]),
     "ccgpu_potential_set_rho_backend": (C.c_int, [vp, C.c_int, C.c_double, C.c_int64]),
     "ccgpu_potential_rho_backend_used": (C.c_int, [vp, C.POINTER(C.c_int)]),
+    "ccgpu_potential_force_backend_used": (C.c_int, [vp, C.POINTER(C.c_int)]),
+    "ccgpu_potential_nufft_status": (C.c_int, [vp, C.POINTER(C.c_int), C.c_char_p, C.c_int64, dp, dp]),
+    "ccgpu_autotune_backend": (C.c_int, [C.c_int, C.c_int64, dp, C.c_int64, dp, dp, C.c_double, C.c_int,
+                                         C.POINTER(C.c_int), dp, dp]),
     "ccgpu_solver_set_rho_backend": (C.c_int, [vp, C.c_int, C.c_double]),
     "ccgpu_copy_probe": (C.c_int, [C.c_int64, C.c_int, dp]),
     "ccgpu_lbfgs_minimize": (C.c_int, [C.c_int, vp, vp, dp, dp, C.c_double, C.c_int, C.c_int,

@@ -32,7 +32,26 @@ def autotune(N, d, chi=None, k=None, basis=None, batch=64, repeats=5):
             "force_ppl": ppl.value, "ms_per_batched_ef": ms.value}
 
 
+def autotune_backend(N, d, chi=None, k=None, basis=None, eps=1e-12, repeats=5):
+    """Time one E+F of this shape on the direct sums and through the NUFFT and record the faster for the shape
+    class (systems with the "nufft" backend follow the record).  Returns (use_nufft, ms_direct, ms_nufft); ms_nufft
+    is None when the constraint set is not eligible."""
+    if basis is None:
+        basis = np.eye(d) * pow(N, 1.0 / d)
+    basis = np.ascontiguousarray(basis, dtype=np.float64)
+    if k is None:
+        k = GetKs(Configuration(basis), getK(chi, 1.0, d))
+    k = np.ascontiguousarray(k, dtype=np.float64)
+    v = np.ones(len(k))
+    use, md, mn = C.c_int(0), C.c_double(0), C.c_double(0)
+    check(lib().ccgpu_autotune_backend(d, N, basis.ctypes.data_as(dp), len(k), k.ctypes.data_as(dp),
+                                       v.ctypes.data_as(dp), float(eps), repeats, C.byref(use), C.byref(md),
+                                       C.byref(mn)))
+    return bool(use.value), md.value, (mn.value if mn.value >= 0 else None)
+
+
 def clear():
+    """Forget every recorded tile shape and backend choice."""
     check(lib().ccgpu_autotune_clear())
 
 

@@ -99,6 +99,11 @@ struct ccgpu_potential {
             }
         }
     }
+    // GetLastForceBackendUsed / GetLastFinufftStatus (CollectiveCoordinatePotential.h:268-289): 1 when the last
+    // all-force pass ran the NUFFT's Type 2 transform; 0 when the last rho pass ran the NUFFT, -1 (FinufftNotCalled)
+    // when it ran the direct sums
+    int last_force_backend = 0;
+    int last_nufft_status = -1;
     void EnsureRho()
     {
         EnsureBuilt();
@@ -106,6 +111,7 @@ struct ccgpu_potential {
             batch.Enqueue(true, false);
             rho_valid = true;
             forces_valid = false;
+            last_nufft_status = sys.nufft_on() ? 0 : -1;
         }
     }
     void EnsureForces()
@@ -114,7 +120,10 @@ struct ccgpu_potential {
         Require(!positions_stale, CCGPU_ERR_STATE,
                 "a Monte Carlo move was accepted without its particle index: bind the configuration again");
         if (!forces_valid) {
-            batch.Enqueue(!rho_valid || coef_stale, true);
+            const bool rho = !rho_valid || coef_stale;
+            batch.Enqueue(rho, true);
+            if (rho) last_nufft_status = sys.nufft_on() ? 0 : -1;
+            last_force_backend = sys.nufft_on() ? 1 : 0;
             coef_stale = false;
             rho_valid = true;
             forces.resize((size_t)sys.N * sys.d);
@@ -548,7 +557,8 @@ int ccgpu_potential_set_rho_backend(ccgpu_potential *p, int backend, double eps,
 {
     return Guard([&] {
         Require(p != nullptr, CCGPU_ERR_ARG, "null handle");
-        Require(backend == 0 || backend == 1, CCGPU_ERR_ARG, "backend must be 0 (direct) or 1 (NUFFT)");
+        Require(backend >= 0 && backend <= 2, CCGPU_ERR_ARG,
+                "backend must be 0 (direct), 1 (NUFFT behind the cost gate) or 2 (NUFFT whenever eligible)");
         Require(eps > 0.0 && std::isfinite(eps), CCGPU_ERR_ARG, "tolerance must be positive and finite");
         Require(max_modes >= 1, CCGPU_ERR_ARG, "max modes must be >= 1");
         p->sys.rho_backend = backend;
@@ -565,6 +575,62 @@ int ccgpu_potential_rho_backend_used(ccgpu_potential *p, int *backend)
         Require(p && backend, CCGPU_ERR_ARG, "null argument");
         p->EnsureBuilt();
         *backend = p->sys.nufft_on() ? 1 : 0;
+    });
+}
+
+int ccgpu_potential_force_backend_used(ccgpu_potential *p, int *backend)
+{
+    return Guard([&] {
+        Require(p && backend, CCGPU_ERR_ARG, "null argument");
+        *backend = p->last_force_backend;
+    });
+}
+
+int ccgpu_potential_nufft_status(ccgpu_potential *p, int *status, char *why, int64_t cap, double *pred_direct_us,
+                                 double *pred_nufft_us)
+{
+    return Guard([&] {
+        Require(p && status, CCGPU_ERR_ARG, "null argument");
+        Require(why == nullptr || cap >= 1, CCGPU_ERR_ARG, "reason buffer without room");
+        *status = p->last_nufft_status;
+        std::string reason;
+        double pd = 0, pn = 0;
+        if (p->sys.rho_backend == 0) reason = "direct backend selected";
+        else if (!p->bound) reason = "no configuration bound";
+        else {
+            p->EnsureBuilt();
+            const NufftState *U = p->sys.nu.get();
+            if (U) {
+                reason = U->active ? "" : U->why;
+                pd = U->pred_direct_us;
+                pn = U->pred_nufft_us;
+            }
+            if (!U || (!U->active && reason.empty())) reason = "not eligible for the NUFFT";
+        }
+        if (why) {
+            const size_t n = std::min(reason.size(), (size_t)(cap - 1));
+            std::memcpy(why, reason.data(), n);
+            why[n] = 0;
+        }
+        if (pred_direct_us) *pred_direct_us = pd;
+        if (pred_nufft_us) *pred_nufft_us = pn;
+    });
+}
+
+int ccgpu_autotune_backend(int dim, int64_t N, const double *basis, int64_t M, const double *k, const double *V,
+                           double eps, int repeats, int *use_nufft, double *ms_direct, double *ms_nufft)
+{
+    return Guard([&] {
+        Require(dim >= 1 && dim <= kMaxDim && basis && N >= 1 && M >= 1 && k && V && repeats >= 1, CCGPU_ERR_ARG,
+                "bad argument");
+        Require(eps > 0.0 && std::isfinite(eps), CCGPU_ERR_ARG, "tolerance must be positive and finite");
+        RequireDevice();
+        bool use = false;
+        double md = 0, mn = 0;
+        AutotuneBackend(dim, N, basis, M, k, V, eps, (int64_t)1 << 22, repeats, use, md, mn);
+        if (use_nufft) *use_nufft = use ? 1 : 0;
+        if (ms_direct) *ms_direct = md;
+        if (ms_nufft) *ms_nufft = std::isfinite(mn) ? mn : -1.0;
     });
 }
 
@@ -1468,7 +1534,8 @@ int ccgpu_solver_set_rho_backend(ccgpu_solver *s, int backend, double eps)
 {
     return Guard([&] {
         Require(s != nullptr, CCGPU_ERR_ARG, "null handle");
-        Require(backend == 0 || backend == 1, CCGPU_ERR_ARG, "backend must be 0 (direct) or 1 (NUFFT)");
+        Require(backend >= 0 && backend <= 2, CCGPU_ERR_ARG,
+                "backend must be 0 (direct), 1 (NUFFT behind the cost gate) or 2 (NUFFT whenever eligible)");
         Require(eps > 0.0 && std::isfinite(eps), CCGPU_ERR_ARG, "tolerance must be positive and finite");
         s->solver.SetRhoBackend(backend, eps);
     });

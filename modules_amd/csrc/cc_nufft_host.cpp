@@ -2,6 +2,7 @@
 // deconvolution tables and the hipFFT plans of one system (method: cc_nufft.h).
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <unordered_set>
 #include <vector>
 
@@ -66,18 +67,34 @@ void CheckFft(hipfftResult r, const char *what)
 
 }  // namespace
 
+NufftCost PredictNufftCost(int d, int64_t N, int64_t M, int w, int64_t npts, int64_t tiles)
+{
+    const int c = std::max(1, std::min(3, d)) - 1;
+    const double points = (double)N * std::pow((double)w, d);
+    NufftCost r;
+    r.direct_us = kDirectCallUs[c] + kDirectPairUs[c] * (double)N * (double)M;
+    r.nufft_us = kNufftCallUs[c] + kNufftPointUs[c] * points + kNufftGridUs[c] * (double)npts +
+                 kNufftTileUs[c] * points / (double)std::max<int64_t>(tiles, 1);
+    return r;
+}
+
 void PrepareNufft(System &S, hipStream_t stream)
 {
     if (!S.nu) S.nu.reset(new NufftState());
     NufftState &U = *S.nu;
     const ConstraintLayout *L = S.layout.get();
-    if (U.layout.get() == L && U.eps == S.nufft_eps && U.N == S.N && U.max_modes == S.nufft_max_modes && (U.active || !U.why.empty()))
+    if (U.layout.get() == L && U.eps == S.nufft_eps && U.N == S.N && U.max_modes == S.nufft_max_modes &&
+        U.backend == S.rho_backend && (U.active || !U.why.empty()))
         return;
     U.layout = S.layout;
     U.eps = S.nufft_eps;
     U.N = S.N;
     U.max_modes = S.nufft_max_modes;
+    U.backend = S.rho_backend;
     U.active = false;
+    U.gated = false;
+    U.measured = false;
+    U.pred_direct_us = U.pred_nufft_us = 0;
     U.why.clear();
     U.DropPlans();
     const int d = S.d;
@@ -154,6 +171,28 @@ void PrepareNufft(System &S, hipStream_t stream)
     if (U.psize > kNufftMaxPad) {
         U.why = "kernel too wide for the padded tiles";
         return;
+    }
+    // cost gate of backend 1: the autotuner's timed choice for this shape class if there is one, else the cost
+    // model with a required predicted speed-up; it depends only on the shape, the settings and that table, so a
+    // system's bits do not depend on timings taken during a run
+    {
+        const NufftCost c = PredictNufftCost(d, S.N, S.M, w, U.npts, U.nbin);
+        U.pred_direct_us = c.direct_us;
+        U.pred_nufft_us = c.nufft_us;
+    }
+    if (S.rho_backend == 1) {
+        bool use = false;
+        U.measured = LookupBackend(d, S.N, S.M, use);
+        if (!U.measured) use = U.pred_direct_us >= kNufftMargin * U.pred_nufft_us;
+        if (!use) {
+            U.gated = true;
+            char buf[160];
+            std::snprintf(buf, sizeof buf, "cost gate: direct sums %s (%.0f us against %.0f us predicted)",
+                          U.measured ? "timed faster by the autotuner" : "predicted faster", U.pred_direct_us,
+                          U.pred_nufft_us);
+            U.why = buf;
+            return;
+        }
     }
     // deconvolution factors p_c(m) = (w/2) int_{-1}^{1} phi(z) cos(pi w m z / nf_c) dz
     std::vector<double> gx, gw;

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -157,10 +158,30 @@ bool LookupTiles(int d, int64_t N, int64_t M, Tiles &t)
     return true;
 }
 
+namespace {
+std::map<std::tuple<int, int, int>, bool> g_backend;  // guarded by g_tiles_mu
+}
+
+void RecordBackend(int d, int64_t N, int64_t M, bool nufft)
+{
+    std::lock_guard<std::mutex> lk(g_tiles_mu);
+    g_backend[TileKey(d, N, M)] = nufft;
+}
+
+bool LookupBackend(int d, int64_t N, int64_t M, bool &nufft)
+{
+    std::lock_guard<std::mutex> lk(g_tiles_mu);
+    auto it = g_backend.find(TileKey(d, N, M));
+    if (it == g_backend.end()) return false;
+    nufft = it->second;
+    return true;
+}
+
 void ClearTiles()
 {
     std::lock_guard<std::mutex> lk(g_tiles_mu);
     g_tiles.clear();
+    g_backend.clear();
 }
 
 Tiles DefaultTiles(int d, int64_t N, int64_t M)
@@ -454,8 +475,8 @@ void System::Prepare(hipStream_t stream)
             weights_dirty = false;
         }
     }
-    if (rho_backend == 1 && lattice && !view() && !mixture && weights.empty()) PrepareNufft(*this, stream);
-    else if (rho_backend == 1 && !weights.empty()) {
+    if (rho_backend != 0 && lattice && !view() && !mixture && weights.empty()) PrepareNufft(*this, stream);
+    else if (rho_backend != 0 && !weights.empty()) {
         if (nu) nu->active = false, nu->layout.reset();
         if (!nu) nu.reset(new NufftState());
         nu->why = "weighted rho";
@@ -1002,6 +1023,52 @@ double Autotune(int d, int64_t N, const double *basis, int64_t M, const double *
     (void)hipEventDestroy(e1);
     RecordTiles(d, N, M, best);
     return best_ms;
+}
+
+void AutotuneBackend(int d, int64_t N, const double *basis, int64_t M, const double *k, const double *v, double eps,
+                     int64_t max_modes, int repeats, bool &use_nufft, double &ms_direct, double &ms_nufft)
+{
+    if (repeats < 1 || N < 1) throw Error(kErrArg, "autotune: bad repeats or N");
+    System S(d);
+    Batch b;
+    S.SetCell(basis);
+    S.SetConstraints(M, k, v);
+    S.N = N;
+    S.nufft_eps = eps;
+    S.nufft_max_modes = max_modes;
+    b.systems.push_back(&S);
+    std::vector<double> x((size_t)(N * d));
+    RandomRelativeConfiguration(7919u, N, d, x.data());
+    hipEvent_t e0, e1;
+    CC_HIP(hipEventCreate(&e0));
+    CC_HIP(hipEventCreate(&e1));
+    auto time = [&](int backend) {
+        S.rho_backend = backend;
+        b.dirty = true;
+        b.Build();
+        if (backend == 2 && !S.nufft_on()) return std::numeric_limits<double>::infinity();
+        CC_HIP(hipMemcpyAsync(S.x.get(), x.data(), x.size() * sizeof(double), hipMemcpyHostToDevice, b.stream));
+        b.Enqueue(true, true);  // warm-up
+        CC_HIP(hipEventRecord(e0, b.stream));
+        for (int r = 0; r < repeats; r++) b.Enqueue(true, true);
+        CC_HIP(hipEventRecord(e1, b.stream));
+        CC_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        CC_HIP(hipEventElapsedTime(&ms, e0, e1));
+        return (double)ms / repeats;
+    };
+    try {
+        ms_direct = time(0);
+        ms_nufft = time(2);
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    use_nufft = ms_nufft < ms_direct;
+    RecordBackend(d, N, M, use_nufft);
 }
 
 }  // namespace ccgpu

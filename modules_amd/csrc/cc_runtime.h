@@ -128,12 +128,20 @@ int64_t EstimateSystemBytes(int d, const double *a, int64_t N, int64_t M, const 
 // consults it.  Filled by Autotune; a system's tiles depend only on its own shape and the table.
 void RecordTiles(int d, int64_t N, int64_t M, const Tiles &t);
 bool LookupTiles(int d, int64_t N, int64_t M, Tiles &t);
+// The same keys for the measured choice between the NUFFT and the direct sums (AutotuneBackend); ClearTiles
+// forgets both tables.
+void RecordBackend(int d, int64_t N, int64_t M, bool nufft);
+bool LookupBackend(int d, int64_t N, int64_t M, bool &nufft);
 void ClearTiles();
 
 // Time candidate tiles on `batch` copies of a random configuration of this shape (E+F per evaluation,
 // hipEvents on the batch stream) and record the fastest.  Returns its milliseconds per batched E+F.
 double Autotune(int d, int64_t N, const double *basis, int64_t M, const double *k, const double *v, int batch,
                 int repeats, Tiles &best);
+// Time one E+F of a random configuration of this shape on the direct sums and through the NUFFT (hipEvents, mean of
+// `repeats`), record the faster for the shape class and return both (ms_nufft infinite when not eligible).
+void AutotuneBackend(int d, int64_t N, const double *basis, int64_t M, const double *k, const double *v, double eps,
+                     int64_t max_modes, int repeats, bool &use_nufft, double &ms_direct, double &ms_nufft);
 
 // Device layout of one constraint set for one cell and segment length: lattice segments (head, length,
 // slot -> constraint, slot weights) or Cartesian wavevectors.  Systems with the same cell, constraints,
@@ -163,8 +171,12 @@ std::shared_ptr<const ConstraintLayout> GetLayout(int d, const double *a, int64_
 // NUFFT plan of one system (cc_nufft.h): grid shape, kernel, deconvolution tables, device buffers and the two
 // hipFFT plans (Type 1 exp(+i) on one grid; Type 2 exp(-i) on one grid, two in 3D).
 struct NufftState {
-    bool active = false;  // eligible: the system evaluates through the NUFFT
-    std::string why;      // reason when not eligible (the direct path is used)
+    bool active = false;  // eligible (and, for backend 1, chosen by the cost gate): the system evaluates by the NUFFT
+    std::string why;      // reason when the direct path is used
+    int backend = 0;      // the system's rho_backend this state was decided for
+    bool gated = false;   // eligible, but the cost gate chose the direct sums
+    bool measured = false;  // the gate followed a timed choice of the autotuner, not the cost model
+    double pred_direct_us = 0, pred_nufft_us = 0;  // the cost model's predictions for one E+F
     std::shared_ptr<const ConstraintLayout> layout;  // plan key (held, so its address is not reused meanwhile)
     double eps = 0;
     int64_t N = -1, max_modes = 0;
@@ -232,8 +244,9 @@ public:
     DevArray<float> coef32;
     int precision = 0;  // 0: FP64; 1: mixed (lattice phases and force terms in FP32)
     // rho backend (SetRhoBackend / SetFinufftTolerance / SetFinufftMaxModes,
-    // potential/CollectiveCoordinatePotential.h:160-212): 0 direct sums, 1 NUFFT when the constraint set is
+    // potential/CollectiveCoordinatePotential.h:160-212): 0 direct sums; 1 NUFFT when the constraint set is
     // eligible (lattice, d <= 3, mode box within the cap, no constraint sharing a mode with another's conjugate)
+    // and the cost gate predicts a win (NufftGate); 2 NUFFT whenever eligible
     int rho_backend = 0;
     double nufft_eps = 1e-12;
     int64_t nufft_max_modes = (int64_t)1 << 22;
@@ -374,5 +387,25 @@ void launch_nufft_coef(hipStream_t, const SysDesc *, const WorkItem *grid, int n
 void launch_nufft_interp(hipStream_t, const SysDesc *, const WorkItem *part, int npart);
 // (re)build the NUFFT plan of a lattice system for its layout, tolerance and N; sets nu->active (cc_nufft_host.cpp)
 void PrepareNufft(System &s, hipStream_t stream);
+
+// Cost model of one E+F of one system on MI355X, in microseconds.  Direct sums: a fixed part plus a part per
+// particle-constraint pair.  NUFFT: a fixed part (its launches and hipFFT calls), a part per kernel point (N w^d
+// spread and as many interpolated), a part per fine-grid point (the FFTs and the grid passes) and a part per kernel
+// point of one output tile (N w^d / tiles: each tile spreads its particles in order, the chain that bounds sparse
+// grids).  Least-squares fits (relative error) to hipEvent timings of both paths, one E+F of one system, over 47
+// shapes: 1D N 1e3-4e5, 2D N 300-1e5, 3D N 300-5e4, chi 0.1-0.45, eps 1e-12 (largest error of a fitted time 15%).
+// They only choose the faster path, never a result's accuracy.  Backend 1 takes the NUFFT when
+// direct >= kNufftMargin x nufft.
+constexpr double kDirectCallUs[3] = {57.0, 51.6, 54.2};
+constexpr double kDirectPairUs[3] = {5.13e-7, 5.81e-7, 7.35e-7};
+constexpr double kNufftCallUs[3] = {78.6, 115.0, 952.9};
+constexpr double kNufftPointUs[3] = {1.02e-4, 1.02e-5, 7.34e-6};
+constexpr double kNufftGridUs[3] = {1.08e-4, 2.30e-4, 7.91e-4};
+constexpr double kNufftTileUs[3] = {5.0e-2, 4.95e-3, 0.0};
+constexpr double kNufftMargin = 1.3;
+struct NufftCost {
+    double direct_us = 0, nufft_us = 0;
+};
+NufftCost PredictNufftCost(int d, int64_t N, int64_t M, int w, int64_t npts, int64_t tiles);
 
 }  // namespace ccgpu

@@ -251,9 +251,12 @@ class BatchSolver:
         check(lib().ccgpu_solver_set_precision(self._h, _precision_code(precision), float(polish)))
 
     def set_rho_backend(self, backend="direct", eps=1e-12):
-        """Collective-density backend of single-species jobs submitted from now on: "direct" or "nufft"
-        (eligible lattice constraint sets; tolerance eps)."""
-        check(lib().ccgpu_solver_set_rho_backend(self._h, 1 if backend == "nufft" else 0, float(eps)))
+        """Collective-density backend of single-species jobs submitted from now on: "direct", "nufft" (eligible
+        lattice constraint sets the cost gate picks) or "nufft_always" (every eligible set); tolerance eps."""
+        codes = {"direct": 0, "nufft": 1, "nufft_always": 2}
+        if backend not in codes:
+            raise ValueError('backend must be "direct", "nufft" or "nufft_always"')
+        check(lib().ccgpu_solver_set_rho_backend(self._h, codes[backend], float(eps)))
 
     def __del__(self):
         h = getattr(self, "_h", None)

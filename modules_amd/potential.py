@@ -378,12 +378,16 @@ class ShiftedCCPotential:
         self.RhoValid = False
 
     def SetRhoBackend(self, backend):
-        """SetRhoBackend (CollectiveCoordinatePotential.h:162-169): "direct" (default) or "nufft" -- the Type 1 /
+        """SetRhoBackend (CollectiveCoordinatePotential.h:162-169): "direct" (default); "nufft" -- the Type 1 /
         Type 2 NUFFT on a fine grid (the reference's FinufftType1 with Type 2 forces) when the constraint set is
-        eligible, the direct sums otherwise (GetLastRhoBackendUsed tells which)."""
-        codes = {"direct": 0, "nufft": 1, "finufft": 1, "FinufftType1": 1, 0: 0, 1: 1}
+        eligible and the cost gate (the autotuner's timed choice for the shape class, else the cost model; the
+        reference's FinufftForceGate, :318-420) predicts a win, the direct sums otherwise; "nufft_always" -- the
+        NUFFT whenever eligible.  GetLastRhoBackendUsed / GetLastFinufftStatus / NufftGateReason tell which and
+        why."""
+        codes = {"direct": 0, "nufft": 1, "finufft": 1, "FinufftType1": 1, "nufft_always": 2, 0: 0, 1: 1, 2: 2}
         if backend not in codes:
-            raise ValueError('ShiftedCCPotential::SetRhoBackend: backend must be "direct" or "nufft"')
+            raise ValueError('ShiftedCCPotential::SetRhoBackend: backend must be "direct", "nufft" or '
+                             '"nufft_always"')
         self._rho_backend = codes[backend]
         self._push_backend()
 
@@ -413,6 +417,31 @@ class ShiftedCCPotential:
         v = C.c_int(0)
         check(lib().ccgpu_potential_rho_backend_used(self._h, C.byref(v)))
         return "nufft" if v.value else "direct"
+
+    def GetLastForceBackendUsed(self):
+        """GetLastForceBackendUsed (CollectiveCoordinatePotential.h:275-285): "nufft" if the last all-force pass
+        used the Type 2 transform, else "direct" (also before any)."""
+        v = C.c_int(0)
+        check(lib().ccgpu_potential_force_backend_used(self._h, C.byref(v)))
+        return "nufft" if v.value else "direct"
+
+    def _nufft_status(self):
+        st, pd, pn = C.c_int(0), C.c_double(0), C.c_double(0)
+        buf = C.create_string_buffer(256)
+        check(lib().ccgpu_potential_nufft_status(self._h, C.byref(st), buf, 256, C.byref(pd), C.byref(pn)))
+        return st.value, buf.value.decode(), pd.value, pn.value
+
+    def GetLastFinufftStatus(self):
+        """GetLastFinufftStatus (CollectiveCoordinatePotential.h:268-273): 0 when the last rho pass ran the NUFFT,
+        -1 (FinufftNotCalled) otherwise."""
+        return self._nufft_status()[0]
+
+    def NufftGateReason(self):
+        """Why the bound system takes the direct sums ("" when it takes the NUFFT) and the cost model's predicted
+        microseconds of one E+F on each path: (reason, direct_us, nufft_us)."""
+        self._require()
+        _, why, pd, pn = self._nufft_status()
+        return why, pd, pn
 
     def UsesLatticePath(self):
         v = C.c_int(0)

@@ -22,7 +22,7 @@ def scales(k, v, N, vol):
     return max(N, 1.0), sumV * N * N / vol, 4.0 * sumV * N * kmax / vol
 
 
-def potential(d, k, v, backend="nufft", eps=1e-12):
+def potential(d, k, v, backend="nufft_always", eps=1e-12):
     pot = ShiftedCCPotential(d)
     for kk, vv in zip(k, v):
         pot.AddConstraint(kk, [vv])
@@ -141,7 +141,7 @@ def test_relax_through_the_nufft_reaches_ground_state():
     assert E < 1e-18
     Eo, _ = po.total_energy_forces(cfg.relative, cfg.basis, k, np.ones(len(k)))
     assert Eo < 1e-17
-    solver = BatchSolver(max_slots=4, check_interval=8, rho_backend="nufft")
+    solver = BatchSolver(max_slots=4, check_interval=8, rho_backend="nufft_always")
     jobs = [solver.submit(RandomConfiguration(s, 2000, 2), k, np.ones(len(k)), Emin=1e-18) for s in (1, 2)]
     solver.set_rho_backend("direct")  # a direct-path job in the same launches
     small = RandomConfiguration(3, 500, 2)
@@ -152,3 +152,84 @@ def test_relax_through_the_nufft_reaches_ground_state():
         r = solver.result(j)
         print(f"nufft solver job: E={r['energy']:.3g} evaluations={r['evaluations']}", flush=True)
         assert r["energy"] < 1e-18
+
+
+def test_cost_gate_keeps_small_systems_on_the_direct_path_bitwise():
+    # FinufftForceGate (CollectiveCoordinatePotential.h:318-420) declines the transforms where the direct sums win;
+    # here the gate covers both transforms, so a declined system carries the direct path's bits
+    from modules_amd import autotune
+    autotune.clear()
+    for N, d, chi, seed in [(500, 2, 0.3, 11), (2000, 3, 0.45, 12), (1000, 1, 0.4, 13)]:
+        cfg = RandomConfiguration(seed, N, d)
+        k = GetKs(cfg, getK(chi, 1.0, d))
+        v = np.ones(len(k))
+        p = potential(d, k, v, backend="nufft")
+        p.SetConfiguration(cfg)
+        assert p.GetLastFinufftStatus() == -1 and p.GetLastForceBackendUsed() == "direct"  # nothing ran yet
+        why, pd, pn = p.NufftGateReason()
+        assert p.GetLastRhoBackendUsed() == "direct" and why.startswith("cost gate") and 0 < pd < pn, (why, pd, pn)
+        E, F = p.EnergyAndAllForce()
+        q = potential(d, k, v, backend="direct")
+        q.SetConfiguration(cfg)
+        Eq, Fq = q.EnergyAndAllForce()
+        assert E == Eq and np.array_equal(F, Fq)
+        assert p.GetLastFinufftStatus() == -1 and p.GetLastForceBackendUsed() == "direct"
+
+
+def test_cost_gate_takes_the_nufft_for_a_large_2d_system():
+    from modules_amd import autotune
+    autotune.clear()
+    cfg = RandomConfiguration(21, 100000, 2)
+    k = GetKs(cfg, getK(0.4, 1.0, 2))
+    v = np.ones(len(k))
+    p = potential(2, k, v, backend="nufft")
+    p.SetConfiguration(cfg)
+    why, pd, pn = p.NufftGateReason()
+    assert p.GetLastRhoBackendUsed() == "nufft" and why == "" and pd >= 1.3 * pn, (why, pd, pn)
+    E, F = p.EnergyAndAllForce()
+    assert p.GetLastFinufftStatus() == 0 and p.GetLastForceBackendUsed() == "nufft"
+    # within the reference's NUFFT-vs-direct bar of the direct GPU path (itself pinned to the oracle)
+    q = potential(2, k, v, backend="direct")
+    q.SetConfiguration(cfg)
+    Eq, Fq = q.EnergyAndAllForce()
+    _, se, sf = scales(k, v, cfg.NumParticle(), po.volume(cfg.basis))
+    assert abs(E - Eq) <= RTOL * se + ATOL and np.max(np.abs(F - Fq)) <= RTOL * sf + ATOL
+    # back to the direct sums: the introspection follows
+    p.SetRhoBackend("direct")
+    p.EnergyAndAllForce()
+    assert p.GetLastFinufftStatus() == -1 and p.GetLastForceBackendUsed() == "direct"
+
+
+def test_autotuned_backend_choice_is_followed():
+    from modules_amd import autotune
+    autotune.clear()
+    cfg = RandomConfiguration(31, 3000, 2)
+    k = GetKs(cfg, getK(0.45, 1.0, 2))
+    use, md, mn = autotune.autotune_backend(3000, 2, k=k, basis=cfg.basis, repeats=3)
+    print(f"autotune backend 2D N=3000: direct {md:.3f} ms, nufft {mn:.3f} ms -> {'nufft' if use else 'direct'}",
+          flush=True)
+    assert md > 0 and mn > 0 and use == (mn < md)
+    p = potential(2, k, np.ones(len(k)), backend="nufft")
+    p.SetConfiguration(cfg)
+    assert p.GetLastRhoBackendUsed() == ("nufft" if use else "direct")
+    if not use:
+        assert "autotuner" in p.NufftGateReason()[0]
+    # a constraint set the NUFFT cannot take times the direct path only
+    kk = np.vstack([k, -k[:3]])
+    use2, md2, mn2 = autotune.autotune_backend(3000, 2, k=kk, basis=cfg.basis, repeats=2)
+    assert not use2 and md2 > 0 and mn2 is None
+    autotune.clear()
+
+
+def test_gated_solver_job_matches_the_direct_job():
+    from modules_amd.optimize import BatchSolver
+    cfg = RandomConfiguration(41, 600, 2)
+    k = GetKs(cfg, getK(0.35, 1.0, 2))
+    out = []
+    for backend in ("nufft", "direct"):
+        solver = BatchSolver(max_slots=2, rho_backend=backend)
+        j = solver.submit(cfg, k, np.ones(len(k)), Emin=1e-18)
+        solver.run(timeout=300)
+        out.append(solver.result(j))
+    assert out[0]["energy"] == out[1]["energy"] and out[0]["evaluations"] == out[1]["evaluations"]
+    assert np.array_equal(out[0]["relative"], out[1]["relative"])

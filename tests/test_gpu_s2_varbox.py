@@ -76,7 +76,7 @@ def test_s2_nufft_path_matches_oracle():
     cfg = RandomConfiguration(6, 4000, 2)
     k = GetKs(cfg, getK(0.4, 1.0, 2))
     rng = np.random.default_rng(6)
-    pot, _, _ = s2_compare(cfg, k, np.ones(len(k)), rng.random(len(k)), backend="nufft", rtol=1e-9)
+    pot, _, _ = s2_compare(cfg, k, np.ones(len(k)), rng.random(len(k)), backend="nufft_always", rtol=1e-9)
     assert pot.GetLastRhoBackendUsed() == "nufft"
 
 
