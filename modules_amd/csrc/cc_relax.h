@@ -14,14 +14,17 @@
 #include "cc_runtime.h"
 #include "plis_core.h"
 
+#include <algorithm>
+
 namespace ccgpu {
 
 constexpr int kRelaxBlock = 256;
-// Jobs of at least kRelaxBigN optimizer variables step with kRelaxBlockBig threads: the step kernel's vector
-// loops are bound by the loads one workgroup keeps in flight, and 512 threads (two waves per SIMD at the kernel's
-// 256 VGPRs) double them.  The choice reads only the job's own size, so its bits do not depend on the mix.
-constexpr int kRelaxBlockBig = 512;
+// Jobs of at least kRelaxBigN optimizer variables step on several blocks (k_relax_step_grid), one per
+// kRelaxGridChunk elements (at most 64): one workgroup's vector loops are bound by the loads it keeps in flight and
+// by one CU's share of L2 bandwidth.  The choice and the block count read only the job's own size, so its bits do
+// not depend on the mix.
 constexpr int64_t kRelaxBigN = 32768;
+constexpr int kRelaxGridChunk = 4096;
 constexpr int kRelaxChunk = 1024;  // vector elements per block of the wide L-BFGS passes
 constexpr double kRelaxRescale[3] = {10000.0, 1.0, 0.01};  // StructureOptimization.cpp:1055-1063
 constexpr double kMaxEnergy = 1e10;                          // core/etc.h:8
@@ -51,13 +54,21 @@ struct RelaxDesc {
     double *part1;   // pass-1 partial dots: nchunks x kPlisMaxHistory x 5
     double *part2;   // pass-2 partial |S|^2, g.S: nchunks x 2
     double *lcoef;   // alpha, (alpha - beta), gamma of the pending direction
+    double *gred;    // large jobs: block totals of the grid step's reductions (2 x 5 x 64)
+    unsigned *bar;   // large jobs: the grid step's barrier words (arrivals, generation, timeout flag)
     int n, mf;
     int sys;         // index of the job's SysDesc in the batch
     int nchunks;     // blocks of kRelaxChunk elements in the wide passes
 };
 
+// rd: n descriptors, the n_big large jobs last; big_items: the nbig_items blocks of the large jobs' grid step
 void launch_relax_step(hipStream_t stream, SysDesc *sys, const RelaxDesc *rd, int n, int n_big,
-                       const WorkItem *wide_items, int nwide);
+                       const WorkItem *wide_items, int nwide, const WorkItem *big_items, int nbig_items);
+// blocks of a large job's grid step
+inline int RelaxGridBlocks(int64_t n)
+{
+    return (int)std::min<int64_t>(64, std::max<int64_t>(1, (n + kRelaxGridChunk - 1) / kRelaxGridChunk));
+}
 
 // One minimisation request and, while resident, its device buffers.
 struct Job {
@@ -101,7 +112,8 @@ struct Job {
     // device side (resident only)
     int lane = 0;  // solver lane (stream) the job is resident on
     std::unique_ptr<System> sys;
-    DevArray<double> sdir, xo, go, uv, gram, part1, part2, lcoef;
+    DevArray<double> sdir, xo, go, uv, gram, part1, part2, lcoef, gred;
+    DevArray<unsigned> bar;
     DevArray<RelaxState> st;
     RelaxState host_state{};
 };
@@ -119,11 +131,11 @@ struct SolverLane {
     hipEvent_t desc_ev = nullptr;  // recorded after the descriptor read of a harvest
     bool desc_pending = false;
     size_t marks_at_read = 0;      // timing marks recorded before that read (complete once it is)
-    DevArray<RelaxDesc> d_rdesc;  // jobs stepped by kRelaxBlock threads, then the n_big large ones
+    DevArray<RelaxDesc> d_rdesc;  // jobs stepped by one block each, then the n_big large ones
     std::vector<int> rd_of_slot;  // index in d_rdesc of each slot's job
     int n_big = 0;
-    DevArray<WorkItem> d_wide;
-    int nwide = 0;
+    DevArray<WorkItem> d_wide, d_big;  // blocks of the wide L-BFGS passes; blocks of the large jobs' grid step
+    int nwide = 0, nbig_items = 0;
     hipGraphExec_t gexec = nullptr;
     int graph_burst = 0;
     bool dirty = true;

@@ -9,11 +9,51 @@
 namespace ccgpu {
 
 // ------------------------------------------------------------------------------------------------
-// Block-cooperative vector operations for PlisAdvance.  Thread t owns elements t, t+B, t+2B, ... of every
-// vector, so element-wise operations need no barrier; reductions are butterfly sums within each
-// wavefront and a fixed-order sum over wavefronts, so every thread gets the same bits in every run.
+// Block-cooperative vector operations for PlisAdvance.  The block owns elements [lo, hi) of every vector (all
+// of them, or with kGrid a contiguous range of a job stepped by several blocks) and thread t the elements
+// lo + t, lo + t + B, ...; element-wise operations need no barrier.  Reductions are butterfly sums within each
+// wavefront and a fixed-order sum over wavefronts; with kGrid the block totals then meet in global memory
+// behind a barrier of the job's blocks and every block sums them with the same butterfly, so every thread of
+// every block gets the same bits in every run and all blocks take the same PLIS decisions.
 // ------------------------------------------------------------------------------------------------
-template <int BS>
+constexpr int kGridMaxBlocks = 64;       // blocks of one job in the grid step (one wavefront combines them)
+constexpr int kGridSpinLimit = 1 << 22;  // barrier polls (each with a short sleep) before a block gives up
+
+// Barrier of the nb blocks of one job: bar[0] counts arrivals, bar[1] is the generation, bar[2] a sticky
+// timeout flag.  Release and acquire at agent scope make each block's global writes before the barrier
+// visible to the others after it (across the XCDs' L2s).  Every wait is bounded: after kGridSpinLimit polls the
+// block records the timeout and goes on, so no launch can hang; the step then fails the job.
+__device__ __forceinline__ bool job_barrier(unsigned *bar, int nb)
+{
+    __shared__ int ok_lds;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        __threadfence();
+        const unsigned gen = __hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned arrived = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == (unsigned)nb - 1) {
+            __hip_atomic_store(bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(bar + 1, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            int polls = 0;
+            while (__hip_atomic_load(bar + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                if (++polls > kGridSpinLimit) {
+                    ok = 0;
+                    __hip_atomic_store(bar + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        __threadfence();
+        ok_lds = ok;
+    }
+    __syncthreads();
+    return ok_lds != 0;
+}
+
+template <int BS, bool kGrid = false>
 struct BlockPlisOps {
     int n, mf;
     double *x, *g, *s, *xo, *go;
@@ -26,6 +66,14 @@ struct BlockPlisOps {
     bool pair_valid = false;
     int ps = -1, py = -1;
     double l1_s = 0, l1_x = 0, max_g = 0, gg = 0, sy = 0;
+    int lo = 0, hi = 0;  // this block's elements
+    // kGrid: block totals (2 alternating slots of 5 x kGridMaxBlocks), the job's barrier, the job's block count and
+    // this block's index; barriers passed and whether one timed out
+    double *gred = nullptr;
+    unsigned *bar = nullptr;
+    int nb = 1, b = 0;
+    mutable int gslot = 0, barriers = 0;
+    mutable bool timed_out = false;
 
     __device__ double *vec(int id) const
     {
@@ -39,6 +87,26 @@ struct BlockPlisOps {
     // alternating LDS slots let one barrier per reduction suffice: a slot is rewritten only two reductions
     // later, after every thread has passed the barrier of the reduction in between.
     mutable int red_slot;
+    // kGrid: the job's total of `cnt` block values (maxima in the positions of `max_mask`; every maximum here is of
+    // non-negative values, so 0 is its identity).  Two slots alternate: a slot is rewritten two reductions later,
+    // after every block has passed the barrier of the reduction in between and so has read it.
+    template <int CNT>
+    __device__ void grid_combine(double (&v)[CNT], unsigned max_mask) const
+    {
+        double *slot = gred + gslot * 5 * kGridMaxBlocks;
+        gslot ^= 1;
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int q = 0; q < CNT; q++) slot[q * kGridMaxBlocks + b] = v[q];
+        if (!job_barrier(bar, nb)) timed_out = true;
+        barriers++;
+        const int lane = threadIdx.x & 63;
+        double t[CNT];
+#pragma unroll
+        for (int q = 0; q < CNT; q++) t[q] = lane < nb ? slot[q * kGridMaxBlocks + lane] : 0.0;
+#pragma unroll
+        for (int q = 0; q < CNT; q++) v[q] = (max_mask >> q) & 1 ? wave_max_dpp(t[q]) : wave_sum_dpp(t[q]);
+    }
     __device__ double block_reduce(double v, bool is_max) const
     {
         v = is_max ? wave_max_dpp(v) : wave_sum_dpp(v);
@@ -49,6 +117,11 @@ struct BlockPlisOps {
         __syncthreads();
         double r = slot[0];
         for (int i = 1; i < BS / 64; i++) r = is_max ? fmax(r, slot[i]) : r + slot[i];
+        if constexpr (kGrid) {
+            double t[1] = {r};
+            grid_combine(t, is_max ? 1u : 0u);
+            r = t[0];
+        }
         return r;
     }
     // five block-wide reductions at once (sums, the maximum in position 2): one barrier
@@ -70,6 +143,30 @@ struct BlockPlisOps {
             for (int i = 1; i < nw; i++) r = q == 2 ? fmax(r, slot[q * nw + i]) : r + slot[q * nw + i];
             v[q] = r;
         }
+        if constexpr (kGrid) grid_combine(v, 1u << 2);
+    }
+    // true in every thread (of every block of the job) if any thread has v
+    __device__ bool any(bool v) const
+    {
+        const bool r = __syncthreads_or(v);
+        if constexpr (kGrid) {
+            double t[1] = {r ? 1.0 : 0.0};
+            grid_combine(t, 1u);
+            return t[0] != 0.0;
+        }
+        return r;
+    }
+    // kGrid: the step's global scalar writes follow a barrier of the job's blocks, so that every block has read
+    // the job's state before block 0 rewrites it
+    __device__ void settle() const
+    {
+        if constexpr (kGrid) {
+            if (barriers == 0) {
+                if (!job_barrier(bar, nb)) timed_out = true;
+                barriers++;
+            }
+            if (__hip_atomic_load(bar + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) timed_out = true;
+        }
     }
     __device__ void touch(int id)
     {
@@ -80,29 +177,29 @@ struct BlockPlisOps {
     // each thread keeps several independent HBM loads in flight (one workgroup per job leaves few waves per CU);
     // with 256 threads the remainder is one predicated batch, so no element waits on a round trip of its own.
     // Sums run in element order either way.
-    static constexpr int kU = BS <= 256 ? 16 : 4;
+    static constexpr int kU = kGrid ? 8 : BS <= 256 ? 16 : 4;  // the grid step's blocks own few elements each
     template <class Load, class Use>
     __device__ __forceinline__ void batches(Load load, Use use) const
     {
-        int e = threadIdx.x;
-        for (; e + (kU - 1) * BS < n; e += kU * BS) {
+        int e = lo + threadIdx.x;
+        for (; e + (kU - 1) * BS < hi; e += kU * BS) {
 #pragma unroll
             for (int u = 0; u < kU; u++) load(u, e + u * BS);
 #pragma unroll
             for (int u = 0; u < kU; u++) use(u, e + u * BS);
         }
         if constexpr (BS <= 256) {
-            if (e < n) {
+            if (e < hi) {
 #pragma unroll
                 for (int u = 0; u < kU; u++)
-                    if (e + u * BS < n) load(u, e + u * BS);
+                    if (e + u * BS < hi) load(u, e + u * BS);
 #pragma unroll
                 for (int u = 0; u < kU; u++)
-                    if (e + u * BS < n) use(u, e + u * BS);
+                    if (e + u * BS < hi) use(u, e + u * BS);
             }
         } else {
             // the 512-thread variant has no registers to spare for a predicated batch: one element at a time
-            for (; e < n; e += BS) {
+            for (; e < hi; e += BS) {
                 load(0, e);
                 use(0, e);
             }
@@ -187,13 +284,13 @@ struct BlockPlisOps {
         double *psv = vec(sv), *pyv = vec(yv);
         constexpr int kP = kU < 8 ? kU : 8;
         double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // |s|_1, |x|_1, max |g|, g.g, s.y
-        int e = threadIdx.x;
-        for (; e < n; e += kP * BS) {
+        int e = lo + threadIdx.x;
+        for (; e < hi; e += kP * BS) {
             double vx[kP], vs[kP], vg[kP], vy[kP];
 #pragma unroll
             for (int u = 0; u < kP; u++) {
                 const int i = e + u * BS;
-                const bool in = i < n;
+                const bool in = i < hi;
                 vx[u] = in ? x[i] : 0.0;
                 vs[u] = in ? psv[i] : 0.0;
                 vg[u] = in ? g[i] : 0.0;
@@ -202,7 +299,7 @@ struct BlockPlisOps {
 #pragma unroll
             for (int u = 0; u < kP; u++) {
                 const int i = e + u * BS;
-                if (i < n) {
+                if (i < hi) {
                     const double ds = vx[u] - vs[u], dy = vg[u] - vy[u];
                     psv[i] = ds;
                     pyv[i] = dy;
@@ -286,15 +383,15 @@ __device__ __forceinline__ double relax_wrap(double x)
 // One optimizer step for every resident job.  Stage 0 consumes the evaluation just made (energy in
 // SysDesc.energy, gradient in SysDesc.grad) and advances PLIS until it needs the next point, a new two-loop
 // direction (left to the wide kernels), or the end of a pass; stage 1 resumes the jobs that asked for a
-// direction.  Grid: one block of kRelaxBlock threads per job (kRelaxBlockBig for the large ones).
-template <int BS>
-__global__ void __launch_bounds__(BS) k_relax_step(SysDesc *__restrict__ sysv, const RelaxDesc *__restrict__ rdv,
-                                                           int stage)
+// direction.  k_relax_step: one block of kRelaxBlock threads per job.  k_relax_step_grid: nb blocks per large
+// job, each owning a contiguous element range; every block runs the same PLIS control on the same reduced
+// scalars, and block 0 writes the job's state.  The global scalar writes (state, flags, rescale) come last, after
+// every block has read them.
+template <int BS, bool kGrid>
+__device__ __forceinline__ void relax_step_body(SysDesc *__restrict__ D, const RelaxDesc &R, int stage, int b, int nb)
 {
     __shared__ double lds_red[2 * 5 * BS / 64];
     __shared__ double lds_u[kPlisMaxHistory], lds_v[kPlisMaxHistory];
-    const RelaxDesc R = rdv[blockIdx.x];
-    SysDesc *D = sysv + R.sys;
     if (!(D->flags & kSysActive)) return;
     if (stage == 1 && R.st->plis.phase != kPhDirection) return;
     RelaxState st = *R.st;
@@ -303,12 +400,26 @@ __global__ void __launch_bounds__(BS) k_relax_step(SysDesc *__restrict__ sysv, c
         lds_v[i] = R.uv[R.mf + i];
     }
     __syncthreads();
-    BlockPlisOps<BS> ops{R.n, R.mf, const_cast<double *>(D->x), D->grad, R.s, R.xo, R.go, lds_red, lds_u, lds_v,
-                         false, 0.0};
+    using Ops = BlockPlisOps<BS, kGrid>;
+    Ops ops{R.n, R.mf, const_cast<double *>(D->x), D->grad, R.s, R.xo, R.go, lds_red, lds_u, lds_v, false, 0.0};
     ops.red_slot = 0;
     ops.res_snorm = 0.0;
+    if constexpr (kGrid) {
+        const int chunk = (R.n + nb - 1) / nb;
+        ops.lo = min(R.n, b * chunk);
+        ops.hi = min(R.n, ops.lo + chunk);
+        ops.gred = R.gred;
+        ops.bar = R.bar;
+        ops.nb = nb;
+        ops.b = b;
+    } else {
+        ops.lo = 0;
+        ops.hi = R.n;
+    }
     int act = kPlisNeedEval;
-    bool finish = false;
+    bool finish = false, failed = false;
+    unsigned clear = 0;        // SysDesc flags to clear at the end
+    double rescale_new = -1.0; // SysDesc rescale to write at the end (< 0: unchanged)
     int calls = 0;  // PlisAdvance runs from one call site (inlined once, state in registers)
     double E = 0.0;
     if (stage == 1) {
@@ -333,79 +444,97 @@ __global__ void __launch_bounds__(BS) k_relax_step(SysDesc *__restrict__ sysv, c
         bool bad = E != E;
         {
             const double *gr = D->grad;
-            double v[BlockPlisOps<BS>::kU];
+            double v[Ops::kU];
             ops.batches([&](int u, int i) { v[u] = gr[i]; }, [&](int u, int) { bad |= !isfinite(v[u]); });
         }
-        if (__syncthreads_or(bad)) {
-            if (threadIdx.x == 0) {
-                st.failed = 1;
-                st.finished = 1;
-                *R.st = st;
-                D->flags &= ~kSysActive;
-            }
-            return;
-        }
-        // Mixed precision: the FP32-phase objective drives the descent until E first falls below `polish` or
-        // the first pass ends; then the system goes to FP64 and the same x is evaluated again as the start of
-        // the interrupted (or next) pass, so the polish iterations and every convergence test see FP64.
-        const bool to_fp64 = (D->flags & kSysMixed) && (E < st.polish || (st.boundary && st.pass > 0));
-        if (to_fp64) {
-            if (!st.boundary) {
-                st.evals += st.plis.nevals;
-                st.boundary = 1;
-            }
-            st.switch_eval = st.device_evals;
-            __syncthreads();
-            if (threadIdx.x == 0) D->flags &= ~kSysMixed;
-        } else if (st.boundary) {
-            // RelaxStructure_NLOPT_inner starts with "if (pot.Energy() < fmin) return" and _Emin stops between
-            // passes once E < Emin; after the third pass the solve ends.
-            if (st.pass >= 3 || E < st.fmin) {
-                finish = true;
-            } else {
-                for (int i = threadIdx.x; i < 2 * R.mf * R.n; i += BS) {
-                    if (i < R.mf * R.n) R.xo[i] = 0.0;
-                    else R.go[i - R.mf * R.n] = 0.0;
-                }
-                for (int i = threadIdx.x; i < R.mf; i += BS) lds_u[i] = lds_v[i] = 0.0;
-                __syncthreads();
-                PlisInit(st.plis, R.n, R.mf, st.maxstep, st.fmin);
-                st.boundary = 0;
-                calls = 2;  // the first call asks for the first evaluation: the one just made
-            }
+        if (ops.any(bad)) {
+            failed = true;
         } else {
-            calls = 1;
+            // Mixed precision: the FP32-phase objective drives the descent until E first falls below `polish` or
+            // the first pass ends; then the system goes to FP64 and the same x is evaluated again as the start of
+            // the interrupted (or next) pass, so the polish iterations and every convergence test see FP64.
+            const bool to_fp64 = (D->flags & kSysMixed) && (E < st.polish || (st.boundary && st.pass > 0));
+            if (to_fp64) {
+                if (!st.boundary) {
+                    st.evals += st.plis.nevals;
+                    st.boundary = 1;
+                }
+                st.switch_eval = st.device_evals;
+                clear |= kSysMixed;
+            } else if (st.boundary) {
+                // RelaxStructure_NLOPT_inner starts with "if (pot.Energy() < fmin) return" and _Emin stops between
+                // passes once E < Emin; after the third pass the solve ends.
+                if (st.pass >= 3 || E < st.fmin) {
+                    finish = true;
+                } else {
+                    for (int p = 0; p < R.mf; p++)
+                        for (int i = ops.lo + threadIdx.x; i < ops.hi; i += BS) {
+                            R.xo[(size_t)p * R.n + i] = 0.0;
+                            R.go[(size_t)p * R.n + i] = 0.0;
+                        }
+                    for (int i = threadIdx.x; i < R.mf; i += BS) lds_u[i] = lds_v[i] = 0.0;
+                    __syncthreads();
+                    PlisInit(st.plis, R.n, R.mf, st.maxstep, st.fmin);
+                    st.boundary = 0;
+                    calls = 2;  // the first call asks for the first evaluation: the one just made
+                }
+            } else {
+                calls = 1;
+            }
         }
     }
     for (int c = 0; c < calls; c++) {
         if (stage == 0 && c == calls - 1) st.plis.f = E;
         act = PlisAdvance(st.plis, ops);
     }
-    if (!finish && act == kPlisDone) {
+    if (!failed && !finish && act == kPlisDone) {
         // end of a pass: s = wrap(x / rescale) (GetConfig), then x = s * next rescale (GetParam)
         st.evals += st.plis.nevals;
         st.status = PlisResult(st.plis);
         const double r_old = kRelaxRescale[st.pass];
         const double r_new = st.pass + 1 <= 2 ? kRelaxRescale[st.pass + 1] : 1.0;
         double *x = const_cast<double *>(D->x);
-        for (int i = threadIdx.x; i < R.n; i += BS) x[i] = relax_wrap(x[i] / r_old) * r_new;
+        for (int i = ops.lo + threadIdx.x; i < ops.hi; i += BS) x[i] = relax_wrap(x[i] / r_old) * r_new;
         st.pass++;
         st.boundary = 1;
-        __syncthreads();
-        if (threadIdx.x == 0) D->rescale = r_new;
+        rescale_new = r_new;
     }
+    ops.settle();
     __syncthreads();
+    if (kGrid && b != 0) return;
     if (threadIdx.x == 0) {
-        if (finish) {
+        if (failed) st.failed = 1;
+        if (ops.timed_out) st.failed = 2;
+        if (failed || finish || ops.timed_out) {
             st.finished = 1;
-            D->flags &= ~kSysActive;
+            clear |= kSysActive;
         }
         *R.st = st;
+        if (rescale_new >= 0.0) D->rescale = rescale_new;
+        if (clear) D->flags &= ~clear;
     }
     for (int i = threadIdx.x; i < R.mf; i += BS) {
         R.uv[i] = lds_u[i];
         R.uv[R.mf + i] = lds_v[i];
     }
+}
+
+template <int BS>
+__global__ void __launch_bounds__(BS) k_relax_step(SysDesc *__restrict__ sysv, const RelaxDesc *__restrict__ rdv,
+                                                   int stage)
+{
+    const RelaxDesc R = rdv[blockIdx.x];
+    relax_step_body<BS, false>(sysv + R.sys, R, stage, 0, 1);
+}
+
+// items: one per block, sys = the job's descriptor index, t0 = the block's index in the job, t1 = the job's blocks
+template <int BS>
+__global__ void __launch_bounds__(BS) k_relax_step_grid(SysDesc *__restrict__ sysv, const RelaxDesc *__restrict__ rdv,
+                                                        const WorkItem *__restrict__ items, int stage)
+{
+    const WorkItem it = items[blockIdx.x];
+    const RelaxDesc R = rdv[it.sys];
+    relax_step_body<BS, true>(sysv + R.sys, R, stage, it.t0, it.t1);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -676,15 +805,15 @@ __global__ void __launch_bounds__(kRelaxBlock) k_relax_stop(SysDesc *sys, const 
 }
 
 void launch_relax_step(hipStream_t stream, SysDesc *sys, const RelaxDesc *rd, int n, int n_big,
-                       const WorkItem *wide_items, int nwide)
+                       const WorkItem *wide_items, int nwide, const WorkItem *big_items, int nbig_items)
 {
     if (n <= 0) return;
-    // jobs [0, n - n_big) step with kRelaxBlock threads, the large ones after them with kRelaxBlockBig
+    // jobs [0, n - n_big) step with one block each, the large ones after them on their grid-step blocks
     auto step = [&](int stage) {
         if (n > n_big) hipLaunchKernelGGL(k_relax_step<kRelaxBlock>, dim3(n - n_big), dim3(kRelaxBlock), 0, stream, sys, rd, stage);
-        if (n_big > 0)
-            hipLaunchKernelGGL(k_relax_step<kRelaxBlockBig>, dim3(n_big), dim3(kRelaxBlockBig), 0, stream, sys,
-                               rd + (n - n_big), stage);
+        if (nbig_items > 0)
+            hipLaunchKernelGGL(k_relax_step_grid<kRelaxBlock>, dim3(nbig_items), dim3(kRelaxBlock), 0, stream, sys, rd,
+                               big_items, stage);
     };
     step(0);
     if (nwide > 0) {
@@ -778,7 +907,7 @@ int64_t Solver::EstimateBytes(const Job &j)
     // positions, history, direction, forces, gradient, cell list and per-job L-BFGS partials
     const int64_t nch = std::max<int64_t>(1, (n + kRelaxChunk - 1) / kRelaxChunk);
     int64_t b = 8 * (n * (2 * j.mf + 6) + j.N * 10 + j.N * 8) + 8 * nch * (kPlisMaxHistory * 5 + 2) +
-                8 * (2 * j.mf * j.mf + 2 * j.mf) + 4096;
+                8 * (2 * j.mf * j.mf + 2 * j.mf) + 8 * 2 * 5 * kGridMaxBlocks + 4096;
     if (j.mixture()) {
         b += 12 * j.N;  // radii and particle ids
         for (size_t q = 0; q < j.spec_idx.size(); q++)
@@ -793,7 +922,7 @@ int64_t Solver::EstimateBytes(const Job &j)
 int64_t Job::DeviceBytes() const
 {
     int64_t b = sdir.bytes() + xo.bytes() + go.bytes() + uv.bytes() + gram.bytes() + part1.bytes() + part2.bytes() +
-                lcoef.bytes() + st.bytes();
+                lcoef.bytes() + gred.bytes() + bar.bytes() + st.bytes();
     if (sys) {
         b += sys->DeviceBytes();
         for (auto &c : sys->children) b += c->DeviceBytes();
@@ -890,6 +1019,12 @@ void Solver::Admit()
             j->part2.Reserve((size_t)nch * 2);
             j->lcoef.Reserve(2 * kPlisMaxHistory + 1);
             j->st.Reserve(1);
+            if (n >= kRelaxBigN) {
+                static const unsigned kZeroBar[4] = {0, 0, 0, 0};
+                j->gred.Reserve(2 * 5 * kGridMaxBlocks);
+                j->bar.Reserve(4);
+                CC_HIP(hipMemcpyAsync(j->bar.get(), kZeroBar, sizeof(kZeroBar), hipMemcpyHostToDevice, L.batch.stream));
+            }
             // x0 then the zero uv block in one host buffer
             staged.emplace_back(n + 2 * j->mf, 0.0);
             std::vector<double> &x0 = staged.back();
@@ -951,10 +1086,10 @@ void Solver::Rebuild(SolverLane &L)
     batch.systems.clear();
     L.retired.clear();  // the lane has drained (every rebuild follows a synchronisation)
     std::vector<RelaxDesc> rd;
-    std::vector<WorkItem> wide;
+    std::vector<WorkItem> wide, big;
     L.slots = L.jobs;
     for (Job *j : L.jobs) batch.systems.push_back(j->sys.get());
-    // L-BFGS descriptors: the jobs stepped by kRelaxBlock threads first, the large ones (kRelaxBlockBig) last
+    // L-BFGS descriptors: the jobs stepped by one block each first, the large ones (grid step) last
     std::vector<size_t> order;
     for (int big = 0; big < 2; big++)
         for (size_t i = 0; i < L.jobs.size(); i++)
@@ -976,8 +1111,14 @@ void Solver::Rebuild(SolverLane &L)
         r.part1 = j->part1.get();
         r.part2 = j->part2.get();
         r.lcoef = j->lcoef.get();
+        r.gred = j->gred.get();
+        r.bar = j->bar.get();
         r.nchunks = (int)std::max<int64_t>(1, (j->N * j->d + kRelaxChunk - 1) / kRelaxChunk);
         for (int c = 0; c < r.nchunks; c++) wide.push_back({(int32_t)rd.size(), c, 0, 0});
+        if (j->N * j->d >= kRelaxBigN) {
+            const int nb = RelaxGridBlocks(j->N * j->d);
+            for (int q = 0; q < nb; q++) big.push_back({(int32_t)rd.size(), q, nb, 0});
+        }
         r.n = (int)(j->N * j->d);
         r.mf = j->mf;
         r.sys = (int)i;
@@ -993,6 +1134,11 @@ void Solver::Rebuild(SolverLane &L)
     L.nwide = (int)wide.size();
     if (!wide.empty())
         CC_HIP(hipMemcpyAsync(L.d_wide.get(), wide.data(), wide.size() * sizeof(WorkItem), hipMemcpyHostToDevice,
+                              batch.stream));
+    L.d_big.Reserve(std::max<size_t>(1, big.size()));
+    L.nbig_items = (int)big.size();
+    if (!big.empty())
+        CC_HIP(hipMemcpyAsync(L.d_big.get(), big.data(), big.size() * sizeof(WorkItem), hipMemcpyHostToDevice,
                               batch.stream));
     if (L.h_desc_cap < L.slots.size()) {
         if (L.h_desc) CC_HIP(hipHostFree(L.h_desc));
@@ -1090,7 +1236,12 @@ void Solver::FinishHarvest()
         j->switch_eval = hs.switch_eval;
         j->t_done = Now();
         j->state = Job::kDone;
-        if (hs.failed) {
+        if (hs.failed == 2) {
+            j->state = Job::kFailed;
+            j->error_code = kErrDevice;
+            j->error = "the blocks of the job's optimizer step lost their barrier (timed out) at device evaluation " +
+                       std::to_string(hs.device_evals);
+        } else if (hs.failed) {
             j->state = Job::kFailed;
             j->error_code = kErrNan;
             j->error = "NaN energy or non-finite gradient at device evaluation " + std::to_string(hs.device_evals) +
@@ -1113,6 +1264,8 @@ void Solver::FinishHarvest()
         j->part1.Release();
         j->part2.Release();
         j->lcoef.Release();
+        j->gred.Release();
+        j->bar.Release();
         j->st.Release();
         bytes_in_use_ -= j->bytes;
     }
@@ -1163,7 +1316,7 @@ void Solver::LaunchBurst(int burst)
         for (int it = 0; it < steps; it++) {
             L.batch.Enqueue(true, true);
             launch_relax_step(L.batch.stream, L.batch.d_desc.get(), L.d_rdesc.get(), (int)L.slots.size(), L.n_big,
-                              L.d_wide.get(), L.nwide);
+                              L.d_wide.get(), L.nwide, L.d_big.get(), L.nbig_items);
         }
     };
     bool nufft = false;  // hipFFT calls stay on plain stream launches

@@ -225,6 +225,13 @@ def test_2d_N20000_fp64_large_job_beside_small():
     alone.run(timeout=300)
     a, b = alone.result(ja), solver.result(js)
     assert a["energy"] == b["energy"] and np.array_equal(a["relative"], b["relative"])
+    # the large job's multi-block step gives the same bits alone
+    alone = BatchSolver(max_slots=1)
+    ja = alone.submit(cfg, k, v, Emin=EMIN)
+    alone.run(timeout=300)
+    a = alone.result(ja)
+    assert a["energy"] == r["energy"] and a["evaluations"] == r["evaluations"]
+    assert np.array_equal(a["relative"], r["relative"])
 
 
 def test_serving_stream_to_spec_against_oracle():
