@@ -460,21 +460,31 @@ __global__ void __launch_bounds__(kBlock) k_rho_reduce(const SysDesc *__restrict
 // soft-core sum.  epart holds one term per slot; thread t sums slots t, t + 256, ... and the block tree adds the
 // threads in a fixed order.  items: (sys)
 // ---------------------------------------------------------------------------------------------
+// v[t] + v[t + B] + v[t + 2B] + ... for thread t (B = kBlock), in that order, with eight loads in flight
+__device__ __forceinline__ double strided_sum(const double *__restrict__ v, int n)
+{
+    double acc = 0.0;
+    int q = threadIdx.x;
+    for (; q + 7 * kBlock < n; q += 8 * kBlock) {
+        double t[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) t[u] = v[q + u * kBlock];
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc += t[u];
+    }
+    for (; q < n; q += kBlock) acc += v[q];
+    return acc;
+}
+
 __global__ void __launch_bounds__(kBlock) k_energy_final(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
 {
     __shared__ double lds[kBlock];
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
     if (!sys_active(sys, S)) return;
-    double acc = 0.0;
-    for (int q = threadIdx.x; q < S.nslots; q += kBlock) acc += S.epart[q];
-    const double cc_sum = block_sum(acc, lds);
+    const double cc_sum = block_sum(strided_sum(S.epart, S.nslots), lds);
     double sc = 0.0;
-    if (S.flags & kSysSoftCore) {
-        double a2 = 0.0;
-        for (int i = threadIdx.x; i < S.N; i += kBlock) a2 += S.sc_epart[i];
-        sc = block_sum(a2, lds);
-    }
+    if (S.flags & kSysSoftCore) sc = block_sum(strided_sum(S.sc_epart, S.N), lds);
     if (threadIdx.x == 0) {
         const double cc = ((S.flags & kSysS2) ? cc_sum : cc_sum / S.vol) + S.shift;
         // LogCCPotential::Energy (CollectiveCoordinatePotential.h:1620-1623): log of the plain form's energy
@@ -607,9 +617,7 @@ __global__ void __launch_bounds__(kBlock) k_mixture_final(const SysDesc *__restr
     if (!sys_active(sys, S)) return;
     if (S.flags & kSysMultiConfig) {
         // ShiftedCCPotential_S2_MultiConfig::Energy (:1818-1844): sum V (S - S0)^2 + Shift over the mean S
-        double acc = 0.0;
-        for (int q = threadIdx.x; q < S.nslots; q += kBlock) acc += S.epart[q];
-        const double cc = block_sum(acc, lds);
+        const double cc = block_sum(strided_sum(S.epart, S.nslots), lds);
         if (threadIdx.x == 0) {
             S.energy[0] = cc + S.shift;
             S.energy[1] = cc + S.shift;
@@ -617,10 +625,7 @@ __global__ void __launch_bounds__(kBlock) k_mixture_final(const SysDesc *__restr
         }
         return;
     }
-    double a = 0.0;
-    if (S.flags & kSysSoftCore)
-        for (int i = threadIdx.x; i < S.N; i += kBlock) a += S.sc_epart[i];
-    const double cp = block_sum(a, lds);
+    const double cp = block_sum((S.flags & kSysSoftCore) ? strided_sum(S.sc_epart, S.N) : 0.0, lds);
     if (threadIdx.x == 0) {
         double cc = 0.0;
         for (int c = 0; c < S.nchild; c++) {

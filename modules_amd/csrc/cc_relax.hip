@@ -552,9 +552,10 @@ __global__ void __launch_bounds__(BS) k_relax_step_grid(SysDesc *__restrict__ sy
 constexpr int kWideBlock = 256;
 constexpr int kWideChunk = kRelaxChunk;  // elements per pass block
 
-// Pass 1: one block per (job, chunk of kWideChunk elements).  Each wavefront keeps the chunk's g, s_1, y_1 in
-// registers (kWideChunk / 64 elements per lane, coalesced) and takes every nw-th history pair: five dot
-// partials per pair, each summed over the wavefront with DPP and written once.
+// Pass 1: one block per (job, chunk of kWideChunk elements, pair group).  Each wavefront keeps the chunk's g, s_1,
+// y_1 in registers (kWideChunk / 64 elements per lane, coalesced) and takes every (nw x groups)-th history pair: five
+// dot partials per pair, each summed over the wavefront with DPP and written once.  The group count only spreads
+// the pairs over more wavefronts (few chunks in flight: a large job alone); every partial is the same.
 __global__ void __launch_bounds__(kWideBlock) k_lbfgs_pass1(const SysDesc *__restrict__ sysv, const RelaxDesc *__restrict__ rdv,
                                                             const WorkItem *__restrict__ items)
 {
@@ -578,7 +579,7 @@ __global__ void __launch_bounds__(kWideBlock) k_lbfgs_pass1(const SysDesc *__res
         s1v[m] = in ? s1[e] : 0.0;
         y1v[m] = in ? y1[e] : 0.0;
     }
-    for (int j = wave; j < k; j += nw) {
+    for (int j = wave + (int)blockIdx.y * nw; j < k; j += nw * (int)gridDim.y) {
         const size_t pj = (size_t)((head + j) % mf) * n;
         const double *sj = R.xo + pj, *yj = R.go + pj;
         double sv[kPer], yv[kPer];
@@ -634,9 +635,18 @@ __global__ void __launch_bounds__(kWideBlock) k_lbfgs_solve(const SysDesc *__res
     const int p1 = head;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int q = threadIdx.x; q < 5 * k; q += kWideBlock) {
+        // chunk partials in chunk order, sixteen loads in flight
+        const double *p = R.part1 + q;
         double t = 0.0;
-#pragma unroll 8
-        for (int c = 0; c < R.nchunks; c++) t += R.part1[(size_t)c * H * 5 + q];
+        int c = 0;
+        for (; c + 15 < R.nchunks; c += 16) {
+            double v[16];
+#pragma unroll
+            for (int u = 0; u < 16; u++) v[u] = p[(size_t)(c + u) * H * 5];
+#pragma unroll
+            for (int u = 0; u < 16; u++) t += v[u];
+        }
+        for (; c < R.nchunks; c++) t += p[(size_t)c * H * 5];
         dots[q] = t;
     }
     for (int i = threadIdx.x; i < k; i += kWideBlock) uu[i] = R.uv[i];
@@ -817,7 +827,8 @@ void launch_relax_step(hipStream_t stream, SysDesc *sys, const RelaxDesc *rd, in
     };
     step(0);
     if (nwide > 0) {
-        hipLaunchKernelGGL(k_lbfgs_pass1, dim3(nwide), dim3(kWideBlock), 0, stream, sys, rd, wide_items);
+        const int groups = std::min(8, std::max(1, 1024 / nwide));
+        hipLaunchKernelGGL(k_lbfgs_pass1, dim3(nwide, groups), dim3(kWideBlock), 0, stream, sys, rd, wide_items);
         hipLaunchKernelGGL(k_lbfgs_solve, dim3(n), dim3(kWideBlock), 0, stream, sys, rd);
         hipLaunchKernelGGL(k_lbfgs_pass2, dim3(nwide), dim3(kWideBlock), 0, stream, sys, rd, wide_items);
     }
