@@ -78,11 +78,17 @@ __device__ __forceinline__ void force_finish(const SysDesc *__restrict__ sys, co
     if (S.grad) {
         // a species view writes its range of the mixture's gradient, at the mixture's current rescale
         const double rescale = S.parent >= 0 ? sys[S.parent].rescale : S.rescale;
+        bool finite = true;
         for (int j = 0; j < d; j++) {
             double dot = 0.0;
             for (int c = 0; c < d; c++) dot += F[c] * S.a[j * d + c];
-            S.grad[(int64_t)i * d + j] = ((-1.0) * dot) / rescale;
+            const double gj = ((-1.0) * dot) / rescale;
+            S.grad[(int64_t)i * d + j] = gj;
+            finite = finite && isfinite(gj);
         }
+        // energy[3] of the system that owns the gradient (cleared by the energy epilogue of the same evaluation)
+        // flags a non-finite entry; every writer stores the same value
+        if (!finite) (S.parent >= 0 ? sys[S.parent].energy : S.energy)[3] = 1.0;
     }
 }
 

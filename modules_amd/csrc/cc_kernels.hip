@@ -491,6 +491,7 @@ __global__ void __launch_bounds__(kBlock) k_energy_final(const SysDesc *__restri
         S.energy[0] = (S.flags & kSysLog) ? log(cc) : sc + cc;
         S.energy[1] = cc;
         S.energy[2] = sc;
+        S.energy[3] = 0.0;  // the force epilogue sets it on a non-finite gradient entry
     }
 }
 
@@ -622,6 +623,7 @@ __global__ void __launch_bounds__(kBlock) k_mixture_final(const SysDesc *__restr
             S.energy[0] = cc + S.shift;
             S.energy[1] = cc + S.shift;
             S.energy[2] = 0.0;
+            S.energy[3] = 0.0;
         }
         return;
     }
@@ -636,6 +638,7 @@ __global__ void __launch_bounds__(kBlock) k_mixture_final(const SysDesc *__restr
         S.energy[0] = e_cc + e_cp;
         S.energy[1] = e_cc;
         S.energy[2] = e_cp;
+        S.energy[3] = 0.0;  // the species' force epilogues set it on a non-finite gradient entry
     }
 }
 

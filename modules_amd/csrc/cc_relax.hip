@@ -156,16 +156,24 @@ struct BlockPlisOps {
         }
         return r;
     }
-    // kGrid: the step's global scalar writes follow a barrier of the job's blocks, so that every block has read
-    // the job's state before block 0 rewrites it
+    // kGrid, block 0: the step's global scalar writes wait until every block of the job has registered that it
+    // read the job's state (bar[3], reset here for the next launch); a timeout anywhere fails the job
     __device__ void settle() const
     {
         if constexpr (kGrid) {
-            if (barriers == 0) {
-                if (!job_barrier(bar, nb)) timed_out = true;
-                barriers++;
+            if (b == 0 && threadIdx.x == 0) {
+                int polls = 0;
+                while (__hip_atomic_load(bar + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != (unsigned)nb) {
+                    if (++polls > kGridSpinLimit) {
+                        timed_out = true;
+                        __hip_atomic_store(bar + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+                __hip_atomic_store(bar + 3, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (__hip_atomic_load(bar + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) timed_out = true;
             }
-            if (__hip_atomic_load(bar + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) timed_out = true;
         }
     }
     __device__ void touch(int id)
@@ -401,6 +409,18 @@ __device__ __forceinline__ void relax_step_body(SysDesc *__restrict__ D, const R
     }
     __syncthreads();
     using Ops = BlockPlisOps<BS, kGrid>;
+    // stage 1 also reads the pass-2 partials here, before any block can rewrite the job's scalars
+    double part_ss = 0.0, part_gs = 0.0;
+    if (stage == 1)
+        for (int c = 0; c < R.nchunks; c++) {
+            part_ss += R.part2[2 * c];
+            part_gs += R.part2[2 * c + 1];
+        }
+    if constexpr (kGrid) {
+        // every block has read the job's state (st, uv, flags, partials): register, so that block 0 can write the
+        // state at the end without a full barrier
+        if (threadIdx.x == 0) __hip_atomic_fetch_add(R.bar + 3, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     Ops ops{R.n, R.mf, const_cast<double *>(D->x), D->grad, R.s, R.xo, R.go, lds_red, lds_u, lds_v, false, 0.0};
     ops.red_slot = 0;
     ops.res_snorm = 0.0;
@@ -424,14 +444,9 @@ __device__ __forceinline__ void relax_step_body(SysDesc *__restrict__ D, const R
     double E = 0.0;
     if (stage == 1) {
         // |S|^2 and g.S of the direction from the pass-2 partials, in chunk order
-        double ss = 0.0, gs = 0.0;
-        for (int c = 0; c < R.nchunks; c++) {
-            ss += R.part2[2 * c];
-            gs += R.part2[2 * c + 1];
-        }
-        ops.res_snorm = sqrt(ss);
+        ops.res_snorm = sqrt(part_ss);
         ops.gs_valid = true;
-        ops.gs_val = gs;
+        ops.gs_val = part_gs;
         calls = 1;
     } else {
         E = D->energy[0];
@@ -441,13 +456,8 @@ __device__ __forceinline__ void relax_step_body(SysDesc *__restrict__ D, const R
         // NotANumberFound on NaN coordinates, potential/StructureOptimization.cpp:458-461; NaN coordinates follow
         // from a NaN energy or gradient at the next trial point).  An infinite energy with a finite gradient stays a
         // valid, if huge, objective value: the line search bisects below its huge-value bound.
-        bool bad = E != E;
-        {
-            const double *gr = D->grad;
-            double v[Ops::kU];
-            ops.batches([&](int u, int i) { v[u] = gr[i]; }, [&](int u, int) { bad |= !isfinite(v[u]); });
-        }
-        if (ops.any(bad)) {
+        // (the force epilogue flags a non-finite gradient entry in energy[3]: a scalar test, no pass over g)
+        if (E != E || D->energy[3] != 0.0) {
             failed = true;
         } else {
             // Mixed precision: the FP32-phase objective drives the descent until E first falls below `polish` or
@@ -499,9 +509,9 @@ __device__ __forceinline__ void relax_step_body(SysDesc *__restrict__ D, const R
         st.boundary = 1;
         rescale_new = r_new;
     }
+    if (kGrid && b != 0) return;
     ops.settle();
     __syncthreads();
-    if (kGrid && b != 0) return;
     if (threadIdx.x == 0) {
         if (failed) st.failed = 1;
         if (ops.timed_out) st.failed = 2;
