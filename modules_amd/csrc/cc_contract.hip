@@ -1,4 +1,5 @@
-// cc_contract.hip -- the FP64 lattice contractions of the CC potential in their batched form (gfx950).
+// cc_contract.hip -- the lattice contractions of the CC potential in their batched form (gfx950): FP64, and the
+// mixed-precision force kernel (FP32 phases and force terms, FP64 sums; rho stays FP64).
 //
 //   rho_k = sum_j exp(2 pi i n_k . s_j)                   (GetRho_Direct, potential/CollectiveCoordinatePotential.h:1255-1286)
 //   G_j   = sum_k c_k n_k,  c_k = A_k cos - B_k sin        (Force / AllForce, :1377-1448; F = -sum_j G_j b_j)
@@ -301,7 +302,7 @@ __global__ void __launch_bounds__(kBlock) k_force_rows(const SysDesc *__restrict
     __shared__ double4 gsum[4][PPL][64];
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
-    if (!sys_active(sys, S)) return;
+    if (!sys_active(sys, S) || (S.flags & kSysMixed)) return;  // mixed systems: k_force_rows_mixed
     sincos_table_fill(L.tab);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int d = S.d;
@@ -360,10 +361,156 @@ __global__ void __launch_bounds__(kBlock) k_force_rows(const SysDesc *__restrict
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Mixed-precision forces (kSysMixed, one particle per lane): rho stays FP64 (k_rho_rows), so the energy and the
+// force coefficients are FP64 values; here each segment's head phase is reduced to [-1/2, 1/2] revolutions in
+// FP64, then its sincos, the phase chains, the coefficient products and the segment's two sums run in FP32, and
+// the segment sums are added to the particle's G in FP64.  The chains keep the FP64 kernel's two-FMA three-term
+// recurrence in 2 phi; its rounding error grows as t^2 u over a chain of t steps (u = 6e-8), a few 1e-6 relative at
+// the 16 steps of a 32-slot segment.  Per slot: 6 FP32 operations against 6 FP64 ones.
+// ---------------------------------------------------------------------------------------------
+template <int SEG>
+struct ForceRowsLds32 {
+    double2 tab[64];
+    float2 tab32[64];
+    float2 coef[kForceStageRows * SEG];
+    int4 head[kForceStageRows];
+    int len[kForceStageRows];
+};
+
+template <int SEG, int L>
+__device__ __forceinline__ void force_rows_segment32(const float2 *cf, int4 h, double hl, int d, const double4 &p,
+                                                     float stx, float sty, float k2, const float2 *tab32,
+                                                     double (&g)[4])
+{
+    float Rc[2][2], Rs[2][2], acc[2], accl[2];
+    {
+        const double u = head_dot(h, p);
+        float sn0, cs0;
+        sincos_rev_tab32((float)(u - __builtin_rint(u)), tab32, sn0, cs0);
+        const float cs1 = __builtin_fmaf(cs0, stx, -sn0 * sty);
+        const float sn1 = __builtin_fmaf(cs0, sty, sn0 * stx);
+        const float k1 = stx + stx;
+        const float cm1 = __builtin_fmaf(k1, cs0, -cs1);  // e_{-1} = 2 cos(phi) e_0 - e_1
+        const float sm1 = __builtin_fmaf(k1, sn0, -sn1);
+        Rc[0][0] = __builtin_fmaf(k1, cm1, -cs0);  // e_{-2}
+        Rs[0][0] = __builtin_fmaf(k1, sm1, -sn0);
+        Rc[0][1] = cs0;
+        Rs[0][1] = sn0;
+        Rc[1][0] = cm1;
+        Rs[1][0] = sm1;
+        Rc[1][1] = cs1;
+        Rs[1][1] = sn1;
+        acc[0] = acc[1] = accl[0] = accl[1] = 0.0f;
+    }
+#pragma unroll
+    for (int t = 0; t < L; t++) {
+        const int ch = t & 1, m = t >> 1;
+        const float2 ab = cf[t];
+        const float cc = Rc[ch][(m + 1) & 1], ss = Rs[ch][(m + 1) & 1];
+        const float c = __builtin_fmaf(ab.x, cc, -ab.y * ss);
+        acc[ch] += c;
+        accl[ch] = __builtin_fmaf((float)t, c, accl[ch]);
+        if (t + 2 < L) {
+            Rc[ch][m & 1] = __builtin_fmaf(k2, cc, -Rc[ch][m & 1]);
+            Rs[ch][m & 1] = __builtin_fmaf(k2, ss, -Rs[ch][m & 1]);
+        }
+    }
+    const double A = (double)acc[0] + (double)acc[1];
+    const double Al = __builtin_fma(hl, A, (double)accl[0] + (double)accl[1]);
+    if (d == 1) {
+        g[0] += Al;
+    } else if (d == 2) {
+        g[0] = __builtin_fma((double)h.x, A, g[0]);
+        g[1] += Al;
+    } else if (d == 3) {
+        g[0] = __builtin_fma((double)h.x, A, g[0]);
+        g[1] = __builtin_fma((double)h.y, A, g[1]);
+        g[2] += Al;
+    } else {
+        g[0] = __builtin_fma((double)h.x, A, g[0]);
+        g[1] = __builtin_fma((double)h.y, A, g[1]);
+        g[2] = __builtin_fma((double)h.z, A, g[2]);
+        g[3] += Al;
+    }
+}
+
+template <int SEG, int L = 2>
+__device__ __forceinline__ void force_rows_dispatch32(int len, const float2 *cf, int4 h, double hl, int d,
+                                                      const double4 &p, float stx, float sty, float k2,
+                                                      const float2 *tab32, double (&g)[4])
+{
+    if constexpr (L >= SEG) {
+        force_rows_segment32<SEG, SEG>(cf, h, hl, d, p, stx, sty, k2, tab32, g);
+    } else {
+        if (len <= L) force_rows_segment32<SEG, L>(cf, h, hl, d, p, stx, sty, k2, tab32, g);
+        else force_rows_dispatch32<SEG, L + 2>(len, cf, h, hl, d, p, stx, sty, k2, tab32, g);
+    }
+}
+
+template <int SEG>
+__global__ void __launch_bounds__(kBlock) k_force_rows_mixed(const SysDesc *__restrict__ sys,
+                                                             const WorkItem *__restrict__ items)
+{
+    __shared__ ForceRowsLds32<SEG> L;
+    __shared__ double4 gsum[4][64];
+    const WorkItem it = items[blockIdx.x];
+    const SysDesc &S = sys[it.sys];
+    if (!sys_active(sys, S) || !(S.flags & kSysMixed)) return;
+    sincos_table_fill(L.tab);
+    __syncthreads();
+    sincos_table_fill32(L.tab32, L.tab);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int d = S.d;
+    const int i = it.t0 * 64 + lane;
+    const bool ok = i < S.N;
+    const double4 p = ok ? S.pos[i] : make_double4(0, 0, 0, 0);
+    const double2 st = ok ? S.step[i] : make_double2(1, 0);
+    const float stx = (float)st.x, sty = (float)st.y;
+    const float c2 = stx * stx - sty * sty;  // cos(2 phi)
+    const float k2 = c2 + c2;
+    double g[4] = {0.0, 0.0, 0.0, 0.0};
+    const int s0 = it.t1 * S.kchunk_len;
+    const int s1 = min(S.nseg, s0 + S.kchunk_len);
+    const float2 *coef = reinterpret_cast<const float2 *>(S.coef32);
+    for (int sb = s0; sb < s1; sb += kForceStageRows) {
+        const int ns = min(kForceStageRows, s1 - sb);
+        __syncthreads();  // the previous group is consumed (and the FP32 table written)
+        for (int x = threadIdx.x; x < ns * SEG; x += kBlock) L.coef[x] = coef[(int64_t)sb * SEG + x];
+        if (threadIdx.x < ns) {
+            L.head[threadIdx.x] = S.seg_head[sb + threadIdx.x];
+            L.len[threadIdx.x] = S.seg_len[sb + threadIdx.x];
+        }
+        __syncthreads();
+        for (int sl = wave; sl < ns; sl += 4) {
+            const int4 hv = L.head[sl];
+            const int4 h = make_int4(__builtin_amdgcn_readfirstlane(hv.x), __builtin_amdgcn_readfirstlane(hv.y),
+                                     __builtin_amdgcn_readfirstlane(hv.z), __builtin_amdgcn_readfirstlane(hv.w));
+            const int len = __builtin_amdgcn_readfirstlane(L.len[sl]);
+            const double hl = (d == 1) ? (double)h.x : (d == 2) ? (double)h.y : (d == 3) ? (double)h.z : (double)h.w;
+            force_rows_dispatch32<SEG>(len, L.coef + sl * SEG, h, hl, d, p, stx, sty, k2, L.tab32, g);
+        }
+    }
+    gsum[wave][lane] = make_double4(g[0], g[1], g[2], g[3]);
+    __syncthreads();
+    if (threadIdx.x < 64 && it.t0 * 64 + (int)threadIdx.x < S.N) {
+        const int l = threadIdx.x, ii = it.t0 * 64 + l;
+        const double4 a = gsum[0][l], b = gsum[1][l], c = gsum[2][l], w = gsum[3][l];
+        const double G[4] = {((a.x + b.x) + c.x) + w.x, ((a.y + b.y) + c.y) + w.y, ((a.z + b.z) + c.z) + w.z,
+                             ((a.w + b.w) + c.w) + w.w};
+        if (S.flags & kSysForceFinal) {
+            force_finish(sys, S, ii, G);
+        } else {
+            reinterpret_cast<double4 *>(S.fpart)[(int64_t)it.t1 * S.N + ii] = make_double4(G[0], G[1], G[2], G[3]);
+        }
+    }
+}
+
 #define CC_INSTANTIATE_ROWS(SEG)                                                                              \
     template __global__ void k_rho_rows<SEG>(const SysDesc *__restrict__, const WorkItem *__restrict__);       \
     template __global__ void k_force_rows<SEG, 1>(const SysDesc *__restrict__, const WorkItem *__restrict__);  \
-    template __global__ void k_force_rows<SEG, 2>(const SysDesc *__restrict__, const WorkItem *__restrict__);
+    template __global__ void k_force_rows<SEG, 2>(const SysDesc *__restrict__, const WorkItem *__restrict__);  \
+    template __global__ void k_force_rows_mixed<SEG>(const SysDesc *__restrict__, const WorkItem *__restrict__);
 CC_INSTANTIATE_ROWS(4)
 CC_INSTANTIATE_ROWS(8)
 CC_INSTANTIATE_ROWS(16)
@@ -380,6 +527,19 @@ void launch_rho_rows(hipStream_t stream, int seg, const SysDesc *sys, const Work
         case 16: hipLaunchKernelGGL(k_rho_rows<16>, grid, block, 0, stream, sys, items); break;
         case 24: hipLaunchKernelGGL(k_rho_rows<24>, grid, block, 0, stream, sys, items); break;
         default: hipLaunchKernelGGL(k_rho_rows<32>, grid, block, 0, stream, sys, items); break;
+    }
+}
+
+void launch_force_rows_mixed(hipStream_t stream, int seg, const SysDesc *sys, const WorkItem *items, int n)
+{
+    if (n <= 0) return;
+    const dim3 grid((unsigned)n), block(kBlock);
+    switch (seg) {
+        case 4: hipLaunchKernelGGL(k_force_rows_mixed<4>, grid, block, 0, stream, sys, items); break;
+        case 8: hipLaunchKernelGGL(k_force_rows_mixed<8>, grid, block, 0, stream, sys, items); break;
+        case 16: hipLaunchKernelGGL(k_force_rows_mixed<16>, grid, block, 0, stream, sys, items); break;
+        case 24: hipLaunchKernelGGL(k_force_rows_mixed<24>, grid, block, 0, stream, sys, items); break;
+        default: hipLaunchKernelGGL(k_force_rows_mixed<32>, grid, block, 0, stream, sys, items); break;
     }
 }
 

@@ -311,9 +311,12 @@ Tiles ResolveTiles(const Tiles &req, const Tiles &def, int precision)
     if (used.seg != 4 && used.seg != 8 && used.seg != 16 && used.seg != 24 && used.seg != 32)
         throw Error(kErrArg, "segment length must be 4, 8, 16, 24 or 32");
     used.rho_kernel = req.rho_kernel >= 0 ? req.rho_kernel : def.rho_kernel;
-    if (precision == 1) used.rho_kernel = 0;  // the mixed-precision body lives in the grouped-power kernel
+    // mixed precision: the particle-row kernels (FP64 rho, the mixed force kernel with one particle per lane), or
+    // the mixed bodies of the grouped-power kernels when kernel 1 is asked for
+    if (precision == 1 && used.rho_kernel == 1) used.rho_kernel = 0;
     used.force_ppl = req.force_ppl > 0 ? req.force_ppl : (def.force_ppl > 0 ? def.force_ppl : 2);
     if (used.force_ppl != 1 && used.force_ppl != 2) throw Error(kErrArg, "force particles per lane must be 1 or 2");
+    if (precision == 1 && used.rho_kernel == 2) used.force_ppl = 1;
     return used;
 }
 
@@ -920,8 +923,12 @@ void Batch::Enqueue(bool rho, bool forces)
             CC_HIP(hipEventRecord(ev_pool_[mk.e0], stream));
         }
         for (const SegGroup &g : seg_groups) {
-            if (g.rho_kernel == 2) launch_force_rows(stream, g.seg, D, I + g.force.off, g.force.n, g.force_ppl);
-            else launch_force_lattice(stream, g.seg, D, I + g.force.off, g.force.n, g.force_ppl);
+            if (g.rho_kernel == 2) {
+                launch_force_rows(stream, g.seg, D, I + g.force.off, g.force.n, g.force_ppl);
+                if (g.any_mixed) launch_force_rows_mixed(stream, g.seg, D, I + g.force.off, g.force.n);
+            } else {
+                launch_force_lattice(stream, g.seg, D, I + g.force.off, g.force.n, g.force_ppl);
+            }
         }
         launch_force_general(stream, D, I + r_f_gen.off, r_f_gen.n);
         if (has_nufft) {

@@ -371,6 +371,7 @@ void launch_force_lattice(hipStream_t, int seg, const SysDesc *, const WorkItem 
 // the particle-row contractions (cc_contract.hip)
 void launch_rho_rows(hipStream_t, int seg, const SysDesc *, const WorkItem *, int);
 void launch_force_rows(hipStream_t, int seg, const SysDesc *, const WorkItem *, int, int ppl);
+void launch_force_rows_mixed(hipStream_t, int seg, const SysDesc *, const WorkItem *, int);
 void launch_force_general(hipStream_t, const SysDesc *, const WorkItem *, int);
 void launch_force_final(hipStream_t, const SysDesc *, const WorkItem *, int);
 // soft-core pair kernel choice for every later launch: 0 automatic, 1 one thread per particle, 2 lane groups per
