@@ -32,7 +32,7 @@ FP64_VALU_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak (AMD data sheet)
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3, help="timed 256-configuration batches")
+    ap.add_argument("--steps", type=int, default=10, help="timed 256-configuration batches")
     ap.add_argument("--warmup", type=int, default=1, help="untimed batches before them")
     ap.add_argument("--emin", type=float, default=1e-18)
     ap.add_argument("--maxstep", type=int, default=10000)
@@ -47,12 +47,13 @@ def parse():
     ap.add_argument("--no-sub", action="store_true", help="headline only: skip the other workloads' records")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run the library's copy kernel over this size first (calibrates rocprof FETCH/WRITE_SIZE)")
-    ap.add_argument("--graph", action="store_true",
-                    help="replay optimizer steps from captured hipGraphs (per-launch kernel timing is then off: the "
-                         "kernel roofline comes from one extra stream-launched batch after the timed ones)")
-    ap.add_argument("--lanes", type=int, default=1, choices=[1, 2],
-                    help="solver streams (2: the L-BFGS kernels of one overlap the E+F kernels of the other; the "
-                         "kernels then share the GPU and their event durations no longer give a per-kernel roofline)")
+    ap.add_argument("--graph", action=argparse.BooleanOptionalAction, default=True,
+                    help="replay optimizer steps from captured hipGraphs (default; per-launch kernel timing is then "
+                         "off, and the kernel roofline comes from a second, stream-launched one-lane solve of the same "
+                         "batches)")
+    ap.add_argument("--lanes", type=int, default=2, choices=[1, 2],
+                    help="solver streams (2, the default: the L-BFGS kernels of one overlap the E+F kernels of the "
+                         "other; the kernels then share the GPU, so the roofline comes from the one-lane solve)")
     ap.add_argument("--autotune", action="store_true", help="time candidate tiles per shape class first")
     ap.add_argument("--tiles", type=str, default=None,
                     help="seg2d/seg3d,rho_kernel,rho_chunk,force_chunk,force_ppl for every shape class of the batch "
@@ -212,7 +213,7 @@ def headline(args):
     else:
         for w in range(W):
             solve([submit(1000 + w)], warm_clock)
-    timing = not args.profiler_run and not args.graph
+    timing = not args.profiler_run and not args.graph and args.lanes == 1
     solver.set_timing(timing)
     solver.timing(reset=True)
     clock = []
@@ -230,16 +231,25 @@ def headline(args):
     roof = _roofline([(res["device_evaluations"], r.N, M, r.d) for res, r, M in results], rho_ms, force_ms, n_rho,
                      n_force) if timing else None
     if roof is None and not args.profiler_run:
-        # graph mode: the kernel roofline from one more batch with stream launches and per-launch events
+        # graph replay or two lanes: per-launch events would miss (graph) or include the other lane's kernels (two
+        # lanes), so the kernel roofline comes from a second solve of the same batches on one stream, with stream
+        # launches and hipEvents around every contraction launch
+        _log("roofline pass: the same batches on one lane with per-launch events")
         solver.set_graph(False)
+        solver.set_lanes(1)
         solver.set_timing(True)
         solver.timing(reset=True)
-        extra = solve([submit(2000)], [])[0]
+        if resident_all:
+            again = [x for g in solve([submit(s) for s in range(K)], []) for x in g]
+        else:
+            again = [x for s in range(K) for x in solve([submit(s)], [])[0]]
         rho_ms, force_ms, n_rho, n_force = solver.timing(reset=True)
-        roof = _roofline([(res["device_evaluations"], r.N, M, r.d) for res, r, M in extra], rho_ms, force_ms, n_rho,
+        roof = _roofline([(res["device_evaluations"], r.N, M, r.d) for res, r, M in again], rho_ms, force_ms, n_rho,
                          n_force)
         if roof:
-            roof["measured_on"] = "one extra stream-launched batch after the timed (graph-replayed) batches"
+            roof["measured_on"] = ("a second solve of the same batches on one stream (stream launches, hipEvents "
+                                   "around each contraction launch); the timed solve ran with graph replay on "
+                                   f"{args.lanes} lane(s)")
     solved = sum(1 for res, r, M in results if res["state"] == BatchSolver.DONE and res["energy"] < args.emin)
     evals = np.array([res["device_evaluations"] for res, r, M in results])
     lat = np.array([res["t_done"] - res["t_admit"] for res, r, M in results])

@@ -239,6 +239,10 @@ class BatchSolver:
         self.set_rho_backend(rho_backend, nufft_eps)
         self._shapes = {}
 
+    def set_lanes(self, lanes):
+        """Solver streams (1 or 2) for jobs admitted from now on; only while no job is resident."""
+        check(lib().ccgpu_solver_set_lanes(self._h, int(lanes)))
+
     def set_graph(self, on=True):
         """Replay the optimizer steps between two host checks from a captured hipGraph (True) or launch them on
         the stream (False)."""
