@@ -184,8 +184,9 @@ struct BlockPlisOps {
     // Element loops visit this thread's elements kU at a time, all loads of a batch before their uses, so that
     // each thread keeps several independent HBM loads in flight (one workgroup per job leaves few waves per CU);
     // with 256 threads the remainder is one predicated batch, so no element waits on a round trip of its own.
-    // Sums run in element order either way.
-    static constexpr int kU = kGrid ? 8 : BS <= 256 ? 16 : 4;  // the grid step's blocks own few elements each
+    // Eight per batch keep the one-block step within two blocks per CU (its launch bound).  Sums run in element
+    // order whatever the batch, so the batch size never changes a bit.
+    static constexpr int kU = kGrid ? 8 : BS <= 256 ? 8 : 4;
     template <class Load, class Use>
     __device__ __forceinline__ void batches(Load load, Use use) const
     {
@@ -530,7 +531,7 @@ __device__ __forceinline__ void relax_step_body(SysDesc *__restrict__ D, const R
 }
 
 template <int BS>
-__global__ void __launch_bounds__(BS) k_relax_step(SysDesc *__restrict__ sysv, const RelaxDesc *__restrict__ rdv,
+__global__ void __launch_bounds__(BS, 2) k_relax_step(SysDesc *__restrict__ sysv, const RelaxDesc *__restrict__ rdv,
                                                    int stage)
 {
     const RelaxDesc R = rdv[blockIdx.x];
