@@ -176,11 +176,15 @@ def headline(args):
             out.append((j, r, len(k)))
         return out
 
+    admit_clock = []
+
     def solve(groups, clock):
         """Admit the submitted batches (inputs uploaded and resident in HBM), then solve them; only the solve and the
-        result read-back are timed."""
+        result read-back are timed (the admission, host-to-HBM uploads included, is timed apart)."""
+        ta = time.perf_counter()
         solver.step(0)
         _sync()
+        admit_clock.append(time.perf_counter() - ta)
         t0 = time.perf_counter()
         solver.run(chunk=4096)
         res = [[(solver.result(j), r, M) for j, r, M in g] for g in groups]
@@ -226,6 +230,7 @@ def headline(args):
         for s in range(K):
             results.extend(solve([submit(s)], clock)[0])
     wall = sum(clock)
+    admit_s = admit_clock[-1] if admit_clock else 0.0
     rho_ms, force_ms, n_rho, n_force = solver.timing(reset=True)
     launches = solver.launch_stats()
     roof = _roofline([(res["device_evaluations"], r.N, M, r.d) for res, r, M in results], rho_ms, force_ms, n_rho,
@@ -277,6 +282,8 @@ def headline(args):
         "solved": solved,
         "solves": len(results),
         "launch_bursts": launches,
+        "pcie_inclusive_value": solved / (wall + admit_s),
+        "admission_s": admit_s,
         "ef_evals_per_s": float(evals.sum()) / wall,
         "device_evals_per_solve_mean": float(evals.mean()),
         "latency_admit_to_done_p50_s": float(np.percentile(lat, 50)),
@@ -663,26 +670,42 @@ def workload_nufft2d(args):
         return {"backend": pot.GetLastRhoBackendUsed(), "reason": why, "predicted_us_direct": pd,
                 "predicted_us_nufft": pn}
 
+    def device_ms(cfg, k):
+        # one E+F of each path on the device (hipEvents, mean of 5; the host copies of the potential path excluded);
+        # the timed choice it records is forgotten again so that the gate reported above stays the cost model's
+        from modules_amd import autotune
+        use, md, mn = autotune.autotune_backend(cfg.NumParticle(), 2, k=np.asarray(k), basis=cfg.basis, repeats=5)
+        autotune.clear()
+        return md, mn
+
     out = {"metric": "stealthy ground states/s at E<1e-18", "unit": "ground_states/s", "dtype": "f64"}
     cfg = RandomConfiguration(4, 20000, 2)
     k = GetKs(cfg, getK(0.4, 1.0, 2))
     tiles = _tune(args, cfg, k)
+    g = gate(cfg, k)
     ms_d, E_d, F_d, _ = ef_ms(cfg, k, "direct", 20)
     ms_n, E_n, F_n, _ = ef_ms(cfg, k, "nufft_always", 20)
-    out["N20000"] = {"M": len(k), "tiles": tiles, "gate": gate(cfg, k),
-                     "ef_ms_direct": ms_d, "ef_ms_nufft": ms_n,
+    dev_d, dev_n = device_ms(cfg, k)
+    out["N20000"] = {"M": len(k), "tiles": tiles, "gate": g,
+                     "ef_ms_direct_device": dev_d, "ef_ms_nufft_device": dev_n,
+                     "ef_ms_direct_host_path": ms_d, "ef_ms_nufft_host_path": ms_n,
                      "nufft_vs_direct_start": {"dE_over_E": abs(E_n - E_d) / abs(E_d),
                                                "max_dF_over_max_F": float(np.max(np.abs(F_n - F_d)) / np.max(np.abs(F_d)))},
                      "direct": solve(cfg, k, "direct"), "nufft": solve(cfg, k, "nufft_always")}
     cfg = RandomConfiguration(5, 100000, 2)
     k = GetKs(cfg, getK(0.4, 1.0, 2))
+    g = gate(cfg, k)
     ms_d, E_d, F_d, _ = ef_ms(cfg, k, "direct", 2)
     ms_n, E_n, F_n, _ = ef_ms(cfg, k, "nufft_always", 10)
+    dev_d, dev_n = device_ms(cfg, k)
     r = solve(cfg, k, "nufft")
-    out["N100000"] = {"M": len(k), "gate": gate(cfg, k), "ef_ms_direct": ms_d, "ef_ms_nufft": ms_n,
+    out["N100000"] = {"M": len(k), "gate": g, "ef_ms_direct_device": dev_d, "ef_ms_nufft_device": dev_n,
+                      "ef_ms_direct_host_path": ms_d, "ef_ms_nufft_host_path": ms_n,
                       "nufft_vs_direct_start": {"dE_over_E": abs(E_n - E_d) / abs(E_d),
                                                 "max_dF_over_max_F": float(np.max(np.abs(F_n - F_d)) / np.max(np.abs(F_d)))},
-                      "nufft": r, "direct_solve_estimate_s": ms_d * r["device_evals"] / 1000}
+                      "nufft": r,
+                      "direct_solve_estimate_s": (dev_d / 1000 + r["wall_s"] / max(r["device_evals"], 1)
+                                                  - dev_n / 1000) * r["device_evals"]}
     out["value"] = (1.0 if r["energy"] < args.emin else 0.0) / r["wall_s"]
     out["workload"] = "2D N=100000 chi=0.4 through the NUFFT (and N=20000 both ways)"
     return out
