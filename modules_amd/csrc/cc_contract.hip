@@ -54,6 +54,13 @@ __device__ __forceinline__ double head_dot(const int4 &h, const double4 &p)
     return __builtin_fma((double)h.x, p.x, __builtin_fma((double)h.y, p.y, __builtin_fma((double)h.z, p.z, (double)h.w * p.w)));
 }
 
+// the same for a head already in doubles (wavefront-uniform, from SGPRs); below 4 dimensions the fourth term is zero
+__device__ __forceinline__ double head_dot_d(const double4 &h, const double4 &p, int d)
+{
+    const double t = d == 4 ? __builtin_fma(h.z, p.z, h.w * p.w) : h.z * p.z;
+    return __builtin_fma(h.x, p.x, __builtin_fma(h.y, p.y, t));
+}
+
 }  // namespace
 
 constexpr int kRowSegs = 16;  // segments per rho wavefront
@@ -205,8 +212,6 @@ template <int SEG>
 struct ForceRowsLds {
     double2 tab[64];
     double2 coef[kForceStageRows * SEG];
-    int4 head[kForceStageRows];
-    int len[kForceStageRows];
 };
 
 // One segment of L slots (its length rounded up to a multiple of 2: a padding slot has zero coefficients) for the
@@ -215,7 +220,7 @@ struct ForceRowsLds {
 // phi run backwards from e_0, e_1).  Per slot and particle: c = A cos - B sin, sum c, sum t c, one step: 6 FP64
 // operations; the slot's coefficients are read once for the PPL particles.
 template <int SEG, int PPL, int L>
-__device__ __forceinline__ void force_rows_segment(const double2 *cf, int4 h, double hl, int d, const double4 (&p)[PPL],
+__device__ __forceinline__ void force_rows_segment(const double2 *cf, const double4 &h, double hl, int d, const double4 (&p)[PPL],
                                                    const double (&stx)[PPL], const double (&sty)[PPL],
                                                    const double (&k2)[PPL], const double2 *tab, double (&g)[PPL][4])
 {
@@ -226,7 +231,7 @@ __device__ __forceinline__ void force_rows_segment(const double2 *cf, int4 h, do
 #pragma unroll
     for (int q = 0; q < PPL; q++) {
         double sn0, cs0;
-        sincos_rev_tab(head_dot(h, p[q]), tab, sn0, cs0);
+        sincos_rev_tab(head_dot_d(h, p[q], d), tab, sn0, cs0);
         const double cs1 = __builtin_fma(cs0, stx[q], -sn0 * sty[q]);
         const double sn1 = __builtin_fma(cs0, sty[q], sn0 * stx[q]);
         const double k1 = stx[q] + stx[q];
@@ -266,23 +271,23 @@ __device__ __forceinline__ void force_rows_segment(const double2 *cf, int4 h, do
         if (d == 1) {
             g[q][0] += Al;
         } else if (d == 2) {
-            g[q][0] = __builtin_fma((double)h.x, A, g[q][0]);
+            g[q][0] = __builtin_fma(h.x, A, g[q][0]);
             g[q][1] += Al;
         } else if (d == 3) {
-            g[q][0] = __builtin_fma((double)h.x, A, g[q][0]);
-            g[q][1] = __builtin_fma((double)h.y, A, g[q][1]);
+            g[q][0] = __builtin_fma(h.x, A, g[q][0]);
+            g[q][1] = __builtin_fma(h.y, A, g[q][1]);
             g[q][2] += Al;
         } else {
-            g[q][0] = __builtin_fma((double)h.x, A, g[q][0]);
-            g[q][1] = __builtin_fma((double)h.y, A, g[q][1]);
-            g[q][2] = __builtin_fma((double)h.z, A, g[q][2]);
+            g[q][0] = __builtin_fma(h.x, A, g[q][0]);
+            g[q][1] = __builtin_fma(h.y, A, g[q][1]);
+            g[q][2] = __builtin_fma(h.z, A, g[q][2]);
             g[q][3] += Al;
         }
     }
 }
 
 template <int SEG, int PPL, int L = 2>
-__device__ __forceinline__ void force_rows_dispatch(int len, const double2 *cf, int4 h, double hl, int d,
+__device__ __forceinline__ void force_rows_dispatch(int len, const double2 *cf, const double4 &h, double hl, int d,
                                                     const double4 (&p)[PPL], const double (&stx)[PPL],
                                                     const double (&sty)[PPL], const double (&k2)[PPL],
                                                     const double2 *tab, double (&g)[PPL][4])
@@ -304,7 +309,7 @@ __global__ void __launch_bounds__(kBlock) k_force_rows(const SysDesc *__restrict
     const SysDesc &S = sys[it.sys];
     if (!sys_active(sys, S) || (S.flags & kSysMixed)) return;  // mixed systems: k_force_rows_mixed
     sincos_table_fill(L.tab);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int d = S.d;
     double4 p[PPL];
     double stx[PPL], sty[PPL], k2[PPL];
@@ -328,17 +333,14 @@ __global__ void __launch_bounds__(kBlock) k_force_rows(const SysDesc *__restrict
         const int ns = min(kForceStageRows, s1 - sb);
         __syncthreads();  // the previous group is consumed
         for (int x = threadIdx.x; x < ns * SEG; x += kBlock) L.coef[x] = coef[(int64_t)sb * SEG + x];
-        if (threadIdx.x < ns) {
-            L.head[threadIdx.x] = S.seg_head[sb + threadIdx.x];
-            L.len[threadIdx.x] = S.seg_len[sb + threadIdx.x];
-        }
         __syncthreads();
         for (int sl = wave; sl < ns; sl += 4) {
-            const int4 hv = L.head[sl];
-            const int4 h = make_int4(__builtin_amdgcn_readfirstlane(hv.x), __builtin_amdgcn_readfirstlane(hv.y),
-                                     __builtin_amdgcn_readfirstlane(hv.z), __builtin_amdgcn_readfirstlane(hv.w));
-            const int len = __builtin_amdgcn_readfirstlane(L.len[sl]);
-            const double hl = (d == 1) ? (double)h.x : (d == 2) ? (double)h.y : (d == 3) ? (double)h.z : (double)h.w;
+            // the segment's head (as doubles) and length by scalar loads: operands straight from SGPRs
+            const int sg = __builtin_amdgcn_readfirstlane(sb + sl);
+            const ConstPtr<double> hq = as_const(reinterpret_cast<const double *>(S.seg_head_d + sg));
+            const double4 h = make_double4(hq[0], hq[1], hq[2], hq[3]);
+            const int len = as_const(S.seg_len)[sg];
+            const double hl = (d == 1) ? h.x : (d == 2) ? h.y : (d == 3) ? h.z : h.w;
             force_rows_dispatch<SEG, PPL>(len, L.coef + sl * SEG, h, hl, d, p, stx, sty, k2, L.tab, g);
         }
     }
@@ -374,18 +376,16 @@ struct ForceRowsLds32 {
     double2 tab[64];
     float2 tab32[64];
     float2 coef[kForceStageRows * SEG];
-    int4 head[kForceStageRows];
-    int len[kForceStageRows];
 };
 
 template <int SEG, int L>
-__device__ __forceinline__ void force_rows_segment32(const float2 *cf, int4 h, double hl, int d, const double4 &p,
+__device__ __forceinline__ void force_rows_segment32(const float2 *cf, const double4 &h, double hl, int d, const double4 &p,
                                                      float stx, float sty, float k2, const float2 *tab32,
                                                      double (&g)[4])
 {
     float Rc[2][2], Rs[2][2], acc[2], accl[2];
     {
-        const double u = head_dot(h, p);
+        const double u = head_dot_d(h, p, d);
         float sn0, cs0;
         sincos_rev_tab32((float)(u - __builtin_rint(u)), tab32, sn0, cs0);
         const float cs1 = __builtin_fmaf(cs0, stx, -sn0 * sty);
@@ -421,22 +421,22 @@ __device__ __forceinline__ void force_rows_segment32(const float2 *cf, int4 h, d
     if (d == 1) {
         g[0] += Al;
     } else if (d == 2) {
-        g[0] = __builtin_fma((double)h.x, A, g[0]);
+        g[0] = __builtin_fma(h.x, A, g[0]);
         g[1] += Al;
     } else if (d == 3) {
-        g[0] = __builtin_fma((double)h.x, A, g[0]);
-        g[1] = __builtin_fma((double)h.y, A, g[1]);
+        g[0] = __builtin_fma(h.x, A, g[0]);
+        g[1] = __builtin_fma(h.y, A, g[1]);
         g[2] += Al;
     } else {
-        g[0] = __builtin_fma((double)h.x, A, g[0]);
-        g[1] = __builtin_fma((double)h.y, A, g[1]);
-        g[2] = __builtin_fma((double)h.z, A, g[2]);
+        g[0] = __builtin_fma(h.x, A, g[0]);
+        g[1] = __builtin_fma(h.y, A, g[1]);
+        g[2] = __builtin_fma(h.z, A, g[2]);
         g[3] += Al;
     }
 }
 
 template <int SEG, int L = 2>
-__device__ __forceinline__ void force_rows_dispatch32(int len, const float2 *cf, int4 h, double hl, int d,
+__device__ __forceinline__ void force_rows_dispatch32(int len, const float2 *cf, const double4 &h, double hl, int d,
                                                       const double4 &p, float stx, float sty, float k2,
                                                       const float2 *tab32, double (&g)[4])
 {
@@ -460,7 +460,7 @@ __global__ void __launch_bounds__(kBlock) k_force_rows_mixed(const SysDesc *__re
     sincos_table_fill(L.tab);
     __syncthreads();
     sincos_table_fill32(L.tab32, L.tab);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int d = S.d;
     const int i = it.t0 * 64 + lane;
     const bool ok = i < S.N;
@@ -477,17 +477,14 @@ __global__ void __launch_bounds__(kBlock) k_force_rows_mixed(const SysDesc *__re
         const int ns = min(kForceStageRows, s1 - sb);
         __syncthreads();  // the previous group is consumed (and the FP32 table written)
         for (int x = threadIdx.x; x < ns * SEG; x += kBlock) L.coef[x] = coef[(int64_t)sb * SEG + x];
-        if (threadIdx.x < ns) {
-            L.head[threadIdx.x] = S.seg_head[sb + threadIdx.x];
-            L.len[threadIdx.x] = S.seg_len[sb + threadIdx.x];
-        }
         __syncthreads();
         for (int sl = wave; sl < ns; sl += 4) {
-            const int4 hv = L.head[sl];
-            const int4 h = make_int4(__builtin_amdgcn_readfirstlane(hv.x), __builtin_amdgcn_readfirstlane(hv.y),
-                                     __builtin_amdgcn_readfirstlane(hv.z), __builtin_amdgcn_readfirstlane(hv.w));
-            const int len = __builtin_amdgcn_readfirstlane(L.len[sl]);
-            const double hl = (d == 1) ? (double)h.x : (d == 2) ? (double)h.y : (d == 3) ? (double)h.z : (double)h.w;
+            // the segment's head (as doubles) and length by scalar loads: operands straight from SGPRs
+            const int sg = __builtin_amdgcn_readfirstlane(sb + sl);
+            const ConstPtr<double> hq = as_const(reinterpret_cast<const double *>(S.seg_head_d + sg));
+            const double4 h = make_double4(hq[0], hq[1], hq[2], hq[3]);
+            const int len = as_const(S.seg_len)[sg];
+            const double hl = (d == 1) ? h.x : (d == 2) ? h.y : (d == 3) ? h.z : h.w;
             force_rows_dispatch32<SEG>(len, L.coef + sl * SEG, h, hl, d, p, stx, sty, k2, L.tab32, g);
         }
     }

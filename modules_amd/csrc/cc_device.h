@@ -49,6 +49,7 @@ struct SysDesc {
 
     // constraints
     const int4 *seg_head;   // lattice: start index of each segment (nseg)
+    const double4 *seg_head_d;  // lattice: the same as doubles (wavefront-uniform scalar loads in the force kernels)
     const int32_t *seg_len; // lattice: constraints in each segment (nseg), non-increasing
     const int32_t *slot_k;  // lattice: constraint index of each slot (nslots), -1 for padding
     const double *slot_v;   // weight V of each slot (nslots); padding slots hold 0
@@ -61,7 +62,7 @@ struct SysDesc {
     double *coef;    // nslots x 2: (2 V/vol) * (Im rho, Re rho)
     float *coef32;   // nslots x 2: the same rounded to FP32 (force terms of the mixed-precision path)
     double *epart;   // nslots: energy term of each slot (V |rho|^2, or V (S - S0)^2)
-    double *energy;  // [0] total, [1] CC part, [2] soft-core part
+    double *energy;  // [0] total, [1] CC part, [2] soft-core part, [3] non-finite gradient flag
 
     // forces
     double *fpart;   // kchunks x N x 4 partials
@@ -104,5 +105,16 @@ __device__ __forceinline__ bool sys_active(const SysDesc *sys, const SysDesc &S)
 struct WorkItem {
     int32_t sys, t0, t1, t2;
 };
+
+// Read-only tables that are the same for every lane of a wavefront (segment heads and lengths) are read through
+// constant-address-space pointers: with a wavefront-uniform index the compiler then uses scalar loads (one fetch
+// per wavefront, operands from SGPRs) although the kernel writes other global memory.
+template <class T>
+using ConstPtr = const __attribute__((address_space(4))) T *;
+template <class T>
+__device__ __forceinline__ ConstPtr<T> as_const(const T *p)
+{
+    return (ConstPtr<T>)p;
+}
 
 }  // namespace ccgpu

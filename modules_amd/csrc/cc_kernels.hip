@@ -647,17 +647,6 @@ __global__ void __launch_bounds__(kBlock) k_mixture_final(const SysDesc *__restr
 // segment data is uniform across the wavefront (scalar loads).  Accumulates G_j = sum_k c_k n_kj with
 // c_k = (2V/vol)(Im rho cos - Re rho sin); then F = -sum_j G_j b_j in k_force_final.
 // ---------------------------------------------------------------------------------------------
-// Read-only tables that are the same for every lane of a wavefront (segment heads, lengths, force
-// coefficients) are read through constant-address-space pointers: the compiler may then use scalar loads
-// (one fetch per wavefront, operands from SGPRs) although the kernel writes other global memory.
-template <class T>
-using ConstPtr = const __attribute__((address_space(4))) T *;
-template <class T>
-__device__ __forceinline__ ConstPtr<T> as_const(const T *p)
-{
-    return (ConstPtr<T>)p;
-}
-
 // Segments are staged through LDS in groups of kForceStage: heads, lengths and coefficients are loaded once per
 // block with vector loads and read back as wavefront-uniform LDS broadcasts.
 constexpr int kForceStage = 16;

@@ -260,6 +260,7 @@ std::shared_ptr<const ConstraintLayout> GetLayout(int d, const double *a, int64_
         L->plan = BuildSegments(d, M, n.data(), S);
         const int64_t nslots = L->plan.nseg * S;
         L->seg_head.Reserve(L->plan.nseg);
+        L->seg_head_d.Reserve(L->plan.nseg);
         L->seg_len.Reserve(L->plan.nseg);
         L->slot_k.Reserve(nslots);
         L->slot_v.Reserve(nslots);
@@ -267,6 +268,12 @@ std::shared_ptr<const ConstraintLayout> GetLayout(int d, const double *a, int64_
         for (int64_t q = 0; q < nslots; q++)
             if (L->plan.slot_k[q] >= 0) sv[q] = v[L->plan.slot_k[q]];
         CC_HIP(hipMemcpyAsync(L->seg_head.get(), L->plan.head.data(), L->plan.nseg * sizeof(int4), hipMemcpyHostToDevice, stream));
+        std::vector<double4> hd(L->plan.nseg);
+        for (int64_t q = 0; q < L->plan.nseg; q++)
+            hd[q] = make_double4(L->plan.head[q * 4], L->plan.head[q * 4 + 1], L->plan.head[q * 4 + 2],
+                                 L->plan.head[q * 4 + 3]);
+        CC_HIP(hipMemcpyAsync(L->seg_head_d.get(), hd.data(), L->plan.nseg * sizeof(double4), hipMemcpyHostToDevice,
+                              stream));
         CC_HIP(hipMemcpyAsync(L->seg_len.get(), L->plan.len.data(), L->plan.nseg * sizeof(int32_t), hipMemcpyHostToDevice, stream));
         CC_HIP(hipMemcpyAsync(L->slot_k.get(), L->plan.slot_k.data(), nslots * sizeof(int32_t), hipMemcpyHostToDevice, stream));
         CC_HIP(hipMemcpyAsync(L->slot_v.get(), sv.data(), nslots * sizeof(double), hipMemcpyHostToDevice, stream));
@@ -563,6 +570,7 @@ void System::Prepare(hipStream_t stream)
     D.cart = owner->cart.get() + off;
     D.step = owner->step.get() + off;
     D.seg_head = layout->seg_head.get();
+    D.seg_head_d = layout->seg_head_d.get();
     D.seg_len = layout->seg_len.get();
     D.slot_k = layout->slot_k.get();
     D.slot_v = layout->slot_v.get();

@@ -155,13 +155,15 @@ struct ConstraintLayout {
     bool lattice = false;
     SegmentPlan plan;
     DevArray<int4> seg_head;
+    DevArray<double4> seg_head_d;
     DevArray<int32_t> seg_len, slot_k;
     DevArray<double> slot_v, slot_s0;
     DevArray<double4> kvec;
     int64_t nslots() const { return lattice ? plan.nseg * S : (int64_t)v.size(); }
     size_t DeviceBytes() const
     {
-        return seg_head.bytes() + seg_len.bytes() + slot_k.bytes() + slot_v.bytes() + slot_s0.bytes() + kvec.bytes();
+        return seg_head.bytes() + seg_head_d.bytes() + seg_len.bytes() + slot_k.bytes() + slot_v.bytes() +
+               slot_s0.bytes() + kvec.bytes();
     }
 };
 // s0: per-constraint targets of a target-S(k) set, or null
