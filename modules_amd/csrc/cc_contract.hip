@@ -185,7 +185,7 @@ __device__ __forceinline__ void rho_rows_dispatch(int wlen, const SysDesc &S, co
 }
 
 template <int SEG>
-__global__ void __launch_bounds__(64) k_rho_rows(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_rho_rows(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
 {
     static_assert(SEG % 4 == 0, "segment length must be a multiple of 4");
     __shared__ RhoRowsLds lds;

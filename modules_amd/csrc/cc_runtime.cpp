@@ -189,7 +189,7 @@ Tiles DefaultTiles(int d, int64_t N, int64_t M)
     Tiles tuned;
     if (LookupTiles(d, N, M, tuned)) return tuned;
     Tiles t;
-    t.seg = d == 2 ? 32 : 24;  // 2D runs are long (up to twice the k-disk radius in index units)
+    t.seg = 24;  // 3D runs average ~14 slots; in 2D 24 keeps the rho kernel at three waves per SIMD (32: two)
     t.rho_chunk = 0;    // resolved in Prepare from the pair count and the segment count
     t.force_chunk = 0;  // likewise
     t.rho_kernel = 2;   // the particle-row contractions
