@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--maxstep", type=int, default=10000)
     ap.add_argument("--history", type=int, default=40)
     ap.add_argument("--check-interval", type=int, default=16)
+    ap.add_argument("--slots", type=int, default=0,
+                    help="solver slots (0: every timed job at once); fewer slots admit the queued jobs as others finish")
     ap.add_argument("--resident", default="all", choices=["all", "step"],
                     help="all: the timed batches are admitted together and advance in the same launches "
                          "(continuous batching, one tail at the end); step: one batch at a time")
@@ -165,6 +167,8 @@ def headline(args):
     K, W = args.steps, args.warmup
     resident_all = args.resident == "all"
     slots = B * (max(K, W, 1) if resident_all else 1)
+    if args.slots > 0:
+        slots = args.slots
     solver = BatchSolver(max_slots=slots, check_interval=args.check_interval, graph=args.graph,
                          precision=args.precision, lanes=args.lanes)
 

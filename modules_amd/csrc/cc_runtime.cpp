@@ -985,8 +985,21 @@ double Autotune(int d, int64_t N, const double *basis, int64_t M, const double *
         S.N = N;
         b.systems.push_back(&S);
     }
-    // candidate shapes: segment length x rho kernel x rho chunk x force chunk (0 = automatic)
+    // candidate shapes: the particle-row kernels (segment length x rho chunk, automatic force chunks, one particle
+    // per lane), then the grouped-power kernels (segment length x rho kernel x rho chunk x force chunk x particles
+    // per lane); a chunk of 0 keeps the automatic rule
     std::vector<Tiles> cands;
+    for (int seg : {16, 24, 32})
+        for (int rc : {0, 256, 1024}) {
+            if (rc > 2 * N) continue;
+            Tiles t;
+            t.seg = seg;
+            t.rho_kernel = 2;
+            t.rho_chunk = rc;
+            t.force_chunk = 0;
+            t.force_ppl = 1;
+            cands.push_back(t);
+        }
     for (int seg : {8, 16, 24})
         for (int rk : {0, 1})
             for (int rc : {128, 256, 512})
