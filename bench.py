@@ -73,8 +73,10 @@ def parse():
                          "large 2D systems through the NUFFT backend against the direct sums; targets: target-S(k) "
                          "(equiluminous) relaxes and a variable-cell solve; md: molecular dynamics steps")
     ap.add_argument("--batch", type=int, default=64, help="seeds64 / five / targets: configurations per step")
-    ap.add_argument("--requests", type=int, default=96, help="serve: number of requests")
-    ap.add_argument("--rate", type=float, default=200.0, help="serve: arrivals per second")
+    ap.add_argument("--requests", type=int, default=2048, help="serve: number of requests")
+    ap.add_argument("--serve-slots", type=int, default=1024, help="serve: solver slots (resident requests at most)")
+    ap.add_argument("--rate", type=float, default=2000.0,
+                    help="serve: arrivals per second (the default saturates the GPU: requests queue)")
     ap.add_argument("--profiler-run", action="store_true",
                     help="for runs under a counter profiler: no hipEvent timing (roofline null), no sub-records")
     ap.add_argument("--profile-evals", type=int, default=0,
@@ -388,7 +390,8 @@ def sub_serving(args, requests=None, rate=None):
     requests = requests or args.requests
     rate = rate or args.rate
     serve(make_stream(8, rate, seed=99), Emin=args.emin, MaxStep=args.maxstep, max_slots=64)  # warm
-    reqs, wall = serve(make_stream(requests, rate), Emin=args.emin, MaxStep=args.maxstep, max_slots=256)
+    reqs, wall = serve(make_stream(requests, rate), Emin=args.emin, MaxStep=args.maxstep, max_slots=args.serve_slots,
+                       graph=args.graph)
     lat = np.array([r.done - r.arrival for r in reqs])
     solved = sum(1 for r in reqs if r.result.get("energy", 1.0) < args.emin)
     return {"workload": f"serving: {requests} Poisson requests at {rate}/s, soft core e0=1 sigma=0.4",

@@ -64,9 +64,10 @@ def make_batch(count=256, master_seed=20261203, N_range=(100, 4000), dims=(2, 3)
     return reqs
 
 
-def serve(reqs, Emin=1e-18, MaxStep=10000, max_slots=64, hbm_budget=200 << 30, step_evals=16, timeout=600):
+def serve(reqs, Emin=1e-18, MaxStep=10000, max_slots=64, hbm_budget=200 << 30, step_evals=16, timeout=600,
+          graph=False):
     """Run the stream in real time.  Returns the requests with results, and the wall time."""
-    solver = BatchSolver(max_slots=max_slots, hbm_budget=hbm_budget, check_interval=step_evals)
+    solver = BatchSolver(max_slots=max_slots, hbm_budget=hbm_budget, check_interval=step_evals, graph=graph)
     pending = sorted(reqs, key=lambda r: r.arrival)
     inflight = {}
     t0 = time.perf_counter()
