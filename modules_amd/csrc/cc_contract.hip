@@ -54,11 +54,11 @@ __device__ __forceinline__ double head_dot(const int4 &h, const double4 &p)
     return __builtin_fma((double)h.x, p.x, __builtin_fma((double)h.y, p.y, __builtin_fma((double)h.z, p.z, (double)h.w * p.w)));
 }
 
-// the same for a head already in doubles (wavefront-uniform, from SGPRs); below 4 dimensions the fourth term is zero
-__device__ __forceinline__ double head_dot_d(const double4 &h, const double4 &p, int d)
+// the same for a head already in doubles (wavefront-uniform, from SGPRs); the components beyond the dimension are
+// zero in both the head and the position, so their terms add exact zeros
+__device__ __forceinline__ double head_dot_d(const double4 &h, const double4 &p)
 {
-    const double t = d == 4 ? __builtin_fma(h.z, p.z, h.w * p.w) : h.z * p.z;
-    return __builtin_fma(h.x, p.x, __builtin_fma(h.y, p.y, t));
+    return __builtin_fma(h.x, p.x, __builtin_fma(h.y, p.y, __builtin_fma(h.z, p.z, h.w * p.w)));
 }
 
 }  // namespace
@@ -90,50 +90,56 @@ __device__ __forceinline__ void rho_rows_run(const SysDesc &S, const WorkItem &i
     const int j0 = it.t1 * S.pchunk_len;
     const int j1 = min(S.N, j0 + S.pchunk_len);
     const bool weighted = S.weight != nullptr;
+    // one particle of the lane's row: its head phase, then the slots.  W: scale the head phase by the staged weight
+    // (weighted systems, and the iteration whose rows run past the staged particles, where the weight is 0)
+    auto particle = [&](int j, auto W) {
+        const double4 p = lds.pos[j];
+        const double2 st = lds.step[j];
+        double sn, cs;
+        sincos_rev_tab(head_dot(h, p), lds.tab, sn, cs);
+        // e_t lives in register pair (x0, y0) for even t and (x1, y1) for odd t, each updated in place by
+        // e_{t+1} = 2 cos(phi) e_t - e_{t-1}; the weight rides along the (linear) recurrence
+        double x0 = cs, y0 = sn;
+        if constexpr (decltype(W)::value) {
+            const double w = lds.wgt[j];
+            x0 = cs * w;
+            y0 = sn * w;
+        }
+        double x1 = __builtin_fma(x0, st.x, -y0 * st.y);
+        double y1 = __builtin_fma(x0, st.y, y0 * st.x);
+        const double k1 = st.x + st.x;
+        re[0] += x0;
+        im[0] += y0;
+#pragma unroll
+        for (int t = 1; t < L; t++) {
+            if (t & 1) {
+                if (t > 1) {
+                    x1 = __builtin_fma(k1, x0, -x1);
+                    y1 = __builtin_fma(k1, y0, -y1);
+                }
+                re[t] += x1;
+                im[t] += y1;
+            } else {
+                x0 = __builtin_fma(k1, x1, -x0);
+                y0 = __builtin_fma(k1, y1, -y0);
+                re[t] += x0;
+                im[t] += y0;
+            }
+        }
+    };
     for (int jb = j0; jb < j1; jb += kRowSub) {
         const int nsub = min(kRowSub, j1 - jb);
         __syncthreads();
-        if (lane < nsub) {
-            lds.pos[lane] = S.pos[jb + lane];
-            lds.step[lane] = S.step[jb + lane];
-            lds.wgt[lane] = weighted ? S.weight[jb + lane] : 1.0;
-        }
+        // every staged entry is defined: past the chunk's end a particle at the origin with weight 0
+        const bool in = lane < nsub;
+        lds.pos[lane] = in ? S.pos[jb + lane] : make_double4(0, 0, 0, 0);
+        lds.step[lane] = in ? S.step[jb + lane] : make_double2(1, 0);
+        lds.wgt[lane] = in ? (weighted ? S.weight[jb + lane] : 1.0) : 0.0;
         __syncthreads();
         const int iters = (nsub + 3) >> 2;
-        for (int q = 0; q < iters; q++) {
-            const int j = 4 * q + row;
-            const bool ok = j < nsub && valid;
-            const int jj = j < nsub ? j : 0;
-            const double4 p = lds.pos[jj];
-            const double2 st = lds.step[jj];
-            const double w = ok ? lds.wgt[jj] : 0.0;
-            double sn, cs;
-            sincos_rev_tab(head_dot(h, p), lds.tab, sn, cs);
-            // e_t lives in register pair (x0, y0) for even t and (x1, y1) for odd t, each updated in place by
-            // e_{t+1} = 2 cos(phi) e_t - e_{t-1}; the weight rides along the (linear) recurrence
-            double x0 = cs * w, y0 = sn * w;
-            double x1 = __builtin_fma(x0, st.x, -y0 * st.y);
-            double y1 = __builtin_fma(x0, st.y, y0 * st.x);
-            const double k1 = st.x + st.x;
-            re[0] += x0;
-            im[0] += y0;
-#pragma unroll
-            for (int t = 1; t < L; t++) {
-                if (t & 1) {
-                    if (t > 1) {
-                        x1 = __builtin_fma(k1, x0, -x1);
-                        y1 = __builtin_fma(k1, y0, -y1);
-                    }
-                    re[t] += x1;
-                    im[t] += y1;
-                } else {
-                    x0 = __builtin_fma(k1, x1, -x0);
-                    y0 = __builtin_fma(k1, y1, -y0);
-                    re[t] += x0;
-                    im[t] += y0;
-                }
-            }
-        }
+        const int nplain = weighted ? 0 : nsub >> 2;  // iterations with all four rows on staged particles
+        for (int q = 0; q < nplain; q++) particle(4 * q + row, std::false_type{});
+        for (int q = nplain; q < iters; q++) particle(4 * q + row, std::true_type{});
     }
     // sum the four rows: after the first step rows 0, 2 hold slots t and rows 1, 3 slots t + SEG/2 (pairs of rows
     // added); after the second, row 0 holds the totals of slots [0, SEG/4), row 2 of [SEG/4, SEG/2), row 1 of
@@ -231,7 +237,7 @@ __device__ __forceinline__ void force_rows_segment(const double2 *cf, const doub
 #pragma unroll
     for (int q = 0; q < PPL; q++) {
         double sn0, cs0;
-        sincos_rev_tab(head_dot_d(h, p[q], d), tab, sn0, cs0);
+        sincos_rev_tab(head_dot_d(h, p[q]), tab, sn0, cs0);
         const double cs1 = __builtin_fma(cs0, stx[q], -sn0 * sty[q]);
         const double sn1 = __builtin_fma(cs0, sty[q], sn0 * stx[q]);
         const double k1 = stx[q] + stx[q];
@@ -255,8 +261,14 @@ __device__ __forceinline__ void force_rows_segment(const double2 *cf, const doub
         for (int q = 0; q < PPL; q++) {
             const double cc = Rc[q][ch][(m + 1) & 1], ss = Rs[q][ch][(m + 1) & 1];
             const double c = __builtin_fma(ab.x, cc, -ab.y * ss);
-            acc[q][ch] += c;
-            accl[q][ch] = __builtin_fma((double)t, c, accl[q][ch]);
+            // the first terms start the sums (t c is 0, c and 2 c exactly; no adds of zero)
+            if (t < 2) {
+                acc[q][ch] = c;
+                if (t == 1) accl[q][1] = c;
+            } else {
+                acc[q][ch] += c;
+                accl[q][ch] = t == 2 ? 2.0 * c : __builtin_fma((double)t, c, accl[q][ch]);
+            }
             if (t + 2 < L) {
                 Rc[q][ch][m & 1] = __builtin_fma(k2[q], cc, -Rc[q][ch][m & 1]);
                 Rs[q][ch][m & 1] = __builtin_fma(k2[q], ss, -Rs[q][ch][m & 1]);
@@ -385,7 +397,7 @@ __device__ __forceinline__ void force_rows_segment32(const float2 *cf, const dou
 {
     float Rc[2][2], Rs[2][2], acc[2], accl[2];
     {
-        const double u = head_dot_d(h, p, d);
+        const double u = head_dot_d(h, p);
         float sn0, cs0;
         sincos_rev_tab32((float)(u - __builtin_rint(u)), tab32, sn0, cs0);
         const float cs1 = __builtin_fmaf(cs0, stx, -sn0 * sty);

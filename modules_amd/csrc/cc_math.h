@@ -53,6 +53,16 @@ __device__ __forceinline__ void sincos_rev(double u, double &s, double &c)
     c = ((q + 1) & 2) ? -cc : cc;
 }
 
+// a * b + c for a constant c (wavefront-uniform, held in SGPRs), always in the three-operand encoding.  Left to
+// itself the compiler picks the two-operand form, whose destination is the addend, and then copies the constant
+// into the destination register first: one extra VALU move per use in the phase loops.
+__device__ __forceinline__ double fma_sc(double a, double b, double c)
+{
+    double d;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "s"(c));
+    return d;
+}
+
 // Table of exp(2 pi i t / 64), t = 0..63, for sincos_rev_tab: threads 0..63 of the block each write one
 // entry (sincos_rev, a few ulp); the caller synchronises before use.
 __device__ __forceinline__ void sincos_table_fill(double2 *tab)
@@ -66,21 +76,27 @@ __device__ __forceinline__ void sincos_table_fill(double2 *tab)
 
 // sin and cos of 2 pi u through the 64-entry table: u = j/64 + f_lo with |f_lo| <= 1/128 (exact split), then
 // exp(2 pi i u) = T[j mod 64] * exp(2 pi i f_lo) with short Taylor series on |x| = |2 pi f_lo| <= pi/64:
-// sin to x^7 (next term below 5e-18), cos to x^8 (next term below 3e-20).  About 22 VALU operations and one
+// sin to x^7 (next term below 5e-18), cos to x^8 (next term below 3e-20).  About 19 VALU operations and one
 // LDS read, against about 36 operations for sincos_rev.
+//
+// The split takes j = rint(64 u) from the sum 64 u + 1.5 * 2^52 (valid for |u| < 2^45): its rounding to an integer
+// is rint's (ties to even, the constant being even), j = sum - constant is exact, and the low word of the sum holds
+// j mod 2^32 in two's complement, so j mod 64 needs no conversion.  f_lo = u - j/64 is exact (the same value as
+// reducing u to [-1/2, 1/2] first and splitting that: the two j differ by a multiple of 64).
 __device__ __forceinline__ void sincos_rev_tab(double u, const double2 *tab, double &s, double &c)
 {
-    const double f = u - __builtin_rint(u);
-    const double jd = __builtin_rint(64.0 * f);
-    const double2 t = tab[((int)jd) & 63];  // read first: the polynomials below cover its latency
-    const double fl = __builtin_fma(-0.015625, jd, f);  // exact (Sterbenz)
+    constexpr double kRound = 6755399441055744.0;  // 1.5 * 2^52
+    const double sum = fma_sc(u, 64.0, kRound);
+    const double jd = sum - kRound;
+    const double2 t = tab[__double2loint(sum) & 63];  // read first: the polynomials below cover its latency
+    const double fl = __builtin_fma(-0.015625, jd, u);  // exact
     const double x = fl * CC_TWO_PI_HI;
     const double x2 = x * x;
-    double ps = __builtin_fma(x2, -1.9841269841269841270e-04, 8.3333333333333333333e-03);
-    ps = __builtin_fma(ps, x2, -1.6666666666666666667e-01);
+    double ps = fma_sc(x2, -1.9841269841269841270e-04, 8.3333333333333333333e-03);
+    ps = fma_sc(ps, x2, -1.6666666666666666667e-01);
     const double sl = __builtin_fma(ps * x2, x, x);
-    double pc = __builtin_fma(x2, 2.4801587301587301587e-05, -1.3888888888888888889e-03);
-    pc = __builtin_fma(pc, x2, 4.1666666666666666667e-02);
+    double pc = fma_sc(x2, 2.4801587301587301587e-05, -1.3888888888888888889e-03);
+    pc = fma_sc(pc, x2, 4.1666666666666666667e-02);
     pc = __builtin_fma(pc, x2, -0.5);
     const double cl = __builtin_fma(pc, x2, 1.0);
     c = __builtin_fma(t.x, cl, -t.y * sl);
