@@ -221,65 +221,57 @@ struct ForceRowsLds {
 };
 
 // One segment of L slots (its length rounded up to a multiple of 2: a padding slot has zero coefficients) for the
-// lane's PPL particles.  Two chains per particle (even and odd slots), each stepped by the three-term recurrence in
-// 2 phi, e_{t+2} = 2 cos(2 phi) e_t - e_{t-2}, in place in two register pairs, from e_{-2}, e_{-1} (the recurrence in
-// phi run backwards from e_0, e_1).  Per slot and particle: c = A cos - B sin, sum c, sum t c, one step: 6 FP64
-// operations; the slot's coefficients are read once for the PPL particles.
+// lane's PPL particles.  One phase chain per particle, e_{t+1} = 2 cos(phi) e_t - e_{t-1}, in place in two register
+// pairs (even and odd t) from e_0 (the head's table sincos) and e_1 = e_0 step.  Per slot and particle:
+// c = A cos - B sin, sum c, sum t c, one step: 6 FP64 operations; the slot's coefficients are read once for the PPL
+// particles.
 template <int SEG, int PPL, int L>
 __device__ __forceinline__ void force_rows_segment(const double2 *cf, const double4 &h, double hl, int d, const double4 (&p)[PPL],
                                                    const double (&stx)[PPL], const double (&sty)[PPL],
-                                                   const double (&k2)[PPL], const double2 *tab, double (&g)[PPL][4])
+                                                   const double (&k1)[PPL], const double2 *tab, double (&g)[PPL][4])
 {
-    // chain ch of particle q, register pair r: R[q][ch][r]; at slot t = 2 m + ch the current phase is
-    // R[q][ch][(m + 1) & 1], and R[q][ch][m & 1] then becomes the phase two slots later
-    double Rc[PPL][2][2], Rs[PPL][2][2];
-    double acc[PPL][2], accl[PPL][2];
+    double x0[PPL], y0[PPL], x1[PPL], y1[PPL];  // e_t for even t, for odd t
+    double acc[PPL], accl[PPL];
 #pragma unroll
     for (int q = 0; q < PPL; q++) {
-        double sn0, cs0;
-        sincos_rev_tab(head_dot_d(h, p[q]), tab, sn0, cs0);
-        const double cs1 = __builtin_fma(cs0, stx[q], -sn0 * sty[q]);
-        const double sn1 = __builtin_fma(cs0, sty[q], sn0 * stx[q]);
-        const double k1 = stx[q] + stx[q];
-        const double cm1 = __builtin_fma(k1, cs0, -cs1);  // e_{-1} = 2 cos(phi) e_0 - e_1
-        const double sm1 = __builtin_fma(k1, sn0, -sn1);
-        Rc[q][0][0] = __builtin_fma(k1, cm1, -cs0);  // e_{-2} = 2 cos(phi) e_{-1} - e_0
-        Rs[q][0][0] = __builtin_fma(k1, sm1, -sn0);
-        Rc[q][0][1] = cs0;
-        Rs[q][0][1] = sn0;
-        Rc[q][1][0] = cm1;
-        Rs[q][1][0] = sm1;
-        Rc[q][1][1] = cs1;
-        Rs[q][1][1] = sn1;
-        acc[q][0] = acc[q][1] = accl[q][0] = accl[q][1] = 0.0;
+        sincos_rev_tab(head_dot_d(h, p[q]), tab, y0[q], x0[q]);
+        x1[q] = __builtin_fma(x0[q], stx[q], -y0[q] * sty[q]);
+        y1[q] = __builtin_fma(x0[q], sty[q], y0[q] * stx[q]);
+        acc[q] = accl[q] = 0.0;
     }
 #pragma unroll
     for (int t = 0; t < L; t++) {
-        const int ch = t & 1, m = t >> 1;
         const double2 ab = cf[t];
 #pragma unroll
         for (int q = 0; q < PPL; q++) {
-            const double cc = Rc[q][ch][(m + 1) & 1], ss = Rs[q][ch][(m + 1) & 1];
-            const double c = __builtin_fma(ab.x, cc, -ab.y * ss);
-            // the first terms start the sums (t c is 0, c and 2 c exactly; no adds of zero)
-            if (t < 2) {
-                acc[q][ch] = c;
-                if (t == 1) accl[q][1] = c;
-            } else {
-                acc[q][ch] += c;
-                accl[q][ch] = t == 2 ? 2.0 * c : __builtin_fma((double)t, c, accl[q][ch]);
+            if (t >= 2) {
+                if (t & 1) {
+                    x1[q] = __builtin_fma(k1[q], x0[q], -x1[q]);
+                    y1[q] = __builtin_fma(k1[q], y0[q], -y1[q]);
+                } else {
+                    x0[q] = __builtin_fma(k1[q], x1[q], -x0[q]);
+                    y0[q] = __builtin_fma(k1[q], y1[q], -y0[q]);
+                }
             }
-            if (t + 2 < L) {
-                Rc[q][ch][m & 1] = __builtin_fma(k2[q], cc, -Rc[q][ch][m & 1]);
-                Rs[q][ch][m & 1] = __builtin_fma(k2[q], ss, -Rs[q][ch][m & 1]);
+            const double cc = (t & 1) ? x1[q] : x0[q], ss = (t & 1) ? y1[q] : y0[q];
+            const double c = __builtin_fma(ab.x, cc, -ab.y * ss);
+            // the first terms start the sums (t c is 0, then c exactly; no adds of zero)
+            if (t == 0) {
+                acc[q] = c;
+            } else if (t == 1) {
+                acc[q] += c;
+                accl[q] = c;
+            } else {
+                acc[q] += c;
+                accl[q] = __builtin_fma((double)t, c, accl[q]);
             }
         }
     }
 #pragma unroll
     for (int q = 0; q < PPL; q++) {
         // sum_t (hl + t) c_t = hl sum c + sum t c; leading components are constant along the segment
-        const double A = acc[q][0] + acc[q][1];
-        const double Al = __builtin_fma(hl, A, accl[q][0] + accl[q][1]);
+        const double A = acc[q];
+        const double Al = __builtin_fma(hl, A, accl[q]);
         if (d == 1) {
             g[q][0] += Al;
         } else if (d == 2) {
@@ -301,14 +293,14 @@ __device__ __forceinline__ void force_rows_segment(const double2 *cf, const doub
 template <int SEG, int PPL, int L = 2>
 __device__ __forceinline__ void force_rows_dispatch(int len, const double2 *cf, const double4 &h, double hl, int d,
                                                     const double4 (&p)[PPL], const double (&stx)[PPL],
-                                                    const double (&sty)[PPL], const double (&k2)[PPL],
+                                                    const double (&sty)[PPL], const double (&k1)[PPL],
                                                     const double2 *tab, double (&g)[PPL][4])
 {
     if constexpr (L >= SEG) {
-        force_rows_segment<SEG, PPL, SEG>(cf, h, hl, d, p, stx, sty, k2, tab, g);
+        force_rows_segment<SEG, PPL, SEG>(cf, h, hl, d, p, stx, sty, k1, tab, g);
     } else {
-        if (len <= L) force_rows_segment<SEG, PPL, L>(cf, h, hl, d, p, stx, sty, k2, tab, g);
-        else force_rows_dispatch<SEG, PPL, L + 2>(len, cf, h, hl, d, p, stx, sty, k2, tab, g);
+        if (len <= L) force_rows_segment<SEG, PPL, L>(cf, h, hl, d, p, stx, sty, k1, tab, g);
+        else force_rows_dispatch<SEG, PPL, L + 2>(len, cf, h, hl, d, p, stx, sty, k1, tab, g);
     }
 }
 
@@ -324,7 +316,7 @@ __global__ void __launch_bounds__(kBlock) k_force_rows(const SysDesc *__restrict
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int d = S.d;
     double4 p[PPL];
-    double stx[PPL], sty[PPL], k2[PPL];
+    double stx[PPL], sty[PPL], k1[PPL];
     double g[PPL][4];
 #pragma unroll
     for (int q = 0; q < PPL; q++) {
@@ -334,8 +326,7 @@ __global__ void __launch_bounds__(kBlock) k_force_rows(const SysDesc *__restrict
         const double2 st = ok ? S.step[i] : make_double2(1, 0);
         stx[q] = st.x;
         sty[q] = st.y;
-        const double c2 = st.x * st.x - st.y * st.y;  // cos(2 phi)
-        k2[q] = c2 + c2;
+        k1[q] = st.x + st.x;  // 2 cos(phi)
         g[q][0] = g[q][1] = g[q][2] = g[q][3] = 0.0;
     }
     const int s0 = it.t1 * S.kchunk_len;
@@ -353,7 +344,7 @@ __global__ void __launch_bounds__(kBlock) k_force_rows(const SysDesc *__restrict
             const double4 h = make_double4(hq[0], hq[1], hq[2], hq[3]);
             const int len = as_const(S.seg_len)[sg];
             const double hl = (d == 1) ? h.x : (d == 2) ? h.y : (d == 3) ? h.z : h.w;
-            force_rows_dispatch<SEG, PPL>(len, L.coef + sl * SEG, h, hl, d, p, stx, sty, k2, L.tab, g);
+            force_rows_dispatch<SEG, PPL>(len, L.coef + sl * SEG, h, hl, d, p, stx, sty, k1, L.tab, g);
         }
     }
     // the four wavefronts' sums, added in wavefront order
