@@ -309,6 +309,7 @@ __global__ void __launch_bounds__(kBlock) k_force_rows(const SysDesc *__restrict
 {
     __shared__ ForceRowsLds<SEG> L;
     __shared__ double4 gsum[4][PPL][64];
+    __shared__ double ered[kBlock];
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
     if (!sys_active(sys, S) || (S.flags & kSysMixed)) return;  // mixed systems: k_force_rows_mixed
@@ -364,6 +365,8 @@ __global__ void __launch_bounds__(kBlock) k_force_rows(const SysDesc *__restrict
             reinterpret_cast<double4 *>(S.fpart)[(int64_t)it.t1 * S.N + i] = make_double4(G[0], G[1], G[2], G[3]);
         }
     }
+    // the system's energy (its slot terms are complete before this pass): the first block of each system
+    if ((S.flags & kSysEnergyInForce) && it.t0 == 0 && it.t1 == 0) energy_finish(S, ered);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -457,6 +460,7 @@ __global__ void __launch_bounds__(kBlock) k_force_rows_mixed(const SysDesc *__re
 {
     __shared__ ForceRowsLds32<SEG> L;
     __shared__ double4 gsum[4][64];
+    __shared__ double ered[kBlock];
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
     if (!sys_active(sys, S) || !(S.flags & kSysMixed)) return;
@@ -504,6 +508,7 @@ __global__ void __launch_bounds__(kBlock) k_force_rows_mixed(const SysDesc *__re
             reinterpret_cast<double4 *>(S.fpart)[(int64_t)it.t1 * S.N + ii] = make_double4(G[0], G[1], G[2], G[3]);
         }
     }
+    if ((S.flags & kSysEnergyInForce) && it.t0 == 0 && it.t1 == 0) energy_finish(S, ered);
 }
 
 #define CC_INSTANTIATE_ROWS(SEG)                                                                              \

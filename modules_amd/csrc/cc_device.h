@@ -27,6 +27,8 @@ enum SysFlags : int32_t {
     kSysMultiConfig = 512,  // mixture of equal sub-configurations sharing one target-S(k) energy of the mean S
     kSysRhoFinal = 1024,    // the rho contraction covers every particle in one chunk and finishes each slot itself
     kSysForceFinal = 2048,  // the force contraction covers every segment in one chunk and finishes each particle
+    kSysEnergyInForce = 4096,  // an E+F evaluation forms the energy in the force contraction (k_energy_final runs
+                               // only for energy-only evaluations)
 };
 
 struct SysDesc {

@@ -92,4 +92,53 @@ __device__ __forceinline__ void force_finish(const SysDesc *__restrict__ sys, co
     }
 }
 
+// block-wide sum of one double per thread (kBlock threads), fixed tree order; every thread gets the result
+__device__ __forceinline__ double block_sum(double v, double *lds)
+{
+    const int tid = threadIdx.x;
+    lds[tid] = v;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (tid < w) lds[tid] = lds[tid] + lds[tid + w];
+        __syncthreads();
+    }
+    double r = lds[0];
+    __syncthreads();
+    return r;
+}
+
+// v[t] + v[t + B] + v[t + 2B] + ... for thread t (B = kBlock), in that order, with eight loads in flight
+__device__ __forceinline__ double strided_sum(const double *__restrict__ v, int n)
+{
+    double acc = 0.0;
+    int q = threadIdx.x;
+    for (; q + 7 * kBlock < n; q += 8 * kBlock) {
+        double t[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) t[u] = v[q + u * kBlock];
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc += t[u];
+    }
+    for (; q < n; q += kBlock) acc += v[q];
+    return acc;
+}
+
+// The system's energy from its complete slot terms (one block of kBlock threads): Phi = sum(epart)/vol + Shift
+// (target-S(k) systems: sum(epart) + Shift), plus the soft-core sum.  Thread t sums slots t, t + kBlock, ... and
+// the block tree adds the threads in a fixed order, so the value does not depend on the kernel that runs it.
+// Writes energy[0..2] (energy[3], the force epilogue's flag, is cleared before the force pass).
+__device__ __forceinline__ void energy_finish(const SysDesc &S, double *lds)
+{
+    const double cc_sum = block_sum(strided_sum(S.epart, S.nslots), lds);
+    double sc = 0.0;
+    if (S.flags & kSysSoftCore) sc = block_sum(strided_sum(S.sc_epart, S.N), lds);
+    if (threadIdx.x == 0) {
+        const double cc = ((S.flags & kSysS2) ? cc_sum : cc_sum / S.vol) + S.shift;
+        // LogCCPotential::Energy (CollectiveCoordinatePotential.h:1620-1623): log of the plain form's energy
+        S.energy[0] = (S.flags & kSysLog) ? log(cc) : sc + cc;
+        S.energy[1] = cc;
+        S.energy[2] = sc;
+    }
+}
+
 }  // namespace ccgpu

@@ -30,21 +30,6 @@ __device__ __forceinline__ double wrap_unit(double x)
     return y;
 }
 
-// block-wide sum of one double per thread, fixed tree order; result valid in thread 0
-__device__ __forceinline__ double block_sum(double v, double *lds)
-{
-    const int tid = threadIdx.x;
-    lds[tid] = v;
-    __syncthreads();
-    for (int w = kBlock / 2; w > 0; w >>= 1) {
-        if (tid < w) lds[tid] = lds[tid] + lds[tid + w];
-        __syncthreads();
-    }
-    double r = lds[0];
-    __syncthreads();
-    return r;
-}
-
 // ---------------------------------------------------------------------------------------------
 // prepare: s = wrap(x / rescale) (FillParticles + Insert, StructureOptimization.cpp:339-350),
 // r = sum_i a_i s_i (UpdateCartesianCoordinates), and the last-axis lattice step exp(2 pi i s_{d-1}).
@@ -65,6 +50,7 @@ __global__ void __launch_bounds__(kBlock) k_prepare(const SysDesc *__restrict__ 
         for (int c = 0; c < d; c++) r[c] = r[c] + S.a[q * d + c] * s[q];
     S.pos[i] = make_double4(s[0], s[1], s[2], s[3]);
     S.cart[i] = make_double4(r[0], r[1], r[2], r[3]);
+    if (i == 0) S.energy[3] = 0.0;  // the force epilogue of this evaluation flags a non-finite gradient entry
     double sn, cs;
     sincos_rev(s[d - 1], sn, cs);
     S.step[i] = make_double2(cs, sn);
@@ -456,43 +442,17 @@ __global__ void __launch_bounds__(kBlock) k_rho_reduce(const SysDesc *__restrict
 }
 
 // ---------------------------------------------------------------------------------------------
-// energy: one block per system; Phi = sum(epart)/vol + Shift (target-S(k) systems: sum(epart) + Shift), plus the
-// soft-core sum.  epart holds one term per slot; thread t sums slots t, t + 256, ... and the block tree adds the
-// threads in a fixed order.  items: (sys)
+// energy: one block per system (energy_finish); also clears the non-finite-gradient flag that the force epilogue
+// of the same evaluation sets.  items: (sys)
 // ---------------------------------------------------------------------------------------------
-// v[t] + v[t + B] + v[t + 2B] + ... for thread t (B = kBlock), in that order, with eight loads in flight
-__device__ __forceinline__ double strided_sum(const double *__restrict__ v, int n)
-{
-    double acc = 0.0;
-    int q = threadIdx.x;
-    for (; q + 7 * kBlock < n; q += 8 * kBlock) {
-        double t[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++) t[u] = v[q + u * kBlock];
-#pragma unroll
-        for (int u = 0; u < 8; u++) acc += t[u];
-    }
-    for (; q < n; q += kBlock) acc += v[q];
-    return acc;
-}
-
 __global__ void __launch_bounds__(kBlock) k_energy_final(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
 {
     __shared__ double lds[kBlock];
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
     if (!sys_active(sys, S)) return;
-    const double cc_sum = block_sum(strided_sum(S.epart, S.nslots), lds);
-    double sc = 0.0;
-    if (S.flags & kSysSoftCore) sc = block_sum(strided_sum(S.sc_epart, S.N), lds);
-    if (threadIdx.x == 0) {
-        const double cc = ((S.flags & kSysS2) ? cc_sum : cc_sum / S.vol) + S.shift;
-        // LogCCPotential::Energy (CollectiveCoordinatePotential.h:1620-1623): log of the plain form's energy
-        S.energy[0] = (S.flags & kSysLog) ? log(cc) : sc + cc;
-        S.energy[1] = cc;
-        S.energy[2] = sc;
-        S.energy[3] = 0.0;  // the force epilogue sets it on a non-finite gradient entry
-    }
+    energy_finish(S, lds);
+    if (threadIdx.x == 0) S.energy[3] = 0.0;
 }
 
 // ---------------------------------------------------------------------------------------------

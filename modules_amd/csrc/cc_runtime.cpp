@@ -553,7 +553,8 @@ void System::Prepare(hipStream_t stream)
               (lattice && precision == 1 && !nufft_on() ? kSysMixed : 0) | (view() ? kSysView : 0) |
               (mixture ? kSysMixture : 0) | (nufft_on() ? kSysNufft : 0) | (s2 ? kSysS2 : 0) |
               (log_form ? kSysLog : 0) | (mixture && nsub > 0 ? kSysMultiConfig : 0) |
-              (rows && N > 0 && pchunks == 1 ? kSysRhoFinal : 0) | (rows && kchunks == 1 ? kSysForceFinal : 0);
+              (rows && N > 0 && pchunks == 1 ? kSysRhoFinal : 0) | (rows && kchunks == 1 ? kSysForceFinal : 0) |
+              (rows && N > 0 && !view() && !log_form ? kSysEnergyInForce : 0);
     D.nseg = (int32_t)nseg();
     D.nslots = (int32_t)nslots;
     D.pchunks = N > 0 ? pchunks : 0;  // no particles: rho sums nothing
@@ -794,6 +795,10 @@ void Batch::Build()
         for (int i = 0; i < ns; i++)
             if (!all[i]->mixture) h_items.push_back({i, 0, 0, 0});
     });
+    add(r_efin_ef, [&] {  // E+F evaluations: the force contraction forms these systems' energy
+        for (int i = 0; i < ns; i++)
+            if (!all[i]->mixture && !(all[i]->desc.flags & kSysEnergyInForce)) h_items.push_back({i, 0, 0, 0});
+    });
     add(r_mix, [&] {
         for (int i = 0; i < ns; i++)
             if (all[i]->mixture) h_items.push_back({i, 0, 0, 0});
@@ -921,7 +926,8 @@ void Batch::Enqueue(bool rho, bool forces)
         }
         launch_rho_reduce(stream, D, I + r_red.off, r_red.n);
         launch_multiconfig_coef(stream, D, I + r_mc.off, r_mc.n);
-        launch_energy_final(stream, D, I + r_efin.off, r_efin.n);
+        const Range &efin = forces ? r_efin_ef : r_efin;
+        launch_energy_final(stream, D, I + efin.off, efin.n);
         launch_mixture_final(stream, D, I + r_mix.off, r_mix.n);
     }
     if (forces) {

@@ -328,7 +328,7 @@ public:
     DevArray<SysDesc> d_desc;
     DevArray<WorkItem> d_items;
     std::vector<WorkItem> h_items;
-    Range r_prep, r_rho_gen, r_red, r_efin, r_f_gen, r_ffin, r_sc_part, r_sc_sys, r_sc_cell, r_mix, r_mc;
+    Range r_prep, r_rho_gen, r_red, r_efin, r_efin_ef, r_f_gen, r_ffin, r_sc_part, r_sc_sys, r_sc_cell, r_mix, r_mc;
     Range r_nu_part, r_nu_sys, r_nu_bins, r_nu_tiles, r_nu_gath, r_nu_slots, r_nu_grid;  // NUFFT systems
     bool has_nufft = false;
     int nu_max_psize = 1;  // largest padded tile of the batch's NUFFT systems (dynamic LDS of the spread kernel)
