@@ -69,11 +69,16 @@ constexpr int kRowSub = 64;   // particles staged in LDS at a time
 // ---------------------------------------------------------------------------------------------
 // rho.  items: (sys, block of 16 segments, particle chunk of pchunk_len particles).  One wavefront per block.
 // ---------------------------------------------------------------------------------------------
+// a staged particle: position, step exp(2 pi i s_last), 2 cos(phi) and weight, one 64-byte record (one address)
+struct RowParticle {
+    double4 pos;
+    double2 step;
+    double k1, w;
+};
+
 struct RhoRowsLds {
-    double2 tab[64];
-    double4 pos[kRowSub];
-    double2 step[kRowSub];
-    double wgt[kRowSub];
+    double2 tab[256];
+    RowParticle part[kRowSub];
 };
 
 // The work item with exactly L slots per segment (L: the block's longest segment rounded up to a multiple of 2; the
@@ -93,21 +98,22 @@ __device__ __forceinline__ void rho_rows_run(const SysDesc &S, const WorkItem &i
     // one particle of the lane's row: its head phase, then the slots.  W: scale the head phase by the staged weight
     // (weighted systems, and the iteration whose rows run past the staged particles, where the weight is 0)
     auto particle = [&](int j, auto W) {
-        const double4 p = lds.pos[j];
-        const double2 st = lds.step[j];
+        const RowParticle &P = lds.part[j];
+        const double4 p = P.pos;
+        const double2 st = P.step;
         double sn, cs;
-        sincos_rev_tab(head_dot(h, p), lds.tab, sn, cs);
+        sincos_rev_tab256(head_dot(h, p), lds.tab, sn, cs);
         // e_t lives in register pair (x0, y0) for even t and (x1, y1) for odd t, each updated in place by
         // e_{t+1} = 2 cos(phi) e_t - e_{t-1}; the weight rides along the (linear) recurrence
         double x0 = cs, y0 = sn;
         if constexpr (decltype(W)::value) {
-            const double w = lds.wgt[j];
+            const double w = P.w;
             x0 = cs * w;
             y0 = sn * w;
         }
         double x1 = __builtin_fma(x0, st.x, -y0 * st.y);
         double y1 = __builtin_fma(x0, st.y, y0 * st.x);
-        const double k1 = st.x + st.x;
+        const double k1 = P.k1;
         re[0] += x0;
         im[0] += y0;
 #pragma unroll
@@ -132,9 +138,12 @@ __device__ __forceinline__ void rho_rows_run(const SysDesc &S, const WorkItem &i
         __syncthreads();
         // every staged entry is defined: past the chunk's end a particle at the origin with weight 0
         const bool in = lane < nsub;
-        lds.pos[lane] = in ? S.pos[jb + lane] : make_double4(0, 0, 0, 0);
-        lds.step[lane] = in ? S.step[jb + lane] : make_double2(1, 0);
-        lds.wgt[lane] = in ? (weighted ? S.weight[jb + lane] : 1.0) : 0.0;
+        RowParticle &P = lds.part[lane];
+        P.pos = in ? S.pos[jb + lane] : make_double4(0, 0, 0, 0);
+        const double2 st = in ? S.step[jb + lane] : make_double2(1, 0);
+        P.step = st;
+        P.k1 = st.x + st.x;
+        P.w = in ? (weighted ? S.weight[jb + lane] : 1.0) : 0.0;
         __syncthreads();
         const int iters = (nsub + 3) >> 2;
         const int nplain = weighted ? 0 : nsub >> 2;  // iterations with all four rows on staged particles
@@ -198,7 +207,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
     if (!sys_active(sys, S)) return;
-    sincos_table_fill(lds.tab);
+    sincos_table_fill256(lds.tab);
     const int lane = threadIdx.x, col = lane & 15;
     const int seg0 = it.t0 * kRowSegs;
     const int seg = seg0 + col;
@@ -216,7 +225,7 @@ constexpr int kForceStageRows = 16;  // segments staged in LDS at a time (4 per 
 
 template <int SEG>
 struct ForceRowsLds {
-    double2 tab[64];
+    double2 tab[256];
     double2 coef[kForceStageRows * SEG];
 };
 
@@ -234,7 +243,7 @@ __device__ __forceinline__ void force_rows_segment(const double2 *cf, const doub
     double acc[PPL], accl[PPL];
 #pragma unroll
     for (int q = 0; q < PPL; q++) {
-        sincos_rev_tab(head_dot_d(h, p[q]), tab, y0[q], x0[q]);
+        sincos_rev_tab256(head_dot_d(h, p[q]), tab, y0[q], x0[q]);
         x1[q] = __builtin_fma(x0[q], stx[q], -y0[q] * sty[q]);
         y1[q] = __builtin_fma(x0[q], sty[q], y0[q] * stx[q]);
         acc[q] = accl[q] = 0.0;
@@ -308,12 +317,16 @@ template <int SEG, int PPL>
 __global__ void __launch_bounds__(kBlock) k_force_rows(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
 {
     __shared__ ForceRowsLds<SEG> L;
-    __shared__ double4 gsum[4][PPL][64];
-    __shared__ double ered[kBlock];
+    // the four wavefronts' sums, then (the first block of a system) the energy's block sums
+    __shared__ union {
+        double4 gsum[4][PPL][64];
+        double ered[kBlock];
+    } R;
+    auto &gsum = R.gsum;
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
     if (!sys_active(sys, S) || (S.flags & kSysMixed)) return;  // mixed systems: k_force_rows_mixed
-    sincos_table_fill(L.tab);
+    sincos_table_fill256(L.tab);
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int d = S.d;
     double4 p[PPL];
@@ -366,7 +379,10 @@ __global__ void __launch_bounds__(kBlock) k_force_rows(const SysDesc *__restrict
         }
     }
     // the system's energy (its slot terms are complete before this pass): the first block of each system
-    if ((S.flags & kSysEnergyInForce) && it.t0 == 0 && it.t1 == 0) energy_finish(S, ered);
+    if ((S.flags & kSysEnergyInForce) && it.t0 == 0 && it.t1 == 0) {
+        __syncthreads();  // every thread has read its sums (the energy's block sums reuse their LDS)
+        energy_finish(S, R.ered);
+    }
 }
 
 // ---------------------------------------------------------------------------------------------

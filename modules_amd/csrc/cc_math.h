@@ -103,6 +103,37 @@ __device__ __forceinline__ void sincos_rev_tab(double u, const double2 *tab, dou
     s = __builtin_fma(t.y, cl, t.x * sl);
 }
 
+// The 256-entry variant of the particle-row contractions: u = j/256 + f with |f| <= 1/512 revolutions (the same
+// exact split), and 2 pi folded into the series in f: sin(2 pi f) = f (c1 + c3 f^2 + c5 f^4) (next term below
+// 9e-18), cos(2 pi f) = 1 + c2 f^2 + c4 f^4 + c6 f^6 (next term below 2e-20), c_k = (+-) (2 pi)^k / k!.  About 16
+// VALU operations and one LDS read.  The table (4 KB) is filled by the whole block.
+__device__ __forceinline__ void sincos_table_fill256(double2 *tab)
+{
+    for (int t = threadIdx.x; t < 256; t += blockDim.x) {
+        double s, c;
+        sincos_rev((double)t * 0.00390625, s, c);
+        tab[t] = make_double2(c, s);
+    }
+}
+
+__device__ __forceinline__ void sincos_rev_tab256(double u, const double2 *tab, double &s, double &c)
+{
+    constexpr double kRound = 6755399441055744.0;  // 1.5 * 2^52: j = rint(256 u) for |u| < 2^43
+    const double sum = fma_sc(u, 256.0, kRound);
+    const double jd = sum - kRound;
+    const double2 t = tab[__double2loint(sum) & 255];  // read first: the polynomials below cover its latency
+    const double f = __builtin_fma(-0.00390625, jd, u);  // exact
+    const double f2 = f * f;
+    double ps = fma_sc(f2, 81.605249276075054203, -41.341702240399760234);
+    ps = fma_sc(ps, f2, 6.2831853071795864769);
+    const double sl = ps * f;
+    double pc = fma_sc(f2, -85.456817206693727736, 64.939394022668291491);
+    pc = fma_sc(pc, f2, -19.739208802178717238);
+    const double cl = __builtin_fma(pc, f2, 1.0);
+    c = __builtin_fma(t.x, cl, -t.y * sl);
+    s = __builtin_fma(t.y, cl, t.x * sl);
+}
+
 // FP32 counterparts for the mixed-precision path: the table as float2 (rounded from the FP64 table), and
 // sin/cos of 2 pi f for an argument already reduced to f in [-1/2, 1/2] revolutions in FP64 (the
 // per-particle range reduction) and rounded to FP32.  Same split as sincos_rev_tab; sin to x^5, cos to x^4
