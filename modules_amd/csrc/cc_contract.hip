@@ -69,7 +69,8 @@ constexpr int kRowSub = 64;   // particles staged in LDS at a time
 // ---------------------------------------------------------------------------------------------
 // rho.  items: (sys, block of 16 segments, particle chunk of pchunk_len particles).  One wavefront per block.
 // ---------------------------------------------------------------------------------------------
-// a staged particle: position, step exp(2 pi i s_last), 2 cos(phi) and weight, one 64-byte record (one address)
+// a staged particle: position, one complex value (the forward chain: the step exp(i phi); the centred chain: the
+// half step exp(i phi / 2)), 2 cos(phi) and weight, one 64-byte record (one address)
 struct RowParticle {
     double4 pos;
     double2 step;
@@ -81,9 +82,139 @@ struct RhoRowsLds {
     RowParticle part[kRowSub];
 };
 
-// The work item with exactly L slots per segment (L: the block's longest segment rounded up to a multiple of 2; the
-// slots beyond a segment's own length are padding, whose sums nothing reads): the particle loop, the row sums and
-// the output.  A specialisation per L keeps the slot loop free of branches.
+// The segment's chain run from its centre (segments of at least 8 slots).  A segment of length len is taken as
+// Le = len rounded up to even slots, centred between slots Le/2 - 1 and Le/2: slot Le/2 + t and slot Le/2 - 1 - t
+// are the phases theta_c +- (t + 1/2) phi of the centre phase theta_c.  With H = exp(i theta_c), C_t =
+// cos((t + 1/2) phi) and S_t = sin((t + 1/2) phi),
+//   e_{+t} = (Hc C - Hs S) + i (Hs C + Hc S),   e_{-t} = (Hc C + Hs S) + i (Hs C - Hc S),
+// so the lane accumulates the four products Hc C, Hs S, Hs C, Hc S (four FMAs for two slots) and C_t, S_t follow
+// the real three-term recurrence x_{t+1} = 2 cos(phi) x_t - x_{t-1} (two FMAs for two slots) from C_0, S_0 =
+// exp(i phi / 2) and C_1 = (2 cos(phi) - 1) C_0, S_1 = (2 cos(phi) + 1) S_0: 3 operations per slot, against 4 for the
+// forward chain, and no e_1 set-up; the chains are half as long, so their rounding (t^2 u) is a quarter.
+// TW: the slot pairs of the block's longest segment.  X[2t], X[2t+1] hold the sums of Hc C_t and Hs S_t, Y[2t],
+// Y[2t+1] those of Hs C_t and Hc S_t.
+template <int SEG, int TW>
+__device__ __forceinline__ void rho_rows_run_centred(const SysDesc &S, const WorkItem &it, RhoRowsLds &lds, int lane,
+                                                     bool valid, int seg, double4 hc, int half)
+{
+    const int row = lane >> 4;
+    double X[SEG], Y[SEG];
+#pragma unroll
+    for (int t = 0; t < SEG; t++) X[t] = Y[t] = 0.0;
+    const int j0 = it.t1 * S.pchunk_len;
+    const int j1 = min(S.N, j0 + S.pchunk_len);
+    const bool weighted = S.weight != nullptr;
+    const int dl = S.d - 1;
+    auto particle = [&](int j, auto W) {
+        const RowParticle &P = lds.part[j];
+        double hs, hcs;
+        sincos_rev_tab256(head_dot_d(hc, P.pos), lds.tab, hs, hcs);
+        if constexpr (decltype(W)::value) {
+            const double w = P.w;
+            hcs *= w;
+            hs *= w;
+        }
+        const double k1 = P.k1;
+        double c0 = P.step.x, s0 = P.step.y;                              // C_0, S_0 (even t)
+        double c1 = __builtin_fma(k1, c0, -c0), s1 = __builtin_fma(k1, s0, s0);  // C_1, S_1 (odd t)
+#pragma unroll
+        for (int t = 0; t < TW; t++) {
+            if (t >= 2) {
+                if (t & 1) {
+                    c1 = __builtin_fma(k1, c0, -c1);
+                    s1 = __builtin_fma(k1, s0, -s1);
+                } else {
+                    c0 = __builtin_fma(k1, c1, -c0);
+                    s0 = __builtin_fma(k1, s1, -s0);
+                }
+            }
+            const double C = (t & 1) ? c1 : c0, Sv = (t & 1) ? s1 : s0;
+            X[2 * t] = __builtin_fma(hcs, C, X[2 * t]);
+            X[2 * t + 1] = __builtin_fma(hs, Sv, X[2 * t + 1]);
+            Y[2 * t] = __builtin_fma(hs, C, Y[2 * t]);
+            Y[2 * t + 1] = __builtin_fma(hcs, Sv, Y[2 * t + 1]);
+        }
+    };
+    for (int jb = j0; jb < j1; jb += kRowSub) {
+        const int nsub = min(kRowSub, j1 - jb);
+        __syncthreads();
+        // every staged entry is defined: past the chunk's end a particle at the origin with weight 0
+        const bool in = lane < nsub;
+        RowParticle &P = lds.part[lane];
+        const double4 p = in ? S.pos[jb + lane] : make_double4(0, 0, 0, 0);
+        P.pos = p;
+        const double2 st = in ? S.step[jb + lane] : make_double2(1, 0);
+        const double pl = dl == 0 ? p.x : dl == 1 ? p.y : dl == 2 ? p.z : p.w;
+        double hsn, hcs;
+        sincos_rev_tab256(0.5 * pl, lds.tab, hsn, hcs);  // exp(i phi / 2), phi = 2 pi s_last
+        P.step = make_double2(hcs, hsn);
+        P.k1 = st.x + st.x;
+        P.w = in ? (weighted ? S.weight[jb + lane] : 1.0) : 0.0;
+        __syncthreads();
+        const int iters = (nsub + 3) >> 2;
+        const int nplain = weighted ? 0 : nsub >> 2;  // iterations with all four rows on staged particles
+        for (int q = 0; q < nplain; q++) particle(4 * q + row, std::false_type{});
+        for (int q = nplain; q < iters; q++) particle(4 * q + row, std::true_type{});
+    }
+    // the four rows' sums, as in the forward chain (entries from 2 TW on are zero): afterwards the lane holds
+    // X[0 .. Q) and Y[0 .. Q), the sums of Q/2 consecutive slot pairs t0 .. t0 + Q/2 - 1
+    constexpr int H = SEG / 2, Q = SEG / 4, NZ = 2 * TW;
+#pragma unroll
+    for (int t = 0; t < (H < NZ ? H : NZ); t++) {
+        double a = X[t], b = X[t + H];
+        swap_rows16(a, b);
+        X[t] = a + b;
+        a = Y[t];
+        b = Y[t + H];
+        swap_rows16(a, b);
+        Y[t] = a + b;
+    }
+#pragma unroll
+    for (int t = 0; t < (Q < NZ ? Q : NZ); t++) {
+        double a = X[t], b = X[t + Q];
+        swap_halves32(a, b);
+        X[t] = a + b;
+        a = Y[t];
+        b = Y[t + Q];
+        swap_halves32(a, b);
+        Y[t] = a + b;
+    }
+    if (!valid) return;
+    const int t0 = (row == 0 ? 0 : row == 2 ? 1 : row == 1 ? 2 : 3) * (Q / 2);
+    // every slot of the segment's SEG is written once: [0, half) by the minus side, [half, half + SEG/2) by the
+    // plus side (from Le on these are padding slots), [half + SEG/2, SEG) as padding
+    const bool fin = (S.flags & kSysRhoFinal) != 0;
+    double2 *out = reinterpret_cast<double2 *>(S.rpart) + (int64_t)it.t1 * S.nslots + seg * SEG;
+    auto put = [&](int s, double re, double im) {
+        if (fin) slot_finish(S, seg * SEG + s, re, im);
+        else out[s] = make_double2(re, im);
+    };
+#pragma unroll
+    for (int u = 0; u < Q / 2; u++) {
+        const int t = t0 + u;
+        const double hcC = X[2 * u], hsS = X[2 * u + 1], hsC = Y[2 * u], hcS = Y[2 * u + 1];
+        put(half + t, hcC - hsS, hsC + hcS);
+        if (half - 1 - t >= 0) put(half - 1 - t, hcC + hsS, hsC - hcS);
+        if (half + H + t < SEG) put(half + H + t, 0.0, 0.0);
+    }
+}
+
+template <int SEG, int TW = 1>
+__device__ __forceinline__ void rho_rows_dispatch_centred(int tw, const SysDesc &S, const WorkItem &it, RhoRowsLds &lds,
+                                                          int lane, bool valid, int seg, double4 hc, int half)
+{
+    if constexpr (TW >= SEG / 2) {
+        rho_rows_run_centred<SEG, SEG / 2>(S, it, lds, lane, valid, seg, hc, half);
+    } else {
+        if (tw <= TW) rho_rows_run_centred<SEG, TW>(S, it, lds, lane, valid, seg, hc, half);
+        else rho_rows_dispatch_centred<SEG, TW + 1>(tw, S, it, lds, lane, valid, seg, hc, half);
+    }
+}
+
+// The forward chain (segments of 4 slots): the work item with exactly L slots per segment (L: the block's longest
+// segment rounded up to a multiple of 2; the slots beyond a segment's own length are padding, whose sums nothing
+// reads): the particle loop, the row sums and the output.  A specialisation per L keeps the slot loop free of
+// branches.
 template <int SEG, int L>
 __device__ __forceinline__ void rho_rows_run(const SysDesc &S, const WorkItem &it, RhoRowsLds &lds, int lane, bool valid,
                                              int seg, int4 h)
@@ -215,7 +346,17 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
     const int4 h = valid ? S.seg_head[seg] : make_int4(0, 0, 0, 0);
     // sorted longest first: the block's first segment bounds the slots any lane needs
     const int wlen = __builtin_amdgcn_readfirstlane(S.seg_len[seg0]);
-    rho_rows_dispatch<SEG>(wlen, S, it, lds, lane, valid, seg, h);
+    if constexpr (SEG >= 8) {
+        // the centre of the lane's segment: the head plus (Le - 1)/2 on the last axis, Le = length rounded up to even
+        const int half = valid ? (S.seg_len[seg] + 1) >> 1 : 1;
+        const double off = (double)half - 0.5;
+        const int dl = S.d - 1;
+        const double4 hc = make_double4((double)h.x + (dl == 0 ? off : 0.0), (double)h.y + (dl == 1 ? off : 0.0),
+                                        (double)h.z + (dl == 2 ? off : 0.0), (double)h.w + (dl == 3 ? off : 0.0));
+        rho_rows_dispatch_centred<SEG>((wlen + 1) >> 1, S, it, lds, lane, valid, seg, hc, half);
+    } else {
+        rho_rows_dispatch<SEG>(wlen, S, it, lds, lane, valid, seg, h);
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
