@@ -23,6 +23,10 @@
 // here (force_finish), otherwise the sums go to fpart.
 //
 // Every sum has a fixed order, so a system's bits do not depend on the other systems of a launch.
+#include <algorithm>
+#include <map>
+#include <mutex>
+
 #include "cc_device.h"
 #include "cc_epilogue.h"
 #include "cc_math.h"
@@ -705,14 +709,64 @@ void launch_force_rows_mixed(hipStream_t stream, int seg, const SysDesc *sys, co
     }
 }
 
-void launch_force_rows(hipStream_t stream, int seg, const SysDesc *sys, const WorkItem *items, int n, int ppl)
+// Dynamic LDS that holds a kernel to at most `blocks` blocks per CU: more than a (blocks + 1)-th of the CU's LDS
+// per block.  0 when no cap is asked for or the kernel's own LDS already enforces it.
+static int LdsPadForBlocks(const void *fn, int blocks)
+{
+    if (blocks <= 0) return 0;
+    static std::mutex mu;
+    static std::map<const void *, int> static_lds;
+    static int lds_cu = 0;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!lds_cu) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess ||
+            lds_cu <= 0)
+            lds_cu = 160 * 1024;
+    }
+    auto it = static_lds.find(fn);
+    if (it == static_lds.end()) {
+        hipFuncAttributes a{};
+        const int v = hipFuncGetAttributes(&a, fn) == hipSuccess ? (int)a.sharedSizeBytes : 0;
+        it = static_lds.emplace(fn, v).first;
+    }
+    const int per_block = lds_cu / (blocks + 1) + 256;
+    return std::max(0, per_block - it->second);
+}
+
+// The attribute queries of LdsPadForBlocks for every force kernel, made before any stream capture that launches
+// them (the solver's constructor calls this)
+void prime_force_rows_attributes()
+{
+#define CC_PRIME(S)                                                               \
+    (void)LdsPadForBlocks(reinterpret_cast<const void *>(&k_force_rows<S, 1>), 1); \
+    (void)LdsPadForBlocks(reinterpret_cast<const void *>(&k_force_rows<S, 2>), 1);
+    CC_PRIME(4)
+    CC_PRIME(8)
+    CC_PRIME(16)
+    CC_PRIME(24)
+    CC_PRIME(32)
+#undef CC_PRIME
+    (void)hipGetLastError();  // a failed query leaves no error for the next launch check
+}
+
+void launch_force_rows(hipStream_t stream, int seg, const SysDesc *sys, const WorkItem *items, int n, int ppl,
+                       int max_blocks_per_cu)
 {
     if (n <= 0) return;
     const dim3 grid((unsigned)n), block(kBlock);
-#define CC_ROWS_CASE(S)                                                                                    \
-    case S:                                                                                                \
-        if (ppl == 2) hipLaunchKernelGGL((k_force_rows<S, 2>), grid, block, 0, stream, sys, items);       \
-        else hipLaunchKernelGGL((k_force_rows<S, 1>), grid, block, 0, stream, sys, items);                \
+#define CC_ROWS_CASE(S)                                                                                          \
+    case S:                                                                                                      \
+        if (ppl == 2) {                                                                                          \
+            const void *fn = reinterpret_cast<const void *>(&k_force_rows<S, 2>);                                \
+            hipLaunchKernelGGL((k_force_rows<S, 2>), grid, block, LdsPadForBlocks(fn, max_blocks_per_cu), stream, \
+                               sys, items);                                                                      \
+        } else {                                                                                                 \
+            const void *fn = reinterpret_cast<const void *>(&k_force_rows<S, 1>);                                \
+            hipLaunchKernelGGL((k_force_rows<S, 1>), grid, block, LdsPadForBlocks(fn, max_blocks_per_cu), stream, \
+                               sys, items);                                                                      \
+        }                                                                                                        \
         break;
     switch (seg) {
         CC_ROWS_CASE(4)

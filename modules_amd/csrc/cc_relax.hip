@@ -853,6 +853,7 @@ Solver::Solver(int max_slots, int64_t hbm_budget) : max_slots_(max_slots), budge
 {
     t0_ = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
     CC_HIP(hipStreamCreateWithFlags(&copy_stream_, hipStreamNonBlocking));
+    prime_force_rows_attributes();
     SetLanes(2);
 }
 
@@ -869,6 +870,13 @@ Solver::~Solver()
     }
 }
 
+// Force-contraction blocks per CU while two lanes share the GPU.  The 24-slot force kernel alone fits 8 blocks
+// (18 KB of LDS each, 8 waves per SIMD) and then occupies every wave slot, so the other lane's L-BFGS kernels wait
+// for its end; at 5 blocks they run beside it.  Measured on the 2560-configuration headline (ground states/s):
+// 8 blocks 1825, 6: 1795, 5: 1912, 4: 1886, 3: 1847; the kernel alone runs about 8% longer at 5 blocks, so the
+// one-lane solver keeps no cap.
+constexpr int kForceBlocksShared = 5;
+
 void Solver::SetLanes(int n)
 {
     if (n < 1 || n > 2) throw Error(kErrArg, "lanes must be 1 or 2");
@@ -878,6 +886,7 @@ void Solver::SetLanes(int n)
         lanes_.emplace_back(new SolverLane());
         lanes_.back()->batch.timing = timing_;
     }
+    for (auto &l : lanes_) l->batch.force_blocks_per_cu = n > 1 ? kForceBlocksShared : 0;
 }
 
 void Solver::SetTiming(bool on)

@@ -938,7 +938,7 @@ void Batch::Enqueue(bool rho, bool forces)
         }
         for (const SegGroup &g : seg_groups) {
             if (g.rho_kernel == 2) {
-                launch_force_rows(stream, g.seg, D, I + g.force.off, g.force.n, g.force_ppl);
+                launch_force_rows(stream, g.seg, D, I + g.force.off, g.force.n, g.force_ppl, force_blocks_per_cu);
                 if (g.any_mixed) launch_force_rows_mixed(stream, g.seg, D, I + g.force.off, g.force.n);
             } else {
                 launch_force_lattice(stream, g.seg, D, I + g.force.off, g.force.n, g.force_ppl);

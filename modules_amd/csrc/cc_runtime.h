@@ -316,6 +316,10 @@ public:
         bool any_mixed;  // some system of the group computes in mixed precision: launch the mixed rho kernel too
         Range rho, force;
     };
+    // cap of force-contraction blocks per CU (0: none).  When another stream's kernels share the GPU (two solver
+    // lanes) a cap leaves wave slots and registers of every CU to the other stream's memory-bound L-BFGS blocks,
+    // which then run beside the force blocks instead of after them
+    int force_blocks_per_cu = 0;
     // optional kernel timing: hipEvents around the rho and force contraction launches of every Enqueue
     bool timing = false;
     double t_rho_ms = 0, t_force_ms = 0;
@@ -372,7 +376,8 @@ void launch_mc_trial(hipStream_t, const SysDesc *, const WorkItem *, int, const 
 void launch_force_lattice(hipStream_t, int seg, const SysDesc *, const WorkItem *, int, int ppl);
 // the particle-row contractions (cc_contract.hip)
 void launch_rho_rows(hipStream_t, int seg, const SysDesc *, const WorkItem *, int);
-void launch_force_rows(hipStream_t, int seg, const SysDesc *, const WorkItem *, int, int ppl);
+void launch_force_rows(hipStream_t, int seg, const SysDesc *, const WorkItem *, int, int ppl, int max_blocks_per_cu);
+void prime_force_rows_attributes();  // before a capture that launches force kernels with a block cap
 void launch_force_rows_mixed(hipStream_t, int seg, const SysDesc *, const WorkItem *, int);
 void launch_force_general(hipStream_t, const SysDesc *, const WorkItem *, int);
 void launch_force_final(hipStream_t, const SysDesc *, const WorkItem *, int);
