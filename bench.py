@@ -141,14 +141,14 @@ def _roofline(jobs, rho_ms, force_ms, n_rho, n_force):
     P = FP64_VALU_PEAK_TFLOPS
     return {
         "bound": "valu",
-        "kernel": "k_force_lattice (force contraction, FP64 VALU)",
+        "kernel": "k_force_rows (force contraction, FP64 VALU)",
         "achieved": ach_f, "peak": P, "unit": "TFLOP/s", "frac": ach_f / P,
         "traffic": None,
         "traffic_note": "HBM bytes per launch are measured by rocprofv3 FETCH_SIZE / WRITE_SIZE passes, not in this "
                         "run (profiles/)",
         "avg_launch_ms": force_ms / max(n_force, 1), "launches": n_force,
         "flops_per_pair": "4d+2",
-        "rho_kernel": {"kernel": "k_rho_lattice", "achieved": ach_r, "frac": ach_r / P,
+        "rho_kernel": {"kernel": "k_rho_rows", "achieved": ach_r, "frac": ach_r / P,
                        "avg_launch_ms": rho_ms / max(n_rho, 1), "launches": n_rho, "flops_per_pair": "2d+1"},
         "ef_pair": {"achieved": ach_ef, "frac": ach_ef / P, "flops_per_pair": "6d+3"},
         "pairs_per_ef_mean": pairs / max(n_force, 1),
