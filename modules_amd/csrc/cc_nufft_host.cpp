@@ -186,10 +186,15 @@ void PrepareNufft(System &S, hipStream_t stream)
         if (!U.measured) use = U.pred_direct_us >= kNufftMargin * U.pred_nufft_us;
         if (!use) {
             U.gated = true;
-            char buf[160];
-            std::snprintf(buf, sizeof buf, "cost gate: direct sums %s (%.0f us against %.0f us predicted)",
-                          U.measured ? "timed faster by the autotuner" : "predicted faster", U.pred_direct_us,
-                          U.pred_nufft_us);
+            char buf[200];
+            if (U.measured)
+                std::snprintf(buf, sizeof buf,
+                              "cost gate: direct sums timed faster by the autotuner (predicted direct %.0f us, "
+                              "NUFFT %.0f us)", U.pred_direct_us, U.pred_nufft_us);
+            else
+                std::snprintf(buf, sizeof buf,
+                              "cost gate: NUFFT not predicted %.1fx faster (predicted direct %.0f us, NUFFT %.0f us)",
+                              kNufftMargin, U.pred_direct_us, U.pred_nufft_us);
             U.why = buf;
             return;
         }

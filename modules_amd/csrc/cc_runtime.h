@@ -400,12 +400,14 @@ void PrepareNufft(System &s, hipStream_t stream);
 // particle-constraint pair.  NUFFT: a fixed part (its launches and hipFFT calls), a part per kernel point (N w^d
 // spread and as many interpolated), a part per fine-grid point (the FFTs and the grid passes) and a part per kernel
 // point of one output tile (N w^d / tiles: each tile spreads its particles in order, the chain that bounds sparse
-// grids).  Least-squares fits (relative error) to hipEvent timings of both paths, one E+F of one system, over 47
-// shapes: 1D N 1e3-4e5, 2D N 300-1e5, 3D N 300-5e4, chi 0.1-0.45, eps 1e-12 (largest error of a fitted time 15%).
-// They only choose the faster path, never a result's accuracy.  Backend 1 takes the NUFFT when
-// direct >= kNufftMargin x nufft.
-constexpr double kDirectCallUs[3] = {57.0, 51.6, 54.2};
-constexpr double kDirectPairUs[3] = {5.13e-7, 5.81e-7, 7.35e-7};
+// grids).  Least-squares fits (relative error) to hipEvent timings of both paths, one E+F of one system: the NUFFT
+// part over 47 shapes (largest error of a fitted time 15%); the direct part refitted to the particle-row kernels
+// (modules_amd/fit_cost_gate.py: 48 shapes, 1D N 1e3-4e5, 2D N 300-1e5, 3D N 300-5e4, chi 0.1-0.45, eps 1e-12;
+// largest error 45%, at the smallest systems, whose time is launch latency).  With these constants the gate picks
+// the timed winner on all 48 shapes.  They only choose the faster path, never a result's accuracy.  Backend 1 takes
+// the NUFFT when direct >= kNufftMargin x nufft.
+constexpr double kDirectCallUs[3] = {48.3, 42.0, 44.0};
+constexpr double kDirectPairUs[3] = {4.22e-7, 4.83e-7, 5.42e-7};
 constexpr double kNufftCallUs[3] = {78.6, 115.0, 952.9};
 constexpr double kNufftPointUs[3] = {1.02e-4, 1.02e-5, 7.34e-6};
 constexpr double kNufftGridUs[3] = {1.08e-4, 2.30e-4, 7.91e-4};
