@@ -92,3 +92,26 @@ def test_varbox_kpoints_match_the_oracle(seed):
     assert len(k) == len(ko) > 0
     assert np.array_equal(n, no)
     assert np.allclose(k, ko, rtol=0, atol=1e-14)
+
+
+def test_cost_gate_fit_recovers_known_constants():
+    # the refit of the gate's direct part: timings made from known constants come back, and the decision count
+    # compares the fitted direct time (with the margin) against the NUFFT's prediction and the timed winner
+    from modules_amd.fit_cost_gate import MARGIN, fit
+    truth = {1: (48.0, 4.0e-7), 2: (42.0, 5.0e-7), 3: (44.0, 6.0e-7)}
+    rows = []
+    for d, (a, b) in truth.items():
+        for N, M in [(300, 100), (2000, 900), (20000, 9000), (100000, 40000), (400000, 160000)]:
+            t = a + b * N * M
+            # a NUFFT that the gate should take exactly when the direct time clears the margin
+            pn = t / (1.5 if N >= 100000 else 0.5)
+            rows.append({"N": N, "d": d, "chi": 0.3, "M": M, "direct_us": t, "nufft_us": pn, "pred_nufft_us": pn})
+    res = fit(rows)
+    for d, (a, b) in truth.items():
+        assert abs(res[d]["call_us"] - a) <= 1e-6 * a and abs(res[d]["pair_us"] - b) <= 1e-9 * b
+        assert res[d]["max_rel_err"] < 1e-9
+        assert res[d]["decisions_agree"] == res[d]["decisions"] == 5
+    # the script's margin is the library's
+    import re
+    hdr = open(os.path.join(ROOT, "modules_amd", "csrc", "cc_runtime.h")).read()
+    assert float(re.search(r"kNufftMargin = ([0-9.]+);", hdr).group(1)) == MARGIN
