@@ -22,7 +22,11 @@ std::string &LastError()
 
 // one ShiftedCCPotential (optionally RepulsiveCCPotential) bound to one configuration on the GPU
 struct ccgpu_potential {
-    explicit ccgpu_potential(int d) : sys(d) { batch.systems.push_back(&sys); }
+    explicit ccgpu_potential(int d) : sys(d)
+    {
+        sys.alone = true;
+        batch.systems.push_back(&sys);
+    }
     System sys;
     Batch batch;
     bool bound = false;
@@ -69,6 +73,7 @@ struct ccgpu_potential {
             sys.children.clear();
             for (int64_t c = 0; c < C; c++) {
                 std::unique_ptr<System> ch(new System(sys.d));
+                ch->alone = true;
                 ch->parent = &sys;
                 ch->s2 = true;
                 ch->fscale = 1.0;

@@ -24,6 +24,7 @@ struct ccgpu_multi {
         : d(dim), mix(dim), e0(epsilon), with_repulsion(std::abs(epsilon) > 1e-10)  // MultiStealthyPotential.h:318-332
     {
         mix.mixture = true;
+        mix.alone = true;
         batch.systems.push_back(&mix);
     }
     int d;
@@ -58,6 +59,7 @@ struct ccgpu_multi {
     {
         for (int64_t q = 0; q < num; q++) {
             std::unique_ptr<System> c(new System(d));
+            c->alone = true;
             c->parent = &mix;
             if (bound) c->SetCell(basis.data());
             mix.children.push_back(std::move(c));

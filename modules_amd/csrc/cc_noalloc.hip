@@ -68,6 +68,7 @@ extern "C" int ccgpu_energy_and_all_force_noalloc(int dim, const double *basis, 
         arena.cap = kScratchBytes;
         ArenaScope scope(arena);  // every DevArray reserved below is carved from the array, none from hipMalloc
         System sys(dim);
+        sys.alone = true;  // the tiles of a bound potential
         Batch batch(nullptr);  // the null stream: no stream is created
         batch.systems.push_back(&sys);
         sys.SetCell(basis);

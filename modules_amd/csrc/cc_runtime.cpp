@@ -347,11 +347,12 @@ int64_t EstimateSystemBytes(int d, const double *a, int64_t N, int64_t M, const 
 }
 
 ChunkPlan PlanChunks(int64_t N, int64_t M, bool lattice, int64_t nunits, const Tiles &used, const Tiles &req,
-                     const Tiles &def)
+                     const Tiles &def, bool alone)
 {
     // Untuned shapes get enough work items for their work and no more, so a lone large system fills the chip and
     // a small one in a big batch does not split its sums needlessly.  The rule reads only the system's own shape
-    // (so do its bits).
+    // and whether it is evaluated alone (so do its bits): a system alone asks for at least kAloneRhoWaves rho
+    // wavefronts and kAloneForceBlocks force blocks, as far as its particles and segments divide.
     ChunkPlan cp;
     const int64_t pairs = std::max<int64_t>(N, 1) * std::max<int64_t>(M, 1);
     const bool rows = lattice && used.rho_kernel == 2;
@@ -360,7 +361,8 @@ ChunkPlan PlanChunks(int64_t N, int64_t M, bool lattice, int64_t nunits, const T
         cp.rho_chunk = rc;
     } else if (rows) {
         // particle-row rho kernel: one wavefront per 16 segments and chunk, about 2^16 pairs each
-        const int64_t want = std::max<int64_t>(1, std::min<int64_t>(8192, pairs >> 16));
+        int64_t want = std::max<int64_t>(1, std::min<int64_t>(8192, pairs >> 16));
+        if (alone) want = std::max<int64_t>(want, kAloneRhoWaves);
         const int64_t sblocks = std::max<int64_t>(1, (nunits + 15) / 16);
         const int64_t pch = std::max<int64_t>(1, (want + sblocks - 1) / sblocks);
         const int64_t c = (std::max<int64_t>(N, 1) + pch - 1) / pch;
@@ -384,7 +386,7 @@ ChunkPlan PlanChunks(int64_t N, int64_t M, bool lattice, int64_t nunits, const T
     } else {
         // one block per ~2^18 pairs (at least 24 per system with the older kernel), each chunk at least 4 units
         // (16 for the particle-row kernel, whose four wavefronts share a chunk)
-        const int64_t lo = rows ? 1 : 24;
+        const int64_t lo = rows ? (alone ? kAloneForceBlocks : 1) : 24;
         const int64_t want_blocks = std::max<int64_t>(lo, std::min<int64_t>(4096, pairs >> 18));
         int64_t want = std::max<int64_t>(1, (want_blocks + pblocks - 1) / pblocks);
         want = std::min<int64_t>(want, std::max<int64_t>(1, nunits / (rows ? 16 : 4)));
@@ -493,7 +495,7 @@ void System::Prepare(hipStream_t stream)
     }
     else if (nu) nu->active = false, nu->layout.reset(), nu->why = "direct backend selected";
 
-    const ChunkPlan cp = PlanChunks(N, M, lattice, nunits, used, tiles, def);
+    const ChunkPlan cp = PlanChunks(N, M, lattice, nunits, used, tiles, def, alone);
     used.rho_chunk = cp.rho_chunk;
     used.force_chunk = cp.force_chunk;
     pchunks = cp.pchunks;

@@ -115,8 +115,12 @@ Tiles DefaultTiles(int d, int64_t N, int64_t M);
 struct ChunkPlan {
     int rho_chunk = 0, pchunks = 1, force_chunk = 0, kchunks = 1;
 };
+// `alone`: the system is evaluated by itself (a potential object, not a solver job), so its work items are planned
+// to spread it over the chip: at least this many rho wavefronts (about four per CU) and force blocks (two per CU).
+constexpr int64_t kAloneRhoWaves = 1024;
+constexpr int64_t kAloneForceBlocks = 512;
 ChunkPlan PlanChunks(int64_t N, int64_t M, bool lattice, int64_t nunits, const Tiles &used, const Tiles &req,
-                     const Tiles &def);
+                     const Tiles &def, bool alone = false);
 // The tiles a system uses: requested values where given, else the defaults; mixed precision takes rho kernel 0.
 Tiles ResolveTiles(const Tiles &req, const Tiles &def, int precision);
 // Device bytes of the constraint-side buffers of one system (N particles, M constraints k in cell a), with the same
@@ -245,6 +249,9 @@ public:
     DevArray<double> rpart, rho, coef, epart, energy, fpart, F, grad;
     DevArray<float> coef32;
     int precision = 0;  // 0: FP64; 1: mixed (lattice phases and force terms in FP32)
+    // evaluated by itself (bound to a potential, multi-species or allocation-free object), not beside other jobs
+    // in a solver: the automatic tiles then plan enough work items to spread it over the chip (PlanChunks)
+    bool alone = false;
     // rho backend (SetRhoBackend / SetFinufftTolerance / SetFinufftMaxModes,
     // potential/CollectiveCoordinatePotential.h:160-212): 0 direct sums; 1 NUFFT when the constraint set is
     // eligible (lattice, d <= 3, mode box within the cap, no constraint sharing a mode with another's conjugate)
