@@ -490,8 +490,9 @@ int ccgpu_solver_step(ccgpu_solver *s, int evaluations, int *pending);
 /* Evaluations between two host checks for finished jobs (default 8). */
 int ccgpu_solver_set_check_interval(ccgpu_solver *s, int evaluations);
 /* Streams the resident jobs are spread over (1 or 2, default 2; only while no job is resident).  Each stream
- * runs its jobs' E+F and L-BFGS kernels, so the latency-bound L-BFGS kernels of one stream overlap the E+F
- * kernels of the other.  Results do not depend on the lane count.  No counterpart in the reference. */
+ * runs its jobs' E+F and L-BFGS kernels, so the memory-bound L-BFGS kernels of one stream overlap the E+F
+ * kernels of the other; with two streams the force kernel is held to 5 blocks per CU to leave them room.
+ * Results do not depend on the lane count.  No counterpart in the reference. */
 int ccgpu_solver_set_lanes(ccgpu_solver *s, int lanes);
 /* Replay the steps between two checks as one captured hipGraph (1) or launch them on the stream (0, default). */
 int ccgpu_solver_set_graph(ccgpu_solver *s, int on);
