@@ -333,6 +333,21 @@ int ccgpu_pack_get(const char *prefix, int spheres, int64_t i, int *dim, double 
 /* Clear(): discard every configuration of the pack. */
 int ccgpu_pack_clear(const char *prefix, int spheres);
 
+/* The MD restart file Prefix.MDDump of CollectiveCoordinateMD (apps/CCOptimization/main.cpp:1891-1908 reads it,
+ * :1936-1946, :1973-1983 and :2005-2011 write it): int32 stage, int64 step, then ParticleMolecularDynamics::
+ * WriteBinary (potential/MD_System.cpp:50-62): the Position record (as in a ConfigurationPack), double TimeStep,
+ * double Time, uint64 NumParticle, uint16 Dimension, double xi, double MassQ, N doubles Mass, and N GeometryVectors
+ * of Velocity and of Acceleration (four doubles each, the release-build layout; components beyond dim zero).
+ * Velocity and acceleration are N x dim here; chars as in ccgpu_pack_add (NULL: "a"). */
+int ccgpu_mddump_write(const char *path, int32_t stage, int64_t step, int dim, double cell_size, const double *basis,
+                       int64_t N, const double *relative, const void *chars, double timestep, double time, double xi,
+                       double Q, const double *mass, const double *velocity, const double *acceleration);
+/* Read it back: the scalars and *N always; the arrays (relative, chars, mass, velocity, acceleration; any may be
+ * NULL) only when cap >= N.  A short, inconsistent or overlong file is an error. */
+int ccgpu_mddump_read(const char *path, int32_t *stage, int64_t *step, int *dim, double *cell_size, double *basis,
+                      int64_t *N, int64_t cap, double *relative, void *chars, double *timestep, double *time,
+                      double *xi, double *Q, double *mass, double *velocity, double *acceleration);
+
 /* MSPB v1 snapshot (apps/CCOptimization/EXPORT_FORMAT.md; MSPBSnapshot::Write/Read, MSPB_Pack.h): encode into buf
  * (*size bytes; buf NULL or cap too small: size only); ids NULL means 0..N-1.  Decode validates as the reference's
  * reader: version, flags, dimension 1-4, exact byte count, finite values, positive radii, unique ids, labels below
@@ -385,7 +400,7 @@ int ccgpu_potential_relax(ccgpu_potential *p, double *relative, int maxstep, dou
  * (wrapped into the cell), a = F / m from the potential's force pass, v += dt/2 a.  After a call the potential
  * stays bound to the final positions.  Random draws (SetRandomSpeed, AndersonEvolve) come from the caller's
  * RandomDouble() stream: `draws` holds the next values of that stream and *used says how many were consumed, so
- * the caller's generator advances as the reference's would.  The MDDump binary restart format is not provided.
+ * the caller's generator advances as the reference's would.
  * ------------------------------------------------------------------------------------------- */
 typedef struct ccgpu_md ccgpu_md;
 /* ParticleMolecularDynamics(Position, TimeStep, Mass) (MD_System.cpp:17-29): cell basis (dim x dim), N relative
@@ -398,6 +413,13 @@ int ccgpu_md_set_velocities(ccgpu_md *m, const double *v);
 /* Position (relative, N x dim), Velocity, Acceleration, Time, TimeStep and xi; any pointer may be null */
 int ccgpu_md_get_state(ccgpu_md *m, double *relative, double *velocity, double *acceleration, double *time,
                        double *timestep, double *xi);
+/* The state the stream constructor ParticleMolecularDynamics(std::istream &) restores (potential/MD_System.cpp:34-49):
+ * Velocity and Acceleration (N x dim; the accelerations count as computed), Time, xi and MassQ.  The positions, masses
+ * and TimeStep are those the handle was created with. */
+int ccgpu_md_restore(ccgpu_md *m, const double *velocity, const double *acceleration, double time, double xi,
+                     double Q);
+/* MassQ (potential/MD_System.h:50), the thermostat mass of InitNoseHoover; 0 before it */
+int ccgpu_md_get_q(ccgpu_md *m, double *Q);
 /* public member TimeStep (potential/MD_System.h:38): the step of the next Evolve calls. */
 int ccgpu_md_set_timestep(ccgpu_md *m, double timestep);
 /* GetKineticEnergy (MD_System.cpp:370-377) */

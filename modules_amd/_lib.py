@@ -78,6 +78,8 @@ SIGNATURES = {
     "ccgpu_md_destroy": (C.c_int, [vp]),
     "ccgpu_md_set_velocities": (C.c_int, [vp, dp]),
     "ccgpu_md_get_state": (C.c_int, [vp, dp, dp, dp, dp, dp, dp]),
+    "ccgpu_md_restore": (C.c_int, [vp, dp, dp, C.c_double, C.c_double, C.c_double]),
+    "ccgpu_md_get_q": (C.c_int, [vp, dp]),
     "ccgpu_md_set_timestep": (C.c_int, [vp, C.c_double]),
     "ccgpu_md_kinetic_energy": (C.c_int, [vp, dp]),
     "ccgpu_md_evolve": (C.c_int, [vp, vp, C.c_int64]),
@@ -157,6 +159,10 @@ SIGNATURES = {
     "ccgpu_pack_get": (C.c_int, [C.c_char_p, C.c_int, C.c_int64, C.POINTER(C.c_int), dp, dp, i64p, C.c_int64, dp,
                                  vp]),
     "ccgpu_pack_clear": (C.c_int, [C.c_char_p, C.c_int]),
+    "ccgpu_mddump_write": (C.c_int, [C.c_char_p, C.c_int32, C.c_int64, C.c_int, C.c_double, dp, C.c_int64, dp, vp,
+                                     C.c_double, C.c_double, C.c_double, C.c_double, dp, dp, dp]),
+    "ccgpu_mddump_read": (C.c_int, [C.c_char_p, C.POINTER(C.c_int32), i64p, C.POINTER(C.c_int), dp, dp, i64p,
+                                    C.c_int64, dp, vp, dp, dp, dp, dp, dp, dp, dp]),
     "ccgpu_mspb_encode": (C.c_int, [C.c_int, dp, C.c_int64, u64p, u32p, dp, dp, C.c_uint64, vp, C.c_int64, i64p]),
     "ccgpu_mspb_decode": (C.c_int, [vp, C.c_int64, C.POINTER(C.c_int), dp, i64p, u64p, C.c_int64, u64p, u32p, dp,
                                     dp]),

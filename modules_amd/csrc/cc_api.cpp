@@ -934,6 +934,29 @@ int ccgpu_md_get_state(ccgpu_md *m, double *relative, double *velocity, double *
     });
 }
 
+int ccgpu_md_restore(ccgpu_md *m, const double *velocity, const double *acceleration, double time, double xi,
+                     double Q)
+{
+    return Guard([&] {
+        Require(m && (m->N == 0 || (velocity && acceleration)), CCGPU_ERR_ARG, "null argument");
+        Require(std::isfinite(time) && std::isfinite(xi) && std::isfinite(Q), CCGPU_ERR_ARG, "bad argument");
+        m->Upload(m->v, velocity);
+        m->Upload(m->a, acceleration);
+        m->accel = true;  // the stream constructor restores Acceleration (MD_System.cpp:47-48)
+        m->time = time;
+        m->Q = Q;
+        CC_HIP(hipMemcpy(m->state.get(), &xi, sizeof(double), hipMemcpyHostToDevice));
+    });
+}
+
+int ccgpu_md_get_q(ccgpu_md *m, double *Q)
+{
+    return Guard([&] {
+        Require(m && Q, CCGPU_ERR_ARG, "null argument");
+        *Q = m->Q;
+    });
+}
+
 int ccgpu_md_set_timestep(ccgpu_md *m, double timestep)
 {
     return Guard([&] {

@@ -452,3 +452,108 @@ int ccgpu_mspbpack_get(const char *prefix, int64_t index, uint8_t *buf, int64_t 
 }
 
 }  // extern "C"
+
+// MD restart file (CollectiveCoordinateMD, apps/CCOptimization/main.cpp:1891-1908, 1936-1946, 1973-1983, 2005-2011;
+// ParticleMolecularDynamics::WriteBinary and its stream constructor, potential/MD_System.cpp:34-63): int32 stage,
+// int64 step, the Position record (PeriodicCellList::WriteBinary, as in a ConfigurationPack), double TimeStep,
+// double Time, uint64 NumParticle (size_t), uint16 Dimension, double xi, double MassQ, NumParticle doubles Mass, and
+// NumParticle GeometryVector for Velocity and for Acceleration, each the four doubles x[MaxDimension] of a release
+// build (core/GeometryVector.h:24-47: the recorded dimension exists only without NDEBUG), components beyond the
+// dimension zero.  Velocity and acceleration cross the ABI as N x dim doubles.
+int ccgpu_mddump_write(const char *path, int32_t stage, int64_t step, int dim, double cell_size, const double *basis,
+                       int64_t N, const double *relative, const void *chars, double timestep, double time, double xi,
+                       double Q, const double *mass, const double *velocity, const double *acceleration)
+{
+    return Guard([&] {
+        Require(path && basis && N >= 0 && (N == 0 || (relative && mass && velocity && acceleration)), CCGPU_ERR_ARG,
+                "bad argument");
+        Require(dim >= 1 && dim <= kMaxDimension, CCGPU_ERR_ARG, "dimension must be 1..4");
+        Require(N <= 2147483647, CCGPU_ERR_ARG, "too many particles for the record's int32 count");
+        std::string b;
+        Put<int32_t>(b, stage);
+        Put<int64_t>(b, step);
+        Put<uint16_t>(b, (uint16_t)dim);
+        Put<double>(b, cell_size);
+        for (int i = 0; i < dim * dim; i++) Put<double>(b, basis[i]);
+        Put<int32_t>(b, (int32_t)N);
+        for (int64_t i = 0; i < N; i++) {
+            for (int c = 0; c < dim; c++) Put<double>(b, relative[i * dim + c]);
+            if (chars) b.append(static_cast<const char *>(chars) + i * 4, 4);
+            else b.append("a\0\0\0", 4);
+        }
+        Put<double>(b, timestep);
+        Put<double>(b, time);
+        Put<uint64_t>(b, (uint64_t)N);
+        Put<uint16_t>(b, (uint16_t)dim);
+        Put<double>(b, xi);
+        Put<double>(b, Q);
+        for (int64_t i = 0; i < N; i++) Put<double>(b, mass[i]);
+        for (const double *vec : {velocity, acceleration})
+            for (int64_t i = 0; i < N; i++)
+                for (int c = 0; c < kMaxDimension; c++) Put<double>(b, c < dim ? vec[i * dim + c] : 0.0);
+        std::ofstream f(path, std::ios::binary | std::ios::trunc);
+        f.write(b.data(), (std::streamsize)b.size());
+        if (!f) throw Error(CCGPU_ERR_STATE, "MDDump write failed");
+    });
+}
+
+int ccgpu_mddump_read(const char *path, int32_t *stage, int64_t *step, int *dim, double *cell_size, double *basis,
+                      int64_t *N, int64_t cap, double *relative, void *chars, double *timestep, double *time,
+                      double *xi, double *Q, double *mass, double *velocity, double *acceleration)
+{
+    return Guard([&] {
+        Require(path && stage && step && dim && cell_size && basis && N && timestep && time && xi && Q, CCGPU_ERR_ARG,
+                "null argument");
+        bool ok = false;
+        const std::string p = ReadFile(path, ok);
+        if (!ok) throw Error(CCGPU_ERR_STATE, std::string("cannot read ") + path);
+        size_t off = 0;
+        int32_t st = 0;
+        int64_t sp = 0;
+        uint16_t d16 = 0;
+        double cs = 0;
+        if (!Get(p, off, st) || !Get(p, off, sp) || !Get(p, off, d16) || !Get(p, off, cs) || d16 < 1 ||
+            d16 > kMaxDimension)
+            throw Error(CCGPU_ERR_STATE, "invalid MDDump header");
+        const int d = d16;
+        for (int k = 0; k < d * d; k++)
+            if (!Get(p, off, basis[k])) throw Error(CCGPU_ERR_STATE, "MDDump truncated");
+        int32_t np = 0;
+        if (!Get(p, off, np) || np < 0) throw Error(CCGPU_ERR_STATE, "invalid MDDump particle count");
+        *stage = st;
+        *step = sp;
+        *dim = d;
+        *cell_size = cs;
+        *N = np;
+        const bool fill = cap >= np;  // the arrays are filled only when they hold every particle
+        for (int64_t q = 0; q < np; q++) {
+            for (int c = 0; c < d; c++) {
+                double v;
+                if (!Get(p, off, v)) throw Error(CCGPU_ERR_STATE, "MDDump truncated");
+                if (fill && relative) relative[q * d + c] = v;
+            }
+            if (off + 4 > p.size()) throw Error(CCGPU_ERR_STATE, "MDDump truncated");
+            if (fill && chars) std::memcpy(static_cast<char *>(chars) + q * 4, p.data() + off, 4);
+            off += 4;
+        }
+        uint64_t n2 = 0;
+        uint16_t d2 = 0;
+        if (!Get(p, off, *timestep) || !Get(p, off, *time) || !Get(p, off, n2) || !Get(p, off, d2) ||
+            !Get(p, off, *xi) || !Get(p, off, *Q))
+            throw Error(CCGPU_ERR_STATE, "MDDump truncated");
+        if (n2 != (uint64_t)np || d2 != d16) throw Error(CCGPU_ERR_STATE, "MDDump particle count or dimension mismatch");
+        for (int64_t q = 0; q < np; q++) {
+            double m;
+            if (!Get(p, off, m)) throw Error(CCGPU_ERR_STATE, "MDDump truncated");
+            if (fill && mass) mass[q] = m;
+        }
+        for (double *vec : {velocity, acceleration})
+            for (int64_t q = 0; q < np; q++)
+                for (int c = 0; c < kMaxDimension; c++) {
+                    double v;
+                    if (!Get(p, off, v)) throw Error(CCGPU_ERR_STATE, "MDDump truncated");
+                    if (fill && vec && c < d) vec[q * d + c] = v;
+                }
+        if (off != p.size()) throw Error(CCGPU_ERR_STATE, "MDDump has trailing bytes");
+    });
+}

@@ -67,6 +67,8 @@ def GetCCO(stream, seed=0, timelimit_in_hour=144, beg_idx=0, vtilde="flat", out=
     from .md import CollectiveCoordinateMD
     from .optimize import CollectiveCoordinateMultiRun
     out = out or sys.stdout
+    # main.cpp:179: the run's time limit, five minutes short of timelimit_in_hour
+    TimeLimit = time.time() + timelimit_in_hour * 3600 - 5 * 60
     tok = _Tokens(stream)
     rngGod = RandomGenerator(seed)
     dim = int(tok.next())
@@ -128,7 +130,7 @@ def GetCCO(stream, seed=0, timelimit_in_hour=144, beg_idx=0, vtilde="flat", out=
     print(f"chi = {chi}", file=out)
 
     if mode == "MD":
-        dt, T, steps, equil, auto = 0.01, -1.0, 100000, 500, False
+        dt, T, steps, equil, auto, restore = 0.01, -1.0, 100000, 500, False, False
         while True:
             cmd = tok.next().lower()
             if cmd == "run":
@@ -144,7 +146,7 @@ def GetCCO(stream, seed=0, timelimit_in_hour=144, beg_idx=0, vtilde="flat", out=
             elif cmd == "autotimestep":
                 auto = True
             elif cmd == "restore":
-                print("the MDDump restart file is not supported; running from the start", file=out)
+                restore = True
             elif cmd == "finufft":
                 potential.SetRhoBackend("nufft")
             else:
@@ -154,11 +156,12 @@ def GetCCO(stream, seed=0, timelimit_in_hour=144, beg_idx=0, vtilde="flat", out=
         if init == "random":
             # a fresh start from the generator, as the reference's GetInitConfigs(0) (main.cpp:384-390)
             start = with_cells(GetInitConfigs(0))
-            return CollectiveCoordinateMD(start, potential, rngGod, dt, T, savename, numConfig, steps, equil, auto)
+            return CollectiveCoordinateMD(start, potential, rngGod, dt, T, savename, numConfig, steps, equil, auto,
+                                          AllowRestore=restore, TimeLimit=TimeLimit)
         start = with_cells(GetInitConfigs(load.NumConfig() - 1))
         done = ConfigurationPack(savename).NumConfig()
         return CollectiveCoordinateMD(start, potential, rngGod, dt, T, savename, max(numConfig - done, 0), steps,
-                                      equil, auto)
+                                      equil, auto, AllowRestore=restore, TimeLimit=TimeLimit)
     if mode == "ground":
         max_steps, tolerance, algorithm = 10000, 1e-14, "LBFGS"
         while True:

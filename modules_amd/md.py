@@ -7,6 +7,8 @@ Every step (velocity Verlet, Nose-Hoover, Anderson) runs on the device through t
 from __future__ import annotations
 
 import ctypes as C
+import os
+import time
 
 import numpy as np
 
@@ -22,6 +24,9 @@ class ParticleMolecularDynamics:
         self.Dimension = Position.GetDimension()
         self.NumParticle = Position.NumParticle()
         self._basis = np.ascontiguousarray(Position.basis, dtype=np.float64).copy()
+        # the Position's cell-list size and particle names travel with it into the restart file
+        self._cell_size = getattr(Position, "cell_size", None)
+        self._names = getattr(Position, "names", None)
         m = np.asarray(Mass, dtype=np.float64)
         self.Mass = np.full(self.NumParticle, float(m)) if m.ndim == 0 else m.copy()
         rel = np.ascontiguousarray(Position.relative, dtype=np.float64)
@@ -47,7 +52,12 @@ class ParticleMolecularDynamics:
 
     @property
     def Position(self):
-        return Configuration(self._basis, self._state()[0])
+        c = Configuration(self._basis, self._state()[0])
+        if self._cell_size:
+            c.cell_size = self._cell_size
+        if self._names is not None:
+            c.names = self._names
+        return c
 
     @property
     def Velocity(self):
@@ -76,6 +86,52 @@ class ParticleMolecularDynamics:
     @property
     def xi(self):
         return self._state()[5]
+
+    @property
+    def MassQ(self):
+        q = C.c_double(0)
+        check(lib().ccgpu_md_get_q(self._h, C.byref(q)))
+        return q.value
+
+    # --- restart file (Prefix.MDDump) ------------------------------------------------------------
+    def WriteDump(self, path, stage, step):
+        """The restart file of CollectiveCoordinateMD: stage and step, then WriteBinary (potential/MD_System.cpp:50-62;
+        layout in include/ccgpu.h, ccgpu_mddump_write)."""
+        from .formats import ConfigurationPack, _default_cell_size
+        rel, v, a, t, dt, xi = self._state()
+        pos = self.Position
+        chars = ConfigurationPack()._chars(pos)
+        n = self.NumParticle
+        check(lib().ccgpu_mddump_write(os.fsencode(str(path)), int(stage), int(step), self.Dimension,
+                                       _default_cell_size(pos), _p(self._basis), n, _p(np.ascontiguousarray(rel)),
+                                       None if chars is None else chars.ctypes.data_as(C.c_void_p), dt, t, xi,
+                                       self.MassQ, _p(np.ascontiguousarray(self.Mass)), _p(np.ascontiguousarray(v)),
+                                       _p(np.ascontiguousarray(a))))
+
+    @classmethod
+    def ReadDump(cls, path):
+        """(stage, step, system) from a restart file: ParticleMolecularDynamics(std::istream &)
+        (potential/MD_System.cpp:34-49) after the stage and step (apps/CCOptimization/main.cpp:1893-1899)."""
+        b = os.fsencode(str(path))
+        st, sp, d, n = C.c_int32(0), C.c_int64(0), C.c_int(0), C.c_int64(0)
+        cs, dt, t, xi, q = C.c_double(0), C.c_double(0), C.c_double(0), C.c_double(0), C.c_double(0)
+        basis = np.zeros(16)
+        args = lambda cap, rel, chars, m, v, a: (b, C.byref(st), C.byref(sp), C.byref(d), C.byref(cs), _p(basis),
+                                                 C.byref(n), cap, rel, chars, C.byref(dt), C.byref(t), C.byref(xi),
+                                                 C.byref(q), m, v, a)
+        check(lib().ccgpu_mddump_read(*args(0, None, None, None, None, None)))
+        N, dd = n.value, d.value
+        rel, v, a = (np.zeros((max(N, 1), dd)) for _ in range(3))
+        mass = np.zeros(max(N, 1))
+        chars = np.zeros(max(N, 1) * 4, dtype=np.uint8)
+        check(lib().ccgpu_mddump_read(*args(N, _p(rel), chars.ctypes.data_as(C.c_void_p), _p(mass), _p(v), _p(a))))
+        pos = Configuration(basis[: dd * dd].reshape(dd, dd), rel[:N])
+        pos.cell_size = cs.value
+        pos.names = [bytes(chars[4 * i: 4 * i + 4]).split(b"\0")[0] for i in range(N)]
+        system = cls(pos, dt.value, mass[:N])
+        check(lib().ccgpu_md_restore(system._h, _p(np.ascontiguousarray(v[:N])), _p(np.ascontiguousarray(a[:N])),
+                                     t.value, xi.value, q.value))
+        return st.value, sp.value, system
 
     # --- dynamics --------------------------------------------------------------------------------
     def Evolve(self, repeat, potential):
@@ -157,33 +213,67 @@ class ParticleMolecularDynamics:
 
 
 def CollectiveCoordinateMD(pConfig: Configuration, pPotential, gen, TimeStep, Temperature, Prefix, SampleNumber,
-                           StepPerSample, EquilibrateSamples, MDAutoTimeStep=False):
-    """CollectiveCoordinateMD (apps/CCOptimization/main.cpp:1875-2010) without the restart file: relax to a
-    ground state (RelaxStructure_NLOPT, 1000 evaluations per pass), equilibrate EquilibrateSamples times (two rounds
-    of SetRandomSpeed + Evolve(StepPerSample / 2)), then sample SampleNumber configurations (AndersonEvolve at 0.01
-    reset probability + Evolve, each StepPerSample / 2 steps) into the ConfigurationPack `Prefix`.  Returns the
-    list of (potential energy, kinetic energy) of the samples."""
+                           StepPerSample, EquilibrateSamples, MDAutoTimeStep=False, AllowRestore=False, TimeLimit=None):
+    """CollectiveCoordinateMD (apps/CCOptimization/main.cpp:1875-2012): relax to a ground state
+    (RelaxStructure_NLOPT, 1000 evaluations per pass), equilibrate EquilibrateSamples times (two rounds of
+    SetRandomSpeed + Evolve(StepPerSample / 2)), then sample SampleNumber configurations (AndersonEvolve at 0.01 reset
+    probability + Evolve, each StepPerSample / 2 steps) into the ConfigurationPack `Prefix`.
+
+    AllowRestore: continue from Prefix.MDDump when it exists (its stage and the step after its step; the pack is
+    then kept, not cleared), and store the final state there at the end.  TimeLimit (seconds since the epoch, as
+    the reference's time_t; None: none): once passed, the state is stored in Prefix.MDDump with the stage and the
+    step about to run, "Not Completed. Please run again" goes to Prefix_continue.txt (equilibration) or
+    continue.txt (sampling), and the function returns -- the reference's behaviour, file names included.  The
+    generator's state is not part of the file, as in the reference.  Returns the list of (potential energy,
+    kinetic energy) of the samples taken in this call."""
     from .formats import ConfigurationPack
     from .optimize import RelaxStructure_NLOPT
     pack = ConfigurationPack(Prefix)
-    pack.Clear()
-    RelaxStructure_NLOPT(pConfig, pPotential, 0.0, 0, 0.0, 1000)
-    pPotential.SetConfiguration(pConfig)
-    system = ParticleMolecularDynamics(pConfig, TimeStep, 1.0)
+    dump = str(Prefix) + ".MDDump"
+    system, stage, step = None, 0, 0
+    if AllowRestore and os.path.isfile(dump):
+        stage, step, system = ParticleMolecularDynamics.ReadDump(dump)
+        pPotential.SetConfiguration(system.Position)
+        step += 1
+    if system is None:
+        pack.Clear()
+        RelaxStructure_NLOPT(pConfig, pPotential, 0.0, 0, 0.0, 1000)
+        pPotential.SetConfiguration(pConfig)
+        pPotential.Energy()
+        system = ParticleMolecularDynamics(pConfig, TimeStep, 1.0)
+    expired = lambda: TimeLimit is not None and time.time() > TimeLimit
     half = StepPerSample // 2
-    for _ in range(EquilibrateSamples):
-        for _ in range(2):
-            system.SetRandomSpeed(Temperature, gen)
-            if MDAutoTimeStep:
-                system.Evolve_AutoTimeStep(half, pPotential, 0.0001)
-            else:
-                system.Evolve(half, pPotential)
+    if stage == 0:
+        stage, step = 1, 0
     out = []
-    for _ in range(SampleNumber):
+    if stage == 1:
+        for i in range(step, EquilibrateSamples):
+            if expired():
+                system.WriteDump(dump, stage, i)
+                with open(str(Prefix) + "_continue.txt", "w") as f:
+                    f.write("Not Completed. Please run again\n")
+                return out
+            for _ in range(2):
+                system.SetRandomSpeed(Temperature, gen)
+                if MDAutoTimeStep:
+                    system.Evolve_AutoTimeStep(half, pPotential, 0.0001)
+                else:
+                    system.Evolve(half, pPotential)
+        stage, step = 2, 0
+    final_step = 0
+    for i in range(step, SampleNumber):
+        if expired():
+            system.WriteDump(dump, stage, i)
+            with open("continue.txt", "w") as f:
+                f.write("Not Completed. Please run again\n")
+            return out
         system.AndersonEvolve(half, pPotential, Temperature, 0.01, gen)
         system.Evolve(half, pPotential)
         result = system.Position
         pPotential.SetConfiguration(result)
         out.append((pPotential.Energy(), system.GetKineticEnergy()))
         pack.AddConfig(result)
+        final_step = i
+    if AllowRestore:
+        system.WriteDump(dump, stage, final_step)
     return out

@@ -175,3 +175,86 @@ def test_sphere_packing_pack(tmp_path):
     assert open(prefix + ".SpherePack", "rb").read() == expected_record(p, radii=[0.5, 0.75], cell=1.5)
     q = SpherePackingPack(prefix).GetConfig(0)
     assert np.array_equal(q.radii, p.radii) and np.array_equal(q.relative, p.relative)
+
+
+def mddump_bytes(stage, step, a, cs, s, names, dt, t, xi, q, mass, v, acc):
+    """The MD restart file assembled with struct from the reference's writes: main.cpp:1938-1941 (int stage,
+    long long step), MD_System.cpp:50-62 (Position record, TimeStep, Time, size_t NumParticle, unsigned short
+    Dimension, xi, MassQ, Mass, then Velocity and Acceleration as GeometryVector = double x[4])."""
+    d, n = a.shape[0], len(s)
+    b = struct.pack("<iq", stage, step) + struct.pack("<Hd", d, cs) + struct.pack("<%dd" % (d * d), *a.reshape(-1))
+    b += struct.pack("<i", n)
+    for i in range(n):
+        b += struct.pack("<%dd" % d, *s[i]) + names[i].ljust(4, b"\0")
+    b += struct.pack("<ddQHdd", dt, t, n, d, xi, q) + struct.pack("<%dd" % n, *mass)
+    for vec in (v, acc):
+        for i in range(n):
+            b += struct.pack("<4d", *(list(vec[i]) + [0.0] * (4 - d)))
+    return b
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_mddump_layout_and_round_trip(tmp_path, dim):
+    # written by the library (ccgpu_mddump_write), compared byte for byte with the documented field list, then
+    # read back (ccgpu_mddump_read) exactly
+    import ctypes as C
+    from modules_amd._lib import check, lib
+    from modules_amd.potential import _p
+    rng = np.random.default_rng(dim)
+    a, _, _, _ = export_case(max(dim, 2), False)
+    a = a[:dim, :dim].copy()
+    n = 5
+    s, v, acc = rng.random((n, dim)), rng.normal(size=(n, dim)), rng.normal(size=(n, dim))
+    mass = rng.uniform(0.5, 2.0, n)
+    names = [b"a", b"b", b"Cu", b"a", b"xyz"]
+    chars = np.frombuffer(b"".join(x.ljust(4, b"\0") for x in names), dtype=np.uint8).copy()
+    path = str(tmp_path / "run.MDDump")
+    check(lib().ccgpu_mddump_write(path.encode(), 2, 17, dim, 0.4, _p(a), n, _p(s), chars.ctypes.data_as(C.c_void_p),
+                                   0.01, 1.25, -0.3, 7.5, _p(mass), _p(v), _p(acc)))
+    raw = open(path, "rb").read()
+    assert raw == mddump_bytes(2, 17, a, 0.4, s, names, 0.01, 1.25, -0.3, 7.5, mass, v, acc)
+    st, sp, d, N = C.c_int32(0), C.c_int64(0), C.c_int(0), C.c_int64(0)
+    cs, dt, t, xi, q = (C.c_double(0) for _ in range(5))
+    basis = np.zeros(16)
+    s2, v2, a2, m2 = np.zeros((n, dim)), np.zeros((n, dim)), np.zeros((n, dim)), np.zeros(n)
+    c2 = np.zeros(4 * n, dtype=np.uint8)
+    check(lib().ccgpu_mddump_read(path.encode(), C.byref(st), C.byref(sp), C.byref(d), C.byref(cs), _p(basis),
+                                  C.byref(N), n, _p(s2), c2.ctypes.data_as(C.c_void_p), C.byref(dt), C.byref(t),
+                                  C.byref(xi), C.byref(q), _p(m2), _p(v2), _p(a2)))
+    assert (st.value, sp.value, d.value, N.value) == (2, 17, dim, n)
+    assert (cs.value, dt.value, t.value, xi.value, q.value) == (0.4, 0.01, 1.25, -0.3, 7.5)
+    assert np.array_equal(basis[:dim * dim], a.reshape(-1))
+    for x, y in ((s2, s), (v2, v), (a2, acc), (m2, mass)):
+        assert np.array_equal(x, y)
+    assert bytes(c2) == bytes(chars)
+
+
+def test_mddump_reader_rejects_damaged_files(tmp_path):
+    import ctypes as C
+    from modules_amd._lib import CCGPUError, check, lib
+    from modules_amd.potential import _p
+    a = np.eye(2) * 3.0
+    s = np.array([[0.1, 0.2], [0.3, 0.4]])
+    good = mddump_bytes(1, 0, a, 0.3, s, [b"a", b"a"], 0.01, 0.0, 0.0, 0.0, [1.0, 1.0], np.zeros((2, 2)),
+                        np.zeros((2, 2)))
+    bad_count = bytearray(good)
+    off = 4 + 8 + 2 + 8 + 32 + 4 + 2 * 20 + 16  # the NumParticle field after the record, TimeStep and Time
+    bad_count[off:off + 8] = struct.pack("<Q", 3)
+    cases = {"short": good[:-1], "long": good + b"\0", "count": bytes(bad_count), "empty": b""}
+    for name, data in cases.items():
+        p = tmp_path / f"{name}.MDDump"
+        p.write_bytes(data)
+        st, sp, d, N = C.c_int32(0), C.c_int64(0), C.c_int(0), C.c_int64(0)
+        cs, dt, t, xi, q = (C.c_double(0) for _ in range(5))
+        with pytest.raises(CCGPUError):
+            check(lib().ccgpu_mddump_read(str(p).encode(), C.byref(st), C.byref(sp), C.byref(d), C.byref(cs),
+                                          _p(np.zeros(16)), C.byref(N), 0, None, None, C.byref(dt), C.byref(t),
+                                          C.byref(xi), C.byref(q), None, None, None))
+    p = tmp_path / "good.MDDump"
+    p.write_bytes(good)
+    st, sp, d, N = C.c_int32(0), C.c_int64(0), C.c_int(0), C.c_int64(0)
+    cs, dt, t, xi, q = (C.c_double(0) for _ in range(5))
+    check(lib().ccgpu_mddump_read(str(p).encode(), C.byref(st), C.byref(sp), C.byref(d), C.byref(cs),
+                                  _p(np.zeros(16)), C.byref(N), 0, None, None, C.byref(dt), C.byref(t), C.byref(xi),
+                                  C.byref(q), None, None, None))
+    assert N.value == 2 and st.value == 1

@@ -133,3 +133,51 @@ def test_softcore_kernels_give_identical_bits(d, N, sigma):
     Eo = po.softcore_energy(cfg.relative, cfg.basis, sigma, 1.0)
     Fo = po.softcore_forces(cfg.relative, cfg.basis, sigma, 1.0)
     assert E1 == pytest.approx(Eo, rel=1e-12) and np.allclose(F1, Fo, rtol=1e-11, atol=1e-12)
+
+
+def test_mddump_restores_the_state_and_its_dynamics(tmp_path):
+    # WriteBinary / ParticleMolecularDynamics(std::istream &) (potential/MD_System.cpp:34-62): a thermostatted run
+    # written to the restart file and read back holds the same state, and both copies then evolve to the same bits
+    cfg, k, pot_a, _ = setup(N=120, d=2)
+    _, _, pot_b, _ = setup(N=120, d=2)
+    md = ParticleMolecularDynamics(cfg, 0.01, np.linspace(0.8, 1.2, 120))
+    md.SetRandomSpeed(0.2, RandomGenerator(3))
+    md.InitNoseHoover(2.5)
+    md.NoseHooverEvolve(15, pot_a, 0.2)
+    path = tmp_path / "run.MDDump"
+    md.WriteDump(path, 2, 41)
+    stage, step, back = ParticleMolecularDynamics.ReadDump(path)
+    assert (stage, step) == (2, 41)
+    assert np.array_equal(back.Position.relative, md.Position.relative)
+    assert np.array_equal(back.Velocity, md.Velocity) and np.array_equal(back.Acceleration, md.Acceleration)
+    assert (back.Time, back.TimeStep, back.xi, back.MassQ) == (md.Time, md.TimeStep, md.xi, md.MassQ)
+    assert md.xi != 0.0 and np.array_equal(back.Mass, md.Mass)
+    md.NoseHooverEvolve(10, pot_a, 0.2)
+    back.NoseHooverEvolve(10, pot_b, 0.2)
+    assert np.array_equal(back.Position.relative, md.Position.relative)
+    assert np.array_equal(back.Velocity, md.Velocity) and back.xi == md.xi
+
+
+def test_collective_coordinate_md_continues_from_its_restart_file(tmp_path, monkeypatch):
+    # CollectiveCoordinateMD with AllowRestore (apps/CCOptimization/main.cpp:1891-1911, 1936-1946, 2005-2011): a run
+    # past its time limit stores stage 1 and the step about to run and says so in Prefix_continue.txt; the next
+    # run continues from the step after it, keeps the pack, and stores stage 2 and the last sample at the end
+    from modules_amd.formats import ConfigurationPack
+    monkeypatch.chdir(tmp_path)
+    cfg, k, pot, _ = setup(N=60, d=2)
+    prefix = str(tmp_path / "md")
+    out = CollectiveCoordinateMD(cfg, pot, RandomGenerator(1), 0.01, 0.1, prefix, 3, 20, 2, AllowRestore=True,
+                                 TimeLimit=0.0)
+    assert out == []
+    assert open(prefix + "_continue.txt").read() == "Not Completed. Please run again\n"
+    stage, step, _ = ParticleMolecularDynamics.ReadDump(prefix + ".MDDump")
+    assert (stage, step) == (1, 0)
+    out = CollectiveCoordinateMD(cfg, pot, RandomGenerator(1), 0.01, 0.1, prefix, 3, 20, 2, AllowRestore=True)
+    assert len(out) == 3 and all(np.isfinite(e) and ek > 0 for e, ek in out)
+    assert ConfigurationPack(prefix).NumConfig() == 3
+    stage, step, system = ParticleMolecularDynamics.ReadDump(prefix + ".MDDump")
+    assert (stage, step) == (2, 2)
+    assert np.array_equal(system.Position.relative, ConfigurationPack(prefix).GetConfig(2).relative)
+    # a run that finds the finished file takes no more samples and keeps the pack
+    assert CollectiveCoordinateMD(cfg, pot, RandomGenerator(1), 0.01, 0.1, prefix, 3, 20, 2, AllowRestore=True) == []
+    assert ConfigurationPack(prefix).NumConfig() == 3
