@@ -155,8 +155,9 @@ def _roofline(jobs, rho_ms, force_ms, n_rho, n_force):
     }
 
 
-def headline(args):
-    """The 256-configuration batch relaxed to E < Emin by the device-resident solver."""
+def headline(args, roofline=True):
+    """The 256-configuration batch relaxed to E < Emin by the device-resident solver (roofline=False: no second,
+    one-lane solve for the kernel roofline)."""
     from modules_amd import RandomConfiguration
     from modules_amd.optimize import BatchSolver
     reqs, ks = batch_shapes()
@@ -241,7 +242,7 @@ def headline(args):
     launches = solver.launch_stats()
     roof = _roofline([(res["device_evaluations"], r.N, M, r.d) for res, r, M in results], rho_ms, force_ms, n_rho,
                      n_force) if timing else None
-    if roof is None and not args.profiler_run:
+    if roof is None and not args.profiler_run and roofline:
         # graph replay or two lanes: per-launch events would miss (graph) or include the other lane's kernels (two
         # lanes), so the kernel roofline comes from a second solve of the same batches on one stream, with stream
         # launches and hipEvents around every contraction launch
@@ -319,6 +320,19 @@ def autotune_classes(reqs, ks, per_class=1):
             continue
         seen[key] = autotune.autotune(r.N, r.d, k=k, batch=4, repeats=2)
     return {str(k): v for k, v in seen.items()}
+
+
+def sub_batch_mixed(args, fp64_value):
+    """The headline's batches in mixed precision (FP32 phases and force terms, FP64 sums, FP64 polish below 1e-8 and
+    FP64 energies at every stop test), the same timed region and settings, no roofline pass."""
+    import copy
+    a = copy.copy(args)
+    a.precision, a.tiles, a.autotune = "mixed", None, False
+    r = headline(a, roofline=False)
+    return {"workload": "the headline's batches in mixed precision (FP64 polish below 1e-8)", "value": r["value"],
+            "unit": "ground_states/s", "solved": r["solved"], "solves": r["solves"], "ms_per_step": r["ms_per_step"],
+            "ef_evals_per_s": r["ef_evals_per_s"], "device_evals_per_solve_mean": r["device_evals_per_solve_mean"],
+            "speedup_over_fp64": r["value"] / fp64_value if fp64_value else None}
 
 
 def sub_seeds64(args, check_oracle=True):
@@ -507,7 +521,8 @@ def main():
         return
     if not args.profiler_run and not args.no_sub:
         subs = {}
-        for name, fn in (("seeds64_3d_N2000", sub_seeds64), ("mixed_2d_N20000", sub_mixed2d),
+        for name, fn in (("batch256_mixed", lambda a: sub_batch_mixed(a, out["value"])),
+                         ("seeds64_3d_N2000", sub_seeds64), ("mixed_2d_N20000", sub_mixed2d),
                          ("serving", sub_serving), ("single_solve_3d_N2000", sub_single)):
             _log(f"sub-record {name}")
             subs[name] = fn(args)
