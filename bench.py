@@ -523,7 +523,10 @@ def main():
         subs = {}
         for name, fn in (("batch256_mixed", lambda a: sub_batch_mixed(a, out["value"])),
                          ("seeds64_3d_N2000", sub_seeds64), ("mixed_2d_N20000", sub_mixed2d),
-                         ("serving", sub_serving), ("single_solve_3d_N2000", sub_single)):
+                         ("serving", sub_serving),
+                         # below the saturating rate: the latency of a request rather than of a queue
+                         ("serving_200_per_s", lambda a: sub_serving(a, requests=400, rate=200.0)),
+                         ("single_solve_3d_N2000", sub_single)):
             _log(f"sub-record {name}")
             subs[name] = fn(args)
         out["workloads"] = subs
