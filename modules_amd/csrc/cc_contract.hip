@@ -5,14 +5,16 @@
 //   G_j   = sum_k c_k n_k,  c_k = A_k cos - B_k sin        (Force / AllForce, :1377-1448; F = -sum_j G_j b_j)
 // with the constraints grouped into segments: runs of consecutive last-axis index at fixed leading indices, at most
 // SEG long.  Along a segment the phases of one particle follow the three-term recurrence
-// e_{t+1} = 2 cos(phi) e_t - e_{t-1} (cos and sin of an arithmetic progression of angles both satisfy it) from one
-// sincos at the segment's head, so a (particle, constraint) pair costs two FMAs for its phase.
+// x_{t+1} = 2 cos(phi) x_t - x_{t-1} (cos and sin of an arithmetic progression of angles both satisfy it) from one
+// sincos per (segment, particle): the force kernels walk the complex phase forward from the segment's head (two
+// FMAs per slot), the rho kernel walks the real cos and sin of (t + 1/2) phi outwards from the segment's centre and
+// forms both slots of a pair from them (three operations per slot).
 //
 // rho (k_rho_rows): one wavefront per work item = 16 segments (the 16 lanes of a row) x 4 particle rows (row r takes
 // every fourth particle of the chunk).  Segments are sorted longest first, so the 16 of a wavefront have similar
-// lengths and the slot loop stops at the longest of them.  Each lane accumulates its segment's SEG slots over its
+// lengths and the slot loop stops at the longest of them.  Each lane accumulates its segment's sums over its
 // particles; the four rows are then summed by two exchange steps (v_permlane16_swap and v_permlane32_swap, which swap
-// rows between two registers), after which each lane holds the complete sums of SEG/4 slots of its segment.  A
+// rows between two registers), after which each lane holds the complete sums of a quarter of its segment's slots.  A
 // system whose particles fit one chunk is finished here (slot_finish); otherwise the chunk sums go to rpart.
 //
 // forces (k_force_rows): one block of 4 wavefronts per work item = one group of 64 x PPL particles (lane = particle)
