@@ -409,7 +409,9 @@ __global__ void __launch_bounds__(kBlock) k_rho_general(const SysDesc *__restric
 // ---------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_rho_reduce(const SysDesc *__restrict__ sys, const WorkItem *__restrict__ items)
 {
-    const WorkItem it = items[blockIdx.x];
+    // items in reverse: the first blocks read the partials the rho launch wrote last, which are still in the
+    // memory-side cache
+    const WorkItem it = items[gridDim.x - 1 - blockIdx.x];
     const SysDesc &S = sys[it.sys];
     if (!sys_active(sys, S)) return;
     const int q = it.t0 * kBlock + threadIdx.x;

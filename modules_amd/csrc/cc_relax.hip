@@ -743,7 +743,9 @@ __global__ void __launch_bounds__(kWideBlock) k_lbfgs_pass2(const SysDesc *__res
     constexpr int kH = 8;
     __shared__ double al[kPlisMaxHistory], cs[kPlisMaxHistory];
     __shared__ double red[2 * nw];
-    const WorkItem it = items[blockIdx.x];
+    // items in reverse: the first blocks read the history that pass 1 read last, which is still in the
+    // memory-side cache
+    const WorkItem it = items[gridDim.x - 1 - blockIdx.x];
     const RelaxDesc R = rdv[it.sys];
     const SysDesc &D = sysv[R.sys];
     if (!(D.flags & kSysActive) || R.st->plis.phase != kPhDirection) return;
