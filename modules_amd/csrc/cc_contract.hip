@@ -32,10 +32,19 @@
 #include "cc_device.h"
 #include "cc_epilogue.h"
 #include "cc_math.h"
+#include "cc_tab256.h"
 
 namespace ccgpu {
 
 namespace {
+
+// the 256-entry sincos table into the block's LDS: a copy of the constant array (4 KB, cache-resident after the
+// first blocks) instead of 256 evaluations of sincos_rev per block; the caller synchronises before use
+__device__ __forceinline__ void sincos_table_load256(double2 *tab)
+{
+    const double2 *src = reinterpret_cast<const double2 *>(kSinCosTab256);
+    for (int t = threadIdx.x; t < 256; t += blockDim.x) tab[t] = src[t];
+}
 
 // rows 1 and 3 of a swapped with rows 0 and 2 of b (rows are 16 lanes)
 __device__ __forceinline__ void swap_rows16(double &a, double &b)
@@ -344,7 +353,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
     if (!sys_active(sys, S)) return;
-    sincos_table_fill256(lds.tab);
+    sincos_table_load256(lds.tab);
     const int lane = threadIdx.x, col = lane & 15;
     const int seg0 = it.t0 * kRowSegs;
     const int seg = seg0 + col;
@@ -473,7 +482,7 @@ __global__ void __launch_bounds__(kBlock) k_force_rows(const SysDesc *__restrict
     const WorkItem it = items[blockIdx.x];
     const SysDesc &S = sys[it.sys];
     if (!sys_active(sys, S) || (S.flags & kSysMixed)) return;  // mixed systems: k_force_rows_mixed
-    sincos_table_fill256(L.tab);
+    sincos_table_load256(L.tab);
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int d = S.d;
     double4 p[PPL];

@@ -106,15 +106,8 @@ __device__ __forceinline__ void sincos_rev_tab(double u, const double2 *tab, dou
 // The 256-entry variant of the particle-row contractions: u = j/256 + f with |f| <= 1/512 revolutions (the same
 // exact split), and 2 pi folded into the series in f: sin(2 pi f) = f (c1 + c3 f^2 + c5 f^4) (next term below
 // 9e-18), cos(2 pi f) = 1 + c2 f^2 + c4 f^4 + c6 f^6 (next term below 2e-20), c_k = (+-) (2 pi)^k / k!.  About 16
-// VALU operations and one LDS read.  The table (4 KB) is filled by the whole block.
-__device__ __forceinline__ void sincos_table_fill256(double2 *tab)
-{
-    for (int t = threadIdx.x; t < 256; t += blockDim.x) {
-        double s, c;
-        sincos_rev((double)t * 0.00390625, s, c);
-        tab[t] = make_double2(c, s);
-    }
-}
+// VALU operations and one LDS read.  The table (4 KB in LDS) is exp(2 pi i t / 256) correctly rounded, copied by
+// the whole block from a constant array (cc_tab256.h).
 
 __device__ __forceinline__ void sincos_rev_tab256(double u, const double2 *tab, double &s, double &c)
 {

@@ -115,3 +115,37 @@ def test_cost_gate_fit_recovers_known_constants():
     import re
     hdr = open(os.path.join(ROOT, "modules_amd", "csrc", "cc_runtime.h")).read()
     assert float(re.search(r"kNufftMargin = ([0-9.]+);", hdr).group(1)) == MARGIN
+
+
+def test_sincos_table_header_is_the_generator_output_and_correctly_rounded():
+    """The kernels' 256-entry table: the committed header is what its generator writes, and every entry is the
+    double nearest to cos / sin(2 pi t / 256), checked independently in 64-bit-mantissa arithmetic (where the exact
+    value is 0, +-1 or +-sqrt(1/2) against the closed form instead)."""
+    import importlib.util
+    import re
+
+    path = os.path.join(ROOT, "modules_amd", "csrc", "gen_sincos_tab256.py")
+    spec = importlib.util.spec_from_file_location("gen_sincos_tab256", path)
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    committed = open(os.path.join(ROOT, "modules_amd", "csrc", "cc_tab256.h")).read()
+    assert committed == gen.header()
+    vals = [float.fromhex(x) for x in re.findall(r"(-?0x[0-9a-f.]+p[+-]\d+)", committed)]
+    tab = np.array(vals).reshape(256, 2)
+    t = np.arange(256)
+    if np.finfo(np.longdouble).nmant >= 63:
+        pi = np.longdouble("3.14159265358979323846264338327950288")
+        ang = np.longdouble(2) * pi * t.astype(np.longdouble) / np.longdouble(256)
+        c = np.cos(ang).astype(np.float64)
+        s = np.sin(ang).astype(np.float64)
+        exact = t % 64 == 0  # 0 and +-1, where the long-double values carry a residue of the rounded pi
+        assert np.array_equal(tab[~exact, 0], c[~exact]) and np.array_equal(tab[~exact, 1], s[~exact])
+    q = t // 64
+    assert np.array_equal(tab[t % 64 == 0], np.array([[1.0, 0.0], [0.0, 1.0], [-1.0, 0.0], [0.0, -1.0]])[q[t % 64 == 0]])
+    r2 = float.fromhex("0x1.6a09e667f3bcdp-1")  # sqrt(1/2) rounded to nearest
+    assert np.array_equal(np.abs(tab[t % 64 == 32]), np.full((4, 2), r2))
+    # a planted one-ulp error in one entry is caught by the independent check
+    bad = tab.copy()
+    bad[37, 0] = np.nextafter(bad[37, 0], 2.0)
+    if np.finfo(np.longdouble).nmant >= 63:
+        assert not np.array_equal(bad[~exact, 0], c[~exact])
