@@ -184,9 +184,10 @@ struct BlockPlisOps {
     // Element loops visit this thread's elements kU at a time, all loads of a batch before their uses, so that
     // each thread keeps several independent HBM loads in flight (one workgroup per job leaves few waves per CU);
     // with 256 threads the remainder is one predicated batch, so no element waits on a round trip of its own.
-    // Eight per batch keep the one-block step within two blocks per CU (its launch bound).  Sums run in element
-    // order whatever the batch, so the batch size never changes a bit.
-    static constexpr int kU = kGrid ? 8 : BS <= 256 ? 8 : 4;
+    // The one-block step takes four per batch: at its launch bound of two blocks per CU (256 registers) eight
+    // spilled 67 registers, 23 of them to scratch, and ran 0.6% slower overall; four spill none.  Sums run in
+    // element order whatever the batch, so the batch size never changes a bit.
+    static constexpr int kU = kGrid ? 8 : 4;
     template <class Load, class Use>
     __device__ __forceinline__ void batches(Load load, Use use) const
     {
