@@ -147,7 +147,11 @@ __device__ __forceinline__ ScCells sc_cells(const SysDesc &S)
         g.reach[c] = 0;
         g.span[c] = 1;
     }
-    for (int c = 0; c < S.d; c++) {
+    // loops over the dimension run to the constant 4 with a guard (here and in sc_cell_terms), so that these small
+    // arrays are indexed by constants and stay in registers
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        if (c >= S.d) continue;
         // reach along axis c: cells of width h_c / n_c, h_c = 2 pi / |b_c|
         double bn = 0.0;
         for (int q = 0; q < S.d; q++) bn += S.b[c * S.d + q] * S.b[c * S.d + q];
@@ -194,7 +198,9 @@ __device__ __forceinline__ void sc_cell_terms(const SysDesc &S, const ScCells &g
     const int d = g.d;
     int rem = o, lin = 0, mult = 1;
     double img[4] = {0, 0, 0, 0};
-    for (int c = 0; c < d; c++) {
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        if (c >= d) continue;
         const int off = rem % g.span[c] - g.reach[c];
         rem /= g.span[c];
         const int n = S.ncell_side[c];
@@ -214,10 +220,15 @@ __device__ __forceinline__ void sc_cell_terms(const SysDesc &S, const ScCells &g
         const double ds[4] = {pj.x + img[0] - p.si[0], pj.y + img[1] - p.si[1], pj.z + img[2] - p.si[2],
                               pj.w + img[3] - p.si[3]};
         double r[4] = {0, 0, 0, 0};
-        for (int q = 0; q < d; q++)
-            for (int c = 0; c < d; c++) r[c] = __builtin_fma(S.a[q * d + c], ds[q], r[c]);
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+                if (q < d && c < d) r[c] = __builtin_fma(S.a[q * d + c], ds[q], r[c]);
         double m2 = 0.0;
-        for (int c = 0; c < d; c++) m2 = __builtin_fma(r[c], r[c], m2);
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            if (c < d) m2 = __builtin_fma(r[c], r[c], m2);
         const double sig = radius ? p.ri + radius[j] : S.sigma;
         if (!(m2 < sig * sig)) continue;
         const double rr = sqrt(m2);
@@ -231,10 +242,14 @@ __device__ __forceinline__ void sc_cell_terms(const SysDesc &S, const ScCells &g
         if (rr < 1e-15) {
             const int gj = S.pid ? S.pid[j] : j;
             f[0] -= pref * ((p.gi > gj) ? -1.0 : 1.0);
-            for (int c = 1; c < d; c++) f[c] -= pref * r[c];
+#pragma unroll
+            for (int c = 1; c < 4; c++)
+                if (c < d) f[c] -= pref * r[c];
         } else {
             const double cf = pref * (1.0 / rr - 1.0 / sig);
-            for (int c = 0; c < d; c++) f[c] = __builtin_fma(-cf, r[c], f[c]);
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+                if (c < d) f[c] = __builtin_fma(-cf, r[c], f[c]);
         }
     }
 }
