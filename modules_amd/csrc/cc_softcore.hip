@@ -316,24 +316,48 @@ __global__ void __launch_bounds__(kBlock) k_sc_pair(const SysDesc *__restrict__ 
     const ScCells g = sc_cells(S);
     const ScSelf p = sc_self(S, i);
     // pairwise summation of the L leaves in order: level k of the stack holds the sum of the last complete run of
-    // 2^k leaves; leaf o is folded up while the low bits of o are set
+    // 2^k leaves; leaf o is folded into levels 0 .. t-1 (t = the number of trailing one bits of o) and stored at
+    // level t.  The levels are visited by an unrolled loop with selects, so the stack is indexed by constants and
+    // stays in registers (indexed by t it lived in scratch memory); the adds are the same, in the same order.
     double sE[7], sf[7][4];
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+        sE[k] = 0.0;
+        for (int c = 0; c < 4; c++) sf[k][c] = 0.0;
+    }
     for (int o = 0; o < g.L; o++) {
         double E = 0.0;
         double f[4] = {0, 0, 0, 0};
         for (int c = o; c < g.total; c += g.L) sc_cell_terms(S, g, p, c, E, f);
-        int k = 0;
-        for (int bits = o; bits & 1; bits >>= 1, k++) {
-            E = sE[k] + E;
-            for (int c = 0; c < 4; c++) f[c] = sf[k][c] + f[c];
+        const int t = __builtin_ctz(~o);  // trailing ones of o
+#pragma unroll
+        for (int k = 0; k < 7; k++) {
+            if (k < t) {
+                E = sE[k] + E;
+#pragma unroll
+                for (int c = 0; c < 4; c++) f[c] = sf[k][c] + f[c];
+            }
         }
-        sE[k] = E;
-        for (int c = 0; c < 4; c++) sf[k][c] = f[c];
+#pragma unroll
+        for (int k = 0; k < 7; k++) {
+            if (k == t) {
+                sE[k] = E;
+#pragma unroll
+                for (int c = 0; c < 4; c++) sf[k][c] = f[c];
+            }
+        }
     }
-    int top = 0;
-    while ((1 << top) < g.L) top++;
-    double f[4] = {sf[top][0], sf[top][1], sf[top][2], sf[top][3]};
-    sc_store(S, i, sE[top], f);
+    const int top = __builtin_ctz((unsigned)g.L);  // L is a power of two
+    double E = 0.0, f[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+        if (k == top) {
+            E = sE[k];
+#pragma unroll
+            for (int c = 0; c < 4; c++) f[c] = sf[k][c];
+        }
+    }
+    sc_store(S, i, E, f);
 }
 
 namespace {
